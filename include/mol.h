@@ -1,0 +1,218 @@
+/*
+ * mol.h — C ABI of the MI355X (gfx950) method-of-lines stencil library.
+ *
+ * The library evaluates the semi-discrete right-hand side f(du, u, p, t) that
+ * MethodOfLines.jl's `discretize(pdesys, MOLFiniteDifference(...))` produces, and
+ * drives it with explicit Runge-Kutta stage loops, adjoints and batched launches.
+ *
+ * A problem is handed over as a plain "stencil IR" descriptor (mol_desc): the grid,
+ * the unknown box of every dependent variable, per-axis operator weight tables,
+ * operator slots (centered / upwind / WENO5 / nonlinear Laplacian / spherical /
+ * mixed), ghost-node closures for Dirichlet / Neumann / Robin / extrapolated
+ * boundary nodes, and small bytecode programs for the pointwise part of each
+ * equation.  The host front-end builds the descriptor from a PDE system, the way
+ * the reference builds its symbolic equations (src/scalar_discretization.jl:1-64).
+ *
+ * Conventions
+ *   - every function returns 0 on success and a negative mol_status otherwise;
+ *     mol_last_error() gives the message.  No exception crosses this boundary.
+ *   - pointers named *_h are host pointers; all other array pointers are device
+ *     pointers.  `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *   - calls are asynchronous on `stream` unless named *_h (those synchronize).
+ *   - one handle must not be used from two threads at once.
+ *   - state vectors use the reference's unknown ordering: variable-major, and within
+ *     a variable the first spatial axis fastest (docs/src/generated/bruss_code.md:48-79,
+ *     src/interface/solution/timedep.jl:30-47).
+ */
+#ifndef MOL_H
+#define MOL_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MOL_ABI_VERSION 1
+#define MOL_MAX_DIMS 3
+#define MOL_MAX_FIELDS 8
+#define MOL_MAX_AUX 8
+#define MOL_MAX_SLOTS 64
+#define MOL_MAX_CLOSURE_K 24
+#define MOL_MAX_GHOST 6
+
+typedef enum {
+  MOL_OK = 0,
+  MOL_ERR_INVALID = -1,   /* malformed descriptor or argument */
+  MOL_ERR_SHAPE = -2,     /* size mismatch / out-of-range sizes */
+  MOL_ERR_HIP = -3,       /* a HIP runtime call failed */
+  MOL_ERR_UNSUPPORTED = -4,
+  MOL_ERR_NOMEM = -5
+} mol_status;
+
+typedef enum { MOL_F64 = 0, MOL_F32 = 1 } mol_dtype;
+
+/* ---- operator weight tables -------------------------------------------------
+ * One table per (operator, axis), weights already scaled by 1/dx^d.
+ * Interior rows: taps at i + off + k, k = 0..len-1, weights pool[w .. w+len).
+ * Rows near a non-periodic lower side: for i <= nlow, row r = i, taps 1..blen,
+ * weights pool[wlow + (r-1)*blen ..).  Near a non-periodic upper side, with
+ * lref = n + ref_shift: for i > lref - nhigh, row r = lref - i + hoff, taps
+ * lref-blen+1 .. lref, weights pool[whigh + (r-1)*blen ..).
+ * This is the row-selection logic of central_difference_weights_and_stencil
+ * (src/discretization/schemes/centered_difference/centered_difference.jl:5-30),
+ * _upwind_difference (schemes/upwind_difference/upwind_difference.jl:1-28) and
+ * get_half_offset_weights_and_stencil (schemes/half_offset_centred_difference.jl:9-37).
+ * Indices are 1-based full-grid node (or half-point) indices, as in the reference. */
+typedef struct {
+  int32_t len, off, w;
+  int32_t nlow, nhigh, blen, wlow, whigh;
+  int32_t hoff;       /* 1: node-centred high rows (r = lref-i+1); 0: half-offset (r = lref-i) */
+  int32_t ref_shift;  /* lref = n + ref_shift (0 normally; -1 for the nonlinear-Laplacian outer operator) */
+} mol_optab;
+
+/* ---- operator slots ---------------------------------------------------------- */
+typedef enum {
+  MOL_SLOT_CENTERED = 0,  /* tab[0]: D^d along axis (generate_cartesian_rules, centered_difference.jl:68-97) */
+  MOL_SLOT_UPWIND = 1,    /* tab[0]: forward (coef <= 0), tab[1]: backward (coef > 0); prog: coefficient,
+                             or -1 for a bare odd derivative (always backward) (upwind_difference.jl:169-199,278-302);
+                             the slot value is coef*D (coef is part of the slot) unless prog == -1 */
+  MOL_SLOT_WENO = 2,      /* Jiang-Shu WENO5 first derivative, eps = eps (WENO.jl:6-57) */
+  MOL_SLOT_NONLINLAP = 3, /* d/dx(a(...) d/dx u): tab[0]=outer, tab[1]=inner, tab[2]=interp; prog = a
+                             (nonlinear_laplacian.jl:28-103) */
+  MOL_SLOT_SPHERICAL = 4, /* r^-2 d/dr(r^2 a d/dr u): tab[0]=D1, tab[1]=D2, tab[2..4]=outer,inner,interp;
+                             prog = a (spherical_laplacian.jl:10-42) */
+  MOL_SLOT_MIXED = 5      /* d2/dx dy u: tab[0]=D1 along axis, tab[1]=D1 along axis2 (2nd_order_mixed_deriv.jl:5-22) */
+} mol_slot_kind;
+
+typedef struct {
+  int32_t kind;
+  int32_t field;          /* operand dependent variable */
+  int32_t axis, axis2;
+  int32_t tab[5];
+  int32_t prog;           /* program id or -1 */
+  double eps;             /* WENO epsilon */
+  double inv_dx;          /* WENO 1/dx */
+} mol_slot;
+
+/* ---- ghost-node closures ------------------------------------------------------
+ * For a non-periodic side of axis `axis` of variable `field`, the nodes between the
+ * domain edge and the unknown box (the boundary node, plus extrapolated nodes when the
+ * unknown box is clipped further) are affine in the first K unknowns of the same line:
+ *   u[ghost g] = sum_{m<K} C[g*K+m] * u[first + m*dir] + c[g] * G(t, boundary coords)
+ * where `first` is the unknown next to the side and dir = +1 (lower) / -1 (upper).
+ * ghost g = 0 is the node adjacent to the box, g = nghost-1 the domain edge node.
+ * G is program `gprog` (or 0 when gprog = -1).  This is what mtkcompile's alias
+ * elimination leaves of the BC / extrapolation equations of
+ * src/discretization/generate_bc_eqs.jl:24-58,238-311,336-392. */
+typedef struct {
+  int32_t field, axis, side;  /* side 0 = lower, 1 = upper */
+  int32_t nghost, K;
+  int32_t wC;                 /* pool offset of C (nghost*K) */
+  int32_t wc;                 /* pool offset of c (nghost) */
+  int32_t gprog;              /* program id of G or -1 */
+} mol_closure;
+
+/* ---- pointwise programs ---------------------------------------------------------
+ * Register-machine bytecode, one 32-bit word per instruction:
+ *   bits 0..7 opcode, 8..11 dst register, 12..15 a register, 16..19 b register,
+ *   bit 20: b is constant pool[imm] instead of a register, bits 21..31 imm.
+ * Programs are [start, start+len) ranges of the code pool; the result is register 0. */
+typedef enum {
+  MOL_OP_CONST = 0,  /* r[d] = cpool[imm] */
+  MOL_OP_FIELD = 1,  /* r[d] = value of variable imm in the current context */
+  MOL_OP_SLOT = 2,   /* r[d] = operator slot imm at the node */
+  MOL_OP_COORD = 3,  /* r[d] = coordinate along axis imm */
+  MOL_OP_TIME = 4,   /* r[d] = t */
+  MOL_OP_PARAM = 5,  /* r[d] = p[imm] */
+  MOL_OP_AUX = 6,    /* r[d] = auxiliary grid array imm */
+  MOL_OP_DINNER = 7, /* r[d] = inner half-point derivative (nonlinear Laplacian context) */
+  MOL_OP_MOV = 8,
+  MOL_OP_ADD = 16, MOL_OP_SUB = 17, MOL_OP_MUL = 18, MOL_OP_DIV = 19,
+  MOL_OP_POW = 20, MOL_OP_MIN = 21, MOL_OP_MAX = 22,
+  MOL_OP_LT = 23, MOL_OP_LE = 24, MOL_OP_GT = 25, MOL_OP_GE = 26, MOL_OP_EQ = 27, MOL_OP_NE = 28,
+  MOL_OP_AND = 29, MOL_OP_OR = 30,
+  MOL_OP_SELECT = 31, /* r[d] = r[d] != 0 ? r[a] : b   (d holds the condition) */
+  MOL_OP_FMA = 32,    /* r[d] = r[a] * b + r[d] */
+  MOL_OP_NEG = 48, MOL_OP_ABS = 49, MOL_OP_SQRT = 50, MOL_OP_EXP = 51, MOL_OP_LOG = 52,
+  MOL_OP_SIN = 53, MOL_OP_COS = 54, MOL_OP_TAN = 55, MOL_OP_TANH = 56, MOL_OP_SINH = 57,
+  MOL_OP_COSH = 58, MOL_OP_ATAN = 59, MOL_OP_ASIN = 60, MOL_OP_ACOS = 61, MOL_OP_SIGN = 62,
+  MOL_OP_POWI = 63,   /* r[d] = r[a]^imm (imm signed 11-bit) */
+  MOL_OP_ERF = 64, MOL_OP_LOG10 = 65, MOL_OP_CBRT = 66, MOL_OP_INV = 67, MOL_OP_SQ = 68,
+  MOL_OP_EXPM1 = 69, MOL_OP_LOG1P = 70, MOL_OP_FLOOR = 71, MOL_OP_CEIL = 72,
+  MOL_OP_HEAVISIDE = 73
+} mol_opcode;
+
+#define MOL_NREG 8
+
+/* ---- the descriptor -------------------------------------------------------------- */
+typedef struct {
+  int32_t abi_version;       /* MOL_ABI_VERSION */
+  int32_t ndim;              /* 1..3 spatial axes */
+  int32_t nfields;           /* dependent variables (ODE states) */
+  int32_t naux;              /* auxiliary full-grid arrays (e.g. a learnable diffusivity field) */
+  int32_t dtype;             /* mol_dtype of state and arithmetic */
+  int32_t nparams;           /* scalar runtime parameters p[] */
+  int64_t n[MOL_MAX_DIMS];   /* grid nodes per axis, both ends included */
+  int32_t periodic[MOL_MAX_DIMS];
+  const double* coord_h[MOL_MAX_DIMS]; /* node coordinates per axis (n[d] values) */
+  int64_t lo[MOL_MAX_FIELDS][MOL_MAX_DIMS]; /* unknown box per variable, 1-based inclusive */
+  int64_t hi[MOL_MAX_FIELDS][MOL_MAX_DIMS];
+  int32_t halo[MOL_MAX_DIMS];               /* largest tap reach of any slot along each axis */
+  int32_t nslots;        const mol_slot* slots_h;
+  int32_t ntabs;         const mol_optab* tabs_h;
+  int32_t nclosures;     const mol_closure* closures_h;
+  int32_t npool;         const double* pool_h;      /* weights / closure coefficients */
+  int32_t ncode;         const uint32_t* code_h;    /* bytecode pool */
+  int32_t ncpool;        const double* cpool_h;     /* program constants */
+  int32_t nprogs;        const int32_t* prog_h;     /* nprogs pairs (start, len) */
+  int32_t rhs_prog[MOL_MAX_FIELDS];                 /* program of each variable's right-hand side */
+} mol_desc;
+
+typedef struct mol_handle mol_handle;
+
+/* Replaces: the ODEFunction built at src/MOL_discretization.jl:171-201 (f = the
+ * RuntimeGeneratedFunction of ModelingToolkit's codegen).  Validates and uploads the
+ * descriptor, selects kernels. */
+int mol_create(const mol_desc* d, mol_handle** out);
+
+/* Replaces: f(du, u, p, t) — the in-place generated RHS (docs/src/generated/bruss_code.md:46-118,
+ * called by OrdinaryDiffEq through prob.f).  u, du: device arrays of unknowns;
+ * p: device array of nparams scalars (may be NULL when nparams == 0), of the state dtype;
+ * aux: device pointers of the naux full-grid arrays (host array of pointers, may be NULL). */
+int mol_rhs(mol_handle* h, void* du, const void* u, const void* p, double t, void* stream);
+
+/* Host-pointer convenience for the literal ODEFunction drop-in: copies in, runs, copies
+ * out, synchronizes. */
+int mol_rhs_h(mol_handle* h, void* du_h, const void* u_h, const void* p_h, double t);
+
+/* Bind auxiliary full-grid arrays (device pointers, naux of them, each prod(n) values). */
+int mol_set_aux(mol_handle* h, void* const* aux);
+
+typedef enum { MOL_RK4 = 0, MOL_SSPRK33 = 1 } mol_scheme;
+typedef enum { MOL_FLAG_KAHAN = 1, MOL_FLAG_GRAPH = 2 } mol_flags;
+
+/* Replaces: solve(prob, RK4() | SSPRK33(); dt, adaptive = false) of OrdinaryDiffEq as used in
+ * benchmark/weno/suite.jl:50-55 — the fixed-step stage loop, device-resident.  u is advanced in
+ * place from t0 by nsteps steps of size dt.  MOL_FLAG_KAHAN: compensated (Kahan) accumulation of
+ * the step update (FP32 state).  MOL_FLAG_GRAPH: stages captured once in a hipGraph and replayed. */
+int mol_integrate(mol_handle* h, void* u, const void* p, double t0, double dt, int64_t nsteps,
+                  int32_t scheme, int32_t flags, void* stream);
+
+/* Number of unknowns (sum over variables of the unknown box sizes). */
+int64_t mol_num_unknowns(const mol_handle* h);
+
+/* Per-handle error message of the last failing call (or the last mol_create failure when h is NULL). */
+const char* mol_last_error(const mol_handle* h);
+
+/* Replaces: nothing to free in the reference (GC); releases device memory of the handle. */
+void mol_destroy(mol_handle* h);
+
+/* Library / device information (for test harnesses). */
+int mol_device_info(char* buf, int buflen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MOL_H */

@@ -1,0 +1,468 @@
+// C ABI of the stencil library: descriptor validation/upload, RHS and stage-loop drivers.
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "mol_internal.h"
+
+namespace mol {
+
+static thread_local std::string g_err;
+void set_global_error(const std::string& msg) { g_err = msg; }
+const char* global_error() { return g_err.c_str(); }
+
+}  // namespace mol
+
+using namespace mol;
+
+namespace {
+
+#define HIPCHK(h, call)                                                           \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      (h)->err = std::string(#call) + ": " + hipGetErrorString(e_);               \
+      return MOL_ERR_HIP;                                                         \
+    }                                                                             \
+  } while (0)
+
+int fail(mol_handle* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  else set_global_error(msg);
+  return code;
+}
+
+template <typename T>
+int upload(mol_handle* h, void** dst, const T* src, size_t count) {
+  if (count == 0) {
+    *dst = nullptr;
+    return 0;
+  }
+  HIPCHK(h, hipMalloc(dst, count * sizeof(T)));
+  HIPCHK(h, hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+void free_handle(mol_handle* h) {
+  if (!h) return;
+  void* ptrs[] = {h->d_tabs, h->d_slots, h->d_clos, h->d_pool, h->d_code, h->d_cpool,
+                  h->d_prog, h->d_coord[0], h->d_coord[1], h->d_coord[2], h->w_y1,
+                  h->w_y2, h->w_acc, h->w_comp, h->d_tnow, h->d_ka64, h->d_ka32};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (h->h_u) (void)hipHostFree(h->h_u);
+  if (h->h_du) (void)hipHostFree(h->h_du);
+  if (h->h_p) (void)hipHostFree(h->h_p);
+  delete h;
+}
+
+// Structural validation of a descriptor against the limits the kernels assume.
+int validate(const mol_desc* d, std::string& msg) {
+  if (!d) { msg = "descriptor is NULL"; return MOL_ERR_INVALID; }
+  if (d->abi_version != MOL_ABI_VERSION) { msg = "ABI version mismatch"; return MOL_ERR_INVALID; }
+  if (d->ndim < 1 || d->ndim > MOL_MAX_DIMS) { msg = "ndim must be 1..3"; return MOL_ERR_INVALID; }
+  if (d->nfields < 1 || d->nfields > MOL_MAX_FIELDS) { msg = "nfields out of range"; return MOL_ERR_INVALID; }
+  if (d->naux < 0 || d->naux > MOL_MAX_AUX) { msg = "naux out of range"; return MOL_ERR_INVALID; }
+  if (d->dtype != MOL_F64 && d->dtype != MOL_F32) { msg = "dtype must be MOL_F64 or MOL_F32"; return MOL_ERR_INVALID; }
+  if (d->nparams < 0) { msg = "nparams < 0"; return MOL_ERR_INVALID; }
+  for (int a = 0; a < d->ndim; ++a) {
+    if (d->n[a] < 2 || d->n[a] > (1 << 28)) { msg = "grid size out of range"; return MOL_ERR_SHAPE; }
+    if (!d->coord_h[a]) { msg = "missing coordinates"; return MOL_ERR_INVALID; }
+    if (d->halo[a] < 0 || d->halo[a] > 16) { msg = "halo out of range"; return MOL_ERR_INVALID; }
+    if (d->periodic[a] && d->n[a] - 1 < 2 * d->halo[a] + 1) { msg = "periodic ring shorter than the stencil"; return MOL_ERR_SHAPE; }
+  }
+  for (int f = 0; f < d->nfields; ++f)
+    for (int a = 0; a < d->ndim; ++a) {
+      if (d->lo[f][a] < 1 || d->hi[f][a] > d->n[a] || d->lo[f][a] > d->hi[f][a]) {
+        msg = "unknown box out of the grid";
+        return MOL_ERR_SHAPE;
+      }
+      if (d->periodic[a] && (d->lo[f][a] != 2 || d->hi[f][a] != d->n[a])) {
+        msg = "periodic axis must hold nodes 2..n as unknowns";
+        return MOL_ERR_SHAPE;
+      }
+    }
+  if (d->nslots < 0 || d->nslots > MOL_MAX_SLOTS) { msg = "nslots out of range"; return MOL_ERR_INVALID; }
+  if (d->nprogs < 0 || (d->nprogs > 0 && !d->prog_h)) { msg = "programs missing"; return MOL_ERR_INVALID; }
+  for (int i = 0; i < d->nprogs; ++i) {
+    int s = d->prog_h[2 * i], l = d->prog_h[2 * i + 1];
+    if (s < 0 || l < 0 || s + l > d->ncode) { msg = "program range out of the code pool"; return MOL_ERR_INVALID; }
+  }
+  for (int pc = 0; pc < d->ncode; ++pc) {
+    uint32_t ins = d->code_h[pc];
+    uint32_t op = ins & 0xff;
+    int imm = (int)(ins >> 21);
+    bool bconst = (ins >> 20) & 1u;
+    if (bconst && imm >= d->ncpool) { msg = "constant index out of range"; return MOL_ERR_INVALID; }
+    if (op == MOL_OP_CONST && imm >= d->ncpool) { msg = "constant index out of range"; return MOL_ERR_INVALID; }
+    if (op == MOL_OP_FIELD && imm >= d->nfields) { msg = "variable index out of range"; return MOL_ERR_INVALID; }
+    if (op == MOL_OP_SLOT && imm >= d->nslots) { msg = "slot index out of range"; return MOL_ERR_INVALID; }
+    if (op == MOL_OP_COORD && imm >= d->ndim) { msg = "axis index out of range"; return MOL_ERR_INVALID; }
+    if (op == MOL_OP_PARAM && imm >= d->nparams) { msg = "parameter index out of range"; return MOL_ERR_INVALID; }
+    if (op == MOL_OP_AUX && imm >= d->naux) { msg = "aux index out of range"; return MOL_ERR_INVALID; }
+  }
+  for (int f = 0; f < d->nfields; ++f)
+    if (d->rhs_prog[f] < 0 || d->rhs_prog[f] >= d->nprogs) { msg = "rhs program missing"; return MOL_ERR_INVALID; }
+  auto tab_ok = [&](int t) { return t >= 0 && t < d->ntabs; };
+  for (int k = 0; k < d->nslots; ++k) {
+    const mol_slot& s = d->slots_h[k];
+    if (s.field < 0 || s.field >= d->nfields || s.axis < 0 || s.axis >= d->ndim) { msg = "slot operand out of range"; return MOL_ERR_INVALID; }
+    if (s.prog >= d->nprogs) { msg = "slot program out of range"; return MOL_ERR_INVALID; }
+    int ntab = 0;
+    switch (s.kind) {
+      case MOL_SLOT_CENTERED: ntab = 1; break;
+      case MOL_SLOT_UPWIND: ntab = 2; break;
+      case MOL_SLOT_WENO: ntab = 0; break;
+      case MOL_SLOT_NONLINLAP: ntab = 3; break;
+      case MOL_SLOT_SPHERICAL: ntab = 5; break;
+      case MOL_SLOT_MIXED:
+        ntab = 2;
+        if (s.axis2 < 0 || s.axis2 >= d->ndim || s.axis2 == s.axis) { msg = "mixed slot axes"; return MOL_ERR_INVALID; }
+        break;
+      default: msg = "unknown slot kind"; return MOL_ERR_INVALID;
+    }
+    for (int q = 0; q < ntab; ++q)
+      if (!tab_ok(s.tab[q])) { msg = "slot table out of range"; return MOL_ERR_INVALID; }
+  }
+  for (int t = 0; t < d->ntabs; ++t) {
+    const mol_optab& tb = d->tabs_h[t];
+    if (tb.len < 0 || tb.w < 0 || tb.w + tb.len > d->npool) { msg = "table weights out of the pool"; return MOL_ERR_INVALID; }
+    if (tb.nlow < 0 || tb.nhigh < 0 || tb.blen < 0) { msg = "table rows"; return MOL_ERR_INVALID; }
+    if (tb.nlow && tb.wlow + tb.nlow * tb.blen > d->npool) { msg = "table low rows out of the pool"; return MOL_ERR_INVALID; }
+    if (tb.nhigh && tb.whigh + tb.nhigh * tb.blen > d->npool) { msg = "table high rows out of the pool"; return MOL_ERR_INVALID; }
+  }
+  for (int c = 0; c < d->nclosures; ++c) {
+    const mol_closure& cl = d->closures_h[c];
+    if (cl.field < 0 || cl.field >= d->nfields || cl.axis < 0 || cl.axis >= d->ndim || cl.side < 0 || cl.side > 1) { msg = "closure target"; return MOL_ERR_INVALID; }
+    if (cl.K < 0 || cl.K > MOL_MAX_CLOSURE_K || cl.nghost < 0 || cl.nghost > MOL_MAX_GHOST) { msg = "closure size"; return MOL_ERR_INVALID; }
+    int64_t boxlen = d->hi[cl.field][cl.axis] - d->lo[cl.field][cl.axis] + 1;
+    if (cl.K > boxlen) { msg = "closure reaches past the unknown box"; return MOL_ERR_SHAPE; }
+    if (cl.wC + cl.nghost * cl.K > d->npool || cl.wc + cl.nghost > d->npool) { msg = "closure weights out of the pool"; return MOL_ERR_INVALID; }
+    if (cl.gprog >= d->nprogs) { msg = "closure program out of range"; return MOL_ERR_INVALID; }
+    if (d->periodic[cl.axis]) { msg = "closure on a periodic axis"; return MOL_ERR_INVALID; }
+  }
+  return 0;
+}
+
+template <typename T>
+void fill_kargs(mol_handle* h, const mol_desc* d, KArgs<T>& ka) {
+  std::memset(&ka, 0, sizeof(ka));
+  ka.ndim = d->ndim;
+  ka.nfields = d->nfields;
+  ka.naux = d->naux;
+  ka.nparams = d->nparams;
+  int64_t off = 0;
+  for (int a = 0; a < 3; ++a) {
+    ka.n[a] = a < d->ndim ? (int32_t)d->n[a] : 1;
+    ka.periodic[a] = a < d->ndim ? d->periodic[a] : 0;
+    ka.halo[a] = a < d->ndim ? d->halo[a] : 0;
+    ka.ulo[a] = 1 << 30;
+    ka.uhi[a] = 0;
+  }
+  for (int f = 0; f < d->nfields; ++f) {
+    int64_t sz = 1;
+    for (int a = 0; a < 3; ++a) {
+      ka.lo[f][a] = a < d->ndim ? (int32_t)d->lo[f][a] : 1;
+      ka.hi[f][a] = a < d->ndim ? (int32_t)d->hi[f][a] : 1;
+      sz *= ka.hi[f][a] - ka.lo[f][a] + 1;
+      ka.ulo[a] = std::min(ka.ulo[a], ka.lo[f][a]);
+      ka.uhi[a] = std::max(ka.uhi[a], ka.hi[f][a]);
+    }
+    ka.fofs[f] = off;
+    off += sz;
+    ka.rhs_prog[f] = d->rhs_prog[f];
+    for (int a = 0; a < 3; ++a) ka.clos_id[f][a][0] = ka.clos_id[f][a][1] = -1;
+  }
+  h->nunk = off;
+  for (int c = 0; c < d->nclosures; ++c) {
+    const mol_closure& cl = d->closures_h[c];
+    ka.clos_id[cl.field][cl.axis][cl.side] = c;
+  }
+  // tile shape: contiguous along x, sized so that tile + halo of every staged array fits LDS
+  int tile[3] = {256, 1, 1};
+  if (d->ndim == 2) { tile[0] = 64; tile[1] = 16; }
+  if (d->ndim == 3) { tile[0] = 32; tile[1] = 8; tile[2] = 4; }
+  const int nload = d->nfields + d->naux;
+  auto vol_of = [&](const int* tl) {
+    int v = 1;
+    for (int a = 0; a < 3; ++a) v *= tl[a] + 2 * ka.halo[a];
+    return v;
+  };
+  for (int a = 0; a < 3; ++a) tile[a] = std::max(1, std::min(tile[a], ka.uhi[a] - ka.ulo[a] + 1));
+  while ((size_t)nload * vol_of(tile) * sizeof(T) > 60000) {
+    int big = 0;
+    for (int a = 1; a < 3; ++a)
+      if (tile[a] > tile[big] / 4) big = a;
+    if (tile[big] <= 1) break;
+    tile[big] = (tile[big] + 1) / 2;
+  }
+  for (int a = 0; a < 3; ++a) {
+    ka.tile[a] = tile[a];
+    ka.ntile[a] = (ka.uhi[a] - ka.ulo[a] + 1 + tile[a] - 1) / tile[a];
+    ka.ext[a] = tile[a] + 2 * ka.halo[a];
+  }
+  ka.vol = ka.ext[0] * ka.ext[1] * ka.ext[2];
+  for (int a = 0; a < 3; ++a) ka.coord[a] = (const double*)h->d_coord[a];
+  ka.tabs = (const Tab*)h->d_tabs;
+  ka.slots = (const Slot*)h->d_slots;
+  ka.clos = (const Clos*)h->d_clos;
+  ka.pool = (const double*)h->d_pool;
+  ka.code = (const uint32_t*)h->d_code;
+  ka.cpool = (const double*)h->d_cpool;
+  ka.prog = (const int32_t*)h->d_prog;
+}
+
+int ensure_workspace(mol_handle* h) {
+  const size_t bytes = (size_t)h->nunk * h->esize();
+  if (!h->w_y1) HIPCHK(h, hipMalloc(&h->w_y1, bytes));
+  if (!h->w_y2) HIPCHK(h, hipMalloc(&h->w_y2, bytes));
+  if (!h->w_acc) HIPCHK(h, hipMalloc(&h->w_acc, bytes));
+  if (!h->w_comp) HIPCHK(h, hipMalloc(&h->w_comp, bytes));
+  if (!h->d_tnow) HIPCHK(h, hipMalloc((void**)&h->d_tnow, sizeof(double)));
+  return 0;
+}
+
+template <typename T>
+KArgs<T>& kargs(mol_handle* h);
+template <>
+KArgs<double>& kargs<double>(mol_handle* h) { return h->ka64; }
+template <>
+KArgs<float>& kargs<float>(mol_handle* h) { return h->ka32; }
+template <typename T>
+KArgs<T>* dkargs(mol_handle* h);
+template <>
+KArgs<double>* dkargs<double>(mol_handle* h) { return h->d_ka64; }
+template <>
+KArgs<float>* dkargs<float>(mol_handle* h) { return h->d_ka32; }
+
+// Upload the host kernel-argument blocks to their device copies.
+int sync_kargs(mol_handle* h) {
+  if (!h->d_ka64) HIPCHK(h, hipMalloc((void**)&h->d_ka64, sizeof(KArgs<double>)));
+  if (!h->d_ka32) HIPCHK(h, hipMalloc((void**)&h->d_ka32, sizeof(KArgs<float>)));
+  HIPCHK(h, hipMemcpy(h->d_ka64, &h->ka64, sizeof(KArgs<double>), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->d_ka32, &h->ka32, sizeof(KArgs<float>), hipMemcpyHostToDevice));
+  return 0;
+}
+
+template <typename T>
+int enqueue_steps(mol_handle* h, T* u, const T* p, double dt, int64_t nsteps, int scheme,
+                  bool kahan, hipStream_t s) {
+  KArgs<T>& ka = kargs<T>(h);
+  KArgs<T>* dka = dkargs<T>(h);
+  T* y1 = (T*)h->w_y1;
+  T* y2 = (T*)h->w_y2;
+  T* acc = (T*)h->w_acc;
+  T* comp = (T*)h->w_comp;
+  for (int64_t step = 0; step < nsteps; ++step) {
+    Epi<T> e{};
+    e.tnow = h->d_tnow;
+    e.p = p;
+    int rc = 0;
+    if (scheme == MOL_RK4) {
+      e.mode = 1; e.out = y1; e.A = u; e.ca = 1.0; e.cf = dt / 2; e.acc = acc; e.acc_keep = 0.0; e.acc_f = 1.0; e.tofs = 0.0;
+      rc |= launch_rhs_generic<T>(ka, dka, u, e, s);
+      e.out = y2; e.cf = dt / 2; e.acc_keep = 1.0; e.acc_f = 2.0; e.tofs = dt / 2;
+      rc |= launch_rhs_generic<T>(ka, dka, y1, e, s);
+      e.out = y1; e.cf = dt; e.tofs = dt / 2;
+      rc |= launch_rhs_generic<T>(ka, dka, y2, e, s);
+      Epi<T> l{};
+      l.tnow = h->d_tnow; l.p = p; l.tofs = dt; l.out = u; l.A = u; l.B = acc; l.cb = dt / 6; l.cf = dt / 6;
+      if (kahan) { l.mode = 2; l.subA = 0; l.comp = comp; }
+      else { l.mode = 1; l.ca = 1.0; }
+      rc |= launch_rhs_generic<T>(ka, dka, y1, l, s);
+    } else {
+      e.mode = 1; e.out = y1; e.A = u; e.ca = 1.0; e.cf = dt; e.tofs = 0.0;
+      rc |= launch_rhs_generic<T>(ka, dka, u, e, s);
+      e.out = y2; e.ca = 0.75; e.B = y1; e.cb = 0.25; e.cf = dt / 4; e.tofs = dt;
+      rc |= launch_rhs_generic<T>(ka, dka, y1, e, s);
+      Epi<T> l{};
+      l.tnow = h->d_tnow; l.p = p; l.tofs = dt / 2; l.out = u; l.A = u; l.B = y2;
+      l.cb = 2.0 / 3.0; l.cf = 2.0 * dt / 3.0;
+      if (kahan) { l.mode = 2; l.subA = 1; l.comp = comp; }
+      else { l.mode = 1; l.ca = 1.0 / 3.0; }
+      rc |= launch_rhs_generic<T>(ka, dka, y2, l, s);
+    }
+    rc |= launch_advance_time(h->d_tnow, dt, s);
+    if (rc) return fail(h, MOL_ERR_HIP, std::string("kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
+  }
+  return 0;
+}
+
+template <typename T>
+int integrate_impl(mol_handle* h, T* u, const T* p, double t0, double dt, int64_t nsteps,
+                   int scheme, int flags, hipStream_t s) {
+  int rc = ensure_workspace(h);
+  if (rc) return rc;
+  const bool kahan = (flags & MOL_FLAG_KAHAN) != 0;
+  HIPCHK(h, hipMemcpyAsync(h->d_tnow, &t0, sizeof(double), hipMemcpyHostToDevice, s));
+  if (kahan) HIPCHK(h, hipMemsetAsync(h->w_comp, 0, (size_t)h->nunk * h->esize(), s));
+  if (!(flags & MOL_FLAG_GRAPH) || nsteps < 4) {
+    return enqueue_steps<T>(h, u, p, dt, nsteps, scheme, kahan, s);
+  }
+  // capture a block of steps once, replay it; remainder launched directly
+  const int64_t chunk = std::min<int64_t>(nsteps, 64);
+  hipStream_t cs;
+  HIPCHK(h, hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  HIPCHK(h, hipStreamSynchronize(s));
+  HIPCHK(h, hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  rc = enqueue_steps<T>(h, u, p, dt, chunk, scheme, kahan, cs);
+  hipError_t ee = hipStreamEndCapture(cs, &g);
+  if (rc || ee != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipStreamDestroy(cs);
+    return rc ? rc : fail(h, MOL_ERR_HIP, "graph capture failed");
+  }
+  HIPCHK(h, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+  const int64_t reps = nsteps / chunk;
+  for (int64_t r = 0; r < reps; ++r) HIPCHK(h, hipGraphLaunch(ge, s));
+  rc = enqueue_steps<T>(h, u, p, dt, nsteps - reps * chunk, scheme, kahan, s);
+  HIPCHK(h, hipStreamSynchronize(s));
+  (void)hipGraphExecDestroy(ge);
+  (void)hipGraphDestroy(g);
+  (void)hipStreamDestroy(cs);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mol_create(const mol_desc* d, mol_handle** out) {
+  if (!out) return fail(nullptr, MOL_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  std::string msg;
+  int rc = validate(d, msg);
+  if (rc) return fail(nullptr, rc, msg);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(nullptr, MOL_ERR_HIP, "no HIP device available");
+  mol_handle* h = new mol_handle();
+  (void)hipGetDevice(&h->device);
+  h->dtype = d->dtype;
+  h->ndim = d->ndim;
+  h->nfields = d->nfields;
+  h->naux = d->naux;
+  h->nparams = d->nparams;
+  for (int a = 0; a < d->ndim; ++a) h->ngrid *= d->n[a];
+  int r = 0;
+  r |= upload(h, &h->d_tabs, d->tabs_h, d->ntabs);
+  r |= upload(h, &h->d_slots, d->slots_h, d->nslots);
+  r |= upload(h, &h->d_clos, d->closures_h, d->nclosures);
+  r |= upload(h, &h->d_pool, d->pool_h, d->npool);
+  r |= upload(h, &h->d_code, d->code_h, d->ncode);
+  r |= upload(h, &h->d_cpool, d->cpool_h, d->ncpool);
+  r |= upload(h, &h->d_prog, d->prog_h, 2 * (size_t)d->nprogs);
+  for (int a = 0; a < d->ndim; ++a) r |= upload(h, &h->d_coord[a], d->coord_h[a], (size_t)d->n[a]);
+  if (r) {
+    set_global_error(h->err);
+    free_handle(h);
+    return MOL_ERR_HIP;
+  }
+  fill_kargs<double>(h, d, h->ka64);
+  fill_kargs<float>(h, d, h->ka32);
+  if (sync_kargs(h)) {
+    set_global_error(h->err);
+    free_handle(h);
+    return MOL_ERR_HIP;
+  }
+  *out = h;
+  return 0;
+}
+
+int mol_set_aux(mol_handle* h, void* const* aux) {
+  if (!h) return fail(nullptr, MOL_ERR_INVALID, "handle is NULL");
+  if (h->naux > 0 && !aux) return fail(h, MOL_ERR_INVALID, "aux pointers missing");
+  for (int a = 0; a < h->naux; ++a) {
+    if (!aux[a]) return fail(h, MOL_ERR_INVALID, "aux pointer is NULL");
+    h->ka64.aux[a] = (const double*)aux[a];
+    h->ka32.aux[a] = (const float*)aux[a];
+  }
+  return sync_kargs(h);
+}
+
+static int check_aux(mol_handle* h) {
+  for (int a = 0; a < h->naux; ++a)
+    if (!h->ka64.aux[a]) return fail(h, MOL_ERR_INVALID, "auxiliary arrays not bound (mol_set_aux)");
+  return 0;
+}
+
+int mol_rhs(mol_handle* h, void* du, const void* u, const void* p, double t, void* stream) {
+  if (!h) return fail(nullptr, MOL_ERR_INVALID, "handle is NULL");
+  if (!du || !u) return fail(h, MOL_ERR_INVALID, "du/u is NULL");
+  if (h->nparams > 0 && !p) return fail(h, MOL_ERR_INVALID, "parameters missing");
+  if (int rc = check_aux(h)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (h->dtype == MOL_F64) {
+    Epi<double> e{};
+    e.mode = 0; e.out = (double*)du; e.tofs = t; e.p = (const double*)p;
+    if (launch_rhs_generic<double>(h->ka64, h->d_ka64, (const double*)u, e, s))
+      return fail(h, MOL_ERR_HIP, std::string("rhs launch: ") + hipGetErrorString(hipGetLastError()));
+  } else {
+    Epi<float> e{};
+    e.mode = 0; e.out = (float*)du; e.tofs = t; e.p = (const float*)p;
+    if (launch_rhs_generic<float>(h->ka32, h->d_ka32, (const float*)u, e, s))
+      return fail(h, MOL_ERR_HIP, std::string("rhs launch: ") + hipGetErrorString(hipGetLastError()));
+  }
+  return 0;
+}
+
+int mol_rhs_h(mol_handle* h, void* du_h, const void* u_h, const void* p_h, double t) {
+  if (!h) return fail(nullptr, MOL_ERR_INVALID, "handle is NULL");
+  if (!du_h || !u_h) return fail(h, MOL_ERR_INVALID, "du/u is NULL");
+  const size_t bytes = (size_t)h->nunk * h->esize();
+  void *du = nullptr, *u = nullptr, *p = nullptr;
+  HIPCHK(h, hipMalloc(&du, bytes));
+  HIPCHK(h, hipMalloc(&u, bytes));
+  if (h->nparams > 0) {
+    HIPCHK(h, hipMalloc(&p, h->nparams * h->esize()));
+    HIPCHK(h, hipMemcpy(p, p_h, h->nparams * h->esize(), hipMemcpyHostToDevice));
+  }
+  HIPCHK(h, hipMemcpy(u, u_h, bytes, hipMemcpyHostToDevice));
+  int rc = mol_rhs(h, du, u, p, t, nullptr);
+  if (rc == 0) {
+    HIPCHK(h, hipDeviceSynchronize());
+    HIPCHK(h, hipMemcpy(du_h, du, bytes, hipMemcpyDeviceToHost));
+  }
+  (void)hipFree(du);
+  (void)hipFree(u);
+  if (p) (void)hipFree(p);
+  return rc;
+}
+
+int mol_integrate(mol_handle* h, void* u, const void* p, double t0, double dt, int64_t nsteps,
+                  int32_t scheme, int32_t flags, void* stream) {
+  if (!h) return fail(nullptr, MOL_ERR_INVALID, "handle is NULL");
+  if (!u) return fail(h, MOL_ERR_INVALID, "u is NULL");
+  if (nsteps < 0) return fail(h, MOL_ERR_INVALID, "nsteps < 0");
+  if (scheme != MOL_RK4 && scheme != MOL_SSPRK33) return fail(h, MOL_ERR_INVALID, "unknown scheme");
+  if (h->nparams > 0 && !p) return fail(h, MOL_ERR_INVALID, "parameters missing");
+  if (int rc = check_aux(h)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (h->dtype == MOL_F64)
+    return integrate_impl<double>(h, (double*)u, (const double*)p, t0, dt, nsteps, scheme, flags, s);
+  return integrate_impl<float>(h, (float*)u, (const float*)p, t0, dt, nsteps, scheme, flags, s);
+}
+
+int64_t mol_num_unknowns(const mol_handle* h) { return h ? h->nunk : -1; }
+
+const char* mol_last_error(const mol_handle* h) {
+  if (h) return h->err.c_str();
+  return global_error();
+}
+
+void mol_destroy(mol_handle* h) { free_handle(h); }
+
+int mol_device_info(char* buf, int buflen) {
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    return MOL_ERR_HIP;
+  if (buf && buflen > 0)
+    std::snprintf(buf, buflen, "%s gcn=%s cus=%d", prop.name, prop.gcnArchName,
+                  prop.multiProcessorCount);
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,183 @@
+"""Finite-difference weights (Fornberg 1988) and the derivative-operator tables built on them.
+
+Host-side setup of the stencil IR.  The arithmetic follows the reference loop for loop so
+that the weights agree to the last bit:
+
+* ``calculate_weights``  — src/discretization/schemes/fornberg_calculate_weights.jl:20-78
+  (including the sum-to-zero correction on the middle entry, :64).
+* ``centered_difference`` — CompleteCenteredDifference,
+  schemes/centered_difference/centered_diff_weights.jl:4-75.
+* ``upwind_difference`` — CompleteUpwindDifference, schemes/upwind_difference/upwind_diff_weights.jl:4-91.
+* ``half_centered_difference`` — CompleteHalfCenteredDifference, schemes/half_offset_weights.jl:4-70.
+* ``boundary_interp_extrap`` — BoundaryInterpolatorExtrapolator, schemes/extrapolation_weights.jl:1-72.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List
+
+import numpy as np
+
+
+def calculate_weights(order: int, x0: float, x) -> np.ndarray:
+    """Weights approximating the ``order``-th derivative at ``x0`` from samples at ``x``.
+
+    Fornberg (1988), Math. Comp. 51, 699-706, restated from
+    src/discretization/schemes/fornberg_calculate_weights.jl:20-78.  For ``order != 0``
+    the entry ``N//2`` (0-based) is corrected so that the weights sum to exactly zero
+    (fornberg_calculate_weights.jl:64).
+    """
+    x = [float(v) for v in x]
+    x0 = float(x0)
+    N = len(x)
+    if not order < N:
+        raise ValueError("Not enough points for the requested order.")
+    M = order
+    c1 = 1.0
+    c4 = x[0] - x0
+    C = [[0.0] * (M + 1) for _ in range(N)]
+    C[0][0] = 1.0
+    for i in range(1, N):
+        mn = min(i, M)
+        c2 = 1.0
+        c5 = c4
+        c4 = x[i] - x0
+        for j in range(0, i):
+            c3 = x[i] - x[j]
+            c2 *= c3
+            if j == i - 1:
+                for s in range(mn, 0, -1):
+                    C[i][s] = c1 * (s * C[i - 1][s - 1] - c5 * C[i - 1][s]) / c2
+                C[i][0] = -c1 * c5 * C[i - 1][0] / c2
+            for s in range(mn, 0, -1):
+                C[j][s] = (c4 * C[j][s] - s * C[j][s - 1]) / c3
+            C[j][0] = c4 * C[j][0] / c3
+        c1 = c2
+    w = [C[k][M] for k in range(N)]
+    if order != 0:
+        # Julia's sum() is pairwise for arrays; for the short stencils used here (N <= 16)
+        # it reduces left to right in a single block.
+        s = 0.0
+        for v in w:
+            s += v
+        w[N // 2] -= s
+    return np.array(w, dtype=np.float64)
+
+
+@dataclass
+class DerivativeOperator:
+    """Weights of one derivative operator along one axis (weights already scaled by 1/dx^d).
+
+    Mirrors the fields of DerivativeOperator (src/discretization/derivative_operator.jl:1-15)
+    that the right-hand side uses.
+    """
+    derivative_order: int
+    approximation_order: int
+    dx: float
+    stencil_length: int
+    stencil_coefs: np.ndarray
+    boundary_stencil_length: int
+    boundary_point_count: int
+    low_boundary_coefs: List[np.ndarray] = field(default_factory=list)
+    high_boundary_coefs: List[np.ndarray] = field(default_factory=list)
+    offside: int = 0
+    kind: str = "centered"      # centered | upwind | half | extrap
+    tap_offset: int = 0          # interior taps at i + tap_offset + k
+    nlow: int = 0                # low rows apply for i <= nlow
+    nhigh: int = 0               # high rows apply for i > lref - nhigh
+    hoff: int = 1                # high-row index: r = lref - i + hoff
+
+
+def _scale(w: np.ndarray, dx: float, d: int) -> np.ndarray:
+    # (1 / dx^d) * w, evaluated as Julia does: the scalar 1/dx^d first, then the product.
+    return (1.0 / (dx ** d)) * w if d > 0 else (1.0 / (dx ** 0)) * w
+
+
+def centered_difference(d: int, p: int, dx: float) -> DerivativeOperator:
+    """CompleteCenteredDifference(d, p, dx), uniform grid (centered_diff_weights.jl:4-75)."""
+    if p <= 1:
+        raise ValueError("approximation_order must be greater than 1.")
+    L = d + p - 1 + (d + p) % 2
+    BL = d + p
+    half = L // 2
+    dummy_x = list(range(-half, half + 1))
+    left_boundary_x = list(range(0, BL))
+    bpc = half
+    stencil = (1.0 / dx ** d) * calculate_weights(d, 0.0, dummy_x)
+    low = [(1.0 / dx ** d) * calculate_weights(d, 1.0 * x0, left_boundary_x)
+           for x0 in left_boundary_x[:half]]
+    high = [v[::-1] * ((-1.0) ** d) for v in low]
+    return DerivativeOperator(d, p, dx, L, stencil, BL, bpc, low, high, 0, "centered",
+                              tap_offset=-half, nlow=bpc, nhigh=bpc, hoff=1)
+
+
+def upwind_difference(d: int, p: int, dx: float, offside: int = 0) -> DerivativeOperator:
+    """CompleteUpwindDifference(d, p, dx, offside), uniform grid (upwind_diff_weights.jl:4-91)."""
+    if offside < 0:
+        raise ValueError("Number of offside points should be non-negative")
+    L = d + p
+    BL = d + p
+    low_count = offside
+    high_count = L - 1 - offside
+    dummy_x = [0.0 - offside + k for k in range(L)]
+    stencil = (1.0 / dx ** d) * calculate_weights(d, 0.0, dummy_x)
+    low_x = [float(k) for k in range(BL)]
+    low = [(1.0 / dx ** d) * calculate_weights(d, 1.0 * x0, low_x)
+           for x0 in [float(k) for k in range(low_count)]]
+    high_x = [-float(k) for k in range(BL)]
+    _high = [((-1.0 / dx) ** d) * calculate_weights(d, 1.0 * x0, high_x)
+             for x0 in [-float(k) for k in range(high_count)]]
+    high = list(reversed(_high))
+    # Row selection of _upwind_difference (upwind_difference.jl:1-28): the positive (backward)
+    # operator uses low rows for i <= offside, the negative (forward) one high rows for
+    # i > n - boundary_point_count with boundary_point_count = high_count.
+    return DerivativeOperator(d, p, dx, L, stencil, BL, high_count, low, high, offside,
+                              "upwind", tap_offset=-offside, nlow=offside, nhigh=high_count,
+                              hoff=1)
+
+
+def half_centered_difference(d: int, p: int, dx: float) -> DerivativeOperator:
+    """CompleteHalfCenteredDifference(d, p, dx), uniform grid (half_offset_weights.jl:4-70).
+
+    Index ``i`` of this operator is a half point between nodes i and i+1.
+    """
+    if p <= 1:
+        raise ValueError("approximation_order must be greater than 1.")
+    L = p + 2 * (d // 2) + (p % 2)
+    BL = d + p
+    endpoint = L // 2
+    dummy_x = list(range(1 - endpoint, endpoint + 1))
+    left_boundary_x = list(range(1, BL + 1))
+    bpc = L // 2
+    xoffset = [1.5 + k for k in range(bpc)]
+    stencil = (1.0 / dx ** d) * calculate_weights(d, 0.5, dummy_x)
+    low = [(1.0 / dx ** d) * calculate_weights(d, off, left_boundary_x) for off in xoffset[:L // 2]]
+    high = [v[::-1] * ((-1.0) ** d) for v in low]
+    return DerivativeOperator(d, p, dx, L, stencil, BL, bpc, low, high, 0, "half",
+                              tap_offset=1 - L // 2, nlow=bpc, nhigh=bpc, hoff=0)
+
+
+def boundary_interp_extrap(p: int, dx: float) -> DerivativeOperator:
+    """BoundaryInterpolatorExtrapolator(p, dx) (extrapolation_weights.jl:1-72): 0th-order
+    Lagrange extrapolation excluding the node itself (weight 0 inserted at its position)."""
+    if p <= 1:
+        raise ValueError("approximation_order must be greater than 1.")
+    L = p - 1 + p % 2
+    BL = p
+    half = L // 2
+    dummy_x = list(range(-half, half + 1))
+    left_boundary_x = list(range(0, BL))
+    bpc = half
+    inner = [v for v in dummy_x if v != 0]
+    w = list(calculate_weights(0, 0.0, inner))
+    w.insert(dummy_x.index(0), 0.0)
+    stencil = np.array(w, dtype=np.float64)
+    low = []
+    for i, x0 in enumerate(left_boundary_x[:bpc]):
+        pts = [v for v in left_boundary_x if v != x0]
+        ww = list(calculate_weights(0, 1.0 * x0, pts))
+        ww.insert(i, 0.0)
+        low.append(np.array(ww, dtype=np.float64))
+    high = [v[::-1].copy() for v in low]
+    return DerivativeOperator(0, p, dx, L, stencil, BL, bpc, low, high, 0, "extrap",
+                              tap_offset=-half, nlow=bpc, nhigh=bpc, hoff=1)

@@ -1,0 +1,248 @@
+"""``discretize`` / ``ODEProblem`` / ``solve``: the drop-in surface of the GPU path.
+
+``discretize(pdesys, MOLFiniteDifference(...))`` returns an ``ODEProblem`` whose ``f`` is
+the HIP right-hand side (the reference returns an ODEProblem whose f is ModelingToolkit's
+generated function, src/discretization/staggered_discretize.jl:19-23 shows the shape).
+``solve(prob, RK4() | SSPRK33(), dt=...)`` runs the device-resident fixed-step stage loop.
+Device memory and streams come from PyTorch; the arithmetic is the HIP library's.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import native
+from .discretization import Lowered, MOLFiniteDifference, lower
+from .pdesys import PDESystem
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise native.NativeLibraryError("no GPU: the MI355X path needs a HIP device")
+    return torch
+
+
+class RK4:
+    """Classic 4-stage Runge-Kutta (OrdinaryDiffEq RK4, fixed step)."""
+    scheme = native.MOL_RK4
+    stages = 4
+
+
+class SSPRK33:
+    """Shu-Osher 3-stage SSP Runge-Kutta (OrdinaryDiffEq SSPRK33, fixed step)."""
+    scheme = native.MOL_SSPRK33
+    stages = 3
+
+
+class GPUHandle:
+    """Owns a ``mol_handle*`` for a lowered system."""
+
+    def __init__(self, low: Lowered):
+        self.lib = native.load_library()
+        self.desc, self._keep = native.pack_descriptor(low)
+        h = C.c_void_p()
+        rc = self.lib.mol_create(C.byref(self.desc), C.byref(h))
+        native.check(self.lib, rc, None)
+        self.h = h
+        self.n_unknowns = int(self.lib.mol_num_unknowns(h))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) is not None and self.h.value:
+                self.lib.mol_destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+
+class ODEProblem:
+    """Semi-discrete problem du/dt = f(u, p, t) with f evaluated on the GPU."""
+
+    def __init__(self, low: Lowered, tspan=None, p=None):
+        self.lowered = low
+        self.u0 = low.u0.copy()
+        self.tspan = tuple(tspan) if tspan is not None else low.tspan
+        self.p = np.asarray(low.param_values if p is None else p, dtype=np.float64)
+        self._handle: Optional[GPUHandle] = None
+        self._aux: Dict[str, object] = {}
+        self._aux_host: Dict[str, np.ndarray] = {}
+
+    # ---------------------------------------------------------------- device resources
+    @property
+    def dtype_np(self):
+        return np.float64 if self.lowered.dtype == "f64" else np.float32
+
+    def torch_dtype(self):
+        torch = _torch()
+        return torch.float64 if self.lowered.dtype == "f64" else torch.float32
+
+    @property
+    def handle(self) -> GPUHandle:
+        if self._handle is None:
+            self._handle = GPUHandle(self.lowered)
+            if self.lowered.aux:
+                missing = [a for a in self.lowered.aux if a not in self._aux_host]
+                if missing:
+                    raise ValueError("grid fields %s have no values: call set_grid_field" % missing)
+                self._bind_aux()
+        return self._handle
+
+    def set_grid_field(self, name: str, values) -> None:
+        """Bind the full-grid values (shape = grid shape, first axis fastest in memory) of a
+        grid-valued parameter."""
+        if name not in self.lowered.aux:
+            raise KeyError(name)
+        v = np.asarray(values, dtype=np.float64)
+        if v.shape != tuple(self.lowered.n) and v.size != int(np.prod(self.lowered.n)):
+            raise ValueError("grid field must have the grid shape %s" % (tuple(self.lowered.n),))
+        self._aux_host[name] = v.reshape(tuple(self.lowered.n))
+        if self._handle is not None:
+            self._bind_aux()
+
+    def _bind_aux(self):
+        torch = _torch()
+        ptrs = (C.c_void_p * max(1, len(self.lowered.aux)))()
+        for i, name in enumerate(self.lowered.aux):
+            host = self._aux_host[name].reshape(-1, order="F")
+            t = torch.tensor(host, dtype=self.torch_dtype(), device="cuda")
+            self._aux[name] = t
+            ptrs[i] = t.data_ptr()
+        native.check(self._handle.lib, self._handle.lib.mol_set_aux(self._handle.h, ptrs), self._handle.h)
+
+    def aux_tensor(self, name: str):
+        self.handle
+        return self._aux[name]
+
+    def device_params(self):
+        torch = _torch()
+        if len(self.p) == 0:
+            return None
+        return torch.tensor(self.p, dtype=self.torch_dtype(), device="cuda")
+
+    # ---------------------------------------------------------------- f(du, u, p, t)
+    def f(self, du, u, p, t: float) -> None:
+        """In-place right-hand side.  ``u``/``du``: CUDA tensors (asynchronous on torch's current
+        stream) or numpy arrays (host path: copy in, run, copy out)."""
+        h = self.handle
+        if isinstance(u, np.ndarray):
+            uu = np.ascontiguousarray(u, dtype=self.dtype_np)
+            out = np.empty_like(uu)
+            pp = np.ascontiguousarray(np.asarray(p if p is not None else self.p), dtype=self.dtype_np)
+            rc = h.lib.mol_rhs_h(h.h, out.ctypes.data, uu.ctypes.data,
+                                 pp.ctypes.data if pp.size else None, float(t))
+            native.check(h.lib, rc, h.h)
+            du[...] = out
+            return
+        torch = _torch()
+        if p is None:
+            p = self.device_params()
+        elif isinstance(p, np.ndarray):
+            p = torch.tensor(p, dtype=self.torch_dtype(), device="cuda")
+        self._check_tensor(u)
+        self._check_tensor(du)
+        stream = torch.cuda.current_stream().cuda_stream
+        rc = h.lib.mol_rhs(h.h, du.data_ptr(), u.data_ptr(), p.data_ptr() if p is not None else None,
+                           float(t), C.c_void_p(stream))
+        native.check(h.lib, rc, h.h)
+
+    def _check_tensor(self, x):
+        if x.dtype != self.torch_dtype() or not x.is_cuda or not x.is_contiguous():
+            raise ValueError("state tensors must be contiguous CUDA tensors of %s" % self.torch_dtype())
+        if x.numel() != self.lowered.n_unknowns:
+            raise ValueError("state has %d entries, the system has %d unknowns" % (x.numel(), self.lowered.n_unknowns))
+
+    def __call__(self, u, p, t):
+        """Out-of-place f(u, p, t)."""
+        if isinstance(u, np.ndarray):
+            du = np.empty_like(u)
+        else:
+            du = u.new_empty(u.shape)
+        self.f(du, u, p, t)
+        return du
+
+    # ---------------------------------------------------------------- integration
+    def integrate(self, u, t0: float, dt: float, nsteps: int, alg=RK4, kahan: bool = False,
+                  graph: bool = False, p=None) -> None:
+        """Advance the device state ``u`` in place by ``nsteps`` fixed steps."""
+        torch = _torch()
+        h = self.handle
+        self._check_tensor(u)
+        if p is None:
+            p = self.device_params()
+        flags = (native.MOL_FLAG_KAHAN if kahan else 0) | (native.MOL_FLAG_GRAPH if graph else 0)
+        stream = torch.cuda.current_stream().cuda_stream
+        rc = h.lib.mol_integrate(h.h, u.data_ptr(), p.data_ptr() if p is not None else None, float(t0),
+                                 float(dt), int(nsteps), alg.scheme, flags, C.c_void_p(stream))
+        native.check(h.lib, rc, h.h)
+
+    # ---------------------------------------------------------------- layout helpers
+    def unflatten(self, u) -> List[np.ndarray]:
+        """Split a state vector into per-variable grids over their unknown boxes (first axis first)."""
+        low = self.lowered
+        u = np.asarray(u)
+        out = []
+        for f, sl in enumerate(low.field_slices()):
+            out.append(u[sl].reshape(low.box_shape(f), order="F"))
+        return out
+
+
+def symbolic_discretize(pdesys: PDESystem, disc: MOLFiniteDifference) -> Lowered:
+    """Lower the PDE system to the stencil IR (no device work)."""
+    return lower(pdesys, disc)
+
+
+def discretize(pdesys: PDESystem, disc: MOLFiniteDifference) -> ODEProblem:
+    """``discretize(pdesys, MOLFiniteDifference(...))`` -> ``ODEProblem``."""
+    low = lower(pdesys, disc)
+    prob = ODEProblem(low)
+    for h, name in zip(low.aux_heads, low.aux):
+        v = pdesys.ps.get(h)
+        if v is not None:
+            prob.set_grid_field(name, v)
+    return prob
+
+
+@dataclass
+class ODESolution:
+    t: List[float]
+    u: List[np.ndarray]
+    prob: ODEProblem
+
+    def __getitem__(self, var):
+        """Per-variable trajectory over the unknown box: array (nt, *box_shape)."""
+        low = self.prob.lowered
+        name = var if isinstance(var, str) else str(getattr(var, "func", var))
+        f = low.fields.index(name)
+        return np.stack([self.prob.unflatten(u)[f] for u in self.u])
+
+
+def solve(prob: ODEProblem, alg=RK4, dt: float = None, adaptive: bool = False, saveat=None,
+          kahan: bool = False, graph: bool = False, tspan=None) -> ODESolution:
+    """Fixed-step solve on the GPU (adaptive stepping is not supported)."""
+    if adaptive:
+        raise NotImplementedError("adaptive time stepping is not supported; pass dt and adaptive=False")
+    if dt is None:
+        raise ValueError("dt is required")
+    torch = _torch()
+    t0, t1 = tspan if tspan is not None else prob.tspan
+    nsteps_total = int(round((t1 - t0) / dt))
+    u = torch.tensor(prob.u0, dtype=prob.torch_dtype(), device="cuda")
+    ts, us = [t0], [prob.u0.astype(prob.dtype_np).copy()]
+    if saveat is None:
+        marks = [nsteps_total]
+    else:
+        every = int(round(saveat / dt))
+        marks = list(range(every, nsteps_total + 1, every))
+        if not marks or marks[-1] != nsteps_total:
+            marks.append(nsteps_total)
+    done = 0
+    for m in marks:
+        prob.integrate(u, t0 + done * dt, dt, m - done, alg, kahan=kahan, graph=graph)
+        done = m
+        ts.append(t0 + done * dt)
+        us.append(u.detach().cpu().numpy().astype(np.float64))
+    return ODESolution(ts, us, prob)

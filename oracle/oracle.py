@@ -1,0 +1,201 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the benchmark's
+CPU-baseline leg.  The product path never imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        sig = {
+            "orc_calculate_weights": [C.c_int, C.c_double, _dp, C.c_int, _dp],
+            "orc_centered_weights": [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _ip],
+            "orc_half_weights": [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _ip],
+            "orc_weno_f": [_dp, C.c_double, C.c_double, _dp],
+            "orc_diffusion1d_rhs": [C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double, _dp],
+            "orc_diffusion1d_integrate": [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double, C.c_double, C.c_longlong, C.c_int],
+            "orc_bruss2d_rhs": [C.c_int, C.c_int, C.c_double, _dp, C.c_double, _dp, C.c_int],
+            "orc_bruss2d_integrate": [C.c_int, C.c_int, C.c_double, _dp, C.c_double, C.c_double, C.c_longlong, C.c_int],
+            "orc_burgers2d_rhs": [C.c_int, C.c_double, _dp, C.c_double, _dp, C.c_int],
+            "orc_burgers2d_integrate": [C.c_int, C.c_double, _dp, C.c_double, C.c_double, C.c_longlong, C.c_int],
+            "orc_weno1d_dirichlet_rhs": [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, _dp, C.c_double, _dp],
+            "orc_diffreact2d_rhs": [C.c_int, C.c_double, C.c_double, _ip, _dp, C.c_double, _dp],
+            "orc_nonlindiff3d_rhs": [C.c_int, C.c_int, _dp, _dp, _dp, C.c_double, _dp, C.c_int],
+            "orc_nonlindiff3d_jvp": [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_longlong, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+            "orc_upwind1d_rhs": [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp, C.c_double, _dp],
+            "orc_mixed2d_rhs": [C.c_int, C.c_int, C.c_double, _dp, C.c_double, _dp],
+            "orc_spherical1d_rhs": [C.c_int, C.c_int, C.c_double, _dp, C.c_double, _dp],
+            "orc_nonlinlap1d_rhs": [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp, C.c_double, _dp],
+            "orc_num_threads": [],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ok(rc):
+    if rc != 0:
+        raise RuntimeError("oracle call failed")
+
+
+def calculate_weights(order, x0, x):
+    x = _f64(x)
+    out = np.empty_like(x)
+    _ok(lib().orc_calculate_weights(order, float(x0), x, len(x), out))
+    return out
+
+
+def centered_weights(d, p, dx):
+    L = d + p - 1 + (d + p) % 2
+    BL = d + p
+    st = np.zeros(L)
+    low = np.zeros(L * BL + 1)
+    high = np.zeros(L * BL + 1)
+    dims = np.zeros(3, dtype=np.int32)
+    lib().orc_centered_weights(d, p, float(dx), st, low, high, dims)
+    bpc = dims[2]
+    return st, low[:bpc * BL].reshape(bpc, BL), high[:bpc * BL].reshape(bpc, BL)
+
+
+def half_weights(d, p, dx):
+    L = p + 2 * (d // 2) + p % 2
+    BL = d + p
+    st = np.zeros(L)
+    low = np.zeros(L * BL + 1)
+    high = np.zeros(L * BL + 1)
+    dims = np.zeros(3, dtype=np.int32)
+    lib().orc_half_weights(d, p, float(dx), st, low, high, dims)
+    bpc = dims[2]
+    return st, low[:bpc * BL].reshape(bpc, BL), high[:bpc * BL].reshape(bpc, BL)
+
+
+def weno_f(u5, eps, dx):
+    out = np.zeros(1)
+    lib().orc_weno_f(_f64(u5), float(eps), float(dx), out)
+    return out[0]
+
+
+def diffusion1d_rhs(n, D, bcl, bch, u, t=0.0, p=2, xlo=0.0, xhi=1.0):
+    u = _f64(u)
+    du = np.empty_like(u)
+    _ok(lib().orc_diffusion1d_rhs(n, xlo, xhi, p, D, _f64(bcl), _f64(bch), u, float(t), du))
+    return du
+
+
+def diffusion1d_integrate(n, D, bcl, bch, u, t0, dt, nsteps, scheme=0, p=2):
+    u = _f64(u).copy()
+    _ok(lib().orc_diffusion1d_integrate(n, p, D, _f64(bcl), _f64(bch), u, t0, dt, nsteps, scheme))
+    return u
+
+
+def bruss2d_rhs(n, u, t, alpha=10.0, p=2, threads=1):
+    u = _f64(u)
+    du = np.empty_like(u)
+    _ok(lib().orc_bruss2d_rhs(n, p, alpha, u, float(t), du, threads))
+    return du
+
+
+def bruss2d_integrate(n, u, t0, dt, nsteps, alpha=10.0, p=2, threads=1):
+    u = _f64(u).copy()
+    _ok(lib().orc_bruss2d_integrate(n, p, alpha, u, t0, dt, nsteps, threads))
+    return u
+
+
+def burgers2d_rhs(n, u, eps=1e-6, t=0.0, threads=1):
+    u = _f64(u)
+    du = np.empty_like(u)
+    _ok(lib().orc_burgers2d_rhs(n, eps, u, float(t), du, threads))
+    return du
+
+
+def burgers2d_integrate(n, u, t0, dt, nsteps, eps=1e-6, threads=1):
+    u = _f64(u).copy()
+    _ok(lib().orc_burgers2d_integrate(n, eps, u, t0, dt, nsteps, threads))
+    return u
+
+
+def weno1d_dirichlet_rhs(n, L, c, eps, gl, gh, u, t=0.0):
+    u = _f64(u)
+    du = np.empty_like(u)
+    _ok(lib().orc_weno1d_dirichlet_rhs(n, L, c, eps, gl, gh, u, float(t), du))
+    return du
+
+
+def diffreact2d_rhs(n, D, k, bctype, u, t=0.0):
+    u = _f64(u)
+    du = np.empty_like(u)
+    _ok(lib().orc_diffreact2d_rhs(n, D, k, np.ascontiguousarray(bctype, dtype=np.int32), u, float(t), du))
+    return du
+
+
+def nonlindiff3d_rhs(n, d0, src, u, t=0.0, p=2, threads=1):
+    u = _f64(u)
+    du = np.empty_like(u)
+    _ok(lib().orc_nonlindiff3d_rhs(n, p, _f64(d0), _f64(src), u, float(t), du, threads))
+    return du
+
+
+def nonlindiff3d_jvp(n, d0, vdir, src, u0, target, dt, nsteps, p=2):
+    L, dL = C.c_double(), C.c_double()
+    _ok(lib().orc_nonlindiff3d_jvp(n, p, _f64(d0), _f64(vdir), _f64(src), _f64(u0), _f64(target), dt, nsteps,
+                                   C.byref(L), C.byref(dL)))
+    return L.value, dL.value
+
+
+def upwind1d_rhs(n, uo, c0, c1, gl, gh, u, t=0.0):
+    u = _f64(u)
+    du = np.empty_like(u)
+    _ok(lib().orc_upwind1d_rhs(n, uo, c0, c1, gl, gh, u, float(t), du))
+    return du
+
+
+def mixed2d_rhs(n, c, u, p=2, t=0.0):
+    u = _f64(u)
+    du = np.empty_like(u)
+    _ok(lib().orc_mixed2d_rhs(n, p, c, u, float(t), du))
+    return du
+
+
+def spherical1d_rhs(n, g1, u, p=2, t=0.0):
+    u = _f64(u)
+    du = np.empty_like(u)
+    _ok(lib().orc_spherical1d_rhs(n, p, g1, u, float(t), du))
+    return du
+
+
+def nonlinlap1d_rhs(n, u, periodic, gl=0.0, gh=0.0, p=2, t=0.0):
+    u = _f64(u)
+    du = np.empty_like(u)
+    _ok(lib().orc_nonlinlap1d_rhs(n, p, int(periodic), gl, gh, u, float(t), du))
+    return du
+
+
+def num_threads():
+    return lib().orc_num_threads()

@@ -1,0 +1,267 @@
+"""GPU parity: the HIP right-hand side and stage loops against the CPU oracle (FP64: 1e-12)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+import molgpu
+from molgpu import systems
+from conftest import assert_parity
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def gpu_rhs(prob, u, t=0.0):
+    torch = _torch()
+    print("  gpu rhs: %s unknowns=%d t=%g" % (prob.lowered.fields, prob.lowered.n_unknowns, t), flush=True)
+    dt = prob.torch_dtype()
+    ug = torch.tensor(np.asarray(u), dtype=dt, device="cuda")
+    du = torch.empty_like(ug)
+    prob.f(du, ug, None, t)
+    torch.cuda.synchronize()
+    return du.cpu().numpy().astype(np.float64)
+
+
+def lap_scale(u, coef, dxs):
+    return np.max(np.abs(u)) * abs(coef) * sum(4.0 / dx ** 2 for dx in dxs) + 1.0
+
+
+# ---------------------------------------------------------------- heat (config 1)
+@pytest.mark.parametrize("bc", ["dirichlet", "neumann", "robin"])
+@pytest.mark.parametrize("p", [2, 4])
+def test_heat1d_rhs(bc, p):
+    bcs = {"dirichlet": ((1, 0, 0), (1, 0, 0)), "neumann": ((0, 1, 0), (0, 1, 0)),
+           "robin": ((2.0, 0.5, 0.3), (1.5, -0.25, -0.2))}[bc]
+    n = 101
+    sys_, disc = systems.heat1d(n=n, D=0.8, bcl=bcs[0], bch=bcs[1], approx_order=p)
+    prob = molgpu.discretize(sys_, disc)
+    rng = np.random.default_rng(1)
+    u = rng.uniform(-1, 1, prob.lowered.n_unknowns)
+    got = gpu_rhs(prob, u)
+    if p == 2:
+        ref = orc.diffusion1d_rhs(n, 0.8, bcs[0], bcs[1], u)
+    else:
+        ref = orc.diffusion1d_rhs(n, 0.8, bcs[0], bcs[1], u, p=4)
+    assert_parity(got, ref, lap_scale(u, 0.8, [0.01]) * 4, what="heat %s p=%d" % (bc, p))
+
+
+def test_heat1d_rk4_closed_form():
+    """Config 1: RK4, dt = 1e-5, 10^4 steps to t = 0.1: R(z)^N sin(pi x) exactly."""
+    torch = _torch()
+    n, D, dt, N = 101, 1.0, 1e-5, 10_000
+    sys_, disc = systems.heat1d(n=n, D=D)
+    prob = molgpu.discretize(sys_, disc)
+    u = torch.tensor(prob.u0, dtype=torch.float64, device="cuda")
+    print("  heat rk4 10^4 steps", flush=True)
+    prob.integrate(u, 0.0, dt, N, molgpu.RK4, graph=True)
+    torch.cuda.synchronize()
+    x = np.arange(1, n - 1) / (n - 1)
+    dx = 1.0 / (n - 1)
+    lam = -(4 * D / dx ** 2) * np.sin(np.pi * dx / 2) ** 2
+    z = dt * lam
+    R = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24
+    got = u.cpu().numpy()
+    assert np.max(np.abs(got - R ** N * prob.u0)) < 1e-12
+    assert np.max(np.abs(got - np.exp(-np.pi ** 2 * D * 0.1) * np.sin(np.pi * x))) < 5e-5
+
+
+# ---------------------------------------------------------------- Brusselator (config 2)
+def test_brusselator_matches_generated_code_on_gpu():
+    fx = json.load(open(os.path.join(GOLD, "bruss_dx025.json")))
+    sys_, disc = systems.brusselator2d(n=5)
+    prob = molgpu.discretize(sys_, disc)
+    for s in fx["samples"]:
+        got = gpu_rhs(prob, np.array(s["u"]), 0.0)
+        np.testing.assert_allclose(got, s["du"], rtol=1e-13, atol=1e-11)
+
+
+@pytest.mark.parametrize("n,t", [(33, 0.0), (65, 1.2), (513, 1.2)])
+def test_brusselator_rhs(n, t):
+    sys_, disc = systems.brusselator2d(n=n)
+    prob = molgpu.discretize(sys_, disc)
+    rng = np.random.default_rng(n)
+    u = rng.uniform(0.5, 1.5, prob.lowered.n_unknowns)
+    got = gpu_rhs(prob, u, t)
+    ref = orc.bruss2d_rhs(n, u, t, threads=8)
+    dx = 1.0 / (n - 1)
+    assert_parity(got, ref, lap_scale(u, 10.0, [dx, dx]), what="bruss n=%d" % n)
+
+
+def test_brusselator_initial_condition_and_rk4():
+    torch = _torch()
+    n = 65
+    sys_, disc = systems.brusselator2d(n=n)
+    prob = molgpu.discretize(sys_, disc)
+    g = np.arange(n) / (n - 1)
+    X, Y = np.meshgrid(g[1:], g[1:], indexing="ij")
+    u0 = np.concatenate([(22 * (Y * (1 - Y)) ** 1.5).reshape(-1, order="F"),
+                         (27 * (X * (1 - X)) ** 1.5).reshape(-1, order="F")])
+    np.testing.assert_allclose(prob.u0, u0, rtol=1e-14, atol=1e-15)
+    dt, N = 2e-6, 50
+    u = torch.tensor(prob.u0, dtype=torch.float64, device="cuda")
+    print("  bruss rk4 %d steps" % N, flush=True)
+    prob.integrate(u, 1.0, dt, N, molgpu.RK4)
+    torch.cuda.synchronize()
+    ref = orc.bruss2d_integrate(n, prob.u0, 1.0, dt, N, threads=8)
+    assert_parity(u.cpu().numpy(), ref, rtol=1e-12, what="bruss rk4")
+
+
+# ---------------------------------------------------------------- Burgers WENO5 (config 3)
+@pytest.mark.parametrize("n", [33, 129])
+def test_burgers_weno_rhs(n):
+    sys_, disc = systems.burgers2d(n=n)
+    prob = molgpu.discretize(sys_, disc)
+    rng = np.random.default_rng(n)
+    for u in (prob.u0, rng.uniform(0.2, 1.2, prob.lowered.n_unknowns)):
+        got = gpu_rhs(prob, u)
+        ref = orc.burgers2d_rhs(n, u, threads=8)
+        assert_parity(got, ref, np.max(np.abs(u)) ** 2 * 4 * (n - 1) + 1, what="burgers n=%d" % n)
+
+
+def test_burgers_ssprk3_matches_oracle():
+    torch = _torch()
+    n = 65
+    sys_, disc = systems.burgers2d(n=n)
+    prob = molgpu.discretize(sys_, disc)
+    umax = np.max(np.abs(prob.u0))
+    dx = 1.0 / (n - 1)
+    dt = 0.4 / (umax / dx + umax / dx)
+    N = 20
+    u = torch.tensor(prob.u0, dtype=torch.float64, device="cuda")
+    print("  burgers ssprk3", flush=True)
+    prob.integrate(u, 0.0, dt, N, molgpu.SSPRK33)
+    torch.cuda.synchronize()
+    ref = orc.burgers2d_integrate(n, prob.u0, 0.0, dt, N, threads=8)
+    assert_parity(u.cpu().numpy(), ref, rtol=1e-12, what="burgers ssprk3")
+
+
+def test_burgers_fp32_kahan_error_vs_fp64_oracle():
+    torch = _torch()
+    n = 65
+    sys_, disc = systems.burgers2d(n=n, dtype="f32")
+    prob = molgpu.discretize(sys_, disc)
+    umax = np.max(np.abs(prob.u0))
+    dx = 1.0 / (n - 1)
+    dt = 0.4 / (2 * umax / dx)
+    N = 40
+    u = torch.tensor(prob.u0, dtype=torch.float32, device="cuda")
+    prob.integrate(u, 0.0, dt, N, molgpu.SSPRK33, kahan=True)
+    torch.cuda.synchronize()
+    ref = orc.burgers2d_integrate(n, prob.u0, 0.0, dt, N, threads=8)
+    err = np.max(np.abs(u.cpu().numpy() - ref))
+    print("  fp32+kahan max error vs fp64 oracle: %.3e" % err)
+    assert err < 5e-5
+
+
+def test_weno_dirichlet_extrapolated_nodes():
+    n, L, c = 41, 2.0, 1.0
+    sys_, disc = systems.weno1d_dirichlet(n, L, c, 0.1, 0.05)
+    prob = molgpu.discretize(sys_, disc)
+    rng = np.random.default_rng(4)
+    u = rng.uniform(0, 1, prob.lowered.n_unknowns)
+    got = gpu_rhs(prob, u)
+    ref = orc.weno1d_dirichlet_rhs(n, L, c, 1e-6, 0.1, 0.05, u)
+    assert_parity(got, ref, 4 * (n - 1) / L + 1, what="weno dirichlet")
+
+
+# ---------------------------------------------------------------- diffusion-reaction (config 4 instance)
+@pytest.mark.parametrize("bct", [(0, 0, 0, 0), (1, 1, 1, 1), (0, 1, 1, 0), (1, 0, 0, 1)])
+def test_diffreact2d_rhs(bct):
+    n = 48
+    sys_, disc = systems.diffreact2d(n, 0.05, 2.5, bct)
+    prob = molgpu.discretize(sys_, disc)
+    rng = np.random.default_rng(sum(bct))
+    u = rng.uniform(0, 1, prob.lowered.n_unknowns)
+    got = gpu_rhs(prob, u)
+    ref = orc.diffreact2d_rhs(n, 0.05, 2.5, bct, u)
+    dx = 1.0 / (n - 1)
+    assert_parity(got, ref, lap_scale(u, 0.05, [dx, dx]), what="diffreact %s" % (bct,))
+
+
+# ---------------------------------------------------------------- nonlinear diffusion (config 5)
+@pytest.mark.parametrize("n", [12, 34])
+def test_nonlindiff3d_rhs(n):
+    sys_, disc = systems.nonlindiff3d(n=n)
+    prob = molgpu.discretize(sys_, disc)
+    d0 = 0.5 + 0.5 * np.abs(systems.gaussian_field3d(n, 1))
+    src = systems.gaussian_field3d(n, 2)
+    prob.set_grid_field("d0", d0)
+    prob.set_grid_field("s", src)
+    rng = np.random.default_rng(n)
+    u = rng.uniform(-1, 1, prob.lowered.n_unknowns)
+    got = gpu_rhs(prob, u)
+    ref = orc.nonlindiff3d_rhs(n, d0.reshape(-1, order="F"), src.reshape(-1, order="F"), u, threads=8)
+    dx = 1.0 / (n - 1)
+    assert_parity(got, ref, lap_scale(u, 2.0 * np.max(d0), [dx] * 3), what="nonlindiff3d n=%d" % n)
+
+
+# ---------------------------------------------------------------- other operator families
+@pytest.mark.parametrize("order", [1, 2])
+def test_upwind_sign_selection(order):
+    n = 61
+    sys_, disc = systems.upwind1d(n, 0.1, 1.0, 0.3, -0.2, order)
+    prob = molgpu.discretize(sys_, disc)
+    rng = np.random.default_rng(order)
+    u = rng.uniform(-1, 1, prob.lowered.n_unknowns)
+    got = gpu_rhs(prob, u)
+    ref = orc.upwind1d_rhs(n, order, 0.1, 1.0, 0.3, -0.2, u)
+    assert_parity(got, ref, 4 * 1.1 * (n - 1) * 2, what="upwind order %d" % order)
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_mixed_derivative(p):
+    n = 25
+    sys_, disc = systems.mixed2d(n, 0.7, p)
+    prob = molgpu.discretize(sys_, disc)
+    rng = np.random.default_rng(p)
+    u = rng.uniform(-1, 1, prob.lowered.n_unknowns)
+    got = gpu_rhs(prob, u)
+    ref = orc.mixed2d_rhs(n, 0.7, u, p=p)
+    dx = 1.0 / (n - 1)
+    assert_parity(got, ref, lap_scale(u, 4.0, [dx, dx]), what="mixed p=%d" % p)
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_spherical_laplacian(p):
+    n = 41
+    sys_, disc = systems.spherical1d(n, 0.25, p)
+    prob = molgpu.discretize(sys_, disc)
+    rng = np.random.default_rng(p + 10)
+    u = rng.uniform(0, 1, prob.lowered.n_unknowns)
+    got = gpu_rhs(prob, u)
+    ref = orc.spherical1d_rhs(n, 0.25, u, p=p)
+    dx = 1.0 / (n - 1)
+    assert_parity(got, ref, lap_scale(u, 8.0, [dx]) * 10, what="spherical p=%d" % p)
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+@pytest.mark.parametrize("p", [2, 4])
+def test_nonlinear_laplacian_1d(periodic, p):
+    n = 41
+    sys_, disc = systems.nonlinlap1d(n, periodic, 0.2, -0.4, p)
+    prob = molgpu.discretize(sys_, disc)
+    rng = np.random.default_rng(p)
+    u = rng.uniform(-1, 1, prob.lowered.n_unknowns)
+    got = gpu_rhs(prob, u)
+    ref = orc.nonlinlap1d_rhs(n, u, periodic, 0.2, -0.4, p=p)
+    dx = 1.0 / (n - 1)
+    assert_parity(got, ref, lap_scale(u, 8.0, [dx]) * 4, what="nonlinlap periodic=%s p=%d" % (periodic, p))
+
+
+def test_host_pointer_entry_point():
+    n = 17
+    sys_, disc = systems.brusselator2d(n=n)
+    prob = molgpu.discretize(sys_, disc)
+    rng = np.random.default_rng(21)
+    u = rng.uniform(0.5, 1.5, prob.lowered.n_unknowns)
+    du = np.empty_like(u)
+    prob.f(du, u, prob.p, 1.3)
+    assert_parity(du, orc.bruss2d_rhs(n, u, 1.3), lap_scale(u, 10.0, [1 / 16] * 2), what="rhs_h")
