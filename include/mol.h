@@ -187,6 +187,13 @@ int mol_rhs(mol_handle* h, void* du, const void* u, const void* p, double t, voi
  * out, synchronizes. */
 int mol_rhs_h(mol_handle* h, void* du_h, const void* u_h, const void* p_h, double t);
 
+/* Tile shape of the stencil kernels (nodes per work-group along x, y, z; 0,0,0 = default).
+ * No counterpart in the reference; used by the autotuner.  At most 4096 nodes per tile. */
+int mol_set_tile(mol_handle* h, int32_t tx, int32_t ty, int32_t tz);
+
+/* Launch plan: info[0..2] tile shape, info[3] core tiles, info[4] generic tiles, info[5] WENO tiles. */
+int mol_get_plan(const mol_handle* h, int32_t* info, int32_t ninfo);
+
 /* Bind auxiliary full-grid arrays (device pointers, naux of them, each prod(n) values). */
 int mol_set_aux(mol_handle* h, void* const* aux);
 
@@ -199,6 +206,28 @@ typedef enum { MOL_FLAG_KAHAN = 1, MOL_FLAG_GRAPH = 2 } mol_flags;
  * the step update (FP32 state).  MOL_FLAG_GRAPH: stages captured once in a hipGraph and replayed. */
 int mol_integrate(mol_handle* h, void* u, const void* p, double t0, double dt, int64_t nsteps,
                   int32_t scheme, int32_t flags, void* stream);
+
+/* ---- adjoint ------------------------------------------------------------------------
+ * No counterpart in the reference (it differentiates through ForwardDiff/Zygote on the CPU).
+ * Supported right-hand sides: every operator slot centred (MOL_SLOT_CENTERED), mixed or a
+ * nonlinear Laplacian, programs built from variables, slots, auxiliary arrays, coordinates,
+ * time and parameters; MOL_ERR_UNSUPPORTED otherwise.
+ *
+ * mol_vjp: lam_dot = (df/du)^T lam, and g_aux[a*ngrid + node] += (df/daux_a)^T lam in FP64
+ * (g_aux may be NULL).  u, lam, lam_dot: unknown vectors; g_aux: naux full-grid FP64 arrays. */
+int mol_vjp(mol_handle* h, const void* u, const void* p, double t, const void* lam, void* lam_dot,
+            double* g_aux, void* stream);
+
+/* mol_forward_checkpointed: mol_integrate (RK4) that also keeps the state every ckpt_every
+ * steps (and u0) in the handle, for a later mol_adjoint of the same run. */
+int mol_forward_checkpointed(mol_handle* h, void* u, const void* p, double t0, double dt,
+                             int64_t nsteps, int32_t ckpt_every, void* stream);
+
+/* mol_adjoint: discrete adjoint of the last checkpointed RK4 run.  On entry lam = dL/du(T);
+ * on exit lam = dL/du(t0) and g_aux += dL/daux (FP64).  Each segment between checkpoints is
+ * recomputed forward once (stage states kept on the device), then swept backwards with four
+ * VJPs per step. */
+int mol_adjoint(mol_handle* h, const void* p, void* lam, double* g_aux, void* stream);
 
 /* Number of unknowns (sum over variables of the unknown box sizes). */
 int64_t mol_num_unknowns(const mol_handle* h);

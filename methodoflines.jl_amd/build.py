@@ -7,7 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "molgpu", "libmolgpu.so")
-SOURCES = ["mol_rhs.hip", "mol_capi.hip"]
+SOURCES = ["mol_rhs.hip", "mol_weno.hip", "mol_capi.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -1,5 +1,6 @@
 // C ABI of the stencil library: descriptor validation/upload, RHS and stage-loop drivers.
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -26,6 +27,11 @@ namespace {
     }                                                                             \
   } while (0)
 
+template <typename T>
+KArgs<T>& kargs(mol_handle* h);
+template <typename T>
+KArgs<T>* dkargs(mol_handle* h);
+
 int fail(mol_handle* h, int code, const std::string& msg) {
   if (h) h->err = msg;
   else set_global_error(msg);
@@ -47,7 +53,7 @@ void free_handle(mol_handle* h) {
   if (!h) return;
   void* ptrs[] = {h->d_tabs, h->d_slots, h->d_clos, h->d_pool, h->d_code, h->d_cpool,
                   h->d_prog, h->d_coord[0], h->d_coord[1], h->d_coord[2], h->w_y1,
-                  h->w_y2, h->w_acc, h->w_comp, h->d_tnow, h->d_ka64, h->d_ka32};
+                  h->w_y2, h->w_acc, h->w_comp, h->d_tnow, h->d_ka64, h->d_ka32, h->d_tiles};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h->h_u) (void)hipHostFree(h->h_u);
@@ -55,6 +61,28 @@ void free_handle(mol_handle* h) {
   if (h->h_p) (void)hipHostFree(h->h_p);
   delete h;
 }
+
+}  // namespace
+
+namespace mol {
+// Maximum stack depth of a program (checked to follow the stack discipline at creation).
+int program_depth(const mol_desc* d, int prog) {
+  const int st = d->prog_h[2 * prog], ln = d->prog_h[2 * prog + 1];
+  int depth = 0, mx = 0;
+  for (int pc = st; pc < st + ln; ++pc) {
+    const uint32_t ins = d->code_h[pc];
+    const int op = ins & 0xff;
+    const bool bc = (ins >> 20) & 1u;
+    if (op < 16) depth += 1;
+    else if (op == MOL_OP_SELECT) depth -= bc ? 1 : 2;
+    else if (op < 48 && !bc) depth -= 1;
+    mx = std::max(mx, depth);
+  }
+  return mx;
+}
+}  // namespace mol
+
+namespace {
 
 // Structural validation of a descriptor against the limits the kernels assume.
 int validate(const mol_desc* d, std::string& msg) {
@@ -103,6 +131,29 @@ int validate(const mol_desc* d, std::string& msg) {
   }
   for (int f = 0; f < d->nfields; ++f)
     if (d->rhs_prog[f] < 0 || d->rhs_prog[f] >= d->nprogs) { msg = "rhs program missing"; return MOL_ERR_INVALID; }
+  // every program must follow the stack discipline the interpreter relies on: an instruction
+  // at depth k writes register k-1 (binary: reads k-2, k-1), leaves push, depth <= MOL_STACK
+  for (int i = 0; i < d->nprogs; ++i) {
+    const int st = d->prog_h[2 * i], ln = d->prog_h[2 * i + 1];
+    int depth = 0;
+    for (int pc = st; pc < st + ln; ++pc) {
+      const uint32_t ins = d->code_h[pc];
+      const int op = ins & 0xff, dst = (ins >> 8) & 15, ra = (ins >> 12) & 15, rb = (ins >> 16) & 15;
+      const bool bc = (ins >> 20) & 1u;
+      bool ok;
+      if (op < 16) { ok = op != MOL_OP_MOV && dst == depth; depth += 1; }
+      else if (op == MOL_OP_FMA) ok = false;
+      else if (op == MOL_OP_SELECT) {
+        if (bc) { ok = dst == depth - 2 && ra == depth - 1; depth -= 1; }
+        else { ok = dst == depth - 3 && ra == depth - 2 && rb == depth - 1; depth -= 2; }
+      } else if (op < 48) {
+        if (bc) ok = dst == depth - 1 && ra == depth - 1;
+        else { ok = dst == depth - 2 && ra == depth - 2 && rb == depth - 1; depth -= 1; }
+      } else ok = dst == depth - 1 && ra == depth - 1;
+      if (!ok || depth < 0 || depth > MOL_STACK) { msg = "program does not follow the stack discipline or is deeper than MOL_STACK"; return MOL_ERR_UNSUPPORTED; }
+    }
+    if (depth != 1) { msg = "program does not leave exactly one value"; return MOL_ERR_INVALID; }
+  }
   auto tab_ok = [&](int t) { return t >= 0 && t < d->ntabs; };
   for (int k = 0; k < d->nslots; ++k) {
     const mol_slot& s = d->slots_h[k];
@@ -179,9 +230,12 @@ void fill_kargs(mol_handle* h, const mol_desc* d, KArgs<T>& ka) {
     ka.clos_id[cl.field][cl.axis][cl.side] = c;
   }
   // tile shape: contiguous along x, sized so that tile + halo of every staged array fits LDS
+  // default: one node per thread (256-node tiles), contiguous along x
   int tile[3] = {256, 1, 1};
-  if (d->ndim == 2) { tile[0] = 64; tile[1] = 16; }
-  if (d->ndim == 3) { tile[0] = 32; tile[1] = 8; tile[2] = 4; }
+  if (d->ndim == 2) { tile[0] = 64; tile[1] = 4; }
+  if (d->ndim == 3) { tile[0] = 32; tile[1] = 4; tile[2] = 2; }
+  if (h->tile_req[0] > 0)
+    for (int a = 0; a < 3; ++a) tile[a] = a < d->ndim ? std::max(1, h->tile_req[a]) : 1;
   const int nload = d->nfields + d->naux;
   auto vol_of = [&](const int* tl) {
     int v = 1;
@@ -189,6 +243,12 @@ void fill_kargs(mol_handle* h, const mol_desc* d, KArgs<T>& ka) {
     return v;
   };
   for (int a = 0; a < 3; ++a) tile[a] = std::max(1, std::min(tile[a], ka.uhi[a] - ka.ulo[a] + 1));
+  while (tile[0] * tile[1] * tile[2] > 4096) {
+    int big = 0;
+    for (int a = 1; a < 3; ++a)
+      if (tile[a] > tile[big]) big = a;
+    tile[big] = (tile[big] + 1) / 2;
+  }
   while ((size_t)nload * vol_of(tile) * sizeof(T) > 60000) {
     int big = 0;
     for (int a = 1; a < 3; ++a)
@@ -210,6 +270,202 @@ void fill_kargs(mol_handle* h, const mol_desc* d, KArgs<T>& ka) {
   ka.code = (const uint32_t*)h->d_code;
   ka.cpool = (const double*)h->d_cpool;
   ka.prog = (const int32_t*)h->d_prog;
+}
+
+template <typename T>
+KCore<T>& kcore(mol_handle* h);
+template <>
+KCore<double>& kcore<double>(mol_handle* h) { return h->kc64; }
+template <>
+KCore<float>& kcore<float>(mol_handle* h) { return h->kc32; }
+
+template <typename T>
+KCore<T>& kcorew(mol_handle* h);
+template <>
+KCore<double>& kcorew<double>(mol_handle* h) { return h->kcw64; }
+template <>
+KCore<float>& kcorew<float>(mol_handle* h) { return h->kcw32; }
+
+template <typename T>
+Launch<T> launch_of(mol_handle* h) {
+  Launch<T> L;
+  if (h->use_weno) {
+    L.kcw = &kcorew<T>(h);
+    L.plan = &h->wplan;
+    L.nweno = h->nweno;
+  }
+  L.ka = &kargs<T>(h);
+  L.dka = dkargs<T>(h);
+  L.kc = &kcore<T>(h);
+  L.tiles = h->d_tiles;
+  L.ncore = h->ncore;
+  L.nbound = h->nbound;
+  return L;
+}
+
+// Magic multiplier of fdiv_magic: ceil(2^32 / d), 0 for d == 1 (d < 2^16).
+uint32_t magic32(uint32_t dv) { return dv <= 1 ? 0u : (uint32_t)(((1ull << 32) + dv - 1) / dv); }
+
+// Node range along an axis where a table uses only its interior row (node-centred index i).
+void interior_range(const mol_optab& tb, int n, int& lo, int& hi) {
+  lo = std::max(lo, tb.nlow + 1);
+  hi = std::min(hi, n + tb.ref_shift - tb.nhigh);
+}
+
+// Build the core-tile kernel arguments and the list of tiles left to the generic kernel.
+template <typename T>
+void build_core(mol_handle* h, const mol_desc* d, KCore<T>& kc, std::vector<int32_t>& gen_tiles,
+                int& ncore) {
+  const KArgs<T>& ka = kargs<T>(h);
+  std::memset(&kc, 0, sizeof(kc));
+  ncore = 0;
+  const int ntot = ka.ntile[0] * ka.ntile[1] * ka.ntile[2];
+  auto all_generic = [&](const char* why) {
+    gen_tiles.resize(ntot);
+    for (int i = 0; i < ntot; ++i) gen_tiles[i] = i;
+    h->core_reason = why;
+  };
+  for (int f = 1; f < d->nfields; ++f)
+    for (int a = 0; a < d->ndim; ++a)
+      if (d->lo[f][a] != d->lo[0][a] || d->hi[f][a] != d->hi[0][a]) return all_generic("variables with different unknown boxes");
+  if (d->ncode > CORE_MAX_CODE || d->ncpool > CORE_MAX_CONST || d->nslots > CORE_MAX_SLOTS ||
+      d->nprogs > CORE_MAX_PROGS)
+    return all_generic("stencil IR larger than the kernel-argument block");
+  kc.ndim = d->ndim;
+  kc.nfields = d->nfields;
+  kc.naux = d->naux;
+  kc.nslots = d->nslots;
+  int64_t box = 1;
+  for (int a = 0; a < 3; ++a) {
+    kc.n[a] = ka.n[a];
+    kc.periodic[a] = ka.periodic[a];
+    kc.lo[a] = ka.lo[0][a];
+    kc.hi[a] = ka.hi[0][a];
+    kc.tile[a] = ka.tile[a];
+    kc.halo[a] = ka.halo[a];
+    kc.ext[a] = ka.ext[a];
+    box *= kc.hi[a] - kc.lo[a] + 1;
+  }
+  kc.vol = ka.vol;
+  kc.box = box;
+  if (kc.vol >= (1 << 16)) return all_generic("tile too large for the index arithmetic");
+  kc.magic0 = magic32((uint32_t)kc.ext[0]);
+  kc.magic1 = magic32((uint32_t)(kc.ext[0] * kc.ext[1]));
+  for (int f = 0; f < d->nfields; ++f) kc.rhs_prog[f] = d->rhs_prog[f];
+  for (int i = 0; i < 2 * d->nprogs; ++i) kc.prog[i] = d->prog_h[i];
+  for (int i = 0; i < d->ncode; ++i) kc.code[i] = d->code_h[i];
+  for (int i = 0; i < d->ncpool; ++i) kc.cpool[i] = d->cpool_h[i];
+  for (int a = 0; a < 3; ++a) kc.coord[a] = ka.coord[a];
+  for (int a = 0; a < MOL_MAX_AUX; ++a) kc.aux[a] = ka.aux[a];
+  // core node range per axis: tile + halo inside the box, interior rows for every slot table
+  int clo[3], chi[3];
+  for (int a = 0; a < 3; ++a) {
+    clo[a] = kc.lo[a];
+    chi[a] = kc.hi[a];
+    if (a < d->ndim && !kc.periodic[a]) {
+      clo[a] = kc.lo[a] + kc.halo[a];
+      chi[a] = kc.hi[a] - kc.halo[a];
+    }
+  }
+  int np = 0;
+  for (int k = 0; k < d->nslots; ++k) {
+    const mol_slot& sl = d->slots_h[k];
+    CoreSlot& cs = kc.slots[k];
+    cs.kind = sl.kind; cs.field = sl.field; cs.axis = sl.axis; cs.axis2 = sl.axis2; cs.prog = sl.prog;
+    cs.eps = sl.eps; cs.inv_dx = sl.inv_dx;
+    int ntab = 0;
+    switch (sl.kind) {
+      case MOL_SLOT_CENTERED: ntab = 1; break;
+      case MOL_SLOT_UPWIND: case MOL_SLOT_MIXED: ntab = 2; break;
+      case MOL_SLOT_NONLINLAP: ntab = 3; break;
+      case MOL_SLOT_SPHERICAL: ntab = 5; break;
+      default: ntab = 0;
+    }
+    for (int q = 0; q < ntab; ++q) {
+      const mol_optab& tb = d->tabs_h[sl.tab[q]];
+      if (np + tb.len > CORE_MAX_POOL) return all_generic("interior weights exceed the kernel-argument block");
+      cs.len[q] = tb.len;
+      cs.off[q] = tb.off;
+      cs.w[q] = np;
+      for (int j = 0; j < tb.len; ++j) kc.pool[np++] = d->pool_h[tb.w + j];
+      const int ax = (sl.kind == MOL_SLOT_MIXED && q == 1) ? sl.axis2 : sl.axis;
+      if (kc.periodic[ax]) continue;
+      const int n = kc.n[ax];
+      const bool half_op = (sl.kind == MOL_SLOT_NONLINLAP && q < 3) || (sl.kind == MOL_SLOT_SPHERICAL && q >= 2);
+      if (!half_op) {
+        interior_range(tb, n, clo[ax], chi[ax]);
+      } else {
+        const int qo = sl.kind == MOL_SLOT_NONLINLAP ? 0 : 2;
+        const mol_optab& to = d->tabs_h[sl.tab[qo]];
+        if (q == qo) {
+          // outer row at index i-1 with lref = n + ref_shift
+          clo[ax] = std::max(clo[ax], to.nlow + 2);
+          chi[ax] = std::min(chi[ax], n + to.ref_shift - to.nhigh + 1);
+        } else {
+          // inner / interpolation rows at the outer taps h = i-1+off_o .. i-1+off_o+len_o-1
+          clo[ax] = std::max(clo[ax], tb.nlow + 2 - to.off);
+          chi[ax] = std::min(chi[ax], n - tb.nhigh + 2 - to.off - to.len);
+        }
+      }
+    }
+  }
+  int ct_lo[3], ct_hi[3];
+  for (int a = 0; a < 3; ++a) {
+    const int T_ = kc.tile[a];
+    if (a >= d->ndim || kc.periodic[a]) {
+      ct_lo[a] = 0;
+      ct_hi[a] = ka.ntile[a] - 1;
+      continue;
+    }
+    const int first = (clo[a] - kc.lo[a] + T_ - 1) / T_;
+    const int last = (chi[a] - kc.lo[a] + 1) / T_ - 1;
+    ct_lo[a] = std::max(0, first);
+    ct_hi[a] = std::min(ka.ntile[a] - 1, last);
+  }
+  bool empty = false;
+  for (int a = 0; a < 3; ++a) empty = empty || ct_hi[a] < ct_lo[a];
+  gen_tiles.clear();
+  if (empty) return all_generic("no tile away from the boundaries");
+  for (int a = 0; a < 3; ++a) {
+    kc.ct_lo[a] = ct_lo[a];
+    kc.ct_n[a] = ct_hi[a] - ct_lo[a] + 1;
+  }
+  ncore = kc.ct_n[0] * kc.ct_n[1] * kc.ct_n[2];
+  for (int tz = 0; tz < ka.ntile[2]; ++tz)
+    for (int ty = 0; ty < ka.ntile[1]; ++ty)
+      for (int tx = 0; tx < ka.ntile[0]; ++tx) {
+        const bool core = tx >= ct_lo[0] && tx <= ct_hi[0] && ty >= ct_lo[1] && ty <= ct_hi[1] &&
+                          tz >= ct_lo[2] && tz <= ct_hi[2];
+        if (!core) gen_tiles.push_back(tx + ka.ntile[0] * (ty + ka.ntile[1] * tz));
+      }
+  h->core_reason.clear();
+}
+
+int setup_paths(mol_handle* h, const mol_desc* d) {
+  std::vector<int32_t> t64, t32;
+  int n64 = 0, n32 = 0;
+  build_core<double>(h, d, h->kc64, t64, n64);
+  build_core<float>(h, d, h->kc32, t32, n32);
+  h->use_weno = false;
+  if (n64 > 0 && t64.empty() && weno_plan(d, h->wplan)) {
+    // every tile is a core tile: retile 64 x 32 for the WENO5 kernel
+    h->kcw64 = h->kc64;
+    h->kcw32 = h->kc32;
+    const int ntx = (h->kc64.hi[0] - h->kc64.lo[0] + WENO_TILE_X) / WENO_TILE_X;
+    const int nty = (h->kc64.hi[1] - h->kc64.lo[1] + WENO_TILE_Y) / WENO_TILE_Y;
+    for (KCore<double>* k : {&h->kcw64}) { k->ct_lo[0] = k->ct_lo[1] = k->ct_lo[2] = 0; k->ct_n[0] = ntx; k->ct_n[1] = nty; k->ct_n[2] = 1; }
+    for (KCore<float>* k : {&h->kcw32}) { k->ct_lo[0] = k->ct_lo[1] = k->ct_lo[2] = 0; k->ct_n[0] = ntx; k->ct_n[1] = nty; k->ct_n[2] = 1; }
+    h->nweno = ntx * nty;
+    h->use_weno = std::getenv("MOL_DISABLE_WENO_FAST") == nullptr;
+  }
+  h->ncore = n64;
+  h->nbound = (int)t64.size();
+  if (h->d_tiles) { (void)hipFree(h->d_tiles); h->d_tiles = nullptr; }
+  if (!t64.empty()) {
+    HIPCHK(h, hipMalloc((void**)&h->d_tiles, t64.size() * sizeof(int32_t)));
+    HIPCHK(h, hipMemcpy(h->d_tiles, t64.data(), t64.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
+  return 0;
 }
 
 int ensure_workspace(mol_handle* h) {
@@ -247,8 +503,7 @@ int sync_kargs(mol_handle* h) {
 template <typename T>
 int enqueue_steps(mol_handle* h, T* u, const T* p, double dt, int64_t nsteps, int scheme,
                   bool kahan, hipStream_t s) {
-  KArgs<T>& ka = kargs<T>(h);
-  KArgs<T>* dka = dkargs<T>(h);
+  const Launch<T> L = launch_of<T>(h);
   T* y1 = (T*)h->w_y1;
   T* y2 = (T*)h->w_y2;
   T* acc = (T*)h->w_acc;
@@ -260,27 +515,27 @@ int enqueue_steps(mol_handle* h, T* u, const T* p, double dt, int64_t nsteps, in
     int rc = 0;
     if (scheme == MOL_RK4) {
       e.mode = 1; e.out = y1; e.A = u; e.ca = 1.0; e.cf = dt / 2; e.acc = acc; e.acc_keep = 0.0; e.acc_f = 1.0; e.tofs = 0.0;
-      rc |= launch_rhs_generic<T>(ka, dka, u, e, s);
+      rc |= launch_rhs<T>(L, u, e, s);
       e.out = y2; e.cf = dt / 2; e.acc_keep = 1.0; e.acc_f = 2.0; e.tofs = dt / 2;
-      rc |= launch_rhs_generic<T>(ka, dka, y1, e, s);
+      rc |= launch_rhs<T>(L, y1, e, s);
       e.out = y1; e.cf = dt; e.tofs = dt / 2;
-      rc |= launch_rhs_generic<T>(ka, dka, y2, e, s);
+      rc |= launch_rhs<T>(L, y2, e, s);
       Epi<T> l{};
       l.tnow = h->d_tnow; l.p = p; l.tofs = dt; l.out = u; l.A = u; l.B = acc; l.cb = dt / 6; l.cf = dt / 6;
       if (kahan) { l.mode = 2; l.subA = 0; l.comp = comp; }
       else { l.mode = 1; l.ca = 1.0; }
-      rc |= launch_rhs_generic<T>(ka, dka, y1, l, s);
+      rc |= launch_rhs<T>(L, y1, l, s);
     } else {
       e.mode = 1; e.out = y1; e.A = u; e.ca = 1.0; e.cf = dt; e.tofs = 0.0;
-      rc |= launch_rhs_generic<T>(ka, dka, u, e, s);
+      rc |= launch_rhs<T>(L, u, e, s);
       e.out = y2; e.ca = 0.75; e.B = y1; e.cb = 0.25; e.cf = dt / 4; e.tofs = dt;
-      rc |= launch_rhs_generic<T>(ka, dka, y1, e, s);
+      rc |= launch_rhs<T>(L, y1, e, s);
       Epi<T> l{};
       l.tnow = h->d_tnow; l.p = p; l.tofs = dt / 2; l.out = u; l.A = u; l.B = y2;
       l.cb = 2.0 / 3.0; l.cf = 2.0 * dt / 3.0;
       if (kahan) { l.mode = 2; l.subA = 1; l.comp = comp; }
       else { l.mode = 1; l.ca = 1.0 / 3.0; }
-      rc |= launch_rhs_generic<T>(ka, dka, y2, l, s);
+      rc |= launch_rhs<T>(L, y2, l, s);
     }
     rc |= launch_advance_time(h->d_tnow, dt, s);
     if (rc) return fail(h, MOL_ERR_HIP, std::string("kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
@@ -339,6 +594,8 @@ int mol_create(const mol_desc* d, mol_handle** out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return fail(nullptr, MOL_ERR_HIP, "no HIP device available");
   mol_handle* h = new mol_handle();
+  h->desc.assign(d);
+  d = &h->desc.d;
   (void)hipGetDevice(&h->device);
   h->dtype = d->dtype;
   h->ndim = d->ndim;
@@ -362,12 +619,46 @@ int mol_create(const mol_desc* d, mol_handle** out) {
   }
   fill_kargs<double>(h, d, h->ka64);
   fill_kargs<float>(h, d, h->ka32);
-  if (sync_kargs(h)) {
+  if (sync_kargs(h) || setup_paths(h, d)) {
     set_global_error(h->err);
     free_handle(h);
     return MOL_ERR_HIP;
   }
   *out = h;
+  return 0;
+}
+
+int mol_set_tile(mol_handle* h, int32_t tx, int32_t ty, int32_t tz) {
+  if (!h) return fail(nullptr, MOL_ERR_INVALID, "handle is NULL");
+  if (tx < 0 || ty < 0 || tz < 0 || (int64_t)std::max(tx, 1) * std::max(ty, 1) * std::max(tz, 1) > 4096)
+    return fail(h, MOL_ERR_INVALID, "tile shape out of range (at most 4096 nodes)");
+  h->tile_req[0] = tx;
+  h->tile_req[1] = ty;
+  h->tile_req[2] = tz;
+  const mol_desc* d = &h->desc.d;
+  const void* aux64[MOL_MAX_AUX];
+  for (int a = 0; a < MOL_MAX_AUX; ++a) aux64[a] = h->ka64.aux[a];
+  fill_kargs<double>(h, d, h->ka64);
+  fill_kargs<float>(h, d, h->ka32);
+  for (int a = 0; a < MOL_MAX_AUX; ++a) {
+    h->ka64.aux[a] = (const double*)aux64[a];
+    h->ka32.aux[a] = (const float*)aux64[a];
+  }
+  if (sync_kargs(h) || setup_paths(h, d)) return MOL_ERR_HIP;
+  for (int a = 0; a < MOL_MAX_AUX; ++a) {
+    h->kc64.aux[a] = (const double*)aux64[a];
+    h->kc32.aux[a] = (const float*)aux64[a];
+    h->kcw64.aux[a] = (const double*)aux64[a];
+    h->kcw32.aux[a] = (const float*)aux64[a];
+  }
+  return 0;
+}
+
+int mol_get_plan(const mol_handle* h, int32_t* info, int32_t ninfo) {
+  if (!h || !info) return MOL_ERR_INVALID;
+  int32_t v[8] = {h->ka64.tile[0], h->ka64.tile[1], h->ka64.tile[2], h->ncore, h->nbound,
+                  h->use_weno ? h->nweno : 0, 0, 0};
+  for (int i = 0; i < ninfo && i < 8; ++i) info[i] = v[i];
   return 0;
 }
 
@@ -378,6 +669,10 @@ int mol_set_aux(mol_handle* h, void* const* aux) {
     if (!aux[a]) return fail(h, MOL_ERR_INVALID, "aux pointer is NULL");
     h->ka64.aux[a] = (const double*)aux[a];
     h->ka32.aux[a] = (const float*)aux[a];
+    h->kc64.aux[a] = (const double*)aux[a];
+    h->kc32.aux[a] = (const float*)aux[a];
+    h->kcw64.aux[a] = (const double*)aux[a];
+    h->kcw32.aux[a] = (const float*)aux[a];
   }
   return sync_kargs(h);
 }
@@ -397,12 +692,12 @@ int mol_rhs(mol_handle* h, void* du, const void* u, const void* p, double t, voi
   if (h->dtype == MOL_F64) {
     Epi<double> e{};
     e.mode = 0; e.out = (double*)du; e.tofs = t; e.p = (const double*)p;
-    if (launch_rhs_generic<double>(h->ka64, h->d_ka64, (const double*)u, e, s))
+    if (launch_rhs<double>(launch_of<double>(h), (const double*)u, e, s))
       return fail(h, MOL_ERR_HIP, std::string("rhs launch: ") + hipGetErrorString(hipGetLastError()));
   } else {
     Epi<float> e{};
     e.mode = 0; e.out = (float*)du; e.tofs = t; e.p = (const float*)p;
-    if (launch_rhs_generic<float>(h->ka32, h->d_ka32, (const float*)u, e, s))
+    if (launch_rhs<float>(launch_of<float>(h), (const float*)u, e, s))
       return fail(h, MOL_ERR_HIP, std::string("rhs launch: ") + hipGetErrorString(hipGetLastError()));
   }
   return 0;
@@ -443,6 +738,18 @@ int mol_integrate(mol_handle* h, void* u, const void* p, double t0, double dt, i
   if (h->dtype == MOL_F64)
     return integrate_impl<double>(h, (double*)u, (const double*)p, t0, dt, nsteps, scheme, flags, s);
   return integrate_impl<float>(h, (float*)u, (const float*)p, t0, dt, nsteps, scheme, flags, s);
+}
+
+int mol_vjp(mol_handle* h, const void*, const void*, double, const void*, void*, double*, void*) {
+  return fail(h, MOL_ERR_UNSUPPORTED, "mol_vjp: not available in this build");
+}
+
+int mol_forward_checkpointed(mol_handle* h, void*, const void*, double, double, int64_t, int32_t, void*) {
+  return fail(h, MOL_ERR_UNSUPPORTED, "mol_forward_checkpointed: not available in this build");
+}
+
+int mol_adjoint(mol_handle* h, const void*, void*, double*, void*) {
+  return fail(h, MOL_ERR_UNSUPPORTED, "mol_adjoint: not available in this build");
 }
 
 int64_t mol_num_unknowns(const mol_handle* h) { return h ? h->nunk : -1; }

@@ -75,6 +75,34 @@ struct Epi {
   const T* p;           // scalar parameters
 };
 
+template <typename T>
+__device__ __forceinline__ void epilogue(const Epi<T>& ep, long long idx, T f) {
+  if (ep.mode == 0) {
+    ep.out[idx] = f;
+    return;
+  }
+  if (ep.mode == 1) {
+    T v = (T)ep.cf * f;
+    if (ep.A) v += (T)ep.ca * ep.A[idx];
+    if (ep.B) v += (T)ep.cb * ep.B[idx];
+    if (ep.acc) {
+      T a = (T)ep.acc_f * f;
+      if (ep.acc_keep != 0.0) a += (T)ep.acc_keep * ep.acc[idx];
+      ep.acc[idx] = a;
+    }
+    ep.out[idx] = v;
+    return;
+  }
+  // mode 2: compensated (Kahan) accumulation of the step update into A
+  const T a = ep.A[idx];
+  T incr = (T)ep.cf * f;
+  if (ep.B) incr += (T)ep.cb * (ep.subA ? (ep.B[idx] - a) : ep.B[idx]);
+  const T y = incr - ep.comp[idx];
+  const T s = a + y;
+  ep.comp[idx] = (s - a) - y;
+  ep.out[idx] = s;
+}
+
 __device__ __forceinline__ int wrap_periodic(int k, int n) {
   // single wrap of the reference's bwrap: k <= 1 -> k + (n-1); k > n -> k - (n-1)
   // (src/discretization/interface_boundary.jl:33-42)
@@ -161,67 +189,178 @@ __device__ __noinline__ T math_unary(uint32_t op, T a) {
 template <typename T>
 __device__ __noinline__ T math_pow(T a, T b) { return pow(a, b); }
 
-// Register-machine interpreter over a context providing the leaf values.
-template <typename T, class Ctx>
-__device__ T run_prog(const KArgs<T>& ka, int prog, Ctx& ctx) {
-  const int start = ka.prog[2 * prog];
-  const int len = ka.prog[2 * prog + 1];
-  T r[MOL_NREG];
-#pragma unroll
-  for (int q = 0; q < MOL_NREG; ++q) r[q] = T(0);
+// Interpreter of the pointwise bytecode.  The host compiler allocates registers as a stack
+// (an instruction at depth d reads r[d] and r[d+1] and writes r[d]), so the register file is
+// kept as a shift-register stack s0 (top) .. s5 in named variables: no indexed array, nothing
+// in scratch.  Leaves push, binary operations pop (or use a constant operand), unary
+// operations rewrite the top.  Programs deeper than MOL_STACK are rejected by the host.
+#define MOL_STACK 6
+
+template <typename T>
+__device__ __forceinline__ T binop(uint32_t op, T a, T b) {
+  switch (op) {
+    case MOL_OP_ADD: return a + b;
+    case MOL_OP_SUB: return a - b;
+    case MOL_OP_MUL: return a * b;
+    case MOL_OP_DIV: return a / b;
+    case MOL_OP_POW: return math_pow(a, b);
+    case MOL_OP_MIN: return fmin(a, b);
+    case MOL_OP_MAX: return fmax(a, b);
+    case MOL_OP_LT: return a < b ? T(1) : T(0);
+    case MOL_OP_LE: return a <= b ? T(1) : T(0);
+    case MOL_OP_GT: return a > b ? T(1) : T(0);
+    case MOL_OP_GE: return a >= b ? T(1) : T(0);
+    case MOL_OP_EQ: return a == b ? T(1) : T(0);
+    case MOL_OP_NE: return a != b ? T(1) : T(0);
+    case MOL_OP_AND: return (a != T(0) && b != T(0)) ? T(1) : T(0);
+    case MOL_OP_OR: return (a != T(0) || b != T(0)) ? T(1) : T(0);
+    default: return a;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T unop(uint32_t op, T a, int imm) {
+  switch (op) {
+    case MOL_OP_NEG: return -a;
+    case MOL_OP_SQ: return a * a;
+    case MOL_OP_INV: return T(1) / a;
+    case MOL_OP_ABS: return fabs(a);
+    case MOL_OP_SQRT: return sqrt(a);
+    case MOL_OP_SIGN: return a > T(0) ? T(1) : (a < T(0) ? T(-1) : T(0));
+    case MOL_OP_HEAVISIDE: return a > T(0) ? T(1) : (a < T(0) ? T(0) : T(0.5));
+    case MOL_OP_POWI: return powi(a, (imm & 0x400) ? imm - 0x800 : imm);
+    default: return math_unary(op, a);
+  }
+}
+
+// code/cpool/prog come from `Src` (kernel arguments or device memory).
+template <typename T, class Src, class Ctx>
+__device__ __forceinline__ T stack_prog(const Src& src, int prog, const Ctx& ctx) {
+  const int start = src.prog[2 * prog];
+  const int len = src.prog[2 * prog + 1];
+  T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0), s4 = T(0), s5 = T(0);
   for (int pc = start; pc < start + len; ++pc) {
-    const uint32_t ins = ka.code[pc];
+    const uint32_t ins = src.code[pc];
     const uint32_t op = ins & 0xffu;
-    const uint32_t d = (ins >> 8) & 0xfu;
-    const uint32_t ia = (ins >> 12) & 0xfu;
-    const uint32_t ib = (ins >> 16) & 0xfu;
     const bool bconst = (ins >> 20) & 1u;
     const int imm = (int)(ins >> 21);
-    const int simm = (imm & 0x400) ? imm - 0x800 : imm;
-    const T a = r[ia];
-    const T b = bconst ? (T)ka.cpool[imm] : r[ib];
-    const T c = r[d];
-    T res;
-    switch (op) {
-      case MOL_OP_CONST: res = (T)ka.cpool[imm]; break;
-      case MOL_OP_FIELD: res = ctx.field(imm); break;
-      case MOL_OP_SLOT: res = ctx.slot(imm); break;
-      case MOL_OP_COORD: res = ctx.coord(imm); break;
-      case MOL_OP_TIME: res = ctx.time(); break;
-      case MOL_OP_PARAM: res = ctx.param(imm); break;
-      case MOL_OP_AUX: res = ctx.aux(imm); break;
-      case MOL_OP_DINNER: res = ctx.dinner(); break;
-      case MOL_OP_MOV: res = a; break;
-      case MOL_OP_ADD: res = a + b; break;
-      case MOL_OP_SUB: res = a - b; break;
-      case MOL_OP_MUL: res = a * b; break;
-      case MOL_OP_DIV: res = a / b; break;
-      case MOL_OP_POW: res = math_pow(a, b); break;
-      case MOL_OP_MIN: res = fmin(a, b); break;
-      case MOL_OP_MAX: res = fmax(a, b); break;
-      case MOL_OP_LT: res = a < b ? T(1) : T(0); break;
-      case MOL_OP_LE: res = a <= b ? T(1) : T(0); break;
-      case MOL_OP_GT: res = a > b ? T(1) : T(0); break;
-      case MOL_OP_GE: res = a >= b ? T(1) : T(0); break;
-      case MOL_OP_EQ: res = a == b ? T(1) : T(0); break;
-      case MOL_OP_NE: res = a != b ? T(1) : T(0); break;
-      case MOL_OP_AND: res = (a != T(0) && b != T(0)) ? T(1) : T(0); break;
-      case MOL_OP_OR: res = (a != T(0) || b != T(0)) ? T(1) : T(0); break;
-      case MOL_OP_SELECT: res = (c != T(0)) ? a : b; break;
-      case MOL_OP_FMA: res = fma(a, b, c); break;
-      case MOL_OP_NEG: res = -a; break;
-      case MOL_OP_ABS: res = fabs(a); break;
-      case MOL_OP_SIGN: res = a > T(0) ? T(1) : (a < T(0) ? T(-1) : T(0)); break;
-      case MOL_OP_POWI: res = powi(a, simm); break;
-      case MOL_OP_INV: res = T(1) / a; break;
-      case MOL_OP_SQ: res = a * a; break;
-      case MOL_OP_HEAVISIDE: res = a > T(0) ? T(1) : (a < T(0) ? T(0) : T(0.5)); break;
-      case MOL_OP_SQRT: res = sqrt(a); break;
-      default: res = op >= 48 ? math_unary(op, a) : c; break;
+    if (op < 16u) {
+      T v;
+      switch (op) {
+        case MOL_OP_CONST: v = (T)src.cpool[imm]; break;
+        case MOL_OP_FIELD: v = ctx.field(imm); break;
+        case MOL_OP_SLOT: v = ctx.slot(imm); break;
+        case MOL_OP_COORD: v = ctx.coord(imm); break;
+        case MOL_OP_TIME: v = ctx.time(); break;
+        case MOL_OP_PARAM: v = ctx.param(imm); break;
+        case MOL_OP_AUX: v = ctx.aux(imm); break;
+        default: v = ctx.dinner(); break;
+      }
+      s5 = s4; s4 = s3; s3 = s2; s2 = s1; s1 = s0; s0 = v;
+    } else if (op < 48u) {
+      if (bconst) {
+        const T c = (T)src.cpool[imm];
+        if (op == MOL_OP_SELECT) {
+          s0 = (s1 != T(0)) ? s0 : c;
+          s1 = s2; s2 = s3; s3 = s4; s4 = s5;
+        } else {
+          s0 = binop(op, s0, c);
+        }
+      } else if (op == MOL_OP_SELECT) {
+        s0 = (s2 != T(0)) ? s1 : s0;
+        s1 = s3; s2 = s4; s3 = s5;
+      } else {
+        s0 = binop(op, s1, s0);
+        s1 = s2; s2 = s3; s3 = s4; s4 = s5;
+      }
+    } else {
+      s0 = unop(op, s0, imm);
     }
-    r[d] = res;
   }
-  return r[0];
+  return s0;
+}
+
+// Vectorised interpreter: one instruction dispatch serves B nodes held by the same lane
+// (register blocking), which divides the scalar-unit cost of interpretation by B.  The stack
+// is DEPTH levels of B values; the host only routes programs of depth <= DEPTH here.
+// Ctx::leaf(op, imm, v) fills v[0..B) with the leaf values of the B nodes.
+template <typename T, int B, int DEPTH, class Src, class Ctx>
+__device__ __forceinline__ void stack_prog_vec(const Src& src, int prog, const Ctx& ctx, T* out) {
+  static_assert(DEPTH >= 2 && DEPTH <= 4, "vector stack depth");
+  const int start = src.prog[2 * prog];
+  const int len = src.prog[2 * prog + 1];
+  T s0[B], s1[B], s2[B], s3[B];
+#pragma unroll
+  for (int b = 0; b < B; ++b) s0[b] = s1[b] = s2[b] = s3[b] = T(0);
+  for (int pc = start; pc < start + len; ++pc) {
+    const uint32_t ins = src.code[pc];
+    const uint32_t op = ins & 0xffu;
+    const bool bconst = (ins >> 20) & 1u;
+    const int imm = (int)(ins >> 21);
+    if (op < 16u) {
+      T v[B];
+      if (op == MOL_OP_CONST) {
+        const T c = (T)src.cpool[imm];
+#pragma unroll
+        for (int b = 0; b < B; ++b) v[b] = c;
+      } else {
+        ctx.leaf(op, imm, v);
+      }
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        if (DEPTH > 3) s3[b] = s2[b];
+        if (DEPTH > 2) s2[b] = s1[b];
+        s1[b] = s0[b];
+        s0[b] = v[b];
+      }
+    } else if (op < 48u) {
+      if (bconst) {
+        const T c = (T)src.cpool[imm];
+        if (op == MOL_OP_SELECT) {
+#pragma unroll
+          for (int b = 0; b < B; ++b) {
+            s0[b] = (s1[b] != T(0)) ? s0[b] : c;
+            s1[b] = s2[b];
+            s2[b] = s3[b];
+          }
+        } else {
+#pragma unroll
+          for (int b = 0; b < B; ++b) s0[b] = binop(op, s0[b], c);
+        }
+      } else if (op == MOL_OP_SELECT) {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+          s0[b] = (s2[b] != T(0)) ? s1[b] : s0[b];
+          s1[b] = s3[b];
+        }
+      } else {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+          s0[b] = binop(op, s1[b], s0[b]);
+          s1[b] = s2[b];
+          s2[b] = s3[b];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int b = 0; b < B; ++b) s0[b] = unop(op, s0[b], imm);
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < B; ++b) out[b] = s0[b];
+}
+
+// Tables of the generic path in device memory, in the layout stack_prog reads.
+struct GlobalProgs {
+  const int32_t* prog;
+  const uint32_t* code;
+  const double* cpool;
+};
+
+template <typename T, class Ctx>
+__device__ __forceinline__ T run_prog(const KArgs<T>& ka, int prog, Ctx& ctx) {
+  GlobalProgs g{ka.prog, ka.code, ka.cpool};
+  return stack_prog<T>(g, prog, ctx);
 }
 
 // Context used for ghost-node programs G(t, boundary coordinates): no variables.

@@ -4,13 +4,44 @@
 #include <string>
 #include <vector>
 #include "mol.h"
+#include "mol_core.h"
 #include "mol_device.h"
 
 namespace mol {
 
+// WENO5 fast-path plan: the WENO slots along x and y and their variables.
+struct WenoPlan {
+  int nx, ny;
+  int sx[2], sy[2];
+  int fx[2], fy[2];
+};
+
+// What one right-hand-side launch needs: the core-tile kernel arguments and the list of tiles
+// left to the generic kernel.
 template <typename T>
-int launch_rhs_generic(const KArgs<T>& ka, const KArgs<T>* dka, const T* u, const Epi<T>& ep,
-                       hipStream_t stream);
+struct Launch {
+  const KArgs<T>* ka = nullptr;      // host copy (sizes)
+  const KArgs<T>* dka = nullptr;     // device copy
+  const KCore<T>* kc = nullptr;      // core kernel arguments (host)
+  const int32_t* tiles = nullptr;    // device list of generic tiles
+  int ncore = 0, nbound = 0;
+  const KCore<T>* kcw = nullptr;     // WENO fast-path arguments (nullptr: not used)
+  const WenoPlan* plan = nullptr;
+  int nweno = 0;
+};
+
+template <typename T>
+int launch_rhs(const Launch<T>& L, const T* u, const Epi<T>& ep, hipStream_t stream);
+
+// WENO5 fast path (2D, periodic, every slot a WENO5 derivative)
+bool weno_plan(const mol_desc* d, WenoPlan& plan);
+int program_depth(const mol_desc* d, int prog);
+template <typename T>
+size_t weno_lds_bytes(int nfields, int nx);
+template <typename T>
+int launch_weno2d(const KCore<T>& kc, const WenoPlan& plan, int nblocks, const T* u, const Epi<T>& ep,
+                  hipStream_t stream);
+constexpr int WENO_TILE_X = 64, WENO_TILE_Y = 32;
 int launch_advance_time(double* tnow, double dt, hipStream_t stream);
 
 // Set the message of the calling thread's last failure (used when no handle exists yet).
@@ -19,7 +50,42 @@ const char* global_error();
 
 }  // namespace mol
 
+// Owned copy of a descriptor (the handle re-plans launches from it).
+struct DescCopy {
+  mol_desc d{};
+  std::vector<double> coord[3];
+  std::vector<mol_slot> slots;
+  std::vector<mol_optab> tabs;
+  std::vector<mol_closure> clos;
+  std::vector<double> pool, cpool;
+  std::vector<uint32_t> code;
+  std::vector<int32_t> prog;
+  void assign(const mol_desc* s) {
+    d = *s;
+    for (int a = 0; a < s->ndim; ++a) {
+      coord[a].assign(s->coord_h[a], s->coord_h[a] + s->n[a]);
+      d.coord_h[a] = coord[a].data();
+    }
+    slots.assign(s->slots_h, s->slots_h + s->nslots);
+    tabs.assign(s->tabs_h, s->tabs_h + s->ntabs);
+    clos.assign(s->closures_h, s->closures_h + s->nclosures);
+    pool.assign(s->pool_h, s->pool_h + s->npool);
+    cpool.assign(s->cpool_h, s->cpool_h + s->ncpool);
+    code.assign(s->code_h, s->code_h + s->ncode);
+    prog.assign(s->prog_h, s->prog_h + 2 * (size_t)s->nprogs);
+    d.slots_h = slots.data();
+    d.tabs_h = tabs.data();
+    d.closures_h = clos.data();
+    d.pool_h = pool.data();
+    d.cpool_h = cpool.data();
+    d.code_h = code.data();
+    d.prog_h = prog.data();
+  }
+};
+
 struct mol_handle {
+  DescCopy desc;
+  int tile_req[3] = {0, 0, 0};            // requested tile shape (0: default)
   int device = 0;
   int dtype = MOL_F64;
   int ndim = 1, nfields = 0, naux = 0, nparams = 0;
@@ -29,6 +95,16 @@ struct mol_handle {
   mol::KArgs<float> ka32{};
   mol::KArgs<double>* d_ka64 = nullptr;   // device copies read by the kernels
   mol::KArgs<float>* d_ka32 = nullptr;
+  mol::KCore<double> kc64{};
+  mol::KCore<float> kc32{};
+  int32_t* d_tiles = nullptr;             // tiles handled by the generic kernel
+  int ncore = 0, nbound = 0;
+  std::string core_reason;                // why the core path is off, if it is
+  bool use_weno = false;                  // WENO5 fast path
+  mol::WenoPlan wplan{};
+  mol::KCore<double> kcw64{};
+  mol::KCore<float> kcw32{};
+  int nweno = 0;
   // device copies of the descriptor tables
   void* d_tabs = nullptr;
   void* d_slots = nullptr;

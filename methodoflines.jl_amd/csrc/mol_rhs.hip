@@ -1,12 +1,13 @@
 // Generic stencil-IR right-hand-side kernel with fused Runge-Kutta stage epilogues.
+#include "mol_core.h"
 #include "mol_device.h"
 #include "mol_internal.h"
 
 namespace mol {
 
 template <typename T>
-__device__ __forceinline__ void tile_origin(const KArgs<T>& ka, int* org) {
-  int b = blockIdx.x;
+__device__ __forceinline__ void tile_origin(const KArgs<T>& ka, const int32_t* tiles, int* org) {
+  int b = tiles ? tiles[blockIdx.x] : (int)blockIdx.x;
   const int tx = b % ka.ntile[0];
   b /= ka.ntile[0];
   const int ty = b % ka.ntile[1];
@@ -38,41 +39,14 @@ __device__ void stage_tile(const KArgs<T>& ka, const T* __restrict__ u, T* lds, 
 }
 
 template <typename T>
-__device__ __forceinline__ void epilogue(const Epi<T>& ep, long long idx, T f) {
-  if (ep.mode == 0) {
-    ep.out[idx] = f;
-    return;
-  }
-  if (ep.mode == 1) {
-    T v = (T)ep.cf * f;
-    if (ep.A) v += (T)ep.ca * ep.A[idx];
-    if (ep.B) v += (T)ep.cb * ep.B[idx];
-    if (ep.acc) {
-      T a = (T)ep.acc_f * f;
-      if (ep.acc_keep != 0.0) a += (T)ep.acc_keep * ep.acc[idx];
-      ep.acc[idx] = a;
-    }
-    ep.out[idx] = v;
-    return;
-  }
-  // mode 2: compensated (Kahan) accumulation of the step update into A
-  const T a = ep.A[idx];
-  T incr = (T)ep.cf * f;
-  if (ep.B) incr += (T)ep.cb * (ep.subA ? (ep.B[idx] - a) : ep.B[idx]);
-  const T y = incr - ep.comp[idx];
-  const T s = a + y;
-  ep.comp[idx] = (s - a) - y;
-  ep.out[idx] = s;
-}
-
-template <typename T>
 __global__ void __launch_bounds__(256) rhs_generic_kernel(const KArgs<T>* __restrict__ kap,
+                                                          const int32_t* __restrict__ tiles,
                                                           const T* __restrict__ u, Epi<T> ep) {
   const KArgs<T>& ka = *kap;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
   int org[3];
-  tile_origin(ka, org);
+  tile_origin(ka, tiles, org);
   const double t = (ep.tnow ? ep.tnow[0] : 0.0) + ep.tofs;
   stage_tile(ka, u, lds, org, t, ep.p);
   __syncthreads();
@@ -117,19 +91,92 @@ __global__ void __launch_bounds__(256) rhs_generic_kernel(const KArgs<T>* __rest
   }
 }
 
+// Core tiles: the stencil IR in the kernel arguments, interior rows, wrap-only staging.
 template <typename T>
-int launch_rhs_generic(const KArgs<T>& ka, const KArgs<T>* dka, const T* u, const Epi<T>& ep,
-                       hipStream_t stream) {
-  const int nblocks = ka.ntile[0] * ka.ntile[1] * ka.ntile[2];
-  const size_t lds = (size_t)(ka.nfields + ka.naux) * ka.vol * sizeof(T);
-  hipLaunchKernelGGL(rhs_generic_kernel<T>, dim3(nblocks), dim3(256), lds, stream, dka, u, ep);
+__global__ void __launch_bounds__(256) rhs_core_kernel(const KCore<T> kc_arg, const T* __restrict__ u,
+                                                       Epi<T> ep) {
+  // The argument block is read in place through the kernel-argument segment (constant address
+  // space, scalar loads); taking the address of the by-value parameter would copy it to scratch.
+#if defined(__HIP_DEVICE_COMPILE__)
+  const KCore<T>& kc = *(const KCore<T>*)(__builtin_amdgcn_kernarg_segment_ptr());
+  (void)kc_arg;
+#else
+  const KCore<T>& kc = kc_arg;  // host pass: never executed
+#endif
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* lds = reinterpret_cast<T*>(smem_raw);
+  int b = blockIdx.x;
+  const int tx = kc.ct_lo[0] + b % kc.ct_n[0];
+  b /= kc.ct_n[0];
+  const int ty = kc.ct_lo[1] + b % kc.ct_n[1];
+  const int tz = kc.ct_lo[2] + b / kc.ct_n[1];
+  const int org0 = kc.lo[0] + tx * kc.tile[0];
+  const int org1 = kc.lo[1] + ty * kc.tile[1];
+  const int org2 = kc.lo[2] + tz * kc.tile[2];
+  const double t = (ep.tnow ? ep.tnow[0] : 0.0) + ep.tofs;
+  const int bx = kc.hi[0] - kc.lo[0] + 1;
+  const int by = kc.hi[1] - kc.lo[1] + 1;
+  const int ext01 = kc.ext[0] * kc.ext[1];
+  // stage: every position of tile + halo is an unknown (or its periodic image)
+  for (int q = threadIdx.x; q < kc.vol; q += blockDim.x) {
+    const int lz = (int)fdiv_magic((uint32_t)q, kc.magic1);
+    const int r = q - lz * ext01;
+    const int ly = (int)fdiv_magic((uint32_t)r, kc.magic0);
+    const int lx = r - ly * kc.ext[0];
+    int p0 = org0 + lx - kc.halo[0], p1 = org1 + ly - kc.halo[1], p2 = org2 + lz - kc.halo[2];
+    if (kc.periodic[0]) p0 = wrap_periodic(p0, kc.n[0]);
+    if (kc.periodic[1]) p1 = wrap_periodic(p1, kc.n[1]);
+    if (kc.periodic[2]) p2 = wrap_periodic(p2, kc.n[2]);
+    const long long li = (long long)(p0 - kc.lo[0]) + (long long)bx * ((p1 - kc.lo[1]) + (long long)by * (p2 - kc.lo[2]));
+    for (int f = 0; f < kc.nfields; ++f) lds[f * kc.vol + q] = u[f * kc.box + li];
+    if (kc.naux) {
+      const long long gi = (long long)(p0 - 1) + (long long)kc.n[0] * ((p1 - 1) + (long long)kc.n[1] * (p2 - 1));
+      for (int a = 0; a < kc.naux; ++a) lds[(kc.nfields + a) * kc.vol + q] = kc.aux[a][gi];
+    }
+  }
+  __syncthreads();
+  CoreView<T> v;
+  v.kc = &kc;
+  v.lds = lds;
+  v.st[0] = 1;
+  v.st[1] = kc.ext[0];
+  v.st[2] = ext01;
+  v.t = t;
+  v.p = ep.p;
+  const int tvol = kc.tile[0] * kc.tile[1] * kc.tile[2];
+  const int t01 = kc.tile[0] * kc.tile[1];
+  for (int q = threadIdx.x; q < tvol; q += blockDim.x) {
+    const int lz = q / t01;
+    const int r = q - lz * t01;
+    const int ly = r / kc.tile[0];
+    const int lx = r - ly * kc.tile[0];
+    v.I[0] = org0 + lx;
+    v.I[1] = org1 + ly;
+    v.I[2] = org2 + lz;
+    if (v.I[0] > kc.hi[0] || v.I[1] > kc.hi[1] || v.I[2] > kc.hi[2]) continue;
+    v.base = (lx + kc.halo[0]) + kc.ext[0] * (ly + kc.halo[1]) + ext01 * (lz + kc.halo[2]);
+    const long long li = (long long)(v.I[0] - kc.lo[0]) + (long long)bx * ((v.I[1] - kc.lo[1]) + (long long)by * (v.I[2] - kc.lo[2]));
+    CoreNodeCtx<T> nc{&v};
+    for (int f = 0; f < kc.nfields; ++f) {
+      const T val = core_prog(kc, kc.rhs_prog[f], nc);
+      epilogue(ep, f * kc.box + li, val);
+    }
+  }
+}
+
+template <typename T>
+int launch_rhs(const Launch<T>& L, const T* u, const Epi<T>& ep, hipStream_t stream) {
+  if (L.kcw) return launch_weno2d<T>(*L.kcw, *L.plan, L.nweno, u, ep, stream);
+  const size_t lds = (size_t)(L.ka->nfields + L.ka->naux) * L.ka->vol * sizeof(T);
+  if (L.ncore > 0)
+    hipLaunchKernelGGL(rhs_core_kernel<T>, dim3(L.ncore), dim3(256), lds, stream, *L.kc, u, ep);
+  if (L.nbound > 0)
+    hipLaunchKernelGGL(rhs_generic_kernel<T>, dim3(L.nbound), dim3(256), lds, stream, L.dka, L.tiles, u, ep);
   return hipGetLastError() == hipSuccess ? 0 : MOL_ERR_HIP;
 }
 
-template int launch_rhs_generic<double>(const KArgs<double>&, const KArgs<double>*, const double*,
-                                        const Epi<double>&, hipStream_t);
-template int launch_rhs_generic<float>(const KArgs<float>&, const KArgs<float>*, const float*,
-                                       const Epi<float>&, hipStream_t);
+template int launch_rhs<double>(const Launch<double>&, const double*, const Epi<double>&, hipStream_t);
+template int launch_rhs<float>(const Launch<float>&, const float*, const Epi<float>&, hipStream_t);
 
 __global__ void advance_time_kernel(double* tnow, double dt) {
   if (threadIdx.x == 0 && blockIdx.x == 0) tnow[0] += dt;

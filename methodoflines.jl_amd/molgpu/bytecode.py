@@ -2,9 +2,11 @@
 
 An instruction is one 32-bit word (see include/mol.h, "pointwise programs"):
 opcode | dst<<8 | a<<12 | b<<16 | bconst<<20 | imm<<21.  Registers are allocated as a
-stack (registers >= dst are free while computing into dst); operands are ordered by
-register need (Sethi-Ullman), so that an expression tree of depth d needs at most d
-registers.  The result of a program is register 0.
+stack (registers >= dst are free while computing into dst): a leaf writes the register at
+the current depth, a binary operation combines the two top registers into the lower one.
+The device interprets the code as a stack machine on that discipline (the library checks
+it).  Operands are ordered by register need (Sethi-Ullman), so an expression tree needs as
+few registers as possible; more than NREG is an error.  The result is register 0.
 """
 from __future__ import annotations
 
@@ -12,7 +14,7 @@ from typing import Callable, Dict, List, Tuple
 
 import sympy as sp
 
-NREG = 8
+NREG = 6   # depth of the device interpreter's shift-register stack (MOL_STACK)
 
 OP = dict(
     CONST=0, FIELD=1, SLOT=2, COORD=3, TIME=4, PARAM=5, AUX=6, DINNER=7, MOV=8,

@@ -77,6 +77,13 @@ _SIGNATURES = {
     "mol_integrate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int64,
                                 C.c_int32, C.c_int32, C.c_void_p]),
     "mol_num_unknowns": (C.c_int64, [C.c_void_p]),
+    "mol_set_tile": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
+    "mol_get_plan": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32]),
+    "mol_vjp": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
+                          C.c_void_p, C.c_void_p]),
+    "mol_forward_checkpointed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                           C.c_int64, C.c_int32, C.c_void_p]),
+    "mol_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mol_last_error": (C.c_char_p, [C.c_void_p]),
     "mol_destroy": (None, [C.c_void_p]),
     "mol_device_info": (C.c_int, [C.c_char_p, C.c_int]),

@@ -113,6 +113,18 @@ class ODEProblem:
             ptrs[i] = t.data_ptr()
         native.check(self._handle.lib, self._handle.lib.mol_set_aux(self._handle.h, ptrs), self._handle.h)
 
+    def set_tile(self, tx: int, ty: int = 1, tz: int = 1) -> None:
+        """Tile shape (nodes per work-group) of the stencil kernels; (0, 0, 0) restores the default."""
+        h = self.handle
+        native.check(h.lib, h.lib.mol_set_tile(h.h, tx, ty, tz), h.h)
+
+    def plan(self) -> dict:
+        h = self.handle
+        info = (C.c_int32 * 8)()
+        h.lib.mol_get_plan(h.h, info, 8)
+        return {"tile": (info[0], info[1], info[2]), "core_tiles": info[3], "generic_tiles": info[4],
+                "weno_tiles": info[5]}
+
     def aux_tensor(self, name: str):
         self.handle
         return self._aux[name]
