@@ -191,7 +191,9 @@ int mol_rhs_h(mol_handle* h, void* du_h, const void* u_h, const void* p_h, doubl
  * No counterpart in the reference; used by the autotuner.  At most 4096 nodes per tile. */
 int mol_set_tile(mol_handle* h, int32_t tx, int32_t ty, int32_t tz);
 
-/* Launch plan: info[0..2] tile shape, info[3] core tiles, info[4] generic tiles, info[5] WENO tiles. */
+/* Launch plan: info[0..2] tile shape, info[3] core tiles, info[4] generic tiles, info[5] WENO tiles,
+ * info[6] 1 when the core tiles run the generated kernel, info[7] 1 when the IR fits the
+ * interpreter's core kernel. */
 int mol_get_plan(const mol_handle* h, int32_t* info, int32_t ninfo);
 
 /* Bind auxiliary full-grid arrays (device pointers, naux of them, each prod(n) values). */
@@ -237,6 +239,13 @@ const char* mol_last_error(const mol_handle* h);
 
 /* Replaces: nothing to free in the reference (GC); releases device memory of the handle. */
 void mol_destroy(mol_handle* h);
+
+/* Generated source of the core-tile kernel for a descriptor (the counterpart of the reference's
+ * build_function codegen, src/MOL_discretization.jl:171-201).  Needs no GPU.  Writes at most
+ * buflen bytes (NUL-terminated) into buf (may be NULL) and returns the source length, or a
+ * negative mol_status (message in mol_last_error(NULL)); tile = tx,ty,tz or NULL for the default.
+ * compile != 0 also compiles it with hiprtc for gfx950 and fails when that does. */
+int64_t mol_codegen(const mol_desc* d, const int32_t* tile, char* buf, int64_t buflen, int32_t compile);
 
 /* Library / device information (for test harnesses). */
 int mol_device_info(char* buf, int buflen);

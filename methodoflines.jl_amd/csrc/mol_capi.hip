@@ -300,6 +300,10 @@ Launch<T> launch_of(mol_handle* h) {
   L.tiles = h->d_tiles;
   L.ncore = h->ncore;
   L.nbound = h->nbound;
+  L.jit = h->jit_fn;
+  const KArgs<T>& ka = kargs<T>(h);
+  for (int a = 0; a < 3; ++a) L.jp.coord[a] = ka.coord[a];
+  for (int a = 0; a < MOL_MAX_AUX; ++a) L.jp.aux[a] = ka.aux[a];
   return L;
 }
 
@@ -324,13 +328,15 @@ void build_core(mol_handle* h, const mol_desc* d, KCore<T>& kc, std::vector<int3
     gen_tiles.resize(ntot);
     for (int i = 0; i < ntot; ++i) gen_tiles[i] = i;
     h->core_reason = why;
+    h->core_interp_ok = false;
   };
   for (int f = 1; f < d->nfields; ++f)
     for (int a = 0; a < d->ndim; ++a)
       if (d->lo[f][a] != d->lo[0][a] || d->hi[f][a] != d->hi[0][a]) return all_generic("variables with different unknown boxes");
-  if (d->ncode > CORE_MAX_CODE || d->ncpool > CORE_MAX_CONST || d->nslots > CORE_MAX_SLOTS ||
-      d->nprogs > CORE_MAX_PROGS)
-    return all_generic("stencil IR larger than the kernel-argument block");
+  // the interpreter's core kernel carries the IR in its argument block; the generated kernel
+  // has no such limit, so the geometry below is computed either way
+  bool fits = !(d->ncode > CORE_MAX_CODE || d->ncpool > CORE_MAX_CONST || d->nslots > CORE_MAX_SLOTS ||
+                d->nprogs > CORE_MAX_PROGS);
   kc.ndim = d->ndim;
   kc.nfields = d->nfields;
   kc.naux = d->naux;
@@ -352,9 +358,11 @@ void build_core(mol_handle* h, const mol_desc* d, KCore<T>& kc, std::vector<int3
   kc.magic0 = magic32((uint32_t)kc.ext[0]);
   kc.magic1 = magic32((uint32_t)(kc.ext[0] * kc.ext[1]));
   for (int f = 0; f < d->nfields; ++f) kc.rhs_prog[f] = d->rhs_prog[f];
-  for (int i = 0; i < 2 * d->nprogs; ++i) kc.prog[i] = d->prog_h[i];
-  for (int i = 0; i < d->ncode; ++i) kc.code[i] = d->code_h[i];
-  for (int i = 0; i < d->ncpool; ++i) kc.cpool[i] = d->cpool_h[i];
+  if (fits) {
+    for (int i = 0; i < 2 * d->nprogs; ++i) kc.prog[i] = d->prog_h[i];
+    for (int i = 0; i < d->ncode; ++i) kc.code[i] = d->code_h[i];
+    for (int i = 0; i < d->ncpool; ++i) kc.cpool[i] = d->cpool_h[i];
+  }
   for (int a = 0; a < 3; ++a) kc.coord[a] = ka.coord[a];
   for (int a = 0; a < MOL_MAX_AUX; ++a) kc.aux[a] = ka.aux[a];
   // core node range per axis: tile + halo inside the box, interior rows for every slot table
@@ -370,7 +378,8 @@ void build_core(mol_handle* h, const mol_desc* d, KCore<T>& kc, std::vector<int3
   int np = 0;
   for (int k = 0; k < d->nslots; ++k) {
     const mol_slot& sl = d->slots_h[k];
-    CoreSlot& cs = kc.slots[k];
+    CoreSlot dummy;
+    CoreSlot& cs = k < CORE_MAX_SLOTS ? kc.slots[k] : dummy;
     cs.kind = sl.kind; cs.field = sl.field; cs.axis = sl.axis; cs.axis2 = sl.axis2; cs.prog = sl.prog;
     cs.eps = sl.eps; cs.inv_dx = sl.inv_dx;
     int ntab = 0;
@@ -383,11 +392,12 @@ void build_core(mol_handle* h, const mol_desc* d, KCore<T>& kc, std::vector<int3
     }
     for (int q = 0; q < ntab; ++q) {
       const mol_optab& tb = d->tabs_h[sl.tab[q]];
-      if (np + tb.len > CORE_MAX_POOL) return all_generic("interior weights exceed the kernel-argument block");
+      if (np + tb.len > CORE_MAX_POOL) fits = false;
       cs.len[q] = tb.len;
       cs.off[q] = tb.off;
       cs.w[q] = np;
-      for (int j = 0; j < tb.len; ++j) kc.pool[np++] = d->pool_h[tb.w + j];
+      if (fits)
+        for (int j = 0; j < tb.len; ++j) kc.pool[np++] = d->pool_h[tb.w + j];
       const int ax = (sl.kind == MOL_SLOT_MIXED && q == 1) ? sl.axis2 : sl.axis;
       if (kc.periodic[ax]) continue;
       const int n = kc.n[ax];
@@ -438,7 +448,32 @@ void build_core(mol_handle* h, const mol_desc* d, KCore<T>& kc, std::vector<int3
                           tz >= ct_lo[2] && tz <= ct_hi[2];
         if (!core) gen_tiles.push_back(tx + ka.ntile[0] * (ty + ka.ntile[1] * tz));
       }
-  h->core_reason.clear();
+  h->core_reason = fits ? "" : "stencil IR larger than the interpreter's kernel-argument block";
+  h->core_interp_ok = fits;
+}
+
+template <typename T>
+JitGeom geom_of(const KCore<T>& kc) {
+  JitGeom g{};
+  for (int a = 0; a < 3; ++a) {
+    g.n[a] = kc.n[a];
+    g.periodic[a] = kc.periodic[a];
+    g.lo[a] = kc.lo[a];
+    g.hi[a] = kc.hi[a];
+    g.tile[a] = kc.tile[a];
+    g.halo[a] = kc.halo[a];
+    g.ext[a] = kc.ext[a];
+    g.ct_lo[a] = kc.ct_lo[a];
+    g.ct_n[a] = kc.ct_n[a];
+  }
+  g.vol = kc.vol;
+  g.box = kc.box;
+  return g;
+}
+
+bool jit_enabled() {
+  const char* e = std::getenv("MOL_JIT");
+  return !(e && e[0] == '0');
 }
 
 int setup_paths(mol_handle* h, const mol_desc* d) {
@@ -447,7 +482,29 @@ int setup_paths(mol_handle* h, const mol_desc* d) {
   build_core<double>(h, d, h->kc64, t64, n64);
   build_core<float>(h, d, h->kc32, t32, n32);
   h->use_weno = false;
-  if (n64 > 0 && t64.empty() && weno_plan(d, h->wplan)) {
+  h->jit_fn = nullptr;
+  h->jit_reason.clear();
+  if (n64 > 0 && jit_enabled()) {
+    std::string err;
+    const std::string src = jit_core_source(d, d->dtype == MOL_F64 ? geom_of(h->kc64) : geom_of(h->kc32), err);
+    if (src.empty()) h->jit_reason = err;
+    else if (jit_kernel(src, h->device, &h->jit_fn, err)) {
+      h->jit_fn = nullptr;
+      h->jit_reason = err;
+    }
+  } else if (n64 > 0) {
+    h->jit_reason = "code generation disabled (MOL_JIT=0)";
+  }
+  if (n64 > 0 && !h->jit_fn && !h->core_interp_ok) {
+    // neither core kernel can run this system: every tile goes to the generic kernel
+    const KArgs<double>& ka = h->ka64;
+    t64.clear();
+    for (int i = 0; i < ka.ntile[0] * ka.ntile[1] * ka.ntile[2]; ++i) t64.push_back(i);
+    n64 = 0;
+  }
+  const char* wf = std::getenv("MOL_WENO2D");
+  const bool want_weno = h->jit_fn == nullptr || (wf && wf[0] == '1');
+  if (want_weno && n64 > 0 && t64.empty() && h->core_interp_ok && weno_plan(d, h->wplan)) {
     // every tile is a core tile: retile 64 x 32 for the WENO5 kernel
     h->kcw64 = h->kc64;
     h->kcw32 = h->kc32;
@@ -657,7 +714,8 @@ int mol_set_tile(mol_handle* h, int32_t tx, int32_t ty, int32_t tz) {
 int mol_get_plan(const mol_handle* h, int32_t* info, int32_t ninfo) {
   if (!h || !info) return MOL_ERR_INVALID;
   int32_t v[8] = {h->ka64.tile[0], h->ka64.tile[1], h->ka64.tile[2], h->ncore, h->nbound,
-                  h->use_weno ? h->nweno : 0, 0, 0};
+                  h->use_weno ? h->nweno : 0, h->jit_fn != nullptr && h->ncore > 0 && !h->use_weno ? 1 : 0,
+                  h->core_interp_ok ? 1 : 0};
   for (int i = 0; i < ninfo && i < 8; ++i) info[i] = v[i];
   return 0;
 }
@@ -760,6 +818,38 @@ const char* mol_last_error(const mol_handle* h) {
 }
 
 void mol_destroy(mol_handle* h) { free_handle(h); }
+
+int64_t mol_codegen(const mol_desc* d, const int32_t* tile, char* buf, int64_t buflen, int32_t compile) {
+  std::string msg;
+  int rc = validate(d, msg);
+  if (rc) return fail(nullptr, rc, msg);
+  mol_handle tmp;
+  tmp.desc.assign(d);
+  d = &tmp.desc.d;
+  if (tile)
+    for (int a = 0; a < 3; ++a) tmp.tile_req[a] = tile[a];
+  tmp.dtype = d->dtype;
+  fill_kargs<double>(&tmp, d, tmp.ka64);
+  fill_kargs<float>(&tmp, d, tmp.ka32);
+  std::vector<int32_t> gen;
+  int ncore = 0;
+  build_core<double>(&tmp, d, tmp.kc64, gen, ncore);
+  if (ncore == 0) return fail(nullptr, MOL_ERR_UNSUPPORTED, "no core tiles: " + tmp.core_reason);
+  std::string err;
+  const std::string src = jit_core_source(d, geom_of(tmp.kc64), err);
+  if (src.empty()) return fail(nullptr, MOL_ERR_UNSUPPORTED, err);
+  if (compile) {
+    std::vector<char> code;
+    std::string log;
+    if (jit_compile(src, code, log)) return fail(nullptr, MOL_ERR_HIP, "hiprtc: " + log);
+  }
+  if (buf && buflen > 0) {
+    const size_t n = std::min<size_t>(src.size(), (size_t)buflen - 1);
+    std::memcpy(buf, src.data(), n);
+    buf[n] = '\0';
+  }
+  return (int64_t)src.size();
+}
 
 int mol_device_info(char* buf, int buflen) {
   int dev = 0;

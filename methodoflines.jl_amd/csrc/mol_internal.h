@@ -16,6 +16,24 @@ struct WenoPlan {
   int fx[2], fy[2];
 };
 
+// Geometry of the core-tile region handed to the code generator.
+struct JitGeom {
+  int n[3], periodic[3], lo[3], hi[3], tile[3], halo[3], ext[3], vol, ct_lo[3], ct_n[3];
+  long long box;
+};
+
+// Pointer arguments of the generated kernel (layout identical to JPtrs of mol_jit_prelude.h).
+template <typename T>
+struct JitPtrs {
+  const double* coord[3];
+  const T* aux[MOL_MAX_AUX];
+};
+
+// Code generation and in-process compilation of the core-tile kernel (mol_jit.cpp).
+std::string jit_core_source(const mol_desc* d, const JitGeom& g, std::string& err);
+int jit_compile(const std::string& src, std::vector<char>& code, std::string& log);
+int jit_kernel(const std::string& src, int device, hipFunction_t* fn, std::string& err);
+
 // What one right-hand-side launch needs: the core-tile kernel arguments and the list of tiles
 // left to the generic kernel.
 template <typename T>
@@ -28,6 +46,8 @@ struct Launch {
   const KCore<T>* kcw = nullptr;     // WENO fast-path arguments (nullptr: not used)
   const WenoPlan* plan = nullptr;
   int nweno = 0;
+  hipFunction_t jit = nullptr;       // generated core-tile kernel (nullptr: interpreter)
+  JitPtrs<T> jp{};
 };
 
 template <typename T>
@@ -100,6 +120,9 @@ struct mol_handle {
   int32_t* d_tiles = nullptr;             // tiles handled by the generic kernel
   int ncore = 0, nbound = 0;
   std::string core_reason;                // why the core path is off, if it is
+  bool core_interp_ok = false;            // the stencil IR fits the interpreter's argument block
+  hipFunction_t jit_fn = nullptr;         // generated core-tile kernel
+  std::string jit_reason;                 // why it is not used, if it is not
   bool use_weno = false;                  // WENO5 fast path
   mol::WenoPlan wplan{};
   mol::KCore<double> kcw64{};

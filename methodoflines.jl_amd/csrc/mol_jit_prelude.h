@@ -1,0 +1,199 @@
+// Device-side skeleton of the generated right-hand-side kernels.
+//
+// This header is not compiled with the library: its text is embedded into libmolgpu.so and
+// handed to hiprtc together with the source that mol_jit.cpp generates for one PDE system.
+// The generated source defines MOL_T (double or float) before including this header, then a
+// struct `Geo` with the tile geometry as constants and the per-node right-hand side
+// `Geo::node`, whose operator weights, constants and programs are folded into straight-line
+// code.  The tile kernel below stages the tile plus halo into LDS with periodic wrap and runs
+// the node function with the fused Runge-Kutta stage epilogue.
+#pragma once
+
+typedef MOL_T T;
+typedef long long i64;
+
+// Layout identical to mol::Epi<T> of the library (passed by value as a kernel argument).
+struct Epi {
+  int mode;
+  int subA;
+  T* out;
+  const T* A;
+  const T* B;
+  T* acc;
+  T* comp;
+  double ca, cb, cf, acc_keep, acc_f;
+  const double* tnow;
+  double tofs;
+  const T* p;
+};
+
+// Node coordinates and auxiliary grid arrays (layout identical to mol::JitPtrs<T>).
+struct JPtrs {
+  const double* coord[3];
+  const T* aux[8];
+};
+
+// Fused epilogue of one Runge-Kutta stage:
+//   mode 0:  out = f
+//   mode 1:  out = ca*A + cb*B + cf*f ;  acc = acc_keep*acc + acc_f*f  (if acc)
+//   mode 2:  compensated update  out = A (+) [cb*(B - subA*A) + cf*f]  with Kahan term comp
+__device__ __forceinline__ void mol_epilogue(const Epi& ep, i64 idx, T f) {
+  if (ep.mode == 0) {
+    ep.out[idx] = f;
+    return;
+  }
+  if (ep.mode == 1) {
+    T v = (T)ep.cf * f;
+    if (ep.A) v += (T)ep.ca * ep.A[idx];
+    if (ep.B) v += (T)ep.cb * ep.B[idx];
+    if (ep.acc) {
+      T a = (T)ep.acc_f * f;
+      if (ep.acc_keep != 0.0) a += (T)ep.acc_keep * ep.acc[idx];
+      ep.acc[idx] = a;
+    }
+    ep.out[idx] = v;
+    return;
+  }
+  const T a = ep.A[idx];
+  T incr = (T)ep.cf * f;
+  if (ep.B) incr += (T)ep.cb * (ep.subA ? (ep.B[idx] - a) : ep.B[idx]);
+  const T y = incr - ep.comp[idx];
+  const T s = a + y;
+  ep.comp[idx] = (s - a) - y;
+  ep.out[idx] = s;
+}
+
+// Single periodic wrap of a node index onto the ring 2..n (node 1 is the image of node n).
+__device__ __forceinline__ int mol_wrap(int k, int n) {
+  return k <= 1 ? k + (n - 1) : (k > n ? k - (n - 1) : k);
+}
+
+__device__ __forceinline__ T mol_sign(T a) { return a > T(0) ? T(1) : (a < T(0) ? T(-1) : T(0)); }
+__device__ __forceinline__ T mol_heaviside(T a) { return a > T(0) ? T(1) : (a < T(0) ? T(0) : T(0.5)); }
+
+// x^e for |e| > 3 by repeated squaring (small exponents are expanded by the generator).
+__device__ __noinline__ T mol_powi(T x, int e) {
+  const bool neg = e < 0;
+  unsigned m = neg ? (unsigned)(-e) : (unsigned)e;
+  T r = T(1), base = x;
+  while (m) {
+    if (m & 1u) r *= base;
+    base *= base;
+    m >>= 1;
+  }
+  return neg ? T(1) / r : r;
+}
+
+// Jiang-Shu WENO5 node derivative (u^-_{i+1/2} - u^+_{i-1/2}) / dx from the five values
+// u(i-2) .. u(i+2).  The candidate reconstructions are written as u(i) + difference/6 so u(i)
+// cancels, and the normalised nonlinear weights gamma_k/q_k / sum_j gamma_j/q_j are cleared
+// to gamma_k P_k / sum_j gamma_j P_j (P_k = product of the other two q_j): one division.
+__device__ __forceinline__ T mol_weno5(T um2, T um1, T u0, T up1, T up2, T eps, T inv_dx) {
+  const T dmm = um1 - um2, dm = u0 - um1, dp = up1 - u0, dpp = up2 - up1;
+  const T D2m = dm - dmm, D20 = dp - dm, D2p = dpp - dp;
+  const T a1 = T(-2) * dp + D2p;
+  const T a2 = dm + dp;
+  const T a3 = T(2) * dm + D2m;
+  const T b1 = T(13.0 / 12.0) * D2p * D2p + eps + T(0.25) * a1 * a1;
+  const T b2 = T(13.0 / 12.0) * D20 * D20 + eps + T(0.25) * a2 * a2;
+  const T b3 = T(13.0 / 12.0) * D2m * D2m + eps + T(0.25) * a3 * a3;
+  const T q1 = b1 * b1, q2 = b2 * b2, q3 = b3 * b3;
+  const T P1 = q2 * q3, P2 = q1 * q3, P3 = q1 * q2;
+  const T A1 = T(0.3) * P1, A2 = T(0.6) * P2, A3 = T(0.1) * P3;
+  const T B1 = T(0.1) * P1, B3 = T(0.3) * P3;
+  const T Dp = A1 + A2 + A3;
+  const T Dm = B1 + A2 + B3;
+  const T ep1 = T(4) * dp - dpp, ep2 = T(2) * dp + dm, ep3 = T(5) * dm - T(2) * dmm;
+  const T em1 = T(-5) * dp + T(2) * dpp, em2 = T(-2) * dm - dp, em3 = T(-4) * dm + dmm;
+  const T Np = A1 * ep1 + A2 * ep2 + A3 * ep3;
+  const T Nm = B1 * em1 + A2 * em2 + B3 * em3;
+  return (Np * Dm - Nm * Dp) / (Dp * Dm) * (inv_dx * T(1.0 / 6.0));
+}
+
+// The same derivative in the reference's normalised-weight form (weno_f_uniform,
+// src/discretization/schemes/WENO/WENO.jl:6-57).  Used in FP32, where the products of four
+// squared smoothness indicators of the cleared form (~eps^8) underflow.
+__device__ __forceinline__ T mol_weno5_ref(T um2, T um1, T u0, T up1, T up2, T eps, T inv_dx) {
+  const T c1 = u0 - T(2) * up1 + up2, c2 = T(3) * u0 - T(4) * up1 + up2;
+  const T c3 = um1 - T(2) * u0 + up1, c4 = um1 - up1;
+  const T c5 = um2 - T(2) * um1 + u0, c6 = um2 - T(4) * um1 + T(3) * u0;
+  const T b1 = T(13) * c1 * c1 / T(12) + c2 * c2 / T(4);
+  const T b2 = T(13) * c3 * c3 / T(12) + c4 * c4 / T(4);
+  const T b3 = T(13) * c5 * c5 / T(12) + c6 * c6 / T(4);
+  const T q1 = (eps + b1) * (eps + b1), q2 = (eps + b2) * (eps + b2), q3 = (eps + b3) * (eps + b3);
+  const T om1 = T(0.1) / q1, om2 = T(0.6) / q2, om3 = T(0.3) / q3;
+  const T op1 = T(0.3) / q1, op2 = T(0.6) / q2, op3 = T(0.1) / q3;
+  const T wmd = om1 + om2 + om3, wpd = op1 + op2 + op3;
+  const T hm1 = (T(11) * u0 - T(7) * up1 + T(2) * up2) / T(6);
+  const T hm2 = (T(5) * u0 - up1 + T(2) * um1) / T(6);
+  const T hm3 = (T(2) * u0 + T(5) * um1 - um2) / T(6);
+  const T hp1 = (T(2) * u0 + T(5) * up1 - up2) / T(6);
+  const T hp2 = (T(5) * u0 + T(2) * up1 - um1) / T(6);
+  const T hp3 = (T(11) * u0 - T(7) * um1 + T(2) * um2) / T(6);
+  const T hp = (op1 / wpd) * hp1 + (op2 / wpd) * hp2 + (op3 / wpd) * hp3;
+  const T hm = (om1 / wmd) * hm1 + (om2 / wmd) * hm2 + (om3 / wmd) * hm3;
+  return (hp - hm) * inv_dx;
+}
+
+// Work-groups are dealt to the 8 XCDs round-robin; remap so that each XCD runs one contiguous
+// range of tiles (neighbouring tiles share halo lines in that XCD's L2).
+template <int NBLK>
+__device__ __forceinline__ int mol_xcd_tile(int b) {
+  constexpr int NX = 8;
+  if (NBLK < 2 * NX) return b;
+  constexpr int q = NBLK / NX, rem = NBLK % NX;
+  const int x = b % NX, i = b / NX;
+  return x * q + (x < rem ? x : rem) + i;
+}
+
+// One tile of the core region: every tile + halo position is an unknown or its periodic image.
+template <class G>
+__device__ __forceinline__ void mol_core_tile(const T* __restrict__ u, const Epi& ep, const JPtrs& jp) {
+  __shared__ T L[G::NARR * G::VOL];
+  int b = mol_xcd_tile<G::NBLK>((int)blockIdx.x);
+  const int tx = G::CTLO0 + b % G::CTN0;
+  b /= G::CTN0;
+  const int ty = G::CTLO1 + b % G::CTN1;
+  const int tz = G::CTLO2 + b / G::CTN1;
+  const int o0 = G::LO0 + tx * G::TX, o1 = G::LO1 + ty * G::TY, o2 = G::LO2 + tz * G::TZ;
+  const double t = (ep.tnow ? ep.tnow[0] : 0.0) + ep.tofs;
+  constexpr int BX = G::HI0 - G::LO0 + 1, BY = G::HI1 - G::LO1 + 1;
+  constexpr int EXY = G::EX * G::EY;
+  for (int q = threadIdx.x; q < G::VOL; q += G::NTHR) {
+    const int lz = q / EXY;
+    const int r = q - lz * EXY;
+    const int ly = r / G::EX;
+    const int lx = r - ly * G::EX;
+    int p0 = o0 + lx - G::HX, p1 = o1 + ly - G::HY, p2 = o2 + lz - G::HZ;
+    if (G::PER0) p0 = mol_wrap(p0, G::N0);
+    if (G::PER1) p1 = mol_wrap(p1, G::N1);
+    if (G::PER2) p2 = mol_wrap(p2, G::N2);
+    const i64 li = (i64)(p0 - G::LO0) + (i64)BX * ((p1 - G::LO1) + (i64)BY * (p2 - G::LO2));
+#pragma unroll
+    for (int f = 0; f < G::NF; ++f) L[f * G::VOL + q] = u[f * G::BOX + li];
+    if (G::NAUX) {
+      const i64 gi = (i64)(p0 - 1) + (i64)G::N0 * ((p1 - 1) + (i64)G::N1 * (p2 - 1));
+#pragma unroll
+      for (int a = 0; a < G::NAUX; ++a) L[(G::NF + a) * G::VOL + q] = jp.aux[a][gi];
+    }
+  }
+  __syncthreads();
+  constexpr int TXY = G::TX * G::TY, TVOL = TXY * G::TZ;
+#pragma unroll
+  for (int r = 0; r < (TVOL + G::NTHR - 1) / G::NTHR; ++r) {
+    const int q = (int)threadIdx.x + r * G::NTHR;
+    if (TVOL % G::NTHR != 0 && q >= TVOL) break;
+    const int lz = q / TXY;
+    const int rr = q - lz * TXY;
+    const int ly = rr / G::TX;
+    const int lx = rr - ly * G::TX;
+    const int I0 = o0 + lx, I1 = o1 + ly, I2 = o2 + lz;
+    if (G::RAGGED && (I0 > G::HI0 || I1 > G::HI1 || I2 > G::HI2)) continue;
+    const int base = (lx + G::HX) + G::EX * (ly + G::HY) + EXY * (lz + G::HZ);
+    T out[G::NF];
+    G::node(L, base, I0, I1, I2, t, ep.p, jp, out);
+    const i64 li = (i64)(I0 - G::LO0) + (i64)BX * ((I1 - G::LO1) + (i64)BY * (I2 - G::LO2));
+#pragma unroll
+    for (int f = 0; f < G::NF; ++f) mol_epilogue(ep, f * G::BOX + li, out[f]);
+  }
+}

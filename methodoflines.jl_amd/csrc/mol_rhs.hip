@@ -168,8 +168,16 @@ template <typename T>
 int launch_rhs(const Launch<T>& L, const T* u, const Epi<T>& ep, hipStream_t stream) {
   if (L.kcw) return launch_weno2d<T>(*L.kcw, *L.plan, L.nweno, u, ep, stream);
   const size_t lds = (size_t)(L.ka->nfields + L.ka->naux) * L.ka->vol * sizeof(T);
-  if (L.ncore > 0)
+  if (L.ncore > 0 && L.jit) {
+    const T* uu = u;
+    Epi<T> e = ep;
+    JitPtrs<T> jp = L.jp;
+    void* args[] = {(void*)&uu, (void*)&e, (void*)&jp};
+    if (hipModuleLaunchKernel(L.jit, L.ncore, 1, 1, 256, 1, 1, 0, stream, args, nullptr) != hipSuccess)
+      return MOL_ERR_HIP;
+  } else if (L.ncore > 0) {
     hipLaunchKernelGGL(rhs_core_kernel<T>, dim3(L.ncore), dim3(256), lds, stream, *L.kc, u, ep);
+  }
   if (L.nbound > 0)
     hipLaunchKernelGGL(rhs_generic_kernel<T>, dim3(L.nbound), dim3(256), lds, stream, L.dka, L.tiles, u, ep);
   return hipGetLastError() == hipSuccess ? 0 : MOL_ERR_HIP;

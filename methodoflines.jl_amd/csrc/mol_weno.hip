@@ -236,6 +236,9 @@ int program_depth(const mol_desc* d, int prog);
 bool weno_plan(const mol_desc* d, WenoPlan& plan) {
   for (int f = 0; f < d->nfields; ++f)
     if (program_depth(d, d->rhs_prog[f]) > 4) return false;
+  // FP64 only: the cleared weight denominators (products of squared smoothness indicators,
+  // ~eps^8) underflow in FP32
+  if (d->dtype != MOL_F64) return false;
   if (d->ndim != 2 || d->naux != 0 || d->nslots < 1 || d->nslots > 4 || d->nfields > 2) return false;
   if (!d->periodic[0] || !d->periodic[1]) return false;
   if (d->hi[0][0] - d->lo[0][0] + 1 < WT_X + WHALO || d->hi[0][1] - d->lo[0][1] + 1 < WT_Y + WHALO) return false;

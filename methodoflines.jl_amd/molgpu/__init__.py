@@ -8,13 +8,13 @@ from .pdesys import (PDESystem, Differential, Interval, Eq, ifelse, parameters, 
 from .discretization import (MOLFiniteDifference, UpwindScheme, WENOScheme, Lowered, lower,
                              uniform_grid)
 from .fornberg import calculate_weights
-from .problem import (ODEProblem, ODESolution, RK4, SSPRK33, discretize, symbolic_discretize,
-                      solve)
+from .problem import (ODEProblem, ODESolution, RK4, SSPRK33, codegen, discretize,
+                      symbolic_discretize, solve)
 from . import native
 
 __all__ = [
     "PDESystem", "Differential", "Interval", "Eq", "ifelse", "parameters", "variables",
     "grid_field", "GridField", "MOLFiniteDifference", "UpwindScheme", "WENOScheme", "Lowered",
     "lower", "uniform_grid", "calculate_weights", "ODEProblem", "ODESolution", "RK4", "SSPRK33",
-    "discretize", "symbolic_discretize", "solve", "native",
+    "codegen", "discretize", "symbolic_discretize", "solve", "native",
 ]

@@ -123,17 +123,22 @@ class ODEProblem:
         info = (C.c_int32 * 8)()
         h.lib.mol_get_plan(h.h, info, 8)
         return {"tile": (info[0], info[1], info[2]), "core_tiles": info[3], "generic_tiles": info[4],
-                "weno_tiles": info[5]}
+                "weno_tiles": info[5], "generated": info[6], "interp_core_ok": info[7]}
 
     def aux_tensor(self, name: str):
         self.handle
         return self._aux[name]
 
     def device_params(self):
+        """Device copy of the problem's scalar parameters (cached while they are unchanged)."""
         torch = _torch()
         if len(self.p) == 0:
             return None
-        return torch.tensor(self.p, dtype=self.torch_dtype(), device="cuda")
+        key = (self.p.tobytes(), self.lowered.dtype)
+        if getattr(self, "_p_key", None) != key:
+            self._p_dev = torch.tensor(self.p, dtype=self.torch_dtype(), device="cuda")
+            self._p_key = key
+        return self._p_dev
 
     # ---------------------------------------------------------------- f(du, u, p, t)
     def f(self, du, u, p, t: float) -> None:
@@ -200,6 +205,20 @@ class ODEProblem:
         for f, sl in enumerate(low.field_slices()):
             out.append(u[sl].reshape(low.box_shape(f), order="F"))
         return out
+
+
+def codegen(low: Lowered, compile: bool = False, tile=None) -> str:
+    """Source of the generated core-tile kernel for a lowered system (no GPU needed);
+    ``compile=True`` also compiles it with hiprtc for gfx950."""
+    lib = native.load_library()
+    desc, keep = native.pack_descriptor(low)
+    tl = (C.c_int32 * 3)(*tile) if tile is not None else None
+    n = lib.mol_codegen(C.byref(desc), tl, None, 0, 0)
+    native.check(lib, n if n < 0 else 0, None)
+    buf = C.create_string_buffer(int(n) + 1)
+    n = lib.mol_codegen(C.byref(desc), tl, buf, len(buf), 1 if compile else 0)
+    native.check(lib, n if n < 0 else 0, None)
+    return buf.value.decode()
 
 
 def symbolic_discretize(pdesys: PDESystem, disc: MOLFiniteDifference) -> Lowered:

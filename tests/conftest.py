@@ -32,6 +32,14 @@ def pytest_collection_modifyitems(config, items):
             it.add_marker(skip)
 
 
+@pytest.fixture(params=["generated", "interpreter"])
+def rhs_path(request, monkeypatch):
+    """Run a right-hand-side test on both device paths of the core tiles: the kernel generated
+    from the stencil IR (compiled at run time) and the bytecode interpreter kernel."""
+    monkeypatch.setenv("MOL_JIT", "1" if request.param == "generated" else "0")
+    return request.param
+
+
 def assert_parity(got, ref, scale=None, rtol=1e-12, what=""):
     """FP64 parity: norm-wise relative error <= rtol, and max error <= rtol * scale, where
     scale bounds the magnitude of the summed terms (cancellation-aware)."""

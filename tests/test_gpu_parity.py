@@ -19,9 +19,20 @@ def _torch():
     return torch
 
 
-def gpu_rhs(prob, u, t=0.0):
+SMALL_TILES = {1: (32, 1, 1), 2: (16, 8, 1), 3: (8, 4, 4)}
+
+
+def gpu_rhs(prob, u, t=0.0, path=None):
+    """RHS on the GPU.  With `path`, small tiles make sure that core tiles exist, and the plan
+    is checked to use the requested core-tile kernel."""
     torch = _torch()
-    print("  gpu rhs: %s unknowns=%d t=%g" % (prob.lowered.fields, prob.lowered.n_unknowns, t), flush=True)
+    print("  gpu rhs: %s unknowns=%d t=%g path=%s" % (prob.lowered.fields, prob.lowered.n_unknowns, t, path),
+          flush=True)
+    if path is not None:
+        prob.set_tile(*SMALL_TILES[prob.lowered.ndim])
+        plan = prob.plan()
+        if plan["core_tiles"] > 0:
+            assert plan["generated"] == (1 if path == "generated" else 0), plan
     dt = prob.torch_dtype()
     ug = torch.tensor(np.asarray(u), dtype=dt, device="cuda")
     du = torch.empty_like(ug)
@@ -37,7 +48,7 @@ def lap_scale(u, coef, dxs):
 # ---------------------------------------------------------------- heat (config 1)
 @pytest.mark.parametrize("bc", ["dirichlet", "neumann", "robin"])
 @pytest.mark.parametrize("p", [2, 4])
-def test_heat1d_rhs(bc, p):
+def test_heat1d_rhs(bc, p, rhs_path):
     bcs = {"dirichlet": ((1, 0, 0), (1, 0, 0)), "neumann": ((0, 1, 0), (0, 1, 0)),
            "robin": ((2.0, 0.5, 0.3), (1.5, -0.25, -0.2))}[bc]
     n = 101
@@ -45,7 +56,7 @@ def test_heat1d_rhs(bc, p):
     prob = molgpu.discretize(sys_, disc)
     rng = np.random.default_rng(1)
     u = rng.uniform(-1, 1, prob.lowered.n_unknowns)
-    got = gpu_rhs(prob, u)
+    got = gpu_rhs(prob, u, path=rhs_path)
     if p == 2:
         ref = orc.diffusion1d_rhs(n, 0.8, bcs[0], bcs[1], u)
     else:
@@ -84,12 +95,12 @@ def test_brusselator_matches_generated_code_on_gpu():
 
 
 @pytest.mark.parametrize("n,t", [(33, 0.0), (65, 1.2), (513, 1.2)])
-def test_brusselator_rhs(n, t):
+def test_brusselator_rhs(n, t, rhs_path):
     sys_, disc = systems.brusselator2d(n=n)
     prob = molgpu.discretize(sys_, disc)
     rng = np.random.default_rng(n)
     u = rng.uniform(0.5, 1.5, prob.lowered.n_unknowns)
-    got = gpu_rhs(prob, u, t)
+    got = gpu_rhs(prob, u, t, path=rhs_path)
     ref = orc.bruss2d_rhs(n, u, t, threads=8)
     dx = 1.0 / (n - 1)
     assert_parity(got, ref, lap_scale(u, 10.0, [dx, dx]), what="bruss n=%d" % n)
@@ -116,12 +127,12 @@ def test_brusselator_initial_condition_and_rk4():
 
 # ---------------------------------------------------------------- Burgers WENO5 (config 3)
 @pytest.mark.parametrize("n", [33, 129])
-def test_burgers_weno_rhs(n):
+def test_burgers_weno_rhs(n, rhs_path):
     sys_, disc = systems.burgers2d(n=n)
     prob = molgpu.discretize(sys_, disc)
     rng = np.random.default_rng(n)
     for u in (prob.u0, rng.uniform(0.2, 1.2, prob.lowered.n_unknowns)):
-        got = gpu_rhs(prob, u)
+        got = gpu_rhs(prob, u, path=rhs_path)
         ref = orc.burgers2d_rhs(n, u, threads=8)
         assert_parity(got, ref, np.max(np.abs(u)) ** 2 * 4 * (n - 1) + 1, what="burgers n=%d" % n)
 
@@ -163,26 +174,26 @@ def test_burgers_fp32_kahan_error_vs_fp64_oracle():
     assert err < 2e-3
 
 
-def test_weno_dirichlet_extrapolated_nodes():
+def test_weno_dirichlet_extrapolated_nodes(rhs_path):
     n, L, c = 41, 2.0, 1.0
     sys_, disc = systems.weno1d_dirichlet(n, L, c, 0.1, 0.05)
     prob = molgpu.discretize(sys_, disc)
     rng = np.random.default_rng(4)
     u = rng.uniform(0, 1, prob.lowered.n_unknowns)
-    got = gpu_rhs(prob, u)
+    got = gpu_rhs(prob, u, path=rhs_path)
     ref = orc.weno1d_dirichlet_rhs(n, L, c, 1e-6, 0.1, 0.05, u)
     assert_parity(got, ref, 4 * (n - 1) / L + 1, what="weno dirichlet")
 
 
 # ---------------------------------------------------------------- diffusion-reaction (config 4 instance)
 @pytest.mark.parametrize("bct", [(0, 0, 0, 0), (1, 1, 1, 1), (0, 1, 1, 0), (1, 0, 0, 1)])
-def test_diffreact2d_rhs(bct):
+def test_diffreact2d_rhs(bct, rhs_path):
     n = 48
     sys_, disc = systems.diffreact2d(n, 0.05, 2.5, bct)
     prob = molgpu.discretize(sys_, disc)
     rng = np.random.default_rng(sum(bct))
     u = rng.uniform(0, 1, prob.lowered.n_unknowns)
-    got = gpu_rhs(prob, u)
+    got = gpu_rhs(prob, u, path=rhs_path)
     ref = orc.diffreact2d_rhs(n, 0.05, 2.5, bct, u)
     dx = 1.0 / (n - 1)
     assert_parity(got, ref, lap_scale(u, 0.05, [dx, dx]), what="diffreact %s" % (bct,))
@@ -190,7 +201,7 @@ def test_diffreact2d_rhs(bct):
 
 # ---------------------------------------------------------------- nonlinear diffusion (config 5)
 @pytest.mark.parametrize("n", [12, 34])
-def test_nonlindiff3d_rhs(n):
+def test_nonlindiff3d_rhs(n, rhs_path):
     sys_, disc = systems.nonlindiff3d(n=n)
     prob = molgpu.discretize(sys_, disc)
     d0 = 0.5 + 0.5 * np.abs(systems.gaussian_field3d(n, 1))
@@ -199,7 +210,7 @@ def test_nonlindiff3d_rhs(n):
     prob.set_grid_field("s", src)
     rng = np.random.default_rng(n)
     u = rng.uniform(-1, 1, prob.lowered.n_unknowns)
-    got = gpu_rhs(prob, u)
+    got = gpu_rhs(prob, u, path=rhs_path)
     ref = orc.nonlindiff3d_rhs(n, d0.reshape(-1, order="F"), src.reshape(-1, order="F"), u, threads=8)
     dx = 1.0 / (n - 1)
     assert_parity(got, ref, lap_scale(u, 2.0 * np.max(d0), [dx] * 3), what="nonlindiff3d n=%d" % n)
@@ -207,38 +218,38 @@ def test_nonlindiff3d_rhs(n):
 
 # ---------------------------------------------------------------- other operator families
 @pytest.mark.parametrize("order", [1, 2])
-def test_upwind_sign_selection(order):
+def test_upwind_sign_selection(order, rhs_path):
     n = 61
     sys_, disc = systems.upwind1d(n, 0.1, 1.0, 0.3, -0.2, order)
     prob = molgpu.discretize(sys_, disc)
     rng = np.random.default_rng(order)
     u = rng.uniform(-1, 1, prob.lowered.n_unknowns)
-    got = gpu_rhs(prob, u)
+    got = gpu_rhs(prob, u, path=rhs_path)
     ref = orc.upwind1d_rhs(n, order, 0.1, 1.0, 0.3, -0.2, u)
     assert_parity(got, ref, 4 * 1.1 * (n - 1) * 2, what="upwind order %d" % order)
 
 
 @pytest.mark.parametrize("p", [2, 4])
-def test_mixed_derivative(p):
+def test_mixed_derivative(p, rhs_path):
     n = 25
     sys_, disc = systems.mixed2d(n, 0.7, p)
     prob = molgpu.discretize(sys_, disc)
     rng = np.random.default_rng(p)
     u = rng.uniform(-1, 1, prob.lowered.n_unknowns)
-    got = gpu_rhs(prob, u)
+    got = gpu_rhs(prob, u, path=rhs_path)
     ref = orc.mixed2d_rhs(n, 0.7, u, p=p)
     dx = 1.0 / (n - 1)
     assert_parity(got, ref, lap_scale(u, 4.0, [dx, dx]), what="mixed p=%d" % p)
 
 
 @pytest.mark.parametrize("p", [2, 4])
-def test_spherical_laplacian(p):
+def test_spherical_laplacian(p, rhs_path):
     n = 41
     sys_, disc = systems.spherical1d(n, 0.25, p)
     prob = molgpu.discretize(sys_, disc)
     rng = np.random.default_rng(p + 10)
     u = rng.uniform(0, 1, prob.lowered.n_unknowns)
-    got = gpu_rhs(prob, u)
+    got = gpu_rhs(prob, u, path=rhs_path)
     ref = orc.spherical1d_rhs(n, 0.25, u, p=p)
     dx = 1.0 / (n - 1)
     assert_parity(got, ref, lap_scale(u, 8.0, [dx]) * 10, what="spherical p=%d" % p)
@@ -246,16 +257,30 @@ def test_spherical_laplacian(p):
 
 @pytest.mark.parametrize("periodic", [True, False])
 @pytest.mark.parametrize("p", [2, 4])
-def test_nonlinear_laplacian_1d(periodic, p):
+def test_nonlinear_laplacian_1d(periodic, p, rhs_path):
     n = 41
     sys_, disc = systems.nonlinlap1d(n, periodic, 0.2, -0.4, p)
     prob = molgpu.discretize(sys_, disc)
     rng = np.random.default_rng(p)
     u = rng.uniform(-1, 1, prob.lowered.n_unknowns)
-    got = gpu_rhs(prob, u)
+    got = gpu_rhs(prob, u, path=rhs_path)
     ref = orc.nonlinlap1d_rhs(n, u, periodic, 0.2, -0.4, p=p)
     dx = 1.0 / (n - 1)
     assert_parity(got, ref, lap_scale(u, 8.0, [dx]) * 4, what="nonlinlap periodic=%s p=%d" % (periodic, p))
+
+
+def test_weno2d_specialised_kernel_matches_oracle(monkeypatch):
+    """The hand-written 2D WENO5 kernel (64 x 32 tiles, shared differences) on request."""
+    monkeypatch.setenv("MOL_WENO2D", "1")
+    n = 129
+    sys_, disc = systems.burgers2d(n=n)
+    prob = molgpu.discretize(sys_, disc)
+    assert prob.plan()["weno_tiles"] > 0
+    rng = np.random.default_rng(5)
+    u = rng.uniform(0.2, 1.2, prob.lowered.n_unknowns)
+    got = gpu_rhs(prob, u)
+    ref = orc.burgers2d_rhs(n, u, threads=8)
+    assert_parity(got, ref, np.max(np.abs(u)) ** 2 * 4 * (n - 1) + 1, what="weno2d n=%d" % n)
 
 
 def test_host_pointer_entry_point():
