@@ -244,8 +244,10 @@ void mol_destroy(mol_handle* h);
  * build_function codegen, src/MOL_discretization.jl:171-201).  Needs no GPU.  Writes at most
  * buflen bytes (NUL-terminated) into buf (may be NULL) and returns the source length, or a
  * negative mol_status (message in mol_last_error(NULL)); tile = tx,ty,tz or NULL for the default.
- * compile != 0 also compiles it with hiprtc for gfx950 and fails when that does. */
-int64_t mol_codegen(const mol_desc* d, const int32_t* tile, char* buf, int64_t buflen, int32_t compile);
+ * flags: MOL_CODEGEN_COMPILE also compiles it with hiprtc for gfx950 and fails when that does;
+ * MOL_CODEGEN_VJP returns the reverse-mode (vector-Jacobian product) kernel instead. */
+enum { MOL_CODEGEN_COMPILE = 1, MOL_CODEGEN_VJP = 2 };
+int64_t mol_codegen(const mol_desc* d, const int32_t* tile, char* buf, int64_t buflen, int32_t flags);
 
 /* Library / device information (for test harnesses). */
 int mol_device_info(char* buf, int buflen);

@@ -53,7 +53,8 @@ void free_handle(mol_handle* h) {
   if (!h) return;
   void* ptrs[] = {h->d_tabs, h->d_slots, h->d_clos, h->d_pool, h->d_code, h->d_cpool,
                   h->d_prog, h->d_coord[0], h->d_coord[1], h->d_coord[2], h->w_y1,
-                  h->w_y2, h->w_acc, h->w_comp, h->d_tnow, h->d_ka64, h->d_ka32, h->d_tiles};
+                  h->w_y2, h->w_acc, h->w_comp, h->d_tnow, h->d_ka64, h->d_ka32, h->d_tiles,
+                  h->ck_u, h->ck_t, h->seg_u, h->seg_t, h->adj_y[0], h->adj_y[1], h->adj_y[2], h->adj_y[3]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h->h_u) (void)hipHostFree(h->h_u);
@@ -195,8 +196,9 @@ int validate(const mol_desc* d, std::string& msg) {
   return 0;
 }
 
+// lds_pos: LDS bytes per tile + halo position (0: the forward kernels' arrays)
 template <typename T>
-void fill_kargs(mol_handle* h, const mol_desc* d, KArgs<T>& ka) {
+void fill_kargs(mol_handle* h, const mol_desc* d, KArgs<T>& ka, size_t lds_pos = 0) {
   std::memset(&ka, 0, sizeof(ka));
   ka.ndim = d->ndim;
   ka.nfields = d->nfields;
@@ -249,7 +251,8 @@ void fill_kargs(mol_handle* h, const mol_desc* d, KArgs<T>& ka) {
       if (tile[a] > tile[big]) big = a;
     tile[big] = (tile[big] + 1) / 2;
   }
-  while ((size_t)nload * vol_of(tile) * sizeof(T) > 60000) {
+  if (lds_pos == 0) lds_pos = (size_t)nload * sizeof(T);
+  while (lds_pos * vol_of(tile) > 60000) {
     int big = 0;
     for (int a = 1; a < 3; ++a)
       if (tile[a] > tile[big] / 4) big = a;
@@ -301,6 +304,7 @@ Launch<T> launch_of(mol_handle* h) {
   L.ncore = h->ncore;
   L.nbound = h->nbound;
   L.jit = h->jit_fn;
+  L.ntiles = h->ka64.ntile[0] * h->ka64.ntile[1] * h->ka64.ntile[2];
   const KArgs<T>& ka = kargs<T>(h);
   for (int a = 0; a < 3; ++a) L.jp.coord[a] = ka.coord[a];
   for (int a = 0; a < MOL_MAX_AUX; ++a) L.jp.aux[a] = ka.aux[a];
@@ -318,9 +322,8 @@ void interior_range(const mol_optab& tb, int n, int& lo, int& hi) {
 
 // Build the core-tile kernel arguments and the list of tiles left to the generic kernel.
 template <typename T>
-void build_core(mol_handle* h, const mol_desc* d, KCore<T>& kc, std::vector<int32_t>& gen_tiles,
-                int& ncore) {
-  const KArgs<T>& ka = kargs<T>(h);
+void build_core(mol_handle* h, const mol_desc* d, const KArgs<T>& ka, KCore<T>& kc,
+                std::vector<int32_t>& gen_tiles, int& ncore) {
   std::memset(&kc, 0, sizeof(kc));
   ncore = 0;
   const int ntot = ka.ntile[0] * ka.ntile[1] * ka.ntile[2];
@@ -452,22 +455,31 @@ void build_core(mol_handle* h, const mol_desc* d, KCore<T>& kc, std::vector<int3
   h->core_interp_ok = fits;
 }
 
+// Geometry handed to the code generator: the tiling of the union box and the core tiles.
 template <typename T>
-JitGeom geom_of(const KCore<T>& kc) {
+JitGeom geom_of(const KArgs<T>& ka, const KCore<T>& kc, int ncore) {
   JitGeom g{};
   for (int a = 0; a < 3; ++a) {
-    g.n[a] = kc.n[a];
-    g.periodic[a] = kc.periodic[a];
-    g.lo[a] = kc.lo[a];
-    g.hi[a] = kc.hi[a];
-    g.tile[a] = kc.tile[a];
-    g.halo[a] = kc.halo[a];
-    g.ext[a] = kc.ext[a];
-    g.ct_lo[a] = kc.ct_lo[a];
-    g.ct_n[a] = kc.ct_n[a];
+    g.n[a] = ka.n[a];
+    g.periodic[a] = ka.periodic[a];
+    g.ulo[a] = ka.ulo[a];
+    g.uhi[a] = ka.uhi[a];
+    g.tile[a] = ka.tile[a];
+    g.halo[a] = ka.halo[a];
+    g.ext[a] = ka.ext[a];
+    g.ntile[a] = ka.ntile[a];
+    g.ct_lo[a] = ncore > 0 ? kc.ct_lo[a] : 0;
+    g.ct_n[a] = ncore > 0 ? kc.ct_n[a] : 0;
   }
-  g.vol = kc.vol;
-  g.box = kc.box;
+  for (int f = 0; f < MOL_MAX_FIELDS; ++f) {
+    for (int a = 0; a < 3; ++a) {
+      g.lo[f][a] = ka.lo[f][a];
+      g.hi[f][a] = ka.hi[f][a];
+    }
+    g.fofs[f] = ka.fofs[f];
+  }
+  g.vol = ka.vol;
+  g.ncore = ncore;
   return g;
 }
 
@@ -479,20 +491,20 @@ bool jit_enabled() {
 int setup_paths(mol_handle* h, const mol_desc* d) {
   std::vector<int32_t> t64, t32;
   int n64 = 0, n32 = 0;
-  build_core<double>(h, d, h->kc64, t64, n64);
-  build_core<float>(h, d, h->kc32, t32, n32);
+  build_core<double>(h, d, h->ka64, h->kc64, t64, n64);
+  build_core<float>(h, d, h->ka32, h->kc32, t32, n32);
   h->use_weno = false;
   h->jit_fn = nullptr;
   h->jit_reason.clear();
-  if (n64 > 0 && jit_enabled()) {
+  if (jit_enabled()) {
     std::string err;
-    const std::string src = jit_core_source(d, d->dtype == MOL_F64 ? geom_of(h->kc64) : geom_of(h->kc32), err);
+    const std::string src = jit_source(d, geom_of(h->ka64, h->kc64, n64), err);
     if (src.empty()) h->jit_reason = err;
-    else if (jit_kernel(src, h->device, &h->jit_fn, err)) {
+    else if (jit_kernel(src, "mol_rhs_tiles", h->device, &h->jit_fn, err)) {
       h->jit_fn = nullptr;
       h->jit_reason = err;
     }
-  } else if (n64 > 0) {
+  } else {
     h->jit_reason = "code generation disabled (MOL_JIT=0)";
   }
   if (n64 > 0 && !h->jit_fn && !h->core_interp_ok) {
@@ -637,6 +649,182 @@ int integrate_impl(mol_handle* h, T* u, const T* p, double t0, double dt, int64_
   return rc;
 }
 
+// Does program `prog` read opcode `op` (e.g. an auxiliary array)?
+bool prog_uses(const mol_desc* d, int prog, uint32_t op) {
+  if (prog < 0) return false;
+  const int st = d->prog_h[2 * prog], ln = d->prog_h[2 * prog + 1];
+  for (int pc = st; pc < st + ln; ++pc)
+    if ((d->code_h[pc] & 0xffu) == op) return true;
+  return false;
+}
+
+// Generate and load the vector-Jacobian product kernel (first use).  Its tiles are sized for
+// the state, the auxiliary arrays and their adjoint buffers in LDS.
+int ensure_vjp(mol_handle* h) {
+  if (h->vjp_fn) return 0;
+  const mol_desc* d = &h->desc.d;
+  KArgs<double> kv;
+  fill_kargs<double>(h, d, kv, (2 * (size_t)d->nfields + d->naux) * h->esize() + (size_t)d->naux * sizeof(double));
+  KCore<double> kc;
+  std::vector<int32_t> gen;
+  int ncore = 0;
+  const std::string reason = h->core_reason;
+  const bool interp_ok = h->core_interp_ok;
+  build_core<double>(h, d, kv, kc, gen, ncore);
+  h->core_reason = reason;
+  h->core_interp_ok = interp_ok;
+  std::string err;
+  const std::string src = jit_vjp_source(d, geom_of(kv, kc, ncore), err);
+  if (src.empty()) return fail(h, MOL_ERR_UNSUPPORTED, "vector-Jacobian product: " + err);
+  if (jit_kernel(src, "mol_vjp_tiles", h->device, &h->vjp_fn, err)) {
+    h->vjp_fn = nullptr;
+    return fail(h, MOL_ERR_HIP, "vector-Jacobian product: " + err);
+  }
+  h->vjp_tiles = kv.ntile[0] * kv.ntile[1] * kv.ntile[2];
+  h->vjp_aux_ok = true;
+  for (int c = 0; c < d->nclosures; ++c)
+    if (prog_uses(d, d->closures_h[c].gprog, MOL_OP_AUX)) h->vjp_aux_ok = false;
+  return 0;
+}
+
+template <typename T>
+int launch_vjp(mol_handle* h, const VArgs<T>& va, hipStream_t s) {
+  const KArgs<T>& ka = kargs<T>(h);
+  JitPtrs<T> jp{};
+  for (int a = 0; a < 3; ++a) jp.coord[a] = ka.coord[a];
+  for (int a = 0; a < MOL_MAX_AUX; ++a) jp.aux[a] = ka.aux[a];
+  VArgs<T> v = va;
+  void* args[] = {(void*)&v, (void*)&jp};
+  const hipError_t e = hipModuleLaunchKernel(h->vjp_fn, h->vjp_tiles, 1, 1, 256, 1, 1, 0, s, args, nullptr);
+  if (e != hipSuccess) return fail(h, MOL_ERR_HIP, std::string("vector-Jacobian product launch: ") + hipGetErrorString(e));
+  return 0;
+}
+
+// One RK4 step from uin that keeps the stage states y1, y2, y3 and writes u_{n+1} to uout;
+// the same arithmetic as the stage loop of enqueue_steps (time from the device clock).
+template <typename T>
+int rk4_step_store(mol_handle* h, const T* p, const T* uin, T* y1, T* y2, T* y3, T* uout, double dt,
+                   hipStream_t s) {
+  const Launch<T> L = launch_of<T>(h);
+  T* acc = (T*)h->w_acc;
+  Epi<T> e{};
+  e.tnow = h->d_tnow;
+  e.p = p;
+  int rc = 0;
+  e.mode = 1; e.out = y1; e.A = uin; e.ca = 1.0; e.cf = dt / 2; e.acc = acc; e.acc_keep = 0.0; e.acc_f = 1.0; e.tofs = 0.0;
+  rc |= launch_rhs<T>(L, uin, e, s);
+  e.out = y2; e.cf = dt / 2; e.acc_keep = 1.0; e.acc_f = 2.0; e.tofs = dt / 2;
+  rc |= launch_rhs<T>(L, y1, e, s);
+  e.out = y3; e.cf = dt; e.tofs = dt / 2;
+  rc |= launch_rhs<T>(L, y2, e, s);
+  Epi<T> l{};
+  l.tnow = h->d_tnow; l.p = p; l.tofs = dt; l.mode = 1; l.out = uout; l.A = uin; l.ca = 1.0; l.B = acc;
+  l.cb = dt / 6; l.cf = dt / 6;
+  rc |= launch_rhs<T>(L, y3, l, s);
+  rc |= launch_advance_time(h->d_tnow, dt, s);
+  if (rc) return fail(h, MOL_ERR_HIP, std::string("kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
+  return 0;
+}
+
+template <typename T>
+int forward_ckpt_impl(mol_handle* h, T* u, const T* p, double t0, double dt, int64_t nsteps, int32_t every,
+                      hipStream_t s) {
+  int rc = ensure_workspace(h);
+  if (rc) return rc;
+  const int64_t nseg = (nsteps + every - 1) / every;
+  const size_t vbytes = (size_t)h->nunk * h->esize();
+  if (h->ck_cap < (size_t)nseg) {
+    if (h->ck_u) (void)hipFree(h->ck_u);
+    if (h->ck_t) (void)hipFree(h->ck_t);
+    h->ck_u = nullptr;
+    h->ck_t = nullptr;
+    h->ck_cap = 0;
+    HIPCHK(h, hipMalloc(&h->ck_u, (size_t)nseg * vbytes));
+    HIPCHK(h, hipMalloc((void**)&h->ck_t, (size_t)nseg * sizeof(double)));
+    h->ck_cap = (size_t)nseg;
+  }
+  h->ck_valid = false;
+  HIPCHK(h, hipMemcpyAsync(h->d_tnow, &t0, sizeof(double), hipMemcpyHostToDevice, s));
+  for (int64_t g = 0; g < nseg; ++g) {
+    const int64_t len = std::min<int64_t>(every, nsteps - g * every);
+    HIPCHK(h, hipMemcpyAsync((char*)h->ck_u + g * vbytes, u, vbytes, hipMemcpyDeviceToDevice, s));
+    HIPCHK(h, hipMemcpyAsync(h->ck_t + g, h->d_tnow, sizeof(double), hipMemcpyDeviceToDevice, s));
+    rc = enqueue_steps<T>(h, u, p, dt, len, MOL_RK4, false, s);
+    if (rc) return rc;
+  }
+  h->ck_nsteps = nsteps;
+  h->ck_every = every;
+  h->ck_dt = dt;
+  h->ck_valid = true;
+  return 0;
+}
+
+template <typename T>
+int adjoint_impl(mol_handle* h, const T* p, T* lam, double* g_aux, hipStream_t s) {
+  int rc = ensure_vjp(h);
+  if (rc) return rc;
+  if (g_aux && !h->vjp_aux_ok)
+    return fail(h, MOL_ERR_UNSUPPORTED, "auxiliary-array gradient through boundary values that read auxiliary arrays");
+  rc = ensure_workspace(h);
+  if (rc) return rc;
+  const int64_t every = h->ck_every, nsteps = h->ck_nsteps;
+  const int64_t nseg = (nsteps + every - 1) / every;
+  const double dt = h->ck_dt;
+  const int64_t n = h->nunk;
+  const size_t vbytes = (size_t)n * h->esize();
+  const size_t need = (size_t)(4 * every + 1);
+  if (h->seg_cap < need) {
+    if (h->seg_u) (void)hipFree(h->seg_u);
+    if (h->seg_t) (void)hipFree(h->seg_t);
+    h->seg_u = nullptr;
+    h->seg_t = nullptr;
+    h->seg_cap = 0;
+    HIPCHK(h, hipMalloc(&h->seg_u, need * vbytes));
+    HIPCHK(h, hipMalloc((void**)&h->seg_t, (size_t)(every + 1) * sizeof(double)));
+    h->seg_cap = need;
+  }
+  for (int k = 0; k < 4; ++k)
+    if (!h->adj_y[k]) HIPCHK(h, hipMalloc(&h->adj_y[k], vbytes));
+  T* S = (T*)h->seg_u;
+  auto st = [&](int64_t j, int k) { return S + (size_t)(4 * j + k) * n; };   // state k of step j
+  T* Y[4] = {(T*)h->adj_y[0], (T*)h->adj_y[1], (T*)h->adj_y[2], (T*)h->adj_y[3]};
+  for (int64_t g = nseg - 1; g >= 0; --g) {
+    const int64_t len = std::min<int64_t>(every, nsteps - g * every);
+    // recompute the segment from its checkpoint, keeping every stage state and step time
+    HIPCHK(h, hipMemcpyAsync(st(0, 0), (char*)h->ck_u + g * vbytes, vbytes, hipMemcpyDeviceToDevice, s));
+    HIPCHK(h, hipMemcpyAsync(h->d_tnow, h->ck_t + g, sizeof(double), hipMemcpyDeviceToDevice, s));
+    for (int64_t j = 0; j < len; ++j) {
+      HIPCHK(h, hipMemcpyAsync(h->seg_t + j, h->d_tnow, sizeof(double), hipMemcpyDeviceToDevice, s));
+      rc = rk4_step_store<T>(h, p, st(j, 0), st(j, 1), st(j, 2), st(j, 3), j + 1 < len ? st(j + 1, 0) : Y[0], dt, s);
+      if (rc) return rc;
+    }
+    // backward sweep: k4 = f(y3, t+dt), k3 = f(y2, t+dt/2), k2 = f(y1, t+dt/2), k1 = f(u, t)
+    for (int64_t j = len - 1; j >= 0; --j) {
+      const int stage_state[4] = {3, 2, 1, 0};
+      const double tofs[4] = {dt, dt / 2, dt / 2, 0.0};
+      const double c1[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
+      const double c2[4] = {0.0, dt, dt / 2, dt / 2};
+      for (int q = 0; q < 4; ++q) {
+        if ((rc = launch_fill_zero<T>(Y[q], n, s))) return fail(h, rc, "zero fill launch failed");
+        VArgs<T> va{};
+        va.u = st(j, stage_state[q]);
+        va.s1 = lam;
+        va.c1 = c1[q];
+        va.s2 = q > 0 ? Y[q - 1] : nullptr;
+        va.c2 = c2[q];
+        va.out = Y[q];
+        va.g_aux = g_aux;
+        va.p = p;
+        va.tnow = h->seg_t + j;
+        va.tofs = tofs[q];
+        if ((rc = launch_vjp<T>(h, va, s))) return rc;
+      }
+      if ((rc = launch_adjoint_sum<T>(lam, Y[0], Y[1], Y[2], Y[3], n, s))) return fail(h, rc, "adjoint update launch failed");
+    }
+  }
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -714,7 +902,7 @@ int mol_set_tile(mol_handle* h, int32_t tx, int32_t ty, int32_t tz) {
 int mol_get_plan(const mol_handle* h, int32_t* info, int32_t ninfo) {
   if (!h || !info) return MOL_ERR_INVALID;
   int32_t v[8] = {h->ka64.tile[0], h->ka64.tile[1], h->ka64.tile[2], h->ncore, h->nbound,
-                  h->use_weno ? h->nweno : 0, h->jit_fn != nullptr && h->ncore > 0 && !h->use_weno ? 1 : 0,
+                  h->use_weno ? h->nweno : 0, h->jit_fn != nullptr && !h->use_weno ? 1 : 0,
                   h->core_interp_ok ? 1 : 0};
   for (int i = 0; i < ninfo && i < 8; ++i) info[i] = v[i];
   return 0;
@@ -798,16 +986,50 @@ int mol_integrate(mol_handle* h, void* u, const void* p, double t0, double dt, i
   return integrate_impl<float>(h, (float*)u, (const float*)p, t0, dt, nsteps, scheme, flags, s);
 }
 
-int mol_vjp(mol_handle* h, const void*, const void*, double, const void*, void*, double*, void*) {
-  return fail(h, MOL_ERR_UNSUPPORTED, "mol_vjp: not available in this build");
+int mol_vjp(mol_handle* h, const void* u, const void* p, double t, const void* lam, void* lam_dot, double* g_aux,
+            void* stream) {
+  if (!h) return fail(nullptr, MOL_ERR_INVALID, "handle is NULL");
+  if (!u || !lam || !lam_dot) return fail(h, MOL_ERR_INVALID, "u/lam/lam_dot is NULL");
+  if (h->nparams > 0 && !p) return fail(h, MOL_ERR_INVALID, "parameters missing");
+  if (int rc = check_aux(h)) return rc;
+  if (int rc = ensure_vjp(h)) return rc;
+  if (g_aux && !h->vjp_aux_ok)
+    return fail(h, MOL_ERR_UNSUPPORTED, "auxiliary-array gradient through boundary values that read auxiliary arrays");
+  hipStream_t s = (hipStream_t)stream;
+  if (h->dtype == MOL_F64) {
+    VArgs<double> va{(const double*)u, (const double*)lam, nullptr, 1.0, 0.0, (double*)lam_dot, g_aux,
+                     (const double*)p, nullptr, t};
+    if (launch_fill_zero<double>((double*)lam_dot, h->nunk, s)) return fail(h, MOL_ERR_HIP, "zero fill launch failed");
+    return launch_vjp<double>(h, va, s);
+  }
+  VArgs<float> va{(const float*)u, (const float*)lam, nullptr, 1.0, 0.0, (float*)lam_dot, g_aux,
+                  (const float*)p, nullptr, t};
+  if (launch_fill_zero<float>((float*)lam_dot, h->nunk, s)) return fail(h, MOL_ERR_HIP, "zero fill launch failed");
+  return launch_vjp<float>(h, va, s);
 }
 
-int mol_forward_checkpointed(mol_handle* h, void*, const void*, double, double, int64_t, int32_t, void*) {
-  return fail(h, MOL_ERR_UNSUPPORTED, "mol_forward_checkpointed: not available in this build");
+int mol_forward_checkpointed(mol_handle* h, void* u, const void* p, double t0, double dt, int64_t nsteps,
+                             int32_t ckpt_every, void* stream) {
+  if (!h) return fail(nullptr, MOL_ERR_INVALID, "handle is NULL");
+  if (!u) return fail(h, MOL_ERR_INVALID, "u is NULL");
+  if (nsteps < 1 || ckpt_every < 1) return fail(h, MOL_ERR_INVALID, "nsteps and ckpt_every must be >= 1");
+  if (h->nparams > 0 && !p) return fail(h, MOL_ERR_INVALID, "parameters missing");
+  if (int rc = check_aux(h)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (h->dtype == MOL_F64)
+    return forward_ckpt_impl<double>(h, (double*)u, (const double*)p, t0, dt, nsteps, ckpt_every, s);
+  return forward_ckpt_impl<float>(h, (float*)u, (const float*)p, t0, dt, nsteps, ckpt_every, s);
 }
 
-int mol_adjoint(mol_handle* h, const void*, void*, double*, void*) {
-  return fail(h, MOL_ERR_UNSUPPORTED, "mol_adjoint: not available in this build");
+int mol_adjoint(mol_handle* h, const void* p, void* lam, double* g_aux, void* stream) {
+  if (!h) return fail(nullptr, MOL_ERR_INVALID, "handle is NULL");
+  if (!lam) return fail(h, MOL_ERR_INVALID, "lam is NULL");
+  if (!h->ck_valid) return fail(h, MOL_ERR_INVALID, "no checkpointed run (call mol_forward_checkpointed first)");
+  if (h->nparams > 0 && !p) return fail(h, MOL_ERR_INVALID, "parameters missing");
+  if (int rc = check_aux(h)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (h->dtype == MOL_F64) return adjoint_impl<double>(h, (const double*)p, (double*)lam, g_aux, s);
+  return adjoint_impl<float>(h, (const float*)p, (float*)lam, g_aux, s);
 }
 
 int64_t mol_num_unknowns(const mol_handle* h) { return h ? h->nunk : -1; }
@@ -819,7 +1041,7 @@ const char* mol_last_error(const mol_handle* h) {
 
 void mol_destroy(mol_handle* h) { free_handle(h); }
 
-int64_t mol_codegen(const mol_desc* d, const int32_t* tile, char* buf, int64_t buflen, int32_t compile) {
+int64_t mol_codegen(const mol_desc* d, const int32_t* tile, char* buf, int64_t buflen, int32_t flags) {
   std::string msg;
   int rc = validate(d, msg);
   if (rc) return fail(nullptr, rc, msg);
@@ -833,12 +1055,12 @@ int64_t mol_codegen(const mol_desc* d, const int32_t* tile, char* buf, int64_t b
   fill_kargs<float>(&tmp, d, tmp.ka32);
   std::vector<int32_t> gen;
   int ncore = 0;
-  build_core<double>(&tmp, d, tmp.kc64, gen, ncore);
-  if (ncore == 0) return fail(nullptr, MOL_ERR_UNSUPPORTED, "no core tiles: " + tmp.core_reason);
+  build_core<double>(&tmp, d, tmp.ka64, tmp.kc64, gen, ncore);
   std::string err;
-  const std::string src = jit_core_source(d, geom_of(tmp.kc64), err);
+  const std::string src = (flags & MOL_CODEGEN_VJP) ? jit_vjp_source(d, geom_of(tmp.ka64, tmp.kc64, ncore), err)
+                                                    : jit_source(d, geom_of(tmp.ka64, tmp.kc64, ncore), err);
   if (src.empty()) return fail(nullptr, MOL_ERR_UNSUPPORTED, err);
-  if (compile) {
+  if (flags & MOL_CODEGEN_COMPILE) {
     std::vector<char> code;
     std::string log;
     if (jit_compile(src, code, log)) return fail(nullptr, MOL_ERR_HIP, "hiprtc: " + log);

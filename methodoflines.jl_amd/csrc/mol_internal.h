@@ -18,8 +18,12 @@ struct WenoPlan {
 
 // Geometry of the core-tile region handed to the code generator.
 struct JitGeom {
-  int n[3], periodic[3], lo[3], hi[3], tile[3], halo[3], ext[3], vol, ct_lo[3], ct_n[3];
-  long long box;
+  int n[3], periodic[3];
+  int lo[MOL_MAX_FIELDS][3], hi[MOL_MAX_FIELDS][3];   // unknown box per variable
+  long long fofs[MOL_MAX_FIELDS];                     // offset of each variable in the state
+  int ulo[3], uhi[3];                                 // union box (tile origin)
+  int tile[3], halo[3], ext[3], vol, ntile[3];
+  int ct_lo[3], ct_n[3], ncore;                       // core tiles (interior rows only)
 };
 
 // Pointer arguments of the generated kernel (layout identical to JPtrs of mol_jit_prelude.h).
@@ -29,10 +33,26 @@ struct JitPtrs {
   const T* aux[MOL_MAX_AUX];
 };
 
-// Code generation and in-process compilation of the core-tile kernel (mol_jit.cpp).
-std::string jit_core_source(const mol_desc* d, const JitGeom& g, std::string& err);
+// Code generation and in-process compilation of the tile kernel (mol_jit.cpp).
+std::string jit_source(const mol_desc* d, const JitGeom& g, std::string& err);
+std::string jit_vjp_source(const mol_desc* d, const JitGeom& g, std::string& err);
 int jit_compile(const std::string& src, std::vector<char>& code, std::string& log);
-int jit_kernel(const std::string& src, int device, hipFunction_t* fn, std::string& err);
+int jit_kernel(const std::string& src, const char* name, int device, hipFunction_t* fn, std::string& err);
+
+// Arguments of the generated vector-Jacobian product kernel (layout identical to VArgs of
+// mol_jit_prelude.h): seed c1*s1 + c2*s2, result accumulated into out, FP64 aux gradients.
+template <typename T>
+struct VArgs {
+  const T* u;
+  const T* s1;
+  const T* s2;
+  double c1, c2;
+  T* out;
+  double* g_aux;
+  const T* p;
+  const double* tnow;
+  double tofs;
+};
 
 // What one right-hand-side launch needs: the core-tile kernel arguments and the list of tiles
 // left to the generic kernel.
@@ -46,7 +66,8 @@ struct Launch {
   const KCore<T>* kcw = nullptr;     // WENO fast-path arguments (nullptr: not used)
   const WenoPlan* plan = nullptr;
   int nweno = 0;
-  hipFunction_t jit = nullptr;       // generated core-tile kernel (nullptr: interpreter)
+  hipFunction_t jit = nullptr;       // generated kernel over all tiles (nullptr: interpreter)
+  int ntiles = 0;
   JitPtrs<T> jp{};
 };
 
@@ -63,6 +84,11 @@ int launch_weno2d(const KCore<T>& kc, const WenoPlan& plan, int nblocks, const T
                   hipStream_t stream);
 constexpr int WENO_TILE_X = 64, WENO_TILE_Y = 32;
 int launch_advance_time(double* tnow, double dt, hipStream_t stream);
+template <typename T>
+int launch_fill_zero(T* p, int64_t n, hipStream_t stream);
+// lam += y3 + y2 + y1 + y0 (the adjoint of one RK4 step)
+template <typename T>
+int launch_adjoint_sum(T* lam, const T* y0, const T* y1, const T* y2, const T* y3, int64_t n, hipStream_t stream);
 
 // Set the message of the calling thread's last failure (used when no handle exists yet).
 void set_global_error(const std::string& msg);
@@ -121,8 +147,23 @@ struct mol_handle {
   int ncore = 0, nbound = 0;
   std::string core_reason;                // why the core path is off, if it is
   bool core_interp_ok = false;            // the stencil IR fits the interpreter's argument block
-  hipFunction_t jit_fn = nullptr;         // generated core-tile kernel
+  hipFunction_t jit_fn = nullptr;         // generated kernel over all tiles
   std::string jit_reason;                 // why it is not used, if it is not
+  // reverse mode
+  hipFunction_t vjp_fn = nullptr;         // generated vector-Jacobian product kernel
+  int vjp_tiles = 0;
+  bool vjp_aux_ok = true;                 // no boundary value reads an auxiliary array
+  void* ck_u = nullptr;                   // checkpoints of the last mol_forward_checkpointed run
+  double* ck_t = nullptr;                 // device clock at each checkpoint
+  size_t ck_cap = 0;
+  int64_t ck_nsteps = 0;
+  int32_t ck_every = 0;
+  double ck_dt = 0.0;
+  bool ck_valid = false;
+  void* seg_u = nullptr;                  // stage states of one recomputed segment
+  double* seg_t = nullptr;                // device clock at each step of the segment
+  size_t seg_cap = 0;
+  void* adj_y[4] = {nullptr, nullptr, nullptr, nullptr};
   bool use_weno = false;                  // WENO5 fast path
   mol::WenoPlan wplan{};
   mol::KCore<double> kcw64{};

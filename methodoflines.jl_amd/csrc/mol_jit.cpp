@@ -10,6 +10,7 @@
 // system.  The interpreter kernels stay as the path for boundary tiles and as the fallback.
 #include <hip/hiprtc.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -39,129 +40,534 @@ std::string lit(double x) {
 
 std::string istr(long long v) { return std::to_string(v); }
 
+// ---------------------------------------------------------------------------------------
+// Expression IR of one node's right-hand side.  Leaves: literals, LDS reads (variable or
+// auxiliary array at a displacement from the node), and opaque expressions that depend on no
+// variable (coordinates, time, parameters).  Operations: the bytecode opcodes plus stencil
+// linear combinations, WENO5 reconstructions, selections and a branch node whose value is
+// chosen by the node index along one axis (boundary rows).
+enum NodeKind { N_LIT, N_LDS, N_EXPR, N_OP, N_BRANCH };
+enum : int { X_LINCOMB = 1000, X_WENO = 1001, X_SPHCOND = 1002, X_SELECT = 1003 };
+
+struct Node {
+  int kind = N_LIT;
+  int op = 0;
+  std::vector<int> a;        // operands
+  std::vector<double> w;     // X_LINCOMB weights; X_WENO: eps, inv_dx
+  double val = 0;            // N_LIT
+  int arr = 0;               // N_LDS: array (variable, or nfields + aux index)
+  long long off = 0;         //        displacement in LDS elements
+  std::string text;          // N_EXPR
+  int imm = 0;               // MOL_OP_POWI exponent
+  // N_BRANCH: region reg[0] when lo <= I_ax <= hi, region reg[k] when I_ax == cval[k], else 0
+  int ax = 0, lo = 0, hi = 0;
+  std::vector<int> cval, reg;
+};
+
+struct Region {
+  std::vector<int> nodes;    // N_OP / N_BRANCH nodes in evaluation order
+  int result = -1;
+};
+
+// A weight row resolved for one node: weights and tap positions relative to the node (in
+// nodes along the row's axis).  `valid` is false where the reference would read outside the
+// table (such nodes are never evaluated).
+struct RowR {
+  std::vector<double> w;
+  std::vector<int> rel;
+  bool valid = true;
+};
+
+// Node-index variant of one axis: the interior-row code (index symbolic) or one concrete index.
+struct Var {
+  bool interior = true;
+  int i = 0;
+};
+
 // Leaf context of a program.
-enum CtxKind { CTX_NODE, CTX_PLAIN, CTX_HALF };
+enum CtxKind { CTX_NODE, CTX_PLAIN, CTX_HALF, CTX_BOUNDARY };
 struct Ctx {
   CtxKind kind = CTX_NODE;
-  int ax = 0, h = 0;              // half point between node+h and node+h+1 along ax
-  const mol_optab* ip = nullptr;  // interpolation row (interior)
-  const mol_optab* dp = nullptr;  // inner derivative row (interior)
+  int ax = 0;                  // half-point axis, or the closure axis (CTX_BOUNDARY)
+  const RowR* ip = nullptr;    // interpolation row at the half point
+  const RowR* dp = nullptr;    // inner derivative row at the half point
   int dfield = 0;
+  int side = 0;                // CTX_BOUNDARY: 0 low (index 1), 1 high (index n)
 };
 
 class Gen {
  public:
   Gen(const mol_desc* d, const JitGeom& g) : d_(d), g_(g) {}
 
-  std::string source(bool f64);
+  std::string source(bool f64);       // forward kernel mol_rhs_tiles
+  std::string source_vjp(bool f64);   // reverse-mode kernel mol_vjp_tiles
   std::string error;
 
  private:
   const mol_desc* d_;
   JitGeom g_;
-  std::ostringstream body_;
-  int nv_ = 0;
-  std::map<int, std::string> slot_var_;
-
-  std::string var() { return "v" + istr(nv_++); }
-  std::string emit(const std::string& expr) {
-    const std::string v = var();
-    body_ << "    const T " << v << " = " << expr << ";\n";
-    return v;
-  }
-  int stride(int ax) const {
-    return ax == 0 ? 1 : (ax == 1 ? g_.ext[0] : g_.ext[0] * g_.ext[1]);
-  }
-  // LDS value of array `arr` at the node displaced by `off` (in elements)
-  std::string at(int arr, long long off) const {
-    long long o = (long long)arr * g_.vol + off;
-    if (o == 0) return "L[b]";
-    return o > 0 ? "L[b + " + istr(o) + "]" : "L[b - " + istr(-o) + "]";
-  }
-  std::string coord_node(int d) const {
-    return "(T)jp.coord[" + istr(d) + "][I" + istr(d) + " - 1]";
-  }
-  std::string coord_tap(int d, int off) const {
-    if (off == 0) return coord_node(d);
-    const std::string k = "I" + istr(d) + (off > 0 ? " + " + istr(off) : " - " + istr(-off));
-    if (g_.periodic[d]) return "(T)jp.coord[" + istr(d) + "][mol_wrap(" + k + ", " + istr(g_.n[d]) + ") - 1]";
-    return "(T)jp.coord[" + istr(d) + "][" + k + " - 1]";
-  }
-  // interior-row stencil sum_k w[k] * arr[node + (off + k) e_ax]
-  std::string stencil(const mol_optab& tb, int arr, int ax, int shift = 0) {
-    std::string e;
-    for (int k = 0; k < tb.len; ++k) {
-      const double w = d_->pool_h[tb.w + k];
-      const std::string term = lit(w) + " * " + at(arr, (long long)(shift + tb.off + k) * stride(ax));
-      e = k == 0 ? term : e + " + " + term;
-    }
-    return e.empty() ? "T(0)" : emit(e);
-  }
-  std::string leaf(uint32_t op, int imm, const Ctx& c);
-  std::string prog(int p, const Ctx& c);
-  std::string slot(int k);
-  std::string nonlinlap(const mol_slot& s, int qo, int qi, int qp);
+  std::vector<Node> nodes_;
+  std::vector<Region> regs_;
+  int cur_ = -1;                      // region being built
+  std::map<int, int> slot_node_;      // slot -> node id (per node function)
+  bool core_ = true;                  // building the interior-rows variant
   bool ok_ = true;
+
   void bad(const std::string& m) {
     if (ok_) error = m;
     ok_ = false;
   }
+  // ---- IR construction
+  int add(Node n) {
+    nodes_.push_back(std::move(n));
+    const int id = (int)nodes_.size() - 1;
+    if (nodes_[id].kind == N_OP || nodes_[id].kind == N_BRANCH) regs_[cur_].nodes.push_back(id);
+    return id;
+  }
+  int lit_node(double v) {
+    Node n;
+    n.kind = N_LIT;
+    n.val = v;
+    return add(n);
+  }
+  int lds(int arr, long long off) {
+    Node n;
+    n.kind = N_LDS;
+    n.arr = arr;
+    n.off = off;
+    return add(n);
+  }
+  int expr(const std::string& t) {
+    Node n;
+    n.kind = N_EXPR;
+    n.text = t;
+    return add(n);
+  }
+  int op(int code, std::vector<int> args, int imm = 0) {
+    Node n;
+    n.kind = N_OP;
+    n.op = code;
+    n.a = std::move(args);
+    n.imm = imm;
+    return add(n);
+  }
+  int lincomb(const std::vector<double>& w, const std::vector<int>& args) {
+    Node n;
+    n.kind = N_OP;
+    n.op = X_LINCOMB;
+    for (size_t k = 0; k < w.size(); ++k)
+      if (w[k] != 0.0) {
+        n.w.push_back(w[k]);
+        n.a.push_back(args[k]);
+      }
+    if (n.a.empty()) return lit_node(0.0);
+    return add(n);
+  }
+  int new_region() {
+    regs_.push_back(Region{});
+    return (int)regs_.size() - 1;
+  }
+
+  int stride(int ax) const { return ax == 0 ? 1 : (ax == 1 ? g_.ext[0] : g_.ext[0] * g_.ext[1]); }
+  std::string coord_node(int d) const { return "(T)jp.coord[" + istr(d) + "][I" + istr(d) + " - 1]"; }
+  std::string coord_tap(int d, int off) const {
+    if (off == 0) return coord_node(d);
+    const std::string k = "I" + istr(d) + (off > 0 ? " + " + istr(off) : " - " + istr(-off));
+    if (g_.periodic[d]) return "(T)jp.coord[" + istr(d) + "][mol_wrap(" + k + ", " + istr(g_.n[d]) + ") - 1]";
+    // ghost positions past the ends read the end coordinate (never an address outside the array)
+    return "(T)jp.coord[" + istr(d) + "][min(max(" + k + ", 1), " + istr(g_.n[d]) + ") - 1]";
+  }
+
+  RowR row(const mol_optab& tb, int ax, const Var& v, int idx_rel);
+  int apply(const RowR& r, int arr, int ax) {
+    if (!r.valid) return lit_node(0.0);
+    std::vector<int> args;
+    for (size_t k = 0; k < r.w.size(); ++k) args.push_back(lds(arr, (long long)r.rel[k] * stride(ax)));
+    return lincomb(r.w, args);
+  }
+  void slot_range(const mol_slot& s, int ax, int& lo, int& hi) const;
+  int leaf(uint32_t op, int imm, const Ctx& c);
+  int prog(int p, const Ctx& c);
+  int slot(int k);
+  int slot_body(int k, const Var* v);
+  int slot_branch(int k, const std::vector<int>& axes, size_t a, Var* v);
+  int nonlinlap(const mol_slot& s, int qo, int qi, int qp, const Var& v);
+  void build_node(bool core, std::vector<int>& outs, int& region);
+
+  // ---- printing
+  std::string ref(int id) const;
+  std::string op_expr(const Node& n, int id) const;
+  void print_region(std::ostream& o, int r) const;
+  void print_node(std::ostream& o, int id) const;
+  void print_reverse(std::ostream& o, int r, const std::vector<std::pair<int, std::string>>& seeds) const;
+  void partials(const Node& n, int id, std::vector<std::pair<int, std::string>>& out) const;
+  std::string stage_fn();
+  std::string scatter_fn();
+  std::string geo_common();
 };
 
-std::string Gen::leaf(uint32_t op, int imm, const Ctx& c) {
-  switch (op) {
-    case MOL_OP_CONST: return lit(d_->cpool_h[imm]);
-    case MOL_OP_TIME: return "(T)t";
-    case MOL_OP_PARAM: return "P[" + istr(imm) + "]";
-    default: break;
-  }
-  if (c.kind == CTX_HALF) {
-    auto interp = [&](int arr) {
-      std::string e;
-      for (int k = 0; k < c.ip->len; ++k) {
-        const std::string term = lit(d_->pool_h[c.ip->w + k]) + " * " +
-                                 at(arr, (long long)(c.h + c.ip->off + k) * stride(c.ax));
-        e = k == 0 ? term : e + " + " + term;
-      }
-      return emit(e);
-    };
-    switch (op) {
-      case MOL_OP_FIELD: return interp(imm);
-      case MOL_OP_AUX: return interp(d_->nfields + imm);
-      case MOL_OP_COORD: {
-        if (imm != c.ax) return coord_node(imm);
-        std::string e;
-        for (int k = 0; k < c.ip->len; ++k) {
-          const std::string term = lit(d_->pool_h[c.ip->w + k]) + " * " + coord_tap(imm, c.h + c.ip->off + k);
-          e = k == 0 ? term : e + " + " + term;
-        }
-        return emit(e);
-      }
-      case MOL_OP_DINNER: {
-        std::string e;
-        for (int k = 0; k < c.dp->len; ++k) {
-          const std::string term = lit(d_->pool_h[c.dp->w + k]) + " * " +
-                                   at(c.dfield, (long long)(c.h + c.dp->off + k) * stride(c.ax));
-          e = k == 0 ? term : e + " + " + term;
-        }
-        return emit(e);
-      }
-      case MOL_OP_SLOT: return "T(0)";
-      default: bad("unknown leaf opcode"); return "T(0)";
+// Row of table `tb` along axis `ax` selected at index (node index + idx_rel), as select_row
+// of the interpreter: boundary rows near non-periodic ends, the interior row elsewhere.
+RowR Gen::row(const mol_optab& tb, int ax, const Var& v, int idx_rel) {
+  RowR r;
+  const int n = g_.n[ax];
+  if (v.interior || g_.periodic[ax]) {
+    for (int k = 0; k < tb.len; ++k) {
+      r.w.push_back(d_->pool_h[tb.w + k]);
+      r.rel.push_back(idx_rel + tb.off + k);
     }
+    return r;
   }
-  switch (op) {
-    case MOL_OP_FIELD: return at(imm, 0);
-    case MOL_OP_AUX: return at(d_->nfields + imm, 0);
-    case MOL_OP_COORD: return coord_node(imm);
-    case MOL_OP_SLOT: return c.kind == CTX_NODE ? slot(imm) : "T(0)";
-    case MOL_OP_DINNER: return "T(0)";
-    default: bad("unknown leaf opcode"); return "T(0)";
+  const int i = v.i + idx_rel;
+  const int lref = n + tb.ref_shift;
+  int a0, cnt;
+  long long w0;
+  if (i <= tb.nlow) {
+    if (i < 1) { r.valid = false; return r; }
+    w0 = tb.wlow + (long long)(i - 1) * tb.blen;
+    a0 = 1;
+    cnt = tb.blen;
+  } else if (i > lref - tb.nhigh) {
+    const int rw = lref - i + tb.hoff;
+    if (rw < 1 || rw > tb.nhigh) { r.valid = false; return r; }
+    w0 = tb.whigh + (long long)(rw - 1) * tb.blen;
+    a0 = lref - tb.blen + 1;
+    cnt = tb.blen;
+  } else {
+    w0 = tb.w;
+    a0 = i + tb.off;
+    cnt = tb.len;
+  }
+  if (w0 < 0 || w0 + cnt > d_->npool) { r.valid = false; return r; }
+  for (int k = 0; k < cnt; ++k) {
+    r.w.push_back(d_->pool_h[w0 + k]);
+    r.rel.push_back(a0 + k - v.i);
+  }
+  return r;
+}
+
+void Gen::slot_range(const mol_slot& s, int ax, int& lo, int& hi) const {
+  lo = -(1 << 30);
+  hi = 1 << 30;
+  if (g_.periodic[ax]) return;
+  const int n = g_.n[ax];
+  auto node_tab = [&](int t) {
+    const mol_optab& tb = d_->tabs_h[t];
+    lo = std::max(lo, tb.nlow + 1);
+    hi = std::min(hi, n + tb.ref_shift - tb.nhigh);
+  };
+  auto half_tabs = [&](int to_i, int ti_i, int tp_i) {
+    const mol_optab& to = d_->tabs_h[to_i];
+    // outer row at index i-1
+    lo = std::max(lo, to.nlow + 2);
+    hi = std::min(hi, n + to.ref_shift - to.nhigh + 1);
+    // inner / interpolation rows at the outer taps h = i-1+off .. i-1+off+len-1
+    for (int t : {ti_i, tp_i}) {
+      const mol_optab& tb = d_->tabs_h[t];
+      lo = std::max(lo, tb.nlow + 2 - to.off);
+      hi = std::min(hi, n + tb.ref_shift - tb.nhigh + 2 - to.off - to.len);
+    }
+  };
+  switch (s.kind) {
+    case MOL_SLOT_CENTERED: node_tab(s.tab[0]); break;
+    case MOL_SLOT_UPWIND: node_tab(s.tab[0]); node_tab(s.tab[1]); break;
+    case MOL_SLOT_MIXED: node_tab(ax == s.axis ? s.tab[0] : s.tab[1]); break;
+    case MOL_SLOT_NONLINLAP: half_tabs(s.tab[0], s.tab[1], s.tab[2]); break;
+    case MOL_SLOT_SPHERICAL:
+      node_tab(s.tab[0]);
+      node_tab(s.tab[1]);
+      half_tabs(s.tab[2], s.tab[3], s.tab[4]);
+      break;
+    default: break;
   }
 }
 
-const char* unary_fn(uint32_t op) {
-  switch (op) {
+int Gen::leaf(uint32_t code, int imm, const Ctx& c) {
+  switch (code) {
+    case MOL_OP_CONST: return lit_node(d_->cpool_h[imm]);
+    case MOL_OP_TIME: return expr("(T)t");
+    case MOL_OP_PARAM: return expr("P[" + istr(imm) + "]");
+    default: break;
+  }
+  if (c.kind == CTX_BOUNDARY) {
+    // ghost-node program G(t, coordinates of the boundary node): no variables
+    const std::string q[3] = {"p0", "p1", "p2"};
+    std::string idx[3] = {q[0], q[1], q[2]};
+    idx[c.ax] = c.side == 0 ? "1" : istr(g_.n[c.ax]);
+    switch (code) {
+      case MOL_OP_COORD: return expr("(T)jp.coord[" + istr(imm) + "][" + idx[imm] + " - 1]");
+      case MOL_OP_AUX:
+        return expr("jp.aux[" + istr(imm) + "][(i64)(" + idx[0] + " - 1) + (i64)" + istr(g_.n[0]) + " * ((" +
+                    idx[1] + " - 1) + (i64)" + istr(g_.n[1]) + " * (" + idx[2] + " - 1))]");
+      default: return lit_node(0.0);
+    }
+  }
+  if (c.kind == CTX_HALF) {
+    switch (code) {
+      case MOL_OP_FIELD: return apply(*c.ip, imm, c.ax);
+      case MOL_OP_AUX: return apply(*c.ip, d_->nfields + imm, c.ax);
+      case MOL_OP_COORD: {
+        if (imm != c.ax) return expr(coord_node(imm));
+        if (!c.ip->valid) return lit_node(0.0);
+        std::string e;
+        for (size_t k = 0; k < c.ip->w.size(); ++k) {
+          const std::string term = lit(c.ip->w[k]) + " * " + coord_tap(imm, c.ip->rel[k]);
+          e = k == 0 ? term : e + " + " + term;
+        }
+        return e.empty() ? lit_node(0.0) : expr("(" + e + ")");
+      }
+      case MOL_OP_DINNER: return apply(*c.dp, c.dfield, c.ax);
+      case MOL_OP_SLOT: return lit_node(0.0);
+      default: bad("unknown leaf opcode"); return lit_node(0.0);
+    }
+  }
+  switch (code) {
+    case MOL_OP_FIELD: return lds(imm, 0);
+    case MOL_OP_AUX: return lds(d_->nfields + imm, 0);
+    case MOL_OP_COORD: return expr(coord_node(imm));
+    case MOL_OP_SLOT: return c.kind == CTX_NODE ? slot(imm) : lit_node(0.0);
+    case MOL_OP_DINNER: return lit_node(0.0);
+    default: bad("unknown leaf opcode"); return lit_node(0.0);
+  }
+}
+
+bool known_unary(uint32_t code) {
+  switch (code) {
+    case MOL_OP_NEG: case MOL_OP_ABS: case MOL_OP_SQRT: case MOL_OP_EXP: case MOL_OP_LOG:
+    case MOL_OP_SIN: case MOL_OP_COS: case MOL_OP_TAN: case MOL_OP_TANH: case MOL_OP_SINH:
+    case MOL_OP_COSH: case MOL_OP_ATAN: case MOL_OP_ASIN: case MOL_OP_ACOS: case MOL_OP_SIGN:
+    case MOL_OP_POWI: case MOL_OP_ERF: case MOL_OP_LOG10: case MOL_OP_CBRT: case MOL_OP_INV:
+    case MOL_OP_SQ: case MOL_OP_EXPM1: case MOL_OP_LOG1P: case MOL_OP_FLOOR: case MOL_OP_CEIL:
+    case MOL_OP_HEAVISIDE:
+      return true;
+    default:
+      return false;
+  }
+}
+
+bool known_binary(uint32_t code) { return code >= MOL_OP_ADD && code <= MOL_OP_OR; }
+
+// Translation of a stack program into IR (same evaluation order as stack_prog).
+int Gen::prog(int p, const Ctx& c) {
+  const int start = d_->prog_h[2 * p], len = d_->prog_h[2 * p + 1];
+  std::vector<int> st;
+  auto pop = [&]() {
+    const int v = st.back();
+    st.pop_back();
+    return v;
+  };
+  for (int pc = start; pc < start + len; ++pc) {
+    const uint32_t ins = d_->code_h[pc];
+    const uint32_t code = ins & 0xffu;
+    const bool bconst = (ins >> 20) & 1u;
+    const int imm = (int)(ins >> 21);
+    if (code < 16u) {
+      if (code > MOL_OP_DINNER) { bad("opcode " + istr(code) + " not supported by the generator"); return lit_node(0); }
+      st.push_back(leaf(code, imm, c));
+      continue;
+    }
+    if (code == MOL_OP_SELECT) {
+      if (bconst) {
+        if (st.size() < 2) { bad("stack underflow"); return lit_node(0); }
+        const int s0 = pop(), s1 = pop();
+        st.push_back(op(X_SELECT, {s1, s0, lit_node(d_->cpool_h[imm])}));
+      } else {
+        if (st.size() < 3) { bad("stack underflow"); return lit_node(0); }
+        const int s0 = pop(), s1 = pop(), s2 = pop();
+        st.push_back(op(X_SELECT, {s2, s1, s0}));
+      }
+      continue;
+    }
+    if (code < 48u) {
+      if (!known_binary(code)) { bad("opcode " + istr(code) + " not supported by the generator"); return lit_node(0); }
+      int a, b;
+      if (bconst) {
+        if (st.empty()) { bad("stack underflow"); return lit_node(0); }
+        a = pop();
+        b = lit_node(d_->cpool_h[imm]);
+      } else {
+        if (st.size() < 2) { bad("stack underflow"); return lit_node(0); }
+        b = pop();
+        a = pop();
+      }
+      st.push_back(op((int)code, {a, b}));
+      continue;
+    }
+    if (!known_unary(code)) { bad("opcode " + istr(code) + " not supported by the generator"); return lit_node(0); }
+    if (st.empty()) { bad("stack underflow"); return lit_node(0); }
+    const int a = pop();
+    st.push_back(op((int)code, {a}, code == MOL_OP_POWI ? ((imm & 0x400) ? imm - 0x800 : imm) : 0));
+  }
+  if (st.size() != 1) { bad("program leaves " + istr((long long)st.size()) + " values"); return lit_node(0); }
+  return st.back();
+}
+
+// Nonlinear Laplacian at a node: outer half-point operator at index i-1, the coefficient
+// program and the inner derivative at each outer tap h
+// (src/discretization/schemes/nonlinear_laplacian/nonlinear_laplacian.jl:20-83).
+int Gen::nonlinlap(const mol_slot& s, int qo, int qi, int qp, const Var& v) {
+  const RowR ro = row(d_->tabs_h[s.tab[qo]], s.axis, v, -1);
+  if (!ro.valid) return lit_node(0.0);
+  std::vector<int> terms;
+  for (size_t k = 0; k < ro.w.size(); ++k) {
+    const int h = ro.rel[k];   // half point between node+h and node+h+1
+    const RowR rp = row(d_->tabs_h[s.tab[qp]], s.axis, v, h);
+    const RowR rd = row(d_->tabs_h[s.tab[qi]], s.axis, v, h);
+    Ctx c;
+    c.kind = CTX_HALF;
+    c.ax = s.axis;
+    c.ip = &rp;
+    c.dp = &rd;
+    c.dfield = s.field;
+    const int di = leaf(MOL_OP_DINNER, 0, c);
+    terms.push_back(s.prog >= 0 ? op(MOL_OP_MUL, {prog(s.prog, c), di}) : di);
+  }
+  return lincomb(ro.w, terms);
+}
+
+// Value of slot k for the node-index variants v[axis].
+int Gen::slot_body(int k, const Var* v) {
+  const mol_slot& s = d_->slots_h[k];
+  const int ax = s.axis;
+  Ctx plain;
+  plain.kind = CTX_PLAIN;
+  switch (s.kind) {
+    case MOL_SLOT_CENTERED:
+      return apply(row(d_->tabs_h[s.tab[0]], ax, v[ax], 0), s.field, ax);
+    case MOL_SLOT_UPWIND: {
+      if (s.prog < 0) return apply(row(d_->tabs_h[s.tab[1]], ax, v[ax], 0), s.field, ax);
+      const int coef = prog(s.prog, plain);
+      const int back = apply(row(d_->tabs_h[s.tab[1]], ax, v[ax], 0), s.field, ax);
+      const int fwd = apply(row(d_->tabs_h[s.tab[0]], ax, v[ax], 0), s.field, ax);
+      const int cond = op(MOL_OP_GT, {coef, lit_node(0.0)});
+      return op(MOL_OP_MUL, {coef, op(X_SELECT, {cond, back, fwd})});
+    }
+    case MOL_SLOT_WENO: {
+      const long long sd = stride(ax);
+      Node n;
+      n.kind = N_OP;
+      n.op = X_WENO;
+      for (int q = -2; q <= 2; ++q) n.a.push_back(lds(s.field, q * sd));
+      n.w = {s.eps, s.inv_dx};
+      return add(n);
+    }
+    case MOL_SLOT_NONLINLAP:
+      return nonlinlap(s, 0, 1, 2, v[ax]);
+    case MOL_SLOT_SPHERICAL: {
+      const int a = s.prog >= 0 ? prog(s.prog, plain) : lit_node(1.0);
+      const int rr = expr(coord_node(ax));
+      const int d2 = apply(row(d_->tabs_h[s.tab[1]], ax, v[ax], 0), s.field, ax);
+      const int d1 = apply(row(d_->tabs_h[s.tab[0]], ax, v[ax], 0), s.field, ax);
+      const int nl = nonlinlap(s, 2, 3, 4, v[ax]);
+      // r ~ 0: 6 a D2 u (src/discretization/schemes/spherical_laplacian/spherical_laplacian.jl:30-32)
+      const int at0 = op(MOL_OP_MUL, {op(MOL_OP_MUL, {lit_node(6.0), a}), d2});
+      const int gen = op(MOL_OP_MUL, {a, op(MOL_OP_ADD, {op(MOL_OP_DIV, {d1, rr}), nl})});
+      return op(X_SELECT, {op(X_SPHCOND, {rr}), at0, gen});
+    }
+    case MOL_SLOT_MIXED: {
+      const RowR rx = row(d_->tabs_h[s.tab[0]], ax, v[ax], 0);
+      const RowR ry = row(d_->tabs_h[s.tab[1]], s.axis2, v[s.axis2], 0);
+      if (!rx.valid || !ry.valid) return lit_node(0.0);
+      std::vector<double> w;
+      std::vector<int> args;
+      for (size_t i = 0; i < rx.w.size(); ++i)
+        for (size_t j = 0; j < ry.w.size(); ++j) {
+          w.push_back(rx.w[i] * ry.w[j]);
+          args.push_back(lds(s.field, (long long)rx.rel[i] * stride(ax) + (long long)ry.rel[j] * stride(s.axis2)));
+        }
+      return lincomb(w, args);
+    }
+    default:
+      bad("slot kind " + istr(s.kind) + " not supported by the generator");
+      return lit_node(0.0);
+  }
+}
+
+// Slot k as branch nodes over the node-index variants of its axes.
+int Gen::slot_branch(int k, const std::vector<int>& axes, size_t a, Var* v) {
+  if (a == axes.size()) return slot_body(k, v);
+  const int ax = axes[a];
+  int lo, hi;
+  slot_range(d_->slots_h[k], ax, lo, hi);
+  std::vector<int> cases;
+  for (int i = g_.ulo[ax]; i <= g_.uhi[ax]; ++i)
+    if (i < lo || i > hi) cases.push_back(i);
+  v[ax] = Var{};
+  if (cases.empty()) return slot_branch(k, axes, a + 1, v);
+  Node br;
+  br.kind = N_BRANCH;
+  br.ax = ax;
+  br.lo = lo;
+  br.hi = hi;
+  const int parent = cur_;
+  // interior region
+  {
+    const int r = new_region();
+    cur_ = r;
+    regs_[r].result = slot_branch(k, axes, a + 1, v);
+    br.reg.push_back(r);
+    br.cval.push_back(0);
+  }
+  for (int c : cases) {
+    const int r = new_region();
+    cur_ = r;
+    v[ax] = Var{false, c};
+    regs_[r].result = slot_branch(k, axes, a + 1, v);
+    br.reg.push_back(r);
+    br.cval.push_back(c);
+  }
+  v[ax] = Var{};
+  cur_ = parent;
+  return add(br);
+}
+
+int Gen::slot(int k) {
+  auto it = slot_node_.find(k);
+  if (it != slot_node_.end()) return it->second;
+  const mol_slot& s = d_->slots_h[k];
+  Var v[3];
+  int r;
+  if (core_ || s.kind == MOL_SLOT_WENO) {
+    r = slot_body(k, v);
+  } else {
+    std::vector<int> axes = {s.axis};
+    if (s.kind == MOL_SLOT_MIXED) axes.push_back(s.axis2);
+    r = slot_branch(k, axes, 0, v);
+  }
+  slot_node_[k] = r;
+  return r;
+}
+
+void Gen::build_node(bool core, std::vector<int>& outs, int& region) {
+  core_ = core;
+  slot_node_.clear();
+  region = new_region();
+  cur_ = region;
+  Ctx node;
+  node.kind = CTX_NODE;
+  outs.clear();
+  for (int f = 0; f < d_->nfields; ++f) outs.push_back(prog(d_->rhs_prog[f], node));
+}
+
+// ---------------------------------------------------------------------------------------
+// printing
+
+std::string Gen::ref(int id) const {
+  const Node& n = nodes_[id];
+  switch (n.kind) {
+    case N_LIT: return lit(n.val);
+    case N_LDS: {
+      const long long o = (long long)n.arr * g_.vol + n.off;
+      if (o == 0) return "L[b]";
+      return o > 0 ? "L[b + " + istr(o) + "]" : "L[b - " + istr(-o) + "]";
+    }
+    case N_EXPR: return n.text;
+    default: return "v" + istr(id);
+  }
+}
+
+const char* unary_fn(uint32_t code) {
+  switch (code) {
     case MOL_OP_ABS: return "fabs";
     case MOL_OP_SQRT: return "sqrt";
     case MOL_OP_EXP: return "exp";
@@ -188,232 +594,479 @@ const char* unary_fn(uint32_t op) {
   }
 }
 
-std::string binexpr(uint32_t op, const std::string& a, const std::string& b) {
-  switch (op) {
-    case MOL_OP_ADD: return "(" + a + " + " + b + ")";
-    case MOL_OP_SUB: return "(" + a + " - " + b + ")";
-    case MOL_OP_MUL: return "(" + a + " * " + b + ")";
-    case MOL_OP_DIV: return "(" + a + " / " + b + ")";
-    case MOL_OP_POW: return "pow(" + a + ", " + b + ")";
-    case MOL_OP_MIN: return "fmin(" + a + ", " + b + ")";
-    case MOL_OP_MAX: return "fmax(" + a + ", " + b + ")";
-    case MOL_OP_LT: return "(" + a + " < " + b + " ? T(1) : T(0))";
-    case MOL_OP_LE: return "(" + a + " <= " + b + " ? T(1) : T(0))";
-    case MOL_OP_GT: return "(" + a + " > " + b + " ? T(1) : T(0))";
-    case MOL_OP_GE: return "(" + a + " >= " + b + " ? T(1) : T(0))";
-    case MOL_OP_EQ: return "(" + a + " == " + b + " ? T(1) : T(0))";
-    case MOL_OP_NE: return "(" + a + " != " + b + " ? T(1) : T(0))";
-    case MOL_OP_AND: return "((" + a + " != T(0) && " + b + " != T(0)) ? T(1) : T(0))";
-    case MOL_OP_OR: return "((" + a + " != T(0) || " + b + " != T(0)) ? T(1) : T(0))";
-    default: return "";
-  }
-}
-
-// Straight-line translation of a stack program (same evaluation order as stack_prog).
-std::string Gen::prog(int p, const Ctx& c) {
-  const int start = d_->prog_h[2 * p], len = d_->prog_h[2 * p + 1];
-  std::vector<std::string> st;
-  for (int pc = start; pc < start + len; ++pc) {
-    const uint32_t ins = d_->code_h[pc];
-    const uint32_t op = ins & 0xffu;
-    const bool bconst = (ins >> 20) & 1u;
-    const int imm = (int)(ins >> 21);
-    if (op < 16u) {
-      if (op > MOL_OP_DINNER) { bad("opcode " + istr(op) + " not supported by the generator"); return "T(0)"; }
-      st.push_back(leaf(op, imm, c));
-      continue;
-    }
-    if (op < 48u) {
-      if (op == MOL_OP_SELECT) {
-        if (bconst) {
-          if (st.size() < 2) { bad("stack underflow"); return "T(0)"; }
-          const std::string s0 = st.back(); st.pop_back();
-          const std::string s1 = st.back(); st.pop_back();
-          st.push_back(emit("(" + s1 + " != T(0)) ? " + s0 + " : " + lit(d_->cpool_h[imm])));
-        } else {
-          if (st.size() < 3) { bad("stack underflow"); return "T(0)"; }
-          const std::string s0 = st.back(); st.pop_back();
-          const std::string s1 = st.back(); st.pop_back();
-          const std::string s2 = st.back(); st.pop_back();
-          st.push_back(emit("(" + s2 + " != T(0)) ? " + s1 + " : " + s0));
-        }
-        continue;
-      }
-      std::string a, b;
-      if (bconst) {
-        if (st.empty()) { bad("stack underflow"); return "T(0)"; }
-        a = st.back(); st.pop_back();
-        b = lit(d_->cpool_h[imm]);
-      } else {
-        if (st.size() < 2) { bad("stack underflow"); return "T(0)"; }
-        b = st.back(); st.pop_back();
-        a = st.back(); st.pop_back();
-      }
-      const std::string e = binexpr(op, a, b);
-      if (e.empty()) { bad("opcode " + istr(op) + " not supported by the generator"); return "T(0)"; }
-      st.push_back(emit(e));
-      continue;
-    }
-    if (st.empty()) { bad("stack underflow"); return "T(0)"; }
-    const std::string a = st.back(); st.pop_back();
-    std::string e;
-    if (op == MOL_OP_NEG) e = "(-" + a + ")";
-    else if (op == MOL_OP_SQ) e = "(" + a + " * " + a + ")";
-    else if (op == MOL_OP_INV) e = "(T(1) / " + a + ")";
-    else if (op == MOL_OP_POWI) {
-      const int ex = (imm & 0x400) ? imm - 0x800 : imm;
-      const int m = ex < 0 ? -ex : ex;
-      std::string r;
-      if (m == 0) r = "T(1)";
-      else if (m == 1) r = a;
-      else if (m == 2) r = "(" + a + " * " + a + ")";
-      else if (m == 3) r = "(" + a + " * " + a + " * " + a + ")";
-      e = m > 3 ? "mol_powi(" + a + ", " + istr(ex) + ")" : (ex < 0 ? "(T(1) / " + r + ")" : r);
-    } else {
-      const char* fn = unary_fn(op);
-      if (!fn) { bad("opcode " + istr(op) + " not supported by the generator"); return "T(0)"; }
-      e = std::string(fn) + "(" + a + ")";
-    }
-    st.push_back(emit(e));
-  }
-  if (st.size() != 1) { bad("program leaves " + istr((long long)st.size()) + " values"); return "T(0)"; }
-  return st.back();
-}
-
-// Nonlinear Laplacian at an interior node: outer half-point operator at index i-1, the
-// coefficient program and the inner derivative at each outer tap h
-// (src/discretization/schemes/nonlinear_laplacian/nonlinear_laplacian.jl:20-83).
-std::string Gen::nonlinlap(const mol_slot& s, int qo, int qi, int qp) {
-  const mol_optab& to = d_->tabs_h[s.tab[qo]];
-  Ctx c;
-  c.kind = CTX_HALF;
-  c.ax = s.axis;
-  c.dp = &d_->tabs_h[s.tab[qi]];
-  c.ip = &d_->tabs_h[s.tab[qp]];
-  c.dfield = s.field;
-  std::string acc;
-  for (int k = 0; k < to.len; ++k) {
-    c.h = to.off + k - 1;
-    const std::string a = s.prog >= 0 ? prog(s.prog, c) : "";
-    const std::string di = leaf(MOL_OP_DINNER, 0, c);
-    const std::string term = emit(lit(d_->pool_h[to.w + k]) + " * " + (a.empty() ? di : "(" + a + " * " + di + ")"));
-    acc = k == 0 ? term : emit(acc + " + " + term);
-  }
-  return acc.empty() ? "T(0)" : acc;
-}
-
-std::string Gen::slot(int k) {
-  auto it = slot_var_.find(k);
-  if (it != slot_var_.end()) return it->second;
-  const mol_slot& s = d_->slots_h[k];
-  const int ax = s.axis;
-  Ctx plain;
-  plain.kind = CTX_PLAIN;
+std::string powi_expr(const std::string& a, int ex) {
+  const int m = ex < 0 ? -ex : ex;
   std::string r;
-  switch (s.kind) {
-    case MOL_SLOT_CENTERED:
-      r = stencil(d_->tabs_h[s.tab[0]], s.field, ax);
-      break;
-    case MOL_SLOT_UPWIND: {
-      if (s.prog < 0) {
-        r = stencil(d_->tabs_h[s.tab[1]], s.field, ax);
-        break;
-      }
-      const std::string coef = prog(s.prog, plain);
-      const std::string back = stencil(d_->tabs_h[s.tab[1]], s.field, ax);
-      const std::string fwd = stencil(d_->tabs_h[s.tab[0]], s.field, ax);
-      r = emit(coef + " * (" + coef + " > T(0) ? " + back + " : " + fwd + ")");
-      break;
-    }
-    case MOL_SLOT_WENO: {
-      const int sd = stride(ax);
-      // FP64: one-division cleared form; FP32: normalised weights (the cleared form underflows)
-      r = emit(std::string(d_->dtype == MOL_F64 ? "mol_weno5(" : "mol_weno5_ref(") + at(s.field, -2LL * sd) + ", " + at(s.field, -sd) + ", " + at(s.field, 0) + ", " +
-               at(s.field, sd) + ", " + at(s.field, 2LL * sd) + ", " + lit(s.eps) + ", " + lit(s.inv_dx) + ")");
-      break;
-    }
-    case MOL_SLOT_NONLINLAP:
-      r = nonlinlap(s, 0, 1, 2);
-      break;
-    case MOL_SLOT_SPHERICAL: {
-      const std::string a = s.prog >= 0 ? prog(s.prog, plain) : "T(1)";
-      const std::string rr = emit(coord_node(ax));
-      const std::string d2 = stencil(d_->tabs_h[s.tab[1]], s.field, ax);
-      const std::string d1 = stencil(d_->tabs_h[s.tab[0]], s.field, ax);
-      const std::string nl = nonlinlap(s, 2, 3, 4);
-      // r ~ 0: 6 a D2 u (src/discretization/schemes/spherical_laplacian/spherical_laplacian.jl:30-32)
-      r = emit("(fabs((double)" + rr + ") <= 1e-6) ? T(6) * " + a + " * " + d2 + " : " + a + " * (" + d1 + " / " +
-               rr + " + " + nl + ")");
-      break;
-    }
-    case MOL_SLOT_MIXED: {
-      const mol_optab& tx = d_->tabs_h[s.tab[0]];
-      const mol_optab& ty = d_->tabs_h[s.tab[1]];
-      std::string acc;
-      for (int i = 0; i < tx.len; ++i) {
-        std::string sy;
-        for (int j = 0; j < ty.len; ++j) {
-          const double w = d_->pool_h[tx.w + i] * d_->pool_h[ty.w + j];
-          const long long off = (long long)(tx.off + i) * stride(ax) + (long long)(ty.off + j) * stride(s.axis2);
-          const std::string term = lit(w) + " * " + at(s.field, off);
-          sy = j == 0 ? term : sy + " + " + term;
-        }
-        const std::string v = emit(sy.empty() ? "T(0)" : sy);
-        acc = i == 0 ? v : emit(acc + " + " + v);
-      }
-      r = acc.empty() ? "T(0)" : acc;
-      break;
-    }
-    default:
-      bad("slot kind " + istr(s.kind) + " not supported by the generator");
-      r = "T(0)";
-  }
-  slot_var_[k] = r;
-  return r;
+  if (m == 0) r = "T(1)";
+  else if (m == 1) r = a;
+  else if (m == 2) r = "(" + a + " * " + a + ")";
+  else if (m == 3) r = "(" + a + " * " + a + " * " + a + ")";
+  else return "mol_powi(" + a + ", " + istr(ex) + ")";
+  return ex < 0 ? "(T(1) / " + r + ")" : r;
 }
 
-std::string Gen::source(bool f64) {
+std::string Gen::op_expr(const Node& n, int id) const {
+  (void)id;
+  auto A = [&](int k) { return ref(n.a[k]); };
+  switch (n.op) {
+    case MOL_OP_ADD: return "(" + A(0) + " + " + A(1) + ")";
+    case MOL_OP_SUB: return "(" + A(0) + " - " + A(1) + ")";
+    case MOL_OP_MUL: return "(" + A(0) + " * " + A(1) + ")";
+    case MOL_OP_DIV: return "(" + A(0) + " / " + A(1) + ")";
+    case MOL_OP_POW: return "pow(" + A(0) + ", " + A(1) + ")";
+    case MOL_OP_MIN: return "fmin(" + A(0) + ", " + A(1) + ")";
+    case MOL_OP_MAX: return "fmax(" + A(0) + ", " + A(1) + ")";
+    case MOL_OP_LT: return "(" + A(0) + " < " + A(1) + " ? T(1) : T(0))";
+    case MOL_OP_LE: return "(" + A(0) + " <= " + A(1) + " ? T(1) : T(0))";
+    case MOL_OP_GT: return "(" + A(0) + " > " + A(1) + " ? T(1) : T(0))";
+    case MOL_OP_GE: return "(" + A(0) + " >= " + A(1) + " ? T(1) : T(0))";
+    case MOL_OP_EQ: return "(" + A(0) + " == " + A(1) + " ? T(1) : T(0))";
+    case MOL_OP_NE: return "(" + A(0) + " != " + A(1) + " ? T(1) : T(0))";
+    case MOL_OP_AND: return "((" + A(0) + " != T(0) && " + A(1) + " != T(0)) ? T(1) : T(0))";
+    case MOL_OP_OR: return "((" + A(0) + " != T(0) || " + A(1) + " != T(0)) ? T(1) : T(0))";
+    case X_SELECT: return "(" + A(0) + " != T(0) ? " + A(1) + " : " + A(2) + ")";
+    case X_SPHCOND: return "(fabs((double)" + A(0) + ") <= 1e-6 ? T(1) : T(0))";
+    case MOL_OP_NEG: return "(-" + A(0) + ")";
+    case MOL_OP_SQ: return "(" + A(0) + " * " + A(0) + ")";
+    case MOL_OP_INV: return "(T(1) / " + A(0) + ")";
+    case MOL_OP_POWI: return powi_expr(A(0), n.imm);
+    case X_LINCOMB: {
+      std::string e;
+      for (size_t k = 0; k < n.a.size(); ++k) {
+        const std::string term = lit(n.w[k]) + " * " + A((int)k);
+        e = k == 0 ? term : e + " + " + term;
+      }
+      return e;
+    }
+    case X_WENO:
+      // FP64: one-division cleared form; FP32: normalised weights (the cleared form underflows)
+      return std::string(d_->dtype == MOL_F64 ? "mol_weno5(" : "mol_weno5_ref(") + A(0) + ", " + A(1) + ", " +
+             A(2) + ", " + A(3) + ", " + A(4) + ", " + lit(n.w[0]) + ", " + lit(n.w[1]) + ")";
+    default: {
+      const char* fn = unary_fn(n.op);
+      return fn ? std::string(fn) + "(" + A(0) + ")" : "T(0)";
+    }
+  }
+}
+
+void Gen::print_node(std::ostream& o, int id) const {
+  const Node& n = nodes_[id];
+  if (n.kind == N_OP) {
+    o << "    const T v" << id << " = " << op_expr(n, id) << ";\n";
+    return;
+  }
+  if (n.kind != N_BRANCH) return;
+  const std::string v = "v" + istr(id), I = "I" + istr(n.ax);
+  o << "    T " << v << ";\n";
+  o << "    if (" << I << " >= " << n.lo << " && " << I << " <= " << n.hi << ") {\n";
+  print_region(o, n.reg[0]);
+  o << "    " << v << " = " << ref(regs_[n.reg[0]].result) << ";\n";
+  o << "    } else switch (" << I << ") {\n";
+  for (size_t k = 1; k < n.reg.size(); ++k) {
+    o << "    case " << n.cval[k] << ": {\n";
+    print_region(o, n.reg[k]);
+    o << "    " << v << " = " << ref(regs_[n.reg[k]].result) << ";\n    break; }\n";
+  }
+  o << "    default: " << v << " = T(0);\n    }\n";
+}
+
+void Gen::print_region(std::ostream& o, int r) const {
+  for (int id : regs_[r].nodes) print_node(o, id);
+}
+
+// Partial derivatives of a node with respect to its operands, times the node's adjoint g<id>.
+void Gen::partials(const Node& n, int id, std::vector<std::pair<int, std::string>>& out) const {
+  const std::string g = "g" + istr(id), v = "v" + istr(id);
+  auto A = [&](int k) { return ref(n.a[k]); };
+  auto dep = [&](int k) { return nodes_[n.a[k]].kind == N_LDS || nodes_[n.a[k]].kind == N_OP || nodes_[n.a[k]].kind == N_BRANCH; };
+  auto put = [&](int k, const std::string& e) {
+    if (dep(k)) out.push_back({n.a[k], e});
+  };
+  switch (n.op) {
+    case MOL_OP_ADD: put(0, g); put(1, g); break;
+    case MOL_OP_SUB: put(0, g); put(1, "(-" + g + ")"); break;
+    case MOL_OP_MUL: put(0, g + " * " + A(1)); put(1, g + " * " + A(0)); break;
+    case MOL_OP_DIV: put(0, g + " / " + A(1)); put(1, "(-" + g + " * " + v + " / " + A(1) + ")"); break;
+    case MOL_OP_POW:
+      put(0, g + " * " + A(1) + " * pow(" + A(0) + ", " + A(1) + " - T(1))");
+      put(1, g + " * " + v + " * log(" + A(0) + ")");
+      break;
+    case MOL_OP_MIN:
+      put(0, "(" + A(0) + " <= " + A(1) + " ? " + g + " : T(0))");
+      put(1, "(" + A(0) + " <= " + A(1) + " ? T(0) : " + g + ")");
+      break;
+    case MOL_OP_MAX:
+      put(0, "(" + A(0) + " >= " + A(1) + " ? " + g + " : T(0))");
+      put(1, "(" + A(0) + " >= " + A(1) + " ? T(0) : " + g + ")");
+      break;
+    case X_SELECT:
+      put(1, "(" + A(0) + " != T(0) ? " + g + " : T(0))");
+      put(2, "(" + A(0) + " != T(0) ? T(0) : " + g + ")");
+      break;
+    case MOL_OP_NEG: put(0, "(-" + g + ")"); break;
+    case MOL_OP_SQ: put(0, "T(2) * " + A(0) + " * " + g); break;
+    case MOL_OP_INV: put(0, "(-" + g + " * " + v + " * " + v + ")"); break;
+    case MOL_OP_ABS: put(0, g + " * mol_sign(" + A(0) + ")"); break;
+    case MOL_OP_SQRT: put(0, g + " * T(0.5) / " + v); break;
+    case MOL_OP_EXP: put(0, g + " * " + v); break;
+    case MOL_OP_LOG: put(0, g + " / " + A(0)); break;
+    case MOL_OP_SIN: put(0, g + " * cos(" + A(0) + ")"); break;
+    case MOL_OP_COS: put(0, "(-" + g + " * sin(" + A(0) + "))"); break;
+    case MOL_OP_TAN: put(0, g + " * (T(1) + " + v + " * " + v + ")"); break;
+    case MOL_OP_TANH: put(0, g + " * (T(1) - " + v + " * " + v + ")"); break;
+    case MOL_OP_SINH: put(0, g + " * cosh(" + A(0) + ")"); break;
+    case MOL_OP_COSH: put(0, g + " * sinh(" + A(0) + ")"); break;
+    case MOL_OP_ATAN: put(0, g + " / (T(1) + " + A(0) + " * " + A(0) + ")"); break;
+    case MOL_OP_ASIN: put(0, g + " / sqrt(T(1) - " + A(0) + " * " + A(0) + ")"); break;
+    case MOL_OP_ACOS: put(0, "(-" + g + " / sqrt(T(1) - " + A(0) + " * " + A(0) + "))"); break;
+    case MOL_OP_ERF: put(0, g + " * T(0x1.20dd750429b6dp+0) * exp(-" + A(0) + " * " + A(0) + ")"); break;
+    case MOL_OP_LOG10: put(0, g + " / (" + A(0) + " * T(0x1.26bb1bbb55516p+1))"); break;
+    case MOL_OP_CBRT: put(0, g + " / (T(3) * " + v + " * " + v + ")"); break;
+    case MOL_OP_EXPM1: put(0, g + " * (" + v + " + T(1))"); break;
+    case MOL_OP_LOG1P: put(0, g + " / (T(1) + " + A(0) + ")"); break;
+    case MOL_OP_POWI:
+      if (n.imm != 0) put(0, g + " * T(" + istr(n.imm) + ") * " + powi_expr(A(0), n.imm - 1));
+      break;
+    case X_LINCOMB:
+      for (size_t k = 0; k < n.a.size(); ++k) put((int)k, lit(n.w[k]) + " * " + g);
+      break;
+    case X_WENO:
+      for (int k = 0; k < 5; ++k) put(k, g + " * wd" + istr(id) + "[" + istr(k) + "]");
+      break;
+    default: break;   // comparisons, sign, floor, ...: zero derivative
+  }
+}
+
+// Reverse sweep of region r: `seeds` are (node, adjoint expression) pairs of its outputs.
+// Adjoints of LDS leaves are summed per position and added to the LDS adjoint buffers
+// (Ab for variables, Aa in FP64 for auxiliary arrays) with LDS atomics.
+void Gen::print_reverse(std::ostream& o, int r, const std::vector<std::pair<int, std::string>>& seeds) const {
+  const Region& R = regs_[r];
+  std::map<std::pair<int, long long>, std::string> acc;
+  for (int id : R.nodes) o << "    T g" << id << " = T(0);\n";
+  auto add_adj = [&](int target, const std::string& e) {
+    const Node& t = nodes_[target];
+    if (t.kind == N_OP || t.kind == N_BRANCH) {
+      o << "    g" << target << " += " << e << ";\n";
+    } else if (t.kind == N_LDS) {
+      std::string& s = acc[{t.arr, t.off}];
+      s = s.empty() ? e : s + " + " + e;
+    }
+  };
+  for (const auto& sd : seeds) add_adj(sd.first, sd.second);
+  for (auto it = R.nodes.rbegin(); it != R.nodes.rend(); ++it) {
+    const int id = *it;
+    const Node& n = nodes_[id];
+    if (n.kind == N_OP) {
+      if (n.op == X_WENO) {
+        o << "    T wd" << id << "[5];\n    mol_weno5_grad(" << ref(n.a[0]) << ", " << ref(n.a[1]) << ", " << ref(n.a[2])
+          << ", " << ref(n.a[3]) << ", " << ref(n.a[4]) << ", " << lit(n.w[0]) << ", " << lit(n.w[1]) << ", wd" << id
+          << ", " << (d_->dtype == MOL_F64 ? "true" : "false") << ");\n";
+      }
+      std::vector<std::pair<int, std::string>> ps;
+      partials(n, id, ps);
+      for (const auto& p : ps) add_adj(p.first, p.second);
+      continue;
+    }
+    // branch: the region of the taken case runs forward again, then backwards
+    const std::string I = "I" + istr(n.ax), g = "g" + istr(id);
+    o << "    if (" << I << " >= " << n.lo << " && " << I << " <= " << n.hi << ") {\n";
+    print_region(o, n.reg[0]);
+    print_reverse(o, n.reg[0], {{regs_[n.reg[0]].result, g}});
+    o << "    } else switch (" << I << ") {\n";
+    for (size_t k = 1; k < n.reg.size(); ++k) {
+      o << "    case " << n.cval[k] << ": {\n";
+      print_region(o, n.reg[k]);
+      print_reverse(o, n.reg[k], {{regs_[n.reg[k]].result, g}});
+      o << "    break; }\n";
+    }
+    o << "    default: break;\n    }\n";
+  }
+  for (const auto& kv : acc) {
+    const int arr = kv.first.first;
+    const long long off = kv.first.second;
+    const std::string pos = off == 0 ? "b" : (off > 0 ? "b + " + istr(off) : "b - " + istr(-off));
+    if (arr < d_->nfields)
+      o << "    atomicAdd(&Ab[" << (long long)arr * g_.vol << " + " << pos << "], " << kv.second << ");\n";
+    else
+      o << "    atomicAdd(&Aa[" << (long long)(arr - d_->nfields) * g_.vol << " + " << pos << "], (double)("
+        << kv.second << "));\n";
+  }
+}
+
+// Slow staging of one position: ring wrap, zero outside the grid and at corners, the unknown
+// inside the variable's box, else the ghost-node closure of the axis it leaves the box along
+// (field_at / closure_value of the interpreter; src/discretization/generate_bc_eqs.jl).
+std::string Gen::stage_fn() {
   std::ostringstream o;
-  o << "// generated right-hand side: core tiles\n";
-  o << "#define MOL_T " << (f64 ? "double" : "float") << "\n";
-  o << "#include \"mol_jit_prelude.h\"\n\n";
-  Ctx node;
-  node.kind = CTX_NODE;
-  std::vector<std::string> outs;
-  for (int f = 0; f < d_->nfields; ++f) outs.push_back(prog(d_->rhs_prog[f], node));
-  if (!ok_) return "";
-  int nblk = g_.ct_n[0] * g_.ct_n[1] * g_.ct_n[2];
-  o << "struct Geo {\n";
-  auto c = [&](const char* name, long long v) { o << "  static constexpr " << (std::strcmp(name, "BOX") ? "int " : "i64 ") << name << " = " << v << ";\n"; };
-  c("NF", d_->nfields); c("NAUX", d_->naux); c("NARR", d_->nfields + d_->naux);
-  c("N0", g_.n[0]); c("N1", g_.n[1]); c("N2", g_.n[2]);
-  c("PER0", g_.periodic[0]); c("PER1", g_.periodic[1]); c("PER2", g_.periodic[2]);
-  c("LO0", g_.lo[0]); c("LO1", g_.lo[1]); c("LO2", g_.lo[2]);
-  c("HI0", g_.hi[0]); c("HI1", g_.hi[1]); c("HI2", g_.hi[2]);
+  o << "  static __device__ T stage(const int f, int p0, int p1, int p2, const double t, const T* __restrict__ P,\n"
+       "                           const T* __restrict__ u, const JPtrs& jp) {\n"
+       "    (void)t; (void)P; (void)jp;\n";
+  for (int a = 0; a < 3; ++a)
+    if (g_.periodic[a]) o << "    p" << a << " = mol_wrap(p" << a << ", " << g_.n[a] << ");\n";
+  o << "    if (p0 < 1 || p0 > " << g_.n[0] << " || p1 < 1 || p1 > " << g_.n[1] << " || p2 < 1 || p2 > "
+    << g_.n[2] << ") return T(0);\n";
+  o << "    switch (f) {\n";
+  for (int f = 0; f < d_->nfields; ++f) {
+    o << "    case " << f << ": {\n";
+    o << "    const int nout = ";
+    for (int a = 0; a < 3; ++a)
+      o << (a ? " + " : "") << "(p" << a << " < " << g_.lo[f][a] << " || p" << a << " > " << g_.hi[f][a] << ")";
+    o << ";\n    if (nout == 0) return u[uidx(" << f << ", p0, p1, p2)];\n    if (nout > 1) return T(0);\n";
+    for (int a = 0; a < d_->ndim; ++a)
+      for (int side = 0; side < 2; ++side) {
+        int cid = -1;
+        for (int c = 0; c < d_->nclosures; ++c) {
+          const mol_closure& cl = d_->closures_h[c];
+          if (cl.field == f && cl.axis == a && cl.side == side) cid = c;
+        }
+        const std::string pa = "p" + istr(a);
+        o << "    if (" << pa << (side == 0 ? " < " + istr(g_.lo[f][a]) : " > " + istr(g_.hi[f][a])) << ") {\n";
+        if (cid < 0) {
+          o << "    return T(0);\n    }\n";
+          continue;
+        }
+        const mol_closure& cl = d_->closures_h[cid];
+        const int first = side == 0 ? g_.lo[f][a] : g_.hi[f][a];
+        const int dir = side == 0 ? 1 : -1;
+        std::string G;
+        if (cl.gprog >= 0) {
+          const int r = new_region();
+          cur_ = r;
+          Ctx bc;
+          bc.kind = CTX_BOUNDARY;
+          bc.ax = a;
+          bc.side = side;
+          regs_[r].result = prog(cl.gprog, bc);
+          print_region(o, r);
+          G = ref(regs_[r].result);
+        }
+        o << "    const int g = " << (side == 0 ? istr(g_.lo[f][a] - 1) + " - " + pa : pa + " - " + istr(g_.hi[f][a] + 1))
+          << ";\n    switch (g) {\n";
+        for (int gi = 0; gi < cl.nghost; ++gi) {
+          std::string e;
+          for (int m = 0; m < cl.K; ++m) {
+            const double c = d_->pool_h[cl.wC + gi * cl.K + m];
+            if (c == 0.0) continue;
+            std::string q[3] = {"p0", "p1", "p2"};
+            q[a] = istr(first + m * dir);
+            const std::string term = lit(c) + " * u[uidx(" + istr(f) + ", " + q[0] + ", " + q[1] + ", " + q[2] + ")]";
+            e = e.empty() ? term : e + " + " + term;
+          }
+          if (cl.gprog >= 0) {
+            const std::string term = lit(d_->pool_h[cl.wc + gi]) + " * " + G;
+            e = e.empty() ? term : e + " + " + term;
+          }
+          o << "    case " << gi << ": return " << (e.empty() ? "T(0)" : e) << ";\n";
+        }
+        o << "    default: return T(0);\n    }\n    }\n";
+      }
+    o << "    return T(0);\n    }\n";
+  }
+  o << "    default: return T(0);\n    }\n  }\n";
+  return o.str();
+}
+
+// Transpose of the staging map for one LDS position holding adjoint v of variable f: the
+// unknown itself, or the unknowns its ghost-node closure combines (c * G carries no variable).
+std::string Gen::scatter_fn() {
+  std::ostringstream o;
+  o << "  static __device__ void scatter(const int f, int p0, int p1, int p2, const T v, T* __restrict__ out) {\n";
+  for (int a = 0; a < 3; ++a)
+    if (g_.periodic[a]) o << "    p" << a << " = mol_wrap(p" << a << ", " << g_.n[a] << ");\n";
+  o << "    if (p0 < 1 || p0 > " << g_.n[0] << " || p1 < 1 || p1 > " << g_.n[1] << " || p2 < 1 || p2 > "
+    << g_.n[2] << ") return;\n";
+  o << "    switch (f) {\n";
+  for (int f = 0; f < d_->nfields; ++f) {
+    o << "    case " << f << ": {\n";
+    o << "    const int nout = ";
+    for (int a = 0; a < 3; ++a)
+      o << (a ? " + " : "") << "(p" << a << " < " << g_.lo[f][a] << " || p" << a << " > " << g_.hi[f][a] << ")";
+    o << ";\n    if (nout == 0) { mol_atomic_add(&out[uidx(" << f << ", p0, p1, p2)], v); return; }\n"
+      << "    if (nout > 1) return;\n";
+    for (int a = 0; a < d_->ndim; ++a)
+      for (int side = 0; side < 2; ++side) {
+        int cid = -1;
+        for (int c = 0; c < d_->nclosures; ++c) {
+          const mol_closure& cl = d_->closures_h[c];
+          if (cl.field == f && cl.axis == a && cl.side == side) cid = c;
+        }
+        if (cid < 0) continue;
+        const mol_closure& cl = d_->closures_h[cid];
+        const std::string pa = "p" + istr(a);
+        const int first = side == 0 ? g_.lo[f][a] : g_.hi[f][a];
+        const int dir = side == 0 ? 1 : -1;
+        o << "    if (" << pa << (side == 0 ? " < " + istr(g_.lo[f][a]) : " > " + istr(g_.hi[f][a])) << ") {\n";
+        o << "    const int g = " << (side == 0 ? istr(g_.lo[f][a] - 1) + " - " + pa : pa + " - " + istr(g_.hi[f][a] + 1))
+          << ";\n    switch (g) {\n";
+        for (int gi = 0; gi < cl.nghost; ++gi) {
+          o << "    case " << gi << ":\n";
+          for (int m = 0; m < cl.K; ++m) {
+            const double c = d_->pool_h[cl.wC + gi * cl.K + m];
+            if (c == 0.0) continue;
+            std::string q[3] = {"p0", "p1", "p2"};
+            q[a] = istr(first + m * dir);
+            o << "      mol_atomic_add(&out[uidx(" << f << ", " << q[0] << ", " << q[1] << ", " << q[2] << ")], " << lit(c)
+              << " * v);\n";
+          }
+          o << "      return;\n";
+        }
+        o << "    default: return;\n    }\n    }\n";
+      }
+    o << "    return;\n    }\n";
+  }
+  o << "    default: return;\n    }\n  }\n";
+  return o.str();
+}
+
+// Geometry constants and index helpers shared by the forward and reverse kernels.
+std::string Gen::geo_common() {
+  std::ostringstream o;
+  auto c = [&](const std::string& name, long long v) { o << "  static constexpr int " << name << " = " << v << ";\n"; };
+  c("NF", d_->nfields);
+  c("NAUX", d_->naux);
+  c("NARR", d_->nfields + d_->naux);
+  for (int a = 0; a < 3; ++a) {
+    const std::string s = istr(a);
+    c("N" + s, g_.n[a]);
+    c("PER" + s, g_.periodic[a]);
+    c("ULO" + s, g_.ulo[a]);
+    c("UHI" + s, g_.uhi[a]);
+    c("NT" + s, g_.ntile[a]);
+  }
   c("TX", g_.tile[0]); c("TY", g_.tile[1]); c("TZ", g_.tile[2]);
   c("HX", g_.halo[0]); c("HY", g_.halo[1]); c("HZ", g_.halo[2]);
   c("EX", g_.ext[0]); c("EY", g_.ext[1]); c("EZ", g_.ext[2]);
   c("VOL", g_.vol);
-  c("CTLO0", g_.ct_lo[0]); c("CTLO1", g_.ct_lo[1]); c("CTLO2", g_.ct_lo[2]);
-  c("CTN0", g_.ct_n[0]); c("CTN1", g_.ct_n[1]);
-  c("NBLK", nblk);
+  c("NBLK", g_.ntile[0] * g_.ntile[1] * g_.ntile[2]);
   c("NTHR", 256);
-  c("BOX", g_.box);
-  bool ragged = false;
+  o << "  static constexpr i64 NGRID = " << (long long)g_.n[0] * g_.n[1] * g_.n[2] << ";\n";
+  // tiles staged with plain loads: tile + halo inside every variable's box (after the wrap)
   for (int a = 0; a < 3; ++a) {
-    const int last = g_.lo[a] + (g_.ct_lo[a] + g_.ct_n[a]) * g_.tile[a] - 1;
-    ragged = ragged || last > g_.hi[a];
+    int slo = 0, shi = g_.ntile[a] - 1;
+    if (!g_.periodic[a]) {
+      int lo = -(1 << 30), hi = 1 << 30;
+      for (int f = 0; f < d_->nfields; ++f) {
+        lo = std::max(lo, g_.lo[f][a]);
+        hi = std::min(hi, g_.hi[f][a]);
+      }
+      slo = g_.ntile[a];
+      shi = -1;
+      for (int t = 0; t < g_.ntile[a]; ++t) {
+        const int o0 = g_.ulo[a] + t * g_.tile[a];
+        if (o0 - g_.halo[a] >= lo && o0 + g_.tile[a] - 1 + g_.halo[a] <= hi) {
+          slo = std::min(slo, t);
+          shi = std::max(shi, t);
+        }
+      }
+    }
+    c("SLO" + istr(a), slo);
+    c("SHI" + istr(a), shi);
   }
-  c("RAGGED", ragged ? 1 : 0);
-  o << "  static __device__ __forceinline__ void node(const T* __restrict__ L, const int b, const int I0, const int I1,\n"
-       "                                              const int I2, const double t, const T* __restrict__ P,\n"
-       "                                              const JPtrs& jp, T* out) {\n";
-  o << "    (void)I0; (void)I1; (void)I2; (void)t; (void)P; (void)jp;\n";
-  o << body_.str();
-  for (int f = 0; f < d_->nfields; ++f) o << "    out[" << f << "] = " << outs[f] << ";\n";
+  // core tiles: interior operator rows for every node
+  for (int a = 0; a < 3; ++a) {
+    c("CLO" + istr(a), g_.ncore > 0 ? g_.ct_lo[a] : 1);
+    c("CHI" + istr(a), g_.ncore > 0 ? g_.ct_lo[a] + g_.ct_n[a] - 1 : 0);
+  }
+  o << "  static __device__ __forceinline__ i64 uidx(const int f, const int p0, const int p1, const int p2) {\n"
+       "    switch (f) {\n";
+  for (int f = 0; f < d_->nfields; ++f) {
+    const long long bx = g_.hi[f][0] - g_.lo[f][0] + 1, by = g_.hi[f][1] - g_.lo[f][1] + 1;
+    o << "    case " << f << ": return (i64)" << g_.fofs[f] << " + (p0 - " << g_.lo[f][0] << ") + (i64)" << bx
+      << " * ((p1 - " << g_.lo[f][1] << ") + (i64)" << by << " * (p2 - " << g_.lo[f][2] << "));\n";
+  }
+  o << "    default: return 0;\n    }\n  }\n";
+  o << "  static __device__ __forceinline__ bool inbox(const int f, const int I0, const int I1, const int I2) {\n"
+       "    switch (f) {\n";
+  for (int f = 0; f < d_->nfields; ++f)
+    o << "    case " << f << ": return I0 >= " << g_.lo[f][0] << " && I0 <= " << g_.hi[f][0] << " && I1 >= "
+      << g_.lo[f][1] << " && I1 <= " << g_.hi[f][1] << " && I2 >= " << g_.lo[f][2] << " && I2 <= " << g_.hi[f][2]
+      << ";\n";
+  o << "    default: return false;\n    }\n  }\n";
+  // full-grid index of an (unwrapped) position, -1 outside the grid
+  o << "  static __device__ __forceinline__ i64 gidx(int p0, int p1, int p2) {\n";
+  for (int a = 0; a < 3; ++a)
+    if (g_.periodic[a]) o << "    p" << a << " = mol_wrap(mol_wrap(p" << a << ", " << g_.n[a] << "), " << g_.n[a] << ");\n";
+  o << "    if (p0 < 1 || p0 > " << g_.n[0] << " || p1 < 1 || p1 > " << g_.n[1] << " || p2 < 1 || p2 > " << g_.n[2]
+    << ") return -1;\n"
+    << "    return (i64)(p0 - 1) + (i64)" << g_.n[0] << " * ((p1 - 1) + (i64)" << g_.n[1] << " * (p2 - 1));\n  }\n";
+  o << "  static __device__ __forceinline__ T stage_aux(const int a, int p0, int p1, int p2, const JPtrs& jp) {\n"
+       "    const i64 gi = gidx(p0, p1, p2);\n"
+       "    return gi < 0 ? T(0) : jp.aux[a][gi];\n  }\n";
+  return o.str();
+}
+
+std::string Gen::source(bool f64) {
+  std::ostringstream o;
+  o << "// generated right-hand side of one PDE system\n";
+  o << "#define MOL_T " << (f64 ? "double" : "float") << "\n";
+  o << "#include \"mol_jit_prelude.h\"\n\n";
+  const std::string stage = stage_fn();
+  std::vector<int> out_core, out_bound;
+  int r_core = -1, r_bound = -1;
+  if (g_.ncore > 0) build_node(true, out_core, r_core);
+  build_node(false, out_bound, r_bound);
+  if (!ok_) return "";
+  o << "struct Geo {\n" << geo_common() << stage;
+  o << "  template <bool CORE>\n"
+       "  static __device__ __forceinline__ void node(const T* __restrict__ L, const int b, const int I0, const int I1,\n"
+       "      const int I2, const double t, const T* __restrict__ P, const JPtrs& jp, T* out) {\n"
+       "    (void)I0; (void)I1; (void)I2; (void)t; (void)P; (void)jp; (void)L; (void)b;\n";
+  auto body = [&](int r, const std::vector<int>& outs) {
+    print_region(o, r);
+    for (int f = 0; f < d_->nfields; ++f) o << "    out[" << f << "] = " << ref(outs[f]) << ";\n";
+  };
+  if (g_.ncore > 0) {
+    o << "    if constexpr (CORE) {\n";
+    body(r_core, out_core);
+    o << "    } else {\n";
+    body(r_bound, out_bound);
+    o << "    }\n";
+  } else {
+    body(r_bound, out_bound);
+  }
   o << "  }\n};\n\n";
-  o << "extern \"C\" __global__ void __launch_bounds__(256) mol_core(const T* __restrict__ u, Epi ep, JPtrs jp) {\n"
-       "  mol_core_tile<Geo>(u, ep, jp);\n}\n";
+  o << "extern \"C\" __global__ void __launch_bounds__(256) mol_rhs_tiles(const T* __restrict__ u, Epi ep, JPtrs jp) {\n"
+       "  mol_full_tile<Geo>(u, ep, jp);\n}\n";
+  return o.str();
+}
+
+std::string Gen::source_vjp(bool f64) {
+  std::ostringstream o;
+  o << "// generated vector-Jacobian product of the right-hand side of one PDE system\n";
+  o << "#define MOL_T " << (f64 ? "double" : "float") << "\n";
+  o << "#include \"mol_jit_prelude.h\"\n\n";
+  const std::string stage = stage_fn();
+  const std::string scatter = scatter_fn();
+  std::vector<int> out_core, out_bound;
+  int r_core = -1, r_bound = -1;
+  if (g_.ncore > 0) build_node(true, out_core, r_core);
+  build_node(false, out_bound, r_bound);
+  if (!ok_) return "";
+  o << "struct Geo {\n" << geo_common() << stage << scatter;
+  o << "  template <bool CORE>\n"
+       "  static __device__ __forceinline__ void vjp(const T* __restrict__ L, T* Ab, double* Aa, const int b,\n"
+       "      const int I0, const int I1, const int I2, const double t, const T* __restrict__ P, const JPtrs& jp,\n"
+       "      const T* lam) {\n"
+       "    (void)I0; (void)I1; (void)I2; (void)t; (void)P; (void)jp; (void)L; (void)b; (void)Ab; (void)Aa;\n";
+  auto body = [&](int r, const std::vector<int>& outs) {
+    print_region(o, r);
+    std::vector<std::pair<int, std::string>> seeds;
+    for (int f = 0; f < d_->nfields; ++f) seeds.push_back({outs[f], "lam[" + istr(f) + "]"});
+    print_reverse(o, r, seeds);
+  };
+  if (g_.ncore > 0) {
+    o << "    if constexpr (CORE) {\n";
+    body(r_core, out_core);
+    o << "    } else {\n";
+    body(r_bound, out_bound);
+    o << "    }\n";
+  } else {
+    body(r_bound, out_bound);
+  }
+  o << "  }\n};\n\n";
+  o << "extern \"C\" __global__ void __launch_bounds__(256) mol_vjp_tiles(VArgs va, JPtrs jp) {\n"
+       "  mol_vjp_tile<Geo>(va, jp);\n}\n";
   return o.str();
 }
 
@@ -435,9 +1088,16 @@ std::map<std::string, CacheEntry> g_jit_cache;   // key: generated source
 
 }  // namespace
 
-std::string jit_core_source(const mol_desc* d, const JitGeom& g, std::string& err) {
+std::string jit_source(const mol_desc* d, const JitGeom& g, std::string& err) {
   Gen gen(d, g);
   std::string s = gen.source(d->dtype == MOL_F64);
+  err = gen.error;
+  return s;
+}
+
+std::string jit_vjp_source(const mol_desc* d, const JitGeom& g, std::string& err) {
+  Gen gen(d, g);
+  std::string s = gen.source_vjp(d->dtype == MOL_F64);
   err = gen.error;
   return s;
 }
@@ -471,7 +1131,7 @@ int jit_compile(const std::string& src, std::vector<char>& code, std::string& lo
 }
 
 // Compile (or fetch from the process-wide cache) and load the generated kernel on `device`.
-int jit_kernel(const std::string& src, int device, hipFunction_t* fn, std::string& err) {
+int jit_kernel(const std::string& src, const char* name, int device, hipFunction_t* fn, std::string& err) {
   std::lock_guard<std::mutex> lk(g_jit_mu);
   CacheEntry& e = g_jit_cache[src];
   if (e.code.empty()) {
@@ -482,9 +1142,9 @@ int jit_kernel(const std::string& src, int device, hipFunction_t* fn, std::strin
       return MOL_ERR_HIP;
     }
     if (const char* dir = std::getenv("MOL_JIT_DUMP")) {
-      char name[64];
-      std::snprintf(name, sizeof(name), "/mol_core_%016llx.hip", (unsigned long long)fnv1a(src));
-      if (FILE* f = std::fopen((std::string(dir) + name).c_str(), "w")) {
+      char fname[64];
+      std::snprintf(fname, sizeof(fname), "/mol_gen_%016llx.hip", (unsigned long long)fnv1a(src));
+      if (FILE* f = std::fopen((std::string(dir) + fname).c_str(), "w")) {
         std::fwrite(src.data(), 1, src.size(), f);
         std::fclose(f);
       }
@@ -500,7 +1160,7 @@ int jit_kernel(const std::string& src, int device, hipFunction_t* fn, std::strin
     }
     it = e.modules.emplace(device, m).first;
   }
-  const hipError_t he = hipModuleGetFunction(fn, it->second, "mol_core");
+  const hipError_t he = hipModuleGetFunction(fn, it->second, name);
   if (he != hipSuccess) {
     err = std::string("hipModuleGetFunction: ") + hipGetErrorString(he);
     return MOL_ERR_HIP;

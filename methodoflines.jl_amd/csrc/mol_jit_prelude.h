@@ -88,52 +88,98 @@ __device__ __noinline__ T mol_powi(T x, int e) {
 // u(i-2) .. u(i+2).  The candidate reconstructions are written as u(i) + difference/6 so u(i)
 // cancels, and the normalised nonlinear weights gamma_k/q_k / sum_j gamma_j/q_j are cleared
 // to gamma_k P_k / sum_j gamma_j P_j (P_k = product of the other two q_j): one division.
-__device__ __forceinline__ T mol_weno5(T um2, T um1, T u0, T up1, T up2, T eps, T inv_dx) {
-  const T dmm = um1 - um2, dm = u0 - um1, dp = up1 - u0, dpp = up2 - up1;
-  const T D2m = dm - dmm, D20 = dp - dm, D2p = dpp - dp;
-  const T a1 = T(-2) * dp + D2p;
-  const T a2 = dm + dp;
-  const T a3 = T(2) * dm + D2m;
-  const T b1 = T(13.0 / 12.0) * D2p * D2p + eps + T(0.25) * a1 * a1;
-  const T b2 = T(13.0 / 12.0) * D20 * D20 + eps + T(0.25) * a2 * a2;
-  const T b3 = T(13.0 / 12.0) * D2m * D2m + eps + T(0.25) * a3 * a3;
-  const T q1 = b1 * b1, q2 = b2 * b2, q3 = b3 * b3;
-  const T P1 = q2 * q3, P2 = q1 * q3, P3 = q1 * q2;
-  const T A1 = T(0.3) * P1, A2 = T(0.6) * P2, A3 = T(0.1) * P3;
-  const T B1 = T(0.1) * P1, B3 = T(0.3) * P3;
-  const T Dp = A1 + A2 + A3;
-  const T Dm = B1 + A2 + B3;
-  const T ep1 = T(4) * dp - dpp, ep2 = T(2) * dp + dm, ep3 = T(5) * dm - T(2) * dmm;
-  const T em1 = T(-5) * dp + T(2) * dpp, em2 = T(-2) * dm - dp, em3 = T(-4) * dm + dmm;
-  const T Np = A1 * ep1 + A2 * ep2 + A3 * ep3;
-  const T Nm = B1 * em1 + A2 * em2 + B3 * em3;
+template <class U>
+__device__ __forceinline__ U mol_weno5_t(U um2, U um1, U u0, U up1, U up2, T eps, T inv_dx) {
+  const U dmm = um1 - um2, dm = u0 - um1, dp = up1 - u0, dpp = up2 - up1;
+  const U D2m = dm - dmm, D20 = dp - dm, D2p = dpp - dp;
+  const U a1 = T(-2) * dp + D2p;
+  const U a2 = dm + dp;
+  const U a3 = T(2) * dm + D2m;
+  const U b1 = T(13.0 / 12.0) * D2p * D2p + eps + T(0.25) * a1 * a1;
+  const U b2 = T(13.0 / 12.0) * D20 * D20 + eps + T(0.25) * a2 * a2;
+  const U b3 = T(13.0 / 12.0) * D2m * D2m + eps + T(0.25) * a3 * a3;
+  const U q1 = b1 * b1, q2 = b2 * b2, q3 = b3 * b3;
+  const U P1 = q2 * q3, P2 = q1 * q3, P3 = q1 * q2;
+  const U A1 = T(0.3) * P1, A2 = T(0.6) * P2, A3 = T(0.1) * P3;
+  const U B1 = T(0.1) * P1, B3 = T(0.3) * P3;
+  const U Dp = A1 + A2 + A3;
+  const U Dm = B1 + A2 + B3;
+  const U ep1 = T(4) * dp - dpp, ep2 = T(2) * dp + dm, ep3 = T(5) * dm - T(2) * dmm;
+  const U em1 = T(-5) * dp + T(2) * dpp, em2 = T(-2) * dm - dp, em3 = T(-4) * dm + dmm;
+  const U Np = A1 * ep1 + A2 * ep2 + A3 * ep3;
+  const U Nm = B1 * em1 + A2 * em2 + B3 * em3;
   return (Np * Dm - Nm * Dp) / (Dp * Dm) * (inv_dx * T(1.0 / 6.0));
 }
 
 // The same derivative in the reference's normalised-weight form (weno_f_uniform,
 // src/discretization/schemes/WENO/WENO.jl:6-57).  Used in FP32, where the products of four
 // squared smoothness indicators of the cleared form (~eps^8) underflow.
-__device__ __forceinline__ T mol_weno5_ref(T um2, T um1, T u0, T up1, T up2, T eps, T inv_dx) {
-  const T c1 = u0 - T(2) * up1 + up2, c2 = T(3) * u0 - T(4) * up1 + up2;
-  const T c3 = um1 - T(2) * u0 + up1, c4 = um1 - up1;
-  const T c5 = um2 - T(2) * um1 + u0, c6 = um2 - T(4) * um1 + T(3) * u0;
-  const T b1 = T(13) * c1 * c1 / T(12) + c2 * c2 / T(4);
-  const T b2 = T(13) * c3 * c3 / T(12) + c4 * c4 / T(4);
-  const T b3 = T(13) * c5 * c5 / T(12) + c6 * c6 / T(4);
-  const T q1 = (eps + b1) * (eps + b1), q2 = (eps + b2) * (eps + b2), q3 = (eps + b3) * (eps + b3);
-  const T om1 = T(0.1) / q1, om2 = T(0.6) / q2, om3 = T(0.3) / q3;
-  const T op1 = T(0.3) / q1, op2 = T(0.6) / q2, op3 = T(0.1) / q3;
-  const T wmd = om1 + om2 + om3, wpd = op1 + op2 + op3;
-  const T hm1 = (T(11) * u0 - T(7) * up1 + T(2) * up2) / T(6);
-  const T hm2 = (T(5) * u0 - up1 + T(2) * um1) / T(6);
-  const T hm3 = (T(2) * u0 + T(5) * um1 - um2) / T(6);
-  const T hp1 = (T(2) * u0 + T(5) * up1 - up2) / T(6);
-  const T hp2 = (T(5) * u0 + T(2) * up1 - um1) / T(6);
-  const T hp3 = (T(11) * u0 - T(7) * um1 + T(2) * um2) / T(6);
-  const T hp = (op1 / wpd) * hp1 + (op2 / wpd) * hp2 + (op3 / wpd) * hp3;
-  const T hm = (om1 / wmd) * hm1 + (om2 / wmd) * hm2 + (om3 / wmd) * hm3;
+template <class U>
+__device__ __forceinline__ U mol_weno5_ref_t(U um2, U um1, U u0, U up1, U up2, T eps, T inv_dx) {
+  const U c1 = u0 - T(2) * up1 + up2, c2 = T(3) * u0 - T(4) * up1 + up2;
+  const U c3 = um1 - T(2) * u0 + up1, c4 = um1 - up1;
+  const U c5 = um2 - T(2) * um1 + u0, c6 = um2 - T(4) * um1 + T(3) * u0;
+  const U b1 = T(13) * c1 * c1 / T(12) + c2 * c2 / T(4);
+  const U b2 = T(13) * c3 * c3 / T(12) + c4 * c4 / T(4);
+  const U b3 = T(13) * c5 * c5 / T(12) + c6 * c6 / T(4);
+  const U q1 = (eps + b1) * (eps + b1), q2 = (eps + b2) * (eps + b2), q3 = (eps + b3) * (eps + b3);
+  const U om1 = T(0.1) / q1, om2 = T(0.6) / q2, om3 = T(0.3) / q3;
+  const U op1 = T(0.3) / q1, op2 = T(0.6) / q2, op3 = T(0.1) / q3;
+  const U wmd = om1 + om2 + om3, wpd = op1 + op2 + op3;
+  const U hm1 = (T(11) * u0 - T(7) * up1 + T(2) * up2) / T(6);
+  const U hm2 = (T(5) * u0 - up1 + T(2) * um1) / T(6);
+  const U hm3 = (T(2) * u0 + T(5) * um1 - um2) / T(6);
+  const U hp1 = (T(2) * u0 + T(5) * up1 - up2) / T(6);
+  const U hp2 = (T(5) * u0 + T(2) * up1 - um1) / T(6);
+  const U hp3 = (T(11) * u0 - T(7) * um1 + T(2) * um2) / T(6);
+  const U hp = (op1 / wpd) * hp1 + (op2 / wpd) * hp2 + (op3 / wpd) * hp3;
+  const U hm = (om1 / wmd) * hm1 + (om2 / wmd) * hm2 + (om3 / wmd) * hm3;
   return (hp - hm) * inv_dx;
 }
+
+__device__ __forceinline__ T mol_weno5(T um2, T um1, T u0, T up1, T up2, T eps, T inv_dx) {
+  return mol_weno5_t<T>(um2, um1, u0, up1, up2, eps, inv_dx);
+}
+__device__ __forceinline__ T mol_weno5_ref(T um2, T um1, T u0, T up1, T up2, T eps, T inv_dx) {
+  return mol_weno5_ref_t<T>(um2, um1, u0, up1, up2, eps, inv_dx);
+}
+
+// Forward-mode dual number (value, derivative) for the gradient of the WENO reconstruction.
+struct MolDual {
+  T v, d;
+};
+__device__ __forceinline__ MolDual operator+(MolDual a, MolDual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ MolDual operator-(MolDual a, MolDual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ MolDual operator*(MolDual a, MolDual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ MolDual operator/(MolDual a, MolDual b) {
+  const T q = a.v / b.v;
+  return {q, (a.d - q * b.d) / b.v};
+}
+__device__ __forceinline__ MolDual operator+(MolDual a, T b) { return {a.v + b, a.d}; }
+__device__ __forceinline__ MolDual operator+(T a, MolDual b) { return {a + b.v, b.d}; }
+__device__ __forceinline__ MolDual operator*(T a, MolDual b) { return {a * b.v, a * b.d}; }
+__device__ __forceinline__ MolDual operator*(MolDual a, T b) { return {a.v * b, a.d * b}; }
+__device__ __forceinline__ MolDual operator/(MolDual a, T b) { return {a.v / b, a.d / b}; }
+__device__ __forceinline__ MolDual operator/(T a, MolDual b) {
+  const T q = a / b.v;
+  return {q, -q * b.d / b.v};
+}
+
+// Gradient of the WENO5 node derivative with respect to its five inputs.
+__device__ __noinline__ void mol_weno5_grad(T um2, T um1, T u0, T up1, T up2, T eps, T inv_dx, T* g, bool cleared) {
+  const T x[5] = {um2, um1, u0, up1, up2};
+  for (int k = 0; k < 5; ++k) {
+    MolDual d[5];
+    for (int j = 0; j < 5; ++j) d[j] = MolDual{x[j], j == k ? T(1) : T(0)};
+    const MolDual r = cleared ? mol_weno5_t<MolDual>(d[0], d[1], d[2], d[3], d[4], eps, inv_dx)
+                              : mol_weno5_ref_t<MolDual>(d[0], d[1], d[2], d[3], d[4], eps, inv_dx);
+    g[k] = r.d;
+  }
+}
+
+// Hardware floating-point atomic add on global memory.
+__device__ __forceinline__ void mol_atomic_add(T* p, T v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void mol_atomic_add_f64(double* p, double v) { unsafeAtomicAdd(p, v); }
 
 // Work-groups are dealt to the 8 XCDs round-robin; remap so that each XCD runs one contiguous
 // range of tiles (neighbouring tiles share halo lines in that XCD's L2).
@@ -146,18 +192,25 @@ __device__ __forceinline__ int mol_xcd_tile(int b) {
   return x * q + (x < rem ? x : rem) + i;
 }
 
-// One tile of the core region: every tile + halo position is an unknown or its periodic image.
+// One tile of the union unknown box.  Tiles whose tile + halo lies inside every variable's
+// unknown box (after the periodic wrap) stage with plain loads; the others stage each position
+// through Geo::stage (ghost-node closures, zero outside the grid and at corners).  Tiles inside
+// the core range run the node function with interior operator rows only (Geo::node<true>);
+// the others select boundary rows per node index (Geo::node<false>).
 template <class G>
-__device__ __forceinline__ void mol_core_tile(const T* __restrict__ u, const Epi& ep, const JPtrs& jp) {
+__device__ __forceinline__ void mol_full_tile(const T* __restrict__ u, const Epi& ep, const JPtrs& jp) {
   __shared__ T L[G::NARR * G::VOL];
   int b = mol_xcd_tile<G::NBLK>((int)blockIdx.x);
-  const int tx = G::CTLO0 + b % G::CTN0;
-  b /= G::CTN0;
-  const int ty = G::CTLO1 + b % G::CTN1;
-  const int tz = G::CTLO2 + b / G::CTN1;
-  const int o0 = G::LO0 + tx * G::TX, o1 = G::LO1 + ty * G::TY, o2 = G::LO2 + tz * G::TZ;
+  const int tx = b % G::NT0;
+  b /= G::NT0;
+  const int ty = b % G::NT1;
+  const int tz = b / G::NT1;
+  const int o0 = G::ULO0 + tx * G::TX, o1 = G::ULO1 + ty * G::TY, o2 = G::ULO2 + tz * G::TZ;
   const double t = (ep.tnow ? ep.tnow[0] : 0.0) + ep.tofs;
-  constexpr int BX = G::HI0 - G::LO0 + 1, BY = G::HI1 - G::LO1 + 1;
+  const bool simple = tx >= G::SLO0 && tx <= G::SHI0 && ty >= G::SLO1 && ty <= G::SHI1 &&
+                      tz >= G::SLO2 && tz <= G::SHI2;
+  const bool core = tx >= G::CLO0 && tx <= G::CHI0 && ty >= G::CLO1 && ty <= G::CHI1 &&
+                    tz >= G::CLO2 && tz <= G::CHI2;
   constexpr int EXY = G::EX * G::EY;
   for (int q = threadIdx.x; q < G::VOL; q += G::NTHR) {
     const int lz = q / EXY;
@@ -165,17 +218,20 @@ __device__ __forceinline__ void mol_core_tile(const T* __restrict__ u, const Epi
     const int ly = r / G::EX;
     const int lx = r - ly * G::EX;
     int p0 = o0 + lx - G::HX, p1 = o1 + ly - G::HY, p2 = o2 + lz - G::HZ;
-    if (G::PER0) p0 = mol_wrap(p0, G::N0);
-    if (G::PER1) p1 = mol_wrap(p1, G::N1);
-    if (G::PER2) p2 = mol_wrap(p2, G::N2);
-    const i64 li = (i64)(p0 - G::LO0) + (i64)BX * ((p1 - G::LO1) + (i64)BY * (p2 - G::LO2));
+    if (simple) {
+      // a ragged last tile reaches up to tile - 1 + halo nodes past n: the second wrap keeps
+      // every staged position on the ring
+      if (G::PER0) p0 = mol_wrap(mol_wrap(p0, G::N0), G::N0);
+      if (G::PER1) p1 = mol_wrap(mol_wrap(p1, G::N1), G::N1);
+      if (G::PER2) p2 = mol_wrap(mol_wrap(p2, G::N2), G::N2);
 #pragma unroll
-    for (int f = 0; f < G::NF; ++f) L[f * G::VOL + q] = u[f * G::BOX + li];
-    if (G::NAUX) {
-      const i64 gi = (i64)(p0 - 1) + (i64)G::N0 * ((p1 - 1) + (i64)G::N1 * (p2 - 1));
+      for (int f = 0; f < G::NF; ++f) L[f * G::VOL + q] = u[G::uidx(f, p0, p1, p2)];
+    } else {
 #pragma unroll
-      for (int a = 0; a < G::NAUX; ++a) L[(G::NF + a) * G::VOL + q] = jp.aux[a][gi];
+      for (int f = 0; f < G::NF; ++f) L[f * G::VOL + q] = G::stage(f, p0, p1, p2, t, ep.p, u, jp);
     }
+#pragma unroll
+    for (int a = 0; a < G::NAUX; ++a) L[(G::NF + a) * G::VOL + q] = G::stage_aux(a, p0, p1, p2, jp);
   }
   __syncthreads();
   constexpr int TXY = G::TX * G::TY, TVOL = TXY * G::TZ;
@@ -188,12 +244,124 @@ __device__ __forceinline__ void mol_core_tile(const T* __restrict__ u, const Epi
     const int ly = rr / G::TX;
     const int lx = rr - ly * G::TX;
     const int I0 = o0 + lx, I1 = o1 + ly, I2 = o2 + lz;
-    if (G::RAGGED && (I0 > G::HI0 || I1 > G::HI1 || I2 > G::HI2)) continue;
+    if (I0 > G::UHI0 || I1 > G::UHI1 || I2 > G::UHI2) continue;
     const int base = (lx + G::HX) + G::EX * (ly + G::HY) + EXY * (lz + G::HZ);
     T out[G::NF];
-    G::node(L, base, I0, I1, I2, t, ep.p, jp, out);
-    const i64 li = (i64)(I0 - G::LO0) + (i64)BX * ((I1 - G::LO1) + (i64)BY * (I2 - G::LO2));
+    if (core) G::template node<true>(L, base, I0, I1, I2, t, ep.p, jp, out);
+    else G::template node<false>(L, base, I0, I1, I2, t, ep.p, jp, out);
 #pragma unroll
-    for (int f = 0; f < G::NF; ++f) mol_epilogue(ep, f * G::BOX + li, out[f]);
+    for (int f = 0; f < G::NF; ++f)
+      if (G::inbox(f, I0, I1, I2)) mol_epilogue(ep, G::uidx(f, I0, I1, I2), out[f]);
+  }
+}
+
+// Arguments of the generated vector-Jacobian product (layout identical to mol::VArgs<T>).
+// The seed is lam = c1*s1 + c2*s2 (s2 may be null); out receives (df/du)^T lam and g_aux the
+// FP64 auxiliary-array gradients (may be null); both are accumulated, not overwritten.
+struct VArgs {
+  const T* u;
+  const T* s1;
+  const T* s2;
+  double c1, c2;
+  T* out;
+  double* g_aux;
+  const T* p;
+  const double* tnow;
+  double tofs;
+};
+
+// One tile of the reverse-mode sweep: stage as the forward kernel, run each node's adjoint
+// (tap adjoints summed in LDS with LDS atomics), then add the LDS adjoints of tile + halo to
+// the unknowns (through the transposed ghost closures) and to the auxiliary gradients.
+template <class G>
+__device__ __forceinline__ void mol_vjp_tile(const VArgs& va, const JPtrs& jp) {
+  __shared__ T L[G::NARR * G::VOL];
+  __shared__ T Ab[G::NF * G::VOL];
+  __shared__ double Aa[(G::NAUX > 0 ? G::NAUX : 1) * G::VOL];
+  int b = mol_xcd_tile<G::NBLK>((int)blockIdx.x);
+  const int tx = b % G::NT0;
+  b /= G::NT0;
+  const int ty = b % G::NT1;
+  const int tz = b / G::NT1;
+  const int o0 = G::ULO0 + tx * G::TX, o1 = G::ULO1 + ty * G::TY, o2 = G::ULO2 + tz * G::TZ;
+  const double t = (va.tnow ? va.tnow[0] : 0.0) + va.tofs;
+  const bool simple = tx >= G::SLO0 && tx <= G::SHI0 && ty >= G::SLO1 && ty <= G::SHI1 &&
+                      tz >= G::SLO2 && tz <= G::SHI2;
+  const bool core = tx >= G::CLO0 && tx <= G::CHI0 && ty >= G::CLO1 && ty <= G::CHI1 &&
+                    tz >= G::CLO2 && tz <= G::CHI2;
+  constexpr int EXY = G::EX * G::EY;
+  for (int q = threadIdx.x; q < G::VOL; q += G::NTHR) {
+    const int lz = q / EXY;
+    const int r = q - lz * EXY;
+    const int ly = r / G::EX;
+    const int lx = r - ly * G::EX;
+    int p0 = o0 + lx - G::HX, p1 = o1 + ly - G::HY, p2 = o2 + lz - G::HZ;
+    if (simple) {
+      if (G::PER0) p0 = mol_wrap(mol_wrap(p0, G::N0), G::N0);
+      if (G::PER1) p1 = mol_wrap(mol_wrap(p1, G::N1), G::N1);
+      if (G::PER2) p2 = mol_wrap(mol_wrap(p2, G::N2), G::N2);
+#pragma unroll
+      for (int f = 0; f < G::NF; ++f) L[f * G::VOL + q] = va.u[G::uidx(f, p0, p1, p2)];
+    } else {
+#pragma unroll
+      for (int f = 0; f < G::NF; ++f) L[f * G::VOL + q] = G::stage(f, p0, p1, p2, t, va.p, va.u, jp);
+    }
+#pragma unroll
+    for (int a = 0; a < G::NAUX; ++a) {
+      L[(G::NF + a) * G::VOL + q] = G::stage_aux(a, p0, p1, p2, jp);
+      Aa[a * G::VOL + q] = 0.0;
+    }
+#pragma unroll
+    for (int f = 0; f < G::NF; ++f) Ab[f * G::VOL + q] = T(0);
+  }
+  __syncthreads();
+  constexpr int TXY = G::TX * G::TY, TVOL = TXY * G::TZ;
+  for (int q = threadIdx.x; q < TVOL; q += G::NTHR) {
+    const int lz = q / TXY;
+    const int rr = q - lz * TXY;
+    const int ly = rr / G::TX;
+    const int lx = rr - ly * G::TX;
+    const int I0 = o0 + lx, I1 = o1 + ly, I2 = o2 + lz;
+    if (I0 > G::UHI0 || I1 > G::UHI1 || I2 > G::UHI2) continue;
+    T lam[G::NF];
+    bool any = false;
+#pragma unroll
+    for (int f = 0; f < G::NF; ++f) {
+      T s = T(0);
+      if (G::inbox(f, I0, I1, I2)) {
+        const i64 k = G::uidx(f, I0, I1, I2);
+        s = (T)va.c1 * va.s1[k];
+        if (va.s2) s += (T)va.c2 * va.s2[k];
+      }
+      lam[f] = s;
+      any = any || s != T(0);
+    }
+    if (!any) continue;
+    const int base = (lx + G::HX) + G::EX * (ly + G::HY) + EXY * (lz + G::HZ);
+    if (core) G::template vjp<true>(L, Ab, Aa, base, I0, I1, I2, t, va.p, jp, lam);
+    else G::template vjp<false>(L, Ab, Aa, base, I0, I1, I2, t, va.p, jp, lam);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < G::VOL; q += G::NTHR) {
+    const int lz = q / EXY;
+    const int r = q - lz * EXY;
+    const int ly = r / G::EX;
+    const int lx = r - ly * G::EX;
+    const int p0 = o0 + lx - G::HX, p1 = o1 + ly - G::HY, p2 = o2 + lz - G::HZ;
+#pragma unroll
+    for (int f = 0; f < G::NF; ++f) {
+      const T v = Ab[f * G::VOL + q];
+      if (v != T(0)) G::scatter(f, p0, p1, p2, v, va.out);
+    }
+    if (G::NAUX > 0 && va.g_aux) {
+      const i64 gi = G::gidx(p0, p1, p2);
+      if (gi >= 0) {
+#pragma unroll
+        for (int a = 0; a < G::NAUX; ++a) {
+          const double v = Aa[a * G::VOL + q];
+          if (v != 0.0) mol_atomic_add_f64(&va.g_aux[a * G::NGRID + gi], v);
+        }
+      }
+    }
   }
 }

@@ -124,9 +124,10 @@ __global__ void __launch_bounds__(256) rhs_core_kernel(const KCore<T> kc_arg, co
     const int ly = (int)fdiv_magic((uint32_t)r, kc.magic0);
     const int lx = r - ly * kc.ext[0];
     int p0 = org0 + lx - kc.halo[0], p1 = org1 + ly - kc.halo[1], p2 = org2 + lz - kc.halo[2];
-    if (kc.periodic[0]) p0 = wrap_periodic(p0, kc.n[0]);
-    if (kc.periodic[1]) p1 = wrap_periodic(p1, kc.n[1]);
-    if (kc.periodic[2]) p2 = wrap_periodic(p2, kc.n[2]);
+    // second wrap: a ragged last tile reaches up to tile - 1 + halo nodes past n
+    if (kc.periodic[0]) p0 = wrap_periodic(wrap_periodic(p0, kc.n[0]), kc.n[0]);
+    if (kc.periodic[1]) p1 = wrap_periodic(wrap_periodic(p1, kc.n[1]), kc.n[1]);
+    if (kc.periodic[2]) p2 = wrap_periodic(wrap_periodic(p2, kc.n[2]), kc.n[2]);
     const long long li = (long long)(p0 - kc.lo[0]) + (long long)bx * ((p1 - kc.lo[1]) + (long long)by * (p2 - kc.lo[2]));
     for (int f = 0; f < kc.nfields; ++f) lds[f * kc.vol + q] = u[f * kc.box + li];
     if (kc.naux) {
@@ -168,14 +169,17 @@ template <typename T>
 int launch_rhs(const Launch<T>& L, const T* u, const Epi<T>& ep, hipStream_t stream) {
   if (L.kcw) return launch_weno2d<T>(*L.kcw, *L.plan, L.nweno, u, ep, stream);
   const size_t lds = (size_t)(L.ka->nfields + L.ka->naux) * L.ka->vol * sizeof(T);
-  if (L.ncore > 0 && L.jit) {
+  if (L.jit) {
+    // generated kernel: every tile, boundary rows and ghost closures included
     const T* uu = u;
     Epi<T> e = ep;
     JitPtrs<T> jp = L.jp;
     void* args[] = {(void*)&uu, (void*)&e, (void*)&jp};
-    if (hipModuleLaunchKernel(L.jit, L.ncore, 1, 1, 256, 1, 1, 0, stream, args, nullptr) != hipSuccess)
+    if (hipModuleLaunchKernel(L.jit, L.ntiles, 1, 1, 256, 1, 1, 0, stream, args, nullptr) != hipSuccess)
       return MOL_ERR_HIP;
-  } else if (L.ncore > 0) {
+    return 0;
+  }
+  if (L.ncore > 0) {
     hipLaunchKernelGGL(rhs_core_kernel<T>, dim3(L.ncore), dim3(256), lds, stream, *L.kc, u, ep);
   }
   if (L.nbound > 0)
@@ -194,5 +198,42 @@ int launch_advance_time(double* tnow, double dt, hipStream_t stream) {
   hipLaunchKernelGGL(advance_time_kernel, dim3(1), dim3(64), 0, stream, tnow, dt);
   return hipGetLastError() == hipSuccess ? 0 : MOL_ERR_HIP;
 }
+
+template <typename T>
+__global__ void fill_zero_kernel(T* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = T(0);
+}
+
+template <typename T>
+int launch_fill_zero(T* p, int64_t n, hipStream_t stream) {
+  if (n <= 0) return 0;
+  const int blocks = (int)std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(fill_zero_kernel<T>, dim3(blocks), dim3(256), 0, stream, p, n);
+  return hipGetLastError() == hipSuccess ? 0 : MOL_ERR_HIP;
+}
+
+// lam += y3 + y2 + y1 + y0, in the order the stage adjoints are produced
+template <typename T>
+__global__ void adjoint_sum_kernel(T* __restrict__ lam, const T* __restrict__ y0, const T* __restrict__ y1,
+                                   const T* __restrict__ y2, const T* __restrict__ y3, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    lam[i] = (((lam[i] + y0[i]) + y1[i]) + y2[i]) + y3[i];
+}
+
+template <typename T>
+int launch_adjoint_sum(T* lam, const T* y0, const T* y1, const T* y2, const T* y3, int64_t n, hipStream_t stream) {
+  if (n <= 0) return 0;
+  const int blocks = (int)std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(adjoint_sum_kernel<T>, dim3(blocks), dim3(256), 0, stream, lam, y0, y1, y2, y3, n);
+  return hipGetLastError() == hipSuccess ? 0 : MOL_ERR_HIP;
+}
+
+template int launch_fill_zero<double>(double*, int64_t, hipStream_t);
+template int launch_fill_zero<float>(float*, int64_t, hipStream_t);
+template int launch_adjoint_sum<double>(double*, const double*, const double*, const double*, const double*, int64_t,
+                                        hipStream_t);
+template int launch_adjoint_sum<float>(float*, const float*, const float*, const float*, const float*, int64_t,
+                                       hipStream_t);
 
 }  // namespace mol

@@ -131,8 +131,9 @@ __global__ void __launch_bounds__(256) weno2d_kernel(const KCore<T> kc_arg, Weno
     const int ly = q / WEXT_X;
     const int lx = q - ly * WEXT_X;
     int p0 = org0 + lx - WHALO, p1 = org1 + ly - WHALO;
-    if (kc.periodic[0]) p0 = wrap_periodic(p0, kc.n[0]);
-    if (kc.periodic[1]) p1 = wrap_periodic(p1, kc.n[1]);
+    // second wrap: a ragged last tile reaches up to tile - 1 + halo nodes past n
+    if (kc.periodic[0]) p0 = wrap_periodic(wrap_periodic(p0, kc.n[0]), kc.n[0]);
+    if (kc.periodic[1]) p1 = wrap_periodic(wrap_periodic(p1, kc.n[1]), kc.n[1]);
     const long long li = (long long)(p0 - kc.lo[0]) + (long long)bx * (p1 - kc.lo[1]);
     for (int f = 0; f < kc.nfields; ++f) lds[f * ldsvol + ly * WRS + lx] = u[f * kc.box + li];
   }

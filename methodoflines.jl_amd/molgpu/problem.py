@@ -93,9 +93,22 @@ class ODEProblem:
 
     def set_grid_field(self, name: str, values) -> None:
         """Bind the full-grid values (shape = grid shape, first axis fastest in memory) of a
-        grid-valued parameter."""
+        grid-valued parameter.  A contiguous CUDA tensor of the state dtype holding the grid in
+        the library's layout (first axis fastest, flat) is bound in place, without a copy."""
         if name not in self.lowered.aux:
             raise KeyError(name)
+        if not isinstance(values, np.ndarray) and hasattr(values, "is_cuda") and values.is_cuda:
+            torch = _torch()
+            if values.dtype != self.torch_dtype() or not values.is_contiguous() or \
+                    values.numel() != int(np.prod(self.lowered.n)):
+                raise ValueError("grid-field tensor must be contiguous %s with %d values"
+                                 % (self.torch_dtype(), int(np.prod(self.lowered.n))))
+            self._aux_dev = getattr(self, "_aux_dev", {})
+            self._aux_dev[name] = values
+            self._aux_host[name] = None
+            if self._handle is not None:
+                self._bind_aux()
+            return
         v = np.asarray(values, dtype=np.float64)
         if v.shape != tuple(self.lowered.n) and v.size != int(np.prod(self.lowered.n)):
             raise ValueError("grid field must have the grid shape %s" % (tuple(self.lowered.n),))
@@ -106,9 +119,13 @@ class ODEProblem:
     def _bind_aux(self):
         torch = _torch()
         ptrs = (C.c_void_p * max(1, len(self.lowered.aux)))()
+        dev = getattr(self, "_aux_dev", {})
         for i, name in enumerate(self.lowered.aux):
-            host = self._aux_host[name].reshape(-1, order="F")
-            t = torch.tensor(host, dtype=self.torch_dtype(), device="cuda")
+            if self._aux_host.get(name) is None and name in dev:
+                t = dev[name]
+            else:
+                host = self._aux_host[name].reshape(-1, order="F")
+                t = torch.tensor(host, dtype=self.torch_dtype(), device="cuda")
             self._aux[name] = t
             ptrs[i] = t.data_ptr()
         native.check(self._handle.lib, self._handle.lib.mol_set_aux(self._handle.h, ptrs), self._handle.h)
@@ -196,6 +213,63 @@ class ODEProblem:
                                  float(dt), int(nsteps), alg.scheme, flags, C.c_void_p(stream))
         native.check(h.lib, rc, h.h)
 
+    # ---------------------------------------------------------------- reverse mode
+    def vjp(self, u, lam, t: float = 0.0, p=None, g_aux=None):
+        """Vector-Jacobian product of the right-hand side: returns (df/du)^T lam at (u, t).
+        With ``g_aux`` (float64 CUDA tensor of naux * prod(n) values) the gradients with respect
+        to the grid fields, (df/daux)^T lam, are added to it."""
+        torch = _torch()
+        h = self.handle
+        self._check_tensor(u)
+        self._check_tensor(lam)
+        if p is None:
+            p = self.device_params()
+        out = torch.empty_like(lam)
+        gptr = self._check_gaux(g_aux)
+        stream = torch.cuda.current_stream().cuda_stream
+        rc = h.lib.mol_vjp(h.h, u.data_ptr(), p.data_ptr() if p is not None else None, float(t), lam.data_ptr(),
+                           out.data_ptr(), gptr, C.c_void_p(stream))
+        native.check(h.lib, rc, h.h)
+        return out
+
+    def _check_gaux(self, g_aux):
+        if g_aux is None:
+            return None
+        torch = _torch()
+        ng = int(np.prod(self.lowered.n))
+        if g_aux.dtype != torch.float64 or not g_aux.is_cuda or not g_aux.is_contiguous() or \
+                g_aux.numel() != ng * len(self.lowered.aux):
+            raise ValueError("g_aux must be a contiguous float64 CUDA tensor of %d values"
+                             % (ng * len(self.lowered.aux)))
+        return C.c_void_p(g_aux.data_ptr())
+
+    def forward_checkpointed(self, u, t0: float, dt: float, nsteps: int, ckpt_every: int, p=None) -> None:
+        """RK4 from t0 (u advanced in place) keeping a checkpoint every ``ckpt_every`` steps for
+        a later :meth:`adjoint`."""
+        torch = _torch()
+        h = self.handle
+        self._check_tensor(u)
+        if p is None:
+            p = self.device_params()
+        stream = torch.cuda.current_stream().cuda_stream
+        rc = h.lib.mol_forward_checkpointed(h.h, u.data_ptr(), p.data_ptr() if p is not None else None, float(t0),
+                                            float(dt), int(nsteps), int(ckpt_every), C.c_void_p(stream))
+        native.check(h.lib, rc, h.h)
+
+    def adjoint(self, lam, g_aux=None, p=None) -> None:
+        """Discrete adjoint of the last checkpointed RK4 run: on entry ``lam`` = dL/du(T), on
+        exit dL/du(t0); grid-field gradients are added to ``g_aux`` (float64)."""
+        torch = _torch()
+        h = self.handle
+        self._check_tensor(lam)
+        if p is None:
+            p = self.device_params()
+        gptr = self._check_gaux(g_aux)
+        stream = torch.cuda.current_stream().cuda_stream
+        rc = h.lib.mol_adjoint(h.h, p.data_ptr() if p is not None else None, lam.data_ptr(), gptr,
+                               C.c_void_p(stream))
+        native.check(h.lib, rc, h.h)
+
     # ---------------------------------------------------------------- layout helpers
     def unflatten(self, u) -> List[np.ndarray]:
         """Split a state vector into per-variable grids over their unknown boxes (first axis first)."""
@@ -207,16 +281,18 @@ class ODEProblem:
         return out
 
 
-def codegen(low: Lowered, compile: bool = False, tile=None) -> str:
-    """Source of the generated core-tile kernel for a lowered system (no GPU needed);
-    ``compile=True`` also compiles it with hiprtc for gfx950."""
+def codegen(low: Lowered, compile: bool = False, tile=None, vjp: bool = False) -> str:
+    """Source of the kernel generated for a lowered system (no GPU needed): the right-hand
+    side, or with ``vjp=True`` its vector-Jacobian product; ``compile=True`` also compiles it
+    with hiprtc for gfx950."""
     lib = native.load_library()
     desc, keep = native.pack_descriptor(low)
     tl = (C.c_int32 * 3)(*tile) if tile is not None else None
-    n = lib.mol_codegen(C.byref(desc), tl, None, 0, 0)
+    what = 2 if vjp else 0
+    n = lib.mol_codegen(C.byref(desc), tl, None, 0, what)
     native.check(lib, n if n < 0 else 0, None)
     buf = C.create_string_buffer(int(n) + 1)
-    n = lib.mol_codegen(C.byref(desc), tl, buf, len(buf), 1 if compile else 0)
+    n = lib.mol_codegen(C.byref(desc), tl, buf, len(buf), what | (1 if compile else 0))
     native.check(lib, n if n < 0 else 0, None)
     return buf.value.decode()
 

@@ -1,0 +1,221 @@
+"""GPU reverse mode: the generated vector-Jacobian product and the checkpointed RK4 adjoint.
+
+Pins: (1) for a linear right-hand side the VJP is the exact transpose of the oracle's matrix;
+(2) for nonlinear ones, <J^T lam, v> equals the central difference of <lam, f(u + e v)> built
+from the CPU oracle (or, for slot kinds without an oracle system here, from the GPU forward
+kernel); (3) the adjoint gradient of L = 1/2 |u(T) - target|^2 with respect to the per-cell
+field d0 matches the oracle's forward-mode (dual-number) derivative through the oracle's own
+RK4 to 1e-8, and Richardson-extrapolated central finite differences of the GPU solve on random
+cells.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+import molgpu
+from molgpu import systems
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _dev(x, dtype=None):
+    torch = _torch()
+    return torch.tensor(np.asarray(x), dtype=dtype or torch.float64, device="cuda")
+
+
+def _nonlin(n, seed=1):
+    sys_, disc = systems.nonlindiff3d(n=n)
+    prob = molgpu.discretize(sys_, disc)
+    d0 = 0.5 + 0.5 * np.abs(systems.gaussian_field3d(n, seed))
+    src = systems.gaussian_field3d(n, seed + 1)
+    prob.set_grid_field("d0", d0)
+    prob.set_grid_field("s", src)
+    return prob, d0.reshape(-1, order="F"), src.reshape(-1, order="F")
+
+
+def test_vjp_is_exact_transpose_for_linear_heat_with_robin_boundaries():
+    """f(u) = A u + b (Robin, 4th order): VJP(lam) = A^T lam, A from oracle RHS columns."""
+    torch = _torch()
+    n = 41
+    bcl, bch = (2.0, 0.5, 0.3), (1.5, -0.25, -0.2)
+    sys_, disc = systems.heat1d(n=n, D=0.8, bcl=bcl, bch=bch, approx_order=4)
+    prob = molgpu.discretize(sys_, disc)
+    prob.set_tile(16)                    # boundary and interior tiles both present
+    N = prob.lowered.n_unknowns
+    b = orc.diffusion1d_rhs(n, 0.8, bcl, bch, np.zeros(N), p=4)
+    A = np.stack([orc.diffusion1d_rhs(n, 0.8, bcl, bch, np.eye(N)[j], p=4) - b for j in range(N)], axis=1)
+    rng = np.random.default_rng(3)
+    lam = rng.standard_normal(N)
+    u = rng.standard_normal(N)
+    print("  heat vjp n=%d" % n, flush=True)
+    got = prob.vjp(_dev(u), _dev(lam)).cpu().numpy()
+    torch.cuda.synchronize()
+    ref = A.T @ lam
+    scale = np.abs(A.T) @ np.abs(lam)
+    assert np.max(np.abs(got - ref) / (scale + 1e-300)) < 1e-13
+
+
+@pytest.mark.parametrize("n", [12, 20])
+def test_vjp_dot_product_nonlinear_diffusion_vs_oracle(n):
+    """<J^T lam, v> and <(df/dd0)^T lam, w> against central differences of the oracle RHS."""
+    torch = _torch()
+    prob, d0, src = _nonlin(n)
+    N = prob.lowered.n_unknowns
+    rng = np.random.default_rng(n)
+    u = rng.uniform(-1, 1, N)
+    lam = rng.standard_normal(N)
+    v = rng.standard_normal(N)
+    w = rng.standard_normal(n ** 3)
+    g = torch.zeros(2 * n ** 3, dtype=torch.float64, device="cuda")
+    print("  nonlin vjp n=%d" % n, flush=True)
+    got = prob.vjp(_dev(u), _dev(lam), t=0.3, g_aux=g).cpu().numpy()
+    gd0 = g[: n ** 3].cpu().numpy()
+    gs = g[n ** 3:].cpu().numpy()
+
+    def F(uu, dd):
+        return orc.nonlindiff3d_rhs(n, dd, src, uu, t=0.3, threads=8)
+
+    def cdiff(fun, eps):
+        return (fun(eps) - fun(-eps)) / (2 * eps)
+
+    for eps_u in (1e-4,):
+        du = cdiff(lambda e: lam @ F(u + e * v, d0), eps_u)
+        du2 = cdiff(lambda e: lam @ F(u + e * v, d0), eps_u / 2)
+        fd = (4 * du2 - du) / 3
+        assert abs(got @ v - fd) <= 1e-8 * abs(fd) + 1e-10 * np.linalg.norm(got) * np.linalg.norm(v)
+    dd = cdiff(lambda e: lam @ F(u, d0 + e * w), 1e-4)
+    dd2 = cdiff(lambda e: lam @ F(u, d0 + e * w), 5e-5)
+    fd = (4 * dd2 - dd) / 3
+    assert abs(gd0 @ w - fd) <= 1e-8 * abs(fd) + 1e-10 * np.linalg.norm(gd0) * np.linalg.norm(w)
+    # the source term enters linearly: df/ds = identity on the unknown nodes
+    assert abs(gs.sum() - lam.sum()) <= 1e-10 * np.abs(lam).sum()
+
+
+def _gpu_dot_fd(prob, u, lam, v, t=0.0, eps=1e-5):
+    """Richardson-extrapolated central difference of <lam, f(u + e v)> with the GPU forward kernel."""
+    torch = _torch()
+
+    def val(e):
+        du = torch.empty(len(u), dtype=torch.float64, device="cuda")
+        prob.f(du, _dev(u + e * v), None, t)
+        return float(lam @ du.cpu().numpy())
+
+    d1 = (val(eps) - val(-eps)) / (2 * eps)
+    d2 = (val(eps / 2) - val(-eps / 2)) / eps
+    return (4 * d2 - d1) / 3
+
+
+@pytest.mark.parametrize("name", ["brusselator", "burgers_weno", "mixed_p4", "spherical", "upwind_o2",
+                                  "nonlinlap_periodic_p4", "weno_dirichlet", "diffreact_mixed_bcs"])
+def test_vjp_dot_product_every_slot_kind(name):
+    """Reverse mode of every operator-slot kind against the forward kernel's directional
+    derivative (small tiles: boundary-row variants and ghost closures included)."""
+    torch = _torch()
+    mk = {
+        "brusselator": (lambda: systems.brusselator2d(33), (0.5, 1.5), 1.2),
+        "burgers_weno": (lambda: systems.burgers2d(33), (0.2, 1.2), 0.0),
+        "mixed_p4": (lambda: systems.mixed2d(21, 0.7, 4), (-1, 1), 0.0),
+        "spherical": (lambda: systems.spherical1d(41, 0.25, 2), (-1, 1), 0.0),
+        "upwind_o2": (lambda: systems.upwind1d(61, 0.1, 1.0, 0.3, -0.2, 2), (-1, 1), 0.0),
+        "nonlinlap_periodic_p4": (lambda: systems.nonlinlap1d(61, True, 0.2, -0.4, 4), (-1, 1), 0.0),
+        "weno_dirichlet": (lambda: systems.weno1d_dirichlet(61, 1.0, 1.0, 0.1, 0.05), (-1, 1), 0.0),
+        "diffreact_mixed_bcs": (lambda: systems.diffreact2d(25, 0.05, 2.5, (0, 1, 2, 0)), (0, 1), 0.0),
+    }
+    build, (lo, hi), t = mk[name]
+    sys_, disc = build()
+    prob = molgpu.discretize(sys_, disc)
+    prob.set_tile(*{1: (16, 1, 1), 2: (8, 8, 1), 3: (8, 4, 4)}[prob.lowered.ndim])
+    N = prob.lowered.n_unknowns
+    rng = np.random.default_rng(7)
+    u = rng.uniform(lo, hi, N)
+    lam = rng.standard_normal(N)
+    v = rng.standard_normal(N)
+    print("  vjp dot %s" % name, flush=True)
+    got = prob.vjp(_dev(u), _dev(lam), t=t).cpu().numpy()
+    fd = _gpu_dot_fd(prob, u, lam, v, t)
+    assert abs(got @ v - fd) <= 1e-7 * abs(fd) + 1e-9 * np.linalg.norm(got) * np.linalg.norm(v), (got @ v, fd)
+
+
+def _loss_and_state(prob, u0, dt, nsteps, target):
+    torch = _torch()
+    u = _dev(u0)
+    prob.integrate(u, 0.0, dt, nsteps, molgpu.RK4)
+    r = u.cpu().numpy() - target
+    return 0.5 * float(r @ r), u
+
+
+def test_adjoint_gradient_matches_forward_mode_oracle_and_finite_differences():
+    """dL/dd0 from the checkpointed discrete adjoint vs the oracle's dual-number forward mode
+    (ForwardDiff counterpart) along a random direction, and vs central differences on cells."""
+    torch = _torch()
+    n, nsteps, every = 16, 24, 5
+    prob, d0, src = _nonlin(n, seed=3)
+    N = prob.lowered.n_unknowns
+    rng = np.random.default_rng(11)
+    u0 = 0.3 * systems.gaussian_field3d(n, 5).reshape(-1, order="F")
+    target = 0.2 * systems.gaussian_field3d(n, 6).reshape(-1, order="F")
+    dx = 1.0 / (n - 1)
+    dt = 0.1 * dx * dx / (2.0 * np.max(d0))
+    # forward with checkpoints, then the adjoint
+    u = _dev(u0)
+    print("  adjoint forward", flush=True)
+    prob.forward_checkpointed(u, 0.0, dt, nsteps, every)
+    uT = u.cpu().numpy()
+    lam = _dev(uT - target)
+    g = torch.zeros(2 * n ** 3, dtype=torch.float64, device="cuda")
+    print("  adjoint backward", flush=True)
+    prob.adjoint(lam, g)
+    gd0 = g[: n ** 3].cpu().numpy()
+    # the checkpointed forward is the plain forward
+    L_gpu, u_plain = _loss_and_state(prob, u0, dt, nsteps, target)
+    assert np.array_equal(u_plain.cpu().numpy(), uT)
+    # oracle forward mode along a random direction
+    vdir = rng.standard_normal(n ** 3)
+    L_orc, dL = orc.nonlindiff3d_jvp(n, d0, vdir, src, u0, target, dt, nsteps)
+    assert abs(L_gpu - L_orc) <= 1e-12 * abs(L_orc)
+    assert abs(gd0 @ vdir - dL) <= 1e-8 * abs(dL), (gd0 @ vdir, dL)
+    # central differences of the GPU solve on random cells (Richardson extrapolated)
+    cells = rng.choice(n ** 3, size=6, replace=False)
+    d0_dev = _dev(d0)
+    prob.set_grid_field("d0", d0_dev)
+    for c in cells:
+        def L_at(e):
+            d0_dev.copy_(_dev(d0))
+            d0_dev[c] += e
+            return _loss_and_state(prob, u0, dt, nsteps, target)[0]
+        h = 1e-3 * d0[c]
+        D1 = (L_at(h) - L_at(-h)) / (2 * h)
+        D2 = (L_at(h / 2) - L_at(-h / 2)) / h
+        fd = (4 * D2 - D1) / 3
+        assert abs(gd0[c] - fd) <= 1e-6 * abs(fd) + 1e-9 * np.max(np.abs(gd0)), (c, gd0[c], fd)
+    d0_dev.copy_(_dev(d0))
+
+
+def test_autograd_function_returns_the_adjoint_gradients():
+    """molgpu.autograd.rk4_solve: gradients w.r.t. u0 and d0 equal the direct adjoint calls."""
+    torch = _torch()
+    from molgpu.autograd import rk4_solve
+    n, nsteps, every = 12, 10, 4
+    prob, d0, src = _nonlin(n, seed=4)
+    dt = 0.1 * (1.0 / (n - 1)) ** 2 / (2.0 * np.max(d0))
+    u0 = _dev(0.3 * systems.gaussian_field3d(n, 7).reshape(-1, order="F")).requires_grad_(True)
+    d0_t = _dev(d0).requires_grad_(True)
+    target = _dev(0.1 * systems.gaussian_field3d(n, 8).reshape(-1, order="F"))
+    print("  autograd solve", flush=True)
+    uT = rk4_solve(prob, u0, 0.0, dt, nsteps, every, {"d0": d0_t})
+    loss = 0.5 * ((uT - target) ** 2).sum()
+    loss.backward()
+    # direct calls
+    u = u0.detach().clone()
+    prob.forward_checkpointed(u, 0.0, dt, nsteps, every)
+    lam = (u - target).contiguous()
+    g = torch.zeros(2 * n ** 3, dtype=torch.float64, device="cuda")
+    prob.adjoint(lam, g)
+    # the reverse sweep sums tap adjoints with atomics: equal up to the order of summation
+    assert torch.allclose(u0.grad, lam, rtol=1e-12, atol=1e-14 * float(lam.abs().max()))
+    assert torch.allclose(d0_t.grad, g[: n ** 3], rtol=1e-12, atol=1e-14 * float(g.abs().max()))

@@ -284,9 +284,19 @@ def test_generated_kernel_compiles_for_gfx950(name):
     import molgpu
     sys_, disc = _codegen_systems()[name]()
     src = molgpu.codegen(lower(sys_, disc), compile=True)
-    assert 'extern "C" __global__' in src and "mol_core_tile<Geo>" in src
+    assert 'extern "C" __global__' in src and "mol_full_tile<Geo>" in src
     # the interpreter is not part of the generated kernel
     assert "stack_prog" not in src and "cpool" not in src
+
+
+@pytest.mark.parametrize("name", sorted(_codegen_systems()))
+def test_generated_vjp_kernel_compiles_for_gfx950(name):
+    """The reverse-mode kernel of every slot kind compiles; LDS tap adjoints use atomics and
+    ghost-node adjoints go through the transposed closures."""
+    import molgpu
+    sys_, disc = _codegen_systems()[name]()
+    src = molgpu.codegen(lower(sys_, disc), compile=True, vjp=True)
+    assert "mol_vjp_tile<Geo>" in src and "atomicAdd(&Ab[" in src and "static __device__ void scatter(" in src
 
 
 def test_generated_kernel_folds_exact_weights():
@@ -298,8 +308,26 @@ def test_generated_kernel_folds_exact_weights():
     assert "static constexpr int PER0 = 1;" in src and "static constexpr int N0 = 65;" in src
 
 
-def test_codegen_reports_systems_without_core_tiles():
+def _c_hex(x: float) -> str:
+    """printf("%a") spelling of a double (trailing zero hex digits dropped)."""
+    m, e = float(x).hex().split("p")
+    m = m.rstrip("0").rstrip(".")
+    return m + "p" + e
+
+
+def test_generated_kernel_covers_boundary_rows_and_closures():
+    """A small Robin problem has no interior tile: the generated kernel selects the boundary
+    rows per node index and stages ghost nodes from the closure (affine in the unknowns plus
+    c * g)."""
     import molgpu
-    sys_, disc = __import__("molgpu").systems.heat1d(n=21)
-    with pytest.raises(native.MolError, match="no core tiles"):
-        molgpu.codegen(lower(sys_, disc))
+    from molgpu import systems as S
+    sys_, disc = S.heat1d(n=21, D=0.8, bcl=(2.0, 0.5, 0.3), bch=(1.5, -0.25, -0.2), approx_order=4)
+    low = lower(sys_, disc)
+    src = molgpu.codegen(low, compile=True)
+    assert "if constexpr (CORE)" not in src          # no core tile at n = 21
+    assert "switch (I0)" in src and "case 2:" in src and "case 20:" in src
+    # the ghost node left of node 2 combines the unknowns at nodes 2..5 with the closure weights
+    cl = [c for c in low.closures if c.side == 0][0]
+    for m in range(4):
+        assert "T(%s) * u[uidx(0, %d, p1, p2)]" % (_c_hex(cl.C[0, m]), 2 + m) in src
+    assert _c_hex(cl.c[0]) in src
