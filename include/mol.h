@@ -187,6 +187,27 @@ int mol_rhs(mol_handle* h, void* du, const void* u, const void* p, double t, voi
  * out, synchronizes. */
 int mol_rhs_h(mol_handle* h, void* du_h, const void* u_h, const void* p_h, double t);
 
+/* Batch of instances (config: many uneven PDE instances advanced together).  No counterpart in
+ * the reference, which solves one ODEProblem per discretize call; replaces a loop of
+ * discretize/solve over instances (e.g. an EnsembleProblem of MOL problems).  The handle's state
+ * vector is the concatenation of the instances' state vectors (instance-major, each in its own
+ * unknown order, first offsets from mol_batch_offsets); mol_rhs, mol_rhs_h and mol_integrate
+ * then act on all instances with one persistent launch per stage.  Instances must share dtype,
+ * dimension and number of variables and take no grid fields; params[i] holds the nparams values
+ * of instance i (host, FP64; fixed for the handle's life, the p argument of mol_rhs/mol_integrate
+ * is ignored).  Instances of one structure (same operators and boundary kinds) share generated
+ * code; numbers (grid sizes, weights, coefficients) are read per instance.  tile = tx,ty,tz for
+ * every instance, or NULL. */
+int mol_create_batched(const mol_desc* const* ds, const double* const* params, int32_t ninst, const int32_t* tile,
+                       mol_handle** out);
+
+/* Generated source of the persistent kernel of a batch (no GPU needed); flags as mol_codegen. */
+int64_t mol_codegen_batched(const mol_desc* const* ds, const double* const* params, int32_t ninst, const int32_t* tile,
+                            char* buf, int64_t buflen, int32_t flags);
+
+/* First unknown of each of the first n instances of a batched handle. */
+int mol_batch_offsets(const mol_handle* h, int64_t* offs, int32_t n);
+
 /* Tile shape of the stencil kernels (nodes per work-group along x, y, z; 0,0,0 = default).
  * No counterpart in the reference; used by the autotuner.  At most 4096 nodes per tile. */
 int mol_set_tile(mol_handle* h, int32_t tx, int32_t ty, int32_t tz);

@@ -1,5 +1,8 @@
 // C ABI of the stencil library: descriptor validation/upload, RHS and stage-loop drivers.
+#include <array>
 #include <cstdio>
+#include <map>
+#include <sstream>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -54,7 +57,8 @@ void free_handle(mol_handle* h) {
   void* ptrs[] = {h->d_tabs, h->d_slots, h->d_clos, h->d_pool, h->d_code, h->d_cpool,
                   h->d_prog, h->d_coord[0], h->d_coord[1], h->d_coord[2], h->w_y1,
                   h->w_y2, h->w_acc, h->w_comp, h->d_tnow, h->d_ka64, h->d_ka32, h->d_tiles,
-                  h->ck_u, h->ck_t, h->seg_u, h->seg_t, h->adj_y[0], h->adj_y[1], h->adj_y[2], h->adj_y[3]};
+                  h->ck_u, h->ck_t, h->seg_u, h->seg_t, h->adj_y[0], h->adj_y[1], h->adj_y[2], h->adj_y[3],
+                  h->d_insts, h->d_wl, h->d_bdata};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h->h_u) (void)hipHostFree(h->h_u);
@@ -197,8 +201,9 @@ int validate(const mol_desc* d, std::string& msg) {
 }
 
 // lds_pos: LDS bytes per tile + halo position (0: the forward kernels' arrays)
+// clamp: shrink the tile to the box (a batch keeps one tile shape for all its instances)
 template <typename T>
-void fill_kargs(mol_handle* h, const mol_desc* d, KArgs<T>& ka, size_t lds_pos = 0) {
+void fill_kargs(mol_handle* h, const mol_desc* d, KArgs<T>& ka, size_t lds_pos = 0, bool clamp = true) {
   std::memset(&ka, 0, sizeof(ka));
   ka.ndim = d->ndim;
   ka.nfields = d->nfields;
@@ -244,7 +249,8 @@ void fill_kargs(mol_handle* h, const mol_desc* d, KArgs<T>& ka, size_t lds_pos =
     for (int a = 0; a < 3; ++a) v *= tl[a] + 2 * ka.halo[a];
     return v;
   };
-  for (int a = 0; a < 3; ++a) tile[a] = std::max(1, std::min(tile[a], ka.uhi[a] - ka.ulo[a] + 1));
+  for (int a = 0; a < 3; ++a)
+    tile[a] = a >= d->ndim ? 1 : (clamp ? std::max(1, std::min(tile[a], ka.uhi[a] - ka.ulo[a] + 1)) : tile[a]);
   while (tile[0] * tile[1] * tile[2] > 4096) {
     int big = 0;
     for (int a = 1; a < 3; ++a)
@@ -304,6 +310,13 @@ Launch<T> launch_of(mol_handle* h) {
   L.ncore = h->ncore;
   L.nbound = h->nbound;
   L.jit = h->jit_fn;
+  if (h->batch) {
+    L.batch = h->batch_fn;
+    L.insts = (const BInst*)h->d_insts;
+    L.wl = (const int32_t*)h->d_wl;
+    L.nwork = h->nwork;
+    L.bgrid = h->batch_grid;
+  }
   L.ntiles = h->ka64.ntile[0] * h->ka64.ntile[1] * h->ka64.ntile[2];
   const KArgs<T>& ka = kargs<T>(h);
   for (int a = 0; a < 3; ++a) L.jp.coord[a] = ka.coord[a];
@@ -825,6 +838,203 @@ int adjoint_impl(mol_handle* h, const T* p, T* lam, double* g_aux, hipStream_t s
   return 0;
 }
 
+// Tiles of one instance whose tile + halo lies inside every variable's box (plain staging).
+void simple_range(const mol_desc* d, const KArgs<double>& ka, int a, int& lo_t, int& hi_t) {
+  if (a >= d->ndim || ka.periodic[a]) {
+    lo_t = 0;
+    hi_t = ka.ntile[a] - 1;
+    return;
+  }
+  int lo = -(1 << 30), hi = 1 << 30;
+  for (int f = 0; f < d->nfields; ++f) {
+    lo = std::max(lo, ka.lo[f][a]);
+    hi = std::min(hi, ka.hi[f][a]);
+  }
+  lo_t = ka.ntile[a];
+  hi_t = -1;
+  for (int t = 0; t < ka.ntile[a]; ++t) {
+    const int o0 = ka.ulo[a] + t * ka.tile[a];
+    if (o0 - ka.halo[a] >= lo && o0 + ka.tile[a] - 1 + ka.halo[a] <= hi) {
+      lo_t = std::min(lo_t, t);
+      hi_t = std::max(hi_t, t);
+    }
+  }
+}
+
+// Host side of a batch: instance headers (device pointers left as offsets into `data`), the
+// work list, the weights/constants/coordinates of every instance and the generated source.
+struct BatchPlan {
+  std::vector<BInst> hdr;
+  std::vector<std::array<int64_t, 5>> dofs;   // offsets of W, K, coord0..2 in data
+  std::vector<double> data;
+  std::vector<int32_t> wl;
+  std::vector<int64_t> inst_off;
+  int64_t nunk = 0;
+  int nclasses = 0;
+  int tile[3] = {1, 1, 1};
+  std::string src;
+};
+
+int plan_batched(const mol_desc* const* ds, const double* const* params, int32_t ninst, const int32_t* tile,
+                 BatchPlan& P) {
+  std::string msg;
+  for (int i = 0; i < ninst; ++i) {
+    if (!ds[i]) return fail(nullptr, MOL_ERR_INVALID, "descriptor " + std::to_string(i) + " is NULL");
+    if (int rc = validate(ds[i], msg)) return fail(nullptr, rc, "instance " + std::to_string(i) + ": " + msg);
+    if (ds[i]->dtype != ds[0]->dtype || ds[i]->ndim != ds[0]->ndim || ds[i]->nfields != ds[0]->nfields)
+      return fail(nullptr, MOL_ERR_INVALID, "instances differ in dtype, dimension or number of variables");
+    if (ds[i]->naux != 0) return fail(nullptr, MOL_ERR_UNSUPPORTED, "batched instances take no grid fields");
+    if (ds[i]->nparams > 0 && (!params || !params[i]))
+      return fail(nullptr, MOL_ERR_INVALID, "parameter values of instance " + std::to_string(i) + " missing");
+  }
+  const size_t esize = ds[0]->dtype == MOL_F64 ? sizeof(double) : sizeof(float);
+  std::map<std::string, int> cls_of;
+  std::vector<std::string> cls_src;
+  P.hdr.assign(ninst, BInst{});
+  P.dofs.assign(ninst, {});
+  int ldsn = 1;
+  for (int i = 0; i < ninst; ++i) {
+    mol_handle th;
+    th.desc.assign(ds[i]);
+    const mol_desc* d = &th.desc.d;
+    th.dtype = d->dtype;
+    if (tile)
+      for (int a = 0; a < 3; ++a) th.tile_req[a] = tile[a];
+    fill_kargs<double>(&th, d, th.ka64, (size_t)d->nfields * esize, false);
+    std::vector<int32_t> gen;
+    int ncore = 0;
+    build_core<double>(&th, d, th.ka64, th.kc64, gen, ncore);
+    const KArgs<double>& ka = th.ka64;
+    ldsn = std::max(ldsn, d->nfields * ka.vol);
+    if (i == 0)
+      for (int a = 0; a < 3; ++a) P.tile[a] = ka.tile[a];
+    std::string err;
+    const std::string src = jit_class_source(d, geom_of(ka, th.kc64, ncore), "GeoX", err);
+    if (src.empty()) return fail(nullptr, MOL_ERR_UNSUPPORTED, "instance " + std::to_string(i) + ": " + err);
+    auto it = cls_of.find(src);
+    int cls;
+    if (it == cls_of.end()) {
+      cls = (int)cls_src.size();
+      cls_of.emplace(src, cls);
+      cls_src.push_back(src);
+    } else {
+      cls = it->second;
+    }
+    BInst& b = P.hdr[i];
+    std::memset(&b, 0, sizeof(b));
+    for (int a = 0; a < 3; ++a) {
+      b.n[a] = ka.n[a];
+      b.ulo[a] = ka.ulo[a];
+      b.uhi[a] = ka.uhi[a];
+      b.nt[a] = ka.ntile[a];
+      simple_range(d, ka, a, b.slo[a], b.shi[a]);
+      b.clo[a] = ncore > 0 ? th.kc64.ct_lo[a] : 1;
+      b.chi[a] = ncore > 0 ? th.kc64.ct_lo[a] + th.kc64.ct_n[a] - 1 : 0;
+    }
+    for (int f = 0; f < MOL_MAX_FIELDS; ++f) {
+      for (int a = 0; a < 3; ++a) {
+        b.lo[f][a] = ka.lo[f][a];
+        b.hi[f][a] = ka.hi[f][a];
+      }
+      b.fofs[f] = P.nunk + ka.fofs[f];
+    }
+    b.cls = cls;
+    b.first = (int32_t)P.wl.size();
+    const int64_t nt = (int64_t)ka.ntile[0] * ka.ntile[1] * ka.ntile[2];
+    if ((int64_t)P.wl.size() + nt > (int64_t)INT32_MAX / 2) return fail(nullptr, MOL_ERR_SHAPE, "batch too large");
+    for (int64_t k = 0; k < nt; ++k) P.wl.push_back(i);
+    P.inst_off.push_back(P.nunk);
+    P.nunk += th.nunk;
+    std::vector<double>& data = P.data;
+    P.dofs[i][0] = (int64_t)data.size();
+    data.insert(data.end(), d->pool_h, d->pool_h + d->npool);
+    P.dofs[i][1] = (int64_t)data.size();
+    data.insert(data.end(), d->cpool_h, d->cpool_h + d->ncpool);
+    for (int k = 0; k < d->nslots; ++k) {
+      data.push_back(d->slots_h[k].eps);
+      data.push_back(d->slots_h[k].inv_dx);
+    }
+    for (int k = 0; k < d->nparams; ++k) data.push_back(params[i][k]);
+    for (int a = 0; a < 3; ++a) {
+      P.dofs[i][2 + a] = (int64_t)data.size();
+      if (a < d->ndim) data.insert(data.end(), d->coord_h[a], d->coord_h[a] + d->n[a]);
+      else data.push_back(0.0);
+    }
+  }
+  P.nclasses = (int)cls_src.size();
+  // one kernel: the class bodies and the persistent loop over the work list
+  std::ostringstream src;
+  src << "// generated persistent kernel of a batch of " << ninst << " instances, " << cls_src.size() << " classes\n";
+  src << "#define MOL_T " << (ds[0]->dtype == MOL_F64 ? "double" : "float") << "\n#include \"mol_jit_prelude.h\"\n\n";
+  src << "static_assert(sizeof(BInst) == " << sizeof(BInst) << " && sizeof(Epi) == "
+      << (ds[0]->dtype == MOL_F64 ? sizeof(Epi<double>) : sizeof(Epi<float>))
+      << ", \"argument layout differs from the library\");\n\n";
+  for (size_t c = 0; c < cls_src.size(); ++c) {
+    std::string body = cls_src[c];
+    const std::string from = "struct GeoX {";
+    body.replace(body.find(from), from.size(), "struct Geo" + std::to_string(c) + " {");
+    src << body << "\n";
+  }
+  src << "constexpr int LDSN = " << ldsn << ";\n";
+  src << "extern \"C\" __global__ void __launch_bounds__(256) mol_batch_tiles(const T* __restrict__ u, Epi ep,\n"
+         "    const BInst* __restrict__ insts, const int* __restrict__ wl, const int nwork) {\n"
+         "  __shared__ T L[LDSN];\n"
+         "  const double t = (ep.tnow ? ep.tnow[0] : 0.0) + ep.tofs;\n"
+         "  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {\n"
+         "    const BInst& hd = insts[wl[w]];\n"
+         "    const int tile = w - hd.first;\n"
+         "    switch (hd.cls) {\n";
+  for (size_t c = 0; c < cls_src.size(); ++c)
+    src << "    case " << c << ": mol_batch_tile<Geo" << c << ">(L, hd, tile, u, ep, t); break;\n";
+  src << "    default: break;\n    }\n  }\n}\n";
+  P.src = src.str();
+  return 0;
+}
+
+int create_batched(const mol_desc* const* ds, const double* const* params, int32_t ninst, const int32_t* tile,
+                   mol_handle** out) {
+  BatchPlan P;
+  if (int rc = plan_batched(ds, params, ninst, tile, P)) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, MOL_ERR_HIP, "no HIP device available");
+  mol_handle* h = new mol_handle();
+  h->batch = true;
+  h->desc.assign(ds[0]);
+  (void)hipGetDevice(&h->device);
+  h->dtype = ds[0]->dtype;
+  h->ndim = ds[0]->ndim;
+  h->nfields = ds[0]->nfields;
+  h->ninst = ninst;
+  for (int a = 0; a < 3; ++a) h->ka64.tile[a] = P.tile[a];
+  h->nunk = P.nunk;
+  h->inst_off = P.inst_off;
+  h->nwork = (int)P.wl.size();
+  h->nclasses = P.nclasses;
+  int r = upload(h, &h->d_bdata, P.data.data(), P.data.size());
+  const double* base = (const double*)h->d_bdata;
+  for (int i = 0; i < ninst; ++i) {
+    P.hdr[i].W = base + P.dofs[i][0];
+    P.hdr[i].K = base + P.dofs[i][1];
+    for (int a = 0; a < 3; ++a) P.hdr[i].coord[a] = base + P.dofs[i][2 + a];
+  }
+  r |= upload(h, &h->d_insts, P.hdr.data(), P.hdr.size());
+  r |= upload(h, &h->d_wl, P.wl.data(), P.wl.size());
+  if (r) {
+    set_global_error(h->err);
+    free_handle(h);
+    return MOL_ERR_HIP;
+  }
+  std::string err;
+  if (jit_kernel(P.src, "mol_batch_tiles", h->device, &h->batch_fn, err)) {
+    set_global_error("batched kernel: " + err);
+    free_handle(h);
+    return MOL_ERR_HIP;
+  }
+  h->batch_grid = std::min(h->nwork, 2048);
+  *out = h;
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -873,8 +1083,45 @@ int mol_create(const mol_desc* d, mol_handle** out) {
   return 0;
 }
 
+int mol_create_batched(const mol_desc* const* ds, const double* const* params, int32_t ninst, const int32_t* tile,
+                       mol_handle** out) {
+  if (!out) return fail(nullptr, MOL_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  if (!ds || ninst < 1) return fail(nullptr, MOL_ERR_INVALID, "no instances");
+  if (tile && ((int64_t)std::max(tile[0], 1) * std::max(tile[1], 1) * std::max(tile[2], 1) > 4096 || tile[0] < 0 ||
+               tile[1] < 0 || tile[2] < 0))
+    return fail(nullptr, MOL_ERR_INVALID, "tile shape out of range (at most 4096 nodes)");
+  return create_batched(ds, params, ninst, tile, out);
+}
+
+int64_t mol_codegen_batched(const mol_desc* const* ds, const double* const* params, int32_t ninst, const int32_t* tile,
+                            char* buf, int64_t buflen, int32_t flags) {
+  if (!ds || ninst < 1) return fail(nullptr, MOL_ERR_INVALID, "no instances");
+  BatchPlan P;
+  if (int rc = plan_batched(ds, params, ninst, tile, P)) return rc;
+  if (flags & MOL_CODEGEN_COMPILE) {
+    std::vector<char> code;
+    std::string log;
+    if (jit_compile(P.src, code, log)) return fail(nullptr, MOL_ERR_HIP, "hiprtc: " + log);
+  }
+  if (buf && buflen > 0) {
+    const size_t n = std::min<size_t>(P.src.size(), (size_t)buflen - 1);
+    std::memcpy(buf, P.src.data(), n);
+    buf[n] = '\0';
+  }
+  return (int64_t)P.src.size();
+}
+
+int mol_batch_offsets(const mol_handle* h, int64_t* offs, int32_t n) {
+  if (!h || !offs) return MOL_ERR_INVALID;
+  if (!h->batch) return MOL_ERR_INVALID;
+  for (int i = 0; i < n && i < h->ninst; ++i) offs[i] = h->inst_off[i];
+  return 0;
+}
+
 int mol_set_tile(mol_handle* h, int32_t tx, int32_t ty, int32_t tz) {
   if (!h) return fail(nullptr, MOL_ERR_INVALID, "handle is NULL");
+  if (h->batch) return fail(h, MOL_ERR_UNSUPPORTED, "batched handles take their tile at creation");
   if (tx < 0 || ty < 0 || tz < 0 || (int64_t)std::max(tx, 1) * std::max(ty, 1) * std::max(tz, 1) > 4096)
     return fail(h, MOL_ERR_INVALID, "tile shape out of range (at most 4096 nodes)");
   h->tile_req[0] = tx;
@@ -901,6 +1148,12 @@ int mol_set_tile(mol_handle* h, int32_t tx, int32_t ty, int32_t tz) {
 
 int mol_get_plan(const mol_handle* h, int32_t* info, int32_t ninfo) {
   if (!h || !info) return MOL_ERR_INVALID;
+  if (h->batch) {
+    // batch: tile, work items, 0, 0, generated, number of classes
+    const int32_t b[8] = {h->ka64.tile[0], h->ka64.tile[1], h->ka64.tile[2], h->nwork, 0, 0, 1, h->nclasses};
+    for (int i = 0; i < ninfo && i < 8; ++i) info[i] = b[i];
+    return 0;
+  }
   int32_t v[8] = {h->ka64.tile[0], h->ka64.tile[1], h->ka64.tile[2], h->ncore, h->nbound,
                   h->use_weno ? h->nweno : 0, h->jit_fn != nullptr && !h->use_weno ? 1 : 0,
                   h->core_interp_ok ? 1 : 0};
@@ -989,6 +1242,7 @@ int mol_integrate(mol_handle* h, void* u, const void* p, double t0, double dt, i
 int mol_vjp(mol_handle* h, const void* u, const void* p, double t, const void* lam, void* lam_dot, double* g_aux,
             void* stream) {
   if (!h) return fail(nullptr, MOL_ERR_INVALID, "handle is NULL");
+  if (h->batch) return fail(h, MOL_ERR_UNSUPPORTED, "reverse mode of batched handles");
   if (!u || !lam || !lam_dot) return fail(h, MOL_ERR_INVALID, "u/lam/lam_dot is NULL");
   if (h->nparams > 0 && !p) return fail(h, MOL_ERR_INVALID, "parameters missing");
   if (int rc = check_aux(h)) return rc;
@@ -1011,6 +1265,7 @@ int mol_vjp(mol_handle* h, const void* u, const void* p, double t, const void* l
 int mol_forward_checkpointed(mol_handle* h, void* u, const void* p, double t0, double dt, int64_t nsteps,
                              int32_t ckpt_every, void* stream) {
   if (!h) return fail(nullptr, MOL_ERR_INVALID, "handle is NULL");
+  if (h->batch) return fail(h, MOL_ERR_UNSUPPORTED, "reverse mode of batched handles");
   if (!u) return fail(h, MOL_ERR_INVALID, "u is NULL");
   if (nsteps < 1 || ckpt_every < 1) return fail(h, MOL_ERR_INVALID, "nsteps and ckpt_every must be >= 1");
   if (h->nparams > 0 && !p) return fail(h, MOL_ERR_INVALID, "parameters missing");
@@ -1023,6 +1278,7 @@ int mol_forward_checkpointed(mol_handle* h, void* u, const void* p, double t0, d
 
 int mol_adjoint(mol_handle* h, const void* p, void* lam, double* g_aux, void* stream) {
   if (!h) return fail(nullptr, MOL_ERR_INVALID, "handle is NULL");
+  if (h->batch) return fail(h, MOL_ERR_UNSUPPORTED, "reverse mode of batched handles");
   if (!lam) return fail(h, MOL_ERR_INVALID, "lam is NULL");
   if (!h->ck_valid) return fail(h, MOL_ERR_INVALID, "no checkpointed run (call mol_forward_checkpointed first)");
   if (h->nparams > 0 && !p) return fail(h, MOL_ERR_INVALID, "parameters missing");

@@ -36,6 +36,21 @@ struct JitPtrs {
 // Code generation and in-process compilation of the tile kernel (mol_jit.cpp).
 std::string jit_source(const mol_desc* d, const JitGeom& g, std::string& err);
 std::string jit_vjp_source(const mol_desc* d, const JitGeom& g, std::string& err);
+std::string jit_class_source(const mol_desc* d, const JitGeom& g, const std::string& name, std::string& err);
+
+// Header of one instance of a batch (layout identical to BInst of mol_jit_prelude.h).
+struct BInst {
+  int32_t n[3];
+  int32_t lo[MOL_MAX_FIELDS][3];
+  int32_t hi[MOL_MAX_FIELDS][3];
+  int64_t fofs[MOL_MAX_FIELDS];
+  int32_t ulo[3], uhi[3], nt[3];
+  int32_t slo[3], shi[3], clo[3], chi[3];
+  int32_t cls, first;
+  const double* coord[3];
+  const double* W;
+  const double* K;
+};
 int jit_compile(const std::string& src, std::vector<char>& code, std::string& log);
 int jit_kernel(const std::string& src, const char* name, int device, hipFunction_t* fn, std::string& err);
 
@@ -68,6 +83,10 @@ struct Launch {
   int nweno = 0;
   hipFunction_t jit = nullptr;       // generated kernel over all tiles (nullptr: interpreter)
   int ntiles = 0;
+  hipFunction_t batch = nullptr;     // persistent kernel over the work list of a batch
+  const BInst* insts = nullptr;
+  const int32_t* wl = nullptr;
+  int nwork = 0, bgrid = 0;
   JitPtrs<T> jp{};
 };
 
@@ -150,6 +169,15 @@ struct mol_handle {
   hipFunction_t jit_fn = nullptr;         // generated kernel over all tiles
   std::string jit_reason;                 // why it is not used, if it is not
   // reverse mode
+  // batch of instances (mol_create_batched)
+  bool batch = false;
+  hipFunction_t batch_fn = nullptr;
+  int nwork = 0, batch_grid = 0, nclasses = 0, ninst = 0;
+  int batch_tile[3] = {0, 0, 0};
+  void* d_insts = nullptr;
+  void* d_wl = nullptr;
+  void* d_bdata = nullptr;                // weights, constants and coordinates of all instances
+  std::vector<int64_t> inst_off;          // first unknown of each instance
   hipFunction_t vjp_fn = nullptr;         // generated vector-Jacobian product kernel
   int vjp_tiles = 0;
   bool vjp_aux_ok = true;                 // no boundary value reads an auxiliary array

@@ -55,6 +55,9 @@ struct Node {
   std::vector<int> a;        // operands
   std::vector<double> w;     // X_LINCOMB weights; X_WENO: eps, inv_dx
   double val = 0;            // N_LIT
+  int src = 0;               // N_LIT: 0 structural constant, 2 constant-pool entry sidx
+  long long sidx = -1;
+  std::vector<long long> wi, wj;   // X_LINCOMB: pool index of each weight (wj >= 0: product wi*wj)
   int arr = 0;               // N_LDS: array (variable, or nfields + aux index)
   long long off = 0;         //        displacement in LDS elements
   std::string text;          // N_EXPR
@@ -74,6 +77,7 @@ struct Region {
 // table (such nodes are never evaluated).
 struct RowR {
   std::vector<double> w;
+  std::vector<long long> widx;   // pool index of each weight
   std::vector<int> rel;
   bool valid = true;
 };
@@ -101,6 +105,9 @@ class Gen {
 
   std::string source(bool f64);       // forward kernel mol_rhs_tiles
   std::string source_vjp(bool f64);   // reverse-mode kernel mol_vjp_tiles
+  // Parametric class body for batched instances: weights, constants, grid sizes and boxes are
+  // read from the instance header `hd`; the text is identical for instances of one structure.
+  std::string source_class(const std::string& name);
   std::string error;
 
  private:
@@ -112,6 +119,33 @@ class Gen {
   std::map<int, int> slot_node_;      // slot -> node id (per node function)
   bool core_ = true;                  // building the interior-rows variant
   bool ok_ = true;
+  bool param_ = false;                // parametric (batched) printing
+
+  // instance-dependent numbers: literal in a specialised kernel, header loads in a class body
+  std::string pw(double v, long long idx) const { return param_ ? "(T)hd.W[" + istr(idx) + "]" : lit(v); }
+  std::string pk(double v, long long idx) const { return param_ ? "(T)hd.K[" + istr(idx) + "]" : lit(v); }
+  std::string NN(int a) const { return param_ ? "hd.n[" + istr(a) + "]" : istr(g_.n[a]); }
+  // node index c along axis a: indices in the upper half are written relative to n
+  std::string IX(int a, long long c) const {
+    if (!param_ || c <= g_.n[a] / 2) return istr(c);
+    return "(hd.n[" + istr(a) + "] - " + istr(g_.n[a] - c) + ")";
+  }
+  std::string LO(int f, int a) const { return param_ ? "hd.lo[" + istr(f) + "][" + istr(a) + "]" : istr(g_.lo[f][a]); }
+  std::string HI(int f, int a) const { return param_ ? "hd.hi[" + istr(f) + "][" + istr(a) + "]" : istr(g_.hi[f][a]); }
+  // position of the m-th unknown a ghost closure combines (counted inwards from the box edge)
+  std::string closure_pos(int f, int a, int side, int m) const {
+    if (!param_) return istr(side == 0 ? g_.lo[f][a] + m : g_.hi[f][a] - m);
+    return side == 0 ? "(" + LO(f, a) + " + " + istr(m) + ")" : "(" + HI(f, a) + " - " + istr(m) + ")";
+  }
+  std::string UIDX(int f, const std::string& q0, const std::string& q1, const std::string& q2) const {
+    return param_ ? "mol_b_uidx(hd, " + istr(f) + ", " + q0 + ", " + q1 + ", " + q2 + ")"
+                  : "uidx(" + istr(f) + ", " + q0 + ", " + q1 + ", " + q2 + ")";
+  }
+  std::string weight(const Node& n, size_t k) const {
+    if (!param_ || n.wi.empty()) return lit(n.w[k]);
+    if (n.wj[k] >= 0) return "(T)(hd.W[" + istr(n.wi[k]) + "] * hd.W[" + istr(n.wj[k]) + "])";
+    return "(T)hd.W[" + istr(n.wi[k]) + "]";
+  }
 
   void bad(const std::string& m) {
     if (ok_) error = m;
@@ -137,6 +171,14 @@ class Gen {
     n.off = off;
     return add(n);
   }
+  int klit_node(int idx) {
+    Node n;
+    n.kind = N_LIT;
+    n.val = d_->cpool_h[idx];
+    n.src = 2;
+    n.sidx = idx;
+    return add(n);
+  }
   int expr(const std::string& t) {
     Node n;
     n.kind = N_EXPR;
@@ -151,14 +193,20 @@ class Gen {
     n.imm = imm;
     return add(n);
   }
-  int lincomb(const std::vector<double>& w, const std::vector<int>& args) {
+  // sum_k w[k] * args[k]; wi/wj: pool indices of the weights (wj[k] >= 0 for products)
+  int lincomb(const std::vector<double>& w, const std::vector<int>& args, const std::vector<long long>& wi = {},
+              const std::vector<long long>& wj = {}) {
     Node n;
     n.kind = N_OP;
     n.op = X_LINCOMB;
     for (size_t k = 0; k < w.size(); ++k)
-      if (w[k] != 0.0) {
+      if (w[k] != 0.0 || param_) {
         n.w.push_back(w[k]);
         n.a.push_back(args[k]);
+        if (!wi.empty()) {
+          n.wi.push_back(wi[k]);
+          n.wj.push_back(wj.empty() ? -1 : wj[k]);
+        }
       }
     if (n.a.empty()) return lit_node(0.0);
     return add(n);
@@ -173,9 +221,9 @@ class Gen {
   std::string coord_tap(int d, int off) const {
     if (off == 0) return coord_node(d);
     const std::string k = "I" + istr(d) + (off > 0 ? " + " + istr(off) : " - " + istr(-off));
-    if (g_.periodic[d]) return "(T)jp.coord[" + istr(d) + "][mol_wrap(" + k + ", " + istr(g_.n[d]) + ") - 1]";
+    if (g_.periodic[d]) return "(T)jp.coord[" + istr(d) + "][mol_wrap(" + k + ", " + NN(d) + ") - 1]";
     // ghost positions past the ends read the end coordinate (never an address outside the array)
-    return "(T)jp.coord[" + istr(d) + "][min(max(" + k + ", 1), " + istr(g_.n[d]) + ") - 1]";
+    return "(T)jp.coord[" + istr(d) + "][min(max(" + k + ", 1), " + NN(d) + ") - 1]";
   }
 
   RowR row(const mol_optab& tb, int ax, const Var& v, int idx_rel);
@@ -183,7 +231,7 @@ class Gen {
     if (!r.valid) return lit_node(0.0);
     std::vector<int> args;
     for (size_t k = 0; k < r.w.size(); ++k) args.push_back(lds(arr, (long long)r.rel[k] * stride(ax)));
-    return lincomb(r.w, args);
+    return lincomb(r.w, args, r.widx);
   }
   void slot_range(const mol_slot& s, int ax, int& lo, int& hi) const;
   int leaf(uint32_t op, int imm, const Ctx& c);
@@ -203,6 +251,15 @@ class Gen {
   void partials(const Node& n, int id, std::vector<std::pair<int, std::string>>& out) const;
   std::string stage_fn();
   std::string scatter_fn();
+  // sizes of the argument structs as the library lays them out, checked in the generated code
+  static std::string layout_asserts(bool f64) {
+    const size_t epi = f64 ? sizeof(Epi<double>) : sizeof(Epi<float>);
+    const size_t jp = f64 ? sizeof(JitPtrs<double>) : sizeof(JitPtrs<float>);
+    const size_t va = f64 ? sizeof(VArgs<double>) : sizeof(VArgs<float>);
+    return "static_assert(sizeof(Epi) == " + istr((long long)epi) + " && sizeof(JPtrs) == " + istr((long long)jp) +
+           " && sizeof(VArgs) == " + istr((long long)va) + " && sizeof(BInst) == " +
+           istr((long long)sizeof(BInst)) + ", \"argument layout differs from the library\");\n\n";
+  }
   std::string geo_common();
 };
 
@@ -214,6 +271,7 @@ RowR Gen::row(const mol_optab& tb, int ax, const Var& v, int idx_rel) {
   if (v.interior || g_.periodic[ax]) {
     for (int k = 0; k < tb.len; ++k) {
       r.w.push_back(d_->pool_h[tb.w + k]);
+      r.widx.push_back(tb.w + k);
       r.rel.push_back(idx_rel + tb.off + k);
     }
     return r;
@@ -241,6 +299,7 @@ RowR Gen::row(const mol_optab& tb, int ax, const Var& v, int idx_rel) {
   if (w0 < 0 || w0 + cnt > d_->npool) { r.valid = false; return r; }
   for (int k = 0; k < cnt; ++k) {
     r.w.push_back(d_->pool_h[w0 + k]);
+    r.widx.push_back(w0 + k);
     r.rel.push_back(a0 + k - v.i);
   }
   return r;
@@ -284,21 +343,24 @@ void Gen::slot_range(const mol_slot& s, int ax, int& lo, int& hi) const {
 
 int Gen::leaf(uint32_t code, int imm, const Ctx& c) {
   switch (code) {
-    case MOL_OP_CONST: return lit_node(d_->cpool_h[imm]);
+    case MOL_OP_CONST: return klit_node(imm);
     case MOL_OP_TIME: return expr("(T)t");
-    case MOL_OP_PARAM: return expr("P[" + istr(imm) + "]");
+    case MOL_OP_PARAM:
+      // batched instances carry their parameter values after the constants, eps and 1/dx
+      if (param_) return expr("(T)hd.K[" + istr(d_->ncpool + 2LL * d_->nslots + imm) + "]");
+      return expr("P[" + istr(imm) + "]");
     default: break;
   }
   if (c.kind == CTX_BOUNDARY) {
     // ghost-node program G(t, coordinates of the boundary node): no variables
     const std::string q[3] = {"p0", "p1", "p2"};
     std::string idx[3] = {q[0], q[1], q[2]};
-    idx[c.ax] = c.side == 0 ? "1" : istr(g_.n[c.ax]);
+    idx[c.ax] = c.side == 0 ? "1" : NN(c.ax);
     switch (code) {
       case MOL_OP_COORD: return expr("(T)jp.coord[" + istr(imm) + "][" + idx[imm] + " - 1]");
       case MOL_OP_AUX:
-        return expr("jp.aux[" + istr(imm) + "][(i64)(" + idx[0] + " - 1) + (i64)" + istr(g_.n[0]) + " * ((" +
-                    idx[1] + " - 1) + (i64)" + istr(g_.n[1]) + " * (" + idx[2] + " - 1))]");
+        return expr("jp.aux[" + istr(imm) + "][(i64)(" + idx[0] + " - 1) + (i64)" + NN(0) + " * ((" +
+                    idx[1] + " - 1) + (i64)" + NN(1) + " * (" + idx[2] + " - 1))]");
       default: return lit_node(0.0);
     }
   }
@@ -311,7 +373,7 @@ int Gen::leaf(uint32_t code, int imm, const Ctx& c) {
         if (!c.ip->valid) return lit_node(0.0);
         std::string e;
         for (size_t k = 0; k < c.ip->w.size(); ++k) {
-          const std::string term = lit(c.ip->w[k]) + " * " + coord_tap(imm, c.ip->rel[k]);
+          const std::string term = pw(c.ip->w[k], c.ip->widx[k]) + " * " + coord_tap(imm, c.ip->rel[k]);
           e = k == 0 ? term : e + " + " + term;
         }
         return e.empty() ? lit_node(0.0) : expr("(" + e + ")");
@@ -370,7 +432,7 @@ int Gen::prog(int p, const Ctx& c) {
       if (bconst) {
         if (st.size() < 2) { bad("stack underflow"); return lit_node(0); }
         const int s0 = pop(), s1 = pop();
-        st.push_back(op(X_SELECT, {s1, s0, lit_node(d_->cpool_h[imm])}));
+        st.push_back(op(X_SELECT, {s1, s0, klit_node(imm)}));
       } else {
         if (st.size() < 3) { bad("stack underflow"); return lit_node(0); }
         const int s0 = pop(), s1 = pop(), s2 = pop();
@@ -384,7 +446,7 @@ int Gen::prog(int p, const Ctx& c) {
       if (bconst) {
         if (st.empty()) { bad("stack underflow"); return lit_node(0); }
         a = pop();
-        b = lit_node(d_->cpool_h[imm]);
+        b = klit_node(imm);
       } else {
         if (st.size() < 2) { bad("stack underflow"); return lit_node(0); }
         b = pop();
@@ -422,7 +484,7 @@ int Gen::nonlinlap(const mol_slot& s, int qo, int qi, int qp, const Var& v) {
     const int di = leaf(MOL_OP_DINNER, 0, c);
     terms.push_back(s.prog >= 0 ? op(MOL_OP_MUL, {prog(s.prog, c), di}) : di);
   }
-  return lincomb(ro.w, terms);
+  return lincomb(ro.w, terms, ro.widx);
 }
 
 // Value of slot k for the node-index variants v[axis].
@@ -449,6 +511,7 @@ int Gen::slot_body(int k, const Var* v) {
       n.op = X_WENO;
       for (int q = -2; q <= 2; ++q) n.a.push_back(lds(s.field, q * sd));
       n.w = {s.eps, s.inv_dx};
+      n.imm = k;
       return add(n);
     }
     case MOL_SLOT_NONLINLAP:
@@ -469,13 +532,16 @@ int Gen::slot_body(int k, const Var* v) {
       const RowR ry = row(d_->tabs_h[s.tab[1]], s.axis2, v[s.axis2], 0);
       if (!rx.valid || !ry.valid) return lit_node(0.0);
       std::vector<double> w;
+      std::vector<long long> wi, wj;
       std::vector<int> args;
       for (size_t i = 0; i < rx.w.size(); ++i)
         for (size_t j = 0; j < ry.w.size(); ++j) {
           w.push_back(rx.w[i] * ry.w[j]);
+          wi.push_back(rx.widx[i]);
+          wj.push_back(ry.widx[j]);
           args.push_back(lds(s.field, (long long)rx.rel[i] * stride(ax) + (long long)ry.rel[j] * stride(s.axis2)));
         }
-      return lincomb(w, args);
+      return lincomb(w, args, wi, wj);
     }
     default:
       bad("slot kind " + istr(s.kind) + " not supported by the generator");
@@ -489,6 +555,8 @@ int Gen::slot_branch(int k, const std::vector<int>& axes, size_t a, Var* v) {
   const int ax = axes[a];
   int lo, hi;
   slot_range(d_->slots_h[k], ax, lo, hi);
+  lo = std::max(lo, g_.ulo[ax]);
+  hi = std::min(hi, g_.uhi[ax]);
   std::vector<int> cases;
   for (int i = g_.ulo[ax]; i <= g_.uhi[ax]; ++i)
     if (i < lo || i > hi) cases.push_back(i);
@@ -555,7 +623,7 @@ void Gen::build_node(bool core, std::vector<int>& outs, int& region) {
 std::string Gen::ref(int id) const {
   const Node& n = nodes_[id];
   switch (n.kind) {
-    case N_LIT: return lit(n.val);
+    case N_LIT: return n.src == 2 ? pk(n.val, n.sidx) : lit(n.val);
     case N_LDS: {
       const long long o = (long long)n.arr * g_.vol + n.off;
       if (o == 0) return "L[b]";
@@ -633,7 +701,7 @@ std::string Gen::op_expr(const Node& n, int id) const {
     case X_LINCOMB: {
       std::string e;
       for (size_t k = 0; k < n.a.size(); ++k) {
-        const std::string term = lit(n.w[k]) + " * " + A((int)k);
+        const std::string term = weight(n, k) + " * " + A((int)k);
         e = k == 0 ? term : e + " + " + term;
       }
       return e;
@@ -641,7 +709,8 @@ std::string Gen::op_expr(const Node& n, int id) const {
     case X_WENO:
       // FP64: one-division cleared form; FP32: normalised weights (the cleared form underflows)
       return std::string(d_->dtype == MOL_F64 ? "mol_weno5(" : "mol_weno5_ref(") + A(0) + ", " + A(1) + ", " +
-             A(2) + ", " + A(3) + ", " + A(4) + ", " + lit(n.w[0]) + ", " + lit(n.w[1]) + ")";
+             A(2) + ", " + A(3) + ", " + A(4) + ", " + pk(n.w[0], d_->ncpool + 2LL * n.imm) + ", " +
+             pk(n.w[1], d_->ncpool + 2LL * n.imm + 1) + ")";
     default: {
       const char* fn = unary_fn(n.op);
       return fn ? std::string(fn) + "(" + A(0) + ")" : "T(0)";
@@ -657,17 +726,17 @@ void Gen::print_node(std::ostream& o, int id) const {
   }
   if (n.kind != N_BRANCH) return;
   const std::string v = "v" + istr(id), I = "I" + istr(n.ax);
-  o << "    T " << v << ";\n";
-  o << "    if (" << I << " >= " << n.lo << " && " << I << " <= " << n.hi << ") {\n";
+  o << "    T " << v << " = T(0);\n";
+  o << "    if (" << I << " >= " << IX(n.ax, n.lo) << " && " << I << " <= " << IX(n.ax, n.hi) << ") {\n";
   print_region(o, n.reg[0]);
   o << "    " << v << " = " << ref(regs_[n.reg[0]].result) << ";\n";
-  o << "    } else switch (" << I << ") {\n";
+  o << "    }";
   for (size_t k = 1; k < n.reg.size(); ++k) {
-    o << "    case " << n.cval[k] << ": {\n";
+    o << " else if (" << I << " == " << IX(n.ax, n.cval[k]) << ") {\n";
     print_region(o, n.reg[k]);
-    o << "    " << v << " = " << ref(regs_[n.reg[k]].result) << ";\n    break; }\n";
+    o << "    " << v << " = " << ref(regs_[n.reg[k]].result) << ";\n    }";
   }
-  o << "    default: " << v << " = T(0);\n    }\n";
+  o << "\n";
 }
 
 void Gen::print_region(std::ostream& o, int r) const {
@@ -728,7 +797,7 @@ void Gen::partials(const Node& n, int id, std::vector<std::pair<int, std::string
       if (n.imm != 0) put(0, g + " * T(" + istr(n.imm) + ") * " + powi_expr(A(0), n.imm - 1));
       break;
     case X_LINCOMB:
-      for (size_t k = 0; k < n.a.size(); ++k) put((int)k, lit(n.w[k]) + " * " + g);
+      for (size_t k = 0; k < n.a.size(); ++k) put((int)k, weight(n, k) + " * " + g);
       break;
     case X_WENO:
       for (int k = 0; k < 5; ++k) put(k, g + " * wd" + istr(id) + "[" + istr(k) + "]");
@@ -760,7 +829,8 @@ void Gen::print_reverse(std::ostream& o, int r, const std::vector<std::pair<int,
     if (n.kind == N_OP) {
       if (n.op == X_WENO) {
         o << "    T wd" << id << "[5];\n    mol_weno5_grad(" << ref(n.a[0]) << ", " << ref(n.a[1]) << ", " << ref(n.a[2])
-          << ", " << ref(n.a[3]) << ", " << ref(n.a[4]) << ", " << lit(n.w[0]) << ", " << lit(n.w[1]) << ", wd" << id
+          << ", " << ref(n.a[3]) << ", " << ref(n.a[4]) << ", " << pk(n.w[0], d_->ncpool + 2LL * n.imm) << ", "
+          << pk(n.w[1], d_->ncpool + 2LL * n.imm + 1) << ", wd" << id
           << ", " << (d_->dtype == MOL_F64 ? "true" : "false") << ");\n";
       }
       std::vector<std::pair<int, std::string>> ps;
@@ -770,17 +840,17 @@ void Gen::print_reverse(std::ostream& o, int r, const std::vector<std::pair<int,
     }
     // branch: the region of the taken case runs forward again, then backwards
     const std::string I = "I" + istr(n.ax), g = "g" + istr(id);
-    o << "    if (" << I << " >= " << n.lo << " && " << I << " <= " << n.hi << ") {\n";
+    o << "    if (" << I << " >= " << IX(n.ax, n.lo) << " && " << I << " <= " << IX(n.ax, n.hi) << ") {\n";
     print_region(o, n.reg[0]);
     print_reverse(o, n.reg[0], {{regs_[n.reg[0]].result, g}});
-    o << "    } else switch (" << I << ") {\n";
+    o << "    }";
     for (size_t k = 1; k < n.reg.size(); ++k) {
-      o << "    case " << n.cval[k] << ": {\n";
+      o << " else if (" << I << " == " << IX(n.ax, n.cval[k]) << ") {\n";
       print_region(o, n.reg[k]);
       print_reverse(o, n.reg[k], {{regs_[n.reg[k]].result, g}});
-      o << "    break; }\n";
+      o << "    }";
     }
-    o << "    default: break;\n    }\n";
+    o << "\n";
   }
   for (const auto& kv : acc) {
     const int arr = kv.first.first;
@@ -799,20 +869,21 @@ void Gen::print_reverse(std::ostream& o, int r, const std::vector<std::pair<int,
 // (field_at / closure_value of the interpreter; src/discretization/generate_bc_eqs.jl).
 std::string Gen::stage_fn() {
   std::ostringstream o;
-  o << "  static __device__ T stage(const int f, int p0, int p1, int p2, const double t, const T* __restrict__ P,\n"
+  o << "  static __device__ T stage(" << (param_ ? "const BInst& hd, " : "")
+    << "const int f, int p0, int p1, int p2, const double t, const T* __restrict__ P,\n"
        "                           const T* __restrict__ u, const JPtrs& jp) {\n"
        "    (void)t; (void)P; (void)jp;\n";
   for (int a = 0; a < 3; ++a)
-    if (g_.periodic[a]) o << "    p" << a << " = mol_wrap(p" << a << ", " << g_.n[a] << ");\n";
-  o << "    if (p0 < 1 || p0 > " << g_.n[0] << " || p1 < 1 || p1 > " << g_.n[1] << " || p2 < 1 || p2 > "
-    << g_.n[2] << ") return T(0);\n";
+    if (g_.periodic[a]) o << "    p" << a << " = mol_wrap(p" << a << ", " << NN(a) << ");\n";
+  o << "    if (p0 < 1 || p0 > " << NN(0) << " || p1 < 1 || p1 > " << NN(1) << " || p2 < 1 || p2 > " << NN(2)
+    << ") return T(0);\n";
   o << "    switch (f) {\n";
   for (int f = 0; f < d_->nfields; ++f) {
     o << "    case " << f << ": {\n";
     o << "    const int nout = ";
     for (int a = 0; a < 3; ++a)
-      o << (a ? " + " : "") << "(p" << a << " < " << g_.lo[f][a] << " || p" << a << " > " << g_.hi[f][a] << ")";
-    o << ";\n    if (nout == 0) return u[uidx(" << f << ", p0, p1, p2)];\n    if (nout > 1) return T(0);\n";
+      o << (a ? " + " : "") << "(p" << a << " < " << LO(f, a) << " || p" << a << " > " << HI(f, a) << ")";
+    o << ";\n    if (nout == 0) return u[" << UIDX(f, "p0", "p1", "p2") << "];\n    if (nout > 1) return T(0);\n";
     for (int a = 0; a < d_->ndim; ++a)
       for (int side = 0; side < 2; ++side) {
         int cid = -1;
@@ -821,14 +892,12 @@ std::string Gen::stage_fn() {
           if (cl.field == f && cl.axis == a && cl.side == side) cid = c;
         }
         const std::string pa = "p" + istr(a);
-        o << "    if (" << pa << (side == 0 ? " < " + istr(g_.lo[f][a]) : " > " + istr(g_.hi[f][a])) << ") {\n";
+        o << "    if (" << pa << (side == 0 ? " < " + LO(f, a) : " > " + HI(f, a)) << ") {\n";
         if (cid < 0) {
           o << "    return T(0);\n    }\n";
           continue;
         }
         const mol_closure& cl = d_->closures_h[cid];
-        const int first = side == 0 ? g_.lo[f][a] : g_.hi[f][a];
-        const int dir = side == 0 ? 1 : -1;
         std::string G;
         if (cl.gprog >= 0) {
           const int r = new_region();
@@ -841,25 +910,25 @@ std::string Gen::stage_fn() {
           print_region(o, r);
           G = ref(regs_[r].result);
         }
-        o << "    const int g = " << (side == 0 ? istr(g_.lo[f][a] - 1) + " - " + pa : pa + " - " + istr(g_.hi[f][a] + 1))
-          << ";\n    switch (g) {\n";
+        o << "    const int g = " << (side == 0 ? "(" + LO(f, a) + " - 1) - " + pa : pa + " - (" + HI(f, a) + " + 1)")
+          << ";\n";
         for (int gi = 0; gi < cl.nghost; ++gi) {
           std::string e;
           for (int m = 0; m < cl.K; ++m) {
             const double c = d_->pool_h[cl.wC + gi * cl.K + m];
-            if (c == 0.0) continue;
+            if (c == 0.0 && !param_) continue;
             std::string q[3] = {"p0", "p1", "p2"};
-            q[a] = istr(first + m * dir);
-            const std::string term = lit(c) + " * u[uidx(" + istr(f) + ", " + q[0] + ", " + q[1] + ", " + q[2] + ")]";
+            q[a] = closure_pos(f, a, side, m);
+            const std::string term = pw(c, cl.wC + gi * cl.K + m) + " * u[" + UIDX(f, q[0], q[1], q[2]) + "]";
             e = e.empty() ? term : e + " + " + term;
           }
           if (cl.gprog >= 0) {
-            const std::string term = lit(d_->pool_h[cl.wc + gi]) + " * " + G;
+            const std::string term = pw(d_->pool_h[cl.wc + gi], cl.wc + gi) + " * " + G;
             e = e.empty() ? term : e + " + " + term;
           }
-          o << "    case " << gi << ": return " << (e.empty() ? "T(0)" : e) << ";\n";
+          o << "    if (g == " << gi << ") return " << (e.empty() ? "T(0)" : e) << ";\n";
         }
-        o << "    default: return T(0);\n    }\n    }\n";
+        o << "    return T(0);\n    }\n";
       }
     o << "    return T(0);\n    }\n";
   }
@@ -873,16 +942,16 @@ std::string Gen::scatter_fn() {
   std::ostringstream o;
   o << "  static __device__ void scatter(const int f, int p0, int p1, int p2, const T v, T* __restrict__ out) {\n";
   for (int a = 0; a < 3; ++a)
-    if (g_.periodic[a]) o << "    p" << a << " = mol_wrap(p" << a << ", " << g_.n[a] << ");\n";
-  o << "    if (p0 < 1 || p0 > " << g_.n[0] << " || p1 < 1 || p1 > " << g_.n[1] << " || p2 < 1 || p2 > "
-    << g_.n[2] << ") return;\n";
+    if (g_.periodic[a]) o << "    p" << a << " = mol_wrap(p" << a << ", " << NN(a) << ");\n";
+  o << "    if (p0 < 1 || p0 > " << NN(0) << " || p1 < 1 || p1 > " << NN(1) << " || p2 < 1 || p2 > " << NN(2)
+    << ") return;\n";
   o << "    switch (f) {\n";
   for (int f = 0; f < d_->nfields; ++f) {
     o << "    case " << f << ": {\n";
     o << "    const int nout = ";
     for (int a = 0; a < 3; ++a)
-      o << (a ? " + " : "") << "(p" << a << " < " << g_.lo[f][a] << " || p" << a << " > " << g_.hi[f][a] << ")";
-    o << ";\n    if (nout == 0) { mol_atomic_add(&out[uidx(" << f << ", p0, p1, p2)], v); return; }\n"
+      o << (a ? " + " : "") << "(p" << a << " < " << LO(f, a) << " || p" << a << " > " << HI(f, a) << ")";
+    o << ";\n    if (nout == 0) { mol_atomic_add(&out[" << UIDX(f, "p0", "p1", "p2") << "], v); return; }\n"
       << "    if (nout > 1) return;\n";
     for (int a = 0; a < d_->ndim; ++a)
       for (int side = 0; side < 2; ++side) {
@@ -894,24 +963,22 @@ std::string Gen::scatter_fn() {
         if (cid < 0) continue;
         const mol_closure& cl = d_->closures_h[cid];
         const std::string pa = "p" + istr(a);
-        const int first = side == 0 ? g_.lo[f][a] : g_.hi[f][a];
-        const int dir = side == 0 ? 1 : -1;
-        o << "    if (" << pa << (side == 0 ? " < " + istr(g_.lo[f][a]) : " > " + istr(g_.hi[f][a])) << ") {\n";
-        o << "    const int g = " << (side == 0 ? istr(g_.lo[f][a] - 1) + " - " + pa : pa + " - " + istr(g_.hi[f][a] + 1))
-          << ";\n    switch (g) {\n";
+        o << "    if (" << pa << (side == 0 ? " < " + LO(f, a) : " > " + HI(f, a)) << ") {\n";
+        o << "    const int g = " << (side == 0 ? "(" + LO(f, a) + " - 1) - " + pa : pa + " - (" + HI(f, a) + " + 1)")
+          << ";\n";
         for (int gi = 0; gi < cl.nghost; ++gi) {
-          o << "    case " << gi << ":\n";
+          o << "    if (g == " << gi << ") {\n";
           for (int m = 0; m < cl.K; ++m) {
             const double c = d_->pool_h[cl.wC + gi * cl.K + m];
-            if (c == 0.0) continue;
+            if (c == 0.0 && !param_) continue;
             std::string q[3] = {"p0", "p1", "p2"};
-            q[a] = istr(first + m * dir);
-            o << "      mol_atomic_add(&out[uidx(" << f << ", " << q[0] << ", " << q[1] << ", " << q[2] << ")], " << lit(c)
+            q[a] = closure_pos(f, a, side, m);
+            o << "      mol_atomic_add(&out[" << UIDX(f, q[0], q[1], q[2]) << "], " << pw(c, cl.wC + gi * cl.K + m)
               << " * v);\n";
           }
-          o << "      return;\n";
+          o << "      return;\n    }\n";
         }
-        o << "    default: return;\n    }\n    }\n";
+        o << "    return;\n    }\n";
       }
     o << "    return;\n    }\n";
   }
@@ -1007,6 +1074,7 @@ std::string Gen::source(bool f64) {
   if (g_.ncore > 0) build_node(true, out_core, r_core);
   build_node(false, out_bound, r_bound);
   if (!ok_) return "";
+  o << layout_asserts(f64);
   o << "struct Geo {\n" << geo_common() << stage;
   o << "  template <bool CORE>\n"
        "  static __device__ __forceinline__ void node(const T* __restrict__ L, const int b, const int I0, const int I1,\n"
@@ -1031,6 +1099,40 @@ std::string Gen::source(bool f64) {
   return o.str();
 }
 
+std::string Gen::source_class(const std::string& name) {
+  param_ = true;
+  std::ostringstream o;
+  const std::string stage = stage_fn();
+  std::vector<int> out_core, out_bound;
+  int r_core = -1, r_bound = -1;
+  build_node(true, out_core, r_core);
+  build_node(false, out_bound, r_bound);
+  if (!ok_) return "";
+  o << "struct " << name << " {\n";
+  auto c = [&](const std::string& nm, long long v) { o << "  static constexpr int " << nm << " = " << v << ";\n"; };
+  c("NF", d_->nfields);
+  c("NARR", d_->nfields);
+  for (int a = 0; a < 3; ++a) c("PER" + istr(a), g_.periodic[a]);
+  c("TX", g_.tile[0]); c("TY", g_.tile[1]); c("TZ", g_.tile[2]);
+  c("HX", g_.halo[0]); c("HY", g_.halo[1]); c("HZ", g_.halo[2]);
+  c("EX", g_.ext[0]); c("EY", g_.ext[1]); c("EZ", g_.ext[2]);
+  c("VOL", g_.vol);
+  c("NTHR", 256);
+  o << stage;
+  o << "  template <bool CORE>\n"
+       "  static __device__ __forceinline__ void node(const BInst& hd, const T* __restrict__ L, const int b, const int I0,\n"
+       "      const int I1, const int I2, const double t, const T* __restrict__ P, const JPtrs& jp, T* out) {\n"
+       "    (void)I0; (void)I1; (void)I2; (void)t; (void)P; (void)jp; (void)L; (void)b; (void)hd;\n"
+       "    if constexpr (CORE) {\n";
+  print_region(o, r_core);
+  for (int f = 0; f < d_->nfields; ++f) o << "    out[" << f << "] = " << ref(out_core[f]) << ";\n";
+  o << "    } else {\n";
+  print_region(o, r_bound);
+  for (int f = 0; f < d_->nfields; ++f) o << "    out[" << f << "] = " << ref(out_bound[f]) << ";\n";
+  o << "    }\n  }\n};\n";
+  return o.str();
+}
+
 std::string Gen::source_vjp(bool f64) {
   std::ostringstream o;
   o << "// generated vector-Jacobian product of the right-hand side of one PDE system\n";
@@ -1043,6 +1145,7 @@ std::string Gen::source_vjp(bool f64) {
   if (g_.ncore > 0) build_node(true, out_core, r_core);
   build_node(false, out_bound, r_bound);
   if (!ok_) return "";
+  o << layout_asserts(f64);
   o << "struct Geo {\n" << geo_common() << stage << scatter;
   o << "  template <bool CORE>\n"
        "  static __device__ __forceinline__ void vjp(const T* __restrict__ L, T* Ab, double* Aa, const int b,\n"
@@ -1091,6 +1194,13 @@ std::map<std::string, CacheEntry> g_jit_cache;   // key: generated source
 std::string jit_source(const mol_desc* d, const JitGeom& g, std::string& err) {
   Gen gen(d, g);
   std::string s = gen.source(d->dtype == MOL_F64);
+  err = gen.error;
+  return s;
+}
+
+std::string jit_class_source(const mol_desc* d, const JitGeom& g, const std::string& name, std::string& err) {
+  Gen gen(d, g);
+  std::string s = gen.source_class(name);
   err = gen.error;
   return s;
 }

@@ -365,3 +365,90 @@ __device__ __forceinline__ void mol_vjp_tile(const VArgs& va, const JPtrs& jp) {
     }
   }
 }
+
+// ---------------------------------------------------------------------------------------
+// Batched instances.  Each instance of a batch has a header with its grid sizes, unknown
+// boxes (offsets into the batch state vector), tile counts and tile classes, and pointers to
+// its weight pool W (the descriptor's pool), constants K (the constant pool, then eps and
+// 1/dx of every slot) and coordinates.  Instances of one structure share a class body
+// (struct with the node function); a persistent kernel walks the work list of (instance, tile)
+// items and dispatches on the class.  Layout identical to mol::BInst.
+struct BInst {
+  int n[3];
+  int lo[8][3];
+  int hi[8][3];
+  i64 fofs[8];
+  int ulo[3], uhi[3], nt[3];
+  int slo[3], shi[3], clo[3], chi[3];
+  int cls, first;
+  const double* coord[3];
+  const double* W;
+  const double* K;
+};
+
+__device__ __forceinline__ i64 mol_b_uidx(const BInst& hd, const int f, const int p0, const int p1, const int p2) {
+  return hd.fofs[f] + (i64)(p0 - hd.lo[f][0]) +
+         (i64)(hd.hi[f][0] - hd.lo[f][0] + 1) *
+             ((i64)(p1 - hd.lo[f][1]) + (i64)(hd.hi[f][1] - hd.lo[f][1] + 1) * (i64)(p2 - hd.lo[f][2]));
+}
+
+__device__ __forceinline__ bool mol_b_inbox(const BInst& hd, const int f, const int I0, const int I1, const int I2) {
+  return I0 >= hd.lo[f][0] && I0 <= hd.hi[f][0] && I1 >= hd.lo[f][1] && I1 <= hd.hi[f][1] && I2 >= hd.lo[f][2] &&
+         I2 <= hd.hi[f][2];
+}
+
+// One tile of one instance (L: the kernel's LDS tile, sized for the largest class).
+template <class G>
+__device__ __forceinline__ void mol_batch_tile(T* L, const BInst& hd, const int tile, const T* __restrict__ u,
+                                               const Epi& ep, const double t) {
+  const int tx = tile % hd.nt[0];
+  const int r1 = tile / hd.nt[0];
+  const int ty = r1 % hd.nt[1];
+  const int tz = r1 / hd.nt[1];
+  const int o0 = hd.ulo[0] + tx * G::TX, o1 = hd.ulo[1] + ty * G::TY, o2 = hd.ulo[2] + tz * G::TZ;
+  const bool simple = tx >= hd.slo[0] && tx <= hd.shi[0] && ty >= hd.slo[1] && ty <= hd.shi[1] &&
+                      tz >= hd.slo[2] && tz <= hd.shi[2];
+  const bool core = tx >= hd.clo[0] && tx <= hd.chi[0] && ty >= hd.clo[1] && ty <= hd.chi[1] &&
+                    tz >= hd.clo[2] && tz <= hd.chi[2];
+  JPtrs jp;
+  jp.coord[0] = hd.coord[0];
+  jp.coord[1] = hd.coord[1];
+  jp.coord[2] = hd.coord[2];
+  const T* P = nullptr;
+  constexpr int EXY = G::EX * G::EY;
+  for (int q = threadIdx.x; q < G::VOL; q += G::NTHR) {
+    const int lz = q / EXY;
+    const int r = q - lz * EXY;
+    const int ly = r / G::EX;
+    const int lx = r - ly * G::EX;
+    int p0 = o0 + lx - G::HX, p1 = o1 + ly - G::HY, p2 = o2 + lz - G::HZ;
+    if (simple) {
+      if (G::PER0) p0 = mol_wrap(mol_wrap(p0, hd.n[0]), hd.n[0]);
+      if (G::PER1) p1 = mol_wrap(mol_wrap(p1, hd.n[1]), hd.n[1]);
+      if (G::PER2) p2 = mol_wrap(mol_wrap(p2, hd.n[2]), hd.n[2]);
+#pragma unroll
+      for (int f = 0; f < G::NF; ++f) L[f * G::VOL + q] = u[mol_b_uidx(hd, f, p0, p1, p2)];
+    } else {
+#pragma unroll
+      for (int f = 0; f < G::NF; ++f) L[f * G::VOL + q] = G::stage(hd, f, p0, p1, p2, t, P, u, jp);
+    }
+  }
+  __syncthreads();
+  constexpr int TXY = G::TX * G::TY, TVOL = TXY * G::TZ;
+  for (int q = threadIdx.x; q < TVOL; q += G::NTHR) {
+    const int lz = q / TXY;
+    const int rr = q - lz * TXY;
+    const int ly = rr / G::TX;
+    const int lx = rr - ly * G::TX;
+    const int I0 = o0 + lx, I1 = o1 + ly, I2 = o2 + lz;
+    if (I0 > hd.uhi[0] || I1 > hd.uhi[1] || I2 > hd.uhi[2]) continue;
+    const int base = (lx + G::HX) + G::EX * (ly + G::HY) + EXY * (lz + G::HZ);
+    T out[G::NF];
+    if (core) G::template node<true>(hd, L, base, I0, I1, I2, t, P, jp, out);
+    else G::template node<false>(hd, L, base, I0, I1, I2, t, P, jp, out);
+#pragma unroll
+    for (int f = 0; f < G::NF; ++f)
+      if (mol_b_inbox(hd, f, I0, I1, I2)) mol_epilogue(ep, mol_b_uidx(hd, f, I0, I1, I2), out[f]);
+  }
+  __syncthreads();
+}

@@ -169,6 +169,17 @@ template <typename T>
 int launch_rhs(const Launch<T>& L, const T* u, const Epi<T>& ep, hipStream_t stream) {
   if (L.kcw) return launch_weno2d<T>(*L.kcw, *L.plan, L.nweno, u, ep, stream);
   const size_t lds = (size_t)(L.ka->nfields + L.ka->naux) * L.ka->vol * sizeof(T);
+  if (L.batch) {
+    const T* uu = u;
+    Epi<T> e = ep;
+    const BInst* ins = L.insts;
+    const int32_t* wl = L.wl;
+    int nw = L.nwork;
+    void* args[] = {(void*)&uu, (void*)&e, (void*)&ins, (void*)&wl, (void*)&nw};
+    if (hipModuleLaunchKernel(L.batch, L.bgrid, 1, 1, 256, 1, 1, 0, stream, args, nullptr) != hipSuccess)
+      return MOL_ERR_HIP;
+    return 0;
+  }
   if (L.jit) {
     // generated kernel: every tile, boundary rows and ghost closures included
     const T* uu = u;

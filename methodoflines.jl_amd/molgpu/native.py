@@ -87,6 +87,11 @@ _SIGNATURES = {
     "mol_last_error": (C.c_char_p, [C.c_void_p]),
     "mol_destroy": (None, [C.c_void_p]),
     "mol_device_info": (C.c_int, [C.c_char_p, C.c_int]),
+    "mol_create_batched": (C.c_int, [C.POINTER(C.POINTER(mol_desc)), C.POINTER(C.POINTER(C.c_double)), C.c_int32,
+                                     C.POINTER(C.c_int32), C.POINTER(C.c_void_p)]),
+    "mol_batch_offsets": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_int32]),
+    "mol_codegen_batched": (C.c_int64, [C.POINTER(C.POINTER(mol_desc)), C.POINTER(C.POINTER(C.c_double)), C.c_int32,
+                                        C.POINTER(C.c_int32), C.c_char_p, C.c_int64, C.c_int32]),
     "mol_codegen": (C.c_int64, [C.POINTER(mol_desc), C.POINTER(C.c_int32), C.c_char_p, C.c_int64, C.c_int32]),
 }
 

@@ -1,0 +1,83 @@
+"""GPU batches: many uneven diffusion-reaction instances in one persistent launch per stage,
+against the oracle per instance and against each instance run on its own."""
+import numpy as np
+import pytest
+
+import oracle as orc
+import molgpu
+from molgpu import systems
+from molgpu.batch import BatchProblem
+from conftest import assert_parity
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _instances(count, seed, dtype="f64", nmin=10, nmax=41):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.integers(nmin, nmax))
+        D = float(rng.uniform(0.01, 0.1))
+        k = float(rng.uniform(0.5, 5.0))
+        bct = tuple(int(b) for b in rng.integers(0, 2, 4))
+        sys_, disc = systems.diffreact2d(n, D, k, bct, dtype=dtype)
+        out.append(((n, D, k, bct), molgpu.discretize(sys_, disc)))
+    return out
+
+
+def test_batched_rhs_matches_oracle_per_instance():
+    torch = _torch()
+    inst = _instances(14, 1)
+    batch = BatchProblem([p for _, p in inst], tile=(16, 8, 1))
+    plan = batch.plan()
+    assert plan["classes"] == len({meta[3] for meta, _ in inst})
+    rng = np.random.default_rng(2)
+    u = rng.uniform(0, 1, batch.n_unknowns)
+    ug = torch.tensor(u, device="cuda")
+    du = torch.empty_like(ug)
+    print("  batch rhs: %d instances, %d unknowns, plan %s" % (len(inst), batch.n_unknowns, plan), flush=True)
+    batch.f(du, ug, 0.0)
+    got = batch.split(du.cpu().numpy())
+    for ((n, D, k, bct), _), ui, gi in zip(inst, batch.split(u), got):
+        ref = orc.diffreact2d_rhs(n, D, k, bct, ui)
+        dx = 1.0 / (n - 1)
+        assert_parity(gi, ref, np.max(np.abs(ui)) * D * 8 / dx ** 2 + k + 1, what="batch n=%d %s" % (n, bct))
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batched_rk4_equals_instances_run_alone(graph):
+    torch = _torch()
+    inst = _instances(9, 3)
+    batch = BatchProblem([p for _, p in inst])
+    dt, N = 1e-4, 20
+    u = torch.tensor(batch.u0, device="cuda")
+    print("  batch rk4 graph=%s" % graph, flush=True)
+    batch.integrate(u, 0.0, dt, N, molgpu.RK4, graph=graph)
+    got = batch.split(u.cpu().numpy())
+    for (_, p), gi in zip(inst, got):
+        ui = torch.tensor(p.u0, device="cuda")
+        p.integrate(ui, 0.0, dt, N, molgpu.RK4)
+        assert_parity(gi, ui.cpu().numpy(), rtol=1e-13, what="batch vs single")
+
+
+def test_batched_fp32_kahan_error_vs_fp64():
+    """FP32 state with the compensated update, against the FP64 batch (error reported)."""
+    torch = _torch()
+    inst64 = _instances(6, 4)
+    inst32 = _instances(6, 4, dtype="f32")
+    b64 = BatchProblem([p for _, p in inst64])
+    b32 = BatchProblem([p for _, p in inst32])
+    dt, N = 2e-4, 200
+    u64 = torch.tensor(b64.u0, device="cuda")
+    u32 = torch.tensor(b32.u0, dtype=torch.float32, device="cuda")
+    print("  batch fp32 kahan", flush=True)
+    b64.integrate(u64, 0.0, dt, N, molgpu.RK4)
+    b32.integrate(u32, 0.0, dt, N, molgpu.RK4, kahan=True)
+    err = float((u32.double() - u64).abs().max())
+    print("  fp32+kahan max error vs fp64: %.3e" % err)
+    assert err < 1e-5
