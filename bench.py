@@ -1,12 +1,19 @@
-"""Benchmark: stencil RHS Gpoint/s of the MI355X path on the 2D Brusselator (512^2 unknowns per
-variable, FP64, periodic, 2nd-order Laplacian), driven by the device-resident RK4 stage loop
-at dt = 1e-5.
+"""Benchmark of the MI355X path of the MethodOfLines.jl right-hand side and stage loop.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--graph 0|1] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--graph 0|1] [--no-cpu] [--quick]
 
-One step = one classic RK4 step = 4 right-hand-side evaluations over all 512^2 points (a point
-is one unknown grid node with both variables).  value = points * 4 * K / elapsed / 1e9.
-Prints one JSON line.
+Headline (`value`): stencil RHS Gpoint/s of the 2D Brusselator (512^2 unknowns per variable,
+FP64, periodic, 2nd-order Laplacian) driven by the device-resident RK4 stage loop at
+dt = 1e-5.  One step = one classic RK4 step = 4 right-hand sides over all 512^2 points (a
+point is one unknown grid node with both variables); value = points * 4 * K / elapsed / 1e9.
+
+The same JSON line carries the other parts of the metric:
+  roofline     the RK4 stage kernel of the timed region against HBM peak;
+  weno5        one FP64 WENO5 Burgers RHS at 1024^2 (and 4096^2, beyond the 256 MB MALL);
+  adjoint      3D nonlinear diffusion 128^3 (2.0 M unknowns), 200 RK4 steps, checkpoints every
+               20 steps: forward and adjoint times, their ratio, and the gradient error;
+  batch        512 diffusion-reaction instances with uneven grids, FP32 + Kahan, 200 RK4 steps;
+  cpu_baseline the C++ CPU restatement (the oracle) on the host, 1 thread, bounded sample.
 """
 import argparse
 import json
@@ -20,14 +27,192 @@ sys.path.insert(0, os.path.join(ROOT, "methodoflines.jl_amd"))
 HBM_PEAK_GBS = 8000.0
 
 
+def events():
+    import torch
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def rhs_time_us(prob, u, reps):
+    """Average duration of back-to-back right-hand-side launches on the current stream."""
+    import ctypes as C
+    import torch
+    h = prob.handle
+    du = torch.empty_like(u)
+    pp = prob.device_params()
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    args = (h.h, C.c_void_p(du.data_ptr()), C.c_void_p(u.data_ptr()),
+            C.c_void_p(pp.data_ptr()) if pp is not None else None, C.c_double(0.0), s)
+    for _ in range(5):
+        h.lib.mol_rhs(*args)
+    torch.cuda.synchronize()
+    e0, e1 = events()
+    e0.record()
+    for _ in range(reps):
+        h.lib.mol_rhs(*args)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) * 1e3 / reps
+
+
+def bench_weno(n, reps):
+    import torch
+    import molgpu
+    from molgpu import systems
+    sys_, disc = systems.burgers2d(n=n + 1)
+    prob = molgpu.discretize(sys_, disc)
+    u = torch.tensor(prob.u0, dtype=torch.float64, device="cuda")
+    us = rhs_time_us(prob, u, reps)
+    pts = n * n
+    ach = pts * 16 / (us * 1e-6) / 1e9
+    return {"grid": "%dx%d" % (n, n), "rhs_us": us, "gpoint_s": pts / (us * 1e-6) / 1e9,
+            "bytes_per_launch": pts * 16, "achieved_gbs": ach, "hbm_frac": ach / HBM_PEAK_GBS}
+
+
+def bench_adjoint(n, nsteps, every, fd_cells):
+    """Config 5: forward (checkpointed) and adjoint times; gradient error against the oracle's
+    forward mode on a 16^3 copy and against central differences of the GPU solve on cells."""
+    import numpy as np
+    import torch
+    import molgpu
+    from molgpu import systems
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    def make(m):
+        s, d = systems.nonlindiff3d(m)
+        p = molgpu.discretize(s, d)
+        d0 = 0.5 + 0.5 * np.abs(systems.gaussian_field3d(m, 1))
+        src = systems.gaussian_field3d(m, 2)
+        p.set_grid_field("d0", d0)
+        p.set_grid_field("s", src)
+        (l0, l1, l2), (h0, h1, h2) = p.lowered.lo[0], p.lowered.hi[0]
+        box = lambda f: f[l0 - 1:h0, l1 - 1:h1, l2 - 1:h2].reshape(-1, order="F")
+        u0 = 0.3 * box(systems.gaussian_field3d(m, 5))
+        tgt = 0.2 * box(systems.gaussian_field3d(m, 6))
+        # explicit-diffusion step: dt = 0.5 * 2.78 * dx^2 / (12 * max D) with D <= 2 max d0
+        dx = 1.0 / (m - 1)
+        dt = 0.5 * 2.78 * dx * dx / (12.0 * 2.0 * d0.max())
+        return p, d0.reshape(-1, order="F"), src.reshape(-1, order="F"), u0, tgt, dt
+
+    p, d0, src, u0, tgt, dt = make(n)
+    ng = n ** 3
+    u0d = torch.tensor(u0, device="cuda")
+    tgd = torch.tensor(tgt, device="cuda")
+    g = torch.zeros(2 * ng, dtype=torch.float64, device="cuda")
+
+    def run():
+        u = u0d.clone()
+        e0, e1 = events()
+        e2 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        p.forward_checkpointed(u, 0.0, dt, nsteps, every)
+        e1.record()
+        lam = (u - tgd).contiguous()
+        g.zero_()
+        p.adjoint(lam, g)
+        e2.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1), e1.elapsed_time(e2), u
+
+    run()
+    f_ms, a_ms, uT = run()
+    gd0 = g[:ng].cpu().numpy()
+    # finite differences of the GPU solve on seeded cells (Richardson-extrapolated central)
+    d0t = torch.tensor(d0, device="cuda")
+    p.set_grid_field("d0", d0t)
+
+    def loss(e, c):
+        d0t.copy_(torch.tensor(d0, device="cuda"))
+        d0t[c] += e
+        u = u0d.clone()
+        p.integrate(u, 0.0, dt, nsteps, molgpu.RK4, graph=True)
+        return 0.5 * float(((u - tgd) ** 2).sum())
+
+    rng = np.random.default_rng(7)
+    cells = rng.choice(ng, size=fd_cells, replace=False)
+    fd_err = 0.0
+    for c in cells:
+        h = 1e-3 * d0[c]
+        D1 = (loss(h, c) - loss(-h, c)) / (2 * h)
+        D2 = (loss(h / 2, c) - loss(-h / 2, c)) / h
+        fd = (4 * D2 - D1) / 3
+        fd_err = max(fd_err, abs(gd0[c] - fd) / max(abs(fd), 1e-300))
+    d0t.copy_(torch.tensor(d0, device="cuda"))
+    # oracle forward mode on a 16^3 copy (same construction), direction seeded
+    p16, d016, src16, u016, tgt16, dt16 = make(16)
+    u = torch.tensor(u016, device="cuda")
+    p16.forward_checkpointed(u, 0.0, dt16, nsteps, every)
+    lam = (u - torch.tensor(tgt16, device="cuda")).contiguous()
+    g16 = torch.zeros(2 * 16 ** 3, dtype=torch.float64, device="cuda")
+    p16.adjoint(lam, g16)
+    v = np.random.default_rng(3).standard_normal(16 ** 3)
+    _, dL = orc.nonlindiff3d_jvp(16, d016, v, src16, u016, tgt16, dt16, nsteps)
+    fm_err = abs(float(g16[:16 ** 3].cpu().numpy() @ v) - dL) / abs(dL)
+    rhs_us = rhs_time_us(p, u0d, 20)
+    return {"grid": "%d^3" % n, "unknowns": p.lowered.n_unknowns, "steps": nsteps, "ckpt_every": every,
+            "dt": dt, "forward_ms": f_ms, "adjoint_ms": a_ms, "ratio": a_ms / f_ms, "rhs_us": rhs_us,
+            "grad_rel_err_forward_mode_16": fm_err, "grad_rel_err_fd_cells": fd_err, "fd_cells": int(fd_cells)}
+
+
+def bench_batch(count, nsteps, quick):
+    """Config 4: `count` diffusion-reaction instances, seeded sizes 32..256 per side, FP32 +
+    Kahan, RK4 in one persistent launch per stage (graph-captured)."""
+    import numpy as np
+    import torch
+    import molgpu
+    from molgpu import systems
+    from molgpu.batch import BatchProblem
+    from molgpu.discretization import lower
+    from molgpu.autotune import tune_batch
+
+    rng = np.random.default_rng(2024)
+    lows = []
+    t0 = time.perf_counter()
+    dts = []
+    for _ in range(count):
+        n = int(rng.integers(32, 257))
+        D = float(rng.uniform(0.01, 0.1))
+        k = float(rng.uniform(0.5, 5.0))
+        bct = tuple(int(b) for b in rng.integers(0, 2, 4))
+        u0 = rng.uniform(0, 1, (n, n))
+        for _ in range(3):     # Jacobi smoothing: 5-point average, boundary rows kept
+            u0[1:-1, 1:-1] = 0.2 * (u0[1:-1, 1:-1] + u0[:-2, 1:-1] + u0[2:, 1:-1] + u0[1:-1, :-2] + u0[1:-1, 2:])
+        sys_, disc = systems.diffreact2d(n, D, k, bct, dtype="f32")
+        low = lower(sys_, disc)
+        (l0, l1), (h0, h1) = low.lo[0], low.hi[0]
+        low.u0 = u0[l0 - 1:h0, l1 - 1:h1].reshape(-1, order="F").astype(np.float32)
+        lows.append(low)
+        dx = 1.0 / (n - 1)
+        dts.append(0.5 * 2.78 * dx * dx / (8 * D))
+    lower_s = time.perf_counter() - t0
+    dt = min(dts)                      # one global step, stable for every instance
+    tile, table = tune_batch(lows, quick=quick)
+    b = BatchProblem(lows, tile=tile)
+    u = torch.tensor(b.u0, dtype=torch.float32, device="cuda")
+    b.integrate(u, 0.0, dt, 4, molgpu.RK4, kahan=True, graph=True)
+    torch.cuda.synchronize()
+    u = torch.tensor(b.u0, dtype=torch.float32, device="cuda")
+    e0, e1 = events()
+    e0.record()
+    b.integrate(u, 0.0, dt, nsteps, molgpu.RK4, kahan=True, graph=True)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    pts = b.n_unknowns
+    return {"instances": count, "unknowns": pts, "classes": b.plan()["classes"], "tile": list(tile),
+            "tuning": table, "steps": nsteps, "dt": dt, "ms": ms, "ms_per_step": ms / nsteps,
+            "gpoint_rhs_s": pts * 4 * nsteps / (ms / 1e3) / 1e9, "lowering_s": lower_s,
+            "finite": bool(torch.isfinite(u).all())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--graph", type=int, default=1)
-    ap.add_argument("--rhs-reps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="headline only (no weno/adjoint/batch sections)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -46,12 +231,11 @@ def main():
     stream = torch.cuda.current_stream()
     u = torch.tensor(prob.u0, dtype=torch.float64, device="cuda")
     u_start = u.clone()
-    # warmup
     prob.integrate(u, 0.0, dt, max(args.warmup, 1), molgpu.RK4, graph=bool(args.graph))
     torch.cuda.synchronize()
     u.copy_(u_start)
     torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0, ev1 = events()
     t0 = time.perf_counter()
     ev0.record(stream)
     prob.integrate(u, 0.0, dt, args.steps, molgpu.RK4, graph=bool(args.graph))
@@ -62,38 +246,15 @@ def main():
     ms_per_step = ms / args.steps
     value = points * 4 * args.steps / (ms / 1e3) / 1e9
 
-    # dominant kernel: the right-hand-side kernel; average launch duration over back-to-back launches
-    du = torch.empty_like(u)
-    uu = u_start.clone()
-    for _ in range(5):
-        prob.f(du, uu, None, 0.0)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(args.rhs_reps):
-        prob.f(du, uu, None, 1.2)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    rhs_us = e0.elapsed_time(e1) * 1e3 / args.rhs_reps
-    rhs_bytes = points * 2 * 16          # read u, v; write du, dv (FP64)
-    achieved = rhs_bytes / (rhs_us * 1e-6) / 1e9
-    rhs_gpts = points / (rhs_us * 1e-6) / 1e9
-
-    cpu = None
-    if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as orc
-        threads = 1
-        ucpu = prob.u0.copy()
-        reps = 0
-        c0 = time.perf_counter()
-        while time.perf_counter() - c0 < 10.0:      # bounded sample: ~10 s of CPU work
-            orc.bruss2d_rhs(n, ucpu, 1.2, threads=threads)
-            reps += 1
-        csec = time.perf_counter() - c0
-        cpu = {"value": points * reps / csec / 1e9, "unit": "Gpoint-RHS/s", "cores": threads, "kind": "port",
-               "sample": "%d right-hand sides of the same 512^2 Brusselator in the C++ oracle (CPU restatement, "
-                         "not MethodOfLines.jl), FP64, 1 thread" % reps}
+    # dominant kernel of the timed region: the generated RHS kernel in RK4 stage mode, four
+    # launches per step (plus a one-thread clock kernel).  Algorithmic bytes per step and point
+    # (2 variables, FP64): stage 1 reads u, writes y1 and acc (48 B); stages 2-3 read y, u, acc,
+    # write y', acc (80 B each); stage 4 reads y3, u, acc and writes u (64 B): 272 B per step.
+    launch_us = ms * 1e3 / (4 * args.steps)
+    stage_bytes = points * 272 // 4
+    achieved = stage_bytes / (launch_us * 1e-6) / 1e9
+    rhs_us = rhs_time_us(prob, u_start, 200)
+    rhs_ach = points * 32 / (rhs_us * 1e-6) / 1e9
 
     out = {
         "metric": "stencil RHS Gpoint/s (2D Brusselator 512^2, FP64, RK4 dt=1e-5)",
@@ -113,11 +274,33 @@ def main():
                    "graph": bool(args.graph)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "rhs_generic_kernel<double> (mol_rhs)", "avg_us": rhs_us,
-                     "bytes_per_launch": rhs_bytes, "rhs_gpoint_s": rhs_gpts},
-        "cpu_baseline": cpu,
+                     "kernel": "mol_rhs_tiles (generated; RK4 stage epilogue)", "avg_us": launch_us,
+                     "bytes_per_launch": stage_bytes,
+                     "rhs_only": {"avg_us": rhs_us, "bytes_per_launch": points * 32, "achieved": rhs_ach,
+                                  "frac": rhs_ach / HBM_PEAK_GBS}},
         "wall_s": wall,
     }
+    if not args.quick:
+        out["weno5"] = {"1024": bench_weno(1024, 200), "4096": bench_weno(4096, 20)}
+        out["adjoint"] = bench_adjoint(128, 200, 20, 8)
+        out["batch"] = bench_batch(512, 200, quick=False)
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        threads = 1
+        ucpu = prob.u0.copy()
+        reps = 0
+        c0 = time.perf_counter()
+        while time.perf_counter() - c0 < 10.0:      # bounded sample: ~10 s of CPU work
+            orc.bruss2d_rhs(n, ucpu, 1.2, threads=threads)
+            reps += 1
+        csec = time.perf_counter() - c0
+        cpu = {"value": points * reps / csec / 1e9, "unit": "Gpoint-RHS/s", "cores": threads, "kind": "port",
+               "sample": "%d right-hand sides of the same 512^2 Brusselator in the C++ oracle (CPU restatement, "
+                         "not MethodOfLines.jl), FP64, 1 thread" % reps}
+    out["cpu_baseline"] = cpu
     print(json.dumps(out))
 
 

@@ -673,9 +673,9 @@ bool prog_uses(const mol_desc* d, int prog, uint32_t op) {
 
 // Generate and load the vector-Jacobian product kernel (first use).  Its tiles are sized for
 // the state, the auxiliary arrays and their adjoint buffers in LDS.
-int ensure_vjp(mol_handle* h) {
-  if (h->vjp_fn) return 0;
-  const mol_desc* d = &h->desc.d;
+// Tiling of the reverse-mode kernel: LDS holds the state, the grid fields and their adjoint
+// buffers (FP64 for the grid fields).
+JitGeom vjp_geometry(mol_handle* h, const mol_desc* d, int* ntiles) {
   KArgs<double> kv;
   fill_kargs<double>(h, d, kv, (2 * (size_t)d->nfields + d->naux) * h->esize() + (size_t)d->naux * sizeof(double));
   KCore<double> kc;
@@ -686,14 +686,23 @@ int ensure_vjp(mol_handle* h) {
   build_core<double>(h, d, kv, kc, gen, ncore);
   h->core_reason = reason;
   h->core_interp_ok = interp_ok;
+  if (ntiles) *ntiles = kv.ntile[0] * kv.ntile[1] * kv.ntile[2];
+  return geom_of(kv, kc, ncore);
+}
+
+int ensure_vjp(mol_handle* h) {
+  if (h->vjp_fn) return 0;
+  const mol_desc* d = &h->desc.d;
+  int ntiles = 0;
+  const JitGeom g = vjp_geometry(h, d, &ntiles);
   std::string err;
-  const std::string src = jit_vjp_source(d, geom_of(kv, kc, ncore), err);
+  const std::string src = jit_vjp_source(d, g, err);
   if (src.empty()) return fail(h, MOL_ERR_UNSUPPORTED, "vector-Jacobian product: " + err);
   if (jit_kernel(src, "mol_vjp_tiles", h->device, &h->vjp_fn, err)) {
     h->vjp_fn = nullptr;
     return fail(h, MOL_ERR_HIP, "vector-Jacobian product: " + err);
   }
-  h->vjp_tiles = kv.ntile[0] * kv.ntile[1] * kv.ntile[2];
+  h->vjp_tiles = ntiles;
   h->vjp_aux_ok = true;
   for (int c = 0; c < d->nclosures; ++c)
     if (prog_uses(d, d->closures_h[c].gprog, MOL_OP_AUX)) h->vjp_aux_ok = false;
@@ -1313,7 +1322,7 @@ int64_t mol_codegen(const mol_desc* d, const int32_t* tile, char* buf, int64_t b
   int ncore = 0;
   build_core<double>(&tmp, d, tmp.ka64, tmp.kc64, gen, ncore);
   std::string err;
-  const std::string src = (flags & MOL_CODEGEN_VJP) ? jit_vjp_source(d, geom_of(tmp.ka64, tmp.kc64, ncore), err)
+  const std::string src = (flags & MOL_CODEGEN_VJP) ? jit_vjp_source(d, vjp_geometry(&tmp, d, nullptr), err)
                                                     : jit_source(d, geom_of(tmp.ka64, tmp.kc64, ncore), err);
   if (src.empty()) return fail(nullptr, MOL_ERR_UNSUPPORTED, err);
   if (flags & MOL_CODEGEN_COMPILE) {

@@ -75,7 +75,7 @@ def burgers2d(n: int = 1025, eps: float = 1e-6, seed: int = 0, dtype: str = "f64
     return sys_, MOLFiniteDifference({x: n, y: n}, t, advection_scheme=WENOScheme(eps), dtype=dtype)
 
 
-def diffreact2d(n: int, D: float, k: float, bctype=(0, 0, 0, 0), u0=None, dtype: str = "f64"):
+def diffreact2d(n: int, D: float, k: float, bctype=(0, 0, 0, 0), dtype: str = "f64"):
     """u_t = D Lap u + k u (1 - u) on [0,1]^2; bctype per side (x-lo, x-hi, y-lo, y-hi):
     0 Dirichlet u = 0, 1 Neumann du/dn = 0."""
     t, x, y = parameters("t x y")

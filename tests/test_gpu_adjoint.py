@@ -38,6 +38,12 @@ def _nonlin(n, seed=1):
     return prob, d0.reshape(-1, order="F"), src.reshape(-1, order="F")
 
 
+def _on_box(prob, field):
+    """A full-grid field restricted to the unknown box (the state layout)."""
+    (l0, l1, l2), (h0, h1, h2) = prob.lowered.lo[0], prob.lowered.hi[0]
+    return field[l0 - 1:h0, l1 - 1:h1, l2 - 1:h2].reshape(-1, order="F")
+
+
 def test_vjp_is_exact_transpose_for_linear_heat_with_robin_boundaries():
     """f(u) = A u + b (Robin, 4th order): VJP(lam) = A^T lam, A from oracle RHS columns."""
     torch = _torch()
@@ -157,8 +163,8 @@ def test_adjoint_gradient_matches_forward_mode_oracle_and_finite_differences():
     prob, d0, src = _nonlin(n, seed=3)
     N = prob.lowered.n_unknowns
     rng = np.random.default_rng(11)
-    u0 = 0.3 * systems.gaussian_field3d(n, 5).reshape(-1, order="F")
-    target = 0.2 * systems.gaussian_field3d(n, 6).reshape(-1, order="F")
+    u0 = 0.3 * _on_box(prob, systems.gaussian_field3d(n, 5))
+    target = 0.2 * _on_box(prob, systems.gaussian_field3d(n, 6))
     dx = 1.0 / (n - 1)
     dt = 0.1 * dx * dx / (2.0 * np.max(d0))
     # forward with checkpoints, then the adjoint
@@ -203,9 +209,9 @@ def test_autograd_function_returns_the_adjoint_gradients():
     n, nsteps, every = 12, 10, 4
     prob, d0, src = _nonlin(n, seed=4)
     dt = 0.1 * (1.0 / (n - 1)) ** 2 / (2.0 * np.max(d0))
-    u0 = _dev(0.3 * systems.gaussian_field3d(n, 7).reshape(-1, order="F")).requires_grad_(True)
+    u0 = _dev(0.3 * _on_box(prob, systems.gaussian_field3d(n, 7))).requires_grad_(True)
     d0_t = _dev(d0).requires_grad_(True)
-    target = _dev(0.1 * systems.gaussian_field3d(n, 8).reshape(-1, order="F"))
+    target = _dev(0.1 * _on_box(prob, systems.gaussian_field3d(n, 8)))
     print("  autograd solve", flush=True)
     uT = rk4_solve(prob, u0, 0.0, dt, nsteps, every, {"d0": d0_t})
     loss = 0.5 * ((uT - target) ** 2).sum()
