@@ -116,7 +116,8 @@ def bench_adjoint(n, nsteps, every, fd_cells):
 
     run()
     f_ms, a_ms, uT = run()
-    gd0 = g[:ng].cpu().numpy()
+    ia = p.lowered.aux.index("d0")
+    gd0 = g[ia * ng:(ia + 1) * ng].cpu().numpy()
     # finite differences of the GPU solve on seeded cells (Richardson-extrapolated central)
     d0t = torch.tensor(d0, device="cuda")
     p.set_grid_field("d0", d0t)
@@ -147,7 +148,8 @@ def bench_adjoint(n, nsteps, every, fd_cells):
     p16.adjoint(lam, g16)
     v = np.random.default_rng(3).standard_normal(16 ** 3)
     _, dL = orc.nonlindiff3d_jvp(16, d016, v, src16, u016, tgt16, dt16, nsteps)
-    fm_err = abs(float(g16[:16 ** 3].cpu().numpy() @ v) - dL) / abs(dL)
+    i16 = p16.lowered.aux.index("d0")
+    fm_err = abs(float(g16[i16 * 16 ** 3:(i16 + 1) * 16 ** 3].cpu().numpy() @ v) - dL) / abs(dL)
     rhs_us = rhs_time_us(p, u0d, 20)
     return {"grid": "%d^3" % n, "unknowns": p.lowered.n_unknowns, "steps": nsteps, "ckpt_every": every,
             "dt": dt, "forward_ms": f_ms, "adjoint_ms": a_ms, "ratio": a_ms / f_ms, "rhs_us": rhs_us,

@@ -501,6 +501,8 @@ bool jit_enabled() {
   return !(e && e[0] == '0');
 }
 
+int sync_kargs(mol_handle* h);
+
 int setup_paths(mol_handle* h, const mol_desc* d) {
   std::vector<int32_t> t64, t32;
   int n64 = 0, n32 = 0;
@@ -509,7 +511,28 @@ int setup_paths(mol_handle* h, const mol_desc* d) {
   h->use_weno = false;
   h->jit_fn = nullptr;
   h->jit_reason.clear();
+  bool retiled = false;
+  const void* aux_bound[MOL_MAX_AUX];
+  for (int a = 0; a < MOL_MAX_AUX; ++a) aux_bound[a] = h->ka64.aux[a];
+  auto refill = [&](size_t pos) {
+    fill_kargs<double>(h, d, h->ka64, pos);
+    fill_kargs<float>(h, d, h->ka32, pos);
+    for (int a = 0; a < MOL_MAX_AUX; ++a) {
+      h->ka64.aux[a] = (const double*)aux_bound[a];
+      h->ka32.aux[a] = (const float*)aux_bound[a];
+    }
+    t64.clear();
+    t32.clear();
+    build_core<double>(h, d, h->ka64, h->kc64, t64, n64);
+    build_core<float>(h, d, h->ka32, h->kc32, t32, n32);
+  };
   if (jit_enabled()) {
+    // tiles sized for the arrays the generated kernel stages (the interpreter stages them all)
+    const size_t pos = jit_lds_per_pos(d, geom_of(h->ka64, h->kc64, n64), h->esize(), false);
+    if (pos > 0 && pos < (size_t)(d->nfields + d->naux) * h->esize()) {
+      refill(pos);
+      retiled = true;
+    }
     std::string err;
     const std::string src = jit_source(d, geom_of(h->ka64, h->kc64, n64), err);
     if (src.empty()) h->jit_reason = err;
@@ -517,9 +540,11 @@ int setup_paths(mol_handle* h, const mol_desc* d) {
       h->jit_fn = nullptr;
       h->jit_reason = err;
     }
+    if (!h->jit_fn && retiled) refill(0);   // back to the interpreter's tiles
   } else {
     h->jit_reason = "code generation disabled (MOL_JIT=0)";
   }
+  if (int rc = sync_kargs(h)) return rc;
   if (n64 > 0 && !h->jit_fn && !h->core_interp_ok) {
     // neither core kernel can run this system: every tile goes to the generic kernel
     const KArgs<double>& ka = h->ka64;
@@ -677,7 +702,20 @@ bool prog_uses(const mol_desc* d, int prog, uint32_t op) {
 // buffers (FP64 for the grid fields).
 JitGeom vjp_geometry(mol_handle* h, const mol_desc* d, int* ntiles) {
   KArgs<double> kv;
+  // staged arrays of the reverse-mode kernel, estimated first with every array staged
   fill_kargs<double>(h, d, kv, (2 * (size_t)d->nfields + d->naux) * h->esize() + (size_t)d->naux * sizeof(double));
+  {
+    KCore<double> kc0;
+    std::vector<int32_t> gen0;
+    int nc0 = 0;
+    const std::string reason = h->core_reason;
+    const bool interp_ok = h->core_interp_ok;
+    build_core<double>(h, d, kv, kc0, gen0, nc0);
+    h->core_reason = reason;
+    h->core_interp_ok = interp_ok;
+    const size_t pos = jit_lds_per_pos(d, geom_of(kv, kc0, nc0), h->esize(), true);
+    if (pos > 0) fill_kargs<double>(h, d, kv, pos);
+  }
   KCore<double> kc;
   std::vector<int32_t> gen;
   int ncore = 0;
@@ -1321,6 +1359,15 @@ int64_t mol_codegen(const mol_desc* d, const int32_t* tile, char* buf, int64_t b
   std::vector<int32_t> gen;
   int ncore = 0;
   build_core<double>(&tmp, d, tmp.ka64, tmp.kc64, gen, ncore);
+  {
+    // the tiles mol_create uses with the generated kernel (sized for its staged arrays)
+    const size_t pos = jit_lds_per_pos(d, geom_of(tmp.ka64, tmp.kc64, ncore), tmp.esize(), false);
+    if (pos > 0 && pos < (size_t)(d->nfields + d->naux) * tmp.esize()) {
+      fill_kargs<double>(&tmp, d, tmp.ka64, pos);
+      gen.clear();
+      build_core<double>(&tmp, d, tmp.ka64, tmp.kc64, gen, ncore);
+    }
+  }
   std::string err;
   const std::string src = (flags & MOL_CODEGEN_VJP) ? jit_vjp_source(d, vjp_geometry(&tmp, d, nullptr), err)
                                                     : jit_source(d, geom_of(tmp.ka64, tmp.kc64, ncore), err);

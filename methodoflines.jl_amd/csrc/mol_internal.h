@@ -36,6 +36,9 @@ struct JitPtrs {
 // Code generation and in-process compilation of the tile kernel (mol_jit.cpp).
 std::string jit_source(const mol_desc* d, const JitGeom& g, std::string& err);
 std::string jit_vjp_source(const mol_desc* d, const JitGeom& g, std::string& err);
+// LDS bytes per tile + halo position of the generated forward (vjp = false) or reverse-mode
+// kernel: the arrays it stages (and their adjoints); 0 when the system cannot be generated.
+size_t jit_lds_per_pos(const mol_desc* d, const JitGeom& g, size_t esize, bool vjp);
 std::string jit_class_source(const mol_desc* d, const JitGeom& g, const std::string& name, std::string& err);
 
 // Header of one instance of a batch (layout identical to BInst of mol_jit_prelude.h).

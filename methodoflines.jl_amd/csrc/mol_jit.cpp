@@ -11,6 +11,7 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -59,7 +60,8 @@ struct Node {
   long long sidx = -1;
   std::vector<long long> wi, wj;   // X_LINCOMB: pool index of each weight (wj >= 0: product wi*wj)
   int arr = 0;               // N_LDS: array (variable, or nfields + aux index)
-  long long off = 0;         //        displacement in LDS elements
+  int dv[3] = {0, 0, 0};     //        displacement from the node in nodes per axis
+  bool core = false;         //        built for the interior-rows variant
   std::string text;          // N_EXPR
   int imm = 0;               // MOL_OP_POWI exponent
   // N_BRANCH: region reg[0] when lo <= I_ax <= hi, region reg[k] when I_ax == cval[k], else 0
@@ -164,12 +166,20 @@ class Gen {
     n.val = v;
     return add(n);
   }
-  int lds(int arr, long long off) {
+  int lds(int arr, int dx, int dy, int dz) {
     Node n;
     n.kind = N_LDS;
     n.arr = arr;
-    n.off = off;
+    n.dv[0] = dx;
+    n.dv[1] = dy;
+    n.dv[2] = dz;
+    n.core = core_;
     return add(n);
+  }
+  int lds_ax(int arr, int ax, int rel) {
+    int d[3] = {0, 0, 0};
+    d[ax] = rel;
+    return lds(arr, d[0], d[1], d[2]);
   }
   int klit_node(int idx) {
     Node n;
@@ -216,7 +226,46 @@ class Gen {
     return (int)regs_.size() - 1;
   }
 
-  int stride(int ax) const { return ax == 0 ? 1 : (ax == 1 ? g_.ext[0] : g_.ext[0] * g_.ext[1]); }
+  // Print geometry: LDS extent and the slot of each staged array (-1: read from global memory).
+  struct PrintGeo {
+    int ext[3] = {1, 1, 1};
+    int vol = 1;
+    std::vector<int> slot;     // per array: LDS slot among the staged arrays
+    std::vector<int> fslot;    // per variable: adjoint slot (reverse mode)
+    std::vector<int> aslot;    // per grid field: adjoint slot (reverse mode)
+  };
+  PrintGeo pg_;
+ public:
+  // Arrays read away from the node (these are staged in LDS) and their reach per axis, over
+  // all variants and over the interior-rows variant only.
+  struct Usage {
+    std::vector<bool> staged;
+    int reach[3] = {0, 0, 0};
+    int core_reach[3] = {0, 0, 0};
+  };
+  // Build the node IR (both variants) and report the array usage; empty on failure.
+  Usage analyze() {
+    std::vector<int> o1, o2;
+    int r1, r2;
+    if (g_.ncore > 0) build_node(true, o1, r1);
+    build_node(false, o2, r2);
+    return ok_ ? usage() : Usage{};
+  }
+  Usage usage() const {
+    Usage u;
+    u.staged.assign(d_->nfields + d_->naux, false);
+    for (const Node& n : nodes_) {
+      if (n.kind != N_LDS) continue;
+      for (int a = 0; a < 3; ++a) {
+        const int r = n.dv[a] < 0 ? -n.dv[a] : n.dv[a];
+        if (r) u.staged[n.arr] = true;
+        u.reach[a] = std::max(u.reach[a], r);
+        if (n.core) u.core_reach[a] = std::max(u.core_reach[a], r);
+      }
+    }
+    return u;
+  }
+ private:
   std::string coord_node(int d) const { return "(T)jp.coord[" + istr(d) + "][I" + istr(d) + " - 1]"; }
   std::string coord_tap(int d, int off) const {
     if (off == 0) return coord_node(d);
@@ -230,7 +279,7 @@ class Gen {
   int apply(const RowR& r, int arr, int ax) {
     if (!r.valid) return lit_node(0.0);
     std::vector<int> args;
-    for (size_t k = 0; k < r.w.size(); ++k) args.push_back(lds(arr, (long long)r.rel[k] * stride(ax)));
+    for (size_t k = 0; k < r.w.size(); ++k) args.push_back(lds_ax(arr, ax, r.rel[k]));
     return lincomb(r.w, args, r.widx);
   }
   void slot_range(const mol_slot& s, int ax, int& lo, int& hi) const;
@@ -251,6 +300,10 @@ class Gen {
   void partials(const Node& n, int id, std::vector<std::pair<int, std::string>>& out) const;
   std::string stage_fn();
   std::string scatter_fn();
+  PrintGeo make_pg(const int* halo, const Usage& us) const;
+  std::string geo_variant(const std::string& name, const int* halo, const Usage& us, bool core, int region,
+                          const std::vector<int>& outs, bool vjp);
+  std::string source_kernel(bool f64, bool vjp);
   // sizes of the argument structs as the library lays them out, checked in the generated code
   static std::string layout_asserts(bool f64) {
     const size_t epi = f64 ? sizeof(Epi<double>) : sizeof(Epi<float>);
@@ -384,8 +437,8 @@ int Gen::leaf(uint32_t code, int imm, const Ctx& c) {
     }
   }
   switch (code) {
-    case MOL_OP_FIELD: return lds(imm, 0);
-    case MOL_OP_AUX: return lds(d_->nfields + imm, 0);
+    case MOL_OP_FIELD: return lds(imm, 0, 0, 0);
+    case MOL_OP_AUX: return lds(d_->nfields + imm, 0, 0, 0);
     case MOL_OP_COORD: return expr(coord_node(imm));
     case MOL_OP_SLOT: return c.kind == CTX_NODE ? slot(imm) : lit_node(0.0);
     case MOL_OP_DINNER: return lit_node(0.0);
@@ -505,11 +558,10 @@ int Gen::slot_body(int k, const Var* v) {
       return op(MOL_OP_MUL, {coef, op(X_SELECT, {cond, back, fwd})});
     }
     case MOL_SLOT_WENO: {
-      const long long sd = stride(ax);
       Node n;
       n.kind = N_OP;
       n.op = X_WENO;
-      for (int q = -2; q <= 2; ++q) n.a.push_back(lds(s.field, q * sd));
+      for (int q = -2; q <= 2; ++q) n.a.push_back(lds_ax(s.field, ax, q));
       n.w = {s.eps, s.inv_dx};
       n.imm = k;
       return add(n);
@@ -539,7 +591,10 @@ int Gen::slot_body(int k, const Var* v) {
           w.push_back(rx.w[i] * ry.w[j]);
           wi.push_back(rx.widx[i]);
           wj.push_back(ry.widx[j]);
-          args.push_back(lds(s.field, (long long)rx.rel[i] * stride(ax) + (long long)ry.rel[j] * stride(s.axis2)));
+          int dd[3] = {0, 0, 0};
+          dd[ax] += rx.rel[i];
+          dd[s.axis2] += ry.rel[j];
+          args.push_back(lds(s.field, dd[0], dd[1], dd[2]));
         }
       return lincomb(w, args, wi, wj);
     }
@@ -625,7 +680,13 @@ std::string Gen::ref(int id) const {
   switch (n.kind) {
     case N_LIT: return n.src == 2 ? pk(n.val, n.sidx) : lit(n.val);
     case N_LDS: {
-      const long long o = (long long)n.arr * g_.vol + n.off;
+      const int sl = pg_.slot.empty() ? n.arr : pg_.slot[n.arr];
+      if (sl < 0) {
+        // read at the node only: straight from global memory
+        if (n.arr < d_->nfields) return "u[" + UIDX(n.arr, "I0", "I1", "I2") + "]";
+        return "jp.aux[" + istr(n.arr - d_->nfields) + "][gidx(I0, I1, I2)]";
+      }
+      const long long o = (long long)sl * pg_.vol + n.dv[0] + (long long)pg_.ext[0] * (n.dv[1] + (long long)pg_.ext[1] * n.dv[2]);
       if (o == 0) return "L[b]";
       return o > 0 ? "L[b + " + istr(o) + "]" : "L[b - " + istr(-o) + "]";
     }
@@ -811,14 +872,14 @@ void Gen::partials(const Node& n, int id, std::vector<std::pair<int, std::string
 // (Ab for variables, Aa in FP64 for auxiliary arrays) with LDS atomics.
 void Gen::print_reverse(std::ostream& o, int r, const std::vector<std::pair<int, std::string>>& seeds) const {
   const Region& R = regs_[r];
-  std::map<std::pair<int, long long>, std::string> acc;
+  std::map<std::array<int, 4>, std::string> acc;   // (array, dx, dy, dz) -> adjoint terms
   for (int id : R.nodes) o << "    T g" << id << " = T(0);\n";
   auto add_adj = [&](int target, const std::string& e) {
     const Node& t = nodes_[target];
     if (t.kind == N_OP || t.kind == N_BRANCH) {
       o << "    g" << target << " += " << e << ";\n";
     } else if (t.kind == N_LDS) {
-      std::string& s = acc[{t.arr, t.off}];
+      std::string& s = acc[{t.arr, t.dv[0], t.dv[1], t.dv[2]}];
       s = s.empty() ? e : s + " + " + e;
     }
   };
@@ -853,14 +914,24 @@ void Gen::print_reverse(std::ostream& o, int r, const std::vector<std::pair<int,
     o << "\n";
   }
   for (const auto& kv : acc) {
-    const int arr = kv.first.first;
-    const long long off = kv.first.second;
+    const int arr = kv.first[0];
+    const bool field = arr < d_->nfields;
+    const int sl = field ? pg_.fslot[arr] : pg_.aslot[arr - d_->nfields];
+    if (sl < 0) {
+      // array read at the node only: its adjoint goes straight to global memory
+      if (field)
+        o << "    mol_atomic_add(&outp[" << UIDX(arr, "I0", "I1", "I2") << "], " << kv.second << ");\n";
+      else
+        o << "    if (gaux) mol_atomic_add_f64(&gaux[" << (arr - d_->nfields) << " * NGRID + gidx(I0, I1, I2)], (double)("
+          << kv.second << "));\n";
+      continue;
+    }
+    const long long off = kv.first[1] + (long long)pg_.ext[0] * (kv.first[2] + (long long)pg_.ext[1] * kv.first[3]);
     const std::string pos = off == 0 ? "b" : (off > 0 ? "b + " + istr(off) : "b - " + istr(-off));
-    if (arr < d_->nfields)
-      o << "    atomicAdd(&Ab[" << (long long)arr * g_.vol << " + " << pos << "], " << kv.second << ");\n";
+    if (field)
+      o << "    atomicAdd(&Ab[" << (long long)sl * pg_.vol << " + " << pos << "], " << kv.second << ");\n";
     else
-      o << "    atomicAdd(&Aa[" << (long long)(arr - d_->nfields) * g_.vol << " + " << pos << "], (double)("
-        << kv.second << "));\n";
+      o << "    atomicAdd(&Aa[" << (long long)sl * pg_.vol << " + " << pos << "], (double)(" << kv.second << "));\n";
   }
 }
 
@@ -986,13 +1057,13 @@ std::string Gen::scatter_fn() {
   return o.str();
 }
 
-// Geometry constants and index helpers shared by the forward and reverse kernels.
+// Geometry-independent part of the generated structs: sizes, boxes, index helpers and the
+// slow staging / scatter functions.
 std::string Gen::geo_common() {
   std::ostringstream o;
   auto c = [&](const std::string& name, long long v) { o << "  static constexpr int " << name << " = " << v << ";\n"; };
   c("NF", d_->nfields);
   c("NAUX", d_->naux);
-  c("NARR", d_->nfields + d_->naux);
   for (int a = 0; a < 3; ++a) {
     const std::string s = istr(a);
     c("N" + s, g_.n[a]);
@@ -1002,39 +1073,16 @@ std::string Gen::geo_common() {
     c("NT" + s, g_.ntile[a]);
   }
   c("TX", g_.tile[0]); c("TY", g_.tile[1]); c("TZ", g_.tile[2]);
-  c("HX", g_.halo[0]); c("HY", g_.halo[1]); c("HZ", g_.halo[2]);
-  c("EX", g_.ext[0]); c("EY", g_.ext[1]); c("EZ", g_.ext[2]);
-  c("VOL", g_.vol);
   c("NBLK", g_.ntile[0] * g_.ntile[1] * g_.ntile[2]);
   c("NTHR", 256);
   o << "  static constexpr i64 NGRID = " << (long long)g_.n[0] * g_.n[1] * g_.n[2] << ";\n";
-  // tiles staged with plain loads: tile + halo inside every variable's box (after the wrap)
-  for (int a = 0; a < 3; ++a) {
-    int slo = 0, shi = g_.ntile[a] - 1;
-    if (!g_.periodic[a]) {
-      int lo = -(1 << 30), hi = 1 << 30;
-      for (int f = 0; f < d_->nfields; ++f) {
-        lo = std::max(lo, g_.lo[f][a]);
-        hi = std::min(hi, g_.hi[f][a]);
-      }
-      slo = g_.ntile[a];
-      shi = -1;
-      for (int t = 0; t < g_.ntile[a]; ++t) {
-        const int o0 = g_.ulo[a] + t * g_.tile[a];
-        if (o0 - g_.halo[a] >= lo && o0 + g_.tile[a] - 1 + g_.halo[a] <= hi) {
-          slo = std::min(slo, t);
-          shi = std::max(shi, t);
-        }
-      }
-    }
-    c("SLO" + istr(a), slo);
-    c("SHI" + istr(a), shi);
-  }
   // core tiles: interior operator rows for every node
   for (int a = 0; a < 3; ++a) {
     c("CLO" + istr(a), g_.ncore > 0 ? g_.ct_lo[a] : 1);
     c("CHI" + istr(a), g_.ncore > 0 ? g_.ct_lo[a] + g_.ct_n[a] - 1 : 0);
   }
+  o << "  static __device__ __forceinline__ bool is_core(const int tx, const int ty, const int tz) {\n"
+       "    return tx >= CLO0 && tx <= CHI0 && ty >= CLO1 && ty <= CHI1 && tz >= CLO2 && tz <= CHI2;\n  }\n";
   o << "  static __device__ __forceinline__ i64 uidx(const int f, const int p0, const int p1, const int p2) {\n"
        "    switch (f) {\n";
   for (int f = 0; f < d_->nfields; ++f) {
@@ -1063,42 +1111,197 @@ std::string Gen::geo_common() {
   return o.str();
 }
 
-std::string Gen::source(bool f64) {
+// Print geometry of a tile variant with halo `halo`: staged arrays get consecutive LDS slots.
+Gen::PrintGeo Gen::make_pg(const int* halo, const Usage& us) const {
+  PrintGeo pg;
+  pg.vol = 1;
+  for (int a = 0; a < 3; ++a) {
+    pg.ext[a] = g_.tile[a] + 2 * halo[a];
+    pg.vol *= pg.ext[a];
+  }
+  const int narr = d_->nfields + d_->naux;
+  pg.slot.assign(narr, -1);
+  pg.fslot.assign(d_->nfields, -1);
+  pg.aslot.assign(d_->naux, -1);
+  int ns = 0, nf = 0, na = 0;
+  for (int k = 0; k < narr; ++k) {
+    if (!us.staged[k]) continue;
+    pg.slot[k] = ns++;
+    if (k < d_->nfields) pg.fslot[k] = nf++;
+    else pg.aslot[k - d_->nfields] = na++;
+  }
+  return pg;
+}
+
+// One tile variant: geometry constants, staging of one position, node function (forward or
+// reverse mode) and, for reverse mode, the flush of one position's adjoints.
+std::string Gen::geo_variant(const std::string& name, const int* halo, const Usage& us, bool core, int region,
+                             const std::vector<int>& outs, bool vjp) {
+  pg_ = make_pg(halo, us);
   std::ostringstream o;
-  o << "// generated right-hand side of one PDE system\n";
+  int nst = 0, nstf = 0, nsta = 0;
+  for (int k = 0; k < d_->nfields + d_->naux; ++k)
+    if (pg_.slot[k] >= 0) (k < d_->nfields ? nstf : nsta)++, ++nst;
+  o << "struct " << name << " : GeoBase {\n";
+  auto c = [&](const std::string& nm, long long v) { o << "  static constexpr int " << nm << " = " << v << ";\n"; };
+  c("CORE", core ? 1 : 0);
+  c("HX", halo[0]); c("HY", halo[1]); c("HZ", halo[2]);
+  c("EX", pg_.ext[0]); c("EY", pg_.ext[1]); c("EZ", pg_.ext[2]);
+  c("VOL", pg_.vol);
+  c("NST", nst); c("NSTF", nstf); c("NSTA", nsta);
+  // tiles staged with plain loads: tile + halo inside every variable's box (after the wrap)
+  for (int a = 0; a < 3; ++a) {
+    int slo = 0, shi = g_.ntile[a] - 1;
+    if (!g_.periodic[a]) {
+      int lo = -(1 << 30), hi = 1 << 30;
+      for (int f = 0; f < d_->nfields; ++f) {
+        lo = std::max(lo, g_.lo[f][a]);
+        hi = std::min(hi, g_.hi[f][a]);
+      }
+      slo = g_.ntile[a];
+      shi = -1;
+      for (int t = 0; t < g_.ntile[a]; ++t) {
+        const int o0 = g_.ulo[a] + t * g_.tile[a];
+        if (o0 - halo[a] >= lo && o0 + g_.tile[a] - 1 + halo[a] <= hi) {
+          slo = std::min(slo, t);
+          shi = std::max(shi, t);
+        }
+      }
+    }
+    c("SLO" + istr(a), slo);
+    c("SHI" + istr(a), shi);
+  }
+  // staging of one tile + halo position (p: unwrapped full-grid index)
+  o << "  static __device__ __forceinline__ void stage_pos(T* L, const int q, int p0, int p1, int p2, const bool simple,\n"
+       "      const double t, const T* __restrict__ P, const T* __restrict__ u, const JPtrs& jp) {\n"
+       "    (void)t; (void)P; (void)jp; (void)u; (void)simple;\n";
+  bool any_field = false;
+  for (int f = 0; f < d_->nfields; ++f) any_field = any_field || pg_.slot[f] >= 0;
+  if (any_field) {
+    o << "    if (simple) {\n      int w0 = p0, w1 = p1, w2 = p2;\n";
+    for (int a = 0; a < 3; ++a)
+      if (g_.periodic[a])
+        o << "      w" << a << " = mol_wrap(mol_wrap(w" << a << ", " << g_.n[a] << "), " << g_.n[a] << ");\n";
+    for (int f = 0; f < d_->nfields; ++f)
+      if (pg_.slot[f] >= 0) o << "      L[" << (long long)pg_.slot[f] * pg_.vol << " + q] = u[uidx(" << f << ", w0, w1, w2)];\n";
+    o << "    } else {\n";
+    for (int f = 0; f < d_->nfields; ++f)
+      if (pg_.slot[f] >= 0)
+        o << "      L[" << (long long)pg_.slot[f] * pg_.vol << " + q] = stage(" << f << ", p0, p1, p2, t, P, u, jp);\n";
+    o << "    }\n";
+  }
+  for (int a = 0; a < d_->naux; ++a)
+    if (pg_.slot[d_->nfields + a] >= 0)
+      o << "    L[" << (long long)pg_.slot[d_->nfields + a] * pg_.vol << " + q] = stage_aux(" << a << ", p0, p1, p2, jp);\n";
+  o << "  }\n";
+  if (!vjp) {
+    o << "  static __device__ __forceinline__ void node(const T* __restrict__ L, const int b, const int I0, const int I1,\n"
+         "      const int I2, const double t, const T* __restrict__ P, const T* __restrict__ u, const JPtrs& jp, T* out) {\n"
+         "    (void)I0; (void)I1; (void)I2; (void)t; (void)P; (void)jp; (void)L; (void)b; (void)u;\n";
+    print_region(o, region);
+    for (int f = 0; f < d_->nfields; ++f) o << "    out[" << f << "] = " << ref(outs[f]) << ";\n";
+    o << "  }\n";
+  } else {
+    o << "  static __device__ __forceinline__ void zero_pos(T* Ab, double* Aa, const int q) {\n"
+         "    (void)Ab; (void)Aa;\n";
+    for (int k = 0; k < nstf; ++k) o << "    Ab[" << (long long)k * pg_.vol << " + q] = T(0);\n";
+    for (int k = 0; k < nsta; ++k) o << "    Aa[" << (long long)k * pg_.vol << " + q] = 0.0;\n";
+    o << "  }\n";
+    o << "  static __device__ __forceinline__ void vjp(const T* __restrict__ L, T* Ab, double* Aa, const int b,\n"
+         "      const int I0, const int I1, const int I2, const double t, const T* __restrict__ P,\n"
+         "      const T* __restrict__ u, const JPtrs& jp, const T* lam, T* __restrict__ outp, double* __restrict__ gaux) {\n"
+         "    (void)I0; (void)I1; (void)I2; (void)t; (void)P; (void)jp; (void)L; (void)b; (void)Ab; (void)Aa; (void)u;\n"
+         "    (void)outp; (void)gaux;\n";
+    print_region(o, region);
+    std::vector<std::pair<int, std::string>> seeds;
+    for (int f = 0; f < d_->nfields; ++f) seeds.push_back({outs[f], "lam[" + istr(f) + "]"});
+    print_reverse(o, region, seeds);
+    o << "  }\n";
+    // adjoints of one tile + halo position: through the transposed staging to the unknowns,
+    // and to the grid-field gradients
+    o << "  static __device__ __forceinline__ void flush_pos(const T* Ab, const double* Aa, const int q, const int p0,\n"
+         "      const int p1, const int p2, T* __restrict__ outp, double* __restrict__ gaux) {\n"
+         "    (void)Ab; (void)Aa; (void)gaux;\n";
+    for (int f = 0; f < d_->nfields; ++f)
+      if (pg_.fslot[f] >= 0)
+        o << "    { const T v = Ab[" << (long long)pg_.fslot[f] * pg_.vol << " + q]; if (v != T(0)) scatter(" << f
+          << ", p0, p1, p2, v, outp); }\n";
+    if (nsta > 0) {
+      o << "    if (gaux) {\n      const i64 gi = gidx(p0, p1, p2);\n      if (gi >= 0) {\n";
+      for (int a = 0; a < d_->naux; ++a)
+        if (pg_.aslot[a] >= 0)
+          o << "        { const double v = Aa[" << (long long)pg_.aslot[a] * pg_.vol << " + q]; if (v != 0.0) "
+            << "mol_atomic_add_f64(&gaux[" << a << " * NGRID + gi], v); }\n";
+      o << "      }\n    }\n";
+    }
+    o << "  }\n";
+  }
+  o << "};\n\n";
+  return o.str();
+}
+
+// Forward (vjp = false) or reverse-mode kernel of one system.
+std::string Gen::source_kernel(bool f64, bool vjp) {
+  std::ostringstream o;
+  o << "// generated " << (vjp ? "vector-Jacobian product of the " : "") << "right-hand side of one PDE system\n";
   o << "#define MOL_T " << (f64 ? "double" : "float") << "\n";
   o << "#include \"mol_jit_prelude.h\"\n\n";
+  o << layout_asserts(f64);
   const std::string stage = stage_fn();
+  const std::string scatter = vjp ? scatter_fn() : "";
   std::vector<int> out_core, out_bound;
   int r_core = -1, r_bound = -1;
   if (g_.ncore > 0) build_node(true, out_core, r_core);
   build_node(false, out_bound, r_bound);
   if (!ok_) return "";
-  o << layout_asserts(f64);
-  o << "struct Geo {\n" << geo_common() << stage;
-  o << "  template <bool CORE>\n"
-       "  static __device__ __forceinline__ void node(const T* __restrict__ L, const int b, const int I0, const int I1,\n"
-       "      const int I2, const double t, const T* __restrict__ P, const JPtrs& jp, T* out) {\n"
-       "    (void)I0; (void)I1; (void)I2; (void)t; (void)P; (void)jp; (void)L; (void)b;\n";
-  auto body = [&](int r, const std::vector<int>& outs) {
-    print_region(o, r);
-    for (int f = 0; f < d_->nfields; ++f) o << "    out[" << f << "] = " << ref(outs[f]) << ";\n";
+  const Usage us = usage();
+  o << "struct GeoBase {\n" << geo_common() << stage << scatter << "};\n\n";
+  int lds = 1, ldsf = 1, ldsa = 1;
+  auto sizes = [&](const PrintGeo& pg) {
+    int ns = 0, nf = 0, na = 0;
+    for (size_t k = 0; k < pg.slot.size(); ++k)
+      if (pg.slot[k] >= 0) ((int)k < d_->nfields ? nf : na)++, ++ns;
+    lds = std::max(lds, ns * pg.vol);
+    ldsf = std::max(ldsf, nf * pg.vol);
+    ldsa = std::max(ldsa, na * pg.vol);
   };
   if (g_.ncore > 0) {
-    o << "    if constexpr (CORE) {\n";
-    body(r_core, out_core);
-    o << "    } else {\n";
-    body(r_bound, out_bound);
-    o << "    }\n";
-  } else {
-    body(r_bound, out_bound);
+    o << geo_variant("GeoC", us.core_reach, us, true, r_core, out_core, vjp);
+    sizes(pg_);
   }
-  o << "  }\n};\n\n";
-  o << "extern \"C\" __global__ void __launch_bounds__(256) mol_rhs_tiles(const T* __restrict__ u, Epi ep, JPtrs jp) {\n"
-       "  mol_full_tile<Geo>(u, ep, jp);\n}\n";
+  o << geo_variant("GeoB", us.reach, us, false, r_bound, out_bound, vjp);
+  sizes(pg_);
+  o << "constexpr int LDSN = " << lds << ", LDSF = " << ldsf << ", LDSA = " << ldsa << ";\n\n";
+  const std::string decode =
+      "  int b = mol_xcd_tile<GeoBase::NBLK>((int)blockIdx.x);\n"
+      "  const int tx = b % GeoBase::NT0;\n  b /= GeoBase::NT0;\n"
+      "  const int ty = b % GeoBase::NT1;\n  const int tz = b / GeoBase::NT1;\n";
+  if (!vjp) {
+    o << "extern \"C\" __global__ void __launch_bounds__(256) mol_rhs_tiles(const T* __restrict__ u, Epi ep, JPtrs jp) {\n"
+         "  __shared__ T L[LDSN];\n" << decode
+      << "  const double t = (ep.tnow ? ep.tnow[0] : 0.0) + ep.tofs;\n";
+    if (g_.ncore > 0)
+      o << "  if (GeoBase::is_core(tx, ty, tz)) mol_full_tile<GeoC>(L, tx, ty, tz, u, ep, jp, t);\n  else ";
+    else
+      o << "  ";
+    o << "mol_full_tile<GeoB>(L, tx, ty, tz, u, ep, jp, t);\n}\n";
+  } else {
+    o << "extern \"C\" __global__ void __launch_bounds__(256) mol_vjp_tiles(VArgs va, JPtrs jp) {\n"
+         "  __shared__ T L[LDSN];\n  __shared__ T Ab[LDSF];\n  __shared__ double Aa[LDSA];\n" << decode
+      << "  const double t = (va.tnow ? va.tnow[0] : 0.0) + va.tofs;\n";
+    if (g_.ncore > 0)
+      o << "  if (GeoBase::is_core(tx, ty, tz)) mol_vjp_tile<GeoC>(L, Ab, Aa, tx, ty, tz, va, jp, t);\n  else ";
+    else
+      o << "  ";
+    o << "mol_vjp_tile<GeoB>(L, Ab, Aa, tx, ty, tz, va, jp, t);\n}\n";
+  }
   return o.str();
 }
 
+std::string Gen::source(bool f64) { return source_kernel(f64, false); }
+std::string Gen::source_vjp(bool f64) { return source_kernel(f64, true); }
+
+// Class body of batched instances: one geometry (the full reach as halo), numbers from `hd`.
 std::string Gen::source_class(const std::string& name) {
   param_ = true;
   std::ostringstream o;
@@ -1108,21 +1311,40 @@ std::string Gen::source_class(const std::string& name) {
   build_node(true, out_core, r_core);
   build_node(false, out_bound, r_bound);
   if (!ok_) return "";
+  const Usage us = usage();
+  pg_ = make_pg(us.reach, us);
+  int nst = 0;
+  for (int v : pg_.slot) nst += v >= 0;
   o << "struct " << name << " {\n";
   auto c = [&](const std::string& nm, long long v) { o << "  static constexpr int " << nm << " = " << v << ";\n"; };
   c("NF", d_->nfields);
-  c("NARR", d_->nfields);
+  c("NST", nst);
   for (int a = 0; a < 3; ++a) c("PER" + istr(a), g_.periodic[a]);
   c("TX", g_.tile[0]); c("TY", g_.tile[1]); c("TZ", g_.tile[2]);
-  c("HX", g_.halo[0]); c("HY", g_.halo[1]); c("HZ", g_.halo[2]);
-  c("EX", g_.ext[0]); c("EY", g_.ext[1]); c("EZ", g_.ext[2]);
-  c("VOL", g_.vol);
+  c("HX", us.reach[0]); c("HY", us.reach[1]); c("HZ", us.reach[2]);
+  c("EX", pg_.ext[0]); c("EY", pg_.ext[1]); c("EZ", pg_.ext[2]);
+  c("VOL", pg_.vol);
   c("NTHR", 256);
   o << stage;
+  o << "  static __device__ __forceinline__ void stage_pos(const BInst& hd, T* L, const int q, int p0, int p1, int p2,\n"
+       "      const bool simple, const double t, const T* __restrict__ P, const T* __restrict__ u, const JPtrs& jp) {\n"
+       "    (void)t; (void)P; (void)jp; (void)u; (void)simple; (void)hd;\n"
+       "    if (simple) {\n";
+  for (int a = 0; a < 3; ++a)
+    if (g_.periodic[a]) o << "      p" << a << " = mol_wrap(mol_wrap(p" << a << ", hd.n[" << a << "]), hd.n[" << a << "]);\n";
+  for (int f = 0; f < d_->nfields; ++f)
+    if (pg_.slot[f] >= 0)
+      o << "      L[" << (long long)pg_.slot[f] * pg_.vol << " + q] = u[mol_b_uidx(hd, " << f << ", p0, p1, p2)];\n";
+  o << "    } else {\n";
+  for (int f = 0; f < d_->nfields; ++f)
+    if (pg_.slot[f] >= 0)
+      o << "      L[" << (long long)pg_.slot[f] * pg_.vol << " + q] = stage(hd, " << f << ", p0, p1, p2, t, P, u, jp);\n";
+  o << "    }\n  }\n";
   o << "  template <bool CORE>\n"
        "  static __device__ __forceinline__ void node(const BInst& hd, const T* __restrict__ L, const int b, const int I0,\n"
-       "      const int I1, const int I2, const double t, const T* __restrict__ P, const JPtrs& jp, T* out) {\n"
-       "    (void)I0; (void)I1; (void)I2; (void)t; (void)P; (void)jp; (void)L; (void)b; (void)hd;\n"
+       "      const int I1, const int I2, const double t, const T* __restrict__ P, const T* __restrict__ u,\n"
+       "      const JPtrs& jp, T* out) {\n"
+       "    (void)I0; (void)I1; (void)I2; (void)t; (void)P; (void)jp; (void)L; (void)b; (void)hd; (void)u;\n"
        "    if constexpr (CORE) {\n";
   print_region(o, r_core);
   for (int f = 0; f < d_->nfields; ++f) o << "    out[" << f << "] = " << ref(out_core[f]) << ";\n";
@@ -1130,46 +1352,6 @@ std::string Gen::source_class(const std::string& name) {
   print_region(o, r_bound);
   for (int f = 0; f < d_->nfields; ++f) o << "    out[" << f << "] = " << ref(out_bound[f]) << ";\n";
   o << "    }\n  }\n};\n";
-  return o.str();
-}
-
-std::string Gen::source_vjp(bool f64) {
-  std::ostringstream o;
-  o << "// generated vector-Jacobian product of the right-hand side of one PDE system\n";
-  o << "#define MOL_T " << (f64 ? "double" : "float") << "\n";
-  o << "#include \"mol_jit_prelude.h\"\n\n";
-  const std::string stage = stage_fn();
-  const std::string scatter = scatter_fn();
-  std::vector<int> out_core, out_bound;
-  int r_core = -1, r_bound = -1;
-  if (g_.ncore > 0) build_node(true, out_core, r_core);
-  build_node(false, out_bound, r_bound);
-  if (!ok_) return "";
-  o << layout_asserts(f64);
-  o << "struct Geo {\n" << geo_common() << stage << scatter;
-  o << "  template <bool CORE>\n"
-       "  static __device__ __forceinline__ void vjp(const T* __restrict__ L, T* Ab, double* Aa, const int b,\n"
-       "      const int I0, const int I1, const int I2, const double t, const T* __restrict__ P, const JPtrs& jp,\n"
-       "      const T* lam) {\n"
-       "    (void)I0; (void)I1; (void)I2; (void)t; (void)P; (void)jp; (void)L; (void)b; (void)Ab; (void)Aa;\n";
-  auto body = [&](int r, const std::vector<int>& outs) {
-    print_region(o, r);
-    std::vector<std::pair<int, std::string>> seeds;
-    for (int f = 0; f < d_->nfields; ++f) seeds.push_back({outs[f], "lam[" + istr(f) + "]"});
-    print_reverse(o, r, seeds);
-  };
-  if (g_.ncore > 0) {
-    o << "    if constexpr (CORE) {\n";
-    body(r_core, out_core);
-    o << "    } else {\n";
-    body(r_bound, out_bound);
-    o << "    }\n";
-  } else {
-    body(r_bound, out_bound);
-  }
-  o << "  }\n};\n\n";
-  o << "extern \"C\" __global__ void __launch_bounds__(256) mol_vjp_tiles(VArgs va, JPtrs jp) {\n"
-       "  mol_vjp_tile<Geo>(va, jp);\n}\n";
   return o.str();
 }
 
@@ -1196,6 +1378,17 @@ std::string jit_source(const mol_desc* d, const JitGeom& g, std::string& err) {
   std::string s = gen.source(d->dtype == MOL_F64);
   err = gen.error;
   return s;
+}
+
+size_t jit_lds_per_pos(const mol_desc* d, const JitGeom& g, size_t esize, bool vjp) {
+  Gen gen(d, g);
+  const Gen::Usage us = gen.analyze();
+  if (us.staged.empty()) return 0;
+  size_t nf = 0, na = 0;
+  for (int k = 0; k < d->nfields + d->naux; ++k)
+    if (us.staged[k]) (k < d->nfields ? nf : na)++;
+  const size_t bytes = vjp ? (2 * nf + na) * esize + na * sizeof(double) : (nf + na) * esize;
+  return bytes > 0 ? bytes : esize;
 }
 
 std::string jit_class_source(const mol_desc* d, const JitGeom& g, const std::string& name, std::string& err) {

@@ -192,48 +192,28 @@ __device__ __forceinline__ int mol_xcd_tile(int b) {
   return x * q + (x < rem ? x : rem) + i;
 }
 
-// One tile of the union unknown box.  Tiles whose tile + halo lies inside every variable's
-// unknown box (after the periodic wrap) stage with plain loads; the others stage each position
-// through Geo::stage (ghost-node closures, zero outside the grid and at corners).  Tiles inside
-// the core range run the node function with interior operator rows only (Geo::node<true>);
-// the others select boundary rows per node index (Geo::node<false>).
+// One tile of the union unknown box, in the tile variant G (interior-rows core variant or the
+// boundary variant, each with its own halo).  Tiles whose tile + halo lies inside every
+// variable's unknown box (after the periodic wrap) stage with plain loads; the others stage
+// each position through Geo::stage (ghost-node closures, zero outside the grid and at
+// corners).  Arrays read only at the node are not staged: the node function loads them.
 template <class G>
-__device__ __forceinline__ void mol_full_tile(const T* __restrict__ u, const Epi& ep, const JPtrs& jp) {
-  __shared__ T L[G::NARR * G::VOL];
-  int b = mol_xcd_tile<G::NBLK>((int)blockIdx.x);
-  const int tx = b % G::NT0;
-  b /= G::NT0;
-  const int ty = b % G::NT1;
-  const int tz = b / G::NT1;
+__device__ __forceinline__ void mol_full_tile(T* L, const int tx, const int ty, const int tz, const T* __restrict__ u,
+                                              const Epi& ep, const JPtrs& jp, const double t) {
   const int o0 = G::ULO0 + tx * G::TX, o1 = G::ULO1 + ty * G::TY, o2 = G::ULO2 + tz * G::TZ;
-  const double t = (ep.tnow ? ep.tnow[0] : 0.0) + ep.tofs;
   const bool simple = tx >= G::SLO0 && tx <= G::SHI0 && ty >= G::SLO1 && ty <= G::SHI1 &&
                       tz >= G::SLO2 && tz <= G::SHI2;
-  const bool core = tx >= G::CLO0 && tx <= G::CHI0 && ty >= G::CLO1 && ty <= G::CHI1 &&
-                    tz >= G::CLO2 && tz <= G::CHI2;
   constexpr int EXY = G::EX * G::EY;
-  for (int q = threadIdx.x; q < G::VOL; q += G::NTHR) {
-    const int lz = q / EXY;
-    const int r = q - lz * EXY;
-    const int ly = r / G::EX;
-    const int lx = r - ly * G::EX;
-    int p0 = o0 + lx - G::HX, p1 = o1 + ly - G::HY, p2 = o2 + lz - G::HZ;
-    if (simple) {
-      // a ragged last tile reaches up to tile - 1 + halo nodes past n: the second wrap keeps
-      // every staged position on the ring
-      if (G::PER0) p0 = mol_wrap(mol_wrap(p0, G::N0), G::N0);
-      if (G::PER1) p1 = mol_wrap(mol_wrap(p1, G::N1), G::N1);
-      if (G::PER2) p2 = mol_wrap(mol_wrap(p2, G::N2), G::N2);
-#pragma unroll
-      for (int f = 0; f < G::NF; ++f) L[f * G::VOL + q] = u[G::uidx(f, p0, p1, p2)];
-    } else {
-#pragma unroll
-      for (int f = 0; f < G::NF; ++f) L[f * G::VOL + q] = G::stage(f, p0, p1, p2, t, ep.p, u, jp);
+  if (G::NST > 0) {
+    for (int q = threadIdx.x; q < G::VOL; q += G::NTHR) {
+      const int lz = q / EXY;
+      const int r = q - lz * EXY;
+      const int ly = r / G::EX;
+      const int lx = r - ly * G::EX;
+      G::stage_pos(L, q, o0 + lx - G::HX, o1 + ly - G::HY, o2 + lz - G::HZ, simple, t, ep.p, u, jp);
     }
-#pragma unroll
-    for (int a = 0; a < G::NAUX; ++a) L[(G::NF + a) * G::VOL + q] = G::stage_aux(a, p0, p1, p2, jp);
+    __syncthreads();
   }
-  __syncthreads();
   constexpr int TXY = G::TX * G::TY, TVOL = TXY * G::TZ;
 #pragma unroll
   for (int r = 0; r < (TVOL + G::NTHR - 1) / G::NTHR; ++r) {
@@ -247,8 +227,7 @@ __device__ __forceinline__ void mol_full_tile(const T* __restrict__ u, const Epi
     if (I0 > G::UHI0 || I1 > G::UHI1 || I2 > G::UHI2) continue;
     const int base = (lx + G::HX) + G::EX * (ly + G::HY) + EXY * (lz + G::HZ);
     T out[G::NF];
-    if (core) G::template node<true>(L, base, I0, I1, I2, t, ep.p, jp, out);
-    else G::template node<false>(L, base, I0, I1, I2, t, ep.p, jp, out);
+    G::node(L, base, I0, I1, I2, t, ep.p, u, jp, out);
 #pragma unroll
     for (int f = 0; f < G::NF; ++f)
       if (G::inbox(f, I0, I1, I2)) mol_epilogue(ep, G::uidx(f, I0, I1, I2), out[f]);
@@ -270,49 +249,24 @@ struct VArgs {
   double tofs;
 };
 
-// One tile of the reverse-mode sweep: stage as the forward kernel, run each node's adjoint
-// (tap adjoints summed in LDS with LDS atomics), then add the LDS adjoints of tile + halo to
-// the unknowns (through the transposed ghost closures) and to the auxiliary gradients.
+// One tile of the reverse-mode sweep in tile variant G: stage as the forward kernel, run each
+// node's adjoint (tap adjoints summed in LDS with LDS atomics; arrays read only at the node go
+// straight to global memory), then add the LDS adjoints of tile + halo to the unknowns (through
+// the transposed ghost closures) and to the grid-field gradients.
 template <class G>
-__device__ __forceinline__ void mol_vjp_tile(const VArgs& va, const JPtrs& jp) {
-  __shared__ T L[G::NARR * G::VOL];
-  __shared__ T Ab[G::NF * G::VOL];
-  __shared__ double Aa[(G::NAUX > 0 ? G::NAUX : 1) * G::VOL];
-  int b = mol_xcd_tile<G::NBLK>((int)blockIdx.x);
-  const int tx = b % G::NT0;
-  b /= G::NT0;
-  const int ty = b % G::NT1;
-  const int tz = b / G::NT1;
+__device__ __forceinline__ void mol_vjp_tile(T* L, T* Ab, double* Aa, const int tx, const int ty, const int tz,
+                                             const VArgs& va, const JPtrs& jp, const double t) {
   const int o0 = G::ULO0 + tx * G::TX, o1 = G::ULO1 + ty * G::TY, o2 = G::ULO2 + tz * G::TZ;
-  const double t = (va.tnow ? va.tnow[0] : 0.0) + va.tofs;
   const bool simple = tx >= G::SLO0 && tx <= G::SHI0 && ty >= G::SLO1 && ty <= G::SHI1 &&
                       tz >= G::SLO2 && tz <= G::SHI2;
-  const bool core = tx >= G::CLO0 && tx <= G::CHI0 && ty >= G::CLO1 && ty <= G::CHI1 &&
-                    tz >= G::CLO2 && tz <= G::CHI2;
   constexpr int EXY = G::EX * G::EY;
   for (int q = threadIdx.x; q < G::VOL; q += G::NTHR) {
     const int lz = q / EXY;
     const int r = q - lz * EXY;
     const int ly = r / G::EX;
     const int lx = r - ly * G::EX;
-    int p0 = o0 + lx - G::HX, p1 = o1 + ly - G::HY, p2 = o2 + lz - G::HZ;
-    if (simple) {
-      if (G::PER0) p0 = mol_wrap(mol_wrap(p0, G::N0), G::N0);
-      if (G::PER1) p1 = mol_wrap(mol_wrap(p1, G::N1), G::N1);
-      if (G::PER2) p2 = mol_wrap(mol_wrap(p2, G::N2), G::N2);
-#pragma unroll
-      for (int f = 0; f < G::NF; ++f) L[f * G::VOL + q] = va.u[G::uidx(f, p0, p1, p2)];
-    } else {
-#pragma unroll
-      for (int f = 0; f < G::NF; ++f) L[f * G::VOL + q] = G::stage(f, p0, p1, p2, t, va.p, va.u, jp);
-    }
-#pragma unroll
-    for (int a = 0; a < G::NAUX; ++a) {
-      L[(G::NF + a) * G::VOL + q] = G::stage_aux(a, p0, p1, p2, jp);
-      Aa[a * G::VOL + q] = 0.0;
-    }
-#pragma unroll
-    for (int f = 0; f < G::NF; ++f) Ab[f * G::VOL + q] = T(0);
+    G::stage_pos(L, q, o0 + lx - G::HX, o1 + ly - G::HY, o2 + lz - G::HZ, simple, t, va.p, va.u, jp);
+    G::zero_pos(Ab, Aa, q);
   }
   __syncthreads();
   constexpr int TXY = G::TX * G::TY, TVOL = TXY * G::TZ;
@@ -338,8 +292,7 @@ __device__ __forceinline__ void mol_vjp_tile(const VArgs& va, const JPtrs& jp) {
     }
     if (!any) continue;
     const int base = (lx + G::HX) + G::EX * (ly + G::HY) + EXY * (lz + G::HZ);
-    if (core) G::template vjp<true>(L, Ab, Aa, base, I0, I1, I2, t, va.p, jp, lam);
-    else G::template vjp<false>(L, Ab, Aa, base, I0, I1, I2, t, va.p, jp, lam);
+    G::vjp(L, Ab, Aa, base, I0, I1, I2, t, va.p, va.u, jp, lam, va.out, va.g_aux);
   }
   __syncthreads();
   for (int q = threadIdx.x; q < G::VOL; q += G::NTHR) {
@@ -347,22 +300,7 @@ __device__ __forceinline__ void mol_vjp_tile(const VArgs& va, const JPtrs& jp) {
     const int r = q - lz * EXY;
     const int ly = r / G::EX;
     const int lx = r - ly * G::EX;
-    const int p0 = o0 + lx - G::HX, p1 = o1 + ly - G::HY, p2 = o2 + lz - G::HZ;
-#pragma unroll
-    for (int f = 0; f < G::NF; ++f) {
-      const T v = Ab[f * G::VOL + q];
-      if (v != T(0)) G::scatter(f, p0, p1, p2, v, va.out);
-    }
-    if (G::NAUX > 0 && va.g_aux) {
-      const i64 gi = G::gidx(p0, p1, p2);
-      if (gi >= 0) {
-#pragma unroll
-        for (int a = 0; a < G::NAUX; ++a) {
-          const double v = Aa[a * G::VOL + q];
-          if (v != 0.0) mol_atomic_add_f64(&va.g_aux[a * G::NGRID + gi], v);
-        }
-      }
-    }
+    G::flush_pos(Ab, Aa, q, o0 + lx - G::HX, o1 + ly - G::HY, o2 + lz - G::HZ, va.out, va.g_aux);
   }
 }
 
@@ -416,24 +354,16 @@ __device__ __forceinline__ void mol_batch_tile(T* L, const BInst& hd, const int 
   jp.coord[2] = hd.coord[2];
   const T* P = nullptr;
   constexpr int EXY = G::EX * G::EY;
-  for (int q = threadIdx.x; q < G::VOL; q += G::NTHR) {
-    const int lz = q / EXY;
-    const int r = q - lz * EXY;
-    const int ly = r / G::EX;
-    const int lx = r - ly * G::EX;
-    int p0 = o0 + lx - G::HX, p1 = o1 + ly - G::HY, p2 = o2 + lz - G::HZ;
-    if (simple) {
-      if (G::PER0) p0 = mol_wrap(mol_wrap(p0, hd.n[0]), hd.n[0]);
-      if (G::PER1) p1 = mol_wrap(mol_wrap(p1, hd.n[1]), hd.n[1]);
-      if (G::PER2) p2 = mol_wrap(mol_wrap(p2, hd.n[2]), hd.n[2]);
-#pragma unroll
-      for (int f = 0; f < G::NF; ++f) L[f * G::VOL + q] = u[mol_b_uidx(hd, f, p0, p1, p2)];
-    } else {
-#pragma unroll
-      for (int f = 0; f < G::NF; ++f) L[f * G::VOL + q] = G::stage(hd, f, p0, p1, p2, t, P, u, jp);
+  if (G::NST > 0) {
+    for (int q = threadIdx.x; q < G::VOL; q += G::NTHR) {
+      const int lz = q / EXY;
+      const int r = q - lz * EXY;
+      const int ly = r / G::EX;
+      const int lx = r - ly * G::EX;
+      G::stage_pos(hd, L, q, o0 + lx - G::HX, o1 + ly - G::HY, o2 + lz - G::HZ, simple, t, P, u, jp);
     }
+    __syncthreads();
   }
-  __syncthreads();
   constexpr int TXY = G::TX * G::TY, TVOL = TXY * G::TZ;
   for (int q = threadIdx.x; q < TVOL; q += G::NTHR) {
     const int lz = q / TXY;
@@ -444,8 +374,8 @@ __device__ __forceinline__ void mol_batch_tile(T* L, const BInst& hd, const int 
     if (I0 > hd.uhi[0] || I1 > hd.uhi[1] || I2 > hd.uhi[2]) continue;
     const int base = (lx + G::HX) + G::EX * (ly + G::HY) + EXY * (lz + G::HZ);
     T out[G::NF];
-    if (core) G::template node<true>(hd, L, base, I0, I1, I2, t, P, jp, out);
-    else G::template node<false>(hd, L, base, I0, I1, I2, t, P, jp, out);
+    if (core) G::template node<true>(hd, L, base, I0, I1, I2, t, P, u, jp, out);
+    else G::template node<false>(hd, L, base, I0, I1, I2, t, P, u, jp, out);
 #pragma unroll
     for (int f = 0; f < G::NF; ++f)
       if (mol_b_inbox(hd, f, I0, I1, I2)) mol_epilogue(ep, mol_b_uidx(hd, f, I0, I1, I2), out[f]);

@@ -267,10 +267,14 @@ class _Lowering:
                 self.param_values.append(float(v))
         # grid fields used but not given values
         for eq in sys.eqs:
-            for a in eq.atoms(sp.Function):
+            for a in sorted(eq.atoms(sp.Function), key=str):
                 if isinstance(a, GridField) and a.func not in self.aux_heads:
                     self.aux_heads.append(a.func)
                     self.aux_values.append(None)
+        # a fixed order (by name): the atoms of an expression come as an unordered set
+        order = sorted(range(len(self.aux_heads)), key=lambda i: self.aux_heads[i].__name__)
+        self.aux_heads = [self.aux_heads[i] for i in order]
+        self.aux_values = [self.aux_values[i] for i in order]
 
     # ---------------------------------------------------------------- boundary conditions
     def parse_bcs(self):

@@ -1,0 +1,181 @@
+"""The generated kernels, compiled for the host (tests/genhost), against the oracle.
+
+The source the library generates for gfx950 runs here with its thread loops collapsed (one
+"thread" per work-group covering the tile), so the forward and reverse-mode code paths —
+interior-row and boundary-row variants, ghost closures, direct (unstaged) arrays, both halos —
+are checked on the CPU at FP64 parity, without a GPU.  Small tiles put most nodes on boundary
+tiles.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "genhost"))
+
+import genhost  # noqa: E402
+import oracle as orc  # noqa: E402
+from conftest import assert_parity  # noqa: E402
+from molgpu import systems  # noqa: E402
+from molgpu.discretization import lower  # noqa: E402
+
+TILES = {1: [(16, 1, 1), None], 2: [(8, 8, 1), None], 3: [(8, 4, 4), None]}
+
+
+def _lap_scale(u, coef, dxs):
+    return np.max(np.abs(u)) * abs(coef) * sum(4.0 / dx ** 2 for dx in dxs) + 1.0
+
+
+def _cases():
+    """(name, builder -> (lowered, u, t, aux, oracle rhs, scale))."""
+    def heat(bc, p):
+        def b():
+            bcs = {"dirichlet": ((1, 0, 0), (1, 0, 0)), "neumann": ((0, 1, 0), (0, 1, 0)),
+                   "robin": ((2.0, 0.5, 0.3), (1.5, -0.25, -0.2))}[bc]
+            n = 61
+            low = lower(*systems.heat1d(n=n, D=0.8, bcl=bcs[0], bch=bcs[1], approx_order=p))
+            u = np.random.default_rng(1).uniform(-1, 1, low.n_unknowns)
+            ref = orc.diffusion1d_rhs(n, 0.8, bcs[0], bcs[1], u, p=p)
+            return low, u, 0.0, None, ref, _lap_scale(u, 0.8, [1 / (n - 1)]) * 4
+        return b
+
+    def bruss():
+        n = 33
+        low = lower(*systems.brusselator2d(n=n))
+        u = np.random.default_rng(2).uniform(0.5, 1.5, low.n_unknowns)
+        return low, u, 1.2, None, orc.bruss2d_rhs(n, u, 1.2), _lap_scale(u, 10.0, [1 / (n - 1)] * 2)
+
+    def burgers():
+        n = 33
+        low = lower(*systems.burgers2d(n=n))
+        u = np.random.default_rng(3).uniform(0.2, 1.2, low.n_unknowns)
+        return low, u, 0.0, None, orc.burgers2d_rhs(n, u), np.max(np.abs(u)) ** 2 * 4 * (n - 1) + 1
+
+    def weno_dir():
+        n, L, c = 41, 2.0, 1.0
+        low = lower(*systems.weno1d_dirichlet(n, L, c, 0.1, 0.05))
+        u = np.random.default_rng(4).uniform(0, 1, low.n_unknowns)
+        return low, u, 0.0, None, orc.weno1d_dirichlet_rhs(n, L, c, 1e-6, 0.1, 0.05, u), 4 * (n - 1) / L + 1
+
+    def diffreact(bct):
+        def b():
+            n = 21
+            low = lower(*systems.diffreact2d(n, 0.05, 2.5, bct))
+            u = np.random.default_rng(5).uniform(0, 1, low.n_unknowns)
+            return low, u, 0.0, None, orc.diffreact2d_rhs(n, 0.05, 2.5, bct, u), _lap_scale(u, 0.05, [1 / (n - 1)] * 2)
+        return b
+
+    def nonlin():
+        n = 12
+        low = lower(*systems.nonlindiff3d(n=n))
+        d0 = 0.5 + 0.5 * np.abs(systems.gaussian_field3d(n, 1))
+        src = systems.gaussian_field3d(n, 2)
+        u = np.random.default_rng(6).uniform(-1, 1, low.n_unknowns)
+        ref = orc.nonlindiff3d_rhs(n, d0.reshape(-1, order="F"), src.reshape(-1, order="F"), u)
+        return low, u, 0.0, {"d0": d0, "s": src}, ref, _lap_scale(u, 2.0 * np.max(d0), [1 / (n - 1)] * 3)
+
+    def upwind(order):
+        def b():
+            n = 61
+            low = lower(*systems.upwind1d(n, 0.1, 1.0, 0.3, -0.2, order))
+            u = np.random.default_rng(order).uniform(-1, 1, low.n_unknowns)
+            return low, u, 0.0, None, orc.upwind1d_rhs(n, order, 0.1, 1.0, 0.3, -0.2, u), 4 * 1.1 * (n - 1) * 2
+        return b
+
+    def mixed(p):
+        def b():
+            n = 25
+            low = lower(*systems.mixed2d(n, 0.7, p))
+            u = np.random.default_rng(p).uniform(-1, 1, low.n_unknowns)
+            return low, u, 0.0, None, orc.mixed2d_rhs(n, 0.7, u, p=p), _lap_scale(u, 4.0, [1 / (n - 1)] * 2)
+        return b
+
+    def spherical(p):
+        def b():
+            n = 41
+            low = lower(*systems.spherical1d(n, 0.25, p))
+            u = np.random.default_rng(p + 10).uniform(0, 1, low.n_unknowns)
+            return low, u, 0.0, None, orc.spherical1d_rhs(n, 0.25, u, p=p), _lap_scale(u, 8.0, [1 / (n - 1)]) * 10
+        return b
+
+    def nonlinlap(periodic, p):
+        def b():
+            n = 41
+            low = lower(*systems.nonlinlap1d(n, periodic, 0.2, -0.4, p))
+            u = np.random.default_rng(p).uniform(-1, 1, low.n_unknowns)
+            ref = orc.nonlinlap1d_rhs(n, u, periodic, 0.2, -0.4, p=p)
+            return low, u, 0.0, None, ref, _lap_scale(u, 8.0, [1 / (n - 1)]) * 4
+        return b
+
+    return {
+        "heat_dirichlet_p2": heat("dirichlet", 2), "heat_neumann_p4": heat("neumann", 4),
+        "heat_robin_p4": heat("robin", 4), "brusselator": bruss, "burgers_weno": burgers,
+        "weno_dirichlet": weno_dir, "diffreact_0000": diffreact((0, 0, 0, 0)),
+        "diffreact_0110": diffreact((0, 1, 1, 0)), "nonlindiff3d": nonlin, "upwind_o1": upwind(1),
+        "upwind_o2": upwind(2), "mixed_p2": mixed(2), "mixed_p4": mixed(4), "spherical_p2": spherical(2),
+        "spherical_p4": spherical(4), "nonlinlap_periodic_p4": nonlinlap(True, 4),
+        "nonlinlap_dirichlet_p2": nonlinlap(False, 2),
+    }
+
+
+CASES = _cases()
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_generated_forward_kernel_matches_oracle(name):
+    low, u, t, aux, ref, scale = CASES[name]()
+    for tile in TILES[low.ndim]:
+        got = genhost.rhs(low, u, t=t, aux=aux, tile=tile)
+        assert_parity(got, ref, scale, what="%s tile %s" % (name, tile))
+
+
+@pytest.mark.parametrize("name", ["heat_robin_p4", "brusselator", "burgers_weno", "weno_dirichlet",
+                                  "diffreact_0110", "nonlindiff3d", "upwind_o2", "mixed_p4", "spherical_p2",
+                                  "nonlinlap_periodic_p4"])
+def test_generated_vjp_kernel_dot_product(name):
+    """<(df/du)^T lam, v> of the generated reverse-mode kernel against the Richardson-extrapolated
+    central difference of <lam, f(u + e v)> with the same system's generated forward kernel
+    (itself pinned to the oracle above)."""
+    low, u, t, aux, ref, scale = CASES[name]()
+    rng = np.random.default_rng(17)
+    lam = rng.standard_normal(low.n_unknowns)
+    v = rng.standard_normal(low.n_unknowns)
+    for tile in TILES[low.ndim]:
+        out, g = genhost.vjp(low, u, lam, t=t, aux=aux, tile=tile)
+
+        def val(e):
+            return lam @ genhost.rhs(low, u + e * v, t=t, aux=aux, tile=tile)
+
+        eps = 1e-4 * max(1.0, np.max(np.abs(u)))
+        d1 = (val(eps) - val(-eps)) / (2 * eps)
+        d2 = (val(eps / 2) - val(-eps / 2)) / eps
+        fd = (4 * d2 - d1) / 3
+        assert abs(out @ v - fd) <= 1e-7 * abs(fd) + 1e-9 * np.linalg.norm(out) * np.linalg.norm(v), (name, tile)
+
+
+def test_generated_vjp_grid_field_gradients_match_oracle_differences():
+    """(df/dd0)^T lam and (df/ds)^T lam of the 3D nonlinear diffusion against the oracle."""
+    low, u, t, aux, ref, scale = CASES["nonlindiff3d"]()
+    n = low.n[0]
+    rng = np.random.default_rng(23)
+    lam = rng.standard_normal(low.n_unknowns)
+    w = rng.standard_normal(n ** 3)
+    d0 = aux["d0"].reshape(-1, order="F")
+    src = aux["s"].reshape(-1, order="F")
+    for tile in TILES[3]:
+        out, g = genhost.vjp(low, u, lam, t=t, aux=aux, tile=tile)
+
+        def val(e):
+            return lam @ orc.nonlindiff3d_rhs(n, d0 + e * w, src, u)
+
+        d1 = (val(1e-4) - val(-1e-4)) / 2e-4
+        d2 = (val(5e-5) - val(-5e-5)) / 1e-4
+        fd = (4 * d2 - d1) / 3
+        gd0, gs = g[low.aux.index("d0")], g[low.aux.index("s")]
+        assert abs(gd0 @ w - fd) <= 1e-8 * abs(fd), (tile, gd0 @ w, fd)
+        # s enters linearly at the node: its gradient is lam on the unknown nodes, 0 elsewhere
+        box = np.zeros((n, n, n))
+        (l0, l1, l2), (h0, h1, h2) = low.lo[0], low.hi[0]
+        box[l0 - 1:h0, l1 - 1:h1, l2 - 1:h2] = lam.reshape((h0 - l0 + 1, h1 - l1 + 1, h2 - l2 + 1), order="F")
+        np.testing.assert_allclose(gs, box.reshape(-1, order="F"), rtol=0, atol=1e-14)

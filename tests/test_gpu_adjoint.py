@@ -38,6 +38,13 @@ def _nonlin(n, seed=1):
     return prob, d0.reshape(-1, order="F"), src.reshape(-1, order="F")
 
 
+def _grad_of(prob, g, name):
+    """Gradient with respect to grid field `name` out of the stacked FP64 gradient array."""
+    ng = int(np.prod(prob.lowered.n))
+    a = prob.lowered.aux.index(name)
+    return g[a * ng:(a + 1) * ng].cpu().numpy()
+
+
 def _on_box(prob, field):
     """A full-grid field restricted to the unknown box (the state layout)."""
     (l0, l1, l2), (h0, h1, h2) = prob.lowered.lo[0], prob.lowered.hi[0]
@@ -80,8 +87,8 @@ def test_vjp_dot_product_nonlinear_diffusion_vs_oracle(n):
     g = torch.zeros(2 * n ** 3, dtype=torch.float64, device="cuda")
     print("  nonlin vjp n=%d" % n, flush=True)
     got = prob.vjp(_dev(u), _dev(lam), t=0.3, g_aux=g).cpu().numpy()
-    gd0 = g[: n ** 3].cpu().numpy()
-    gs = g[n ** 3:].cpu().numpy()
+    gd0 = _grad_of(prob, g, "d0")
+    gs = _grad_of(prob, g, "s")
 
     def F(uu, dd):
         return orc.nonlindiff3d_rhs(n, dd, src, uu, t=0.3, threads=8)
@@ -176,7 +183,7 @@ def test_adjoint_gradient_matches_forward_mode_oracle_and_finite_differences():
     g = torch.zeros(2 * n ** 3, dtype=torch.float64, device="cuda")
     print("  adjoint backward", flush=True)
     prob.adjoint(lam, g)
-    gd0 = g[: n ** 3].cpu().numpy()
+    gd0 = _grad_of(prob, g, "d0")
     # the checkpointed forward is the plain forward
     L_gpu, u_plain = _loss_and_state(prob, u0, dt, nsteps, target)
     assert np.array_equal(u_plain.cpu().numpy(), uT)
@@ -224,4 +231,6 @@ def test_autograd_function_returns_the_adjoint_gradients():
     prob.adjoint(lam, g)
     # the reverse sweep sums tap adjoints with atomics: equal up to the order of summation
     assert torch.allclose(u0.grad, lam, rtol=1e-12, atol=1e-14 * float(lam.abs().max()))
-    assert torch.allclose(d0_t.grad, g[: n ** 3], rtol=1e-12, atol=1e-14 * float(g.abs().max()))
+    a = prob.lowered.aux.index("d0")
+    gd = g[a * n ** 3:(a + 1) * n ** 3]
+    assert torch.allclose(d0_t.grad, gd, rtol=1e-12, atol=1e-14 * float(g.abs().max()))

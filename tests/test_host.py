@@ -284,7 +284,7 @@ def test_generated_kernel_compiles_for_gfx950(name):
     import molgpu
     sys_, disc = _codegen_systems()[name]()
     src = molgpu.codegen(lower(sys_, disc), compile=True)
-    assert 'extern "C" __global__' in src and "mol_full_tile<Geo>" in src
+    assert 'extern "C" __global__' in src and "mol_full_tile<GeoB>" in src
     # the interpreter is not part of the generated kernel
     assert "stack_prog" not in src and "cpool" not in src
 
@@ -296,7 +296,7 @@ def test_generated_vjp_kernel_compiles_for_gfx950(name):
     import molgpu
     sys_, disc = _codegen_systems()[name]()
     src = molgpu.codegen(lower(sys_, disc), compile=True, vjp=True)
-    assert "mol_vjp_tile<Geo>" in src and "atomicAdd(&Ab[" in src and "static __device__ void scatter(" in src
+    assert "mol_vjp_tile<GeoB>" in src and "static __device__ void scatter(" in src
 
 
 def test_generated_kernel_folds_exact_weights():
