@@ -68,6 +68,34 @@ def bench_weno(n, reps):
             "bytes_per_launch": pts * 16, "achieved_gbs": ach, "hbm_frac": ach / HBM_PEAK_GBS}
 
 
+def bench_rhs3d(n, reps):
+    """One FP64 RHS of the 3D nonlinear diffusion (config 5 system) at n^3 nodes."""
+    import numpy as np
+    import torch
+    import molgpu
+    from molgpu import systems
+    s, d = systems.nonlindiff3d(n)
+    p = molgpu.discretize(s, d)
+    p.set_grid_field("d0", 0.5 + 0.5 * np.abs(systems.gaussian_field3d(n, 1)))
+    p.set_grid_field("s", systems.gaussian_field3d(n, 2))
+    u = torch.tensor(np.random.default_rng(0).uniform(-1, 1, p.lowered.n_unknowns), device="cuda")
+    us = rhs_time_us(p, u, reps)
+    pts = p.lowered.n_unknowns
+    ach = pts * 32 / (us * 1e-6) / 1e9
+    lam = torch.randn_like(u)
+    for _ in range(3):
+        p.vjp(u, lam)
+    torch.cuda.synchronize()
+    e0, e1 = events()
+    e0.record()
+    for _ in range(20):
+        p.vjp(u, lam)
+    e1.record()
+    torch.cuda.synchronize()
+    return {"grid": "%d^3" % n, "unknowns": pts, "plan": p.plan(), "rhs_us": us, "achieved_gbs": ach,
+            "hbm_frac": ach / HBM_PEAK_GBS, "vjp_us": e0.elapsed_time(e1) * 1e3 / 20}
+
+
 def bench_adjoint(n, nsteps, every, fd_cells):
     """Config 5: forward (checkpointed) and adjoint times; gradient error against the oracle's
     forward mode on a 16^3 copy and against central differences of the GPU solve on cells."""
@@ -122,21 +150,26 @@ def bench_adjoint(n, nsteps, every, fd_cells):
     d0t = torch.tensor(d0, device="cuda")
     p.set_grid_field("d0", d0t)
 
-    def loss(e, c):
+    def final(e, c):
         d0t.copy_(torch.tensor(d0, device="cuda"))
         d0t[c] += e
         u = u0d.clone()
         p.integrate(u, 0.0, dt, nsteps, molgpu.RK4, graph=True)
-        return 0.5 * float(((u - tgd) ** 2).sum())
+        return u
+
+    def dloss(e, c):
+        # L(+e) - L(-e) = 1/2 sum (u+ - u-)(u+ + u- - 2 target): no cancellation of two losses
+        up, um = final(e, c), final(-e, c)
+        return 0.5 * float(((up - um) * (up + um - 2 * tgd)).sum())
 
     rng = np.random.default_rng(7)
     cells = rng.choice(ng, size=fd_cells, replace=False)
     fd_err = 0.0
     for c in cells:
-        h = 1e-3 * d0[c]
-        D1 = (loss(h, c) - loss(-h, c)) / (2 * h)
-        D2 = (loss(h / 2, c) - loss(-h / 2, c)) / h
-        fd = (4 * D2 - D1) / 3
+        h = 1e-2 * d0[c]
+        D1 = dloss(h, c) / (2 * h)
+        D2 = dloss(h / 2, c) / h
+        fd = (4 * D2 - D1) / 3        # Richardson: O(h^4)
         fd_err = max(fd_err, abs(gd0[c] - fd) / max(abs(fd), 1e-300))
     d0t.copy_(torch.tensor(d0, device="cuda"))
     # oracle forward mode on a 16^3 copy (same construction), direction seeded
@@ -215,6 +248,11 @@ def main():
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only (no weno/adjoint/batch sections)")
+    ap.add_argument("--section", default="all",
+                    choices=["all", "weno", "adjoint", "batch", "rhs3d", "copy"],
+                    help="run one section alone (for profiling); prints its JSON")
+    ap.add_argument("--adj-steps", type=int, default=200)
+    ap.add_argument("--reps", type=int, default=200, help="back-to-back RHS launches of the weno/rhs3d sections")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -224,6 +262,26 @@ def main():
     import torch
     import molgpu
     from molgpu import systems
+
+    if args.section != "all":
+        if args.section == "weno":
+            res = {"1024": bench_weno(1024, args.reps)}
+        elif args.section == "adjoint":
+            res = bench_adjoint(128, args.adj_steps, 20, 2)
+        elif args.section == "batch":
+            res = bench_batch(512, 50, quick=True)
+        elif args.section == "copy":
+            # calibration of the HBM byte counters: device copies of a known size
+            x = torch.ones(8 << 20, dtype=torch.float64, device="cuda")
+            y = torch.empty_like(x)
+            for _ in range(10):
+                y.copy_(x)
+            torch.cuda.synchronize()
+            res = {"bytes_read_per_copy": x.numel() * 8, "bytes_written_per_copy": x.numel() * 8, "copies": 10}
+        else:
+            res = bench_rhs3d(128, args.reps)
+        print(json.dumps({"section": args.section, "result": res}))
+        return
 
     n = 513
     sys_, disc = systems.brusselator2d(n=n)

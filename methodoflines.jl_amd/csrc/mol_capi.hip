@@ -581,7 +581,7 @@ int ensure_workspace(mol_handle* h) {
   if (!h->w_y2) HIPCHK(h, hipMalloc(&h->w_y2, bytes));
   if (!h->w_acc) HIPCHK(h, hipMalloc(&h->w_acc, bytes));
   if (!h->w_comp) HIPCHK(h, hipMalloc(&h->w_comp, bytes));
-  if (!h->d_tnow) HIPCHK(h, hipMalloc((void**)&h->d_tnow, sizeof(double)));
+  if (!h->d_tnow) HIPCHK(h, hipMalloc((void**)&h->d_tnow, 2 * sizeof(double)));
   return 0;
 }
 
@@ -616,8 +616,10 @@ int enqueue_steps(mol_handle* h, T* u, const T* p, double dt, int64_t nsteps, in
   T* acc = (T*)h->w_acc;
   T* comp = (T*)h->w_comp;
   for (int64_t step = 0; step < nsteps; ++step) {
+    double* tcur = h->d_tnow + h->clk;
+    double* tnxt = h->d_tnow + (1 - h->clk);
     Epi<T> e{};
-    e.tnow = h->d_tnow;
+    e.tnow = tcur;
     e.p = p;
     int rc = 0;
     if (scheme == MOL_RK4) {
@@ -628,7 +630,8 @@ int enqueue_steps(mol_handle* h, T* u, const T* p, double dt, int64_t nsteps, in
       e.out = y1; e.cf = dt; e.tofs = dt / 2;
       rc |= launch_rhs<T>(L, y2, e, s);
       Epi<T> l{};
-      l.tnow = h->d_tnow; l.p = p; l.tofs = dt; l.out = u; l.A = u; l.B = acc; l.cb = dt / 6; l.cf = dt / 6;
+      l.tnow = tcur; l.p = p; l.tofs = dt; l.out = u; l.A = u; l.B = acc; l.cb = dt / 6; l.cf = dt / 6;
+      l.tnext = tnxt; l.tadv = dt;
       if (kahan) { l.mode = 2; l.subA = 0; l.comp = comp; }
       else { l.mode = 1; l.ca = 1.0; }
       rc |= launch_rhs<T>(L, y1, l, s);
@@ -638,13 +641,14 @@ int enqueue_steps(mol_handle* h, T* u, const T* p, double dt, int64_t nsteps, in
       e.out = y2; e.ca = 0.75; e.B = y1; e.cb = 0.25; e.cf = dt / 4; e.tofs = dt;
       rc |= launch_rhs<T>(L, y1, e, s);
       Epi<T> l{};
-      l.tnow = h->d_tnow; l.p = p; l.tofs = dt / 2; l.out = u; l.A = u; l.B = y2;
+      l.tnow = tcur; l.p = p; l.tofs = dt / 2; l.out = u; l.A = u; l.B = y2;
+      l.tnext = tnxt; l.tadv = dt;
       l.cb = 2.0 / 3.0; l.cf = 2.0 * dt / 3.0;
       if (kahan) { l.mode = 2; l.subA = 1; l.comp = comp; }
       else { l.mode = 1; l.ca = 1.0 / 3.0; }
       rc |= launch_rhs<T>(L, y2, l, s);
     }
-    rc |= launch_advance_time(h->d_tnow, dt, s);
+    h->clk ^= 1;
     if (rc) return fail(h, MOL_ERR_HIP, std::string("kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
   }
   return 0;
@@ -656,7 +660,7 @@ int integrate_impl(mol_handle* h, T* u, const T* p, double t0, double dt, int64_
   int rc = ensure_workspace(h);
   if (rc) return rc;
   const bool kahan = (flags & MOL_FLAG_KAHAN) != 0;
-  HIPCHK(h, hipMemcpyAsync(h->d_tnow, &t0, sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(h->d_tnow + h->clk, &t0, sizeof(double), hipMemcpyHostToDevice, s));
   if (kahan) HIPCHK(h, hipMemsetAsync(h->w_comp, 0, (size_t)h->nunk * h->esize(), s));
   if (!(flags & MOL_FLAG_GRAPH) || nsteps < 4) {
     return enqueue_steps<T>(h, u, p, dt, nsteps, scheme, kahan, s);
@@ -767,8 +771,9 @@ int rk4_step_store(mol_handle* h, const T* p, const T* uin, T* y1, T* y2, T* y3,
                    hipStream_t s) {
   const Launch<T> L = launch_of<T>(h);
   T* acc = (T*)h->w_acc;
+  double* tcur = h->d_tnow + h->clk;
   Epi<T> e{};
-  e.tnow = h->d_tnow;
+  e.tnow = tcur;
   e.p = p;
   int rc = 0;
   e.mode = 1; e.out = y1; e.A = uin; e.ca = 1.0; e.cf = dt / 2; e.acc = acc; e.acc_keep = 0.0; e.acc_f = 1.0; e.tofs = 0.0;
@@ -778,10 +783,10 @@ int rk4_step_store(mol_handle* h, const T* p, const T* uin, T* y1, T* y2, T* y3,
   e.out = y3; e.cf = dt; e.tofs = dt / 2;
   rc |= launch_rhs<T>(L, y2, e, s);
   Epi<T> l{};
-  l.tnow = h->d_tnow; l.p = p; l.tofs = dt; l.mode = 1; l.out = uout; l.A = uin; l.ca = 1.0; l.B = acc;
-  l.cb = dt / 6; l.cf = dt / 6;
+  l.tnow = tcur; l.p = p; l.tofs = dt; l.mode = 1; l.out = uout; l.A = uin; l.ca = 1.0; l.B = acc;
+  l.cb = dt / 6; l.cf = dt / 6; l.tnext = h->d_tnow + (1 - h->clk); l.tadv = dt;
   rc |= launch_rhs<T>(L, y3, l, s);
-  rc |= launch_advance_time(h->d_tnow, dt, s);
+  h->clk ^= 1;
   if (rc) return fail(h, MOL_ERR_HIP, std::string("kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
   return 0;
 }
@@ -804,11 +809,11 @@ int forward_ckpt_impl(mol_handle* h, T* u, const T* p, double t0, double dt, int
     h->ck_cap = (size_t)nseg;
   }
   h->ck_valid = false;
-  HIPCHK(h, hipMemcpyAsync(h->d_tnow, &t0, sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(h->d_tnow + h->clk, &t0, sizeof(double), hipMemcpyHostToDevice, s));
   for (int64_t g = 0; g < nseg; ++g) {
     const int64_t len = std::min<int64_t>(every, nsteps - g * every);
     HIPCHK(h, hipMemcpyAsync((char*)h->ck_u + g * vbytes, u, vbytes, hipMemcpyDeviceToDevice, s));
-    HIPCHK(h, hipMemcpyAsync(h->ck_t + g, h->d_tnow, sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHK(h, hipMemcpyAsync(h->ck_t + g, h->d_tnow + h->clk, sizeof(double), hipMemcpyDeviceToDevice, s));
     rc = enqueue_steps<T>(h, u, p, dt, len, MOL_RK4, false, s);
     if (rc) return rc;
   }
@@ -848,14 +853,19 @@ int adjoint_impl(mol_handle* h, const T* p, T* lam, double* g_aux, hipStream_t s
   T* S = (T*)h->seg_u;
   auto st = [&](int64_t j, int k) { return S + (size_t)(4 * j + k) * n; };   // state k of step j
   T* Y[4] = {(T*)h->adj_y[0], (T*)h->adj_y[1], (T*)h->adj_y[2], (T*)h->adj_y[3]};
+  // the stage adjoints accumulate with atomics: cleared here, then by every step's sum kernel
+  for (int k = 0; k < 4; ++k)
+    if ((rc = launch_fill_zero<T>(Y[k], n, s))) return fail(h, rc, "zero fill launch failed");
   for (int64_t g = nseg - 1; g >= 0; --g) {
     const int64_t len = std::min<int64_t>(every, nsteps - g * every);
     // recompute the segment from its checkpoint, keeping every stage state and step time
     HIPCHK(h, hipMemcpyAsync(st(0, 0), (char*)h->ck_u + g * vbytes, vbytes, hipMemcpyDeviceToDevice, s));
-    HIPCHK(h, hipMemcpyAsync(h->d_tnow, h->ck_t + g, sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHK(h, hipMemcpyAsync(h->d_tnow + h->clk, h->ck_t + g, sizeof(double), hipMemcpyDeviceToDevice, s));
     for (int64_t j = 0; j < len; ++j) {
-      HIPCHK(h, hipMemcpyAsync(h->seg_t + j, h->d_tnow, sizeof(double), hipMemcpyDeviceToDevice, s));
-      rc = rk4_step_store<T>(h, p, st(j, 0), st(j, 1), st(j, 2), st(j, 3), j + 1 < len ? st(j + 1, 0) : Y[0], dt, s);
+      HIPCHK(h, hipMemcpyAsync(h->seg_t + j, h->d_tnow + h->clk, sizeof(double), hipMemcpyDeviceToDevice, s));
+      // the state after the segment is not needed: it goes to the integrator's scratch
+      rc = rk4_step_store<T>(h, p, st(j, 0), st(j, 1), st(j, 2), st(j, 3), j + 1 < len ? st(j + 1, 0) : (T*)h->w_y2,
+                             dt, s);
       if (rc) return rc;
     }
     // backward sweep: k4 = f(y3, t+dt), k3 = f(y2, t+dt/2), k2 = f(y1, t+dt/2), k1 = f(u, t)
@@ -865,7 +875,6 @@ int adjoint_impl(mol_handle* h, const T* p, T* lam, double* g_aux, hipStream_t s
       const double c1[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
       const double c2[4] = {0.0, dt, dt / 2, dt / 2};
       for (int q = 0; q < 4; ++q) {
-        if ((rc = launch_fill_zero<T>(Y[q], n, s))) return fail(h, rc, "zero fill launch failed");
         VArgs<T> va{};
         va.u = st(j, stage_state[q]);
         va.s1 = lam;
@@ -1027,6 +1036,7 @@ int plan_batched(const mol_desc* const* ds, const double* const* params, int32_t
          "    const BInst* __restrict__ insts, const int* __restrict__ wl, const int nwork) {\n"
          "  __shared__ T L[LDSN];\n"
          "  const double t = (ep.tnow ? ep.tnow[0] : 0.0) + ep.tofs;\n"
+         "  mol_advance_clock(ep);\n"
          "  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {\n"
          "    const BInst& hd = insts[wl[w]];\n"
          "    const int tile = w - hd.first;\n"

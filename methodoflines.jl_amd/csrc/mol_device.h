@@ -73,7 +73,16 @@ struct Epi {
   const double* tnow;   // device time of the step start (nullptr: 0)
   double tofs;          // stage offset added to *tnow (or the absolute time if tnow is null)
   const T* p;           // scalar parameters
+  double* tnext;        // last stage of a step: the next step's clock, *tnext = *tnow + tadv
+  double tadv;
 };
+
+// Device clock of the stage loop: the last stage of a step writes the next step's start time
+// into the other buffer of a double-buffered clock (no kernel of the launch reads it).
+template <typename T>
+__device__ __forceinline__ void advance_clock(const Epi<T>& ep) {
+  if (ep.tnext && blockIdx.x == 0 && threadIdx.x == 0) ep.tnext[0] = ep.tnow[0] + ep.tadv;
+}
 
 template <typename T>
 __device__ __forceinline__ void epilogue(const Epi<T>& ep, long long idx, T f) {

@@ -105,12 +105,11 @@ template <typename T>
 int launch_weno2d(const KCore<T>& kc, const WenoPlan& plan, int nblocks, const T* u, const Epi<T>& ep,
                   hipStream_t stream);
 constexpr int WENO_TILE_X = 64, WENO_TILE_Y = 32;
-int launch_advance_time(double* tnow, double dt, hipStream_t stream);
 template <typename T>
 int launch_fill_zero(T* p, int64_t n, hipStream_t stream);
-// lam += y3 + y2 + y1 + y0 (the adjoint of one RK4 step)
+// lam += y0 + y1 + y2 + y3 (the adjoint of one RK4 step), then y0..y3 = 0
 template <typename T>
-int launch_adjoint_sum(T* lam, const T* y0, const T* y1, const T* y2, const T* y3, int64_t n, hipStream_t stream);
+int launch_adjoint_sum(T* lam, T* y0, T* y1, T* y2, T* y3, int64_t n, hipStream_t stream);
 
 // Set the message of the calling thread's last failure (used when no handle exists yet).
 void set_global_error(const std::string& msg);
@@ -214,7 +213,8 @@ struct mol_handle {
   void* w_y2 = nullptr;
   void* w_acc = nullptr;
   void* w_comp = nullptr;
-  double* d_tnow = nullptr;
+  double* d_tnow = nullptr;                // double-buffered device clock: d_tnow[clk] is current
+  int clk = 0;
   // rhs_h staging
   void* h_u = nullptr;
   void* h_du = nullptr;

@@ -1279,7 +1279,8 @@ std::string Gen::source_kernel(bool f64, bool vjp) {
   if (!vjp) {
     o << "extern \"C\" __global__ void __launch_bounds__(256) mol_rhs_tiles(const T* __restrict__ u, Epi ep, JPtrs jp) {\n"
          "  __shared__ T L[LDSN];\n" << decode
-      << "  const double t = (ep.tnow ? ep.tnow[0] : 0.0) + ep.tofs;\n";
+      << "  const double t = (ep.tnow ? ep.tnow[0] : 0.0) + ep.tofs;\n"
+         "  mol_advance_clock(ep);\n";
     if (g_.ncore > 0)
       o << "  if (GeoBase::is_core(tx, ty, tz)) mol_full_tile<GeoC>(L, tx, ty, tz, u, ep, jp, t);\n  else ";
     else

@@ -25,7 +25,13 @@ struct Epi {
   const double* tnow;
   double tofs;
   const T* p;
+  double* tnext;   // last stage of a step: *tnext = *tnow + tadv (double-buffered device clock)
+  double tadv;
 };
+
+__device__ __forceinline__ void mol_advance_clock(const Epi& ep) {
+  if (ep.tnext && blockIdx.x == 0 && threadIdx.x == 0) ep.tnext[0] = ep.tnow[0] + ep.tadv;
+}
 
 // Node coordinates and auxiliary grid arrays (layout identical to mol::JitPtrs<T>).
 struct JPtrs {

@@ -48,6 +48,7 @@ __global__ void __launch_bounds__(256) rhs_generic_kernel(const KArgs<T>* __rest
   int org[3];
   tile_origin(ka, tiles, org);
   const double t = (ep.tnow ? ep.tnow[0] : 0.0) + ep.tofs;
+  advance_clock(ep);
   stage_tile(ka, u, lds, org, t, ep.p);
   __syncthreads();
 
@@ -114,6 +115,7 @@ __global__ void __launch_bounds__(256) rhs_core_kernel(const KCore<T> kc_arg, co
   const int org1 = kc.lo[1] + ty * kc.tile[1];
   const int org2 = kc.lo[2] + tz * kc.tile[2];
   const double t = (ep.tnow ? ep.tnow[0] : 0.0) + ep.tofs;
+  advance_clock(ep);
   const int bx = kc.hi[0] - kc.lo[0] + 1;
   const int by = kc.hi[1] - kc.lo[1] + 1;
   const int ext01 = kc.ext[0] * kc.ext[1];
@@ -190,25 +192,19 @@ int launch_rhs(const Launch<T>& L, const T* u, const Epi<T>& ep, hipStream_t str
       return MOL_ERR_HIP;
     return 0;
   }
+  // the device clock advances in the first kernel of the launch only
+  Epi<T> e2 = ep;
   if (L.ncore > 0) {
     hipLaunchKernelGGL(rhs_core_kernel<T>, dim3(L.ncore), dim3(256), lds, stream, *L.kc, u, ep);
+    e2.tnext = nullptr;
   }
   if (L.nbound > 0)
-    hipLaunchKernelGGL(rhs_generic_kernel<T>, dim3(L.nbound), dim3(256), lds, stream, L.dka, L.tiles, u, ep);
+    hipLaunchKernelGGL(rhs_generic_kernel<T>, dim3(L.nbound), dim3(256), lds, stream, L.dka, L.tiles, u, e2);
   return hipGetLastError() == hipSuccess ? 0 : MOL_ERR_HIP;
 }
 
 template int launch_rhs<double>(const Launch<double>&, const double*, const Epi<double>&, hipStream_t);
 template int launch_rhs<float>(const Launch<float>&, const float*, const Epi<float>&, hipStream_t);
-
-__global__ void advance_time_kernel(double* tnow, double dt) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) tnow[0] += dt;
-}
-
-int launch_advance_time(double* tnow, double dt, hipStream_t stream) {
-  hipLaunchKernelGGL(advance_time_kernel, dim3(1), dim3(64), 0, stream, tnow, dt);
-  return hipGetLastError() == hipSuccess ? 0 : MOL_ERR_HIP;
-}
 
 template <typename T>
 __global__ void fill_zero_kernel(T* __restrict__ p, int64_t n) {
@@ -224,16 +220,22 @@ int launch_fill_zero(T* p, int64_t n, hipStream_t stream) {
   return hipGetLastError() == hipSuccess ? 0 : MOL_ERR_HIP;
 }
 
-// lam += y3 + y2 + y1 + y0, in the order the stage adjoints are produced
+// lam += y0 + y1 + y2 + y3 in the order the stage adjoints are produced; the stage-adjoint
+// buffers are cleared for the next step (they accumulate with atomics)
 template <typename T>
-__global__ void adjoint_sum_kernel(T* __restrict__ lam, const T* __restrict__ y0, const T* __restrict__ y1,
-                                   const T* __restrict__ y2, const T* __restrict__ y3, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+__global__ void adjoint_sum_kernel(T* __restrict__ lam, T* __restrict__ y0, T* __restrict__ y1, T* __restrict__ y2,
+                                   T* __restrict__ y3, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     lam[i] = (((lam[i] + y0[i]) + y1[i]) + y2[i]) + y3[i];
+    y0[i] = T(0);
+    y1[i] = T(0);
+    y2[i] = T(0);
+    y3[i] = T(0);
+  }
 }
 
 template <typename T>
-int launch_adjoint_sum(T* lam, const T* y0, const T* y1, const T* y2, const T* y3, int64_t n, hipStream_t stream) {
+int launch_adjoint_sum(T* lam, T* y0, T* y1, T* y2, T* y3, int64_t n, hipStream_t stream) {
   if (n <= 0) return 0;
   const int blocks = (int)std::min<int64_t>((n + 255) / 256, 8192);
   hipLaunchKernelGGL(adjoint_sum_kernel<T>, dim3(blocks), dim3(256), 0, stream, lam, y0, y1, y2, y3, n);
@@ -242,9 +244,7 @@ int launch_adjoint_sum(T* lam, const T* y0, const T* y1, const T* y2, const T* y
 
 template int launch_fill_zero<double>(double*, int64_t, hipStream_t);
 template int launch_fill_zero<float>(float*, int64_t, hipStream_t);
-template int launch_adjoint_sum<double>(double*, const double*, const double*, const double*, const double*, int64_t,
-                                        hipStream_t);
-template int launch_adjoint_sum<float>(float*, const float*, const float*, const float*, const float*, int64_t,
-                                       hipStream_t);
+template int launch_adjoint_sum<double>(double*, double*, double*, double*, double*, int64_t, hipStream_t);
+template int launch_adjoint_sum<float>(float*, float*, float*, float*, float*, int64_t, hipStream_t);
 
 }  // namespace mol

@@ -125,6 +125,7 @@ __global__ void __launch_bounds__(256) weno2d_kernel(const KCore<T> kc_arg, Weno
   const int org1 = kc.lo[1] + ty * WT_Y;
   const int bx = kc.hi[0] - kc.lo[0] + 1;
   const double t = (ep.tnow ? ep.tnow[0] : 0.0) + ep.tofs;
+  advance_clock(ep);
 
   // stage tile + halo (rows of WEXT_X, padded to WRS)
   for (int q = threadIdx.x; q < WEXT_X * WEXT_Y; q += blockDim.x) {

@@ -31,7 +31,7 @@ class Epi(C.Structure):
     _fields_ = [("mode", C.c_int32), ("subA", C.c_int32), ("out", C.c_void_p), ("A", C.c_void_p), ("B", C.c_void_p),
                 ("acc", C.c_void_p), ("comp", C.c_void_p), ("ca", C.c_double), ("cb", C.c_double),
                 ("cf", C.c_double), ("acc_keep", C.c_double), ("acc_f", C.c_double), ("tnow", C.c_void_p),
-                ("tofs", C.c_double), ("p", C.c_void_p)]
+                ("tofs", C.c_double), ("p", C.c_void_p), ("tnext", C.c_void_p), ("tadv", C.c_double)]
 
 
 class JPtrs(C.Structure):
