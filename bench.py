@@ -10,8 +10,12 @@ point is one unknown grid node with both variables); value = points * 4 * K / el
 The same JSON line carries the other parts of the metric:
   roofline     the RK4 stage kernel of the timed region against HBM peak;
   weno5        one FP64 WENO5 Burgers RHS at 1024^2 (and 4096^2, beyond the 256 MB MALL);
-  adjoint      3D nonlinear diffusion 128^3 (2.0 M unknowns), 200 RK4 steps, checkpoints every
-               20 steps: forward and adjoint times, their ratio, and the gradient error;
+  adjoint      3D nonlinear diffusion 128^3 (2.0 M unknowns), 200 RK4 steps: plain forward time,
+               then per checkpoint mode (every 20 steps with recomputation, and the automatic
+               choice, which keeps every stage state when it fits in device memory) the
+               recording forward and the adjoint sweep; ratio = adjoint sweep / plain forward
+               of the automatic mode; gradient errors against the oracle's forward mode and
+               against central differences;
   batch        512 diffusion-reaction instances with uneven grids, FP32 + Kahan, 200 RK4 steps;
   cpu_baseline the C++ CPU restatement (the oracle) on the host, 1 thread, bounded sample.
 """
@@ -128,12 +132,12 @@ def bench_adjoint(n, nsteps, every, fd_cells):
     tgd = torch.tensor(tgt, device="cuda")
     g = torch.zeros(2 * ng, dtype=torch.float64, device="cuda")
 
-    def run():
+    def run(ck):
         u = u0d.clone()
         e0, e1 = events()
         e2 = torch.cuda.Event(enable_timing=True)
         e0.record()
-        p.forward_checkpointed(u, 0.0, dt, nsteps, every)
+        p.forward_checkpointed(u, 0.0, dt, nsteps, ck)
         e1.record()
         lam = (u - tgd).contiguous()
         g.zero_()
@@ -142,8 +146,25 @@ def bench_adjoint(n, nsteps, every, fd_cells):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1), e1.elapsed_time(e2), u
 
-    run()
-    f_ms, a_ms, uT = run()
+    def plain():
+        u = u0d.clone()
+        e0, e1 = events()
+        e0.record()
+        p.integrate(u, 0.0, dt, nsteps, molgpu.RK4, graph=True)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    plain()
+    fwd_ms = plain()
+    modes = {}
+    auto = p.checkpoint_interval(nsteps)
+    for ck in sorted({every, auto}, reverse=True):
+        run(ck)
+        fc_ms, a_ms, uT = run(ck)
+        modes["ckpt_every=%d" % ck] = {"forward_ckpt_ms": fc_ms, "adjoint_ms": a_ms, "ratio": a_ms / fwd_ms,
+                                       "forward_plus_adjoint_ratio": (fc_ms + a_ms) / fwd_ms}
+    # g holds the gradient of the last mode run (the automatic choice)
     ia = p.lowered.aux.index("d0")
     gd0 = g[ia * ng:(ia + 1) * ng].cpu().numpy()
     # finite differences of the GPU solve on seeded cells (Richardson-extrapolated central)
@@ -175,7 +196,7 @@ def bench_adjoint(n, nsteps, every, fd_cells):
     # oracle forward mode on a 16^3 copy (same construction), direction seeded
     p16, d016, src16, u016, tgt16, dt16 = make(16)
     u = torch.tensor(u016, device="cuda")
-    p16.forward_checkpointed(u, 0.0, dt16, nsteps, every)
+    p16.forward_checkpointed(u, 0.0, dt16, nsteps, auto)
     lam = (u - torch.tensor(tgt16, device="cuda")).contiguous()
     g16 = torch.zeros(2 * 16 ** 3, dtype=torch.float64, device="cuda")
     p16.adjoint(lam, g16)
@@ -184,9 +205,11 @@ def bench_adjoint(n, nsteps, every, fd_cells):
     i16 = p16.lowered.aux.index("d0")
     fm_err = abs(float(g16[i16 * 16 ** 3:(i16 + 1) * 16 ** 3].cpu().numpy() @ v) - dL) / abs(dL)
     rhs_us = rhs_time_us(p, u0d, 20)
-    return {"grid": "%d^3" % n, "unknowns": p.lowered.n_unknowns, "steps": nsteps, "ckpt_every": every,
-            "dt": dt, "forward_ms": f_ms, "adjoint_ms": a_ms, "ratio": a_ms / f_ms, "rhs_us": rhs_us,
-            "grad_rel_err_forward_mode_16": fm_err, "grad_rel_err_fd_cells": fd_err, "fd_cells": int(fd_cells)}
+    best = modes["ckpt_every=%d" % auto]
+    return {"grid": "%d^3" % n, "unknowns": p.lowered.n_unknowns, "steps": nsteps, "dt": dt,
+            "forward_ms": fwd_ms, "ckpt_every": auto, "adjoint_ms": best["adjoint_ms"], "ratio": best["ratio"],
+            "modes": modes, "rhs_us": rhs_us, "grad_rel_err_forward_mode_16": fm_err,
+            "grad_rel_err_fd_cells": fd_err, "fd_cells": int(fd_cells)}
 
 
 def bench_batch(count, nsteps, quick):

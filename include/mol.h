@@ -242,14 +242,16 @@ int mol_vjp(mol_handle* h, const void* u, const void* p, double t, const void* l
             double* g_aux, void* stream);
 
 /* mol_forward_checkpointed: mol_integrate (RK4) that also keeps the state every ckpt_every
- * steps (and u0) in the handle, for a later mol_adjoint of the same run. */
+ * steps (and u0) in the handle, for a later mol_adjoint of the same run.  ckpt_every = 0 keeps
+ * u and the three stage states of every step instead (4 * nsteps state vectors of device
+ * memory; MOL_ERR_HIP if they do not fit): the adjoint then recomputes nothing. */
 int mol_forward_checkpointed(mol_handle* h, void* u, const void* p, double t0, double dt,
                              int64_t nsteps, int32_t ckpt_every, void* stream);
 
 /* mol_adjoint: discrete adjoint of the last checkpointed RK4 run.  On entry lam = dL/du(T);
  * on exit lam = dL/du(t0) and g_aux += dL/daux (FP64).  Each segment between checkpoints is
  * recomputed forward once (stage states kept on the device), then swept backwards with four
- * VJPs per step. */
+ * VJPs per step; after a ckpt_every = 0 run the sweep uses the stored stage states. */
 int mol_adjoint(mol_handle* h, const void* p, void* lam, double* g_aux, void* stream);
 
 /* Number of unknowns (sum over variables of the unknown box sizes). */

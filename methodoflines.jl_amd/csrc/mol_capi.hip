@@ -203,7 +203,8 @@ int validate(const mol_desc* d, std::string& msg) {
 // lds_pos: LDS bytes per tile + halo position (0: the forward kernels' arrays)
 // clamp: shrink the tile to the box (a batch keeps one tile shape for all its instances)
 template <typename T>
-void fill_kargs(mol_handle* h, const mol_desc* d, KArgs<T>& ka, size_t lds_pos = 0, bool clamp = true) {
+void fill_kargs(mol_handle* h, const mol_desc* d, KArgs<T>& ka, size_t lds_pos = 0, bool clamp = true,
+                bool jit_tiles = false, size_t lds_cap = 60000) {
   std::memset(&ka, 0, sizeof(ka));
   ka.ndim = d->ndim;
   ka.nfields = d->nfields;
@@ -258,7 +259,41 @@ void fill_kargs(mol_handle* h, const mol_desc* d, KArgs<T>& ka, size_t lds_pos =
     tile[big] = (tile[big] + 1) / 2;
   }
   if (lds_pos == 0) lds_pos = (size_t)nload * sizeof(T);
-  while (lds_pos * vol_of(tile) > 60000) {
+  if (jit_tiles && h->tile_req[0] <= 0 && d->ndim >= 2) {
+    // generated kernels run several nodes per thread along the slowest axis: the first
+    // candidate whose tile + halo of the staged arrays fits the LDS budget (about three to four
+    // work-groups of 256 threads per compute unit) and that still gives every compute unit
+    // four tiles; small grids fall back to the candidate with the most tiles
+    static const int cand2[][3] = {{64, 16, 1}, {32, 16, 1}, {64, 8, 1}, {64, 4, 1}, {32, 8, 1}, {32, 4, 1}};
+    static const int cand3[][3] = {{16, 8, 8}, {16, 8, 4}, {16, 4, 8}, {16, 4, 4}, {32, 4, 2}, {16, 4, 2}, {8, 4, 4}};
+    const int (*cand)[3] = d->ndim == 2 ? cand2 : cand3;
+    const int nc = d->ndim == 2 ? 6 : 7;
+    const size_t budget = d->ndim == 2 ? 40960 : 57344;
+    int best = -1;
+    int64_t best_tiles = -1;
+    for (int i = 0; i < nc; ++i) {
+      int tl[3];
+      int64_t nt = 1;
+      for (int a = 0; a < 3; ++a) {
+        const int ext = ka.uhi[a] - ka.ulo[a] + 1;
+        tl[a] = a >= d->ndim ? 1 : std::max(1, std::min(cand[i][a], ext));
+        nt *= (ext + tl[a] - 1) / tl[a];
+      }
+      if (lds_pos * vol_of(tl) > budget) continue;
+      if (nt >= 1024) {
+        best = i;
+        break;
+      }
+      if (nt > best_tiles) {
+        best_tiles = nt;
+        best = i;
+      }
+    }
+    if (best >= 0)
+      for (int a = 0; a < 3; ++a)
+        tile[a] = a >= d->ndim ? 1 : std::max(1, std::min(cand[best][a], ka.uhi[a] - ka.ulo[a] + 1));
+  }
+  while (lds_pos * vol_of(tile) > lds_cap) {
     int big = 0;
     for (int a = 1; a < 3; ++a)
       if (tile[a] > tile[big] / 4) big = a;
@@ -310,6 +345,7 @@ Launch<T> launch_of(mol_handle* h) {
   L.ncore = h->ncore;
   L.nbound = h->nbound;
   L.jit = h->jit_fn;
+  L.jgrid = h->jit_grid > 0 ? h->jit_grid : h->ka64.ntile[0] * h->ka64.ntile[1] * h->ka64.ntile[2];
   if (h->batch) {
     L.batch = h->batch_fn;
     L.insts = (const BInst*)h->d_insts;
@@ -503,6 +539,22 @@ bool jit_enabled() {
 
 int sync_kargs(mol_handle* h);
 
+// Work-groups of a persistent launch over `ntiles` tiles: every one that fits on the device at
+// once (MOL_BPC=k: k per compute unit), a multiple of 8 (tile k + grid stays on the XCD of tile
+// k), at most one per tile.
+int persistent_grid(hipFunction_t fn, int ntiles) {
+  int dev = 0, ncu = 0, occ = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+    return ntiles;
+  if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, 0) != hipSuccess || occ <= 0) occ = 1;
+  if (const char* e = std::getenv("MOL_BPC")) occ = std::max(1, std::atoi(e));
+  int64_t g = (int64_t)occ * ncu;
+  if (g >= ntiles) return ntiles;
+  g = std::max<int64_t>(8, g / 8 * 8);
+  return (int)std::min<int64_t>(g, ntiles);
+}
+
 int setup_paths(mol_handle* h, const mol_desc* d) {
   std::vector<int32_t> t64, t32;
   int n64 = 0, n32 = 0;
@@ -515,8 +567,9 @@ int setup_paths(mol_handle* h, const mol_desc* d) {
   const void* aux_bound[MOL_MAX_AUX];
   for (int a = 0; a < MOL_MAX_AUX; ++a) aux_bound[a] = h->ka64.aux[a];
   auto refill = [&](size_t pos) {
-    fill_kargs<double>(h, d, h->ka64, pos);
-    fill_kargs<float>(h, d, h->ka32, pos);
+    const bool jt = pos > 0;
+    fill_kargs<double>(h, d, h->ka64, pos, true, jt);
+    fill_kargs<float>(h, d, h->ka32, pos, true, jt);
     for (int a = 0; a < MOL_MAX_AUX; ++a) {
       h->ka64.aux[a] = (const double*)aux_bound[a];
       h->ka32.aux[a] = (const float*)aux_bound[a];
@@ -529,10 +582,8 @@ int setup_paths(mol_handle* h, const mol_desc* d) {
   if (jit_enabled()) {
     // tiles sized for the arrays the generated kernel stages (the interpreter stages them all)
     const size_t pos = jit_lds_per_pos(d, geom_of(h->ka64, h->kc64, n64), h->esize(), false);
-    if (pos > 0 && pos < (size_t)(d->nfields + d->naux) * h->esize()) {
-      refill(pos);
-      retiled = true;
-    }
+    refill(pos > 0 ? pos : h->esize());
+    retiled = true;
     std::string err;
     const std::string src = jit_source(d, geom_of(h->ka64, h->kc64, n64), err);
     if (src.empty()) h->jit_reason = err;
@@ -540,6 +591,9 @@ int setup_paths(mol_handle* h, const mol_desc* d) {
       h->jit_fn = nullptr;
       h->jit_reason = err;
     }
+    const int ntl = h->ka64.ntile[0] * h->ka64.ntile[1] * h->ka64.ntile[2];
+    if (h->jit_fn)
+      h->jit_grid = src.find("mol_tiles_persistent<") != std::string::npos ? persistent_grid(h->jit_fn, ntl) : ntl;
     if (!h->jit_fn && retiled) refill(0);   // back to the interpreter's tiles
   } else {
     h->jit_reason = "code generation disabled (MOL_JIT=0)";
@@ -718,7 +772,20 @@ JitGeom vjp_geometry(mol_handle* h, const mol_desc* d, int* ntiles) {
     h->core_reason = reason;
     h->core_interp_ok = interp_ok;
     const size_t pos = jit_lds_per_pos(d, geom_of(kv, kc0, nc0), h->esize(), true);
-    if (pos > 0) fill_kargs<double>(h, d, kv, pos);
+    // MOL_VJP_TILE=tx,ty,tz asks for a tile of the reverse-mode kernel (LDS up to 160 KB)
+    int req[3] = {0, 0, 0};
+    const char* e = std::getenv("MOL_VJP_TILE");
+    if (e && std::sscanf(e, "%d,%d,%d", &req[0], &req[1], &req[2]) == 3 && req[0] > 0) {
+      int keep[3];
+      for (int a = 0; a < 3; ++a) {
+        keep[a] = h->tile_req[a];
+        h->tile_req[a] = std::max(1, req[a]);
+      }
+      fill_kargs<double>(h, d, kv, pos > 0 ? pos : h->esize(), true, false, 160000);
+      for (int a = 0; a < 3; ++a) h->tile_req[a] = keep[a];
+    } else if (pos > 0) {
+      fill_kargs<double>(h, d, kv, pos, true, true);
+    }
   }
   KCore<double> kc;
   std::vector<int32_t> gen;
@@ -796,8 +863,44 @@ int forward_ckpt_impl(mol_handle* h, T* u, const T* p, double t0, double dt, int
                       hipStream_t s) {
   int rc = ensure_workspace(h);
   if (rc) return rc;
-  const int64_t nseg = (nsteps + every - 1) / every;
   const size_t vbytes = (size_t)h->nunk * h->esize();
+  h->ck_valid = false;
+  if (every == 0) {
+    // keep u and the three stage states of every step: the adjoint then runs no forward
+    // recomputation (4 * nsteps state vectors of device memory)
+    const size_t need = (size_t)(4 * nsteps + 1);
+    if (h->seg_cap < need) {
+      if (h->seg_u) (void)hipFree(h->seg_u);
+      if (h->seg_t) (void)hipFree(h->seg_t);
+      h->seg_u = nullptr;
+      h->seg_t = nullptr;
+      h->seg_cap = 0;
+      if (hipMalloc(&h->seg_u, need * vbytes) != hipSuccess) {
+        (void)hipGetLastError();
+        h->seg_u = nullptr;
+        return fail(h, MOL_ERR_HIP, "not enough device memory to keep every stage state (use ckpt_every >= 1)");
+      }
+      HIPCHK(h, hipMalloc((void**)&h->seg_t, (size_t)(nsteps + 1) * sizeof(double)));
+      h->seg_cap = need;
+    }
+    const int64_t n = h->nunk;
+    T* S = (T*)h->seg_u;
+    auto st = [&](int64_t j, int k) { return S + (size_t)(4 * j + k) * n; };
+    HIPCHK(h, hipMemcpyAsync(h->d_tnow + h->clk, &t0, sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(h, hipMemcpyAsync(st(0, 0), u, vbytes, hipMemcpyDeviceToDevice, s));
+    for (int64_t j = 0; j < nsteps; ++j) {
+      HIPCHK(h, hipMemcpyAsync(h->seg_t + j, h->d_tnow + h->clk, sizeof(double), hipMemcpyDeviceToDevice, s));
+      rc = rk4_step_store<T>(h, p, st(j, 0), st(j, 1), st(j, 2), st(j, 3), j + 1 < nsteps ? st(j + 1, 0) : u, dt, s);
+      if (rc) return rc;
+    }
+    h->ck_nsteps = nsteps;
+    h->ck_every = (int32_t)std::min<int64_t>(nsteps, INT32_MAX);
+    h->ck_dt = dt;
+    h->ck_stored = true;
+    h->ck_valid = true;
+    return 0;
+  }
+  const int64_t nseg = (nsteps + every - 1) / every;
   if (h->ck_cap < (size_t)nseg) {
     if (h->ck_u) (void)hipFree(h->ck_u);
     if (h->ck_t) (void)hipFree(h->ck_t);
@@ -820,6 +923,7 @@ int forward_ckpt_impl(mol_handle* h, T* u, const T* p, double t0, double dt, int
   h->ck_nsteps = nsteps;
   h->ck_every = every;
   h->ck_dt = dt;
+  h->ck_stored = false;
   h->ck_valid = true;
   return 0;
 }
@@ -838,6 +942,7 @@ int adjoint_impl(mol_handle* h, const T* p, T* lam, double* g_aux, hipStream_t s
   const int64_t n = h->nunk;
   const size_t vbytes = (size_t)n * h->esize();
   const size_t need = (size_t)(4 * every + 1);
+  if (h->ck_stored && h->seg_cap < need) return fail(h, MOL_ERR_INVALID, "stored stage states were released");
   if (h->seg_cap < need) {
     if (h->seg_u) (void)hipFree(h->seg_u);
     if (h->seg_t) (void)hipFree(h->seg_t);
@@ -859,9 +964,12 @@ int adjoint_impl(mol_handle* h, const T* p, T* lam, double* g_aux, hipStream_t s
   for (int64_t g = nseg - 1; g >= 0; --g) {
     const int64_t len = std::min<int64_t>(every, nsteps - g * every);
     // recompute the segment from its checkpoint, keeping every stage state and step time
-    HIPCHK(h, hipMemcpyAsync(st(0, 0), (char*)h->ck_u + g * vbytes, vbytes, hipMemcpyDeviceToDevice, s));
-    HIPCHK(h, hipMemcpyAsync(h->d_tnow + h->clk, h->ck_t + g, sizeof(double), hipMemcpyDeviceToDevice, s));
-    for (int64_t j = 0; j < len; ++j) {
+    // (a run that kept them all is one segment with nothing to recompute)
+    if (!h->ck_stored) {
+      HIPCHK(h, hipMemcpyAsync(st(0, 0), (char*)h->ck_u + g * vbytes, vbytes, hipMemcpyDeviceToDevice, s));
+      HIPCHK(h, hipMemcpyAsync(h->d_tnow + h->clk, h->ck_t + g, sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    for (int64_t j = 0; !h->ck_stored && j < len; ++j) {
       HIPCHK(h, hipMemcpyAsync(h->seg_t + j, h->d_tnow + h->clk, sizeof(double), hipMemcpyDeviceToDevice, s));
       // the state after the segment is not needed: it goes to the integrator's scratch
       rc = rk4_step_store<T>(h, p, st(j, 0), st(j, 1), st(j, 2), st(j, 3), j + 1 < len ? st(j + 1, 0) : (T*)h->w_y2,
@@ -1324,7 +1432,7 @@ int mol_forward_checkpointed(mol_handle* h, void* u, const void* p, double t0, d
   if (!h) return fail(nullptr, MOL_ERR_INVALID, "handle is NULL");
   if (h->batch) return fail(h, MOL_ERR_UNSUPPORTED, "reverse mode of batched handles");
   if (!u) return fail(h, MOL_ERR_INVALID, "u is NULL");
-  if (nsteps < 1 || ckpt_every < 1) return fail(h, MOL_ERR_INVALID, "nsteps and ckpt_every must be >= 1");
+  if (nsteps < 1 || ckpt_every < 0) return fail(h, MOL_ERR_INVALID, "nsteps must be >= 1 and ckpt_every >= 0");
   if (h->nparams > 0 && !p) return fail(h, MOL_ERR_INVALID, "parameters missing");
   if (int rc = check_aux(h)) return rc;
   hipStream_t s = (hipStream_t)stream;
@@ -1372,11 +1480,9 @@ int64_t mol_codegen(const mol_desc* d, const int32_t* tile, char* buf, int64_t b
   {
     // the tiles mol_create uses with the generated kernel (sized for its staged arrays)
     const size_t pos = jit_lds_per_pos(d, geom_of(tmp.ka64, tmp.kc64, ncore), tmp.esize(), false);
-    if (pos > 0 && pos < (size_t)(d->nfields + d->naux) * tmp.esize()) {
-      fill_kargs<double>(&tmp, d, tmp.ka64, pos);
-      gen.clear();
-      build_core<double>(&tmp, d, tmp.ka64, tmp.kc64, gen, ncore);
-    }
+    fill_kargs<double>(&tmp, d, tmp.ka64, pos > 0 ? pos : tmp.esize(), true, true);
+    gen.clear();
+    build_core<double>(&tmp, d, tmp.ka64, tmp.kc64, gen, ncore);
   }
   std::string err;
   const std::string src = (flags & MOL_CODEGEN_VJP) ? jit_vjp_source(d, vjp_geometry(&tmp, d, nullptr), err)

@@ -86,6 +86,7 @@ struct Launch {
   int nweno = 0;
   hipFunction_t jit = nullptr;       // generated kernel over all tiles (nullptr: interpreter)
   int ntiles = 0;
+  int jgrid = 0;                  // work-groups of the persistent generated kernel
   hipFunction_t batch = nullptr;     // persistent kernel over the work list of a batch
   const BInst* insts = nullptr;
   const int32_t* wl = nullptr;
@@ -180,6 +181,7 @@ struct mol_handle {
   void* d_wl = nullptr;
   void* d_bdata = nullptr;                // weights, constants and coordinates of all instances
   std::vector<int64_t> inst_off;          // first unknown of each instance
+  int jit_grid = 0;                       // work-groups of the persistent generated RHS kernel
   hipFunction_t vjp_fn = nullptr;         // generated vector-Jacobian product kernel
   int vjp_tiles = 0;
   bool vjp_aux_ok = true;                 // no boundary value reads an auxiliary array
@@ -190,7 +192,8 @@ struct mol_handle {
   int32_t ck_every = 0;
   double ck_dt = 0.0;
   bool ck_valid = false;
-  void* seg_u = nullptr;                  // stage states of one recomputed segment
+  bool ck_stored = false;                 // the run kept every stage state (no recomputation)
+  void* seg_u = nullptr;                  // stage states of one recomputed segment (or the whole run)
   double* seg_t = nullptr;                // device clock at each step of the segment
   size_t seg_cap = 0;
   void* adj_y[4] = {nullptr, nullptr, nullptr, nullptr};

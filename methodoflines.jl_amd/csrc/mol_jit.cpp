@@ -314,6 +314,35 @@ class Gen {
            istr((long long)sizeof(BInst)) + ", \"argument layout differs from the library\");\n\n";
   }
   std::string geo_common();
+  // the state vector or the full grid has 2^31 entries or more
+  bool wide() const {
+    long long nunk = 0;
+    for (int f = 0; f < d_->nfields; ++f) {
+      long long sz = 1;
+      for (int a = 0; a < 3; ++a) sz *= g_.hi[f][a] - g_.lo[f][a] + 1;
+      nunk = std::max(nunk, (long long)g_.fofs[f] + sz);
+    }
+    const long long ngrid = (long long)g_.n[0] * g_.n[1] * g_.n[2];
+    return nunk >= (1LL << 31) - 1 || ngrid >= (1LL << 31) - 1;
+  }
+  // blocking axis and nodes per thread along it: the slowest axis with more than one tile
+  // node; RB divides the tile along it and leaves at least one node group per thread
+  void blocking(int& ba, int& rb) const {
+    ba = 0;
+    rb = 1;
+    for (int a = 1; a < d_->ndim; ++a)
+      if (g_.tile[a] > 1) ba = a;
+    if (ba == 0) return;
+    const long long vol = (long long)g_.tile[0] * g_.tile[1] * g_.tile[2];
+    // MOL_RB=k asks for k nodes per thread whenever k divides the tile (tests of small tiles)
+    const char* e = std::getenv("MOL_RB");
+    const int want = e ? std::max(1, std::atoi(e)) : 4;
+    for (int r = want; r > 1; r /= 2)
+      if (g_.tile[ba] % r == 0 && (e || vol / r >= 256)) {
+        rb = r;
+        return;
+      }
+  }
 };
 
 // Row of table `tb` along axis `ax` selected at index (node index + idx_rel), as select_row
@@ -1075,7 +1104,15 @@ std::string Gen::geo_common() {
   c("TX", g_.tile[0]); c("TY", g_.tile[1]); c("TZ", g_.tile[2]);
   c("NBLK", g_.ntile[0] * g_.ntile[1] * g_.ntile[2]);
   c("NTHR", 256);
+  // register blocking: each thread runs RB consecutive nodes along axis BA (the slowest tiled
+  // axis), so taps and subexpressions shared by neighbours along BA are loaded and computed once
+  int ba = 0, rb = 1;
+  blocking(ba, rb);
+  c("BA", ba);
+  c("RB", rb);
   o << "  static constexpr i64 NGRID = " << (long long)g_.n[0] * g_.n[1] * g_.n[2] << ";\n";
+  // 32-bit index arithmetic unless the state vector or the grid reaches 2^31 entries
+  o << "  using ix = " << (wide() ? "i64" : "int") << ";\n";
   // core tiles: interior operator rows for every node
   for (int a = 0; a < 3; ++a) {
     c("CLO" + istr(a), g_.ncore > 0 ? g_.ct_lo[a] : 1);
@@ -1083,12 +1120,12 @@ std::string Gen::geo_common() {
   }
   o << "  static __device__ __forceinline__ bool is_core(const int tx, const int ty, const int tz) {\n"
        "    return tx >= CLO0 && tx <= CHI0 && ty >= CLO1 && ty <= CHI1 && tz >= CLO2 && tz <= CHI2;\n  }\n";
-  o << "  static __device__ __forceinline__ i64 uidx(const int f, const int p0, const int p1, const int p2) {\n"
+  o << "  static __device__ __forceinline__ ix uidx(const int f, const int p0, const int p1, const int p2) {\n"
        "    switch (f) {\n";
   for (int f = 0; f < d_->nfields; ++f) {
     const long long bx = g_.hi[f][0] - g_.lo[f][0] + 1, by = g_.hi[f][1] - g_.lo[f][1] + 1;
-    o << "    case " << f << ": return (i64)" << g_.fofs[f] << " + (p0 - " << g_.lo[f][0] << ") + (i64)" << bx
-      << " * ((p1 - " << g_.lo[f][1] << ") + (i64)" << by << " * (p2 - " << g_.lo[f][2] << "));\n";
+    o << "    case " << f << ": return (ix)" << g_.fofs[f] << " + (p0 - " << g_.lo[f][0] << ") + (ix)" << bx
+      << " * ((p1 - " << g_.lo[f][1] << ") + (ix)" << by << " * (p2 - " << g_.lo[f][2] << "));\n";
   }
   o << "    default: return 0;\n    }\n  }\n";
   o << "  static __device__ __forceinline__ bool inbox(const int f, const int I0, const int I1, const int I2) {\n"
@@ -1099,14 +1136,17 @@ std::string Gen::geo_common() {
       << ";\n";
   o << "    default: return false;\n    }\n  }\n";
   // full-grid index of an (unwrapped) position, -1 outside the grid
-  o << "  static __device__ __forceinline__ i64 gidx(int p0, int p1, int p2) {\n";
+  o << "  static __device__ __forceinline__ ix gidx(int p0, int p1, int p2) {\n";
   for (int a = 0; a < 3; ++a)
     if (g_.periodic[a]) o << "    p" << a << " = mol_wrap(mol_wrap(p" << a << ", " << g_.n[a] << "), " << g_.n[a] << ");\n";
   o << "    if (p0 < 1 || p0 > " << g_.n[0] << " || p1 < 1 || p1 > " << g_.n[1] << " || p2 < 1 || p2 > " << g_.n[2]
     << ") return -1;\n"
-    << "    return (i64)(p0 - 1) + (i64)" << g_.n[0] << " * ((p1 - 1) + (i64)" << g_.n[1] << " * (p2 - 1));\n  }\n";
+    << "    return gidx_in(p0, p1, p2);\n  }\n";
+  // full-grid index of a position inside the grid (no wrap, no check)
+  o << "  static __device__ __forceinline__ ix gidx_in(const int p0, const int p1, const int p2) {\n"
+    << "    return (ix)(p0 - 1) + (ix)" << g_.n[0] << " * ((p1 - 1) + (ix)" << g_.n[1] << " * (p2 - 1));\n  }\n";
   o << "  static __device__ __forceinline__ T stage_aux(const int a, int p0, int p1, int p2, const JPtrs& jp) {\n"
-       "    const i64 gi = gidx(p0, p1, p2);\n"
+       "    const ix gi = gidx(p0, p1, p2);\n"
        "    return gi < 0 ? T(0) : jp.aux[a][gi];\n  }\n";
   return o.str();
 }
@@ -1171,29 +1211,59 @@ std::string Gen::geo_variant(const std::string& name, const int* halo, const Usa
     c("SLO" + istr(a), slo);
     c("SHI" + istr(a), shi);
   }
-  // staging of one tile + halo position (p: unwrapped full-grid index)
-  o << "  static __device__ __forceinline__ void stage_pos(T* L, const int q, int p0, int p1, int p2, const bool simple,\n"
-       "      const double t, const T* __restrict__ P, const T* __restrict__ u, const JPtrs& jp) {\n"
-       "    (void)t; (void)P; (void)jp; (void)u; (void)simple;\n";
-  bool any_field = false;
-  for (int f = 0; f < d_->nfields; ++f) any_field = any_field || pg_.slot[f] >= 0;
-  if (any_field) {
-    o << "    if (simple) {\n      int w0 = p0, w1 = p1, w2 = p2;\n";
-    for (int a = 0; a < 3; ++a)
-      if (g_.periodic[a])
-        o << "      w" << a << " = mol_wrap(mol_wrap(w" << a << ", " << g_.n[a] << "), " << g_.n[a] << ");\n";
-    for (int f = 0; f < d_->nfields; ++f)
-      if (pg_.slot[f] >= 0) o << "      L[" << (long long)pg_.slot[f] * pg_.vol << " + q] = u[uidx(" << f << ", w0, w1, w2)];\n";
-    o << "    } else {\n";
-    for (int f = 0; f < d_->nfields; ++f)
-      if (pg_.slot[f] >= 0)
-        o << "      L[" << (long long)pg_.slot[f] * pg_.vol << " + q] = stage(" << f << ", p0, p1, p2, t, P, u, jp);\n";
-    o << "    }\n";
+  // tiles staged without the periodic wrap: tile + halo inside every variable's box unwrapped
+  for (int a = 0; a < 3; ++a) {
+    int lo = -(1 << 30), hi = 1 << 30;
+    for (int f = 0; f < d_->nfields; ++f) {
+      lo = std::max(lo, g_.lo[f][a]);
+      hi = std::min(hi, g_.hi[f][a]);
+    }
+    int flo = g_.ntile[a], fhi = -1;
+    for (int t = 0; t < g_.ntile[a]; ++t) {
+      const int o0 = g_.ulo[a] + t * g_.tile[a];
+      if (o0 - halo[a] >= lo && o0 + g_.tile[a] - 1 + halo[a] <= hi) {
+        flo = std::min(flo, t);
+        fhi = std::max(fhi, t);
+      }
+    }
+    c("FLO" + istr(a), flo);
+    c("FHI" + istr(a), fhi);
   }
+  // values of the staged arrays at one tile + halo position (p: unwrapped full-grid index),
+  // v[slot]; mode 0: plain loads, 1: plain loads after the periodic wrap, 2: through the ghost
+  // closures
+  o << "  static __device__ __forceinline__ void stage_vals(T* v, int p0, int p1, int p2, const int mode,\n"
+       "      const double t, const T* __restrict__ P, const T* __restrict__ u, const JPtrs& jp) {\n"
+       "    (void)t; (void)P; (void)jp; (void)u; (void)v;\n";
+  o << "    if (mode == 2) {\n";
+  for (int f = 0; f < d_->nfields; ++f)
+    if (pg_.slot[f] >= 0) o << "      v[" << pg_.slot[f] << "] = stage(" << f << ", p0, p1, p2, t, P, u, jp);\n";
   for (int a = 0; a < d_->naux; ++a)
     if (pg_.slot[d_->nfields + a] >= 0)
-      o << "    L[" << (long long)pg_.slot[d_->nfields + a] * pg_.vol << " + q] = stage_aux(" << a << ", p0, p1, p2, jp);\n";
+      o << "      v[" << pg_.slot[d_->nfields + a] << "] = stage_aux(" << a << ", p0, p1, p2, jp);\n";
+  o << "      return;\n    }\n";
+  bool any_per = false;
+  for (int a = 0; a < 3; ++a) any_per = any_per || g_.periodic[a];
+  if (any_per) {
+    o << "    if (mode == 1) {\n";
+    for (int a = 0; a < 3; ++a)
+      if (g_.periodic[a])
+        o << "      p" << a << " = mol_wrap(mol_wrap(p" << a << ", " << g_.n[a] << "), " << g_.n[a] << ");\n";
+    o << "    }\n";
+  }
+  for (int f = 0; f < d_->nfields; ++f)
+    if (pg_.slot[f] >= 0) o << "    v[" << pg_.slot[f] << "] = u[uidx(" << f << ", p0, p1, p2)];\n";
+  for (int a = 0; a < d_->naux; ++a)
+    if (pg_.slot[d_->nfields + a] >= 0)
+      o << "    v[" << pg_.slot[d_->nfields + a] << "] = jp.aux[" << a << "][gidx_in(p0, p1, p2)];\n";
   o << "  }\n";
+  o << "  static __device__ __forceinline__ void stage_pos(T* L, const int q, int p0, int p1, int p2, const int mode,\n"
+       "      const double t, const T* __restrict__ P, const T* __restrict__ u, const JPtrs& jp) {\n"
+       "    T v[NST > 0 ? NST : 1];\n"
+       "    stage_vals(v, p0, p1, p2, mode, t, P, u, jp);\n"
+       "#pragma unroll\n"
+       "    for (int s = 0; s < NST; ++s) L[s * VOL + q] = v[s];\n"
+       "  }\n";
   if (!vjp) {
     o << "  static __device__ __forceinline__ void node(const T* __restrict__ L, const int b, const int I0, const int I1,\n"
          "      const int I2, const double t, const T* __restrict__ P, const T* __restrict__ u, const JPtrs& jp, T* out) {\n"
@@ -1256,7 +1326,7 @@ std::string Gen::source_kernel(bool f64, bool vjp) {
   if (!ok_) return "";
   const Usage us = usage();
   o << "struct GeoBase {\n" << geo_common() << stage << scatter << "};\n\n";
-  int lds = 1, ldsf = 1, ldsa = 1;
+  int lds = 1, ldsf = 1, ldsa = 1, npf = 1;
   auto sizes = [&](const PrintGeo& pg) {
     int ns = 0, nf = 0, na = 0;
     for (size_t k = 0; k < pg.slot.size(); ++k)
@@ -1264,6 +1334,7 @@ std::string Gen::source_kernel(bool f64, bool vjp) {
     lds = std::max(lds, ns * pg.vol);
     ldsf = std::max(ldsf, nf * pg.vol);
     ldsa = std::max(ldsa, na * pg.vol);
+    npf = std::max(npf, pg.vol);
   };
   if (g_.ncore > 0) {
     o << geo_variant("GeoC", us.core_reach, us, true, r_core, out_core, vjp);
@@ -1271,12 +1342,27 @@ std::string Gen::source_kernel(bool f64, bool vjp) {
   }
   o << geo_variant("GeoB", us.reach, us, false, r_bound, out_bound, vjp);
   sizes(pg_);
-  o << "constexpr int LDSN = " << lds << ", LDSF = " << ldsf << ", LDSA = " << ldsa << ";\n\n";
+  o << "constexpr int LDSN = " << lds << ", LDSF = " << ldsf << ", LDSA = " << ldsa << ";\n";
+  // staging registers per thread of the persistent loop: positions of the larger tile + halo
+  o << "constexpr int NPF = (" << npf << " + GeoBase::NTHR - 1) / GeoBase::NTHR;\n\n";
   const std::string decode =
       "  int b = mol_xcd_tile<GeoBase::NBLK>((int)blockIdx.x);\n"
       "  const int tx = b % GeoBase::NT0;\n  b /= GeoBase::NT0;\n"
       "  const int ty = b % GeoBase::NT1;\n  const int tz = b / GeoBase::NT1;\n";
-  if (!vjp) {
+  int nst = 0;
+  for (bool b : us.staged) nst += b ? 1 : 0;
+  const bool persistent = nst > 0 && ((npf + 255) / 256) * nst <= 16;
+  if (!vjp && persistent) {
+    // persistent: gridDim.x work-groups walk the tiles, the next tile's staging loads in flight
+    // during the current tile's node phase (few staging values per thread)
+    o << "extern \"C\" __global__ void __launch_bounds__(256) mol_rhs_tiles(const T* __restrict__ u, Epi ep, JPtrs jp) {\n"
+         "  __shared__ T L[LDSN];\n"
+         "  const double t = (ep.tnow ? ep.tnow[0] : 0.0) + ep.tofs;\n"
+         "  mol_advance_clock(ep);\n"
+      << "  mol_tiles_persistent<" << (g_.ncore > 0 ? "GeoC" : "GeoB") << ", GeoB, " << (g_.ncore > 0 ? "true" : "false")
+      << ", NPF>(L, u, ep, jp, t);\n}\n";
+  } else if (!vjp) {
+    // one work-group per tile
     o << "extern \"C\" __global__ void __launch_bounds__(256) mol_rhs_tiles(const T* __restrict__ u, Epi ep, JPtrs jp) {\n"
          "  __shared__ T L[LDSN];\n" << decode
       << "  const double t = (ep.tnow ? ep.tnow[0] : 0.0) + ep.tofs;\n"

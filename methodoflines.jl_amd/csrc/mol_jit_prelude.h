@@ -43,7 +43,8 @@ struct JPtrs {
 //   mode 0:  out = f
 //   mode 1:  out = ca*A + cb*B + cf*f ;  acc = acc_keep*acc + acc_f*f  (if acc)
 //   mode 2:  compensated update  out = A (+) [cb*(B - subA*A) + cf*f]  with Kahan term comp
-__device__ __forceinline__ void mol_epilogue(const Epi& ep, i64 idx, T f) {
+template <class I>
+__device__ __forceinline__ void mol_epilogue(const Epi& ep, const I idx, T f) {
   if (ep.mode == 0) {
     ep.out[idx] = f;
     return;
@@ -198,45 +199,196 @@ __device__ __forceinline__ int mol_xcd_tile(int b) {
   return x * q + (x < rem ? x : rem) + i;
 }
 
+// Coordinates (x, y, z) of position q of an EX x EY x ... box, advanced by a fixed stride
+// without divisions (the staging loops visit q, q + S, q + 2S, ...).
+template <int EX, int EY, int S>
+struct MolWalk {
+  int x, y, z;
+  __device__ __forceinline__ explicit MolWalk(const int q) {
+    z = q / (EX * EY);
+    const int r = q - z * (EX * EY);
+    y = r / EX;
+    x = r - y * EX;
+  }
+  __device__ __forceinline__ void step() {
+    constexpr int DZ = S / (EX * EY), R = S % (EX * EY), DY = R / EX, DX = R % EX;
+    x += DX;
+    y += DY;
+    z += DZ;
+    if (x >= EX) { x -= EX; ++y; }
+    if (y >= EY) { y -= EY; ++z; }
+  }
+};
+
+// Staging mode of a tile in variant G: 0 plain loads, 1 plain loads after the periodic wrap,
+// 2 through the ghost-node closures (tile + halo leaves a variable's unknown box).
+template <class G>
+__device__ __forceinline__ int mol_stage_mode(const int tx, const int ty, const int tz) {
+  if (tx >= G::FLO0 && tx <= G::FHI0 && ty >= G::FLO1 && ty <= G::FHI1 && tz >= G::FLO2 && tz <= G::FHI2) return 0;
+  if (tx >= G::SLO0 && tx <= G::SHI0 && ty >= G::SLO1 && ty <= G::SHI1 && tz >= G::SLO2 && tz <= G::SHI2) return 1;
+  return 2;
+}
+
+// Tile + halo of the staged arrays into LDS.
+template <class G>
+__device__ __forceinline__ void mol_stage_tile(T* L, const int o0, const int o1, const int o2, const int mode,
+                                               const double t, const T* __restrict__ P, const T* __restrict__ u,
+                                               const JPtrs& jp) {
+  MolWalk<G::EX, G::EY, G::NTHR> w((int)threadIdx.x);
+  for (int q = threadIdx.x; q < G::VOL; q += G::NTHR, w.step())
+    G::stage_pos(L, q, o0 + w.x - G::HX, o1 + w.y - G::HY, o2 + w.z - G::HZ, mode, t, P, u, jp);
+}
+
+// Node phase of one staged tile in variant G: each thread runs G::RB consecutive nodes along
+// axis G::BA; their node functions see the same LDS tile at offsets that differ by a constant,
+// so shared taps and subexpressions are loaded and computed once.  Nodes of a ragged tile past
+// the unknown box run at the last node of the box (every load stays inside the tile + halo
+// and the arrays) and store nothing.
+template <class G>
+__device__ __forceinline__ void mol_tile_nodes(const T* L, const int tx, const int ty, const int tz,
+                                               const T* __restrict__ u, const Epi& ep, const JPtrs& jp,
+                                               const double t) {
+  const int o0 = G::ULO0 + tx * G::TX, o1 = G::ULO1 + ty * G::TY, o2 = G::ULO2 + tz * G::TZ;
+  constexpr int RB = G::RB;
+  constexpr int PX = G::TX, PY = G::BA == 1 ? G::TY / RB : G::TY, PZ = G::BA == 2 ? G::TZ / RB : G::TZ;
+  constexpr int PXY = PX * PY, PV = PXY * PZ;
+  constexpr int STEP = G::BA == 1 ? G::EX : (G::BA == 2 ? G::EX * G::EY : 1);
+#pragma unroll
+  for (int r = 0; r < (PV + G::NTHR - 1) / G::NTHR; ++r) {
+    const int q = (int)threadIdx.x + r * G::NTHR;
+    if (PV % G::NTHR != 0 && q >= PV) break;
+    const int lz0 = q / PXY;
+    const int rr = q - lz0 * PXY;
+    const int ly0 = rr / PX;
+    const int lx = rr - ly0 * PX;
+    const int ly = G::BA == 1 ? ly0 * RB : ly0, lz = G::BA == 2 ? lz0 * RB : lz0;
+    const int base = (lx + G::HX) + G::EX * (ly + G::HY) + G::EX * G::EY * (lz + G::HZ);
+    T out[RB][G::NF];
+    int J[RB][3];
+#pragma unroll
+    for (int k = 0; k < RB; ++k) {
+      J[k][0] = o0 + lx;
+      J[k][1] = o1 + ly + (G::BA == 1 ? k : 0);
+      J[k][2] = o2 + lz + (G::BA == 2 ? k : 0);
+      G::node(L, base + k * STEP, min(J[k][0], (int)G::UHI0), min(J[k][1], (int)G::UHI1), min(J[k][2], (int)G::UHI2), t,
+              ep.p, u, jp, out[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < RB; ++k) {
+      if (J[k][0] > G::UHI0 || J[k][1] > G::UHI1 || J[k][2] > G::UHI2) continue;
+#pragma unroll
+      for (int f = 0; f < G::NF; ++f)
+        if (G::inbox(f, J[k][0], J[k][1], J[k][2])) mol_epilogue(ep, G::uidx(f, J[k][0], J[k][1], J[k][2]), out[k][f]);
+    }
+  }
+}
+
+// Loads of the tile + halo of variant G into registers R[i][slot] (position threadIdx.x + i *
+// NTHR): issued for the next tile while the current one runs its node phase.
+template <class G, int NPF, int NSR>
+__device__ __forceinline__ void mol_fetch(T (&R)[NPF][NSR], const int tx, const int ty, const int tz, const double t,
+                                          const T* __restrict__ P, const T* __restrict__ u, const JPtrs& jp) {
+  const int o0 = G::ULO0 + tx * G::TX, o1 = G::ULO1 + ty * G::TY, o2 = G::ULO2 + tz * G::TZ;
+  const int mode = mol_stage_mode<G>(tx, ty, tz);
+  MolWalk<G::EX, G::EY, G::NTHR> w((int)threadIdx.x);
+#pragma unroll
+  for (int i = 0; i < NPF; ++i) {
+    if (i * G::NTHR >= G::VOL) break;
+    if ((int)threadIdx.x + i * G::NTHR < G::VOL)
+      G::stage_vals(R[i], o0 + w.x - G::HX, o1 + w.y - G::HY, o2 + w.z - G::HZ, mode, t, P, u, jp);
+    w.step();
+  }
+}
+
+template <class G, int NPF, int NSR>
+__device__ __forceinline__ void mol_put(T* L, const T (&R)[NPF][NSR]) {
+#pragma unroll
+  for (int i = 0; i < NPF; ++i) {
+    const int q = (int)threadIdx.x + i * G::NTHR;
+    if (i * G::NTHR >= G::VOL) break;
+    if (q < G::VOL)
+#pragma unroll
+      for (int s = 0; s < G::NST; ++s) L[s * G::VOL + q] = R[i][s];
+  }
+}
+
 // One tile of the union unknown box, in the tile variant G (interior-rows core variant or the
-// boundary variant, each with its own halo).  Tiles whose tile + halo lies inside every
-// variable's unknown box (after the periodic wrap) stage with plain loads; the others stage
-// each position through Geo::stage (ghost-node closures, zero outside the grid and at
-// corners).  Arrays read only at the node are not staged: the node function loads them.
+// boundary variant, each with its own halo): stage (mol_stage_tile), then the node phase.
 template <class G>
 __device__ __forceinline__ void mol_full_tile(T* L, const int tx, const int ty, const int tz, const T* __restrict__ u,
                                               const Epi& ep, const JPtrs& jp, const double t) {
-  const int o0 = G::ULO0 + tx * G::TX, o1 = G::ULO1 + ty * G::TY, o2 = G::ULO2 + tz * G::TZ;
-  const bool simple = tx >= G::SLO0 && tx <= G::SHI0 && ty >= G::SLO1 && ty <= G::SHI1 &&
-                      tz >= G::SLO2 && tz <= G::SHI2;
-  constexpr int EXY = G::EX * G::EY;
   if (G::NST > 0) {
-    for (int q = threadIdx.x; q < G::VOL; q += G::NTHR) {
-      const int lz = q / EXY;
-      const int r = q - lz * EXY;
-      const int ly = r / G::EX;
-      const int lx = r - ly * G::EX;
-      G::stage_pos(L, q, o0 + lx - G::HX, o1 + ly - G::HY, o2 + lz - G::HZ, simple, t, ep.p, u, jp);
-    }
+    const int o0 = G::ULO0 + tx * G::TX, o1 = G::ULO1 + ty * G::TY, o2 = G::ULO2 + tz * G::TZ;
+    mol_stage_tile<G>(L, o0, o1, o2, mol_stage_mode<G>(tx, ty, tz), t, ep.p, u, jp);
     __syncthreads();
   }
-  constexpr int TXY = G::TX * G::TY, TVOL = TXY * G::TZ;
-#pragma unroll
-  for (int r = 0; r < (TVOL + G::NTHR - 1) / G::NTHR; ++r) {
-    const int q = (int)threadIdx.x + r * G::NTHR;
-    if (TVOL % G::NTHR != 0 && q >= TVOL) break;
-    const int lz = q / TXY;
-    const int rr = q - lz * TXY;
-    const int ly = rr / G::TX;
-    const int lx = rr - ly * G::TX;
-    const int I0 = o0 + lx, I1 = o1 + ly, I2 = o2 + lz;
-    if (I0 > G::UHI0 || I1 > G::UHI1 || I2 > G::UHI2) continue;
-    const int base = (lx + G::HX) + G::EX * (ly + G::HY) + EXY * (lz + G::HZ);
-    T out[G::NF];
-    G::node(L, base, I0, I1, I2, t, ep.p, u, jp, out);
-#pragma unroll
-    for (int f = 0; f < G::NF; ++f)
-      if (G::inbox(f, I0, I1, I2)) mol_epilogue(ep, G::uidx(f, I0, I1, I2), out[f]);
+  mol_tile_nodes<G>(L, tx, ty, tz, u, ep, jp, t);
+}
+
+// Persistent loop over the tiles of a launch of gridDim.x work-groups (a multiple of 8 when
+// smaller than the tile count, so that tile k + gridDim.x stays on the XCD of tile k): the
+// staging loads of the next tile are in flight while the current tile runs its node phase.
+// GC: interior-rows variant (used when HASC and the tile is a core tile), GB: boundary variant;
+// both stage the same arrays.
+template <class GC, class GB, bool HASC, int NPF>
+__device__ __forceinline__ void mol_tiles_persistent(T* L, const T* __restrict__ u, const Epi& ep, const JPtrs& jp,
+                                                     const double t) {
+  constexpr int NSR = GB::NST > 0 ? GB::NST : 1;
+  constexpr int NBLK = GB::NBLK;
+  int k = (int)blockIdx.x;
+  if (k >= NBLK) return;
+  auto decode = [](const int lin, int& x, int& y, int& z) {
+    int b = mol_xcd_tile<NBLK>(lin);
+    x = b % GB::NT0;
+    b /= GB::NT0;
+    y = b % GB::NT1;
+    z = b / GB::NT1;
+  };
+  // more than 16 staging values per thread would cost occupancy: stage each tile in turn
+  if (NPF * NSR > 16) {
+    for (; k < NBLK; k += (int)gridDim.x) {
+      int tx, ty, tz;
+      decode(k, tx, ty, tz);
+      if (HASC && GB::is_core(tx, ty, tz)) mol_full_tile<GC>(L, tx, ty, tz, u, ep, jp, t);
+      else mol_full_tile<GB>(L, tx, ty, tz, u, ep, jp, t);
+      if (GB::NST > 0 && k + (int)gridDim.x < NBLK) __syncthreads();
+    }
+    return;
+  }
+  T R[NPF][NSR];
+  int tx, ty, tz;
+  decode(k, tx, ty, tz);
+  bool core = HASC && GB::is_core(tx, ty, tz);
+  if (GB::NST > 0) {
+    if (HASC && core) mol_fetch<GC, NPF, NSR>(R, tx, ty, tz, t, ep.p, u, jp);
+    else mol_fetch<GB, NPF, NSR>(R, tx, ty, tz, t, ep.p, u, jp);
+  }
+  while (true) {
+    if (GB::NST > 0) {
+      if (HASC && core) mol_put<GC, NPF, NSR>(L, R);
+      else mol_put<GB, NPF, NSR>(L, R);
+      __syncthreads();
+    }
+    const int kn = k + (int)gridDim.x;
+    int nx = 0, ny = 0, nz = 0;
+    bool ncore = false;
+    if (kn < NBLK) {
+      decode(kn, nx, ny, nz);
+      ncore = HASC && GB::is_core(nx, ny, nz);
+      if (GB::NST > 0) {
+        if (HASC && ncore) mol_fetch<GC, NPF, NSR>(R, nx, ny, nz, t, ep.p, u, jp);
+        else mol_fetch<GB, NPF, NSR>(R, nx, ny, nz, t, ep.p, u, jp);
+      }
+    }
+    if (HASC && core) mol_tile_nodes<GC>(L, tx, ty, tz, u, ep, jp, t);
+    else mol_tile_nodes<GB>(L, tx, ty, tz, u, ep, jp, t);
+    if (kn >= NBLK) break;
+    if (GB::NST > 0) __syncthreads();
+    k = kn;
+    tx = nx;
+    ty = ny;
+    tz = nz;
+    core = ncore;
   }
 }
 
@@ -263,16 +415,14 @@ template <class G>
 __device__ __forceinline__ void mol_vjp_tile(T* L, T* Ab, double* Aa, const int tx, const int ty, const int tz,
                                              const VArgs& va, const JPtrs& jp, const double t) {
   const int o0 = G::ULO0 + tx * G::TX, o1 = G::ULO1 + ty * G::TY, o2 = G::ULO2 + tz * G::TZ;
-  const bool simple = tx >= G::SLO0 && tx <= G::SHI0 && ty >= G::SLO1 && ty <= G::SHI1 &&
-                      tz >= G::SLO2 && tz <= G::SHI2;
   constexpr int EXY = G::EX * G::EY;
-  for (int q = threadIdx.x; q < G::VOL; q += G::NTHR) {
-    const int lz = q / EXY;
-    const int r = q - lz * EXY;
-    const int ly = r / G::EX;
-    const int lx = r - ly * G::EX;
-    G::stage_pos(L, q, o0 + lx - G::HX, o1 + ly - G::HY, o2 + lz - G::HZ, simple, t, va.p, va.u, jp);
-    G::zero_pos(Ab, Aa, q);
+  {
+    const int mode = mol_stage_mode<G>(tx, ty, tz);
+    MolWalk<G::EX, G::EY, G::NTHR> w((int)threadIdx.x);
+    for (int q = threadIdx.x; q < G::VOL; q += G::NTHR, w.step()) {
+      G::stage_pos(L, q, o0 + w.x - G::HX, o1 + w.y - G::HY, o2 + w.z - G::HZ, mode, t, va.p, va.u, jp);
+      G::zero_pos(Ab, Aa, q);
+    }
   }
   __syncthreads();
   constexpr int TXY = G::TX * G::TY, TVOL = TXY * G::TZ;
@@ -289,7 +439,7 @@ __device__ __forceinline__ void mol_vjp_tile(T* L, T* Ab, double* Aa, const int 
     for (int f = 0; f < G::NF; ++f) {
       T s = T(0);
       if (G::inbox(f, I0, I1, I2)) {
-        const i64 k = G::uidx(f, I0, I1, I2);
+        const auto k = G::uidx(f, I0, I1, I2);
         s = (T)va.c1 * va.s1[k];
         if (va.s2) s += (T)va.c2 * va.s2[k];
       }
@@ -301,13 +451,9 @@ __device__ __forceinline__ void mol_vjp_tile(T* L, T* Ab, double* Aa, const int 
     G::vjp(L, Ab, Aa, base, I0, I1, I2, t, va.p, va.u, jp, lam, va.out, va.g_aux);
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < G::VOL; q += G::NTHR) {
-    const int lz = q / EXY;
-    const int r = q - lz * EXY;
-    const int ly = r / G::EX;
-    const int lx = r - ly * G::EX;
-    G::flush_pos(Ab, Aa, q, o0 + lx - G::HX, o1 + ly - G::HY, o2 + lz - G::HZ, va.out, va.g_aux);
-  }
+  MolWalk<G::EX, G::EY, G::NTHR> w((int)threadIdx.x);
+  for (int q = threadIdx.x; q < G::VOL; q += G::NTHR, w.step())
+    G::flush_pos(Ab, Aa, q, o0 + w.x - G::HX, o1 + w.y - G::HY, o2 + w.z - G::HZ, va.out, va.g_aux);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -47,12 +47,15 @@ def _function():
     return _RK4Adjoint
 
 
-def rk4_solve(prob: ODEProblem, u0, t0: float, dt: float, nsteps: int, ckpt_every: int = 20,
+def rk4_solve(prob: ODEProblem, u0, t0: float, dt: float, nsteps: int, ckpt_every: Optional[int] = 20,
               grid_fields: Optional[Dict[str, object]] = None):
     """Differentiable RK4 solve: returns u(t0 + nsteps*dt).  ``grid_fields`` maps grid-field
     names to contiguous CUDA tensors (flat, first axis fastest) that are bound for the solve and
-    receive gradients.  The problem keeps one checkpoint set: backward must follow its own
-    forward before the next solve of the same problem."""
+    receive gradients.  ``ckpt_every``: steps between checkpoints, 0 to keep every stage state,
+    None to choose by free device memory.  The problem keeps one checkpoint set: backward must
+    follow its own forward before the next solve of the same problem."""
     names = tuple(grid_fields.keys()) if grid_fields else ()
     fields = tuple(grid_fields[k] for k in names) if grid_fields else ()
+    if ckpt_every is None:
+        ckpt_every = prob.checkpoint_interval(nsteps)
     return _function().apply(u0, prob, float(t0), float(dt), int(nsteps), int(ckpt_every), names, *fields)

@@ -243,12 +243,27 @@ class ODEProblem:
                              % (ng * len(self.lowered.aux)))
         return C.c_void_p(g_aux.data_ptr())
 
-    def forward_checkpointed(self, u, t0: float, dt: float, nsteps: int, ckpt_every: int, p=None) -> None:
+    def checkpoint_interval(self, nsteps: int, budget: float = 0.5) -> int:
+        """``ckpt_every`` for a run of ``nsteps`` RK4 steps: 0 (keep every stage state, no
+        recomputation in the adjoint) when 4 * nsteps state vectors fit in ``budget`` of the free
+        device memory, else about sqrt(nsteps) steps between checkpoints."""
+        torch = _torch()
+        free, _ = torch.cuda.mem_get_info()
+        need = 4 * int(nsteps) * self.lowered.n_unknowns * (8 if self.lowered.dtype == "f64" else 4)
+        if need <= budget * free:
+            return 0
+        return max(1, int(round(float(nsteps) ** 0.5)))
+
+    def forward_checkpointed(self, u, t0: float, dt: float, nsteps: int, ckpt_every: Optional[int] = None,
+                             p=None) -> None:
         """RK4 from t0 (u advanced in place) keeping a checkpoint every ``ckpt_every`` steps for
-        a later :meth:`adjoint`."""
+        a later :meth:`adjoint`; ``ckpt_every=0`` keeps every stage state instead (the adjoint
+        recomputes nothing), ``None`` picks with :meth:`checkpoint_interval`."""
         torch = _torch()
         h = self.handle
         self._check_tensor(u)
+        if ckpt_every is None:
+            ckpt_every = self.checkpoint_interval(nsteps)
         if p is None:
             p = self.device_params()
         stream = torch.cuda.current_stream().cuda_stream

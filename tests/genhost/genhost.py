@@ -83,12 +83,14 @@ def _ptrs(low, aux):
     return jp, keep
 
 
-def rhs(low, u, t=0.0, p=None, aux=None, tile=None):
-    """Generated forward kernel on the CPU: du = f(u, p, t) (FP64 systems)."""
+def rhs(low, u, t=0.0, p=None, aux=None, tile=None, grid=None):
+    """Generated forward kernel on the CPU: du = f(u, p, t) (FP64 systems).  ``grid``: number of
+    work-groups of the persistent launch (default one per tile)."""
     import molgpu
     src = molgpu.codegen(low, tile=tile)
     entry = """
 extern "C" __attribute__((visibility("default"))) void run(const T* u, Epi ep, JPtrs jp, int nblk) {
+  gridDim.x = nblk;
   for (int b = 0; b < nblk; ++b) { blockIdx.x = b; mol_rhs_tiles(u, ep, jp); }
 }
 """
@@ -103,7 +105,7 @@ extern "C" __attribute__((visibility("default"))) void run(const T* u, Epi ep, J
     ep.p = pv.ctypes.data if pv.size else None
     jp, keep = _ptrs(low, aux or {})
     lib.run.argtypes = [C.c_void_p, Epi, JPtrs, C.c_int]
-    lib.run(u.ctypes.data, ep, jp, _nblocks(src))
+    lib.run(u.ctypes.data, ep, jp, grid if grid is not None else _nblocks(src))
     return du
 
 
@@ -113,6 +115,7 @@ def vjp(low, u, lam, t=0.0, p=None, aux=None, tile=None):
     src = molgpu.codegen(low, tile=tile, vjp=True)
     entry = """
 extern "C" __attribute__((visibility("default"))) void run(VArgs va, JPtrs jp, int nblk) {
+  gridDim.x = nblk;
   for (int b = 0; b < nblk; ++b) { blockIdx.x = b; mol_vjp_tiles(va, jp); }
 }
 """

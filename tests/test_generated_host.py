@@ -179,3 +179,29 @@ def test_generated_vjp_grid_field_gradients_match_oracle_differences():
         (l0, l1, l2), (h0, h1, h2) = low.lo[0], low.hi[0]
         box[l0 - 1:h0, l1 - 1:h1, l2 - 1:h2] = lam.reshape((h0 - l0 + 1, h1 - l1 + 1, h2 - l2 + 1), order="F")
         np.testing.assert_allclose(gs, box.reshape(-1, order="F"), rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("rb", [2, 4])
+@pytest.mark.parametrize("name", ["brusselator", "burgers_weno", "diffreact_0110", "nonlindiff3d", "mixed_p4"])
+def test_generated_forward_kernel_register_blocked(name, rb, monkeypatch):
+    """Several nodes per thread along the slowest axis (MOL_RB forces it on small tiles),
+    including tiles that overhang the unknown box, against the oracle."""
+    monkeypatch.setenv("MOL_RB", str(rb))
+    low, u, t, aux, ref, scale = CASES[name]()
+    tiles = {2: [(8, 8, 1), (16, 4, 1)], 3: [(8, 4, 4), (4, 4, 8)]}[low.ndim]
+    for tile in tiles:
+        src = __import__("molgpu").codegen(low, tile=tile)
+        assert "static constexpr int RB = %d;" % rb in src
+        got = genhost.rhs(low, u, t=t, aux=aux, tile=tile)
+        assert_parity(got, ref, scale, what="%s tile %s RB %d" % (name, tile, rb))
+
+
+@pytest.mark.parametrize("grid", [8, 3])
+@pytest.mark.parametrize("name", ["brusselator", "diffreact_0110", "nonlindiff3d", "heat_robin_p4"])
+def test_generated_forward_kernel_persistent_launch(name, grid):
+    """Fewer work-groups than tiles: each walks several tiles (core and boundary variants in
+    turn) with the next tile's staging prefetched, against the oracle."""
+    low, u, t, aux, ref, scale = CASES[name]()
+    tile = {1: (16, 1, 1), 2: (8, 8, 1), 3: (4, 4, 4)}[low.ndim]
+    got = genhost.rhs(low, u, t=t, aux=aux, tile=tile, grid=grid)
+    assert_parity(got, ref, scale, what="%s grid %d" % (name, grid))

@@ -234,3 +234,29 @@ def test_autograd_function_returns_the_adjoint_gradients():
     a = prob.lowered.aux.index("d0")
     gd = g[a * n ** 3:(a + 1) * n ** 3]
     assert torch.allclose(d0_t.grad, gd, rtol=1e-12, atol=1e-14 * float(g.abs().max()))
+
+
+def test_adjoint_with_stored_stages_equals_checkpointed_adjoint():
+    """ckpt_every=0 keeps every stage state in the forward run: the same forward state, and the
+    adjoint (no recomputation) equal to the checkpointed adjoint up to the order of the atomic
+    sums; the automatic choice picks stored stages for a small run."""
+    torch = _torch()
+    n, nsteps = 12, 9
+    prob, d0, src = _nonlin(n, seed=5)
+    dt = 0.1 * (1.0 / (n - 1)) ** 2 / (2.0 * np.max(d0))
+    u0 = 0.3 * _on_box(prob, systems.gaussian_field3d(n, 9))
+    target = _dev(0.1 * _on_box(prob, systems.gaussian_field3d(n, 10)))
+    res = {}
+    for every in (4, 0):
+        print("  stored-stage adjoint, ckpt_every=%d" % every, flush=True)
+        u = _dev(u0)
+        prob.forward_checkpointed(u, 0.0, dt, nsteps, every)
+        lam = (u - target).contiguous()
+        g = torch.zeros(2 * n ** 3, dtype=torch.float64, device="cuda")
+        prob.adjoint(lam, g)
+        res[every] = (u.clone(), lam.clone(), g.clone())
+    assert torch.equal(res[0][0], res[4][0])
+    for k in (1, 2):
+        a, b = res[0][k], res[4][k]
+        assert torch.allclose(a, b, rtol=1e-12, atol=1e-14 * float(b.abs().max())), k
+    assert prob.checkpoint_interval(nsteps) == 0

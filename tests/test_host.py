@@ -284,7 +284,7 @@ def test_generated_kernel_compiles_for_gfx950(name):
     import molgpu
     sys_, disc = _codegen_systems()[name]()
     src = molgpu.codegen(lower(sys_, disc), compile=True)
-    assert 'extern "C" __global__' in src and "mol_full_tile<GeoB>" in src
+    assert 'extern "C" __global__' in src and "struct GeoB" in src
     # the interpreter is not part of the generated kernel
     assert "stack_prog" not in src and "cpool" not in src
 
