@@ -58,7 +58,7 @@ void free_handle(mol_handle* h) {
                   h->d_prog, h->d_coord[0], h->d_coord[1], h->d_coord[2], h->w_y1,
                   h->w_y2, h->w_acc, h->w_comp, h->d_tnow, h->d_ka64, h->d_ka32, h->d_tiles,
                   h->ck_u, h->ck_t, h->seg_u, h->seg_t, h->adj_y[0], h->adj_y[1], h->adj_y[2], h->adj_y[3],
-                  h->d_insts, h->d_wl, h->d_bdata};
+                  h->d_insts, h->d_wl, h->d_bdata, h->vjp_bf, h->vjp_ba};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h->h_u) (void)hipHostFree(h->h_u);
@@ -204,7 +204,7 @@ int validate(const mol_desc* d, std::string& msg) {
 // clamp: shrink the tile to the box (a batch keeps one tile shape for all its instances)
 template <typename T>
 void fill_kargs(mol_handle* h, const mol_desc* d, KArgs<T>& ka, size_t lds_pos = 0, bool clamp = true,
-                bool jit_tiles = false, size_t lds_cap = 60000) {
+                int jit_tiles = 0, size_t lds_cap = 60000) {
   std::memset(&ka, 0, sizeof(ka));
   ka.ndim = d->ndim;
   ka.nfields = d->nfields;
@@ -261,14 +261,17 @@ void fill_kargs(mol_handle* h, const mol_desc* d, KArgs<T>& ka, size_t lds_pos =
   if (lds_pos == 0) lds_pos = (size_t)nload * sizeof(T);
   if (jit_tiles && h->tile_req[0] <= 0 && d->ndim >= 2) {
     // generated kernels run several nodes per thread along the slowest axis: the first
-    // candidate whose tile + halo of the staged arrays fits the LDS budget (about three to four
-    // work-groups of 256 threads per compute unit) and that still gives every compute unit
-    // four tiles; small grids fall back to the candidate with the most tiles
+    // candidate whose tile + halo of the staged arrays fits the LDS budget (forward: about
+    // three to four work-groups of 256 threads per compute unit; reverse mode (jit_tiles 2):
+    // one work-group of up to 1024 threads and 160 KB) and that still gives every compute unit
+    // four tiles (reverse mode: one); small grids fall back to the candidate with the most tiles
     static const int cand2[][3] = {{64, 16, 1}, {32, 16, 1}, {64, 8, 1}, {64, 4, 1}, {32, 8, 1}, {32, 4, 1}};
     static const int cand3[][3] = {{16, 8, 8}, {16, 8, 4}, {16, 4, 8}, {16, 4, 4}, {32, 4, 2}, {16, 4, 2}, {8, 4, 4}};
     const int (*cand)[3] = d->ndim == 2 ? cand2 : cand3;
     const int nc = d->ndim == 2 ? 6 : 7;
-    const size_t budget = d->ndim == 2 ? 40960 : 57344;
+    const bool rev = jit_tiles == 2;
+    const size_t budget = rev ? 163000 : (d->ndim == 2 ? 40960 : 57344);
+    const int64_t enough = rev ? 256 : 1024;
     int best = -1;
     int64_t best_tiles = -1;
     for (int i = 0; i < nc; ++i) {
@@ -280,7 +283,7 @@ void fill_kargs(mol_handle* h, const mol_desc* d, KArgs<T>& ka, size_t lds_pos =
         nt *= (ext + tl[a] - 1) / tl[a];
       }
       if (lds_pos * vol_of(tl) > budget) continue;
-      if (nt >= 1024) {
+      if (nt >= enough) {
         best = i;
         break;
       }
@@ -346,6 +349,7 @@ Launch<T> launch_of(mol_handle* h) {
   L.nbound = h->nbound;
   L.jit = h->jit_fn;
   L.jgrid = h->jit_grid > 0 ? h->jit_grid : h->ka64.ntile[0] * h->ka64.ntile[1] * h->ka64.ntile[2];
+  L.jthr = h->jit_nthr;
   if (h->batch) {
     L.batch = h->batch_fn;
     L.insts = (const BInst*)h->d_insts;
@@ -508,6 +512,7 @@ void build_core(mol_handle* h, const mol_desc* d, const KArgs<T>& ka, KCore<T>& 
 template <typename T>
 JitGeom geom_of(const KArgs<T>& ka, const KCore<T>& kc, int ncore) {
   JitGeom g{};
+  g.nthr = 256;
   for (int a = 0; a < 3; ++a) {
     g.n[a] = ka.n[a];
     g.periodic[a] = ka.periodic[a];
@@ -542,17 +547,28 @@ int sync_kargs(mol_handle* h);
 // Work-groups of a persistent launch over `ntiles` tiles: every one that fits on the device at
 // once (MOL_BPC=k: k per compute unit), a multiple of 8 (tile k + grid stays on the XCD of tile
 // k), at most one per tile.
-int persistent_grid(hipFunction_t fn, int ntiles) {
+int persistent_grid(hipFunction_t fn, int ntiles, int nthr) {
   int dev = 0, ncu = 0, occ = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
     return ntiles;
-  if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, 0) != hipSuccess || occ <= 0) occ = 1;
+  if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, nthr, 0) != hipSuccess || occ <= 0) occ = 1;
   if (const char* e = std::getenv("MOL_BPC")) occ = std::max(1, std::atoi(e));
   int64_t g = (int64_t)occ * ncu;
   if (g >= ntiles) return ntiles;
   g = std::max<int64_t>(8, g / 8 * 8);
   return (int)std::min<int64_t>(g, ntiles);
+}
+
+// Threads per work-group of a generated kernel: 256 for the right-hand side; the reverse-mode
+// kernel runs one thread per tile node (256 to 1024).  MOL_NTHR / MOL_VJP_NTHR set them.
+int kernel_threads(const JitGeom& g, bool vjp) {
+  const char* e = std::getenv(vjp ? "MOL_VJP_NTHR" : "MOL_NTHR");
+  int n = 256;
+  if (vjp) n = g.tile[0] * g.tile[1] * g.tile[2];
+  if (e) n = std::atoi(e);
+  n = (std::max(64, std::min(1024, n)) + 63) / 64 * 64;
+  return vjp ? std::max(256, n) : n;
 }
 
 int setup_paths(mol_handle* h, const mol_desc* d) {
@@ -585,7 +601,10 @@ int setup_paths(mol_handle* h, const mol_desc* d) {
     refill(pos > 0 ? pos : h->esize());
     retiled = true;
     std::string err;
-    const std::string src = jit_source(d, geom_of(h->ka64, h->kc64, n64), err);
+    JitGeom jg = geom_of(h->ka64, h->kc64, n64);
+    jg.nthr = kernel_threads(jg, false);
+    h->jit_nthr = jg.nthr;
+    const std::string src = jit_source(d, jg, err);
     if (src.empty()) h->jit_reason = err;
     else if (jit_kernel(src, "mol_rhs_tiles", h->device, &h->jit_fn, err)) {
       h->jit_fn = nullptr;
@@ -593,7 +612,8 @@ int setup_paths(mol_handle* h, const mol_desc* d) {
     }
     const int ntl = h->ka64.ntile[0] * h->ka64.ntile[1] * h->ka64.ntile[2];
     if (h->jit_fn)
-      h->jit_grid = src.find("mol_tiles_persistent<") != std::string::npos ? persistent_grid(h->jit_fn, ntl) : ntl;
+      h->jit_grid =
+          src.find("mol_tiles_persistent<") != std::string::npos ? persistent_grid(h->jit_fn, ntl, jg.nthr) : ntl;
     if (!h->jit_fn && retiled) refill(0);   // back to the interpreter's tiles
   } else {
     h->jit_reason = "code generation disabled (MOL_JIT=0)";
@@ -781,10 +801,10 @@ JitGeom vjp_geometry(mol_handle* h, const mol_desc* d, int* ntiles) {
         keep[a] = h->tile_req[a];
         h->tile_req[a] = std::max(1, req[a]);
       }
-      fill_kargs<double>(h, d, kv, pos > 0 ? pos : h->esize(), true, false, 160000);
+      fill_kargs<double>(h, d, kv, pos > 0 ? pos : h->esize(), true, 0, 163000);
       for (int a = 0; a < 3; ++a) h->tile_req[a] = keep[a];
     } else if (pos > 0) {
-      fill_kargs<double>(h, d, kv, pos, true, true);
+      fill_kargs<double>(h, d, kv, pos, true, 2, 163000);
     }
   }
   KCore<double> kc;
@@ -796,7 +816,9 @@ JitGeom vjp_geometry(mol_handle* h, const mol_desc* d, int* ntiles) {
   h->core_reason = reason;
   h->core_interp_ok = interp_ok;
   if (ntiles) *ntiles = kv.ntile[0] * kv.ntile[1] * kv.ntile[2];
-  return geom_of(kv, kc, ncore);
+  JitGeom g = geom_of(kv, kc, ncore);
+  g.nthr = kernel_threads(g, true);
+  return g;
 }
 
 int ensure_vjp(mol_handle* h) {
@@ -807,11 +829,37 @@ int ensure_vjp(mol_handle* h) {
   std::string err;
   const std::string src = jit_vjp_source(d, g, err);
   if (src.empty()) return fail(h, MOL_ERR_UNSUPPORTED, "vector-Jacobian product: " + err);
-  if (jit_kernel(src, "mol_vjp_tiles", h->device, &h->vjp_fn, err)) {
+  if (jit_kernel(src, "mol_vjp_tiles", h->device, &h->vjp_fn, err) ||
+      jit_kernel(src, "mol_vjp_gather", h->device, &h->vjp_gather_fn, err)) {
     h->vjp_fn = nullptr;
     return fail(h, MOL_ERR_HIP, "vector-Jacobian product: " + err);
   }
+  // per-tile buffers: VOLX positions per staged array and tile (numbers of the generated code)
+  auto num_after = [&](const std::string& key, size_t from) -> long long {
+    const size_t k = src.find(key, from);
+    return k == std::string::npos ? -1 : std::atoll(src.c_str() + k + key.size());
+  };
+  const size_t gb = src.find("struct GeoB");
+  const long long volx = num_after("constexpr int VOLX = ", 0);
+  const long long nstf = num_after("static constexpr int NSTF = ", gb);
+  const long long nsta = num_after("static constexpr int NSTA = ", gb);
+  if (gb == std::string::npos || volx <= 0 || nstf < 0 || nsta < 0) {
+    h->vjp_fn = nullptr;
+    return fail(h, MOL_ERR_UNSUPPORTED, "vector-Jacobian product: buffer layout not found");
+  }
+  if (h->vjp_bf) (void)hipFree(h->vjp_bf);
+  if (h->vjp_ba) (void)hipFree(h->vjp_ba);
+  h->vjp_bf = h->vjp_ba = nullptr;
+  if (nstf > 0 && hipMalloc(&h->vjp_bf, (size_t)ntiles * volx * nstf * h->esize()) != hipSuccess) {
+    h->vjp_fn = nullptr;
+    return fail(h, MOL_ERR_HIP, "vector-Jacobian product: tile buffer allocation failed");
+  }
+  if (nsta > 0 && hipMalloc(&h->vjp_ba, (size_t)ntiles * volx * nsta * sizeof(double)) != hipSuccess) {
+    h->vjp_fn = nullptr;
+    return fail(h, MOL_ERR_HIP, "vector-Jacobian product: tile buffer allocation failed");
+  }
   h->vjp_tiles = ntiles;
+  h->vjp_nthr = g.nthr;
   h->vjp_aux_ok = true;
   for (int c = 0; c < d->nclosures; ++c)
     if (prog_uses(d, d->closures_h[c].gprog, MOL_OP_AUX)) h->vjp_aux_ok = false;
@@ -825,9 +873,16 @@ int launch_vjp(mol_handle* h, const VArgs<T>& va, hipStream_t s) {
   for (int a = 0; a < 3; ++a) jp.coord[a] = ka.coord[a];
   for (int a = 0; a < MOL_MAX_AUX; ++a) jp.aux[a] = ka.aux[a];
   VArgs<T> v = va;
+  v.bf = (T*)h->vjp_bf;
+  v.ba = (double*)h->vjp_ba;
   void* args[] = {(void*)&v, (void*)&jp};
-  const hipError_t e = hipModuleLaunchKernel(h->vjp_fn, h->vjp_tiles, 1, 1, 256, 1, 1, 0, s, args, nullptr);
+  hipError_t e = hipModuleLaunchKernel(h->vjp_fn, h->vjp_tiles, 1, 1, h->vjp_nthr, 1, 1, 0, s, args, nullptr);
   if (e != hipSuccess) return fail(h, MOL_ERR_HIP, std::string("vector-Jacobian product launch: ") + hipGetErrorString(e));
+  // gather: every unknown, then every grid position of every grid field (when g_aux is given)
+  const int64_t items = h->nunk + (v.g_aux ? (int64_t)h->desc.d.naux * ka.n[0] * ka.n[1] * ka.n[2] : 0);
+  void* gargs[] = {(void*)&v};
+  e = hipModuleLaunchKernel(h->vjp_gather_fn, (unsigned)((items + 255) / 256), 1, 1, 256, 1, 1, 0, s, gargs, nullptr);
+  if (e != hipSuccess) return fail(h, MOL_ERR_HIP, std::string("vector-Jacobian gather launch: ") + hipGetErrorString(e));
   return 0;
 }
 
@@ -1462,6 +1517,12 @@ const char* mol_last_error(const mol_handle* h) {
 
 void mol_destroy(mol_handle* h) { free_handle(h); }
 
+JitGeom fwd_geom(const mol_handle& h, int ncore) {
+  JitGeom g = geom_of(h.ka64, h.kc64, ncore);
+  g.nthr = kernel_threads(g, false);
+  return g;
+}
+
 int64_t mol_codegen(const mol_desc* d, const int32_t* tile, char* buf, int64_t buflen, int32_t flags) {
   std::string msg;
   int rc = validate(d, msg);
@@ -1486,7 +1547,7 @@ int64_t mol_codegen(const mol_desc* d, const int32_t* tile, char* buf, int64_t b
   }
   std::string err;
   const std::string src = (flags & MOL_CODEGEN_VJP) ? jit_vjp_source(d, vjp_geometry(&tmp, d, nullptr), err)
-                                                    : jit_source(d, geom_of(tmp.ka64, tmp.kc64, ncore), err);
+                                                    : jit_source(d, fwd_geom(tmp, ncore), err);
   if (src.empty()) return fail(nullptr, MOL_ERR_UNSUPPORTED, err);
   if (flags & MOL_CODEGEN_COMPILE) {
     std::vector<char> code;

@@ -24,6 +24,7 @@ struct JitGeom {
   int ulo[3], uhi[3];                                 // union box (tile origin)
   int tile[3], halo[3], ext[3], vol, ntile[3];
   int ct_lo[3], ct_n[3], ncore;                       // core tiles (interior rows only)
+  int nthr;                                           // threads per work-group (multiple of 64)
 };
 
 // Pointer arguments of the generated kernel (layout identical to JPtrs of mol_jit_prelude.h).
@@ -70,6 +71,8 @@ struct VArgs {
   const T* p;
   const double* tnow;
   double tofs;
+  T* bf;            // per-tile adjoint buffers of the staged variables
+  double* ba;       // per-tile adjoint buffers of the staged grid fields
 };
 
 // What one right-hand-side launch needs: the core-tile kernel arguments and the list of tiles
@@ -87,6 +90,7 @@ struct Launch {
   hipFunction_t jit = nullptr;       // generated kernel over all tiles (nullptr: interpreter)
   int ntiles = 0;
   int jgrid = 0;                  // work-groups of the persistent generated kernel
+  int jthr = 256;                 // threads per work-group of the generated kernel
   hipFunction_t batch = nullptr;     // persistent kernel over the work list of a batch
   const BInst* insts = nullptr;
   const int32_t* wl = nullptr;
@@ -182,7 +186,12 @@ struct mol_handle {
   void* d_bdata = nullptr;                // weights, constants and coordinates of all instances
   std::vector<int64_t> inst_off;          // first unknown of each instance
   int jit_grid = 0;                       // work-groups of the persistent generated RHS kernel
+  int jit_nthr = 256;                     // threads per work-group of the generated RHS kernel
+  int vjp_nthr = 256;                     // threads per work-group of the generated VJP kernel
   hipFunction_t vjp_fn = nullptr;         // generated vector-Jacobian product kernel
+  hipFunction_t vjp_gather_fn = nullptr;  // its gather of the per-tile adjoint buffers
+  void* vjp_bf = nullptr;                 // per-tile adjoint buffers: staged variables (T)
+  void* vjp_ba = nullptr;                 //   and staged grid fields (FP64)
   int vjp_tiles = 0;
   bool vjp_aux_ok = true;                 // no boundary value reads an auxiliary array
   void* ck_u = nullptr;                   // checkpoints of the last mol_forward_checkpointed run

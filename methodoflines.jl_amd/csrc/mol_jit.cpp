@@ -338,7 +338,7 @@ class Gen {
     const char* e = std::getenv("MOL_RB");
     const int want = e ? std::max(1, std::atoi(e)) : 4;
     for (int r = want; r > 1; r /= 2)
-      if (g_.tile[ba] % r == 0 && (e || vol / r >= 256)) {
+      if (g_.tile[ba] % r == 0 && (e || vol / r >= g_.nthr)) {
         rb = r;
         return;
       }
@@ -947,12 +947,13 @@ void Gen::print_reverse(std::ostream& o, int r, const std::vector<std::pair<int,
     const bool field = arr < d_->nfields;
     const int sl = field ? pg_.fslot[arr] : pg_.aslot[arr - d_->nfields];
     if (sl < 0) {
-      // array read at the node only: its adjoint goes straight to global memory
+      // array read at the node only: its adjoint goes straight to global memory (this node is
+      // the only writer of its position in a launch)
       if (field)
-        o << "    mol_atomic_add(&outp[" << UIDX(arr, "I0", "I1", "I2") << "], " << kv.second << ");\n";
+        o << "    outp[" << UIDX(arr, "I0", "I1", "I2") << "] += " << kv.second << ";\n";
       else
-        o << "    if (gaux) mol_atomic_add_f64(&gaux[" << (arr - d_->nfields) << " * NGRID + gidx(I0, I1, I2)], (double)("
-          << kv.second << "));\n";
+        o << "    if (gaux) gaux[" << (arr - d_->nfields) << " * NGRID + gidx(I0, I1, I2)] += (double)(" << kv.second
+          << ");\n";
       continue;
     }
     const long long off = kv.first[1] + (long long)pg_.ext[0] * (kv.first[2] + (long long)pg_.ext[1] * kv.first[3]);
@@ -1103,7 +1104,7 @@ std::string Gen::geo_common() {
   }
   c("TX", g_.tile[0]); c("TY", g_.tile[1]); c("TZ", g_.tile[2]);
   c("NBLK", g_.ntile[0] * g_.ntile[1] * g_.ntile[2]);
-  c("NTHR", 256);
+  c("NTHR", g_.nthr);
   // register blocking: each thread runs RB consecutive nodes along axis BA (the slowest tiled
   // axis), so taps and subexpressions shared by neighbours along BA are loaded and computed once
   int ba = 0, rb = 1;
@@ -1287,22 +1288,32 @@ std::string Gen::geo_variant(const std::string& name, const int* halo, const Usa
     for (int f = 0; f < d_->nfields; ++f) seeds.push_back({outs[f], "lam[" + istr(f) + "]"});
     print_reverse(o, region, seeds);
     o << "  }\n";
-    // adjoints of one tile + halo position: through the transposed staging to the unknowns,
-    // and to the grid-field gradients
+    // adjoints of one tile + halo position: positions of a variable's unknown box (after the
+    // periodic wrap) and of the grid go to the tile's buffers (slot stride vs; zero elsewhere),
+    // ghost positions through the transposed closures to the unknowns (atomics)
     o << "  static __device__ __forceinline__ void flush_pos(const T* Ab, const double* Aa, const int q, const int p0,\n"
-         "      const int p1, const int p2, T* __restrict__ outp, double* __restrict__ gaux) {\n"
-         "    (void)Ab; (void)Aa; (void)gaux;\n";
+         "      const int p1, const int p2, T* __restrict__ outp, T* __restrict__ bft, double* __restrict__ bat,\n"
+         "      const int vs) {\n"
+         "    (void)Ab; (void)Aa; (void)bat; (void)vs; (void)outp;\n";
+    if (nstf > 0) {
+      o << "    int w0 = p0, w1 = p1, w2 = p2;\n";
+      for (int a = 0; a < 3; ++a)
+        if (g_.periodic[a])
+          o << "    w" << a << " = mol_wrap(mol_wrap(w" << a << ", " << g_.n[a] << "), " << g_.n[a] << ");\n";
+    }
     for (int f = 0; f < d_->nfields; ++f)
-      if (pg_.fslot[f] >= 0)
-        o << "    { const T v = Ab[" << (long long)pg_.fslot[f] * pg_.vol << " + q]; if (v != T(0)) scatter(" << f
-          << ", p0, p1, p2, v, outp); }\n";
+      if (pg_.fslot[f] >= 0) {
+        const long long k = pg_.fslot[f];
+        o << "    { const T v = Ab[" << k * pg_.vol << " + q];\n"
+          << "      if (inbox(" << f << ", w0, w1, w2)) bft[" << k << " * vs + q] = v;\n"
+          << "      else { bft[" << k << " * vs + q] = T(0); if (v != T(0)) scatter(" << f << ", p0, p1, p2, v, outp); } }\n";
+      }
     if (nsta > 0) {
-      o << "    if (gaux) {\n      const i64 gi = gidx(p0, p1, p2);\n      if (gi >= 0) {\n";
+      o << "    if (bat) {\n      const bool in = gidx(p0, p1, p2) >= 0;\n";
       for (int a = 0; a < d_->naux; ++a)
         if (pg_.aslot[a] >= 0)
-          o << "        { const double v = Aa[" << (long long)pg_.aslot[a] * pg_.vol << " + q]; if (v != 0.0) "
-            << "mol_atomic_add_f64(&gaux[" << a << " * NGRID + gi], v); }\n";
-      o << "      }\n    }\n";
+          o << "      bat[" << pg_.aslot[a] << " * vs + q] = in ? Aa[" << (long long)pg_.aslot[a] * pg_.vol << " + q] : 0.0;\n";
+      o << "    }\n";
     }
     o << "  }\n";
   }
@@ -1351,11 +1362,11 @@ std::string Gen::source_kernel(bool f64, bool vjp) {
       "  const int ty = b % GeoBase::NT1;\n  const int tz = b / GeoBase::NT1;\n";
   int nst = 0;
   for (bool b : us.staged) nst += b ? 1 : 0;
-  const bool persistent = nst > 0 && ((npf + 255) / 256) * nst <= 16;
+  const bool persistent = nst > 0 && ((npf + g_.nthr - 1) / g_.nthr) * nst <= 16;
   if (!vjp && persistent) {
     // persistent: gridDim.x work-groups walk the tiles, the next tile's staging loads in flight
     // during the current tile's node phase (few staging values per thread)
-    o << "extern \"C\" __global__ void __launch_bounds__(256) mol_rhs_tiles(const T* __restrict__ u, Epi ep, JPtrs jp) {\n"
+    o << "extern \"C\" __global__ void __launch_bounds__(" << g_.nthr << ") mol_rhs_tiles(const T* __restrict__ u, Epi ep, JPtrs jp) {\n"
          "  __shared__ T L[LDSN];\n"
          "  const double t = (ep.tnow ? ep.tnow[0] : 0.0) + ep.tofs;\n"
          "  mol_advance_clock(ep);\n"
@@ -1363,7 +1374,7 @@ std::string Gen::source_kernel(bool f64, bool vjp) {
       << ", NPF>(L, u, ep, jp, t);\n}\n";
   } else if (!vjp) {
     // one work-group per tile
-    o << "extern \"C\" __global__ void __launch_bounds__(256) mol_rhs_tiles(const T* __restrict__ u, Epi ep, JPtrs jp) {\n"
+    o << "extern \"C\" __global__ void __launch_bounds__(" << g_.nthr << ") mol_rhs_tiles(const T* __restrict__ u, Epi ep, JPtrs jp) {\n"
          "  __shared__ T L[LDSN];\n" << decode
       << "  const double t = (ep.tnow ? ep.tnow[0] : 0.0) + ep.tofs;\n"
          "  mol_advance_clock(ep);\n";
@@ -1373,14 +1384,60 @@ std::string Gen::source_kernel(bool f64, bool vjp) {
       o << "  ";
     o << "mol_full_tile<GeoB>(L, tx, ty, tz, u, ep, jp, t);\n}\n";
   } else {
-    o << "extern \"C\" __global__ void __launch_bounds__(256) mol_vjp_tiles(VArgs va, JPtrs jp) {\n"
+    // tile kernel: reverse sweep of every node, tile + halo adjoints to per-tile buffers
+    o << "constexpr int VOLX = " << npf << ";\n\n";
+    o << "extern \"C\" __global__ void __launch_bounds__(" << g_.nthr << ") mol_vjp_tiles(VArgs va, JPtrs jp) {\n"
          "  __shared__ T L[LDSN];\n  __shared__ T Ab[LDSF];\n  __shared__ double Aa[LDSA];\n" << decode
       << "  const double t = (va.tnow ? va.tnow[0] : 0.0) + va.tofs;\n";
     if (g_.ncore > 0)
-      o << "  if (GeoBase::is_core(tx, ty, tz)) mol_vjp_tile<GeoC>(L, Ab, Aa, tx, ty, tz, va, jp, t);\n  else ";
+      o << "  if (GeoBase::is_core(tx, ty, tz)) mol_vjp_tile<GeoC, VOLX>(L, Ab, Aa, tx, ty, tz, va, jp, t);\n  else ";
     else
       o << "  ";
-    o << "mol_vjp_tile<GeoB>(L, Ab, Aa, tx, ty, tz, va, jp, t);\n}\n";
+    o << "mol_vjp_tile<GeoB, VOLX>(L, Ab, Aa, tx, ty, tz, va, jp, t);\n}\n\n";
+    // gather kernel: each unknown of a staged variable (then each grid position of a staged
+    // grid field, when g_aux is given) sums the buffers of the tiles whose tile + halo covers it
+    const std::string gc = g_.ncore > 0 ? "GeoC" : "GeoB", hasc = g_.ncore > 0 ? "true" : "false";
+    long long nunk = 0;
+    for (int f = 0; f < d_->nfields; ++f) {
+      long long sz = 1;
+      for (int a = 0; a < 3; ++a) sz *= g_.hi[f][a] - g_.lo[f][a] + 1;
+      nunk = std::max(nunk, (long long)g_.fofs[f] + sz);
+    }
+    o << "extern \"C\" __global__ void __launch_bounds__(256) mol_vjp_gather(VArgs va) {\n"
+         "  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;\n"
+         "  (void)va;\n";
+    o << "  if (i < " << nunk << "LL) {\n";
+    for (int f = 0; f < d_->nfields; ++f) {
+      if (pg_.fslot[f] < 0) continue;
+      long long sz = 1;
+      for (int a = 0; a < 3; ++a) sz *= g_.hi[f][a] - g_.lo[f][a] + 1;
+      const int bx = g_.hi[f][0] - g_.lo[f][0] + 1, by = g_.hi[f][1] - g_.lo[f][1] + 1;
+      o << "    if (i >= " << g_.fofs[f] << "LL && i < " << (long long)g_.fofs[f] + sz << "LL) {\n"
+        << "      const int r = (int)(i - " << g_.fofs[f] << "LL), r1 = r / " << bx << ";\n"
+        << "      const int p0 = " << g_.lo[f][0] << " + (r - r1 * " << bx << "), p1 = " << g_.lo[f][1] << " + r1 % " << by
+        << ", p2 = " << g_.lo[f][2] << " + r1 / " << by << ";\n"
+        << "      va.out[i] += (T)mol_gather_pos<" << gc << ", GeoB, " << hasc << ", VOLX, GeoB::NSTF>(va.bf, "
+        << pg_.fslot[f] << ", p0, p1, p2);\n    }\n";
+    }
+    o << "    return;\n  }\n";
+    bool any_aux = false;
+    for (int a = 0; a < d_->naux; ++a) any_aux = any_aux || pg_.aslot[a] >= 0;
+    if (any_aux) {
+      o << "  if (!va.g_aux) return;\n"
+        << "  const i64 j = i - " << nunk << "LL;\n"
+        << "  if (j >= " << d_->naux << " * GeoBase::NGRID) return;\n"
+        << "  const int a = (int)(j / GeoBase::NGRID), g = (int)(j - (i64)a * GeoBase::NGRID);\n"
+        << "  const int g1 = g / " << g_.n[0] << ";\n"
+        << "  const int p0 = 1 + (g - g1 * " << g_.n[0] << "), p1 = 1 + g1 % " << g_.n[1] << ", p2 = 1 + g1 / " << g_.n[1]
+        << ";\n";
+      for (int a = 0; a < 3; ++a)
+        if (g_.periodic[a]) o << "  if (p" << a << " == 1) return;   // image of node n\n";
+      for (int a = 0; a < d_->naux; ++a)
+        if (pg_.aslot[a] >= 0)
+          o << "  if (a == " << a << ") va.g_aux[j] += mol_gather_pos<" << gc << ", GeoB, " << hasc
+            << ", VOLX, GeoB::NSTA>(va.ba, " << pg_.aslot[a] << ", p0, p1, p2);\n";
+    }
+    o << "}\n";
   }
   return o.str();
 }

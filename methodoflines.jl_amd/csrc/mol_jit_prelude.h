@@ -394,7 +394,9 @@ __device__ __forceinline__ void mol_tiles_persistent(T* L, const T* __restrict__
 
 // Arguments of the generated vector-Jacobian product (layout identical to mol::VArgs<T>).
 // The seed is lam = c1*s1 + c2*s2 (s2 may be null); out receives (df/du)^T lam and g_aux the
-// FP64 auxiliary-array gradients (may be null); both are accumulated, not overwritten.
+// FP64 auxiliary-array gradients (may be null); both are accumulated, not overwritten.  bf and
+// ba hold each tile's tile + halo adjoints of the staged variables and grid fields between the
+// tile kernel and the gather kernel.
 struct VArgs {
   const T* u;
   const T* s1;
@@ -405,13 +407,66 @@ struct VArgs {
   const T* p;
   const double* tnow;
   double tofs;
+  T* bf;
+  double* ba;
 };
+
+// Floor of a / b for b > 0.
+__device__ __forceinline__ int mol_floordiv(const int a, const int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+// Tiles along one axis whose tile + halo (halo H, the larger of the two variants) contains
+// position p or, on a periodic axis with period P, one of its images p + kP: tile indices and
+// the position relative to each tile's origin.  At most 10 entries.
+template <int TL, int H, int NT, int ULO, bool PER, int P>
+__device__ __forceinline__ int mol_cover(const int p, int* ts, int* ls) {
+  int n = 0;
+#pragma unroll
+  for (int k = PER ? -2 : 0; k <= (PER ? 2 : 0); ++k) {
+    const int pp = p + k * P;
+    const int tlo = max(0, -mol_floordiv(-(pp - ULO - TL + 1 - H), TL));
+    const int thi = min(NT - 1, mol_floordiv(pp - ULO + H, TL));
+    for (int t = tlo; t <= thi && n < 10; ++t) {
+      ts[n] = t;
+      ls[n] = pp - (ULO + t * TL);
+      ++n;
+    }
+  }
+  return n;
+}
+
+// Sum over the tiles covering position (p0, p1, p2) of slot s of their adjoint buffers (base:
+// buffer of tile 0, stride NS * VOLX per tile): each tile in its own variant's layout.
+template <class GC, class GB, bool HASC, int VOLX, int NS, typename B>
+__device__ __forceinline__ double mol_gather_pos(const B* __restrict__ base, const int s, const int p0, const int p1,
+                                                 const int p2) {
+  int t0[10], l0[10], t1[10], l1[10], t2[10], l2[10];
+  const int n0 = mol_cover<GB::TX, GB::HX, GB::NT0, GB::ULO0, GB::PER0 != 0, GB::N0 - 1>(p0, t0, l0);
+  const int n1 = mol_cover<GB::TY, GB::HY, GB::NT1, GB::ULO1, GB::PER1 != 0, GB::N1 - 1>(p1, t1, l1);
+  const int n2 = mol_cover<GB::TZ, GB::HZ, GB::NT2, GB::ULO2, GB::PER2 != 0, GB::N2 - 1>(p2, t2, l2);
+  double sum = 0.0;
+  for (int c = 0; c < n2; ++c)
+    for (int b = 0; b < n1; ++b)
+      for (int a = 0; a < n0; ++a) {
+        const int tl = t0[a] + GB::NT0 * (t1[b] + GB::NT1 * t2[c]);
+        int q;
+        if (HASC && GB::is_core(t0[a], t1[b], t2[c])) {
+          if (l0[a] < -GC::HX || l0[a] >= GC::TX + GC::HX || l1[b] < -GC::HY || l1[b] >= GC::TY + GC::HY ||
+              l2[c] < -GC::HZ || l2[c] >= GC::TZ + GC::HZ)
+            continue;
+          q = (l0[a] + GC::HX) + GC::EX * ((l1[b] + GC::HY) + GC::EY * (l2[c] + GC::HZ));
+        } else {
+          q = (l0[a] + GB::HX) + GB::EX * ((l1[b] + GB::HY) + GB::EY * (l2[c] + GB::HZ));
+        }
+        sum += (double)base[(i64)tl * (NS * VOLX) + (i64)s * VOLX + q];
+      }
+  return sum;
+}
 
 // One tile of the reverse-mode sweep in tile variant G: stage as the forward kernel, run each
 // node's adjoint (tap adjoints summed in LDS with LDS atomics; arrays read only at the node go
 // straight to global memory), then add the LDS adjoints of tile + halo to the unknowns (through
 // the transposed ghost closures) and to the grid-field gradients.
-template <class G>
+template <class G, int VOLX>
 __device__ __forceinline__ void mol_vjp_tile(T* L, T* Ab, double* Aa, const int tx, const int ty, const int tz,
                                              const VArgs& va, const JPtrs& jp, const double t) {
   const int o0 = G::ULO0 + tx * G::TX, o1 = G::ULO1 + ty * G::TY, o2 = G::ULO2 + tz * G::TZ;
@@ -451,9 +506,14 @@ __device__ __forceinline__ void mol_vjp_tile(T* L, T* Ab, double* Aa, const int 
     G::vjp(L, Ab, Aa, base, I0, I1, I2, t, va.p, va.u, jp, lam, va.out, va.g_aux);
   }
   __syncthreads();
+  // tile + halo adjoints to this tile's buffers (plain stores; the gather kernel sums the
+  // buffers of the tiles covering each position), ghost positions through the closures
+  const i64 tl = tx + (i64)G::NT0 * (ty + (i64)G::NT1 * tz);
+  T* bft = va.bf + tl * (G::NSTF * VOLX);
+  double* bat = va.g_aux ? va.ba + tl * (G::NSTA * VOLX) : nullptr;
   MolWalk<G::EX, G::EY, G::NTHR> w((int)threadIdx.x);
   for (int q = threadIdx.x; q < G::VOL; q += G::NTHR, w.step())
-    G::flush_pos(Ab, Aa, q, o0 + w.x - G::HX, o1 + w.y - G::HY, o2 + w.z - G::HZ, va.out, va.g_aux);
+    G::flush_pos(Ab, Aa, q, o0 + w.x - G::HX, o1 + w.y - G::HY, o2 + w.z - G::HZ, va.out, bft, bat, VOLX);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -188,7 +188,7 @@ int launch_rhs(const Launch<T>& L, const T* u, const Epi<T>& ep, hipStream_t str
     Epi<T> e = ep;
     JitPtrs<T> jp = L.jp;
     void* args[] = {(void*)&uu, (void*)&e, (void*)&jp};
-    if (hipModuleLaunchKernel(L.jit, L.jgrid, 1, 1, 256, 1, 1, 0, stream, args, nullptr) != hipSuccess)
+    if (hipModuleLaunchKernel(L.jit, L.jgrid, 1, 1, L.jthr, 1, 1, 0, stream, args, nullptr) != hipSuccess)
       return MOL_ERR_HIP;
     return 0;
   }

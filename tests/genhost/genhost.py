@@ -8,6 +8,7 @@ generated forward and reverse-mode code against the oracle without a GPU.
 import ctypes as C
 import hashlib
 import os
+import re
 import subprocess
 import tempfile
 
@@ -23,7 +24,7 @@ def _host_source(src: str, entry: str) -> str:
     prelude = open(PRELUDE).read()
     shim = open(os.path.join(HERE, "shim.h")).read()
     src = src.replace('#include "mol_jit_prelude.h"', shim + "\n" + prelude)
-    src = src.replace("static constexpr int NTHR = 256;", "static constexpr int NTHR = 1;")
+    src = re.sub(r"static constexpr int NTHR = \d+;", "static constexpr int NTHR = 1;", src)
     return src + entry
 
 
@@ -41,7 +42,7 @@ class JPtrs(C.Structure):
 class VArgs(C.Structure):
     _fields_ = [("u", C.c_void_p), ("s1", C.c_void_p), ("s2", C.c_void_p), ("c1", C.c_double), ("c2", C.c_double),
                 ("out", C.c_void_p), ("g_aux", C.c_void_p), ("p", C.c_void_p), ("tnow", C.c_void_p),
-                ("tofs", C.c_double)]
+                ("tofs", C.c_double), ("bf", C.c_void_p), ("ba", C.c_void_p)]
 
 
 def _compile(src: str) -> C.CDLL:
@@ -63,10 +64,13 @@ def _compile(src: str) -> C.CDLL:
 
 
 def _nblocks(src: str) -> int:
-    for line in src.splitlines():
-        if "static constexpr int NBLK = " in line:
-            return int(line.split("=")[1].strip().rstrip(";"))
-    raise ValueError("no NBLK")
+    return _const(src, "static constexpr int NBLK = ")
+
+
+def _const(src: str, key: str, start: str = "") -> int:
+    i = src.index(start) if start else 0
+    j = src.index(key, i) + len(key)
+    return int(src[j:src.index(";", j)].split(",")[0])
 
 
 def _ptrs(low, aux):
@@ -114,9 +118,16 @@ def vjp(low, u, lam, t=0.0, p=None, aux=None, tile=None):
     import molgpu
     src = molgpu.codegen(low, tile=tile, vjp=True)
     entry = """
-extern "C" __attribute__((visibility("default"))) void run(VArgs va, JPtrs jp, int nblk) {
+extern "C" __attribute__((visibility("default"))) void run(VArgs va, JPtrs jp, int nblk, long long items) {
   gridDim.x = nblk;
+  threadIdx.x = 0;
   for (int b = 0; b < nblk; ++b) { blockIdx.x = b; mol_vjp_tiles(va, jp); }
+  for (long long i = 0; i < items; ++i) {
+    blockIdx.x = (unsigned)(i / 256);
+    threadIdx.x = (unsigned)(i % 256);
+    mol_vjp_gather(va);
+  }
+  threadIdx.x = 0;
 }
 """
     lib = _compile(_host_source(src, entry))
@@ -135,6 +146,13 @@ extern "C" __attribute__((visibility("default"))) void run(VArgs va, JPtrs jp, i
     va.p = pv.ctypes.data if pv.size else None
     va.tofs = t
     jp, keep = _ptrs(low, aux or {})
-    lib.run.argtypes = [VArgs, JPtrs, C.c_int]
-    lib.run(va, jp, _nblocks(src))
+    nblk = _nblocks(src)
+    volx = _const(src, "constexpr int VOLX = ")
+    bf = np.zeros(max(1, nblk * volx * _const(src, "static constexpr int NSTF = ", "struct GeoB")))
+    ba = np.zeros(max(1, nblk * volx * _const(src, "static constexpr int NSTA = ", "struct GeoB")))
+    va.bf = bf.ctypes.data
+    va.ba = ba.ctypes.data
+    items = low.n_unknowns + (len(low.aux) * ng if low.aux else 0)
+    lib.run.argtypes = [VArgs, JPtrs, C.c_int, C.c_longlong]
+    lib.run(va, jp, nblk, items)
     return out, g[: len(low.aux) * ng].reshape(len(low.aux), ng) if low.aux else None

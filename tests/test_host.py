@@ -291,12 +291,14 @@ def test_generated_kernel_compiles_for_gfx950(name):
 
 @pytest.mark.parametrize("name", sorted(_codegen_systems()))
 def test_generated_vjp_kernel_compiles_for_gfx950(name):
-    """The reverse-mode kernel of every slot kind compiles; LDS tap adjoints use atomics and
-    ghost-node adjoints go through the transposed closures."""
+    """The reverse-mode kernels of every slot kind compile: the tile kernel (LDS tap adjoints
+    with LDS atomics, ghost-node adjoints through the transposed closures, tile + halo adjoints
+    to per-tile buffers) and the gather kernel that sums the buffers."""
     import molgpu
     sys_, disc = _codegen_systems()[name]()
     src = molgpu.codegen(lower(sys_, disc), compile=True, vjp=True)
-    assert "mol_vjp_tile<GeoB>" in src and "static __device__ void scatter(" in src
+    assert "mol_vjp_tile<GeoB, VOLX>" in src and "static __device__ void scatter(" in src
+    assert "mol_vjp_gather(VArgs va)" in src
 
 
 def test_generated_kernel_folds_exact_weights():
