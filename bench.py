@@ -9,7 +9,9 @@ point is one unknown grid node with both variables); value = points * 4 * K / el
 
 The same JSON line carries the other parts of the metric:
   roofline     the RK4 stage kernel of the timed region against HBM peak;
-  weno5        one FP64 WENO5 Burgers RHS at 1024^2 (and 4096^2, beyond the 256 MB MALL);
+  weno5        one FP64 WENO5 Burgers RHS at 1024^2 (and 4096^2, beyond the 256 MB MALL), and the
+               1024^2 SSPRK3 solve (CFL 0.4, 500 steps) in FP64, FP32 + Kahan and plain FP32 with
+               the FP32 errors against FP64 and the FP64 error against the oracle;
   adjoint      3D nonlinear diffusion 128^3 (2.0 M unknowns), 200 RK4 steps: plain forward time,
                then per checkpoint mode (every 20 steps with recomputation, and the automatic
                choice, which keeps every stage state when it fits in device memory) the
@@ -29,6 +31,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "methodoflines.jl_amd"))
 
 HBM_PEAK_GBS = 8000.0
+
+# HBM bytes per RK4 stage launch of the headline workload, from two rocprofv3 --pmc runs of
+# `bench.py --quick --no-cpu --steps 50` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, averaged
+# over the stage launches), each scaled by what the counter reads for a device copy of 64 MiB
+# (`bench.py --section copy`: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); summary in
+# profiles/brusselator512_rk4_stage_kernel.md.  Below the 17.8 MB of algorithmic bytes: halo
+# re-reads and the stage vectors written by one launch and read by the next hit in L2.
+STAGE_TRAFFIC_BYTES = (3960.27 * 2.0 + 5869.53 * 1.0) * 1024
 
 
 def events():
@@ -70,6 +80,67 @@ def bench_weno(n, reps):
     ach = pts * 16 / (us * 1e-6) / 1e9
     return {"grid": "%dx%d" % (n, n), "rhs_us": us, "gpoint_s": pts / (us * 1e-6) / 1e9,
             "bytes_per_launch": pts * 16, "achieved_gbs": ach, "hbm_frac": ach / HBM_PEAK_GBS}
+
+
+def bench_weno_solve(n, nsteps, oracle_steps):
+    """The WENO5 Burgers workload as an integration: SSPRK3 at CFL 0.4 for `nsteps` steps in FP64,
+    in FP32 with the compensated update and in plain FP32; errors of the FP32 runs against the
+    FP64 run, and of the FP64 run against the oracle's SSPRK3 over `oracle_steps` steps."""
+    import numpy as np
+    import torch
+    import molgpu
+    from molgpu import systems
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    p64 = molgpu.discretize(*systems.burgers2d(n=n + 1))
+    p32 = molgpu.discretize(*systems.burgers2d(n=n + 1, dtype="f32"))
+    u0 = np.asarray(p64.u0, dtype=np.float64)
+    dx = 1.0 / n
+    dt = 0.4 * dx / float(np.max(np.abs(u0)))
+    res = {"grid": "%dx%d" % (n, n), "scheme": "SSPRK33", "cfl": 0.4, "dt": dt, "steps": nsteps}
+    finals = {}
+    for name, prob, kahan in (("fp64", p64, False), ("fp32_kahan", p32, True), ("fp32", p32, False)):
+        dtype = torch.float64 if name == "fp64" else torch.float32
+        u = torch.tensor(u0, dtype=dtype, device="cuda")
+        prob.integrate(u, 0.0, dt, 5, molgpu.SSPRK33, kahan=kahan, graph=True)
+        u = torch.tensor(u0, dtype=dtype, device="cuda")
+        torch.cuda.synchronize()
+        e0, e1 = events()
+        e0.record()
+        prob.integrate(u, 0.0, dt, nsteps, molgpu.SSPRK33, kahan=kahan, graph=True)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        res[name] = {"ms": ms, "ms_per_step": ms / nsteps,
+                     "gpoint_rhs_s": n * n * 3 * nsteps / (ms / 1e3) / 1e9}
+    # errors in chunks of 50 steps, at the last chunk where the FP64 solution is finite (the
+    # scheme applies WENO5 to u_x without upwinding, as the reference does, and the shocks of
+    # this initial state break it down near t = 0.15 at 1024^2)
+    us = {name: torch.tensor(u0, dtype=torch.float64 if name == "fp64" else torch.float32, device="cuda")
+          for name in ("fp64", "fp32_kahan", "fp32")}
+    done = 0
+    while done < nsteps:
+        k = min(50, nsteps - done)
+        for name, prob, kahan in (("fp64", p64, False), ("fp32_kahan", p32, True), ("fp32", p32, False)):
+            prob.integrate(us[name], done * dt, dt, k, molgpu.SSPRK33, kahan=kahan, graph=True)
+        if not bool(torch.isfinite(us["fp64"]).all()):
+            break
+        done += k
+        for name in ("fp64", "fp32_kahan", "fp32"):
+            finals[name] = us[name].double().clone()
+    res["finite_steps"] = done
+    if finals:
+        res["max_abs_u"] = float(finals["fp64"].abs().max())
+        for name in ("fp32_kahan", "fp32"):
+            res[name]["max_abs_err_vs_fp64"] = float((finals[name] - finals["fp64"]).abs().max())
+    # FP64 run against the oracle over the first steps
+    u = torch.tensor(u0, device="cuda")
+    p64.integrate(u, 0.0, dt, oracle_steps, molgpu.SSPRK33)
+    ref = orc.burgers2d_integrate(n + 1, u0, 0.0, dt, oracle_steps, threads=min(16, os.cpu_count() or 1))
+    res["fp64_vs_oracle"] = {"steps": oracle_steps,
+                             "max_abs_err": float(np.max(np.abs(u.cpu().numpy() - ref))),
+                             "max_abs_u": float(np.max(np.abs(ref)))}
+    return res
 
 
 def bench_rhs3d(n, reps):
@@ -224,7 +295,7 @@ def bench_batch(count, nsteps, quick):
     from molgpu.autotune import tune_batch
 
     rng = np.random.default_rng(2024)
-    lows = []
+    lows, lows64 = [], []
     t0 = time.perf_counter()
     dts = []
     for _ in range(count):
@@ -235,11 +306,13 @@ def bench_batch(count, nsteps, quick):
         u0 = rng.uniform(0, 1, (n, n))
         for _ in range(3):     # Jacobi smoothing: 5-point average, boundary rows kept
             u0[1:-1, 1:-1] = 0.2 * (u0[1:-1, 1:-1] + u0[:-2, 1:-1] + u0[2:, 1:-1] + u0[1:-1, :-2] + u0[1:-1, 2:])
-        sys_, disc = systems.diffreact2d(n, D, k, bct, dtype="f32")
-        low = lower(sys_, disc)
-        (l0, l1), (h0, h1) = low.lo[0], low.hi[0]
-        low.u0 = u0[l0 - 1:h0, l1 - 1:h1].reshape(-1, order="F").astype(np.float32)
-        lows.append(low)
+        for dtype, out in (("f32", lows), ("f64", lows64)):
+            sys_, disc = systems.diffreact2d(n, D, k, bct, dtype=dtype)
+            low = lower(sys_, disc)
+            (l0, l1), (h0, h1) = low.lo[0], low.hi[0]
+            box = u0[l0 - 1:h0, l1 - 1:h1].reshape(-1, order="F")
+            low.u0 = box.astype(np.float32) if dtype == "f32" else box.astype(np.float32).astype(np.float64)
+            out.append(low)
         dx = 1.0 / (n - 1)
         dts.append(0.5 * 2.78 * dx * dx / (8 * D))
     lower_s = time.perf_counter() - t0
@@ -257,10 +330,22 @@ def bench_batch(count, nsteps, quick):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1)
     pts = b.n_unknowns
+    # error of the FP32 runs (with and without the compensated update) against the same batch in
+    # FP64 from the same (FP32-representable) initial state
+    b64 = BatchProblem(lows64, tile=tile)
+    u64 = torch.tensor(b64.u0, dtype=torch.float64, device="cuda")
+    b64.integrate(u64, 0.0, dt, nsteps, molgpu.RK4, graph=True)
+    u32 = torch.tensor(b.u0, dtype=torch.float32, device="cuda")
+    b.integrate(u32, 0.0, dt, nsteps, molgpu.RK4, kahan=False, graph=True)
+    torch.cuda.synchronize()
+    scale = float(u64.abs().max())
+    err_k = float((u.double() - u64).abs().max())
+    err_p = float((u32.double() - u64).abs().max())
     return {"instances": count, "unknowns": pts, "classes": b.plan()["classes"], "tile": list(tile),
             "tuning": table, "steps": nsteps, "dt": dt, "ms": ms, "ms_per_step": ms / nsteps,
             "gpoint_rhs_s": pts * 4 * nsteps / (ms / 1e3) / 1e9, "lowering_s": lower_s,
-            "finite": bool(torch.isfinite(u).all())}
+            "finite": bool(torch.isfinite(u).all()),
+            "err_vs_fp64": {"fp32_kahan_max_abs": err_k, "fp32_plain_max_abs": err_p, "max_abs_u": scale}}
 
 
 def main():
@@ -272,7 +357,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only (no weno/adjoint/batch sections)")
     ap.add_argument("--section", default="all",
-                    choices=["all", "weno", "adjoint", "batch", "rhs3d", "copy"],
+                    choices=["all", "weno", "wenosolve", "adjoint", "batch", "rhs3d", "copy"],
                     help="run one section alone (for profiling); prints its JSON")
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--reps", type=int, default=200, help="back-to-back RHS launches of the weno/rhs3d sections")
@@ -289,6 +374,8 @@ def main():
     if args.section != "all":
         if args.section == "weno":
             res = {"1024": bench_weno(1024, args.reps)}
+        elif args.section == "wenosolve":
+            res = bench_weno_solve(1024, 500, 3)
         elif args.section == "adjoint":
             res = bench_adjoint(128, args.adj_steps, 20, 2)
         elif args.section == "batch":
@@ -356,7 +443,7 @@ def main():
                                "RK4 fixed dt=1e-5", "points": points, "rhs_per_step": 4,
                    "graph": bool(args.graph)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": STAGE_TRAFFIC_BYTES,
                      "kernel": "mol_rhs_tiles (generated; RK4 stage epilogue)", "avg_us": launch_us,
                      "bytes_per_launch": stage_bytes,
                      "rhs_only": {"avg_us": rhs_us, "bytes_per_launch": points * 32, "achieved": rhs_ach,
@@ -364,7 +451,8 @@ def main():
         "wall_s": wall,
     }
     if not args.quick:
-        out["weno5"] = {"1024": bench_weno(1024, 200), "4096": bench_weno(4096, 20)}
+        out["weno5"] = {"1024": bench_weno(1024, 200), "4096": bench_weno(4096, 20),
+                        "solve_1024": bench_weno_solve(1024, 500, 3)}
         out["adjoint"] = bench_adjoint(128, 200, 20, 8)
         out["batch"] = bench_batch(512, 200, quick=False)
 

@@ -854,6 +854,7 @@ int ensure_vjp(mol_handle* h) {
     h->vjp_fn = nullptr;
     return fail(h, MOL_ERR_HIP, "vector-Jacobian product: tile buffer allocation failed");
   }
+  h->vjp_ba_count = (size_t)ntiles * volx * nsta;
   if (nsta > 0 && hipMalloc(&h->vjp_ba, (size_t)ntiles * volx * nsta * sizeof(double)) != hipSuccess) {
     h->vjp_fn = nullptr;
     return fail(h, MOL_ERR_HIP, "vector-Jacobian product: tile buffer allocation failed");
@@ -866,8 +867,9 @@ int ensure_vjp(mol_handle* h) {
   return 0;
 }
 
+// Tile kernel (when `tiles`) then the gather kernel for the parts va.mode selects.
 template <typename T>
-int launch_vjp(mol_handle* h, const VArgs<T>& va, hipStream_t s) {
+int launch_vjp(mol_handle* h, const VArgs<T>& va, hipStream_t s, bool tiles = true) {
   const KArgs<T>& ka = kargs<T>(h);
   JitPtrs<T> jp{};
   for (int a = 0; a < 3; ++a) jp.coord[a] = ka.coord[a];
@@ -876,8 +878,12 @@ int launch_vjp(mol_handle* h, const VArgs<T>& va, hipStream_t s) {
   v.bf = (T*)h->vjp_bf;
   v.ba = (double*)h->vjp_ba;
   void* args[] = {(void*)&v, (void*)&jp};
-  hipError_t e = hipModuleLaunchKernel(h->vjp_fn, h->vjp_tiles, 1, 1, h->vjp_nthr, 1, 1, 0, s, args, nullptr);
-  if (e != hipSuccess) return fail(h, MOL_ERR_HIP, std::string("vector-Jacobian product launch: ") + hipGetErrorString(e));
+  hipError_t e = hipSuccess;
+  if (tiles) {
+    e = hipModuleLaunchKernel(h->vjp_fn, h->vjp_tiles, 1, 1, h->vjp_nthr, 1, 1, 0, s, args, nullptr);
+    if (e != hipSuccess) return fail(h, MOL_ERR_HIP, std::string("vector-Jacobian product launch: ") + hipGetErrorString(e));
+  }
+  if (!(v.mode & (VJP_GATHER_F | VJP_GATHER_A))) return 0;
   // gather: every unknown, then every grid position of every grid field (when g_aux is given)
   const int64_t items = h->nunk + (v.g_aux ? (int64_t)h->desc.d.naux * ka.n[0] * ka.n[1] * ka.n[2] : 0);
   void* gargs[] = {(void*)&v};
@@ -1013,9 +1019,13 @@ int adjoint_impl(mol_handle* h, const T* p, T* lam, double* g_aux, hipStream_t s
   T* S = (T*)h->seg_u;
   auto st = [&](int64_t j, int k) { return S + (size_t)(4 * j + k) * n; };   // state k of step j
   T* Y[4] = {(T*)h->adj_y[0], (T*)h->adj_y[1], (T*)h->adj_y[2], (T*)h->adj_y[3]};
-  // the stage adjoints accumulate with atomics: cleared here, then by every step's sum kernel
+  // the stage adjoints accumulate: cleared here, then by every step's sum kernel; the tile
+  // buffers of the grid-field adjoints accumulate over the whole sweep and are gathered once
   for (int k = 0; k < 4; ++k)
     if ((rc = launch_fill_zero<T>(Y[k], n, s))) return fail(h, rc, "zero fill launch failed");
+  const bool acc_aux = g_aux && h->vjp_ba && h->vjp_ba_count > 0;
+  if (acc_aux && (rc = launch_fill_zero<double>((double*)h->vjp_ba, (int64_t)h->vjp_ba_count, s)))
+    return fail(h, rc, "zero fill launch failed");
   for (int64_t g = nseg - 1; g >= 0; --g) {
     const int64_t len = std::min<int64_t>(every, nsteps - g * every);
     // recompute the segment from its checkpoint, keeping every stage state and step time
@@ -1049,10 +1059,17 @@ int adjoint_impl(mol_handle* h, const T* p, T* lam, double* g_aux, hipStream_t s
         va.p = p;
         va.tnow = h->seg_t + j;
         va.tofs = tofs[q];
+        va.mode = acc_aux ? (VJP_GATHER_F | VJP_ACC_A) : (VJP_GATHER_F | VJP_GATHER_A);
         if ((rc = launch_vjp<T>(h, va, s))) return rc;
       }
       if ((rc = launch_adjoint_sum<T>(lam, Y[0], Y[1], Y[2], Y[3], n, s))) return fail(h, rc, "adjoint update launch failed");
     }
+  }
+  if (acc_aux) {
+    VArgs<T> va{};
+    va.g_aux = g_aux;
+    va.mode = VJP_GATHER_A;
+    if ((rc = launch_vjp<T>(h, va, s, false))) return rc;
   }
   return 0;
 }
@@ -1473,11 +1490,13 @@ int mol_vjp(mol_handle* h, const void* u, const void* p, double t, const void* l
   if (h->dtype == MOL_F64) {
     VArgs<double> va{(const double*)u, (const double*)lam, nullptr, 1.0, 0.0, (double*)lam_dot, g_aux,
                      (const double*)p, nullptr, t};
+    va.mode = VJP_GATHER_F | VJP_GATHER_A;
     if (launch_fill_zero<double>((double*)lam_dot, h->nunk, s)) return fail(h, MOL_ERR_HIP, "zero fill launch failed");
     return launch_vjp<double>(h, va, s);
   }
   VArgs<float> va{(const float*)u, (const float*)lam, nullptr, 1.0, 0.0, (float*)lam_dot, g_aux,
                   (const float*)p, nullptr, t};
+  va.mode = VJP_GATHER_F | VJP_GATHER_A;
   if (launch_fill_zero<float>((float*)lam_dot, h->nunk, s)) return fail(h, MOL_ERR_HIP, "zero fill launch failed");
   return launch_vjp<float>(h, va, s);
 }

@@ -73,7 +73,9 @@ struct VArgs {
   double tofs;
   T* bf;            // per-tile adjoint buffers of the staged variables
   double* ba;       // per-tile adjoint buffers of the staged grid fields
+  int mode;         // VJP_GATHER_F | VJP_GATHER_A | VJP_ACC_A
 };
+enum { VJP_GATHER_F = 1, VJP_GATHER_A = 2, VJP_ACC_A = 4 };
 
 // What one right-hand-side launch needs: the core-tile kernel arguments and the list of tiles
 // left to the generic kernel.
@@ -192,6 +194,7 @@ struct mol_handle {
   hipFunction_t vjp_gather_fn = nullptr;  // its gather of the per-tile adjoint buffers
   void* vjp_bf = nullptr;                 // per-tile adjoint buffers: staged variables (T)
   void* vjp_ba = nullptr;                 //   and staged grid fields (FP64)
+  size_t vjp_ba_count = 0;                //   (values in vjp_ba)
   int vjp_tiles = 0;
   bool vjp_aux_ok = true;                 // no boundary value reads an auxiliary array
   void* ck_u = nullptr;                   // checkpoints of the last mol_forward_checkpointed run

@@ -1293,8 +1293,8 @@ std::string Gen::geo_variant(const std::string& name, const int* halo, const Usa
     // ghost positions through the transposed closures to the unknowns (atomics)
     o << "  static __device__ __forceinline__ void flush_pos(const T* Ab, const double* Aa, const int q, const int p0,\n"
          "      const int p1, const int p2, T* __restrict__ outp, T* __restrict__ bft, double* __restrict__ bat,\n"
-         "      const int vs) {\n"
-         "    (void)Ab; (void)Aa; (void)bat; (void)vs; (void)outp;\n";
+         "      const int vs, const bool acc) {\n"
+         "    (void)Ab; (void)Aa; (void)bat; (void)vs; (void)outp; (void)acc;\n";
     if (nstf > 0) {
       o << "    int w0 = p0, w1 = p1, w2 = p2;\n";
       for (int a = 0; a < 3; ++a)
@@ -1312,7 +1312,8 @@ std::string Gen::geo_variant(const std::string& name, const int* halo, const Usa
       o << "    if (bat) {\n      const bool in = gidx(p0, p1, p2) >= 0;\n";
       for (int a = 0; a < d_->naux; ++a)
         if (pg_.aslot[a] >= 0)
-          o << "      bat[" << pg_.aslot[a] << " * vs + q] = in ? Aa[" << (long long)pg_.aslot[a] * pg_.vol << " + q] : 0.0;\n";
+          o << "      { double* b = &bat[" << pg_.aslot[a] << " * vs + q]; const double v = in ? Aa["
+            << (long long)pg_.aslot[a] * pg_.vol << " + q] : 0.0; *b = acc ? *b + v : v; }\n";
       o << "    }\n";
     }
     o << "  }\n";
@@ -1406,7 +1407,8 @@ std::string Gen::source_kernel(bool f64, bool vjp) {
     o << "extern \"C\" __global__ void __launch_bounds__(256) mol_vjp_gather(VArgs va) {\n"
          "  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;\n"
          "  (void)va;\n";
-    o << "  if (i < " << nunk << "LL) {\n";
+    o << "  if (i < " << nunk << "LL) {\n"
+         "    if (!(va.mode & 1)) return;\n";
     for (int f = 0; f < d_->nfields; ++f) {
       if (pg_.fslot[f] < 0) continue;
       long long sz = 1;
@@ -1423,7 +1425,7 @@ std::string Gen::source_kernel(bool f64, bool vjp) {
     bool any_aux = false;
     for (int a = 0; a < d_->naux; ++a) any_aux = any_aux || pg_.aslot[a] >= 0;
     if (any_aux) {
-      o << "  if (!va.g_aux) return;\n"
+      o << "  if (!va.g_aux || !(va.mode & 2)) return;\n"
         << "  const i64 j = i - " << nunk << "LL;\n"
         << "  if (j >= " << d_->naux << " * GeoBase::NGRID) return;\n"
         << "  const int a = (int)(j / GeoBase::NGRID), g = (int)(j - (i64)a * GeoBase::NGRID);\n"

@@ -130,18 +130,21 @@ __device__ __forceinline__ U mol_weno5_ref_t(U um2, U um1, U u0, U up1, U up2, T
   const U b2 = T(13) * c3 * c3 / T(12) + c4 * c4 / T(4);
   const U b3 = T(13) * c5 * c5 / T(12) + c6 * c6 / T(4);
   const U q1 = (eps + b1) * (eps + b1), q2 = (eps + b2) * (eps + b2), q3 = (eps + b3) * (eps + b3);
-  const U om1 = T(0.1) / q1, om2 = T(0.6) / q2, om3 = T(0.3) / q3;
-  const U op1 = T(0.3) / q1, op2 = T(0.6) / q2, op3 = T(0.1) / q3;
+  // the six weights share three reciprocals and each normalisation one division
+  const U r1 = T(1) / q1, r2 = T(1) / q2, r3 = T(1) / q3;
+  const U om1 = T(0.1) * r1, om2 = T(0.6) * r2, om3 = T(0.3) * r3;
+  const U op1 = T(0.3) * r1, op2 = T(0.6) * r2, op3 = T(0.1) * r3;
   const U wmd = om1 + om2 + om3, wpd = op1 + op2 + op3;
-  const U hm1 = (T(11) * u0 - T(7) * up1 + T(2) * up2) / T(6);
-  const U hm2 = (T(5) * u0 - up1 + T(2) * um1) / T(6);
-  const U hm3 = (T(2) * u0 + T(5) * um1 - um2) / T(6);
-  const U hp1 = (T(2) * u0 + T(5) * up1 - up2) / T(6);
-  const U hp2 = (T(5) * u0 + T(2) * up1 - um1) / T(6);
-  const U hp3 = (T(11) * u0 - T(7) * um1 + T(2) * um2) / T(6);
-  const U hp = (op1 / wpd) * hp1 + (op2 / wpd) * hp2 + (op3 / wpd) * hp3;
-  const U hm = (om1 / wmd) * hm1 + (om2 / wmd) * hm2 + (om3 / wmd) * hm3;
-  return (hp - hm) * inv_dx;
+  // candidate values times 6 (the 1/6 goes into the final scale)
+  const U hm1 = T(11) * u0 - T(7) * up1 + T(2) * up2;
+  const U hm2 = T(5) * u0 - up1 + T(2) * um1;
+  const U hm3 = T(2) * u0 + T(5) * um1 - um2;
+  const U hp1 = T(2) * u0 + T(5) * up1 - up2;
+  const U hp2 = T(5) * u0 + T(2) * up1 - um1;
+  const U hp3 = T(11) * u0 - T(7) * um1 + T(2) * um2;
+  const U hp = (op1 * hp1 + op2 * hp2 + op3 * hp3) / wpd;
+  const U hm = (om1 * hm1 + om2 * hm2 + om3 * hm3) / wmd;
+  return (hp - hm) * (inv_dx * T(1.0 / 6.0));
 }
 
 __device__ __forceinline__ T mol_weno5(T um2, T um1, T u0, T up1, T up2, T eps, T inv_dx) {
@@ -409,6 +412,8 @@ struct VArgs {
   double tofs;
   T* bf;
   double* ba;
+  int mode;   // 1: gather the variables, 2: gather the grid fields, 4: add the tile kernel's
+              // grid-field adjoints to the buffers instead of storing them
 };
 
 // Floor of a / b for b > 0.
@@ -420,6 +425,14 @@ __device__ __forceinline__ int mol_floordiv(const int a, const int b) { return a
 template <int TL, int H, int NT, int ULO, bool PER, int P>
 __device__ __forceinline__ int mol_cover(const int p, int* ts, int* ls) {
   int n = 0;
+  if (!PER && H <= TL) {
+    // own tile and at most one neighbour on each side
+    const int r = p - ULO, t = mol_floordiv(r, TL), l = r - t * TL;
+    if (l < H && t - 1 >= 0 && t - 1 < NT) { ts[n] = t - 1; ls[n] = l + TL; ++n; }
+    if (t >= 0 && t < NT) { ts[n] = t; ls[n] = l; ++n; }
+    if (l >= TL - H && t + 1 >= 0 && t + 1 < NT) { ts[n] = t + 1; ls[n] = l - TL; ++n; }
+    return n;
+  }
 #pragma unroll
   for (int k = PER ? -2 : 0; k <= (PER ? 2 : 0); ++k) {
     const int pp = p + k * P;
@@ -511,9 +524,10 @@ __device__ __forceinline__ void mol_vjp_tile(T* L, T* Ab, double* Aa, const int 
   const i64 tl = tx + (i64)G::NT0 * (ty + (i64)G::NT1 * tz);
   T* bft = va.bf + tl * (G::NSTF * VOLX);
   double* bat = va.g_aux ? va.ba + tl * (G::NSTA * VOLX) : nullptr;
+  const bool acc = (va.mode & 4) != 0;
   MolWalk<G::EX, G::EY, G::NTHR> w((int)threadIdx.x);
   for (int q = threadIdx.x; q < G::VOL; q += G::NTHR, w.step())
-    G::flush_pos(Ab, Aa, q, o0 + w.x - G::HX, o1 + w.y - G::HY, o2 + w.z - G::HZ, va.out, bft, bat, VOLX);
+    G::flush_pos(Ab, Aa, q, o0 + w.x - G::HX, o1 + w.y - G::HY, o2 + w.z - G::HZ, va.out, bft, bat, VOLX, acc);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -42,7 +42,7 @@ class JPtrs(C.Structure):
 class VArgs(C.Structure):
     _fields_ = [("u", C.c_void_p), ("s1", C.c_void_p), ("s2", C.c_void_p), ("c1", C.c_double), ("c2", C.c_double),
                 ("out", C.c_void_p), ("g_aux", C.c_void_p), ("p", C.c_void_p), ("tnow", C.c_void_p),
-                ("tofs", C.c_double), ("bf", C.c_void_p), ("ba", C.c_void_p)]
+                ("tofs", C.c_double), ("bf", C.c_void_p), ("ba", C.c_void_p), ("mode", C.c_int32)]
 
 
 def _compile(src: str) -> C.CDLL:
@@ -152,6 +152,7 @@ extern "C" __attribute__((visibility("default"))) void run(VArgs va, JPtrs jp, i
     ba = np.zeros(max(1, nblk * volx * _const(src, "static constexpr int NSTA = ", "struct GeoB")))
     va.bf = bf.ctypes.data
     va.ba = ba.ctypes.data
+    va.mode = 3
     items = low.n_unknowns + (len(low.aux) * ng if low.aux else 0)
     lib.run.argtypes = [VArgs, JPtrs, C.c_int, C.c_longlong]
     lib.run(va, jp, nblk, items)
