@@ -231,13 +231,18 @@ int mol_integrate(mol_handle* h, void* u, const void* p, double t0, double dt, i
                   int32_t scheme, int32_t flags, void* stream);
 
 /* ---- adjoint ------------------------------------------------------------------------
- * No counterpart in the reference (it differentiates through ForwardDiff/Zygote on the CPU).
- * Supported right-hand sides: every operator slot centred (MOL_SLOT_CENTERED), mixed or a
- * nonlinear Laplacian, programs built from variables, slots, auxiliary arrays, coordinates,
- * time and parameters; MOL_ERR_UNSUPPORTED otherwise.
+ * Replaces: reverse-mode AD of the generated f (what SciMLSensitivity's adjoint methods
+ * obtain by differentiating the RuntimeGeneratedFunction of src/MOL_discretization.jl:171-201
+ * with Zygote/ReverseDiff on the CPU); the reference itself ships no hand-written adjoint.
+ * Every operator slot kind and every program of the generated code is supported (the reverse
+ * sweep is generated from the same IR); batched handles are not, and a grid-field gradient
+ * through a boundary value that reads a grid field is MOL_ERR_UNSUPPORTED.
  *
  * mol_vjp: lam_dot = (df/du)^T lam, and g_aux[a*ngrid + node] += (df/daux_a)^T lam in FP64
- * (g_aux may be NULL).  u, lam, lam_dot: unknown vectors; g_aux: naux full-grid FP64 arrays. */
+ * (g_aux may be NULL).  u, lam, lam_dot: unknown vectors; g_aux: naux full-grid FP64 arrays.
+ * Each tile writes its tile + halo adjoints to its own buffer and a gather kernel adds the
+ * buffers of the tiles covering each node in a fixed order; only adjoints of ghost nodes reach
+ * the unknowns with atomics (through the transposed closures). */
 int mol_vjp(mol_handle* h, const void* u, const void* p, double t, const void* lam, void* lam_dot,
             double* g_aux, void* stream);
 
