@@ -3,12 +3,14 @@
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--graph 0|1] [--no-cpu] [--quick]
 
 Headline (`value`): stencil RHS Gpoint/s of the 2D Brusselator (512^2 unknowns per variable,
-FP64, periodic, 2nd-order Laplacian) driven by the device-resident RK4 stage loop at
-dt = 1e-5.  One step = one classic RK4 step = 4 right-hand sides over all 512^2 points (a
-point is one unknown grid node with both variables); value = points * 4 * K / elapsed / 1e9.
+FP64, periodic, 2nd-order Laplacian) driven by the device-resident RK4 loop at dt = 1e-5.
+One step = one classic RK4 step = 4 right-hand sides over all 512^2 points (a point is one
+unknown grid node with both variables); value = points * 4 * K / elapsed / 1e9.  The step
+runs as one fused launch (the four stages on a tile with a 4-node halo in LDS) where the
+system allows it; `roofline.stage_kernels` reports the four-launches-per-step path too.
 
 The same JSON line carries the other parts of the metric:
-  roofline     the RK4 stage kernel of the timed region against HBM peak;
+  roofline     the dominant kernel of the timed region (the fused step kernel) against HBM peak;
   weno5        one FP64 WENO5 Burgers RHS at 1024^2 (and 4096^2, beyond the 256 MB MALL), and the
                1024^2 SSPRK3 solve (CFL 0.4, 500 steps) in FP64, FP32 + Kahan and plain FP32 with
                the FP32 errors against FP64 and the FP64 error against the oracle;
@@ -39,6 +41,9 @@ HBM_PEAK_GBS = 8000.0
 # profiles/brusselator512_rk4_stage_kernel.md.  Below the 17.8 MB of algorithmic bytes: halo
 # re-reads and the stage vectors written by one launch and read by the next hit in L2.
 STAGE_TRAFFIC_BYTES = (3960.27 * 2.0 + 5869.53 * 1.0) * 1024
+# The same measurement for the fused RK4 step kernel (one launch per step): FETCH_SIZE 2372.6
+# KiB, WRITE_SIZE 4096.0 KiB per launch (8.39 MB algorithmic: the halo re-reads hit in L2).
+FUSED_TRAFFIC_BYTES = (2372.6 * 2.0 + 4096.0 * 1.0) * 1024
 
 
 def events():
@@ -354,6 +359,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--unfused", action="store_true", help="time the four stage launches per step only")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only (no weno/adjoint/batch sections)")
     ap.add_argument("--section", default="all",
@@ -401,28 +407,48 @@ def main():
     stream = torch.cuda.current_stream()
     u = torch.tensor(prob.u0, dtype=torch.float64, device="cuda")
     u_start = u.clone()
-    prob.integrate(u, 0.0, dt, max(args.warmup, 1), molgpu.RK4, graph=bool(args.graph))
-    torch.cuda.synchronize()
-    u.copy_(u_start)
-    torch.cuda.synchronize()
-    ev0, ev1 = events()
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    prob.integrate(u, 0.0, dt, args.steps, molgpu.RK4, graph=bool(args.graph))
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = ev0.elapsed_time(ev1)
+    fused = prob.plan()["fused_tiles"] > 0 and not args.unfused
+
+    def timed(use_fused):
+        u.copy_(u_start)
+        prob.integrate(u, 0.0, dt, max(args.warmup, 1), molgpu.RK4, graph=bool(args.graph), fused=use_fused)
+        torch.cuda.synchronize()
+        u.copy_(u_start)
+        torch.cuda.synchronize()
+        ev0, ev1 = events()
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        prob.integrate(u, 0.0, dt, args.steps, molgpu.RK4, graph=bool(args.graph), fused=use_fused)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1), time.perf_counter() - t0
+
+    # the stage-kernel path (four launches per step) first, for the comparison line
+    ms_st, _ = timed(False)
+    ms, wall = timed(fused)
     ms_per_step = ms / args.steps
     value = points * 4 * args.steps / (ms / 1e3) / 1e9
 
-    # dominant kernel of the timed region: the generated RHS kernel in RK4 stage mode, four
-    # launches per step (plus a one-thread clock kernel).  Algorithmic bytes per step and point
-    # (2 variables, FP64): stage 1 reads u, writes y1 and acc (48 B); stages 2-3 read y, u, acc,
-    # write y', acc (80 B each); stage 4 reads y3, u, acc and writes u (64 B): 272 B per step.
-    launch_us = ms * 1e3 / (4 * args.steps)
-    stage_bytes = points * 272 // 4
-    achieved = stage_bytes / (launch_us * 1e-6) / 1e9
+    # Stage-kernel path: the generated RHS kernel in RK4 stage mode, four launches per step.
+    # Algorithmic bytes per step and point (2 variables, FP64): stage 1 reads u, writes y1 and acc
+    # (48 B); stages 2-3 read y, u, acc, write y', acc (80 B each); stage 4 reads y3, u, acc and
+    # writes u (64 B): 272 B per step, 68 B per launch on average.
+    st_us = ms_st * 1e3 / (4 * args.steps)
+    st_bytes = points * 272 // 4
+    st_ach = st_bytes / (st_us * 1e-6) / 1e9
+    stage = {"kernel": "mol_rhs_tiles (generated; RK4 stage epilogue)", "avg_us": st_us, "bytes_per_launch": st_bytes,
+             "achieved": st_ach, "frac": st_ach / HBM_PEAK_GBS, "traffic": STAGE_TRAFFIC_BYTES,
+             "value": points * 4 * args.steps / (ms_st / 1e3) / 1e9}
+    if fused:
+        # Fused path: one launch per step reads u (with a 4-node halo, from L2) and writes u:
+        # 32 B per point algorithmic (2 variables, FP64).
+        launch_us = ms * 1e3 / args.steps
+        bytes_launch = points * 32
+        kname = "mol_rk4_fused (generated; whole RK4 step per launch)"
+        traffic = FUSED_TRAFFIC_BYTES
+    else:
+        launch_us, bytes_launch, kname, traffic = st_us, st_bytes, stage["kernel"], STAGE_TRAFFIC_BYTES
+    achieved = bytes_launch / (launch_us * 1e-6) / 1e9
     rhs_us = rhs_time_us(prob, u_start, 200)
     rhs_ach = points * 32 / (rhs_us * 1e-6) / 1e9
 
@@ -441,11 +467,11 @@ def main():
         "data": "synthetic: the configured initial condition u=22(y(1-y))^1.5, v=27(x(1-x))^1.5",
         "config": {"workload": "brusselator2d periodic 512x512 unknowns/variable, 2 variables, approx_order 2, "
                                "RK4 fixed dt=1e-5", "points": points, "rhs_per_step": 4,
-                   "graph": bool(args.graph)},
+                   "graph": bool(args.graph), "fused_step": fused},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": STAGE_TRAFFIC_BYTES,
-                     "kernel": "mol_rhs_tiles (generated; RK4 stage epilogue)", "avg_us": launch_us,
-                     "bytes_per_launch": stage_bytes,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": kname, "avg_us": launch_us, "bytes_per_launch": bytes_launch,
+                     "stage_kernels": stage,
                      "rhs_only": {"avg_us": rhs_us, "bytes_per_launch": points * 32, "achieved": rhs_ach,
                                   "frac": rhs_ach / HBM_PEAK_GBS}},
         "wall_s": wall,

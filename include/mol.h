@@ -214,19 +214,22 @@ int mol_set_tile(mol_handle* h, int32_t tx, int32_t ty, int32_t tz);
 
 /* Launch plan: info[0..2] tile shape, info[3] core tiles, info[4] generic tiles, info[5] WENO tiles,
  * info[6] 1 when the core tiles run the generated kernel, info[7] 1 when the IR fits the
- * interpreter's core kernel. */
+ * interpreter's core kernel, info[8] tiles of the fused RK4 step kernel (0: none). */
 int mol_get_plan(const mol_handle* h, int32_t* info, int32_t ninfo);
 
 /* Bind auxiliary full-grid arrays (device pointers, naux of them, each prod(n) values). */
 int mol_set_aux(mol_handle* h, void* const* aux);
 
 typedef enum { MOL_RK4 = 0, MOL_SSPRK33 = 1 } mol_scheme;
-typedef enum { MOL_FLAG_KAHAN = 1, MOL_FLAG_GRAPH = 2 } mol_flags;
+typedef enum { MOL_FLAG_KAHAN = 1, MOL_FLAG_GRAPH = 2, MOL_FLAG_UNFUSED = 4 } mol_flags;
 
 /* Replaces: solve(prob, RK4() | SSPRK33(); dt, adaptive = false) of OrdinaryDiffEq as used in
  * benchmark/weno/suite.jl:50-55 — the fixed-step stage loop, device-resident.  u is advanced in
  * place from t0 by nsteps steps of size dt.  MOL_FLAG_KAHAN: compensated (Kahan) accumulation of
- * the step update (FP32 state).  MOL_FLAG_GRAPH: stages captured once in a hipGraph and replayed. */
+ * the step update (FP32 state).  MOL_FLAG_GRAPH: stages captured once in a hipGraph and replayed.
+ * Periodic 1D/2D systems of reach 1 run RK4 (without Kahan) as one fused launch per step (the
+ * four stages on a tile with a 4-node halo, same arithmetic as the stage launches);
+ * MOL_FLAG_UNFUSED runs the four stage launches instead. */
 int mol_integrate(mol_handle* h, void* u, const void* p, double t0, double dt, int64_t nsteps,
                   int32_t scheme, int32_t flags, void* stream);
 
@@ -273,8 +276,9 @@ void mol_destroy(mol_handle* h);
  * buflen bytes (NUL-terminated) into buf (may be NULL) and returns the source length, or a
  * negative mol_status (message in mol_last_error(NULL)); tile = tx,ty,tz or NULL for the default.
  * flags: MOL_CODEGEN_COMPILE also compiles it with hiprtc for gfx950 and fails when that does;
- * MOL_CODEGEN_VJP returns the reverse-mode (vector-Jacobian product) kernel instead. */
-enum { MOL_CODEGEN_COMPILE = 1, MOL_CODEGEN_VJP = 2 };
+ * MOL_CODEGEN_VJP returns the reverse-mode (vector-Jacobian product) kernel instead,
+ * MOL_CODEGEN_FUSED the fused RK4 step kernel (MOL_ERR_UNSUPPORTED when the system has none). */
+enum { MOL_CODEGEN_COMPILE = 1, MOL_CODEGEN_VJP = 2, MOL_CODEGEN_FUSED = 4 };
 int64_t mol_codegen(const mol_desc* d, const int32_t* tile, char* buf, int64_t buflen, int32_t flags);
 
 /* Library / device information (for test harnesses). */

@@ -571,6 +571,74 @@ int kernel_threads(const JitGeom& g, bool vjp) {
   return vjp ? std::max(256, n) : n;
 }
 
+// Fused RK4 step kernel for periodic 1D/2D systems of reach 1 (MOL_FUSED=0 disables it): the
+// tile + 4-node halo of u and two stage buffers in LDS, one launch per step.
+bool fused_geometry(mol_handle* h, const mol_desc* d, const KArgs<double>& ka, const KCore<double>& kc, int ncore,
+                    JitGeom& g, std::string& why) {
+  if (d->ndim > 2 || d->nclosures > 0) { why = "not a periodic 1D/2D system"; return false; }
+  for (int a = 0; a < d->ndim; ++a)
+    if (!ka.periodic[a] || ka.halo[a] > 1) { why = "not periodic with reach 1"; return false; }
+  static const int cand2[][3] = {{32, 16, 1}, {32, 8, 1}, {64, 8, 1}, {16, 16, 1}, {16, 8, 1}};
+  static const int cand1[][3] = {{256, 1, 1}, {128, 1, 1}, {64, 1, 1}};
+  const int (*cand)[3] = d->ndim == 2 ? cand2 : cand1;
+  const int nc = d->ndim == 2 ? 5 : 3;
+  int best = -1;
+  int64_t best_n = -1;
+  for (int i = 0; i < nc; ++i) {
+    int64_t vol = 1, tv = 1, nt = 1;
+    for (int a = 0; a < 3; ++a) {
+      const int ext = ka.uhi[a] - ka.ulo[a] + 1;
+      const int tl = a < d->ndim ? std::max(1, std::min(cand[i][a], ext)) : 1;
+      vol *= tl + 8 * (a < d->ndim ? ka.halo[a] : 0);
+      tv *= tl;
+      nt *= (ext + tl - 1) / tl;
+    }
+    const size_t lds = (size_t)(3 * vol + tv) * d->nfields * h->esize();
+    if (lds > 65536) continue;
+    if (nt >= 256) { best = i; break; }
+    if (nt > best_n) { best_n = nt; best = i; }
+  }
+  if (best < 0) { why = "tile does not fit LDS"; return false; }
+  g = geom_of(ka, kc, ncore);
+  g.vol = 1;
+  for (int a = 0; a < 3; ++a) {
+    const int ext = ka.uhi[a] - ka.ulo[a] + 1;
+    g.tile[a] = a < d->ndim ? std::max(1, std::min(cand[best][a], ext)) : 1;
+    if (h->tile_req[0] > 0) g.tile[a] = a < d->ndim ? std::max(1, std::min(h->tile_req[a], ext)) : 1;
+    int ft[3];
+    const char* fe = std::getenv("MOL_FUSED_TILE");
+    if (fe && std::sscanf(fe, "%d,%d,%d", &ft[0], &ft[1], &ft[2]) == 3 && ft[0] > 0)
+      g.tile[a] = a < d->ndim ? std::max(1, std::min(ft[a], ext)) : 1;
+    g.ntile[a] = (ext + g.tile[a] - 1) / g.tile[a];
+    g.ext[a] = g.tile[a] + 2 * g.halo[a];
+    g.vol *= g.ext[a];
+  }
+  // two threads per tile node (every stage region is larger than the tile), at most 1024
+  g.nthr = std::max(256, std::min(1024, (2 * g.tile[0] * g.tile[1] * g.tile[2] + 63) / 64 * 64));
+  if (const char* ne = std::getenv("MOL_FUSED_NTHR")) g.nthr = (std::max(64, std::min(1024, std::atoi(ne))) + 63) / 64 * 64;
+  return true;
+}
+
+void setup_fused(mol_handle* h, const mol_desc* d, const KArgs<double>& ka, const KCore<double>& kc, int ncore) {
+  h->fused_fn = nullptr;
+  h->fused_tiles = 0;
+  h->fused_reason.clear();
+  const char* e = std::getenv("MOL_FUSED");
+  if (e && e[0] == '0') { h->fused_reason = "disabled (MOL_FUSED=0)"; return; }
+  JitGeom g;
+  if (!fused_geometry(h, d, ka, kc, ncore, g, h->fused_reason)) return;
+  std::string err;
+  const std::string src = jit_fused_source(d, g, err);
+  if (src.empty()) { h->fused_reason = err; return; }
+  if (jit_kernel(src, "mol_rk4_fused", h->device, &h->fused_fn, err)) {
+    h->fused_fn = nullptr;
+    h->fused_reason = err;
+    return;
+  }
+  h->fused_tiles = g.ntile[0] * g.ntile[1] * g.ntile[2];
+  h->fused_nthr = g.nthr;
+}
+
 int setup_paths(mol_handle* h, const mol_desc* d) {
   std::vector<int32_t> t64, t32;
   int n64 = 0, n32 = 0;
@@ -615,6 +683,7 @@ int setup_paths(mol_handle* h, const mol_desc* d) {
       h->jit_grid =
           src.find("mol_tiles_persistent<") != std::string::npos ? persistent_grid(h->jit_fn, ntl, jg.nthr) : ntl;
     if (!h->jit_fn && retiled) refill(0);   // back to the interpreter's tiles
+    if (h->jit_fn) setup_fused(h, d, h->ka64, h->kc64, n64);
   } else {
     h->jit_reason = "code generation disabled (MOL_JIT=0)";
   }
@@ -689,6 +758,28 @@ int enqueue_steps(mol_handle* h, T* u, const T* p, double dt, int64_t nsteps, in
   T* y2 = (T*)h->w_y2;
   T* acc = (T*)h->w_acc;
   T* comp = (T*)h->w_comp;
+  if (scheme == MOL_RK4 && !kahan && h->fused_fn && !h->no_fuse) {
+    // one launch per step, the state alternating between u and the workspace
+    T* src = u;
+    T* dst = y1;
+    for (int64_t step = 0; step < nsteps; ++step) {
+      Epi<T> e{};
+      e.tnow = h->d_tnow + h->clk;
+      e.tnext = h->d_tnow + (1 - h->clk);
+      e.tadv = dt;
+      e.p = p;
+      JitPtrs<T> jp = L.jp;
+      const T* sp = src;
+      void* args[] = {(void*)&sp, (void*)&dst, (void*)&e, (void*)&jp};
+      if (hipModuleLaunchKernel(h->fused_fn, h->fused_tiles, 1, 1, h->fused_nthr, 1, 1, 0, s, args, nullptr) !=
+          hipSuccess)
+        return fail(h, MOL_ERR_HIP, "fused RK4 launch failed");
+      h->clk ^= 1;
+      std::swap(src, dst);
+    }
+    if (src != u) HIPCHK(h, hipMemcpyAsync(u, src, (size_t)h->nunk * sizeof(T), hipMemcpyDeviceToDevice, s));
+    return 0;
+  }
   for (int64_t step = 0; step < nsteps; ++step) {
     double* tcur = h->d_tnow + h->clk;
     double* tnxt = h->d_tnow + (1 - h->clk);
@@ -734,6 +825,7 @@ int integrate_impl(mol_handle* h, T* u, const T* p, double t0, double dt, int64_
   int rc = ensure_workspace(h);
   if (rc) return rc;
   const bool kahan = (flags & MOL_FLAG_KAHAN) != 0;
+  h->no_fuse = (flags & MOL_FLAG_UNFUSED) != 0;
   HIPCHK(h, hipMemcpyAsync(h->d_tnow + h->clk, &t0, sizeof(double), hipMemcpyHostToDevice, s));
   if (kahan) HIPCHK(h, hipMemsetAsync(h->w_comp, 0, (size_t)h->nunk * h->esize(), s));
   if (!(flags & MOL_FLAG_GRAPH) || nsteps < 4) {
@@ -1391,10 +1483,10 @@ int mol_get_plan(const mol_handle* h, int32_t* info, int32_t ninfo) {
     for (int i = 0; i < ninfo && i < 8; ++i) info[i] = b[i];
     return 0;
   }
-  int32_t v[8] = {h->ka64.tile[0], h->ka64.tile[1], h->ka64.tile[2], h->ncore, h->nbound,
+  int32_t v[9] = {h->ka64.tile[0], h->ka64.tile[1], h->ka64.tile[2], h->ncore, h->nbound,
                   h->use_weno ? h->nweno : 0, h->jit_fn != nullptr && !h->use_weno ? 1 : 0,
-                  h->core_interp_ok ? 1 : 0};
-  for (int i = 0; i < ninfo && i < 8; ++i) info[i] = v[i];
+                  h->core_interp_ok ? 1 : 0, h->fused_fn ? h->fused_tiles : 0};
+  for (int i = 0; i < ninfo && i < 9; ++i) info[i] = v[i];
   return 0;
 }
 
@@ -1565,8 +1657,14 @@ int64_t mol_codegen(const mol_desc* d, const int32_t* tile, char* buf, int64_t b
     build_core<double>(&tmp, d, tmp.ka64, tmp.kc64, gen, ncore);
   }
   std::string err;
-  const std::string src = (flags & MOL_CODEGEN_VJP) ? jit_vjp_source(d, vjp_geometry(&tmp, d, nullptr), err)
-                                                    : jit_source(d, fwd_geom(tmp, ncore), err);
+  std::string src;
+  if (flags & MOL_CODEGEN_FUSED) {
+    JitGeom g;
+    if (fused_geometry(&tmp, d, tmp.ka64, tmp.kc64, ncore, g, err)) src = jit_fused_source(d, g, err);
+  } else {
+    src = (flags & MOL_CODEGEN_VJP) ? jit_vjp_source(d, vjp_geometry(&tmp, d, nullptr), err)
+                                    : jit_source(d, fwd_geom(tmp, ncore), err);
+  }
   if (src.empty()) return fail(nullptr, MOL_ERR_UNSUPPORTED, err);
   if (flags & MOL_CODEGEN_COMPILE) {
     std::vector<char> code;

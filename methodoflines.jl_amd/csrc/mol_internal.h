@@ -37,6 +37,8 @@ struct JitPtrs {
 // Code generation and in-process compilation of the tile kernel (mol_jit.cpp).
 std::string jit_source(const mol_desc* d, const JitGeom& g, std::string& err);
 std::string jit_vjp_source(const mol_desc* d, const JitGeom& g, std::string& err);
+// Fused RK4 step kernel (mol_rk4_fused); empty with err set when the system does not qualify.
+std::string jit_fused_source(const mol_desc* d, const JitGeom& g, std::string& err);
 // LDS bytes per tile + halo position of the generated forward (vjp = false) or reverse-mode
 // kernel: the arrays it stages (and their adjoints); 0 when the system cannot be generated.
 size_t jit_lds_per_pos(const mol_desc* d, const JitGeom& g, size_t esize, bool vjp);
@@ -189,6 +191,11 @@ struct mol_handle {
   std::vector<int64_t> inst_off;          // first unknown of each instance
   int jit_grid = 0;                       // work-groups of the persistent generated RHS kernel
   int jit_nthr = 256;                     // threads per work-group of the generated RHS kernel
+  hipFunction_t fused_fn = nullptr;       // generated fused RK4 step (periodic 1D/2D systems)
+  int fused_tiles = 0;
+  int fused_nthr = 256;
+  std::string fused_reason;               // why there is no fused kernel
+  bool no_fuse = false;                   // this integration runs the stage kernels (MOL_FLAG_UNFUSED)
   int vjp_nthr = 256;                     // threads per work-group of the generated VJP kernel
   hipFunction_t vjp_fn = nullptr;         // generated vector-Jacobian product kernel
   hipFunction_t vjp_gather_fn = nullptr;  // its gather of the per-tile adjoint buffers

@@ -107,6 +107,9 @@ class Gen {
 
   std::string source(bool f64);       // forward kernel mol_rhs_tiles
   std::string source_vjp(bool f64);   // reverse-mode kernel mol_vjp_tiles
+  // One whole classic RK4 step per launch (mol_rk4_fused): periodic systems in 1D/2D, every
+  // variable staged; empty (with error) when the system does not qualify.
+  std::string source_fused(bool f64);
   // Parametric class body for batched instances: weights, constants, grid sizes and boxes are
   // read from the instance header `hd`; the text is identical for instances of one structure.
   std::string source_class(const std::string& name);
@@ -122,6 +125,7 @@ class Gen {
   bool core_ = true;                  // building the interior-rows variant
   bool ok_ = true;
   bool param_ = false;                // parametric (batched) printing
+  bool fused_ = false;                // fused RK4 step printing (coordinates from LDS)
 
   // instance-dependent numbers: literal in a specialised kernel, header loads in a class body
   std::string pw(double v, long long idx) const { return param_ ? "(T)hd.W[" + istr(idx) + "]" : lit(v); }
@@ -266,9 +270,17 @@ class Gen {
     return u;
   }
  private:
-  std::string coord_node(int d) const { return "(T)jp.coord[" + istr(d) + "][I" + istr(d) + " - 1]"; }
+  std::string coord_node(int d) const {
+    // fused step kernel: coordinates of the tile + halo staged in LDS, indexed by the
+    // unwrapped node index relative to the first staged position
+    if (fused_) return "(T)mol_fc[" + istr(d) + "][I" + istr(d) + " - mol_fo[" + istr(d) + "]]";
+    return "(T)jp.coord[" + istr(d) + "][I" + istr(d) + " - 1]";
+  }
   std::string coord_tap(int d, int off) const {
     if (off == 0) return coord_node(d);
+    if (fused_)
+      return "(T)mol_fc[" + istr(d) + "][I" + istr(d) + (off > 0 ? " + " : " - ") + istr(off < 0 ? -off : off) +
+             " - mol_fo[" + istr(d) + "]]";
     const std::string k = "I" + istr(d) + (off > 0 ? " + " + istr(off) : " - " + istr(-off));
     if (g_.periodic[d]) return "(T)jp.coord[" + istr(d) + "][mol_wrap(" + k + ", " + NN(d) + ") - 1]";
     // ghost positions past the ends read the end coordinate (never an address outside the array)
@@ -1444,6 +1456,65 @@ std::string Gen::source_kernel(bool f64, bool vjp) {
   return o.str();
 }
 
+// Fused RK4 step: the tile + 4 * reach halo of u is staged once; the four stages run on
+// regions shrinking by the reach (tile + 3r, + 2r, + r, tile) with the stage states in LDS,
+// and only the tile's new state is written.  The update arithmetic is the stage-kernel
+// epilogue's, so the step equals four stage launches.
+std::string Gen::source_fused(bool f64) {
+  for (int a = 0; a < d_->ndim; ++a)
+    if (!g_.periodic[a]) { bad("fused RK4: axis " + istr(a) + " is not periodic"); return ""; }
+  if (d_->ndim > 2) { bad("fused RK4: more than two dimensions"); return ""; }
+  if (d_->nclosures > 0) { bad("fused RK4: ghost closures"); return ""; }
+  for (int f = 1; f < d_->nfields; ++f)
+    for (int a = 0; a < 3; ++a)
+      if (g_.lo[f][a] != g_.lo[0][a] || g_.hi[f][a] != g_.hi[0][a]) { bad("fused RK4: unequal unknown boxes"); return ""; }
+  std::vector<int> out_core;
+  int r_core = -1;
+  fused_ = true;
+  build_node(true, out_core, r_core);
+  if (!ok_) return "";
+  Usage us = usage();
+  for (int k = d_->nfields; k < d_->nfields + d_->naux; ++k)
+    if (us.staged[k]) { bad("fused RK4: a grid field read away from the node"); return ""; }
+  for (int f = 0; f < d_->nfields; ++f) us.staged[f] = true;
+  int halo[3] = {0, 0, 0}, reach[3] = {0, 0, 0};
+  for (int a = 0; a < d_->ndim; ++a) {
+    reach[a] = us.reach[a];
+    halo[a] = 4 * us.reach[a];
+  }
+  std::ostringstream o;
+  o << "// generated fused RK4 step of one PDE system\n";
+  o << "#define MOL_T " << (f64 ? "double" : "float") << "\n";
+  o << "#include \"mol_jit_prelude.h\"\n\n";
+  o << layout_asserts(f64);
+  // coordinates of the tile + halo (per axis) and the unwrapped index of its first position
+  int emax = 1;
+  for (int a = 0; a < 3; ++a) emax = std::max(emax, g_.tile[a] + 2 * halo[a]);
+  o << "__shared__ double mol_fc[3][" << emax << "];\n__shared__ int mol_fo[3];\n\n";
+  const std::string stage = stage_fn();
+  o << "struct GeoBase {\n" << geo_common() << stage << "};\n\n";
+  std::string var = geo_variant("GeoF", halo, us, true, r_core, out_core, false);
+  // stage regions: the reach per axis
+  const std::string close = "};\n\n";
+  var = var.substr(0, var.size() - close.size());
+  o << var;
+  o << "  static constexpr int RX = " << reach[0] << ", RY = " << reach[1] << ", RZ = " << reach[2] << ";\n";
+  o << close;
+  o << "extern \"C\" __global__ void __launch_bounds__(" << g_.nthr
+    << ") mol_rk4_fused(const T* __restrict__ u, T* __restrict__ unew, Epi ep, JPtrs jp) {\n"
+       "  __shared__ T U[GeoF::NST * GeoF::VOL];\n  __shared__ T Ya[GeoF::NST * GeoF::VOL];\n"
+       "  __shared__ T Yb[GeoF::NST * GeoF::VOL];\n"
+       "  __shared__ T ACC[GeoF::NF * GeoF::TX * GeoF::TY * GeoF::TZ];\n"
+       "  int b = mol_xcd_tile<GeoBase::NBLK>((int)blockIdx.x);\n"
+       "  const int tx = b % GeoBase::NT0;\n  b /= GeoBase::NT0;\n"
+       "  const int ty = b % GeoBase::NT1;\n  const int tz = b / GeoBase::NT1;\n"
+       "  const double t = ep.tnow ? ep.tnow[0] : 0.0;\n"
+       "  mol_advance_clock(ep);\n"
+       "  mol_rk4_fused_tile<GeoF>(U, Ya, Yb, ACC, &mol_fc[0][0], (int)(sizeof(mol_fc[0]) / sizeof(double)), mol_fo,\n"
+       "                           tx, ty, tz, u, unew, ep, jp, t);\n}\n";
+  return o.str();
+}
+
 std::string Gen::source(bool f64) { return source_kernel(f64, false); }
 std::string Gen::source_vjp(bool f64) { return source_kernel(f64, true); }
 
@@ -1518,6 +1589,13 @@ std::mutex g_jit_mu;
 std::map<std::string, CacheEntry> g_jit_cache;   // key: generated source
 
 }  // namespace
+
+std::string jit_fused_source(const mol_desc* d, const JitGeom& g, std::string& err) {
+  Gen gen(d, g);
+  std::string s = gen.source_fused(d->dtype == MOL_F64);
+  err = gen.error;
+  return s;
+}
 
 std::string jit_source(const mol_desc* d, const JitGeom& g, std::string& err) {
   Gen gen(d, g);

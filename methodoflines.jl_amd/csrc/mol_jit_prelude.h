@@ -395,6 +395,87 @@ __device__ __forceinline__ void mol_tiles_persistent(T* L, const T* __restrict__
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// Fused classic RK4 step (periodic systems): stage S of tile (o0, o1, o2) on the tile grown by
+// (4 - S) reach per axis.  Lin holds the stage input (u, y1, y2, y3), Lout receives the stage
+// output y_S = u + c_S dt k_S, ACC the running k1 + 2 k2 + 2 k3 on the tile; stage 4 writes
+// u + dt/6 k4 + dt/6 ACC.  The arithmetic is that of the stage kernels' epilogue (mode 1 with
+// ca = 1, acc_f = 1 or 2, acc_keep = 0 or 1), so a fused step equals four stage launches.
+template <class G, int S>
+__device__ __forceinline__ void mol_rk4_stage(const T* Lin, T* Lout, const T* U, T* ACC, const int o0, const int o1,
+                                              const int o2, const T* __restrict__ u, T* __restrict__ unew,
+                                              const Epi& ep, const JPtrs& jp, const double t, const double dt) {
+  constexpr int M = 4 - S;
+  constexpr int W0 = G::TX + 2 * M * G::RX, W1 = G::TY + 2 * M * G::RY, W2 = G::TZ + 2 * M * G::RZ;
+  constexpr int NV = W0 * W1 * W2, TVOL = G::TX * G::TY * G::TZ;
+  const double ts = t + (S == 1 ? 0.0 : (S == 4 ? dt : dt / 2));
+  const double cf = S == 1 || S == 2 ? dt / 2 : (S == 3 ? dt : dt / 6);
+  for (int q = threadIdx.x; q < NV; q += G::NTHR) {
+    const int z = q / (W0 * W1);
+    const int r = q - z * (W0 * W1);
+    const int y = r / W0;
+    const int x = r - y * W0;
+    const int lx = x - M * G::RX, ly = y - M * G::RY, lz = z - M * G::RZ;   // tile-relative
+    const int b = (lx + G::HX) + G::EX * ((ly + G::HY) + G::EY * (lz + G::HZ));
+    const int I0 = o0 + lx, I1 = o1 + ly, I2 = o2 + lz;
+    // unwrapped indices: coordinates come from the staged copy, grid fields wrap in gidx
+    T out[G::NF];
+    G::node(Lin, b, I0, I1, I2, ts, ep.p, u, jp, out);
+    const int tl = lx + G::TX * (ly + G::TY * lz);
+    const bool in_tile = M == 0 || (lx >= 0 && lx < G::TX && ly >= 0 && ly < G::TY && lz >= 0 && lz < G::TZ);
+#pragma unroll
+    for (int f = 0; f < G::NF; ++f) {
+      const T uv = U[f * G::VOL + b];
+      T v = (T)cf * out[f];
+      v += uv;
+      if (S < 4) {
+        Lout[f * G::VOL + b] = v;
+        if (in_tile) {
+          T a = (S == 1 ? T(1) : T(2)) * out[f];
+          if (S > 1) a += ACC[f * TVOL + tl];
+          ACC[f * TVOL + tl] = a;
+        }
+      } else {
+        v += (T)cf * ACC[f * TVOL + tl];
+        if (I0 <= G::UHI0 && I1 <= G::UHI1 && I2 <= G::UHI2) unew[G::uidx(f, I0, I1, I2)] = v;
+      }
+    }
+  }
+}
+
+template <class G>
+__device__ __forceinline__ void mol_rk4_fused_tile(T* U, T* Ya, T* Yb, T* ACC, double* fc, const int fcs, int* fo,
+                                                   const int tx, const int ty, const int tz, const T* __restrict__ u,
+                                                   T* __restrict__ unew, const Epi& ep, const JPtrs& jp,
+                                                   const double t) {
+  const int o0 = G::ULO0 + tx * G::TX, o1 = G::ULO1 + ty * G::TY, o2 = G::ULO2 + tz * G::TZ;
+  const double dt = ep.tadv;
+  mol_stage_tile<G>(U, o0, o1, o2, mol_stage_mode<G>(tx, ty, tz), t, ep.p, u, jp);
+  // coordinates of the tile + halo positions (periodic images wrapped), per axis
+  {
+    const int org[3] = {o0 - G::HX, o1 - G::HY, o2 - G::HZ};
+    const int ext[3] = {G::EX, G::EY, G::EZ};
+    const int nn[3] = {G::N0, G::N1, G::N2};
+    const bool per[3] = {G::PER0 != 0, G::PER1 != 0, G::PER2 != 0};
+    for (int a = (int)threadIdx.x; a < 3; a += G::NTHR) fo[a] = org[a];
+    for (int a = 0; a < 3; ++a)
+      if (jp.coord[a])
+        for (int e = threadIdx.x; e < ext[a]; e += G::NTHR) {
+          int p = org[a] + e;
+          if (per[a]) p = mol_wrap(mol_wrap(p, nn[a]), nn[a]);
+          fc[a * fcs + e] = jp.coord[a][min(max(p, 1), nn[a]) - 1];
+        }
+  }
+  __syncthreads();
+  mol_rk4_stage<G, 1>(U, Ya, U, ACC, o0, o1, o2, u, unew, ep, jp, t, dt);
+  __syncthreads();
+  mol_rk4_stage<G, 2>(Ya, Yb, U, ACC, o0, o1, o2, u, unew, ep, jp, t, dt);
+  __syncthreads();
+  mol_rk4_stage<G, 3>(Yb, Ya, U, ACC, o0, o1, o2, u, unew, ep, jp, t, dt);
+  __syncthreads();
+  mol_rk4_stage<G, 4>(Ya, nullptr, U, ACC, o0, o1, o2, u, unew, ep, jp, t, dt);
+}
+
 // Arguments of the generated vector-Jacobian product (layout identical to mol::VArgs<T>).
 // The seed is lam = c1*s1 + c2*s2 (s2 may be null); out receives (df/du)^T lam and g_aux the
 // FP64 auxiliary-array gradients (may be null); both are accumulated, not overwritten.  bf and

@@ -17,7 +17,7 @@ MOL_MAX_FIELDS = 8
 MOL_MAX_AUX = 8
 MOL_F64, MOL_F32 = 0, 1
 MOL_RK4, MOL_SSPRK33 = 0, 1
-MOL_FLAG_KAHAN, MOL_FLAG_GRAPH = 1, 2
+MOL_FLAG_KAHAN, MOL_FLAG_GRAPH, MOL_FLAG_UNFUSED = 1, 2, 4
 
 LIB_NAME = "libmolgpu.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)

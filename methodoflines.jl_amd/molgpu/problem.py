@@ -137,10 +137,10 @@ class ODEProblem:
 
     def plan(self) -> dict:
         h = self.handle
-        info = (C.c_int32 * 8)()
-        h.lib.mol_get_plan(h.h, info, 8)
+        info = (C.c_int32 * 9)()
+        h.lib.mol_get_plan(h.h, info, 9)
         return {"tile": (info[0], info[1], info[2]), "core_tiles": info[3], "generic_tiles": info[4],
-                "weno_tiles": info[5], "generated": info[6], "interp_core_ok": info[7]}
+                "weno_tiles": info[5], "generated": info[6], "interp_core_ok": info[7], "fused_tiles": info[8]}
 
     def aux_tensor(self, name: str):
         self.handle
@@ -200,14 +200,16 @@ class ODEProblem:
 
     # ---------------------------------------------------------------- integration
     def integrate(self, u, t0: float, dt: float, nsteps: int, alg=RK4, kahan: bool = False,
-                  graph: bool = False, p=None) -> None:
-        """Advance the device state ``u`` in place by ``nsteps`` fixed steps."""
+                  graph: bool = False, p=None, fused: bool = True) -> None:
+        """Advance the device state ``u`` in place by ``nsteps`` fixed steps.  ``fused=False``
+        runs RK4 as four stage launches per step even where the fused step kernel exists."""
         torch = _torch()
         h = self.handle
         self._check_tensor(u)
         if p is None:
             p = self.device_params()
-        flags = (native.MOL_FLAG_KAHAN if kahan else 0) | (native.MOL_FLAG_GRAPH if graph else 0)
+        flags = (native.MOL_FLAG_KAHAN if kahan else 0) | (native.MOL_FLAG_GRAPH if graph else 0) | \
+            (0 if fused else native.MOL_FLAG_UNFUSED)
         stream = torch.cuda.current_stream().cuda_stream
         rc = h.lib.mol_integrate(h.h, u.data_ptr(), p.data_ptr() if p is not None else None, float(t0),
                                  float(dt), int(nsteps), alg.scheme, flags, C.c_void_p(stream))
@@ -296,14 +298,14 @@ class ODEProblem:
         return out
 
 
-def codegen(low: Lowered, compile: bool = False, tile=None, vjp: bool = False) -> str:
+def codegen(low: Lowered, compile: bool = False, tile=None, vjp: bool = False, fused: bool = False) -> str:
     """Source of the kernel generated for a lowered system (no GPU needed): the right-hand
-    side, or with ``vjp=True`` its vector-Jacobian product; ``compile=True`` also compiles it
-    with hiprtc for gfx950."""
+    side, with ``vjp=True`` its vector-Jacobian product, with ``fused=True`` the fused RK4 step;
+    ``compile=True`` also compiles it with hiprtc for gfx950."""
     lib = native.load_library()
     desc, keep = native.pack_descriptor(low)
     tl = (C.c_int32 * 3)(*tile) if tile is not None else None
-    what = 2 if vjp else 0
+    what = 2 if vjp else (4 if fused else 0)
     n = lib.mol_codegen(C.byref(desc), tl, None, 0, what)
     native.check(lib, n if n < 0 else 0, None)
     buf = C.create_string_buffer(int(n) + 1)

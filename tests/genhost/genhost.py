@@ -157,3 +157,30 @@ extern "C" __attribute__((visibility("default"))) void run(VArgs va, JPtrs jp, i
     lib.run.argtypes = [VArgs, JPtrs, C.c_int, C.c_longlong]
     lib.run(va, jp, nblk, items)
     return out, g[: len(low.aux) * ng].reshape(len(low.aux), ng) if low.aux else None
+
+
+def rk4_fused(low, u, t0, dt, p=None, aux=None, tile=None):
+    """Generated fused RK4 step kernel on the CPU: one classic RK4 step from (u, t0)."""
+    import molgpu
+    src = molgpu.codegen(low, tile=tile, fused=True)
+    entry = """
+extern "C" __attribute__((visibility("default"))) void run(const T* u, T* unew, Epi ep, JPtrs jp, int nblk) {
+  gridDim.x = nblk;
+  threadIdx.x = 0;
+  for (int b = 0; b < nblk; ++b) { blockIdx.x = b; mol_rk4_fused(u, unew, ep, jp); }
+}
+"""
+    lib = _compile(_host_source(src, entry))
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    unew = np.zeros_like(u)
+    clock = np.array([t0, 0.0])
+    pv = np.ascontiguousarray(np.asarray(low.param_values if p is None else p, dtype=np.float64).reshape(-1))
+    ep = Epi()
+    ep.tnow = clock.ctypes.data
+    ep.tnext = clock.ctypes.data + 8
+    ep.tadv = dt
+    ep.p = pv.ctypes.data if pv.size else None
+    jp, keep = _ptrs(low, aux or {})
+    lib.run.argtypes = [C.c_void_p, C.c_void_p, Epi, JPtrs, C.c_int]
+    lib.run(u.ctypes.data, unew.ctypes.data, ep, jp, _nblocks(src))
+    return unew, clock[1]

@@ -205,3 +205,26 @@ def test_generated_forward_kernel_persistent_launch(name, grid):
     tile = {1: (16, 1, 1), 2: (8, 8, 1), 3: (4, 4, 4)}[low.ndim]
     got = genhost.rhs(low, u, t=t, aux=aux, tile=tile, grid=grid)
     assert_parity(got, ref, scale, what="%s grid %d" % (name, grid))
+
+
+@pytest.mark.parametrize("n,tile", [(33, None), (33, (16, 8, 1)), (37, (16, 16, 1)), (29, (8, 8, 1))])
+def test_generated_fused_rk4_step_matches_stage_arithmetic(n, tile):
+    """The fused RK4 step (four stages on a tile with a 4-node halo in LDS) of the periodic
+    Brusselator, including ragged tilings, against the same step built from four evaluations
+    of the oracle RHS with the stage kernels' update arithmetic, and the device clock advance."""
+    low = lower(*systems.brusselator2d(n=n))
+    u = np.random.default_rng(n).uniform(0.5, 1.5, low.n_unknowns)
+    t, dt = 1.1 - 6e-6, 1e-5  # the forcing switches on at t = 1.1 in the last stage
+    got, tnext = genhost.rk4_fused(low, u, t, dt, tile=tile)
+    f = lambda y, s: orc.bruss2d_rhs(n, y, s)
+    k1 = f(u, t)
+    y1 = dt / 2 * k1 + u
+    k2 = f(y1, t + dt / 2)
+    y2 = dt / 2 * k2 + u
+    k3 = f(y2, t + dt / 2)
+    y3 = dt * k3 + u
+    k4 = f(y3, t + dt)
+    acc = (1.0 * k1 + 2.0 * k2) + 2.0 * k3
+    ref = (dt / 6 * k4 + u) + dt / 6 * acc
+    assert_parity(got, ref, np.max(np.abs(u)) * 2, what="fused n=%d tile %s" % (n, tile))
+    assert tnext == t + dt

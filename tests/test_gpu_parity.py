@@ -125,6 +125,30 @@ def test_brusselator_initial_condition_and_rk4():
     assert_parity(u.cpu().numpy(), ref, rtol=1e-12, what="bruss rk4")
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_brusselator_fused_rk4_step_equals_stage_launches(graph):
+    """The fused RK4 step kernel (one launch per step) against the four stage launches of the
+    same steps, across the forcing switch at t = 1.1, with an odd step count (the state ends
+    in the workspace and is copied back)."""
+    torch = _torch()
+    n = 129
+    prob = molgpu.discretize(*systems.brusselator2d(n=n))
+    assert prob.plan()["fused_tiles"] > 0
+    dt, N, t0 = 1e-7, 69, 1.1 - 3.3e-6     # stable: dt * 8 * alpha / dx^2 = 0.13
+    res = {}
+    for fused in (True, False):
+        u = torch.tensor(prob.u0, dtype=torch.float64, device="cuda")
+        print("  bruss fused=%s graph=%s" % (fused, graph), flush=True)
+        prob.integrate(u, t0, dt, N, molgpu.RK4, graph=graph, fused=fused)
+        torch.cuda.synchronize()
+        res[fused] = u.cpu().numpy()
+    diff = np.max(np.abs(res[True] - res[False]))
+    print("  fused vs stage launches: max abs difference %.3e" % diff)
+    assert_parity(res[True], res[False], rtol=1e-13, what="fused rk4")
+    ref = orc.bruss2d_integrate(n, prob.u0, t0, dt, N, threads=8)
+    assert_parity(res[True], ref, rtol=1e-12, what="fused rk4 vs oracle")
+
+
 # ---------------------------------------------------------------- Burgers WENO5 (config 3)
 @pytest.mark.parametrize("n", [33, 129])
 def test_burgers_weno_rhs(n, rhs_path):
