@@ -528,31 +528,107 @@ __device__ __forceinline__ int mol_cover(const int p, int* ts, int* ls) {
   return n;
 }
 
+// The (at most) three tiles along one axis whose tile + halo H contains position p: the left
+// neighbour, the own tile and the right neighbour (slots 0, 1, 2), with p relative to each
+// tile's origin; on a periodic axis of period P the neighbours across the ends hold the images
+// p + P and p - P.  Exact when H <= TL and, on a periodic axis, the last tile is at least H wide.
+struct MolCov {
+  int t[3], l[3];
+  bool v[3];
+};
+template <int TL, int H, int NT, int ULO, bool PER, int P>
+__device__ __forceinline__ MolCov mol_cover3(const int p) {
+  MolCov c;
+  const int r = p - ULO, t = mol_floordiv(r, TL), l = r - t * TL;
+  c.t[1] = t;
+  c.l[1] = l;
+  c.v[1] = t >= 0 && t < NT;
+  if (t - 1 >= 0 || !PER) {
+    c.t[0] = t - 1;
+    c.l[0] = l + TL;
+    c.v[0] = l < H && t - 1 >= 0 && t - 1 < NT;
+  } else {
+    c.t[0] = NT - 1;
+    c.l[0] = r + P - (NT - 1) * TL;
+    c.v[0] = c.l[0] < TL + H;
+  }
+  if (t + 1 < NT || !PER) {
+    c.t[2] = t + 1;
+    c.l[2] = l - TL;
+    c.v[2] = l >= TL - H && t + 1 >= 0 && t + 1 < NT;
+  } else {
+    c.t[2] = 0;
+    c.l[2] = r - P;
+    c.v[2] = c.l[2] >= -H;
+  }
+  return c;
+}
+
+template <class G, int A>
+struct MolAxis {
+  static constexpr int TL = A == 0 ? G::TX : (A == 1 ? G::TY : G::TZ);
+  static constexpr int H = A == 0 ? G::HX : (A == 1 ? G::HY : G::HZ);
+  static constexpr int NT = A == 0 ? G::NT0 : (A == 1 ? G::NT1 : G::NT2);
+  static constexpr int ULO = A == 0 ? G::ULO0 : (A == 1 ? G::ULO1 : G::ULO2);
+  static constexpr int UHI = A == 0 ? G::UHI0 : (A == 1 ? G::UHI1 : G::UHI2);
+  static constexpr bool PER = (A == 0 ? G::PER0 : (A == 1 ? G::PER1 : G::PER2)) != 0;
+  static constexpr int P = (A == 0 ? G::N0 : (A == 1 ? G::N1 : G::N2)) - 1;
+  static constexpr bool FAST = H <= TL && (!PER || (UHI - ULO + 1) - (NT - 1) * TL >= H);
+};
+
+// Adjoint-buffer value of slot s at tile-relative position (l0, l1, l2) of tile (t0, t1, t2), in
+// the tile's own variant layout, or 0 outside that variant's tile + halo.
+template <class GC, class GB, bool HASC, int VOLX, int NS, typename B>
+__device__ __forceinline__ double mol_tile_buf(const B* __restrict__ base, const int s, const int t0, const int t1,
+                                               const int t2, const int l0, const int l1, const int l2) {
+  const int tl = t0 + GB::NT0 * (t1 + GB::NT1 * t2);
+  int q;
+  if (HASC && GB::is_core(t0, t1, t2)) {
+    if (l0 < -GC::HX || l0 >= GC::TX + GC::HX || l1 < -GC::HY || l1 >= GC::TY + GC::HY || l2 < -GC::HZ ||
+        l2 >= GC::TZ + GC::HZ)
+      return 0.0;
+    q = (l0 + GC::HX) + GC::EX * ((l1 + GC::HY) + GC::EY * (l2 + GC::HZ));
+  } else {
+    if (l0 < -GB::HX || l0 >= GB::TX + GB::HX || l1 < -GB::HY || l1 >= GB::TY + GB::HY || l2 < -GB::HZ ||
+        l2 >= GB::TZ + GB::HZ)
+      return 0.0;
+    q = (l0 + GB::HX) + GB::EX * ((l1 + GB::HY) + GB::EY * (l2 + GB::HZ));
+  }
+  return (double)base[(i64)tl * (NS * VOLX) + (i64)s * VOLX + q];
+}
+
 // Sum over the tiles covering position (p0, p1, p2) of slot s of their adjoint buffers (base:
-// buffer of tile 0, stride NS * VOLX per tile): each tile in its own variant's layout.
+// buffer of tile 0, stride NS * VOLX per tile), in a fixed order: each tile in its own
+// variant's layout.
 template <class GC, class GB, bool HASC, int VOLX, int NS, typename B>
 __device__ __forceinline__ double mol_gather_pos(const B* __restrict__ base, const int s, const int p0, const int p1,
                                                  const int p2) {
+  using A0 = MolAxis<GB, 0>;
+  using A1 = MolAxis<GB, 1>;
+  using A2 = MolAxis<GB, 2>;
+  double sum = 0.0;
+  if (A0::FAST && A1::FAST && A2::FAST) {
+    const MolCov c0 = mol_cover3<A0::TL, A0::H, A0::NT, A0::ULO, A0::PER, A0::P>(p0);
+    const MolCov c1 = mol_cover3<A1::TL, A1::H, A1::NT, A1::ULO, A1::PER, A1::P>(p1);
+    const MolCov c2 = mol_cover3<A2::TL, A2::H, A2::NT, A2::ULO, A2::PER, A2::P>(p2);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+          if (c0.v[a] && c1.v[b] && c2.v[c])
+            sum += mol_tile_buf<GC, GB, HASC, VOLX, NS>(base, s, c0.t[a], c1.t[b], c2.t[c], c0.l[a], c1.l[b], c2.l[c]);
+    return sum;
+  }
   int t0[10], l0[10], t1[10], l1[10], t2[10], l2[10];
   const int n0 = mol_cover<GB::TX, GB::HX, GB::NT0, GB::ULO0, GB::PER0 != 0, GB::N0 - 1>(p0, t0, l0);
   const int n1 = mol_cover<GB::TY, GB::HY, GB::NT1, GB::ULO1, GB::PER1 != 0, GB::N1 - 1>(p1, t1, l1);
   const int n2 = mol_cover<GB::TZ, GB::HZ, GB::NT2, GB::ULO2, GB::PER2 != 0, GB::N2 - 1>(p2, t2, l2);
-  double sum = 0.0;
   for (int c = 0; c < n2; ++c)
     for (int b = 0; b < n1; ++b)
-      for (int a = 0; a < n0; ++a) {
-        const int tl = t0[a] + GB::NT0 * (t1[b] + GB::NT1 * t2[c]);
-        int q;
-        if (HASC && GB::is_core(t0[a], t1[b], t2[c])) {
-          if (l0[a] < -GC::HX || l0[a] >= GC::TX + GC::HX || l1[b] < -GC::HY || l1[b] >= GC::TY + GC::HY ||
-              l2[c] < -GC::HZ || l2[c] >= GC::TZ + GC::HZ)
-            continue;
-          q = (l0[a] + GC::HX) + GC::EX * ((l1[b] + GC::HY) + GC::EY * (l2[c] + GC::HZ));
-        } else {
-          q = (l0[a] + GB::HX) + GB::EX * ((l1[b] + GB::HY) + GB::EY * (l2[c] + GB::HZ));
-        }
-        sum += (double)base[(i64)tl * (NS * VOLX) + (i64)s * VOLX + q];
-      }
+      for (int a = 0; a < n0; ++a)
+        sum += mol_tile_buf<GC, GB, HASC, VOLX, NS>(base, s, t0[a], t1[b], t2[c], l0[a], l1[b], l2[c]);
   return sum;
 }
 
