@@ -83,7 +83,12 @@ typedef enum {
                              (nonlinear_laplacian.jl:28-103) */
   MOL_SLOT_SPHERICAL = 4, /* r^-2 d/dr(r^2 a d/dr u): tab[0]=D1, tab[1]=D2, tab[2..4]=outer,inner,interp;
                              prog = a (spherical_laplacian.jl:10-42) */
-  MOL_SLOT_MIXED = 5      /* d2/dx dy u: tab[0]=D1 along axis, tab[1]=D1 along axis2 (2nd_order_mixed_deriv.jl:5-22) */
+  MOL_SLOT_MIXED = 5,     /* d2/dx dy u: tab[0]=D1 along axis, tab[1]=D1 along axis2 (2nd_order_mixed_deriv.jl:5-22) */
+  MOL_SLOT_WENO_NU = 6    /* node-centred WENO5 first derivative on the axis's (nonuniform) coordinates, eps = eps
+                             (weno_f_nonuniform, schemes/WENO/nonuniform_weno.jl:120-212): targets 1, 2 on taps
+                             1..5 and 4, 5 on taps n-4..n at non-periodic ends, target 3 on i-2..i+2 elsewhere
+                             (get_f_taps_coords, schemes/function_scheme/function_scheme.jl:1-34); generated
+                             kernels only */
 } mol_slot_kind;
 
 typedef struct {

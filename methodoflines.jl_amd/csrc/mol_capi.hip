@@ -169,6 +169,10 @@ int validate(const mol_desc* d, std::string& msg) {
       case MOL_SLOT_CENTERED: ntab = 1; break;
       case MOL_SLOT_UPWIND: ntab = 2; break;
       case MOL_SLOT_WENO: ntab = 0; break;
+      case MOL_SLOT_WENO_NU:
+        ntab = 0;
+        if (d->n[s.axis] < 5) { msg = "nonuniform WENO needs at least 5 nodes"; return MOL_ERR_INVALID; }
+        break;
       case MOL_SLOT_NONLINLAP: ntab = 3; break;
       case MOL_SLOT_SPHERICAL: ntab = 5; break;
       case MOL_SLOT_MIXED:
@@ -446,6 +450,14 @@ void build_core(mol_handle* h, const mol_desc* d, const KArgs<T>& ka, KCore<T>& 
       case MOL_SLOT_SPHERICAL: ntab = 5; break;
       default: ntab = 0;
     }
+    if (sl.kind == MOL_SLOT_WENO_NU) {
+      // generated kernels only; one-sided targets at nodes 1, 2, n-1, n of a non-periodic axis
+      fits = false;
+      if (!kc.periodic[sl.axis]) {
+        clo[sl.axis] = std::max(clo[sl.axis], 3);
+        chi[sl.axis] = std::min(chi[sl.axis], kc.n[sl.axis] - 2);
+      }
+    }
     for (int q = 0; q < ntab; ++q) {
       const mol_optab& tb = d->tabs_h[sl.tab[q]];
       if (np + tb.len > CORE_MAX_POOL) fits = false;
@@ -688,6 +700,9 @@ int setup_paths(mol_handle* h, const mol_desc* d) {
     h->jit_reason = "code generation disabled (MOL_JIT=0)";
   }
   if (int rc = sync_kargs(h)) return rc;
+  for (int k = 0; k < d->nslots && !h->jit_fn; ++k)
+    if (d->slots_h[k].kind == MOL_SLOT_WENO_NU)
+      return fail(h, MOL_ERR_UNSUPPORTED, "nonuniform WENO runs in generated kernels only (" + h->jit_reason + ")");
   if (n64 > 0 && !h->jit_fn && !h->core_interp_ok) {
     // neither core kernel can run this system: every tile goes to the generic kernel
     const KArgs<double>& ka = h->ka64;
@@ -1212,6 +1227,9 @@ int plan_batched(const mol_desc* const* ds, const double* const* params, int32_t
     if (ds[i]->dtype != ds[0]->dtype || ds[i]->ndim != ds[0]->ndim || ds[i]->nfields != ds[0]->nfields)
       return fail(nullptr, MOL_ERR_INVALID, "instances differ in dtype, dimension or number of variables");
     if (ds[i]->naux != 0) return fail(nullptr, MOL_ERR_UNSUPPORTED, "batched instances take no grid fields");
+    for (int k = 0; k < ds[i]->nslots; ++k)
+      if (ds[i]->slots_h[k].kind == MOL_SLOT_WENO_NU)
+        return fail(nullptr, MOL_ERR_UNSUPPORTED, "batched instances take no nonuniform WENO operators");
     if (ds[i]->nparams > 0 && (!params || !params[i]))
       return fail(nullptr, MOL_ERR_INVALID, "parameter values of instance " + std::to_string(i) + " missing");
   }

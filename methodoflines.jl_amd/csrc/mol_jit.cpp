@@ -48,7 +48,7 @@ std::string istr(long long v) { return std::to_string(v); }
 // linear combinations, WENO5 reconstructions, selections and a branch node whose value is
 // chosen by the node index along one axis (boundary rows).
 enum NodeKind { N_LIT, N_LDS, N_EXPR, N_OP, N_BRANCH };
-enum : int { X_LINCOMB = 1000, X_WENO = 1001, X_SPHCOND = 1002, X_SELECT = 1003 };
+enum : int { X_LINCOMB = 1000, X_WENO = 1001, X_SPHCOND = 1002, X_SELECT = 1003, X_WENONU = 1004 };
 
 struct Node {
   int kind = N_LIT;
@@ -63,7 +63,8 @@ struct Node {
   int dv[3] = {0, 0, 0};     //        displacement from the node in nodes per axis
   bool core = false;         //        built for the interior-rows variant
   std::string text;          // N_EXPR
-  int imm = 0;               // MOL_OP_POWI exponent
+  int imm = 0;               // MOL_OP_POWI exponent; X_WENO / X_WENONU: slot
+  int target = 3;            // X_WENONU: reconstruction node 1..5 of the stencil (operands 0..4: u, 5..9: x)
   // N_BRANCH: region reg[0] when lo <= I_ax <= hi, region reg[k] when I_ax == cval[k], else 0
   int ax = 0, lo = 0, hi = 0;
   std::vector<int> cval, reg;
@@ -431,6 +432,10 @@ void Gen::slot_range(const mol_slot& s, int ax, int& lo, int& hi) const {
       node_tab(s.tab[1]);
       half_tabs(s.tab[2], s.tab[3], s.tab[4]);
       break;
+    case MOL_SLOT_WENO_NU:
+      lo = std::max(lo, 3);
+      hi = std::min(hi, n - 2);
+      break;
     default: break;
   }
 }
@@ -606,6 +611,38 @@ int Gen::slot_body(int k, const Var* v) {
       n.w = {s.eps, s.inv_dx};
       n.imm = k;
       return add(n);
+    }
+    case MOL_SLOT_WENO_NU: {
+      // stencil and target (get_f_taps_coords, function_scheme.jl:1-34): one-sided at nodes
+      // 1, 2 and n-1, n of a non-periodic axis, centred elsewhere
+      const int n = g_.n[ax];
+      int first = -2, target = 3;
+      if (!g_.periodic[ax] && !v[ax].interior) {
+        const int c = v[ax].i;
+        if (c <= 2) {
+          target = c;
+          first = 1 - c;
+        } else if (c >= n - 1) {
+          target = 5 - (n - c);
+          first = n - 4 - c;
+        }
+      }
+      Node nd;
+      nd.kind = N_OP;
+      nd.op = X_WENONU;
+      nd.target = target;
+      for (int q = 0; q < 5; ++q) nd.a.push_back(lds_ax(s.field, ax, first + q));
+      const std::string I = "I" + istr(ax), cx = "jp.coord[" + istr(ax) + "]";
+      for (int q = 0; q < 5; ++q) {
+        const int off = first + q;
+        std::string pos = off == 0 ? I : "(" + I + (off > 0 ? " + " : " - ") + istr(off > 0 ? off : -off) + ")";
+        if (!g_.periodic[ax] && !v[ax].interior) pos = IX(ax, v[ax].i + off);
+        nd.a.push_back(expr(g_.periodic[ax] ? "mol_pcoord(" + cx + ", " + pos + ", " + NN(ax) + ")"
+                                            : cx + "[" + pos + " - 1]"));
+      }
+      nd.w = {s.eps};
+      nd.imm = k;
+      return add(nd);
     }
     case MOL_SLOT_NONLINLAP:
       return nonlinlap(s, 0, 1, 2, v[ax]);
@@ -808,6 +845,10 @@ std::string Gen::op_expr(const Node& n, int id) const {
       }
       return e;
     }
+    case X_WENONU:
+      return "mol_weno5_nu<" + istr(n.target) + ">(" + A(0) + ", " + A(1) + ", " + A(2) + ", " + A(3) + ", " + A(4) +
+             ", " + A(5) + ", " + A(6) + ", " + A(7) + ", " + A(8) + ", " + A(9) + ", " +
+             pk(n.w[0], d_->ncpool + 2LL * n.imm) + ")";
     case X_WENO:
       // FP64: one-division cleared form; FP32: normalised weights (the cleared form underflows)
       return std::string(d_->dtype == MOL_F64 ? "mol_weno5(" : "mol_weno5_ref(") + A(0) + ", " + A(1) + ", " +
@@ -902,6 +943,7 @@ void Gen::partials(const Node& n, int id, std::vector<std::pair<int, std::string
       for (size_t k = 0; k < n.a.size(); ++k) put((int)k, weight(n, k) + " * " + g);
       break;
     case X_WENO:
+    case X_WENONU:
       for (int k = 0; k < 5; ++k) put(k, g + " * wd" + istr(id) + "[" + istr(k) + "]");
       break;
     default: break;   // comparisons, sign, floor, ...: zero derivative
@@ -934,6 +976,11 @@ void Gen::print_reverse(std::ostream& o, int r, const std::vector<std::pair<int,
           << ", " << ref(n.a[3]) << ", " << ref(n.a[4]) << ", " << pk(n.w[0], d_->ncpool + 2LL * n.imm) << ", "
           << pk(n.w[1], d_->ncpool + 2LL * n.imm + 1) << ", wd" << id
           << ", " << (d_->dtype == MOL_F64 ? "true" : "false") << ");\n";
+      }
+      if (n.op == X_WENONU) {
+        o << "    T wd" << id << "[5];\n    mol_weno5_nu_grad<" << n.target << ">(";
+        for (int q = 0; q < 10; ++q) o << ref(n.a[q]) << ", ";
+        o << pk(n.w[0], d_->ncpool + 2LL * n.imm) << ", wd" << id << ");\n";
       }
       std::vector<std::pair<int, std::string>> ps;
       partials(n, id, ps);

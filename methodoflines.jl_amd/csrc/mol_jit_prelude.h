@@ -187,6 +187,139 @@ __device__ __noinline__ void mol_weno5_grad(T um2, T um1, T u0, T up1, T up2, T 
   }
 }
 
+// ---- node-centred WENO5 on nonuniform coordinates (weno_f_nonuniform,
+//      src/discretization/schemes/WENO/nonuniform_weno.jl): Fornberg 3-point sub-stencil weights
+//      (:5-46), Simpson smoothness indicators (:54-73), target geometry (:76-82), closed-form
+//      ideal weights (:85-118), Shi-Hu-Shu splitting with theta = 3 (:120-163).  Coordinates and
+//      weights in FP64, values in U (T, or MolDual for the gradient).
+__device__ __forceinline__ void mol_fornberg3(const double a0, const double a1, const double a2, const double xt,
+                                              double* m0, double* m1, double* m2) {
+  double n1m0 = 1.0, n1m1 = 0.0, n1m2 = 0.0, c1 = 1.0;
+  double c2 = a1 - a0;
+  const double r1 = c1 / c2;
+  const double tA = a1 - xt;
+  const double a1m0 = (tA * n1m0) / c2, a1m1 = (tA * n1m1 - n1m0) / c2, a1m2 = (tA * n1m2 - 2.0 * n1m1) / c2;
+  const double sB = a0 - xt;
+  const double a2m0 = r1 * (-(sB)*n1m0), a2m1 = r1 * (n1m0 - sB * n1m1), a2m2 = r1 * (2.0 * n1m1 - sB * n1m2);
+  c1 = c2;
+  n1m0 = a1m0;
+  n1m1 = a1m1;
+  n1m2 = a1m2;
+  c2 = (a2 - a0) * (a2 - a1);
+  const double r2 = c1 / c2;
+  const double c3a = a2 - a0, c3b = a2 - a1, tA2 = a2 - xt;
+  m0[0] = (tA2 * n1m0) / c3a;
+  m1[0] = (tA2 * n1m1 - n1m0) / c3a;
+  m2[0] = (tA2 * n1m2 - 2.0 * n1m1) / c3a;
+  m0[1] = (tA2 * a2m0) / c3b;
+  m1[1] = (tA2 * a2m1 - a2m0) / c3b;
+  m2[1] = (tA2 * a2m2 - 2.0 * a2m1) / c3b;
+  const double sB2 = a1 - xt;
+  m0[2] = r2 * (-(sB2)*a2m0);
+  m1[2] = r2 * (a2m0 - sB2 * a2m1);
+  m2[2] = r2 * (2.0 * a2m1 - sB2 * a2m2);
+}
+
+__device__ __forceinline__ double mol_val(double v) { return v; }
+__device__ __forceinline__ float mol_val(float v) { return v; }
+__device__ __forceinline__ T mol_val(MolDual v) { return v.v; }
+
+template <class U>
+__device__ __forceinline__ U mol_dot3(const double* w, U a, U b, U c) {
+  return (T)w[0] * a + (T)w[1] * b + (T)w[2] * c;
+}
+
+template <class U>
+__device__ __forceinline__ void mol_nu_beta_r(const double a0, const double a1, const double a2, U ua, U ub, U uc,
+                                              const double xi, const double xL, const double xM, const double xph,
+                                              const double dx, U& beta, U& r) {
+  double m0[3], m1i[3], m1L[3], m1M[3], m2M[3], m1R[3], t2[3];
+  mol_fornberg3(a0, a1, a2, xi, m0, m1i, t2);
+  mol_fornberg3(a0, a1, a2, xL, m0, m1L, t2);
+  mol_fornberg3(a0, a1, a2, xM, m0, m1M, m2M);
+  mol_fornberg3(a0, a1, a2, xph, m0, m1R, t2);
+  r = mol_dot3(m1i, ua, ub, uc);
+  const U pL = mol_dot3(m1L, ua, ub, uc), pM = mol_dot3(m1M, ua, ub, uc), pR = mol_dot3(m1R, ua, ub, uc);
+  const U pp = mol_dot3(m2M, ua, ub, uc);
+  const U I1 = (T)(dx / 6) * (pL * pL + T(4) * (pM * pM) + pR * pR);
+  const U I2 = (T)dx * (pp * pp);
+  const U v = (T)dx * I1 + (T)(dx * dx * dx) * I2;
+  beta = mol_val(v) > T(0) ? v : U{T(0)};
+}
+
+template <int TG, class U>
+__device__ __forceinline__ U mol_weno5_nu_t(U u1, U u2, U u3, U u4, U u5, const double x1, const double x2,
+                                            const double x3, const double x4, const double x5, const T eps) {
+  double xi, xL, xph, d0, d2;
+  if (TG == 1) { xi = x1; xL = x1; xph = (x1 + x2) / 2; }
+  else if (TG == 2) { xi = x2; xL = (x1 + x2) / 2; xph = (x2 + x3) / 2; }
+  else if (TG == 4) { xi = x4; xL = (x3 + x4) / 2; xph = (x4 + x5) / 2; }
+  else if (TG == 5) { xi = x5; xL = (x4 + x5) / 2; xph = x5; }
+  else { xi = x3; xL = (x2 + x3) / 2; xph = (x3 + x4) / 2; }
+  if (TG == 1) {
+    d0 = ((2 * x1 - x2 - x3) * (x1 - x4) * (x1 - x5) + (x1 - x3) * (x1 - x5) * (x1 - x2) +
+          (x1 - x3) * (x1 - x4) * (x1 - x2)) / ((2 * x1 - x2 - x3) * (x1 - x4) * (x1 - x5));
+    d2 = ((x1 - x3) * (x1 - x4) * (x1 - x2)) / ((-x1 + x5) * (2 * x1 - x3 - x4) * (-x2 + x5));
+  } else if (TG == 2) {
+    d0 = ((x2 - x4) * (x2 - x5)) / ((x1 - x4) * (x1 - x5));
+    d2 = ((-x1 + x2) * (x2 - x3) * (x2 - x4)) / ((-x1 + x5) * (2 * x2 - x3 - x4) * (-x2 + x5));
+  } else if (TG == 4) {
+    d0 = ((-x2 + x4) * (-x3 + x4) * (x4 - x5)) / ((x1 - x4) * (x1 - x5) * (-x2 - x3 + 2 * x4));
+    d2 = ((-x1 + x4) * (-x2 + x4)) / ((-x1 + x5) * (-x2 + x5));
+  } else if (TG == 5) {
+    d0 = ((-x2 + x5) * (-x3 + x5) * (-x4 + x5)) / ((x1 - x4) * (x1 - x5) * (-x2 - x3 + 2 * x5));
+    d2 = ((-x1 - x4 + 2 * x5) * (-x2 + x5) * (-x3 + x5) + (-x1 + x5) * (-x2 - x3 + 2 * x5) * (-x4 + x5)) /
+         ((-x1 + x5) * (-x2 + x5) * (-x3 - x4 + 2 * x5));
+  } else {
+    d0 = ((x3 - x4) * (x3 - x5)) / ((x1 - x4) * (x1 - x5));
+    d2 = ((x3 - x1) * (x3 - x2)) / ((x5 - x1) * (x5 - x2));
+  }
+  const double dx = xph - xL, xM = (xL + xph) / 2;
+  U b0, b1, b2, r0, r1, r2;
+  mol_nu_beta_r(x1, x2, x3, u1, u2, u3, xi, xL, xM, xph, dx, b0, r0);
+  mol_nu_beta_r(x2, x3, x4, u2, u3, u4, xi, xL, xM, xph, dx, b1, r1);
+  mol_nu_beta_r(x3, x4, x5, u3, u4, u5, xi, xL, xM, xph, dx, b2, r2);
+  const double d1 = 1.0 - d0 - d2, th = 3.0, half = 0.5;
+  const double dp0 = half * (d0 + th * fabs(d0)), dp1 = half * (d1 + th * fabs(d1)), dp2 = half * (d2 + th * fabs(d2));
+  const double dm0 = dp0 - d0, dm1 = dp1 - d1, dm2 = dp2 - d2;
+  const double sgp = dp0 + dp1 + dp2, sgm = dm0 + dm1 + dm2;
+  const U e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
+  const U q0 = e0 * e0, q1 = e1 * e1, q2 = e2 * e2;
+  const U ap0 = (T)(dp0 / sgp) / q0, ap1 = (T)(dp1 / sgp) / q1, ap2 = (T)(dp2 / sgp) / q2;
+  const U sp = ap0 + ap1 + ap2;
+  const U am0 = (T)(dm0 / sgm) / q0, am1 = (T)(dm1 / sgm) / q1, am2 = (T)(dm2 / sgm) / q2;
+  const U sm = am0 + am1 + am2;
+  const U Rp = (ap0 / sp) * r0 + (ap1 / sp) * r1 + (ap2 / sp) * r2;
+  const U Rm = (am0 / sm) * r0 + (am1 / sm) * r1 + (am2 / sm) * r2;
+  return (T)sgp * Rp - (T)sgm * Rm;
+}
+
+template <int TG>
+__device__ __forceinline__ T mol_weno5_nu(T u1, T u2, T u3, T u4, T u5, double x1, double x2, double x3, double x4,
+                                          double x5, T eps) {
+  return mol_weno5_nu_t<TG, T>(u1, u2, u3, u4, u5, x1, x2, x3, x4, x5, eps);
+}
+
+// Coordinate of (unwrapped) node k of a periodic axis: the image across an end is continued by
+// the period x[n] - x[1] (bcoord, src/discretization/interface_boundary.jl:120-151).
+__device__ __forceinline__ double mol_pcoord(const double* x, const int k, const int n) {
+  if (k <= 1) return x[k + n - 2] - (x[n - 1] - x[0]);
+  if (k > n) return x[k - n] + (x[n - 1] - x[0]);
+  return x[k - 1];
+}
+
+// Gradient of the nonuniform WENO5 derivative with respect to its five values.
+template <int TG>
+__device__ __noinline__ void mol_weno5_nu_grad(T u1, T u2, T u3, T u4, T u5, double x1, double x2, double x3,
+                                               double x4, double x5, T eps, T* g) {
+  const T x[5] = {u1, u2, u3, u4, u5};
+  for (int k = 0; k < 5; ++k) {
+    MolDual d[5];
+    for (int j = 0; j < 5; ++j) d[j] = MolDual{x[j], j == k ? T(1) : T(0)};
+    g[k] = mol_weno5_nu_t<TG, MolDual>(d[0], d[1], d[2], d[3], d[4], x1, x2, x3, x4, x5, eps).d;
+  }
+}
+
 // Hardware floating-point atomic add on global memory.
 __device__ __forceinline__ void mol_atomic_add(T* p, T v) { unsafeAtomicAdd(p, v); }
 __device__ __forceinline__ void mol_atomic_add_f64(double* p, double v) { unsafeAtomicAdd(p, v); }

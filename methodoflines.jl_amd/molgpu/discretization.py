@@ -29,7 +29,7 @@ from . import fornberg
 from .bytecode import Compiler, ConstPool, CompileError
 from .pdesys import GridField, PDESystem
 
-SLOT_CENTERED, SLOT_UPWIND, SLOT_WENO, SLOT_NONLINLAP, SLOT_SPHERICAL, SLOT_MIXED = range(6)
+SLOT_CENTERED, SLOT_UPWIND, SLOT_WENO, SLOT_NONLINLAP, SLOT_SPHERICAL, SLOT_MIXED, SLOT_WENO_NU = range(7)
 
 
 class UpwindScheme:
@@ -244,14 +244,23 @@ class _Lowering:
                 raise ValueError("Variable %s has no step size" % x)
             dx = disc.dxs[x]
             if isinstance(dx, (list, tuple, np.ndarray)):
-                raise NotImplementedError("nonuniform grids are not supported by the GPU path yet")
-            g = uniform_grid(dom[x].lo, dom[x].hi, dx)
+                # nonuniform grid: the nodes themselves (generate_grid, discretize_vars.jl:219-233)
+                g = np.asarray(dx, dtype=np.float64).copy()
+                if g.ndim != 1 or len(g) < 5 or not np.all(np.diff(g) > 0):
+                    raise ValueError("a nonuniform grid must be at least 5 strictly increasing nodes")
+            else:
+                g = uniform_grid(dom[x].lo, dom[x].hi, dx)
             self.coords.append(g)
             self.n.append(len(g))
         self.dx = []
+        self.nonuniform = []
         for x, g in zip(self.axes, self.coords):
             dxv = self.disc.dxs[x]
-            if isinstance(dxv, (int, np.integer)) and not isinstance(dxv, bool):
+            nu = isinstance(dxv, (list, tuple, np.ndarray))
+            self.nonuniform.append(nu)
+            if nu:
+                dxv = float("nan")
+            elif isinstance(dxv, (int, np.integer)) and not isinstance(dxv, bool):
                 dxv = (dom[x].hi - dom[x].lo) / (dxv - 1)
             self.dx.append(float(dxv))
         self.tspan = (float(dom[self.t].lo), float(dom[self.t].hi)) if self.t in dom else (0.0, 1.0)
@@ -367,6 +376,9 @@ class _Lowering:
         if key in self.table_index:
             return self.table_index[key]
         kind, axis = key[0], key[1]
+        if self.nonuniform[axis]:
+            raise NotImplementedError("only WENO first derivatives are supported on a nonuniform grid (axis %s)"
+                                      % self.axes[axis])
         dx = self.dx[axis]
         ref_shift = 0
         if kind == "centered":
@@ -552,6 +564,10 @@ class _Lowering:
             if order % 2 == 0:
                 k = self.slot(dict(kind=SLOT_CENTERED, field=fvar, axis=ax, axis2=0,
                                    tab=[self.table(("centered", ax, order))], prog=-1, eps=0.0, inv_dx=0.0))
+            elif order == 1 and isinstance(self.scheme, WENOScheme) and self.nonuniform[ax]:
+                # node-centred nonuniform WENO5 (weno_f_nonuniform, schemes/WENO/nonuniform_weno.jl)
+                k = self.slot(dict(kind=SLOT_WENO_NU, field=fvar, axis=ax, axis2=0, tab=[0] * 0, prog=-1,
+                                   eps=self.scheme.epsilon, inv_dx=0.0))
             elif order == 1 and isinstance(self.scheme, WENOScheme):
                 k = self.slot(dict(kind=SLOT_WENO, field=fvar, axis=ax, axis2=0, tab=[0] * 0, prog=-1,
                                    eps=self.scheme.epsilon, inv_dx=1.0 / self.dx[ax]))
@@ -661,7 +677,8 @@ class _Lowering:
                     for d in pde.atoms(sp.Derivative):
                         for v, cnt in d.variable_count:
                             if v == x and cnt % 2 == 1:
-                                if cnt == 1 and isinstance(self.scheme, WENOScheme):
+                                # WENO extent: 2 on uniform grids, 0 on nonuniform ones (WENO.jl:94-95)
+                                if cnt == 1 and isinstance(self.scheme, WENOScheme) and not self.nonuniform[a]:
                                     ext = max(ext, 2)
                 el, eu = (ext if not self.periodic[a] else 0), (ext if not self.periodic[a] else 0)
                 self.extent[f][a] = (el, eu)
@@ -706,6 +723,8 @@ class _Lowering:
                             if d == 0:
                                 row[bnode] = row.get(bnode, 0.0) + cval
                             else:
+                                if self.nonuniform[a]:
+                                    raise NotImplementedError("derivative boundary conditions on a nonuniform grid")
                                 op = fornberg.centered_difference(d, self.p, self.dx[a])
                                 w, first = self._central_row(op, bnode, n)
                                 for k, wk in enumerate(w):
@@ -716,7 +735,7 @@ class _Lowering:
                     vl = self.vlower[f][a] if side == 0 else self.vupper[f][a]
                     ext = el if side == 0 else eu
                     ninterp = ext - vl
-                    eop = fornberg.boundary_interp_extrap(max(6, self.p), self.dx[a])
+                    eop = fornberg.boundary_interp_extrap(max(6, self.p), self.dx[a]) if ext > 0 else None
                     while ninterp >= vl:
                         node = (1 + ninterp) if side == 0 else (n - ninterp)
                         if node in rows:
@@ -802,6 +821,13 @@ class _Lowering:
                     spans = []
                     if s["kind"] == SLOT_WENO:
                         spans.append((i - 2, i + 2))
+                    elif s["kind"] == SLOT_WENO_NU:
+                        if not per and i <= 2:
+                            spans.append((1, 5))
+                        elif not per and i > n - 2:
+                            spans.append((n - 4, n))
+                        else:
+                            spans.append((i - 2, i + 2))
                     elif s["kind"] in (SLOT_CENTERED, SLOT_UPWIND, SLOT_MIXED):
                         tabs = s["tab"][:2] if s["kind"] == SLOT_UPWIND else [s["tab"][0 if ax == s["axis"] else 1]]
                         for tb in tabs:

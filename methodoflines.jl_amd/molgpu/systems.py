@@ -153,6 +153,36 @@ def weno1d_dirichlet(n: int, L: float, c: float, gl: float, gh: float, eps: floa
     return sys_, MOLFiniteDifference({x: n}, t, advection_scheme=WENOScheme(eps))
 
 
+def stretched_grid(n: int, L: float, seed: int = 0, spread: float = 0.5):
+    """n strictly increasing nodes on [0, L]: uniform spacing perturbed by seeded factors in
+    [1 - spread, 1 + spread] and rescaled to end at L."""
+    rng = np.random.default_rng(seed)
+    h = 1.0 + spread * rng.uniform(-1.0, 1.0, n - 1)
+    x = np.concatenate([[0.0], np.cumsum(h)])
+    x = x * (L / x[-1])
+    x[-1] = L
+    return x
+
+
+def weno1d_nonuniform(x_nodes, c: float, gl: float = 0.0, gh: float = 0.0, periodic: bool = False,
+                      eps: float = 1e-6):
+    """u_t = -c u_x with WENO5 on the given (nonuniform) nodes: Dirichlet u = gl, gh at the ends,
+    or periodic; u0 = a Gaussian bump."""
+    t, x = parameters("t x")
+    u = variables("u")
+    Dt, Dx = Differential(t), Differential(x)
+    x_nodes = np.asarray(x_nodes, dtype=np.float64)
+    L0, L1 = float(x_nodes[0]), float(x_nodes[-1])
+    eq = Eq(Dt(u(t, x)), -c * Dx(u(t, x)))
+    ic = sp.exp(-((x - (L0 + L1) / 2) / (0.2 * (L1 - L0))) ** 2)
+    if periodic:
+        bcs = [Eq(u(0, x), ic), Eq(u(t, L0), u(t, L1))]
+    else:
+        bcs = [Eq(u(0, x), ic), Eq(u(t, L0), gl), Eq(u(t, L1), gh)]
+    sys_ = PDESystem([eq], bcs, [Interval(t, 0.0, 1.0), Interval(x, L0, L1)], [t, x], [u(t, x)])
+    return sys_, MOLFiniteDifference({x: x_nodes}, t, advection_scheme=WENOScheme(eps))
+
+
 def mixed2d(n: int, c: float, approx_order: int = 2):
     t, x, y = parameters("t x y")
     u = variables("u")

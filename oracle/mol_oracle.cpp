@@ -465,6 +465,139 @@ S weno_f(S u_m2, S u_m1, S u_0, S u_p1, S u_p2, double eps, double dx) {
   return (hp - hm) / S(dx);
 }
 
+// ---- nonuniform node-centred WENO-5 first derivative (weno_f_nonuniform,
+//      src/discretization/schemes/WENO/nonuniform_weno.jl): Fornberg 3-point sub-stencil weights
+//      (:5-46), Simpson smoothness indicators (:54-73), target geometry (:76-82), closed-form
+//      ideal weights (:85-118), Shi-Hu-Shu positive/negative splitting with theta = 3 (:120-163).
+static void fornberg3(const double al[3], double xt, double m0[3], double m1[3], double m2[3]) {
+  const double a0 = al[0], a1 = al[1], a2 = al[2];
+  double n1m0 = 1.0, n1m1 = 0.0, n1m2 = 0.0, c1 = 1.0;
+  double c2 = a1 - a0;
+  const double r1 = c1 / c2;
+  const double tA = a1 - xt;
+  const double a1m0 = (tA * n1m0) / c2, a1m1 = (tA * n1m1 - n1m0) / c2, a1m2 = (tA * n1m2 - 2.0 * n1m1) / c2;
+  const double sB = a0 - xt;
+  const double a2m0 = r1 * (-(sB)*n1m0), a2m1 = r1 * (n1m0 - sB * n1m1), a2m2 = r1 * (2.0 * n1m1 - sB * n1m2);
+  c1 = c2;
+  n1m0 = a1m0; n1m1 = a1m1; n1m2 = a1m2;
+  const double n2m0 = a2m0, n2m1 = a2m1, n2m2 = a2m2;
+  c2 = (a2 - a0) * (a2 - a1);
+  const double r2 = c1 / c2;
+  const double c3a = a2 - a0, c3b = a2 - a1, tA2 = a2 - xt;
+  m0[0] = (tA2 * n1m0) / c3a; m1[0] = (tA2 * n1m1 - n1m0) / c3a; m2[0] = (tA2 * n1m2 - 2.0 * n1m1) / c3a;
+  m0[1] = (tA2 * n2m0) / c3b; m1[1] = (tA2 * n2m1 - n2m0) / c3b; m2[1] = (tA2 * n2m2 - 2.0 * n2m1) / c3b;
+  const double sB2 = a1 - xt;
+  m0[2] = r2 * (-(sB2)*n2m0); m1[2] = r2 * (n2m0 - sB2 * n2m1); m2[2] = r2 * (2.0 * n2m1 - sB2 * n2m2);
+}
+
+template <typename S>
+static S dot3(const double w[3], S a, S b, S c) { return S(w[0]) * a + S(w[1]) * b + S(w[2]) * c; }
+
+template <typename S>
+static void substencil_beta_r(const double al[3], S ua, S ub, S uc, double xi, double xL, double xM, double xph,
+                              double dx, S* beta, S* r) {
+  double m0[3], m1i[3], m1L[3], m1M[3], m2M[3], m1R[3], t[3];
+  fornberg3(al, xi, m0, m1i, t);
+  fornberg3(al, xL, m0, m1L, t);
+  fornberg3(al, xM, m0, m1M, m2M);
+  fornberg3(al, xph, m0, m1R, t);
+  *r = dot3(m1i, ua, ub, uc);
+  const S pL = dot3(m1L, ua, ub, uc), pM = dot3(m1M, ua, ub, uc), pR = dot3(m1R, ua, ub, uc);
+  const S pp = dot3(m2M, ua, ub, uc);
+  const S I1 = S(dx / 6) * (pL * pL + S(4.0) * (pM * pM) + pR * pR);
+  const S I2 = S(dx) * (pp * pp);
+  const S v = S(dx) * I1 + S(dx * dx * dx) * I2;
+  *beta = val(v) > 0.0 ? v : S(0.0);
+}
+
+static void weno_ideal(int target, double x1, double x2, double x3, double x4, double x5, double* d0, double* d2) {
+  switch (target) {
+    case 1:
+      *d0 = ((2 * x1 - x2 - x3) * (x1 - x4) * (x1 - x5) + (x1 - x3) * (x1 - x5) * (x1 - x2) +
+             (x1 - x3) * (x1 - x4) * (x1 - x2)) / ((2 * x1 - x2 - x3) * (x1 - x4) * (x1 - x5));
+      *d2 = ((x1 - x3) * (x1 - x4) * (x1 - x2)) / ((-x1 + x5) * (2 * x1 - x3 - x4) * (-x2 + x5));
+      return;
+    case 2:
+      *d0 = ((x2 - x4) * (x2 - x5)) / ((x1 - x4) * (x1 - x5));
+      *d2 = ((-x1 + x2) * (x2 - x3) * (x2 - x4)) / ((-x1 + x5) * (2 * x2 - x3 - x4) * (-x2 + x5));
+      return;
+    case 4:
+      *d0 = ((-x2 + x4) * (-x3 + x4) * (x4 - x5)) / ((x1 - x4) * (x1 - x5) * (-x2 - x3 + 2 * x4));
+      *d2 = ((-x1 + x4) * (-x2 + x4)) / ((-x1 + x5) * (-x2 + x5));
+      return;
+    case 5:
+      *d0 = ((-x2 + x5) * (-x3 + x5) * (-x4 + x5)) / ((x1 - x4) * (x1 - x5) * (-x2 - x3 + 2 * x5));
+      *d2 = ((-x1 - x4 + 2 * x5) * (-x2 + x5) * (-x3 + x5) + (-x1 + x5) * (-x2 - x3 + 2 * x5) * (-x4 + x5)) /
+            ((-x1 + x5) * (-x2 + x5) * (-x3 - x4 + 2 * x5));
+      return;
+    default:
+      *d0 = ((x3 - x4) * (x3 - x5)) / ((x1 - x4) * (x1 - x5));
+      *d2 = ((x3 - x1) * (x3 - x2)) / ((x5 - x1) * (x5 - x2));
+  }
+}
+
+template <typename S>
+S weno_nu(const S* u, const double* x, double eps, int target) {
+  const double x1 = x[0], x2 = x[1], x3 = x[2], x4 = x[3], x5 = x[4];
+  double xi, xL, xph;
+  switch (target) {
+    case 1: xi = x1; xL = x1; xph = (x1 + x2) / 2; break;
+    case 2: xi = x2; xL = (x1 + x2) / 2; xph = (x2 + x3) / 2; break;
+    case 4: xi = x4; xL = (x3 + x4) / 2; xph = (x4 + x5) / 2; break;
+    case 5: xi = x5; xL = (x4 + x5) / 2; xph = x5; break;
+    default: xi = x3; xL = (x2 + x3) / 2; xph = (x3 + x4) / 2;
+  }
+  const double dx = xph - xL, xM = (xL + xph) / 2;
+  const double s0[3] = {x1, x2, x3}, s1[3] = {x2, x3, x4}, s2[3] = {x3, x4, x5};
+  S b0, b1, b2, r0, r1, r2;
+  substencil_beta_r(s0, u[0], u[1], u[2], xi, xL, xM, xph, dx, &b0, &r0);
+  substencil_beta_r(s1, u[1], u[2], u[3], xi, xL, xM, xph, dx, &b1, &r1);
+  substencil_beta_r(s2, u[2], u[3], u[4], xi, xL, xM, xph, dx, &b2, &r2);
+  double d0, d2;
+  weno_ideal(target, x1, x2, x3, x4, x5, &d0, &d2);
+  const double d1 = 1.0 - d0 - d2, th = 3.0, half = 0.5;
+  const double dp0 = half * (d0 + th * std::fabs(d0)), dp1 = half * (d1 + th * std::fabs(d1)),
+               dp2 = half * (d2 + th * std::fabs(d2));
+  const double dm0 = dp0 - d0, dm1 = dp1 - d1, dm2 = dp2 - d2;
+  const double sgp = dp0 + dp1 + dp2, sgm = dm0 + dm1 + dm2;
+  const S e(eps);
+  const S ap0 = S(dp0 / sgp) / ((e + b0) * (e + b0)), ap1 = S(dp1 / sgp) / ((e + b1) * (e + b1)),
+          ap2 = S(dp2 / sgp) / ((e + b2) * (e + b2));
+  const S sp = ap0 + ap1 + ap2;
+  const S am0 = S(dm0 / sgm) / ((e + b0) * (e + b0)), am1 = S(dm1 / sgm) / ((e + b1) * (e + b1)),
+          am2 = S(dm2 / sgm) / ((e + b2) * (e + b2));
+  const S sm = am0 + am1 + am2;
+  const S Rp = (ap0 / sp) * r0 + (ap1 / sp) * r1 + (ap2 / sp) * r2;
+  const S Rm = (am0 / sm) * r0 + (am1 / sm) * r1 + (am2 / sm) * r2;
+  return S(sgp) * Rp - S(sgm) * Rm;
+}
+
+// Nonuniform WENO at node I along axis a (get_f_taps_coords, function_scheme.jl:1-34): near a
+// non-periodic end the one-sided targets 1, 2 (taps 1..5) and 4, 5 (taps n-4..n); elsewhere
+// target 3 on taps I-2..I+2, wrapped on a periodic axis with the coordinates continued across
+// the end (bcoord, interface_boundary.jl:120-151).
+template <typename S>
+S weno_nu_at(const Field<S>& F, const int* I, int a, double eps) {
+  const Space& sp = *F.sp;
+  const Axis& ax = sp.ax[a];
+  const int n = ax.n;
+  int first = I[a] - 2, target = 3;
+  if (!ax.periodic && I[a] <= 2) { first = 1; target = I[a]; }
+  else if (!ax.periodic && I[a] > n - 2) { first = n - 4; target = 5 - (n - I[a]); }
+  S v[5];
+  double xs[5];
+  int J[3] = {I[0], I[1], I[2]};
+  for (int q = 0; q < 5; ++q) {
+    const int k = first + q;
+    J[a] = sp.wrap(a, k);
+    v[q] = F.get(J[0], J[1], J[2]);
+    if (k <= 1 && ax.periodic) xs[q] = ax.x[k + n - 1 - 1] - (ax.x[n - 1] - ax.x[0]);
+    else if (k > n && ax.periodic) xs[q] = ax.x[k + 1 - n - 1] + (ax.x[n - 1] - ax.x[0]);
+    else xs[q] = ax.x[k - 1];
+  }
+  return weno_nu(v, xs, eps, target);
+}
+
 template <typename S>
 S weno_at(const Field<S>& F, const int* I, int a, double eps) {
   const Space& sp = *F.sp;
@@ -670,6 +803,39 @@ int orc_half_weights(int d, int p, double dx, double* st, double* low, double* h
     std::memcpy(high + r * o.BL, o.high[r].data(), sizeof(double) * o.BL);
   }
   return 0;
+}
+
+int orc_weno_nu(const double* u5, const double* x5, double eps, int target, double* out) {
+  if (target < 1 || target > 5) return -1;
+  *out = weno_nu(u5, x5, eps, target);
+  return 0;
+}
+
+// u_t = -c u_x with nonuniform WENO on the given nodes x[0..n-1]: Dirichlet u = gl, gh at the
+// ends (unknowns 2..n-1, WENO extent 0 on nonuniform grids: WENO.jl:94-95) or periodic.
+int orc_weno_nu1d_rhs(int n, const double* x, double c, double eps, int periodic, double gl, double gh,
+                      const double* u, double t, double* du) {
+  try {
+    System<double> s;
+    s.sp.nd = 1;
+    make_axis(s.sp.ax[0], n, x[0], x[n - 1], periodic != 0);
+    for (int i = 0; i < n; ++i) s.sp.ax[0].x[i] = x[i];
+    s.F.resize(1);
+    Field<double>& f = s.F[0];
+    f.ext_lo[0] = f.ext_hi[0] = 0;
+    if (!periodic) {
+      f.side[0][0].has = true; f.side[0][0].a = {1.0}; f.side[0][0].g = [gl](double, const int*) { return gl; };
+      f.side[0][1].has = true; f.side[0][1].a = {1.0}; f.side[0][1].g = [gh](double, const int*) { return gh; };
+    }
+    box_default(s, f);
+    s.node_rhs = [c, eps](System<double>& S, const int* I, double, double* out) {
+      out[0] = -(c * weno_nu_at(S.F[0], I, 0, eps));
+    };
+    s.rhs(u, t, du, 1);
+    return 0;
+  } catch (...) {
+    return -1;
+  }
 }
 
 int orc_weno_f(const double* u5, double eps, double dx, double* out) {

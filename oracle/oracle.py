@@ -32,6 +32,9 @@ def lib():
             "orc_centered_weights": [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _ip],
             "orc_half_weights": [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _ip],
             "orc_weno_f": [_dp, C.c_double, C.c_double, _dp],
+            "orc_weno_nu": [_dp, _dp, C.c_double, C.c_int, _dp],
+            "orc_weno_nu1d_rhs": [C.c_int, _dp, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, _dp,
+                                  C.c_double, _dp],
             "orc_diffusion1d_rhs": [C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double, _dp],
             "orc_diffusion1d_integrate": [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double, C.c_double, C.c_longlong, C.c_int],
             "orc_bruss2d_rhs": [C.c_int, C.c_int, C.c_double, _dp, C.c_double, _dp, C.c_int],
@@ -100,6 +103,23 @@ def weno_f(u5, eps, dx):
     out = np.zeros(1)
     lib().orc_weno_f(_f64(u5), float(eps), float(dx), out)
     return out[0]
+
+
+def weno_nu(u5, x5, eps, target):
+    """Nonuniform WENO-5 first derivative at target node 1..5 of the 5-node stencil."""
+    out = np.zeros(1)
+    _ok(lib().orc_weno_nu(_f64(u5), _f64(x5), float(eps), int(target), out))
+    return float(out[0])
+
+
+def weno_nu1d_rhs(x, c, eps, periodic, gl, gh, u, t=0.0):
+    """u_t = -c u_x with nonuniform WENO5 on nodes x (Dirichlet gl/gh or periodic)."""
+    x = _f64(x)
+    u = _f64(u)
+    du = np.empty_like(u)
+    _ok(lib().orc_weno_nu1d_rhs(len(x), x, float(c), float(eps), 1 if periodic else 0, float(gl), float(gh),
+                                u, float(t), du))
+    return du
 
 
 def diffusion1d_rhs(n, D, bcl, bch, u, t=0.0, p=2, xlo=0.0, xhi=1.0):

@@ -124,7 +124,8 @@ def _gpu_dot_fd(prob, u, lam, v, t=0.0, eps=1e-5):
 
 
 @pytest.mark.parametrize("name", ["brusselator", "burgers_weno", "mixed_p4", "spherical", "upwind_o2",
-                                  "nonlinlap_periodic_p4", "weno_dirichlet", "diffreact_mixed_bcs"])
+                                  "nonlinlap_periodic_p4", "weno_dirichlet", "diffreact_mixed_bcs",
+                                  "weno_nonuniform", "weno_nonuniform_periodic"])
 def test_vjp_dot_product_every_slot_kind(name):
     """Reverse mode of every operator-slot kind against the forward kernel's directional
     derivative (small tiles: boundary-row variants and ghost closures included)."""
@@ -138,6 +139,10 @@ def test_vjp_dot_product_every_slot_kind(name):
         "nonlinlap_periodic_p4": (lambda: systems.nonlinlap1d(61, True, 0.2, -0.4, 4), (-1, 1), 0.0),
         "weno_dirichlet": (lambda: systems.weno1d_dirichlet(61, 1.0, 1.0, 0.1, 0.05), (-1, 1), 0.0),
         "diffreact_mixed_bcs": (lambda: systems.diffreact2d(25, 0.05, 2.5, (0, 1, 2, 0)), (0, 1), 0.0),
+        "weno_nonuniform": (lambda: systems.weno1d_nonuniform(systems.stretched_grid(61, 1.0, 4), 1.0, 0.1, 0.05),
+                            (-1, 1), 0.0),
+        "weno_nonuniform_periodic": (lambda: systems.weno1d_nonuniform(systems.stretched_grid(61, 1.0, 4), 1.0,
+                                                                       periodic=True), (-1, 1), 0.0),
     }
     build, (lo, hi), t = mk[name]
     sys_, disc = build()

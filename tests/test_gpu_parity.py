@@ -209,6 +209,24 @@ def test_weno_dirichlet_extrapolated_nodes(rhs_path):
     assert_parity(got, ref, 4 * (n - 1) / L + 1, what="weno dirichlet")
 
 
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("n,tile", [(41, (16, 1, 1)), (41, None), (3001, None)])
+def test_weno_nonuniform_rhs(periodic, n, tile):
+    """Node-centred WENO-5 on a stretched grid (generated kernel only: the interpreter has no
+    nonuniform WENO operator)."""
+    c = 1.3
+    x = systems.stretched_grid(n, 2.0, seed=3)
+    sys_, disc = systems.weno1d_nonuniform(x, c, 0.2, -0.1, periodic=periodic)
+    prob = molgpu.discretize(sys_, disc)
+    if tile is not None:
+        prob.set_tile(*tile)
+    assert prob.plan()["generated"] == 1
+    u = np.random.default_rng(n).uniform(-1, 1, prob.lowered.n_unknowns)
+    got = gpu_rhs(prob, u)
+    ref = orc.weno_nu1d_rhs(x, c, 1e-6, periodic, 0.2, -0.1, u)
+    assert_parity(got, ref, np.max(np.abs(ref)) + 1, what="weno nonuniform")
+
+
 # ---------------------------------------------------------------- diffusion-reaction (config 4 instance)
 @pytest.mark.parametrize("bct", [(0, 0, 0, 0), (1, 1, 1, 1), (0, 1, 1, 0), (1, 0, 0, 1)])
 def test_diffreact2d_rhs(bct, rhs_path):
