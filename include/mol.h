@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define MOL_ABI_VERSION 1
+#define MOL_ABI_VERSION 2
 #define MOL_MAX_DIMS 3
 #define MOL_MAX_FIELDS 8
 #define MOL_MAX_AUX 8
@@ -64,12 +64,18 @@ typedef enum { MOL_F64 = 0, MOL_F32 = 1 } mol_dtype;
  * (src/discretization/schemes/centered_difference/centered_difference.jl:5-30),
  * _upwind_difference (schemes/upwind_difference/upwind_difference.jl:1-28) and
  * get_half_offset_weights_and_stencil (schemes/half_offset_centred_difference.jl:9-37).
- * Indices are 1-based full-grid node (or half-point) indices, as in the reference. */
+ * Indices are 1-based full-grid node (or half-point) indices, as in the reference.
+ * Nonuniform axes (pn_hi > 0): the interior row differs per node; node i in pn_lo..pn_hi
+ * takes weights pool[w + (i - pn_lo)*len ..), taps at i + off + k as above; interior nodes
+ * outside that range have no row (the reference's operator has none there).  Built as
+ * CompleteCenteredDifference / CompleteUpwindDifference on nodes
+ * (centered_diff_weights.jl:77-153, upwind_diff_weights.jl:93-171); generated kernels only. */
 typedef struct {
   int32_t len, off, w;
   int32_t nlow, nhigh, blen, wlow, whigh;
   int32_t hoff;       /* 1: node-centred high rows (r = lref-i+1); 0: half-offset (r = lref-i) */
   int32_t ref_shift;  /* lref = n + ref_shift (0 normally; -1 for the nonlinear-Laplacian outer operator) */
+  int32_t pn_lo, pn_hi; /* per-node interior rows (nonuniform axis) for nodes pn_lo..pn_hi; 0, 0: one row */
 } mol_optab;
 
 /* ---- operator slots ---------------------------------------------------------- */

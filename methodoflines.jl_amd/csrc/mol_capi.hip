@@ -181,13 +181,29 @@ int validate(const mol_desc* d, std::string& msg) {
         break;
       default: msg = "unknown slot kind"; return MOL_ERR_INVALID;
     }
-    for (int q = 0; q < ntab; ++q)
+    for (int q = 0; q < ntab; ++q) {
       if (!tab_ok(s.tab[q])) { msg = "slot table out of range"; return MOL_ERR_INVALID; }
+      const mol_optab& tb = d->tabs_h[s.tab[q]];
+      if (tb.pn_hi > 0) {
+        if (s.kind != MOL_SLOT_CENTERED && s.kind != MOL_SLOT_UPWIND) {
+          msg = "per-node rows serve centered and upwind slots only";
+          return MOL_ERR_UNSUPPORTED;
+        }
+        if (d->periodic[s.axis] || tb.pn_hi > d->n[s.axis]) { msg = "per-node rows on this axis"; return MOL_ERR_INVALID; }
+      }
+    }
   }
   for (int t = 0; t < d->ntabs; ++t) {
     const mol_optab& tb = d->tabs_h[t];
     if (tb.len < 0 || tb.w < 0 || tb.w + tb.len > d->npool) { msg = "table weights out of the pool"; return MOL_ERR_INVALID; }
     if (tb.nlow < 0 || tb.nhigh < 0 || tb.blen < 0) { msg = "table rows"; return MOL_ERR_INVALID; }
+    if (tb.pn_hi != 0 || tb.pn_lo != 0) {
+      if (tb.pn_lo < 1 || tb.pn_hi < tb.pn_lo) { msg = "per-node row range"; return MOL_ERR_INVALID; }
+      if ((int64_t)tb.w + (int64_t)(tb.pn_hi - tb.pn_lo + 1) * tb.len > d->npool) {
+        msg = "per-node rows out of the pool";
+        return MOL_ERR_INVALID;
+      }
+    }
     if (tb.nlow && tb.wlow + tb.nlow * tb.blen > d->npool) { msg = "table low rows out of the pool"; return MOL_ERR_INVALID; }
     if (tb.nhigh && tb.whigh + tb.nhigh * tb.blen > d->npool) { msg = "table high rows out of the pool"; return MOL_ERR_INVALID; }
   }
@@ -365,6 +381,7 @@ Launch<T> launch_of(mol_handle* h) {
   const KArgs<T>& ka = kargs<T>(h);
   for (int a = 0; a < 3; ++a) L.jp.coord[a] = ka.coord[a];
   for (int a = 0; a < MOL_MAX_AUX; ++a) L.jp.aux[a] = ka.aux[a];
+  L.jp.pool = ka.pool;
   return L;
 }
 
@@ -375,6 +392,20 @@ uint32_t magic32(uint32_t dv) { return dv <= 1 ? 0u : (uint32_t)(((1ull << 32) +
 void interior_range(const mol_optab& tb, int n, int& lo, int& hi) {
   lo = std::max(lo, tb.nlow + 1);
   hi = std::min(hi, n + tb.ref_shift - tb.nhigh);
+  if (tb.pn_hi > 0) {
+    lo = std::max(lo, tb.pn_lo);
+    hi = std::min(hi, tb.pn_hi);
+  }
+}
+
+// The descriptor uses operators only the generated kernels implement: nonuniform WENO5 or
+// per-node weight rows (nonuniform centered / upwind).
+bool generated_only(const mol_desc* d) {
+  for (int k = 0; k < d->nslots; ++k)
+    if (d->slots_h[k].kind == MOL_SLOT_WENO_NU) return true;
+  for (int t = 0; t < d->ntabs; ++t)
+    if (d->tabs_h[t].pn_hi > 0) return true;
+  return false;
 }
 
 // Build the core-tile kernel arguments and the list of tiles left to the generic kernel.
@@ -460,7 +491,7 @@ void build_core(mol_handle* h, const mol_desc* d, const KArgs<T>& ka, KCore<T>& 
     }
     for (int q = 0; q < ntab; ++q) {
       const mol_optab& tb = d->tabs_h[sl.tab[q]];
-      if (np + tb.len > CORE_MAX_POOL) fits = false;
+      if (np + tb.len > CORE_MAX_POOL || tb.pn_hi > 0) fits = false;
       cs.len[q] = tb.len;
       cs.off[q] = tb.off;
       cs.w[q] = np;
@@ -700,9 +731,9 @@ int setup_paths(mol_handle* h, const mol_desc* d) {
     h->jit_reason = "code generation disabled (MOL_JIT=0)";
   }
   if (int rc = sync_kargs(h)) return rc;
-  for (int k = 0; k < d->nslots && !h->jit_fn; ++k)
-    if (d->slots_h[k].kind == MOL_SLOT_WENO_NU)
-      return fail(h, MOL_ERR_UNSUPPORTED, "nonuniform WENO runs in generated kernels only (" + h->jit_reason + ")");
+  if (!h->jit_fn && generated_only(d))
+    return fail(h, MOL_ERR_UNSUPPORTED,
+                "nonuniform-grid operators run in generated kernels only (" + h->jit_reason + ")");
   if (n64 > 0 && !h->jit_fn && !h->core_interp_ok) {
     // neither core kernel can run this system: every tile goes to the generic kernel
     const KArgs<double>& ka = h->ka64;
@@ -981,6 +1012,7 @@ int launch_vjp(mol_handle* h, const VArgs<T>& va, hipStream_t s, bool tiles = tr
   JitPtrs<T> jp{};
   for (int a = 0; a < 3; ++a) jp.coord[a] = ka.coord[a];
   for (int a = 0; a < MOL_MAX_AUX; ++a) jp.aux[a] = ka.aux[a];
+  jp.pool = ka.pool;
   VArgs<T> v = va;
   v.bf = (T*)h->vjp_bf;
   v.ba = (double*)h->vjp_ba;
@@ -1227,9 +1259,8 @@ int plan_batched(const mol_desc* const* ds, const double* const* params, int32_t
     if (ds[i]->dtype != ds[0]->dtype || ds[i]->ndim != ds[0]->ndim || ds[i]->nfields != ds[0]->nfields)
       return fail(nullptr, MOL_ERR_INVALID, "instances differ in dtype, dimension or number of variables");
     if (ds[i]->naux != 0) return fail(nullptr, MOL_ERR_UNSUPPORTED, "batched instances take no grid fields");
-    for (int k = 0; k < ds[i]->nslots; ++k)
-      if (ds[i]->slots_h[k].kind == MOL_SLOT_WENO_NU)
-        return fail(nullptr, MOL_ERR_UNSUPPORTED, "batched instances take no nonuniform WENO operators");
+    if (generated_only(ds[i]))
+      return fail(nullptr, MOL_ERR_UNSUPPORTED, "batched instances take no nonuniform-grid operators");
     if (ds[i]->nparams > 0 && (!params || !params[i]))
       return fail(nullptr, MOL_ERR_INVALID, "parameter values of instance " + std::to_string(i) + " missing");
   }

@@ -14,7 +14,7 @@
 namespace mol {
 
 struct Tab {
-  int32_t len, off, w, nlow, nhigh, blen, wlow, whigh, hoff, ref_shift;
+  int32_t len, off, w, nlow, nhigh, blen, wlow, whigh, hoff, ref_shift, pn_lo, pn_hi;
 };
 static_assert(sizeof(Tab) == sizeof(mol_optab), "table layout");
 

@@ -32,6 +32,7 @@ template <typename T>
 struct JitPtrs {
   const double* coord[3];
   const T* aux[MOL_MAX_AUX];
+  const double* pool;   // descriptor weight pool (per-node rows of nonuniform axes)
 };
 
 // Code generation and in-process compilation of the tile kernel (mol_jit.cpp).

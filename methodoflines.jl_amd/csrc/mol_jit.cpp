@@ -59,6 +59,7 @@ struct Node {
   int src = 0;               // N_LIT: 0 structural constant, 2 constant-pool entry sidx
   long long sidx = -1;
   std::vector<long long> wi, wj;   // X_LINCOMB: pool index of each weight (wj >= 0: product wi*wj)
+  std::vector<std::string> wx;     // X_LINCOMB: weights read at run time (per-node rows)
   int arr = 0;               // N_LDS: array (variable, or nfields + aux index)
   int dv[3] = {0, 0, 0};     //        displacement from the node in nodes per axis
   bool core = false;         //        built for the interior-rows variant
@@ -81,6 +82,7 @@ struct Region {
 struct RowR {
   std::vector<double> w;
   std::vector<long long> widx;   // pool index of each weight
+  std::vector<std::string> wx;   // per-node rows at a symbolic node index: weights read from jp.pool
   std::vector<int> rel;
   bool valid = true;
 };
@@ -149,6 +151,7 @@ class Gen {
                   : "uidx(" + istr(f) + ", " + q0 + ", " + q1 + ", " + q2 + ")";
   }
   std::string weight(const Node& n, size_t k) const {
+    if (!n.wx.empty()) return n.wx[k];
     if (!param_ || n.wi.empty()) return lit(n.w[k]);
     if (n.wj[k] >= 0) return "(T)(hd.W[" + istr(n.wi[k]) + "] * hd.W[" + istr(n.wj[k]) + "])";
     return "(T)hd.W[" + istr(n.wi[k]) + "]";
@@ -293,6 +296,15 @@ class Gen {
     if (!r.valid) return lit_node(0.0);
     std::vector<int> args;
     for (size_t k = 0; k < r.w.size(); ++k) args.push_back(lds_ax(arr, ax, r.rel[k]));
+    if (!r.wx.empty()) {
+      Node n;
+      n.kind = N_OP;
+      n.op = X_LINCOMB;
+      n.w = r.w;
+      n.a = args;
+      n.wx = r.wx;
+      return add(n);
+    }
     return lincomb(r.w, args, r.widx);
   }
   void slot_range(const mol_slot& s, int ax, int& lo, int& hi) const;
@@ -363,6 +375,24 @@ class Gen {
 RowR Gen::row(const mol_optab& tb, int ax, const Var& v, int idx_rel) {
   RowR r;
   const int n = g_.n[ax];
+  if ((v.interior || g_.periodic[ax]) && tb.pn_hi > 0) {
+    // per-node row of a nonuniform axis at the symbolic node index: weights read from the pool
+    if (param_) {
+      bad("per-node weight rows in a batched kernel");
+      r.valid = false;
+      return r;
+    }
+    const std::string I = "I" + istr(ax);
+    const std::string pos = idx_rel == 0 ? I : "(" + I + (idx_rel > 0 ? " + " : " - ") + istr(std::abs(idx_rel)) + ")";
+    for (int k = 0; k < tb.len; ++k) {
+      r.w.push_back(1.0);
+      r.widx.push_back(-1);
+      r.wx.push_back("(T)jp.pool[" + istr((long long)tb.w + k) + " + (" + pos + " - " + istr(tb.pn_lo) + ") * " +
+                     istr(tb.len) + "]");
+      r.rel.push_back(idx_rel + tb.off + k);
+    }
+    return r;
+  }
   if (v.interior || g_.periodic[ax]) {
     for (int k = 0; k < tb.len; ++k) {
       r.w.push_back(d_->pool_h[tb.w + k]);
@@ -388,6 +418,10 @@ RowR Gen::row(const mol_optab& tb, int ax, const Var& v, int idx_rel) {
     cnt = tb.blen;
   } else {
     w0 = tb.w;
+    if (tb.pn_hi > 0) {
+      if (i < tb.pn_lo || i > tb.pn_hi) { r.valid = false; return r; }
+      w0 = tb.w + (long long)(i - tb.pn_lo) * tb.len;
+    }
     a0 = i + tb.off;
     cnt = tb.len;
   }
@@ -409,6 +443,10 @@ void Gen::slot_range(const mol_slot& s, int ax, int& lo, int& hi) const {
     const mol_optab& tb = d_->tabs_h[t];
     lo = std::max(lo, tb.nlow + 1);
     hi = std::min(hi, n + tb.ref_shift - tb.nhigh);
+    if (tb.pn_hi > 0) {
+      lo = std::max(lo, tb.pn_lo);
+      hi = std::min(hi, tb.pn_hi);
+    }
   };
   auto half_tabs = [&](int to_i, int ti_i, int tp_i) {
     const mol_optab& to = d_->tabs_h[to_i];

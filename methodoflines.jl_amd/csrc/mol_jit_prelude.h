@@ -37,6 +37,7 @@ __device__ __forceinline__ void mol_advance_clock(const Epi& ep) {
 struct JPtrs {
   const double* coord[3];
   const T* aux[8];
+  const double* pool;   // weight pool: per-node rows of nonuniform axes
 };
 
 // Fused epilogue of one Runge-Kutta stage:

@@ -377,8 +377,9 @@ class _Lowering:
             return self.table_index[key]
         kind, axis = key[0], key[1]
         if self.nonuniform[axis]:
-            raise NotImplementedError("only WENO first derivatives are supported on a nonuniform grid (axis %s)"
-                                      % self.axes[axis])
+            self.tables.append(Table(self._nonuniform_table(key), 0))
+            self.table_index[key] = len(self.tables) - 1
+            return self.table_index[key]
         dx = self.dx[axis]
         ref_shift = 0
         if kind == "centered":
@@ -394,6 +395,26 @@ class _Lowering:
         self.tables.append(Table(op, ref_shift))
         self.table_index[key] = len(self.tables) - 1
         return self.table_index[key]
+
+    def _nonuniform_table(self, key):
+        """Per-node operator rows on a nonuniform axis: centered derivatives
+        (centered_diff_weights.jl:77-153; no interface/periodic boundaries, centered_difference.jl:36)
+        and first-order upwind derivatives (upwind_diff_weights.jl:93-171)."""
+        kind, axis = key[0], key[1]
+        name = self.axes[axis]
+        if self.periodic[axis]:
+            raise NotImplementedError("centered / upwind operators on a periodic nonuniform axis (%s)" % name)
+        x = self.coords[axis]
+        if kind == "centered":
+            return fornberg.centered_difference_nonuniform(key[2], self.p, x)
+        if kind == "upwind":
+            d, order, offside = key[2], key[3], key[4]
+            if d + order > 2:
+                raise NotImplementedError("upwind stencils wider than two nodes on a nonuniform axis (%s): the "
+                                          "reference's backward rows end before the last nodes" % name)
+            return fornberg.upwind_difference_nonuniform(d, order, x, offside)
+        raise NotImplementedError("%s operators on a nonuniform axis (%s)" % (
+            "half-point (nonlinear / spherical Laplacian)" if kind == "half" else kind, name))
 
     def slot(self, desc: dict) -> int:
         key = tuple(sorted((k, v if not isinstance(v, list) else tuple(v)) for k, v in desc.items()))
@@ -552,6 +573,8 @@ class _Lowering:
                 raise NotImplementedError("time derivative inside the right-hand side: %s" % d)
             if len(vc) == 2 and vc[0][1] == 1 and vc[1][1] == 1:
                 ax, ay = self.axes.index(vc[0][0]), self.axes.index(vc[1][0])
+                if self.nonuniform[ax] or self.nonuniform[ay]:
+                    raise NotImplementedError("mixed derivatives on a nonuniform axis")
                 k = self.slot(dict(kind=SLOT_MIXED, field=fvar, axis=ax, axis2=ay,
                                    tab=[self.table(("centered", ax, 1)), self.table(("centered", ay, 1))],
                                    prog=-1, eps=0.0, inv_dx=0.0))
@@ -724,8 +747,9 @@ class _Lowering:
                                 row[bnode] = row.get(bnode, 0.0) + cval
                             else:
                                 if self.nonuniform[a]:
-                                    raise NotImplementedError("derivative boundary conditions on a nonuniform grid")
-                                op = fornberg.centered_difference(d, self.p, self.dx[a])
+                                    op = fornberg.centered_difference_nonuniform(d, self.p, self.coords[a])
+                                else:
+                                    op = fornberg.centered_difference(d, self.p, self.dx[a])
                                 w, first = self._central_row(op, bnode, n)
                                 for k, wk in enumerate(w):
                                     row[first + k] = row.get(first + k, 0.0) + cval * wk
@@ -831,7 +855,12 @@ class _Lowering:
                     elif s["kind"] in (SLOT_CENTERED, SLOT_UPWIND, SLOT_MIXED):
                         tabs = s["tab"][:2] if s["kind"] == SLOT_UPWIND else [s["tab"][0 if ax == s["axis"] else 1]]
                         for tb in tabs:
-                            spans.append(row_span(self.tables[tb].op, i, n, per))
+                            op = self.tables[tb].op
+                            if op.pn_lo > 0 and not (i <= op.nlow or i > n - op.nhigh or op.pn_lo <= i <= op.pn_hi):
+                                raise NotImplementedError(
+                                    "node %d along %s has no row in the reference's nonuniform %s operator"
+                                    % (i, self.axes[ax], op.kind))
+                            spans.append(row_span(op, i, n, per))
                     else:
                         if s["kind"] == SLOT_SPHERICAL:
                             for tb in s["tab"][:2]:

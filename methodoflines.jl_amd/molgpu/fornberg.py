@@ -8,6 +8,9 @@ that the weights agree to the last bit:
 * ``centered_difference`` — CompleteCenteredDifference,
   schemes/centered_difference/centered_diff_weights.jl:4-75.
 * ``upwind_difference`` — CompleteUpwindDifference, schemes/upwind_difference/upwind_diff_weights.jl:4-91.
+* ``centered_difference_nonuniform`` — CompleteCenteredDifference on nodes,
+  centered_diff_weights.jl:77-153.
+* ``upwind_difference_nonuniform`` — CompleteUpwindDifference on nodes, upwind_diff_weights.jl:93-171.
 * ``half_centered_difference`` — CompleteHalfCenteredDifference, schemes/half_offset_weights.jl:4-70.
 * ``boundary_interp_extrap`` — BoundaryInterpolatorExtrapolator, schemes/extrapolation_weights.jl:1-72.
 """
@@ -86,6 +89,8 @@ class DerivativeOperator:
     nlow: int = 0                # low rows apply for i <= nlow
     nhigh: int = 0               # high rows apply for i > lref - nhigh
     hoff: int = 1                # high-row index: r = lref - i + hoff
+    pn_lo: int = 0               # > 0: per-node interior rows, stencil_coefs[i - pn_lo] for node i
+    pn_hi: int = 0               #      in pn_lo..pn_hi (nonuniform grids)
 
 
 def _scale(w: np.ndarray, dx: float, d: int) -> np.ndarray:
@@ -134,6 +139,71 @@ def upwind_difference(d: int, p: int, dx: float, offside: int = 0) -> Derivative
     return DerivativeOperator(d, p, dx, L, stencil, BL, high_count, low, high, offside,
                               "upwind", tap_offset=-offside, nlow=offside, nhigh=high_count,
                               hoff=1)
+
+
+def centered_difference_nonuniform(d: int, p: int, x) -> DerivativeOperator:
+    """CompleteCenteredDifference(d, p, x) on the nodes x (centered_diff_weights.jl:77-153).
+
+    Interior node i (bpc < i <= n - bpc) takes Fornberg weights at x[i] on x[i-half .. i+half];
+    nodes 1..bpc take weights on the first BL nodes and nodes n-bpc+1..n on the last BL, with
+    those nodes measured from x[1] (resp. x[n-BL]) by cumulative sums of the spacings, as the
+    reference forms them.  High row r (1-based) belongs to node n - r + 1.
+    """
+    if p <= 1:
+        raise ValueError("approximation_order must be greater than 1.")
+    x = np.asarray(x, dtype=np.float64)
+    n = len(x)
+    L = d + p - 1 + (d + p) % 2
+    BL = d + p
+    half = L // 2
+    if n < max(L, BL):
+        raise ValueError("the grid has fewer nodes than the stencil")
+    dx = x[1:] - x[:-1]
+    low_x = np.concatenate([[0.0], np.cumsum(dx[:BL - 1])])
+    high_x = np.cumsum(dx[len(dx) - BL:])
+    rows = np.array([calculate_weights(d, x[i - 1], x[i - 1 - half:i + half]) for i in range(half + 1, n - half + 1)])
+    low = [calculate_weights(d, low_x[r], low_x) for r in range(half)]
+    high = [calculate_weights(d, high_x[BL - 1 - r], high_x) for r in range(half)]
+    return DerivativeOperator(d, p, float("nan"), L, rows, BL, half, low, high, 0, "centered",
+                              tap_offset=-half, nlow=half, nhigh=half, hoff=1, pn_lo=half + 1, pn_hi=n - half)
+
+
+def upwind_difference_nonuniform(d: int, p: int, x, offside: int = 0) -> DerivativeOperator:
+    """CompleteUpwindDifference(d, p, x, offside) on the nodes x (upwind_diff_weights.jl:93-171),
+    with the row selection of _upwind_difference on nonuniform grids (upwind_difference.jl:131-162).
+
+    The reference builds rows for nodes i = offside+1 .. n-hc (hc = L-1-offside), Fornberg
+    weights at x[i] on x[i-offside .. i+L-1-offside], then stores the operator with offside 0;
+    the selection reads row k = II for node II.  So the forward operator (offside 0) gives node
+    II its own row, and the backward one (offside L-1) gives node II the row built for node
+    II + L - 1, applied to the taps II-L+1 .. II.  Both are reproduced: per-node row of node
+    II = the reference's row k = II.  High rows of the forward operator: node n - r + 1 takes
+    weights at x[n-hc+r] on the last BL nodes (r = 1..hc, in the reference's order).
+    """
+    if offside < 0:
+        raise ValueError("Number of offside points should be non-negative")
+    x = np.asarray(x, dtype=np.float64)
+    n = len(x)
+    L = d + p
+    BL = d + p
+    if offside > L - 1:
+        raise ValueError("Number of offside points should be less than or equal to the stencil length")
+    lc, hc = offside, L - 1 - offside
+    built = [calculate_weights(d, x[i - 1], x[i - 1 - offside:i - 1 + L - offside]) for i in range(lc + 1, n - hc + 1)]
+    high = [calculate_weights(d, x0, x[n - BL:]) for x0 in x[n - hc:]]
+    if offside == 0:
+        # forward: node i in 1..n-hc, taps i .. i+L-1; high rows for i > n - hc
+        return DerivativeOperator(d, p, float("nan"), L, np.array(built), BL, hc, [], high, 0, "upwind",
+                                  tap_offset=0, nlow=0, nhigh=hc, hoff=1, pn_lo=1, pn_hi=n - hc)
+    if offside != L - 1:
+        raise NotImplementedError("nonuniform upwind operators are built with offside 0 or L-1")
+    # backward: node II in L..n-L+1 (taps II-L+1 >= 1, reference row II exists)
+    lo_node, hi_node = L, n - L + 1
+    if hi_node < lo_node:
+        raise ValueError("the grid has too few nodes for the upwind stencil")
+    rows = np.array(built[lo_node - 1:hi_node])
+    return DerivativeOperator(d, p, float("nan"), L, rows, BL, 0, [], [], 0, "upwind",
+                              tap_offset=-(L - 1), nlow=0, nhigh=0, hoff=1, pn_lo=lo_node, pn_hi=hi_node)
 
 
 def half_centered_difference(d: int, p: int, dx: float) -> DerivativeOperator:

@@ -11,7 +11,7 @@ from typing import List
 
 import numpy as np
 
-MOL_ABI_VERSION = 1
+MOL_ABI_VERSION = 2
 MOL_MAX_DIMS = 3
 MOL_MAX_FIELDS = 8
 MOL_MAX_AUX = 8
@@ -35,7 +35,7 @@ class MolError(RuntimeError):
 
 class mol_optab(C.Structure):
     _fields_ = [(n, C.c_int32) for n in
-                ("len", "off", "w", "nlow", "nhigh", "blen", "wlow", "whigh", "hoff", "ref_shift")]
+                ("len", "off", "w", "nlow", "nhigh", "blen", "wlow", "whigh", "hoff", "ref_shift", "pn_lo", "pn_hi")]
 
 
 class mol_slot(C.Structure):
@@ -127,6 +127,12 @@ def _arr(a, ctype):
     return a, a.ctypes.data_as(C.POINTER(ctype))
 
 
+def weight_pool(low) -> np.ndarray:
+    """The weight pool of ``pack_descriptor(low)`` (tables, then closures), as an array."""
+    d, keep = pack_descriptor(low)
+    return np.ctypeslib.as_array(d.pool_h, shape=(max(1, d.npool),)).copy()
+
+
 def pack_descriptor(low) -> tuple:
     """Build a ``mol_desc`` for a lowered system.  Returns (desc, keepalive)."""
     keep: List = []
@@ -156,7 +162,8 @@ def pack_descriptor(low) -> tuple:
         t.len = op.stencil_length
         t.off = op.tap_offset
         t.w = len(pool)
-        pool.extend(float(v) for v in op.stencil_coefs)
+        t.pn_lo, t.pn_hi = op.pn_lo, op.pn_hi
+        pool.extend(float(v) for v in np.asarray(op.stencil_coefs, dtype=np.float64).reshape(-1))
         t.blen = op.boundary_stencil_length
         t.nlow = op.nlow
         t.wlow = len(pool)

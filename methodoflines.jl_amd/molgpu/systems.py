@@ -183,6 +183,31 @@ def weno1d_nonuniform(x_nodes, c: float, gl: float = 0.0, gh: float = 0.0, perio
     return sys_, MOLFiniteDifference({x: x_nodes}, t, advection_scheme=WENOScheme(eps))
 
 
+def nonuniform1d(x_nodes, D: float, c: float, approx_order: int = 2, upwind_order: int = 1,
+                 bcl=(1.0, 0.0, 0.0), bch=(1.0, 0.0, 0.0)):
+    """u_t = D u_xx - c u_x on the given (nonuniform) nodes; a0 u + a1 u_x = g at each end;
+    u0 = a Gaussian bump."""
+    t, x = parameters("t x")
+    u = variables("u")
+    Dt, Dx = Differential(t), Differential(x)
+    x_nodes = np.asarray(x_nodes, dtype=np.float64)
+    L0, L1 = float(x_nodes[0]), float(x_nodes[-1])
+    rhs = D * (Dx ** 2)(u(t, x)) if D != 0 else 0
+    if c != 0:
+        rhs = rhs - c * Dx(u(t, x))
+    eq = Eq(Dt(u(t, x)), rhs)
+
+    def bc(xv, coefs):
+        a0, a1, g = coefs
+        return Eq(a0 * u(t, xv) + a1 * Dx(u(t, xv)), g)
+
+    ic = sp.exp(-((x - (L0 + L1) / 2) / (0.2 * (L1 - L0))) ** 2)
+    bcs = [Eq(u(0, x), ic), bc(L0, bcl), bc(L1, bch)]
+    sys_ = PDESystem([eq], bcs, [Interval(t, 0.0, 1.0), Interval(x, L0, L1)], [t, x], [u(t, x)])
+    return sys_, MOLFiniteDifference({x: x_nodes}, t, approx_order=approx_order,
+                                     advection_scheme=UpwindScheme(upwind_order))
+
+
 def mixed2d(n: int, c: float, approx_order: int = 2):
     t, x, y = parameters("t x y")
     u = variables("u")
@@ -207,7 +232,8 @@ def spherical1d(n: int, g1: float, approx_order: int = 2):
     return sys_, MOLFiniteDifference({r: n}, t, approx_order=approx_order)
 
 
-def nonlinlap1d(n: int, periodic: bool, gl: float = 0.0, gh: float = 0.0, approx_order: int = 2):
+def nonlinlap1d(n, periodic: bool, gl: float = 0.0, gh: float = 0.0, approx_order: int = 2):
+    """u_t = d/dx((1 + u^2) u_x) on [0, 1]; n nodes, or the nodes themselves (spanning [0, 1])."""
     t, x = parameters("t x")
     u = variables("u")
     Dt, Dx = Differential(t), Differential(x)

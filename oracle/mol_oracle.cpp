@@ -101,6 +101,13 @@ struct Op {
   int d = 0, L = 0, BL = 0, bpc = 0, offside = 0;
   std::vector<double> st;
   std::vector<std::vector<double>> low, high;
+  int pn_lo = 0;                              // nonuniform: interior row of node i is pn[i - pn_lo]
+  std::vector<std::vector<double>> pn;
+  const std::vector<double>& row(int i) const {
+    if (pn.empty()) return st;
+    if (i < pn_lo || i >= pn_lo + (int)pn.size()) throw std::runtime_error("node without an operator row");
+    return pn[i - pn_lo];
+  }
 };
 
 static std::vector<double> scaled(const std::vector<double>& w, double s) {
@@ -147,6 +154,62 @@ Op upwind(int d, int p, double dx, int offside) {
   std::vector<std::vector<double>> h;
   for (int r = 0; r < hc; ++r) h.push_back(scaled(calculate_weights(d, -(double)r, hx), std::pow(-1.0 / dx, d)));
   o.high.assign(h.rbegin(), h.rend());
+  return o;
+}
+
+// CompleteCenteredDifference(d, p, x) on nodes (centered_diff_weights.jl:77-153): node i in
+// bpc+1 .. n-bpc takes Fornberg weights at x[i] on x[i-half .. i+half]; nodes 1..bpc and
+// n-bpc+1..n take weights on the first / last BL nodes, measured from x[1] and from x[n-BL]
+// by running sums of the spacings (:87-89); high row r belongs to node n - r (0-based r).
+Op centered_nu(int d, int p, const std::vector<double>& x) {
+  Op o;
+  o.d = d;
+  o.L = d + p - 1 + (d + p) % 2;
+  o.BL = d + p;
+  const int half = o.L / 2, n = (int)x.size();
+  o.bpc = half;
+  std::vector<double> dx(n - 1), lx(o.BL), hx(o.BL);
+  for (int i = 0; i + 1 < n; ++i) dx[i] = x[i + 1] - x[i];
+  lx[0] = 0.0;
+  for (int k = 1; k < o.BL; ++k) lx[k] = lx[k - 1] + dx[k - 1];
+  double acc = 0.0;
+  for (int k = 0; k < o.BL; ++k) { acc += dx[n - 1 - o.BL + k]; hx[k] = acc; }
+  o.pn_lo = half + 1;
+  for (int i = half + 1; i <= n - half; ++i) {
+    std::vector<double> pts(x.begin() + (i - 1 - half), x.begin() + (i + half));
+    o.pn.push_back(calculate_weights(d, x[i - 1], pts));
+  }
+  for (int r = 0; r < half; ++r) {
+    o.low.push_back(calculate_weights(d, lx[r], lx));
+    o.high.push_back(calculate_weights(d, hx[o.BL - 1 - r], hx));
+  }
+  return o;
+}
+
+// CompleteUpwindDifference(d, p, x, offside) on nodes (upwind_diff_weights.jl:93-171) with the row
+// selection of _upwind_difference (upwind_difference.jl:131-162).  Rows are built for nodes
+// i = offside+1 .. n-hc at x[i] on x[i-offside .. i+L-1-offside] and the operator is stored with
+// offside 0, so node II reads the row built for node II + offside: its own row for the forward
+// operator (offside 0; high rows at x[n-hc+1..n] in that order for the last hc nodes), the row of
+// node II+L-1 for the backward one (offside L-1; taps II-L+1..II, no low rows).
+Op upwind_nu(int d, int p, const std::vector<double>& x, int offside) {
+  Op o;
+  o.d = d;
+  o.L = d + p;
+  o.BL = d + p;
+  const int n = (int)x.size(), hc = o.L - 1 - offside;
+  o.offside = 0;
+  o.bpc = hc;
+  std::vector<std::vector<double>> built;
+  for (int i = offside + 1; i <= n - hc; ++i) {
+    std::vector<double> pts(x.begin() + (i - 1 - offside), x.begin() + (i - 1 + o.L - offside));
+    built.push_back(calculate_weights(d, x[i - 1], pts));
+  }
+  std::vector<double> hx(x.end() - o.BL, x.end());
+  for (int k = 0; k < hc; ++k) o.high.push_back(calculate_weights(d, x[n - hc + k], hx));
+  // node II (1-based) reads built[II - 1]
+  o.pn_lo = 1;
+  o.pn = built;
   return o;
 }
 
@@ -199,6 +262,7 @@ struct Axis {
   double dx = 0.0;
   std::vector<double> x;
   bool periodic = false;
+  bool nonuniform = false;   // x holds arbitrary nodes: operators are built on them
 };
 
 struct Space {
@@ -288,7 +352,7 @@ void solve_dense(std::vector<double> A, std::vector<S>& b, int n) {
 static void central_row(const Op& op, int i, int n, const std::vector<double>** w, int* first) {
   if (i <= op.bpc) { *w = &op.low[i - 1]; *first = 1; }
   else if (i > n - op.bpc) { *w = &op.high[n - i]; *first = n - op.BL + 1; }
-  else { *w = &op.st; *first = i - op.L / 2; }
+  else { *w = &op.row(i); *first = i - op.L / 2; }
 }
 
 // Solve the boundary equations of every grid line outside the unknown box along each
@@ -321,7 +385,7 @@ void fill_boundaries(Field<S>& F, double t) {
         for (size_t dd = 0; dd < sd.a.size(); ++dd) {
           if (sd.a[dd] == 0.0) continue;
           if (dd == 0) { e.terms.push_back({bnode, sd.a[0]}); continue; }
-          Op op = centered((int)dd, F.p, sp.ax[a].dx);
+          Op op = sp.ax[a].nonuniform ? centered_nu((int)dd, F.p, sp.ax[a].x) : centered((int)dd, F.p, sp.ax[a].dx);
           const std::vector<double>* w; int first;
           central_row(op, bnode, n, &w, &first);
           for (size_t k = 0; k < w->size(); ++k) e.terms.push_back({first + (int)k, sd.a[dd] * (*w)[k]});
@@ -406,7 +470,7 @@ S central(const Field<S>& F, const Op& op, const int* I, int a) {
   int first;
   if (!per && I[a] <= op.bpc) { w = &op.low[I[a] - 1]; first = 1; }
   else if (!per && I[a] > n - op.bpc) { w = &op.high[n - I[a]]; first = n - op.BL + 1; }
-  else { w = &op.st; first = I[a] - op.L / 2; }
+  else { w = &op.row(I[a]); first = I[a] - op.L / 2; }
   S s(0.0);
   int J[3] = {I[0], I[1], I[2]};
   for (size_t k = 0; k < w->size(); ++k) {
@@ -425,11 +489,12 @@ S upwind_at(const Field<S>& F, const Op& fwd, const Op& bwd, const int* I, int a
   int first;
   if (!positive) {
     if (!per && I[a] > n - fwd.bpc) { w = &fwd.high[n - I[a]]; first = n - fwd.BL + 1; }
-    else { w = &fwd.st; first = I[a]; }
+    else { w = &fwd.row(I[a]); first = I[a]; }
   } else {
     if (!per && I[a] <= bwd.offside) { w = &bwd.low[I[a] - 1]; first = 1; }
-    else { w = &bwd.st; first = I[a] - bwd.L + 1; }
+    else { w = &bwd.row(I[a]); first = I[a] - bwd.L + 1; }
   }
+  if (!per && (first < 1 || first + (int)w->size() - 1 > n)) throw std::runtime_error("upwind taps outside the axis");
   S s(0.0);
   int J[3] = {I[0], I[1], I[2]};
   for (size_t k = 0; k < w->size(); ++k) {
@@ -865,6 +930,51 @@ int orc_diffusion1d_rhs(int n, double xlo, double xhi, int p, double D, const do
     Op op = centered(2, p, s.sp.ax[0].dx);
     s.node_rhs = [D, op](System<double>& S, const int* I, double, double* out) {
       out[0] = D * central(S.F[0], op, I, 0);
+    };
+    s.rhs(u, t, du, 1);
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
+// ---- u_t = D u_xx - c u_x on the nodes x[0..n-1] (nonuniform), approximation order p; the
+//      advection term upwinded with order uo (cardinalized coefficient c: backward rows where
+//      c > 0, upwind_difference.jl:184-199) or, with uo = 0, centred; a0 u + a1 u_x = g at each end.
+int orc_nonuniform1d_rhs(int n, const double* x, double D, double c, int p, int uo, const double* bcl,
+                         const double* bch, const double* u, double t, double* du) {
+  try {
+    System<double> s;
+    s.p = p;
+    s.sp.nd = 1;
+    make_axis(s.sp.ax[0], n, x[0], x[n - 1], false);
+    for (int i = 0; i < n; ++i) s.sp.ax[0].x[i] = x[i];
+    s.sp.ax[0].nonuniform = true;
+    s.sp.ax[0].dx = std::nan("");
+    s.F.resize(1);
+    Field<double>& f = s.F[0];
+    for (int side = 0; side < 2; ++side) {
+      const double* b = side ? bch : bcl;
+      f.side[0][side].has = true;
+      f.side[0][side].a = {b[0], b[1]};
+      const double g = b[2];
+      f.side[0][side].g = [g](double, const int*) { return g; };
+    }
+    box_default(s, f);
+    f.p = p;
+    const std::vector<double> xs(x, x + n);
+    auto d2 = std::make_shared<Op>(centered_nu(2, p, xs));
+    auto d1 = std::make_shared<Op>(centered_nu(1, p, xs));
+    auto fw = std::make_shared<Op>(uo > 0 ? upwind_nu(1, uo, xs, 0) : Op());
+    auto bw = std::make_shared<Op>(uo > 0 ? upwind_nu(1, uo, xs, uo) : Op());
+    s.node_rhs = [D, c, uo, d2, d1, fw, bw](System<double>& S, const int* I, double, double* out) {
+      const double diff = D == 0.0 ? 0.0 : D * central(S.F[0], *d2, I, 0);
+      double adv = 0.0;
+      if (c != 0.0) {
+        if (uo > 0) adv = c > 0 ? c * upwind_at(S.F[0], *fw, *bw, I, 0, true) : c * upwind_at(S.F[0], *fw, *bw, I, 0, false);
+        else adv = c * central(S.F[0], *d1, I, 0);
+      }
+      out[0] = diff - adv;
     };
     s.rhs(u, t, du, 1);
     return 0;

@@ -35,6 +35,8 @@ def lib():
             "orc_weno_nu": [_dp, _dp, C.c_double, C.c_int, _dp],
             "orc_weno_nu1d_rhs": [C.c_int, _dp, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, _dp,
                                   C.c_double, _dp],
+            "orc_nonuniform1d_rhs": [C.c_int, _dp, C.c_double, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp,
+                                     C.c_double, _dp],
             "orc_diffusion1d_rhs": [C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double, _dp],
             "orc_diffusion1d_integrate": [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double, C.c_double, C.c_longlong, C.c_int],
             "orc_bruss2d_rhs": [C.c_int, C.c_int, C.c_double, _dp, C.c_double, _dp, C.c_int],
@@ -119,6 +121,17 @@ def weno_nu1d_rhs(x, c, eps, periodic, gl, gh, u, t=0.0):
     du = np.empty_like(u)
     _ok(lib().orc_weno_nu1d_rhs(len(x), x, float(c), float(eps), 1 if periodic else 0, float(gl), float(gh),
                                 u, float(t), du))
+    return du
+
+
+def nonuniform1d_rhs(x, D, c, p, uo, bcl, bch, u, t=0.0):
+    """u_t = D u_xx - c u_x on the nodes x: centred (uo = 0) or upwind (order uo) advection,
+    a0 u + a1 u_x = g at each end (bcl, bch = (a0, a1, g))."""
+    x = _f64(x)
+    u = _f64(u)
+    du = np.empty_like(u)
+    _ok(lib().orc_nonuniform1d_rhs(len(x), x, float(D), float(c), int(p), int(uo), _f64(bcl), _f64(bch), u,
+                                   float(t), du))
     return du
 
 

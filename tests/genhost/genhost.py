@@ -36,7 +36,7 @@ class Epi(C.Structure):
 
 
 class JPtrs(C.Structure):
-    _fields_ = [("coord", C.c_void_p * 3), ("aux", C.c_void_p * 8)]
+    _fields_ = [("coord", C.c_void_p * 3), ("aux", C.c_void_p * 8), ("pool", C.c_void_p)]
 
 
 class VArgs(C.Structure):
@@ -74,8 +74,12 @@ def _const(src: str, key: str, start: str = "") -> int:
 
 
 def _ptrs(low, aux):
+    from molgpu import native
     jp = JPtrs()
     keep = []
+    pool = np.ascontiguousarray(native.weight_pool(low), dtype=np.float64)
+    keep.append(pool)
+    jp.pool = pool.ctypes.data
     for a in range(low.ndim):
         c = np.ascontiguousarray(low.coords[a], dtype=np.float64)
         keep.append(c)

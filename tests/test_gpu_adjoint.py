@@ -125,7 +125,7 @@ def _gpu_dot_fd(prob, u, lam, v, t=0.0, eps=1e-5):
 
 @pytest.mark.parametrize("name", ["brusselator", "burgers_weno", "mixed_p4", "spherical", "upwind_o2",
                                   "nonlinlap_periodic_p4", "weno_dirichlet", "diffreact_mixed_bcs",
-                                  "weno_nonuniform", "weno_nonuniform_periodic"])
+                                  "weno_nonuniform", "weno_nonuniform_periodic", "nonuniform_centered_upwind"])
 def test_vjp_dot_product_every_slot_kind(name):
     """Reverse mode of every operator-slot kind against the forward kernel's directional
     derivative (small tiles: boundary-row variants and ghost closures included)."""
@@ -143,6 +143,8 @@ def test_vjp_dot_product_every_slot_kind(name):
                             (-1, 1), 0.0),
         "weno_nonuniform_periodic": (lambda: systems.weno1d_nonuniform(systems.stretched_grid(61, 1.0, 4), 1.0,
                                                                        periodic=True), (-1, 1), 0.0),
+        "nonuniform_centered_upwind": (lambda: systems.nonuniform1d(systems.stretched_grid(61, 1.0, 4), 0.3, 0.7, 4,
+                                                                    1, (0, 1, 0.5), (1, 2, 0.1)), (-1, 1), 0.0),
     }
     build, (lo, hi), t = mk[name]
     sys_, disc = build()

@@ -227,6 +227,60 @@ def test_weno_nonuniform_rhs(periodic, n, tile):
     assert_parity(got, ref, np.max(np.abs(ref)) + 1, what="weno nonuniform")
 
 
+NU_BCS = [((1, 0, 0.3), (1, 0, -0.2)), ((0, 1, 0.5), (1, 0, 0.0)), ((1, 2, 0.1), (2, -1, 0.4))]
+
+
+@pytest.mark.parametrize("bc", range(3))
+@pytest.mark.parametrize("p,c", [(2, 0.8), (4, -0.8), (6, 0.0)])
+@pytest.mark.parametrize("n,tile", [(41, (8, 1, 1)), (5001, None)])
+def test_nonuniform_centered_upwind_rhs(bc, p, c, n, tile):
+    """Per-node centered / upwind rows on a stretched grid, Dirichlet / Neumann / Robin ends."""
+    bcl, bch = NU_BCS[bc]
+    x = systems.stretched_grid(n, 1.5, seed=2)
+    sys_, disc = systems.nonuniform1d(x, 0.7, c, p, 1, bcl, bch)
+    prob = molgpu.discretize(sys_, disc)
+    if tile is not None:
+        prob.set_tile(*tile)
+    u = np.random.default_rng(p).uniform(-1, 1, prob.lowered.n_unknowns)
+    got = gpu_rhs(prob, u)
+    ref = orc.nonuniform1d_rhs(x, 0.7, c, p, 1, bcl, bch, u)
+    h = np.min(np.diff(x))
+    assert_parity(got, ref, np.max(np.abs(u)) * (0.7 * 8 / h ** 2 + 2 / h) + 1, what="nonuniform fd")
+
+
+def test_nonuniform_operators_refuse_the_interpreter(monkeypatch):
+    """With code generation switched off the handle fails loudly instead of running a kernel
+    that has no per-node rows."""
+    monkeypatch.setenv("MOL_JIT", "0")
+    x = systems.stretched_grid(41, 1.0, seed=2)
+    sys_, disc = systems.nonuniform1d(x, 0.7, 0.5, 2, 1)
+    with pytest.raises(molgpu.native.MolError, match="generated kernels only"):
+        prob = molgpu.discretize(sys_, disc)
+        gpu_rhs(prob, np.zeros(prob.lowered.n_unknowns))
+
+
+def test_nonuniform_rk4_matches_rk4_over_the_oracle():
+    n, p, c = 61, 4, 0.6
+    bcl, bch = (0, 1, 0.2), (1, 0, 0.1)
+    x = systems.stretched_grid(n, 1.0, seed=7)
+    sys_, disc = systems.nonuniform1d(x, 0.05, c, p, 1, bcl, bch)
+    prob = molgpu.discretize(sys_, disc)
+    u0 = np.asarray(prob.u0, dtype=np.float64)
+    dt, nsteps = 2e-4, 40
+    F = lambda w: orc.nonuniform1d_rhs(x, 0.05, c, p, 1, bcl, bch, w)
+    ref = u0.copy()
+    for _ in range(nsteps):
+        k1 = F(ref); k2 = F(ref + dt / 2 * k1); k3 = F(ref + dt / 2 * k2); k4 = F(ref + dt * k3)
+        ref = ref + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+    torch = _torch()
+    print("  nonuniform rk4 n=%d steps=%d" % (n, nsteps), flush=True)
+    ug = torch.tensor(u0, dtype=torch.float64, device="cuda")
+    prob.integrate(ug, 0.0, dt, nsteps, molgpu.RK4)
+    torch.cuda.synchronize()
+    got = ug.cpu().numpy()
+    assert np.max(np.abs(got - ref)) <= 1e-11 * (1 + np.max(np.abs(ref)))
+
+
 # ---------------------------------------------------------------- diffusion-reaction (config 4 instance)
 @pytest.mark.parametrize("bct", [(0, 0, 0, 0), (1, 1, 1, 1), (0, 1, 1, 0), (1, 0, 0, 1)])
 def test_diffreact2d_rhs(bct, rhs_path):

@@ -1,0 +1,149 @@
+"""Centered and upwind operators on nonuniform grids (CompleteCenteredDifference /
+CompleteUpwindDifference on nodes: src/discretization/schemes/centered_difference/
+centered_diff_weights.jl:77-153, schemes/upwind_difference/upwind_diff_weights.jl:93-171,
+row selection centered_difference.jl:32-57 and upwind_difference.jl:131-162).
+
+The oracle's nonuniform path is pinned by: equality with the uniform path (itself pinned to the
+reference's tables and fixtures) when the nodes are uniform; exactness of the discrete
+operator and of the Neumann / Robin closures for polynomials the stencils reproduce; the
+order of convergence on a smoothly stretched grid.  The generated kernels (compiled for the
+CPU) are checked against the oracle, and their reverse mode against finite differences.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from molgpu import systems
+from molgpu.discretization import lower
+from molgpu import fornberg
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "genhost"))
+import genhost  # noqa: E402
+
+BCS = [((1, 0, 0.3), (1, 0, -0.2)), ((0, 1, 0.5), (1, 0, 0.0)), ((1, 2, 0.1), (2, -1, 0.4))]
+
+
+@pytest.mark.parametrize("p", [2, 4, 6])
+@pytest.mark.parametrize("bcl,bch", BCS)
+def test_uniform_nodes_reproduce_the_uniform_operator(p, bcl, bch):
+    n = 41
+    x = np.linspace(0.0, 1.0, n)
+    u = np.random.default_rng(p).uniform(-1, 1, n - 2)
+    a = orc.nonuniform1d_rhs(x, 0.7, 0.0, p, 1, bcl, bch, u)
+    b = orc.diffusion1d_rhs(n, 0.7, bcl, bch, u, p=p)
+    assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
+
+
+@pytest.mark.parametrize("c", [0.9, -0.9])
+def test_uniform_nodes_reproduce_the_uniform_upwind_operator(c):
+    n = 41
+    x = np.linspace(0.0, 1.0, n)
+    u = np.random.default_rng(3).uniform(-1, 1, n - 2)
+    a = orc.nonuniform1d_rhs(x, 0.0, c, 2, 1, (1, 0, 0.3), (1, 0, -0.2), u)
+    b = orc.upwind1d_rhs(n, 1, c, 0.0, 0.3, -0.2, u)
+    assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
+
+
+@pytest.mark.parametrize("p,deg", [(2, 2), (4, 4), (6, 6)])
+@pytest.mark.parametrize("kind", ["dirichlet", "neumann", "robin"])
+def test_second_derivative_and_closures_exact_for_polynomials(p, deg, kind):
+    """Interior rows use p+1 nodes (exact to degree p), boundary rows and the closures' first-
+    derivative rows p+2 and p+1 nodes: for a degree-p polynomial with boundary data taken from
+    it, every unknown gets D u'' exactly on a random nonuniform grid."""
+    rng = np.random.default_rng(p + len(kind))
+    n = 31
+    x = systems.stretched_grid(n, 1.3, seed=p, spread=0.6) + 0.4
+    coef = rng.uniform(-1, 1, deg + 1)
+    P = np.polynomial.Polynomial(coef)
+    d1, d2 = P.deriv(), P.deriv(2)
+    a, b = x[0], x[-1]
+    if kind == "dirichlet":
+        bcl, bch = (1, 0, P(a)), (1, 0, P(b))
+    elif kind == "neumann":
+        bcl, bch = (0, 1, d1(a)), (0, 1, d1(b))
+    else:
+        bcl, bch = (1, 0.5, P(a) + 0.5 * d1(a)), (2, -1, 2 * P(b) - d1(b))
+    u = P(x[1:-1])
+    du = orc.nonuniform1d_rhs(x, 0.7, 0.0, p, 1, bcl, bch, u)
+    ref = 0.7 * d2(x[1:-1])
+    assert np.max(np.abs(du - ref)) <= 1e-8 * (1 + np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_convergence_order_on_a_smoothly_stretched_grid(p):
+    errs = []
+    ns = [41, 81, 161]
+    for n in ns:
+        xi = np.linspace(0, 1, n)
+        x = xi + 0.15 * np.sin(np.pi * xi) / np.pi
+        f = lambda z: np.sin(3 * z) + z * z
+        u = f(x[1:-1])
+        du = orc.nonuniform1d_rhs(x, 1.0, 0.0, p, 1, (1, 0, f(x[0])), (1, 0, f(x[-1])), u)
+        errs.append(np.max(np.abs(du - (-9 * np.sin(3 * x[1:-1]) + 2))))
+    orders = [np.log2(errs[k] / errs[k + 1]) for k in range(len(ns) - 1)]
+    assert orders[-1] > p - 0.3, (errs, orders)
+
+
+def test_forward_upwind_exact_for_linear_data():
+    """Negative wind: node i takes its own forward row (Fornberg at x[i] on x[i], x[i+1])."""
+    n = 25
+    x = systems.stretched_grid(n, 2.0, seed=9, spread=0.7)
+    u = 0.4 + 1.7 * x[1:-1]
+    du = orc.nonuniform1d_rhs(x, 0.0, -1.1, 2, 1, (1, 0, 0.4 + 1.7 * x[0]), (1, 0, 0.4 + 1.7 * x[-1]), u)
+    assert np.max(np.abs(du - 1.1 * 1.7)) <= 1e-12
+
+
+def test_operator_row_bookkeeping():
+    """Row ranges of the per-node operators: centered rows for bpc+1..n-bpc, forward rows for
+    1..n-hc with hc high rows, backward rows for L..n-L+1 (the reference builds no row past it)."""
+    x = systems.stretched_grid(20, 1.0, seed=1)
+    c = fornberg.centered_difference_nonuniform(2, 4, x)
+    assert (c.pn_lo, c.pn_hi, c.nlow, c.nhigh, c.stencil_length, c.boundary_stencil_length) == (3, 18, 2, 2, 5, 6)
+    assert c.stencil_coefs.shape == (16, 5)
+    f = fornberg.upwind_difference_nonuniform(1, 1, x, 0)
+    assert (f.pn_lo, f.pn_hi, f.nhigh, f.tap_offset) == (1, 19, 1, 0)
+    b = fornberg.upwind_difference_nonuniform(1, 1, x, 1)
+    assert (b.pn_lo, b.pn_hi, b.nlow, b.tap_offset) == (2, 19, 0, -1)
+    # row sums vanish (first and second derivatives)
+    assert np.max(np.abs(c.stencil_coefs.sum(axis=1))) <= 1e-9 * np.max(np.abs(c.stencil_coefs))
+
+
+def test_lowering_rejects_what_the_reference_does_not_build():
+    x = systems.stretched_grid(21, 1.0, seed=1)
+    with pytest.raises(NotImplementedError):
+        lower(*systems.nonuniform1d(x, 0.0, 1.0, 2, 2))          # upwind order 2
+    with pytest.raises(NotImplementedError):
+        lower(*systems.nonlinlap1d(x, False))                    # half-point operators
+
+
+@pytest.mark.parametrize("p", [2, 4])
+@pytest.mark.parametrize("c", [0.0, 0.8, -0.8])
+@pytest.mark.parametrize("bcl,bch", BCS)
+def test_generated_kernel_matches_oracle(p, c, bcl, bch):
+    n = 41
+    x = systems.stretched_grid(n, 1.5, seed=2)
+    low = lower(*systems.nonuniform1d(x, 0.7, c, p, 1, bcl, bch))
+    u = np.random.default_rng(p).uniform(-1, 1, low.n_unknowns)
+    ref = orc.nonuniform1d_rhs(x, 0.7, c, p, 1, bcl, bch, u)
+    for tile in [(8, 1, 1), None]:
+        got = genhost.rhs(low, u, tile=tile)
+        assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("c", [0.8, -0.8])
+def test_generated_reverse_mode_matches_finite_differences(c):
+    n = 41
+    x = systems.stretched_grid(n, 1.5, seed=2)
+    bcl, bch = (0, 1, 0.5), (1, 2, 0.1)
+    low = lower(*systems.nonuniform1d(x, 0.7, c, 4, 1, bcl, bch))
+    rng = np.random.default_rng(5)
+    N = low.n_unknowns
+    u, lam, v = rng.uniform(-1, 1, N), rng.uniform(-1, 1, N), rng.uniform(-1, 1, N)
+    g, _ = genhost.vjp(low, u, lam, tile=(8, 1, 1))
+    # the map is affine in u: the central difference is exact up to rounding
+    F = lambda w: lam @ orc.nonuniform1d_rhs(x, 0.7, c, 4, 1, bcl, bch, w)
+    fd = (F(u + v) - F(u - v)) / 2
+    assert abs(g @ v - fd) <= 1e-10 * (abs(fd) + np.linalg.norm(g) * np.linalg.norm(v))
