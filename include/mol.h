@@ -198,6 +198,17 @@ int mol_rhs(mol_handle* h, void* du, const void* u, const void* p, double t, voi
  * out, synchronizes. */
 int mol_rhs_h(mol_handle* h, void* du_h, const void* u_h, const void* p_h, double t);
 
+/* Replaces: the reshaping of a solution into full-grid arrays per variable in
+ * PDETimeSeriesSolution (src/interface/solution/timedep.jl:19-93), whose boundary nodes are
+ * observed values of the eliminated boundary equations (timedep.jl:46, generate_bc_eqs.jl:238-392)
+ * and whose corner / edge nodes are 0 (generate_bc_eqs.jl:396-416).  Writes, on `stream`, the
+ * full grid of every variable from the unknowns u at time t: full[f*G + i0 + n0*(i1 + n1*i2)]
+ * (G = n0*n1*n2): node 1 of a periodic axis is first replaced by node n (the ring of unknowns
+ * is 2..n), then unknown-box nodes are copied, ghost nodes outside the box along one
+ * non-periodic axis come from their closures and nodes outside it along two or more axes are
+ * 0.  Device pointers; full holds nfields*G values of the handle's type. */
+int mol_expand(mol_handle* h, void* full, const void* u, const void* p, double t, void* stream);
+
 /* Batch of instances (config: many uneven PDE instances advanced together).  No counterpart in
  * the reference, which solves one ODEProblem per discretize call; replaces a loop of
  * discretize/solve over instances (e.g. an EnsembleProblem of MOL problems).  The handle's state

@@ -1580,6 +1580,22 @@ int mol_rhs(mol_handle* h, void* du, const void* u, const void* p, double t, voi
   return 0;
 }
 
+int mol_expand(mol_handle* h, void* full, const void* u, const void* p, double t, void* stream) {
+  if (!h) return fail(nullptr, MOL_ERR_INVALID, "handle is NULL");
+  if (h->batch) return fail(h, MOL_ERR_UNSUPPORTED, "batched handles have no full-grid expansion");
+  if (!full || !u) return fail(h, MOL_ERR_INVALID, "full/u is NULL");
+  if (h->nparams > 0 && !p) return fail(h, MOL_ERR_INVALID, "parameters missing");
+  if (int rc = check_aux(h)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int rc = h->dtype == MOL_F64
+                     ? launch_expand<double>(launch_of<double>(h), (const double*)u, (const double*)p, t,
+                                             (double*)full, s)
+                     : launch_expand<float>(launch_of<float>(h), (const float*)u, (const float*)p, t,
+                                            (float*)full, s);
+  if (rc) return fail(h, rc, std::string("expand launch: ") + hipGetErrorString(hipGetLastError()));
+  return 0;
+}
+
 int mol_rhs_h(mol_handle* h, void* du_h, const void* u_h, const void* p_h, double t) {
   if (!h) return fail(nullptr, MOL_ERR_INVALID, "handle is NULL");
   if (!du_h || !u_h) return fail(h, MOL_ERR_INVALID, "du/u is NULL");

@@ -106,6 +106,10 @@ struct Launch {
 template <typename T>
 int launch_rhs(const Launch<T>& L, const T* u, const Epi<T>& ep, hipStream_t stream);
 
+// Full grid of every variable from the unknowns at time t (mol_expand).
+template <typename T>
+int launch_expand(const Launch<T>& L, const T* u, const T* p, double t, T* full, hipStream_t stream);
+
 // WENO5 fast path (2D, periodic, every slot a WENO5 derivative)
 bool weno_plan(const mol_desc* d, WenoPlan& plan);
 int program_depth(const mol_desc* d, int prog);

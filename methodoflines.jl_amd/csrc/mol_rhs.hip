@@ -206,6 +206,41 @@ int launch_rhs(const Launch<T>& L, const T* u, const Epi<T>& ep, hipStream_t str
 template int launch_rhs<double>(const Launch<double>&, const double*, const Epi<double>&, hipStream_t);
 template int launch_rhs<float>(const Launch<float>&, const float*, const Epi<float>&, hipStream_t);
 
+// Full-grid values of every variable (variable-major, first axis fastest): unknowns, the
+// observed boundary values of the ghost closures, periodic aliases and the corner zeros, as
+// field_at gives them.  One thread per full-grid node and variable.
+template <typename T>
+__global__ void __launch_bounds__(256) expand_kernel(const KArgs<T>* __restrict__ kap, const T* __restrict__ u,
+                                                     const T* __restrict__ p, double t, T* __restrict__ full,
+                                                     int64_t ngrid) {
+  const KArgs<T>& ka = *kap;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ngrid * ka.nfields) return;
+  const int f = (int)(q / ngrid);
+  int64_t r = q - (int64_t)f * ngrid;
+  int I[3] = {1, 1, 1};
+  for (int d = 0; d < ka.ndim; ++d) {
+    I[d] = (int)(r % ka.n[d]) + 1;
+    r /= ka.n[d];
+  }
+  full[q] = field_at(ka, u, f, I, t, p);
+}
+
+template <typename T>
+int launch_expand(const Launch<T>& L, const T* u, const T* p, double t, T* full, hipStream_t stream) {
+  const KArgs<T>& ka = *L.ka;
+  int64_t ngrid = 1;
+  for (int d = 0; d < ka.ndim; ++d) ngrid *= ka.n[d];
+  const int64_t items = ngrid * ka.nfields;
+  if (items <= 0) return 0;
+  const int64_t blocks = (items + 255) / 256;
+  if (blocks > 0x7fffffffLL) return MOL_ERR_INVALID;
+  hipLaunchKernelGGL(expand_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, stream, L.dka, u, p, t, full, ngrid);
+  return hipGetLastError() == hipSuccess ? 0 : MOL_ERR_HIP;
+}
+template int launch_expand<double>(const Launch<double>&, const double*, const double*, double, double*, hipStream_t);
+template int launch_expand<float>(const Launch<float>&, const float*, const float*, double, float*, hipStream_t);
+
 template <typename T>
 __global__ void fill_zero_kernel(T* __restrict__ p, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)

@@ -73,6 +73,7 @@ _SIGNATURES = {
     "mol_create": (C.c_int, [C.POINTER(mol_desc), C.POINTER(C.c_void_p)]),
     "mol_rhs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),
     "mol_rhs_h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]),
+    "mol_expand": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),
     "mol_set_aux": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "mol_integrate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int64,
                                 C.c_int32, C.c_int32, C.c_void_p]),

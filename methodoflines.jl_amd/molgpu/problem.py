@@ -183,6 +183,29 @@ class ODEProblem:
                            float(t), C.c_void_p(stream))
         native.check(h.lib, rc, h.h)
 
+    def expand(self, u, t: float, p=None):
+        """Full grids of every variable from the state ``u`` (CUDA tensor) at time ``t``: a tensor
+        (nfields, n_1, ..., n_d) indexed [f, i_1, ..., i_d] (0-based node indices) holding the
+        unknowns, the observed boundary values of the boundary equations, periodic aliases and 0
+        at corners, as the reference's solution arrays (PDETimeSeriesSolution,
+        src/interface/solution/timedep.jl:19-93)."""
+        torch = _torch()
+        h = self.handle
+        self._check_tensor(u)
+        if p is None:
+            p = self.device_params()
+        elif isinstance(p, np.ndarray):
+            p = torch.tensor(p, dtype=self.torch_dtype(), device="cuda")
+        low = self.lowered
+        shape = [int(v) for v in low.n[:low.ndim]]
+        full = torch.empty([low.nfields] + shape[::-1], dtype=self.torch_dtype(), device="cuda")
+        stream = torch.cuda.current_stream().cuda_stream
+        rc = h.lib.mol_expand(h.h, full.data_ptr(), u.data_ptr(), p.data_ptr() if p is not None else None,
+                              float(t), C.c_void_p(stream))
+        native.check(h.lib, rc, h.h)
+        # the device layout has the first axis fastest
+        return full.permute([0] + list(range(low.ndim, 0, -1)))
+
     def _check_tensor(self, x):
         if x.dtype != self.torch_dtype() or not x.is_cuda or not x.is_contiguous():
             raise ValueError("state tensors must be contiguous CUDA tensors of %s" % self.torch_dtype())
@@ -332,15 +355,30 @@ def discretize(pdesys: PDESystem, disc: MOLFiniteDifference) -> ODEProblem:
 
 @dataclass
 class ODESolution:
+    """Saved states of a fixed-step solve, indexed as the reference's PDETimeSeriesSolution
+    (src/interface/solution/timedep.jl:19-93): ``sol[u]`` is the full-grid trajectory of a
+    variable, shape (nt, n_1, ..., n_d), boundary nodes included; ``sol[x]`` the grid of an
+    axis; ``sol.t`` the save times.  ``sol.u`` keeps the state vectors (unknowns only)."""
     t: List[float]
     u: List[np.ndarray]
     prob: ODEProblem
+    full: List[np.ndarray] = None
 
-    def __getitem__(self, var):
-        """Per-variable trajectory over the unknown box: array (nt, *box_shape)."""
+    def _field(self, var) -> int:
         low = self.prob.lowered
         name = var if isinstance(var, str) else str(getattr(var, "func", var))
-        f = low.fields.index(name)
+        return low.fields.index(name)
+
+    def __getitem__(self, var):
+        low = self.prob.lowered
+        if var in low.axes:
+            return np.asarray(low.coords[low.axes.index(var)])
+        f = self._field(var)
+        return np.stack([g[f] for g in self.full])
+
+    def interior(self, var):
+        """Per-variable trajectory over the unknown box: array (nt, *box_shape)."""
+        f = self._field(var)
         return np.stack([self.prob.unflatten(u)[f] for u in self.u])
 
 
@@ -356,6 +394,7 @@ def solve(prob: ODEProblem, alg=RK4, dt: float = None, adaptive: bool = False, s
     nsteps_total = int(round((t1 - t0) / dt))
     u = torch.tensor(prob.u0, dtype=prob.torch_dtype(), device="cuda")
     ts, us = [t0], [prob.u0.astype(prob.dtype_np).copy()]
+    fulls = [prob.expand(u, t0).cpu().numpy().astype(np.float64)]
     if saveat is None:
         marks = [nsteps_total]
     else:
@@ -369,4 +408,5 @@ def solve(prob: ODEProblem, alg=RK4, dt: float = None, adaptive: bool = False, s
         done = m
         ts.append(t0 + done * dt)
         us.append(u.detach().cpu().numpy().astype(np.float64))
-    return ODESolution(ts, us, prob)
+        fulls.append(prob.expand(u, ts[-1]).cpu().numpy().astype(np.float64))
+    return ODESolution(ts, us, prob, fulls)
