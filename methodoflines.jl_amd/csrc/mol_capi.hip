@@ -185,8 +185,8 @@ int validate(const mol_desc* d, std::string& msg) {
       if (!tab_ok(s.tab[q])) { msg = "slot table out of range"; return MOL_ERR_INVALID; }
       const mol_optab& tb = d->tabs_h[s.tab[q]];
       if (tb.pn_hi > 0) {
-        if (s.kind != MOL_SLOT_CENTERED && s.kind != MOL_SLOT_UPWIND) {
-          msg = "per-node rows serve centered and upwind slots only";
+        if (s.kind == MOL_SLOT_MIXED) {
+          msg = "per-node rows do not serve mixed-derivative slots";
           return MOL_ERR_UNSUPPORTED;
         }
         if (d->periodic[s.axis] || tb.pn_hi > d->n[s.axis]) { msg = "per-node rows on this axis"; return MOL_ERR_INVALID; }
@@ -510,10 +510,18 @@ void build_core(mol_handle* h, const mol_desc* d, const KArgs<T>& ka, KCore<T>& 
           // outer row at index i-1 with lref = n + ref_shift
           clo[ax] = std::max(clo[ax], to.nlow + 2);
           chi[ax] = std::min(chi[ax], n + to.ref_shift - to.nhigh + 1);
+          if (to.pn_hi > 0) {
+            clo[ax] = std::max(clo[ax], to.pn_lo + 1);
+            chi[ax] = std::min(chi[ax], to.pn_hi + 1);
+          }
         } else {
           // inner / interpolation rows at the outer taps h = i-1+off_o .. i-1+off_o+len_o-1
           clo[ax] = std::max(clo[ax], tb.nlow + 2 - to.off);
           chi[ax] = std::min(chi[ax], n - tb.nhigh + 2 - to.off - to.len);
+          if (tb.pn_hi > 0) {
+            clo[ax] = std::max(clo[ax], tb.pn_lo + 1 - to.off);
+            chi[ax] = std::min(chi[ax], tb.pn_hi + 2 - to.off - to.len);
+          }
         }
       }
     }

@@ -453,11 +453,19 @@ void Gen::slot_range(const mol_slot& s, int ax, int& lo, int& hi) const {
     // outer row at index i-1
     lo = std::max(lo, to.nlow + 2);
     hi = std::min(hi, n + to.ref_shift - to.nhigh + 1);
+    if (to.pn_hi > 0) {
+      lo = std::max(lo, to.pn_lo + 1);
+      hi = std::min(hi, to.pn_hi + 1);
+    }
     // inner / interpolation rows at the outer taps h = i-1+off .. i-1+off+len-1
     for (int t : {ti_i, tp_i}) {
       const mol_optab& tb = d_->tabs_h[t];
       lo = std::max(lo, tb.nlow + 2 - to.off);
       hi = std::min(hi, n + tb.ref_shift - tb.nhigh + 2 - to.off - to.len);
+      if (tb.pn_hi > 0) {
+        lo = std::max(lo, tb.pn_lo + 1 - to.off);
+        hi = std::min(hi, tb.pn_hi + 2 - to.off - to.len);
+      }
     }
   };
   switch (s.kind) {
@@ -510,7 +518,8 @@ int Gen::leaf(uint32_t code, int imm, const Ctx& c) {
         if (!c.ip->valid) return lit_node(0.0);
         std::string e;
         for (size_t k = 0; k < c.ip->w.size(); ++k) {
-          const std::string term = pw(c.ip->w[k], c.ip->widx[k]) + " * " + coord_tap(imm, c.ip->rel[k]);
+          const std::string wk = c.ip->wx.empty() ? pw(c.ip->w[k], c.ip->widx[k]) : c.ip->wx[k];
+          const std::string term = wk + " * " + coord_tap(imm, c.ip->rel[k]);
           e = k == 0 ? term : e + " + " + term;
         }
         return e.empty() ? lit_node(0.0) : expr("(" + e + ")");
@@ -620,6 +629,15 @@ int Gen::nonlinlap(const mol_slot& s, int qo, int qi, int qp, const Var& v) {
     c.dfield = s.field;
     const int di = leaf(MOL_OP_DINNER, 0, c);
     terms.push_back(s.prog >= 0 ? op(MOL_OP_MUL, {prog(s.prog, c), di}) : di);
+  }
+  if (!ro.wx.empty()) {
+    Node n;
+    n.kind = N_OP;
+    n.op = X_LINCOMB;
+    n.w = ro.w;
+    n.a = terms;
+    n.wx = ro.wx;
+    return add(n);
   }
   return lincomb(ro.w, terms, ro.widx);
 }

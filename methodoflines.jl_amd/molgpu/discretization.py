@@ -377,7 +377,8 @@ class _Lowering:
             return self.table_index[key]
         kind, axis = key[0], key[1]
         if self.nonuniform[axis]:
-            self.tables.append(Table(self._nonuniform_table(key), 0))
+            ref_shift = key[4] if key[0] == "half" and len(key) > 4 else 0
+            self.tables.append(Table(self._nonuniform_table(key), ref_shift))
             self.table_index[key] = len(self.tables) - 1
             return self.table_index[key]
         dx = self.dx[axis]
@@ -398,12 +399,14 @@ class _Lowering:
 
     def _nonuniform_table(self, key):
         """Per-node operator rows on a nonuniform axis: centered derivatives
-        (centered_diff_weights.jl:77-153; no interface/periodic boundaries, centered_difference.jl:36)
-        and first-order upwind derivatives (upwind_diff_weights.jl:93-171)."""
+        (centered_diff_weights.jl:77-153; no interface/periodic boundaries, centered_difference.jl:36),
+        first-order upwind derivatives (upwind_diff_weights.jl:93-171) and the half-point
+        operators of the nonlinear Laplacian (half_offset_weights.jl:72-150; the outer one on the
+        midpoints of the nodes, differential_discretizer.jl:44-51)."""
         kind, axis = key[0], key[1]
         name = self.axes[axis]
         if self.periodic[axis]:
-            raise NotImplementedError("centered / upwind operators on a periodic nonuniform axis (%s)" % name)
+            raise NotImplementedError("operators on a periodic nonuniform axis (%s)" % name)
         x = self.coords[axis]
         if kind == "centered":
             return fornberg.centered_difference_nonuniform(key[2], self.p, x)
@@ -413,8 +416,11 @@ class _Lowering:
                 raise NotImplementedError("upwind stencils wider than two nodes on a nonuniform axis (%s): the "
                                           "reference's backward rows end before the last nodes" % name)
             return fornberg.upwind_difference_nonuniform(d, order, x, offside)
-        raise NotImplementedError("%s operators on a nonuniform axis (%s)" % (
-            "half-point (nonlinear / spherical Laplacian)" if kind == "half" else kind, name))
+        if kind == "half":
+            d, p = key[2], key[3]
+            outer = len(key) > 4 and key[4] == -1
+            return fornberg.half_centered_difference_nonuniform(d, p, (x[:-1] + x[1:]) / 2 if outer else x)
+        raise NotImplementedError("%s operators on a nonuniform axis (%s)" % (kind, name))
 
     def slot(self, desc: dict) -> int:
         key = tuple(sorted((k, v if not isinstance(v, list) else tuple(v)) for k, v in desc.items()))
@@ -814,6 +820,13 @@ class _Lowering:
                     out.append(Closure(f, a, side, C, c, gprog))
         return out
 
+    def _check_row(self, op, i, lref, ax):
+        """A node (or half point) i whose interior row is missing from a per-node operator: the
+        reference's operator has no row there either."""
+        if op.pn_lo > 0 and not (i <= op.nlow or i > lref - op.nhigh or op.pn_lo <= i <= op.pn_hi):
+            raise NotImplementedError("index %d along %s has no row in the reference's nonuniform %s operator"
+                                      % (i, self.axes[ax], op.kind))
+
     # ---------------------------------------------------------------- halo
     def halo(self) -> List[int]:
         reach = [0] * self.ndim
@@ -856,22 +869,22 @@ class _Lowering:
                         tabs = s["tab"][:2] if s["kind"] == SLOT_UPWIND else [s["tab"][0 if ax == s["axis"] else 1]]
                         for tb in tabs:
                             op = self.tables[tb].op
-                            if op.pn_lo > 0 and not (i <= op.nlow or i > n - op.nhigh or op.pn_lo <= i <= op.pn_hi):
-                                raise NotImplementedError(
-                                    "node %d along %s has no row in the reference's nonuniform %s operator"
-                                    % (i, self.axes[ax], op.kind))
+                            self._check_row(op, i, n, ax)
                             spans.append(row_span(op, i, n, per))
                     else:
                         if s["kind"] == SLOT_SPHERICAL:
                             for tb in s["tab"][:2]:
+                                self._check_row(self.tables[tb].op, i, n, ax)
                                 spans.append(row_span(self.tables[tb].op, i, n, per))
                             to, ti, tp = s["tab"][2:5]
                         else:
                             to, ti, tp = s["tab"][:3]
+                        self._check_row(self.tables[to].op, i - 1, n - 1, ax)
                         h0, h1 = row_span(self.tables[to].op, i - 1, n, per, -1)
                         for h in range(h0, h1 + 1):
-                            spans.append(row_span(self.tables[ti].op, h, n, per))
-                            spans.append(row_span(self.tables[tp].op, h, n, per))
+                            for tb in (ti, tp):
+                                self._check_row(self.tables[tb].op, h, n, ax)
+                                spans.append(row_span(self.tables[tb].op, h, n, per))
                     for a0, a1 in spans:
                         reach[ax] = max(reach[ax], i - a0, a1 - i)
         return reach

@@ -12,6 +12,7 @@ that the weights agree to the last bit:
   centered_diff_weights.jl:77-153.
 * ``upwind_difference_nonuniform`` — CompleteUpwindDifference on nodes, upwind_diff_weights.jl:93-171.
 * ``half_centered_difference`` — CompleteHalfCenteredDifference, schemes/half_offset_weights.jl:4-70.
+* ``half_centered_difference_nonuniform`` — the same on nodes, half_offset_weights.jl:72-150.
 * ``boundary_interp_extrap`` — BoundaryInterpolatorExtrapolator, schemes/extrapolation_weights.jl:1-72.
 """
 from __future__ import annotations
@@ -225,6 +226,32 @@ def half_centered_difference(d: int, p: int, dx: float) -> DerivativeOperator:
     high = [v[::-1] * ((-1.0) ** d) for v in low]
     return DerivativeOperator(d, p, dx, L, stencil, BL, bpc, low, high, 0, "half",
                               tap_offset=1 - L // 2, nlow=bpc, nhigh=bpc, hoff=0)
+
+
+def half_centered_difference_nonuniform(d: int, p: int, x) -> DerivativeOperator:
+    """CompleteHalfCenteredDifference(d, p, x) on the nodes x (half_offset_weights.jl:72-150).
+
+    Index i is the half point hx[i] = (x[i] + x[i+1]) / 2.  Half points bpc+1 .. n-bpc take
+    Fornberg weights at hx[i] on x[i-L/2+1 .. i+L/2] (row selection
+    half_offset_centred_difference.jl:40-69 reads row i - bpc for index i); the first bpc half
+    points take weights at hx[1..bpc] on the first BL nodes; high row r (1-based, index
+    len - r) takes weights at hx[len - r] on the last BL nodes, in the reference's order.
+    """
+    if p <= 1:
+        raise ValueError("approximation_order must be greater than 1.")
+    x = np.asarray(x, dtype=np.float64)
+    n = len(x)
+    L = p + 2 * (d // 2) + (p % 2)
+    BL = d + p
+    ep = L // 2
+    if n < max(L, BL) + 1:
+        raise ValueError("the grid has fewer nodes than the stencil")
+    hx = (x[:-1] + x[1:]) / 2
+    rows = np.array([calculate_weights(d, hx[i - 1], x[i - ep:i + ep]) for i in range(ep + 1, n - ep + 1)])
+    low = [calculate_weights(d, hx[k], x[:BL]) for k in range(ep)]
+    high = [calculate_weights(d, hx[len(hx) - 1 - k], x[n - BL:]) for k in range(ep)]
+    return DerivativeOperator(d, p, float("nan"), L, rows, BL, ep, low, high, 0, "half",
+                              tap_offset=1 - ep, nlow=ep, nhigh=ep, hoff=0, pn_lo=ep + 1, pn_hi=n - ep)
 
 
 def boundary_interp_extrap(p: int, dx: float) -> DerivativeOperator:

@@ -222,7 +222,8 @@ def mixed2d(n: int, c: float, approx_order: int = 2):
     return sys_, MOLFiniteDifference({x: n, y: n}, t, approx_order=approx_order)
 
 
-def spherical1d(n: int, g1: float, approx_order: int = 2):
+def spherical1d(n, g1: float, approx_order: int = 2):
+    """u_t = r^-2 d/dr(r^2 u_r) on [0, 1], u_r(0) = 0, u(1) = g1; n nodes, or the nodes themselves."""
     t, r = parameters("t r")
     u = variables("u")
     Dt, Dr = Differential(t), Differential(r)

@@ -233,6 +233,32 @@ Op half_op(int d, int p, double dx) {
   return o;
 }
 
+// CompleteHalfCenteredDifference(d, p, x) on nodes (half_offset_weights.jl:72-150): index i is the
+// half point hx[i] = (x[i] + x[i+1]) / 2; half points ep+1 .. n-ep take weights at hx[i] on
+// x[i-ep+1 .. i+ep] (the row selection reads row i - bpc, half_offset_centred_difference.jl:40-69),
+// low rows at hx[1..ep] on the first BL nodes, high row k at hx[len-k] on the last BL nodes.
+Op half_nu(int d, int p, const std::vector<double>& x) {
+  Op o;
+  o.d = d;
+  o.L = p + 2 * (d / 2) + (p % 2);
+  o.BL = d + p;
+  const int ep = o.L / 2, n = (int)x.size();
+  o.bpc = ep;
+  std::vector<double> hx(n - 1);
+  for (int i = 0; i + 1 < n; ++i) hx[i] = (x[i] + x[i + 1]) / 2;
+  o.pn_lo = ep + 1;
+  for (int i = ep + 1; i <= n - ep; ++i) {
+    std::vector<double> pts(x.begin() + (i - ep), x.begin() + (i + ep));
+    o.pn.push_back(calculate_weights(d, hx[i - 1], pts));
+  }
+  std::vector<double> lx(x.begin(), x.begin() + o.BL), rx(x.end() - o.BL, x.end());
+  for (int k = 0; k < ep; ++k) {
+    o.low.push_back(calculate_weights(d, hx[k], lx));
+    o.high.push_back(calculate_weights(d, hx[n - 2 - k], rx));
+  }
+  return o;
+}
+
 Op extrap_op(int p) {
   Op o;
   o.L = p - 1 + p % 2;
@@ -679,7 +705,25 @@ S weno_at(const Field<S>& F, const int* I, int a, double eps) {
 static void half_row(const Op& op, int i, int len, bool per, const std::vector<double>** w, int* first, int* cnt) {
   if (!per && i <= op.bpc) { *w = &op.low[i - 1]; *first = 1; *cnt = op.BL; }
   else if (!per && i > len - op.bpc) { *w = &op.high[len - i - 1]; *first = len - op.BL + 1; *cnt = op.BL; }
-  else { *w = &op.st; *first = i + 1 - op.L / 2; *cnt = op.L; }
+  else { *w = &op.row(i); *first = i + 1 - op.L / 2; *cnt = op.L; }
+}
+
+// The three half-point operators of the nonlinear Laplacian on a nonuniform axis: the outer one
+// on the midpoints of the nodes (differential_discretizer.jl:44-51), the inner derivative and
+// the max(4, p)-point interpolation on the nodes (:97-98); cached per axis.
+struct HalfNU { Op outer, inner, interp; };
+static const HalfNU& cached_half_nu(const std::vector<double>& x, int p) {
+  thread_local std::vector<std::pair<std::vector<double>, std::pair<int, std::unique_ptr<HalfNU>>>> cache;
+  for (auto& e : cache)
+    if (e.second.first == p && e.first == x) return *e.second.second;
+  auto h = std::make_unique<HalfNU>();
+  std::vector<double> mid(x.size() - 1);
+  for (size_t i = 0; i + 1 < x.size(); ++i) mid[i] = (x[i] + x[i + 1]) / 2;
+  h->outer = half_nu(1, p, mid);
+  h->inner = half_nu(1, p, x);
+  h->interp = half_nu(0, std::max(4, p), x);
+  cache.emplace_back(x, std::make_pair(p, std::move(h)));
+  return *cache.back().second.second;
 }
 
 // Half-offset operators are rebuilt for every node by the reference's rules; cache them per
@@ -705,9 +749,10 @@ S nonlinlap(const std::vector<const Field<S>*>& fields, int var, const std::vect
   const int n = sp.ax[a].n;
   const bool per = sp.ax[a].periodic;
   const double dx = sp.ax[a].dx;
-  const Op& outer = cached_half(1, p, dx);
-  const Op& inner = outer;
-  const Op& interp = cached_half(0, std::max(4, p), dx);
+  const HalfNU* nu = sp.ax[a].nonuniform ? &cached_half_nu(sp.ax[a].x, p) : nullptr;
+  const Op& outer = nu ? nu->outer : cached_half(1, p, dx);
+  const Op& inner = nu ? nu->inner : outer;
+  const Op& interp = nu ? nu->interp : cached_half(0, std::max(4, p), dx);
   const std::vector<double>* wo; int fo, co;
   half_row(outer, I[a] - 1, n - 1, per, &wo, &fo, &co);
   S acc(0.0);
@@ -1223,6 +1268,66 @@ int orc_spherical1d_rhs(int n, int p, double g1, const double* u, double t, doub
       std::vector<const Field<double>*> fl{&S.F[0]}, ax;
       CoefFn<double> coef = [](const double*, double, const int*, const double*) { return 1.0; };
       out[0] = a * (central(S.F[0], d1, I, 0) / r + nonlinlap(fl, 0, ax, coef, I, 0, S.p));
+    };
+    s.rhs(u, t, du, 1);
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
+// ---- the same two systems on nodes x[0..n-1] spanning [0, 1] (nonuniform; Dirichlet only for the
+//      nonlinear diffusion: the reference has no periodic nonuniform half-point operators)
+int orc_nonlinlap1d_nu_rhs(int n, const double* x, int p, double gl, double gh, const double* u, double t,
+                           double* du) {
+  try {
+    System<double> s;
+    s.p = p;
+    s.sp.nd = 1;
+    make_axis(s.sp.ax[0], n, x[0], x[n - 1], false);
+    for (int i = 0; i < n; ++i) s.sp.ax[0].x[i] = x[i];
+    s.sp.ax[0].nonuniform = true;
+    s.sp.ax[0].dx = std::nan("");
+    s.F.resize(1);
+    Field<double>& f = s.F[0];
+    f.side[0][0].has = true; f.side[0][0].a = {1.0}; f.side[0][0].g = [gl](double, const int*) { return gl; };
+    f.side[0][1].has = true; f.side[0][1].a = {1.0}; f.side[0][1].g = [gh](double, const int*) { return gh; };
+    box_default(s, f);
+    s.node_rhs = [](System<double>& S, const int* I, double, double* out) {
+      std::vector<const Field<double>*> fl{&S.F[0]}, ax;
+      CoefFn<double> coef = [](const double* uu, double, const int*, const double*) { return 1.0 + uu[0] * uu[0]; };
+      out[0] = nonlinlap(fl, 0, ax, coef, I, 0, S.p);
+    };
+    s.rhs(u, t, du, 1);
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
+int orc_spherical1d_nu_rhs(int n, const double* x, int p, double g1, const double* u, double t, double* du) {
+  try {
+    System<double> s;
+    s.p = p;
+    s.sp.nd = 1;
+    make_axis(s.sp.ax[0], n, x[0], x[n - 1], false);
+    for (int i = 0; i < n; ++i) s.sp.ax[0].x[i] = x[i];
+    s.sp.ax[0].nonuniform = true;
+    s.sp.ax[0].dx = std::nan("");
+    s.F.resize(1);
+    Field<double>& f = s.F[0];
+    f.side[0][0].has = true; f.side[0][0].a = {0.0, 1.0}; f.side[0][0].g = [](double, const int*) { return 0.0; };
+    f.side[0][1].has = true; f.side[0][1].a = {1.0}; f.side[0][1].g = [g1](double, const int*) { return g1; };
+    box_default(s, f);
+    const std::vector<double> xs(x, x + n);
+    auto d1 = std::make_shared<Op>(centered_nu(1, p, xs));
+    auto d2 = std::make_shared<Op>(centered_nu(2, p, xs));
+    s.node_rhs = [d1, d2](System<double>& S, const int* I, double, double* out) {
+      const double r = S.sp.ax[0].x[I[0] - 1];
+      if (std::fabs(r) <= 1e-6) { out[0] = 6.0 * central(S.F[0], *d2, I, 0); return; }
+      std::vector<const Field<double>*> fl{&S.F[0]}, ax;
+      CoefFn<double> coef = [](const double*, double, const int*, const double*) { return 1.0; };
+      out[0] = central(S.F[0], *d1, I, 0) / r + nonlinlap(fl, 0, ax, coef, I, 0, S.p);
     };
     s.rhs(u, t, du, 1);
     return 0;

@@ -37,6 +37,8 @@ def lib():
                                   C.c_double, _dp],
             "orc_nonuniform1d_rhs": [C.c_int, _dp, C.c_double, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp,
                                      C.c_double, _dp],
+            "orc_nonlinlap1d_nu_rhs": [C.c_int, _dp, C.c_int, C.c_double, C.c_double, _dp, C.c_double, _dp],
+            "orc_spherical1d_nu_rhs": [C.c_int, _dp, C.c_int, C.c_double, _dp, C.c_double, _dp],
             "orc_diffusion1d_rhs": [C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double, _dp],
             "orc_diffusion1d_integrate": [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double, C.c_double, C.c_longlong, C.c_int],
             "orc_bruss2d_rhs": [C.c_int, C.c_int, C.c_double, _dp, C.c_double, _dp, C.c_int],
@@ -132,6 +134,22 @@ def nonuniform1d_rhs(x, D, c, p, uo, bcl, bch, u, t=0.0):
     du = np.empty_like(u)
     _ok(lib().orc_nonuniform1d_rhs(len(x), x, float(D), float(c), int(p), int(uo), _f64(bcl), _f64(bch), u,
                                    float(t), du))
+    return du
+
+
+def nonlinlap1d_nu_rhs(x, p, gl, gh, u, t=0.0):
+    """u_t = d/dx((1 + u^2) u_x) on the nodes x, Dirichlet gl / gh."""
+    x, u = _f64(x), _f64(u)
+    du = np.empty_like(u)
+    _ok(lib().orc_nonlinlap1d_nu_rhs(len(x), x, int(p), float(gl), float(gh), u, float(t), du))
+    return du
+
+
+def spherical1d_nu_rhs(x, p, g1, u, t=0.0):
+    """Spherical Laplacian on the nodes x (r from 0), u_r(0) = 0, u(1) = g1."""
+    x, u = _f64(x), _f64(u)
+    du = np.empty_like(u)
+    _ok(lib().orc_spherical1d_nu_rhs(len(x), x, int(p), float(g1), u, float(t), du))
     return du
 
 

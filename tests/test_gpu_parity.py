@@ -248,6 +248,27 @@ def test_nonuniform_centered_upwind_rhs(bc, p, c, n, tile):
     assert_parity(got, ref, np.max(np.abs(u)) * (0.7 * 8 / h ** 2 + 2 / h) + 1, what="nonuniform fd")
 
 
+@pytest.mark.parametrize("p", [2, 4])
+@pytest.mark.parametrize("kind", ["nonlinlap", "spherical"])
+@pytest.mark.parametrize("n,tile", [(41, (8, 1, 1)), (3001, None)])
+def test_nonuniform_nonlinear_and_spherical_laplacians(p, kind, n, tile):
+    x = systems.stretched_grid(n, 1.0, seed=6)
+    if kind == "nonlinlap":
+        sys_, disc = systems.nonlinlap1d(x, False, 0.2, -0.3, p)
+        ref_fn = lambda w: orc.nonlinlap1d_nu_rhs(x, p, 0.2, -0.3, w)
+    else:
+        sys_, disc = systems.spherical1d(x, 0.4, p)
+        ref_fn = lambda w: orc.spherical1d_nu_rhs(x, p, 0.4, w)
+    prob = molgpu.discretize(sys_, disc)
+    if tile is not None:
+        prob.set_tile(*tile)
+    u = np.random.default_rng(p).uniform(-1, 1, prob.lowered.n_unknowns)
+    got = gpu_rhs(prob, u)
+    ref = ref_fn(u)
+    h = np.min(np.diff(x))
+    assert_parity(got, ref, 2 * 8 / h ** 2 + 1, what="nonuniform %s p=%d" % (kind, p))
+
+
 def test_nonuniform_operators_refuse_the_interpreter(monkeypatch):
     """With code generation switched off the handle fails loudly instead of running a kernel
     that has no per-node rows."""

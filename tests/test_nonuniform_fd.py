@@ -116,7 +116,63 @@ def test_lowering_rejects_what_the_reference_does_not_build():
     with pytest.raises(NotImplementedError):
         lower(*systems.nonuniform1d(x, 0.0, 1.0, 2, 2))          # upwind order 2
     with pytest.raises(NotImplementedError):
-        lower(*systems.nonlinlap1d(x, False))                    # half-point operators
+        lower(*systems.nonlinlap1d(x, True))                     # periodic nonuniform half-point operators
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_uniform_nodes_reproduce_the_uniform_nonlinear_and_spherical_laplacians(p):
+    for n in (21, 41):
+        x = np.linspace(0.0, 1.0, n)
+        u = np.random.default_rng(n + p).uniform(-1, 1, n - 2)
+        a = orc.nonlinlap1d_nu_rhs(x, p, 0.2, -0.3, u)
+        b = orc.nonlinlap1d_rhs(n, u, False, 0.2, -0.3, p=p)
+        assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
+        a = orc.spherical1d_nu_rhs(x, p, 0.4, u)
+        b = orc.spherical1d_rhs(n, 0.4, u, p=p)
+        assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
+
+
+def test_nonlinear_laplacian_closed_form_on_a_nonuniform_grid():
+    """p = 2, a = 1 + u^2, u = al + be x: the inner two-point difference is exact (be), the
+    4-point interpolation reproduces u at the half points, and the outer two-point difference
+    over the midpoints differentiates the quadratic a(u) be exactly at the midpoint of the two
+    midpoints, xb = (x[i-1] + 2 x[i] + x[i+1]) / 4, where the reference's outer operator, built
+    on the midpoints (differential_discretizer.jl:44-51), evaluates it."""
+    x = systems.stretched_grid(41, 1.0, seed=6, spread=0.6)
+    al, be = 0.3, 0.7
+    du = orc.nonlinlap1d_nu_rhs(x, 2, al, al + be, al + be * x[1:-1])
+    xb = (x[:-2] + 2 * x[1:-1] + x[2:]) / 4
+    ref = 2 * be ** 2 * (al + be * xb)
+    assert np.max(np.abs(du - ref)) <= 1e-10
+    # and not at the node itself
+    assert np.max(np.abs(du - 2 * be ** 2 * (al + be * x[1:-1]))) > 1e-4
+
+
+@pytest.mark.parametrize("p", [2, 4])
+@pytest.mark.parametrize("kind", ["nonlinlap", "spherical"])
+def test_generated_nonlinear_and_spherical_laplacians_match_oracle(p, kind):
+    x = systems.stretched_grid(41, 1.0, seed=6)
+    if kind == "nonlinlap":
+        low = lower(*systems.nonlinlap1d(x, False, 0.2, -0.3, p))
+        ref_fn = lambda w: orc.nonlinlap1d_nu_rhs(x, p, 0.2, -0.3, w)
+    else:
+        low = lower(*systems.spherical1d(x, 0.4, p))
+        ref_fn = lambda w: orc.spherical1d_nu_rhs(x, p, 0.4, w)
+    assert low.n_unknowns == len(x) - 2
+    u = np.random.default_rng(p).uniform(-1, 1, low.n_unknowns)
+    ref = ref_fn(u)
+    for tile in [(8, 1, 1), None]:
+        got = genhost.rhs(low, u, tile=tile)
+        assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+    rng = np.random.default_rng(9)
+    lam, v = rng.uniform(-1, 1, low.n_unknowns), rng.uniform(-1, 1, low.n_unknowns)
+    g, _ = genhost.vjp(low, u, lam, tile=(8, 1, 1))
+    F = lambda w: lam @ ref_fn(w)
+    h = 1e-5
+    d1 = (F(u + h * v) - F(u - h * v)) / (2 * h)
+    d2 = (F(u + h / 2 * v) - F(u - h / 2 * v)) / h
+    fd = (4 * d2 - d1) / 3
+    assert abs(g @ v - fd) <= 1e-8 * (abs(fd) + 1)
 
 
 @pytest.mark.parametrize("p", [2, 4])
