@@ -21,6 +21,8 @@ The same JSON line carries the other parts of the metric:
                of the automatic mode; gradient errors against the oracle's forward mode and
                against central differences;
   batch        512 diffusion-reaction instances with uneven grids, FP32 + Kahan, 200 RK4 steps;
+  heat         1D heat, 101 nodes, RK4 at dt = 1e-5 to t = 0.1: time per step and the error
+               against the analytic solution and against the oracle's RK4;
   cpu_baseline the C++ CPU restatement (the oracle) on the host, 1 thread, bounded sample.
 """
 import argparse
@@ -100,8 +102,9 @@ def bench_weno_solve(n, nsteps, oracle_steps):
     p64 = molgpu.discretize(*systems.burgers2d(n=n + 1))
     p32 = molgpu.discretize(*systems.burgers2d(n=n + 1, dtype="f32"))
     u0 = np.asarray(p64.u0, dtype=np.float64)
-    dx = 1.0 / n
-    dt = 0.4 * dx / float(np.max(np.abs(u0)))
+    dx = dy = 1.0 / n
+    umax = float(np.max(np.abs(u0)))
+    dt = 0.4 / (umax / dx + umax / dy)          # CFL 0.4 over both axes
     res = {"grid": "%dx%d" % (n, n), "scheme": "SSPRK33", "cfl": 0.4, "dt": dt, "steps": nsteps}
     finals = {}
     for name, prob, kahan in (("fp64", p64, False), ("fp32_kahan", p32, True), ("fp32", p32, False)):
@@ -261,13 +264,15 @@ def bench_adjoint(n, nsteps, every, fd_cells):
 
     rng = np.random.default_rng(7)
     cells = rng.choice(ng, size=fd_cells, replace=False)
-    fd_err = 0.0
+    fd_err, fd_abs, fd_max = 0.0, 0.0, 0.0
     for c in cells:
         h = 1e-2 * d0[c]
         D1 = dloss(h, c) / (2 * h)
         D2 = dloss(h / 2, c) / h
         fd = (4 * D2 - D1) / 3        # Richardson: O(h^4)
         fd_err = max(fd_err, abs(gd0[c] - fd) / max(abs(fd), 1e-300))
+        fd_abs = max(fd_abs, abs(gd0[c] - fd))
+        fd_max = max(fd_max, abs(fd))
     d0t.copy_(torch.tensor(d0, device="cuda"))
     # oracle forward mode on a 16^3 copy (same construction), direction seeded
     p16, d016, src16, u016, tgt16, dt16 = make(16)
@@ -285,7 +290,39 @@ def bench_adjoint(n, nsteps, every, fd_cells):
     return {"grid": "%d^3" % n, "unknowns": p.lowered.n_unknowns, "steps": nsteps, "dt": dt,
             "forward_ms": fwd_ms, "ckpt_every": auto, "adjoint_ms": best["adjoint_ms"], "ratio": best["ratio"],
             "modes": modes, "rhs_us": rhs_us, "grad_rel_err_forward_mode_16": fm_err,
-            "grad_rel_err_fd_cells": fd_err, "fd_cells": int(fd_cells)}
+            "grad_rel_err_fd_cells": fd_err, "grad_err_fd_cells_vs_max": fd_abs / max(fd_max, 1e-300),
+            "fd_cells": int(fd_cells)}
+
+
+def bench_heat(n=101, D=1.0, dt=1e-5, t_end=0.1):
+    """Config 0: 1D heat u_t = D u_xx, Dirichlet 0, u0 = sin(pi x), RK4 at dt to t_end: time per
+    step, error against the analytic solution exp(-pi^2 D t) sin(pi x) and against the oracle's RK4
+    over the same steps."""
+    import numpy as np
+    import torch
+    import molgpu
+    from molgpu import systems
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    prob = molgpu.discretize(*systems.heat1d(n=n, D=D))
+    nsteps = int(round(t_end / dt))
+    u = torch.tensor(prob.u0, dtype=torch.float64, device="cuda")
+    prob.integrate(u, 0.0, dt, 100, molgpu.RK4, graph=True)
+    u = torch.tensor(prob.u0, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    e0, e1 = events()
+    e0.record()
+    prob.integrate(u, 0.0, dt, nsteps, molgpu.RK4, graph=True)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    got = u.cpu().numpy()
+    x = prob.lowered.coords[0][1:-1]
+    exact = np.exp(-np.pi ** 2 * D * t_end) * np.sin(np.pi * x)
+    ref = orc.diffusion1d_integrate(n, D, (1.0, 0.0, 0.0), (1.0, 0.0, 0.0), prob.u0, 0.0, dt, nsteps)
+    return {"n": n, "steps": nsteps, "dt": dt, "ms": ms, "us_per_step": ms * 1e3 / nsteps,
+            "max_err_vs_analytic": float(np.max(np.abs(got - exact))),
+            "max_diff_vs_oracle_rk4": float(np.max(np.abs(got - ref)))}
 
 
 def bench_batch(count, nsteps, quick):
@@ -363,7 +400,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only (no weno/adjoint/batch sections)")
     ap.add_argument("--section", default="all",
-                    choices=["all", "weno", "wenosolve", "adjoint", "batch", "rhs3d", "copy"],
+                    choices=["all", "weno", "wenosolve", "adjoint", "batch", "rhs3d", "copy", "heat"],
                     help="run one section alone (for profiling); prints its JSON")
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--reps", type=int, default=200, help="back-to-back RHS launches of the weno/rhs3d sections")
@@ -386,6 +423,8 @@ def main():
             res = bench_adjoint(128, args.adj_steps, 20, 2)
         elif args.section == "batch":
             res = bench_batch(512, 50, quick=True)
+        elif args.section == "heat":
+            res = bench_heat()
         elif args.section == "copy":
             # calibration of the HBM byte counters: device copies of a known size
             x = torch.ones(8 << 20, dtype=torch.float64, device="cuda")
@@ -479,7 +518,8 @@ def main():
     if not args.quick:
         out["weno5"] = {"1024": bench_weno(1024, 200), "4096": bench_weno(4096, 20),
                         "solve_1024": bench_weno_solve(1024, 500, 3)}
-        out["adjoint"] = bench_adjoint(128, 200, 20, 8)
+        out["adjoint"] = bench_adjoint(128, 200, 20, 32)
+        out["heat"] = bench_heat()
         out["batch"] = bench_batch(512, 200, quick=False)
 
     cpu = None
