@@ -98,6 +98,32 @@ def diffreact2d(n: int, D: float, k: float, bctype=(0, 0, 0, 0), dtype: str = "f
     return sys_, MOLFiniteDifference({x: n, y: n}, t, dtype=dtype)
 
 
+def diffreact_grids(grids, D: float, k: float, bctype, approx_order: int = 2, dtype: str = "f64"):
+    """u_t = D Lap u + k u (1 - u) on [0,1]^d, d = len(grids); grids[a]: a node count or the nodes
+    (spanning [0, 1]); bctype per side (x-lo, x-hi, y-lo, ...): 0 Dirichlet u = 0, 1 Neumann 0."""
+    names = "x y z"[:2 * len(grids) - 1]
+    xs = list(parameters("t " + names))
+    t, xs = xs[0], xs[1:]
+    u = variables("u")
+    Dt = Differential(t)
+    U = u(t, *xs)
+    eq = Eq(Dt(U), D * sum((Differential(x) ** 2)(U) for x in xs) + k * U * (1 - U))
+    ic = 1
+    for x in xs:
+        ic = ic * sp.sin(sp.pi * x)
+    bcs = [Eq(u(0, *xs), ic)]
+    for a, x in enumerate(xs):
+        for side in (0, 1):
+            args = [t] + list(xs)
+            args[a + 1] = side
+            if bctype[2 * a + side] == 0:
+                bcs.append(Eq(u(*args), 0))
+            else:
+                bcs.append(Eq(Differential(x)(u(*args)), 0))
+    sys_ = PDESystem([eq], bcs, [Interval(t, 0.0, 1.0)] + [Interval(x, 0.0, 1.0) for x in xs], [t] + xs, [U])
+    return sys_, MOLFiniteDifference({x: g for x, g in zip(xs, grids)}, t, approx_order=approx_order, dtype=dtype)
+
+
 def nonlindiff3d(n: int = 128, approx_order: int = 2, dtype: str = "f64"):
     """u_t = div(d0 (1 + u^2) grad u) + s on [0,1]^3 with Neumann BCs; d0 and s are grid fields."""
     t, x, y, z = parameters("t x y z")

@@ -1144,6 +1144,50 @@ int orc_diffreact2d_rhs(int n, double D, double k, const int* bctype, const doub
   }
 }
 
+// ---- u_t = D sum_a Daa u + k u (1 - u) in nd = 1..3 dimensions on [0,1]^nd; axis a has n[a]
+//      nodes, uniform, or the given nodes x[a] (nonuniform) when x[a] is not null; approximation
+//      order p; side (a, s) Dirichlet 0 (bctype 0) or Neumann 0 (bctype 1).
+int orc_diffreact_nd_rhs(int nd, const int* n, const double* const* x, int p, double D, double k,
+                         const int* bctype, const double* u, double t, double* du) {
+  try {
+    System<double> s;
+    s.p = p;
+    s.sp.nd = nd;
+    std::vector<std::shared_ptr<Op>> ops;
+    for (int a = 0; a < nd; ++a) {
+      make_axis(s.sp.ax[a], n[a], 0.0, 1.0, false);
+      if (x && x[a]) {
+        for (int i = 0; i < n[a]; ++i) s.sp.ax[a].x[i] = x[a][i];
+        s.sp.ax[a].nonuniform = true;
+        s.sp.ax[a].dx = std::nan("");
+        ops.push_back(std::make_shared<Op>(centered_nu(2, p, s.sp.ax[a].x)));
+      } else {
+        ops.push_back(std::make_shared<Op>(centered(2, p, s.sp.ax[a].dx)));
+      }
+    }
+    s.F.resize(1);
+    Field<double>& f = s.F[0];
+    for (int a = 0; a < nd; ++a)
+      for (int sd = 0; sd < 2; ++sd) {
+        f.side[a][sd].has = true;
+        f.side[a][sd].a = bctype[2 * a + sd] == 0 ? std::vector<double>{1.0} : std::vector<double>{0.0, 1.0};
+        f.side[a][sd].g = [](double, const int*) { return 0.0; };
+      }
+    box_default(s, f);
+    f.p = p;
+    s.node_rhs = [D, k, ops, nd](System<double>& S, const int* I, double, double* out) {
+      const double uu = S.F[0].get(I[0], I[1], I[2]);
+      double lap = 0.0;
+      for (int a = 0; a < nd; ++a) lap += central(S.F[0], *ops[a], I, a);
+      out[0] = D * lap + k * uu * (1.0 - uu);
+    };
+    s.rhs(u, t, du, 1);
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
 }  // extern "C"
 
 // ---- 3D nonlinear diffusion u_t = sum_a Da(d0 (1 + u^2) Da u) + s on [0,1]^3, Neumann 0.

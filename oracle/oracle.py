@@ -39,6 +39,8 @@ def lib():
                                      C.c_double, _dp],
             "orc_nonlinlap1d_nu_rhs": [C.c_int, _dp, C.c_int, C.c_double, C.c_double, _dp, C.c_double, _dp],
             "orc_spherical1d_nu_rhs": [C.c_int, _dp, C.c_int, C.c_double, _dp, C.c_double, _dp],
+            "orc_diffreact_nd_rhs": [C.c_int, _ip, C.POINTER(C.c_void_p), C.c_int, C.c_double, C.c_double, _ip,
+                                     _dp, C.c_double, _dp],
             "orc_diffusion1d_rhs": [C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double, _dp],
             "orc_diffusion1d_integrate": [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double, C.c_double, C.c_longlong, C.c_int],
             "orc_bruss2d_rhs": [C.c_int, C.c_int, C.c_double, _dp, C.c_double, _dp, C.c_int],
@@ -150,6 +152,20 @@ def spherical1d_nu_rhs(x, p, g1, u, t=0.0):
     x, u = _f64(x), _f64(u)
     du = np.empty_like(u)
     _ok(lib().orc_spherical1d_nu_rhs(len(x), x, int(p), float(g1), u, float(t), du))
+    return du
+
+
+def diffreact_nd_rhs(grids, p, D, k, bctype, u, t=0.0):
+    """u_t = D Lap u + k u (1 - u) on [0,1]^d; grids[a]: a node count (uniform) or the nodes;
+    bctype per side (x-lo, x-hi, y-lo, ...): 0 Dirichlet 0, 1 Neumann 0."""
+    nd = len(grids)
+    keep = [None if np.isscalar(g) else _f64(g) for g in grids]
+    n = np.array([int(g) if np.isscalar(g) else len(g) for g in grids] + [1] * (3 - nd), dtype=np.int32)
+    xs = (C.c_void_p * 3)(*[(k_.ctypes.data if k_ is not None else None) for k_ in keep] + [None] * (3 - nd))
+    u = _f64(u)
+    du = np.empty_like(u)
+    _ok(lib().orc_diffreact_nd_rhs(nd, n, xs, int(p), float(D), float(k),
+                                   np.ascontiguousarray(bctype, dtype=np.int32), u, float(t), du))
     return du
 
 

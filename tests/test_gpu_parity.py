@@ -269,6 +269,32 @@ def test_nonuniform_nonlinear_and_spherical_laplacians(p, kind, n, tile):
     assert_parity(got, ref, 2 * 8 / h ** 2 + 1, what="nonuniform %s p=%d" % (kind, p))
 
 
+@pytest.mark.parametrize("which", ["x", "xy", "xz3d"])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_nonuniform_multidimensional_diffusion(which, dtype):
+    """Per-node rows on 2D / 3D grids (small tiles and the default), FP64 at 1e-12 and FP32
+    against the FP64 oracle at a single-precision tolerance."""
+    xg = systems.stretched_grid(129 if "3d" not in which else 40, 1.0, seed=4)
+    yg = systems.stretched_grid(97, 1.0, seed=5)
+    zg = systems.stretched_grid(33, 1.0, seed=7)
+    grids = {"x": [xg, 97], "xy": [xg, yg], "xz3d": [xg, 36, zg]}[which]
+    bct = (0, 1, 1, 0) + ((1, 0) if len(grids) == 3 else ())
+    sys_, disc = systems.diffreact_grids(grids, 0.05, 2.5, bct, 2, dtype=dtype)
+    for tile in ([(16, 8, 1), None] if len(grids) == 2 else [(8, 4, 4), None]):
+        prob = molgpu.discretize(sys_, disc)
+        if tile is not None:
+            prob.set_tile(*tile)
+        u = np.random.default_rng(2).uniform(0, 1, prob.lowered.n_unknowns)
+        got = gpu_rhs(prob, u)
+        ref = orc.diffreact_nd_rhs(grids, 2, 0.05, 2.5, bct, u)
+        h = min(np.min(np.diff(g)) for g in grids if not np.isscalar(g))
+        scale = 0.05 * len(grids) * 4 / h ** 2 + 3
+        if dtype == "f64":
+            assert_parity(got, ref, scale, what="nonuniform %s" % which)
+        else:
+            assert np.max(np.abs(got - ref)) <= 2e-6 * scale, np.max(np.abs(got - ref))
+
+
 def test_nonuniform_operators_refuse_the_interpreter(monkeypatch):
     """With code generation switched off the handle fails loudly instead of running a kernel
     that has no per-node rows."""

@@ -203,3 +203,31 @@ def test_generated_reverse_mode_matches_finite_differences(c):
     F = lambda w: lam @ orc.nonuniform1d_rhs(x, 0.7, c, 4, 1, bcl, bch, w)
     fd = (F(u + v) - F(u - v)) / 2
     assert abs(g @ v - fd) <= 1e-10 * (abs(fd) + np.linalg.norm(g) * np.linalg.norm(v))
+
+
+@pytest.mark.parametrize("bct", [(0, 0, 0, 0), (0, 1, 1, 0), (1, 1, 1, 1)])
+def test_uniform_counts_reproduce_the_2d_oracle(bct):
+    n = 24
+    u = np.random.default_rng(1).uniform(0, 1, (n - 2) ** 2)
+    a = orc.diffreact_nd_rhs([n, n], 2, 0.05, 2.5, bct, u)
+    b = orc.diffreact2d_rhs(n, 0.05, 2.5, bct, u)
+    assert np.max(np.abs(a - b)) <= 1e-14 * np.max(np.abs(b))
+
+
+@pytest.mark.parametrize("which", ["x", "y", "xy", "xz3d"])
+@pytest.mark.parametrize("p", [2, 4])
+def test_generated_multidimensional_nonuniform_matches_oracle(which, p):
+    """Per-node rows on one or more axes of a 2D / 3D grid, with the register-blocked axis among
+    them (tiles 8x4, 16x8, 8x4x4 and the default), Dirichlet / Neumann sides."""
+    xg = systems.stretched_grid(29 if "3d" not in which else 14, 1.0, seed=4)
+    yg = systems.stretched_grid(23, 1.0, seed=5)
+    zg = systems.stretched_grid(11, 1.0, seed=7)
+    grids = {"x": [xg, 23], "y": [31, yg], "xy": [xg, yg], "xz3d": [xg, 12, zg]}[which]
+    bct = (0, 1, 1, 0) + ((1, 0) if len(grids) == 3 else ())
+    low = lower(*systems.diffreact_grids(grids, 0.05, 2.5, bct, p))
+    u = np.random.default_rng(p).uniform(0, 1, low.n_unknowns)
+    ref = orc.diffreact_nd_rhs(grids, p, 0.05, 2.5, bct, u)
+    tiles = [(8, 4, 4), None] if len(grids) == 3 else [(8, 4, 1), (16, 8, 1), None]
+    for tile in tiles:
+        got = genhost.rhs(low, u, tile=tile)
+        assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
