@@ -381,6 +381,28 @@ class ODESolution:
         f = self._field(var)
         return np.stack([self.prob.unflatten(u)[f] for u in self.u])
 
+    def __call__(self, t, *x, dv=None):
+        """Value of variable ``dv`` (default: the first) at time t and position x by multilinear
+        interpolation over the save times and the grid, as the reference's solution interpolates
+        (Gridded(Linear()), src/interface/solution/solution_utils.jl:3-32).  Arguments broadcast
+        against each other."""
+        from scipy.interpolate import RegularGridInterpolator
+        low = self.prob.lowered
+        if len(x) != low.ndim:
+            raise ValueError("expected %d spatial coordinates" % low.ndim)
+        f = 0 if dv is None else self._field(dv)
+        if len(self.t) < 2:
+            raise ValueError("interpolation needs at least two saved times")
+        key = ("interp", f)
+        cache = self.__dict__.setdefault("_interp", {})
+        if key not in cache:
+            grid = (np.asarray(self.t, dtype=np.float64),) + tuple(np.asarray(c, dtype=np.float64)
+                                                                   for c in low.coords[:low.ndim])
+            cache[key] = RegularGridInterpolator(grid, np.stack([g[f] for g in self.full]), method="linear")
+        pts = np.broadcast_arrays(np.asarray(t, dtype=np.float64), *[np.asarray(v, dtype=np.float64) for v in x])
+        out = cache[key](np.stack([p_.reshape(-1) for p_ in pts], axis=-1))
+        return out.reshape(pts[0].shape) if pts[0].ndim else float(out[0])
+
 
 def solve(prob: ODEProblem, alg=RK4, dt: float = None, adaptive: bool = False, saveat=None,
           kahan: bool = False, graph: bool = False, tspan=None) -> ODESolution:

@@ -351,3 +351,27 @@ def test_batched_kernel_has_one_class_per_boundary_pattern():
     src = codegen_batched(lows, compile=True)
     assert src.count("struct Geo") == len(pats)
     assert "hd.W[" in src and "hd.K[" in src and "mol_batch_tiles" in src
+
+
+def test_solution_interpolation_is_multilinear():
+    """sol(t, x, y; dv) interpolates the saved full grids linearly in t and in each axis
+    (Gridded(Linear()), src/interface/solution/solution_utils.jl:3-32): exact on data that is
+    linear in each variable separately, equal to the grid values at grid points."""
+    import numpy as np
+    import molgpu
+    from molgpu import systems
+    from molgpu.discretization import lower
+    low = lower(*systems.diffreact2d(9, 0.05, 2.5, (0, 1, 1, 0)))
+    prob = molgpu.ODEProblem(low)
+    x, y = low.coords[0], low.coords[1]
+    ts = [0.0, 0.5, 1.5]
+    f = lambda t, X, Y: (1 + 2 * t) * (0.3 + X) * (2 - Y) + 0.7 * t * Y
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    full = [np.stack([f(t, X, Y)]) for t in ts]
+    sol = molgpu.ODESolution(ts, [None] * 3, prob, full)
+    rng = np.random.default_rng(0)
+    tq, xq, yq = rng.uniform(0, 1.5, 50), rng.uniform(0, 1, 50), rng.uniform(0, 1, 50)
+    np.testing.assert_allclose(sol(tq, xq, yq, dv="u"), f(tq, xq, yq), rtol=1e-13, atol=1e-13)
+    assert sol(0.5, x[3], y[5]) == f(0.5, x[3], y[5])
+    np.testing.assert_array_equal(sol["u"], np.stack([g[0] for g in full]))
+    np.testing.assert_array_equal(sol[low.axes[1]], y)
