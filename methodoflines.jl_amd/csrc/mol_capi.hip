@@ -626,9 +626,12 @@ int kernel_threads(const JitGeom& g, bool vjp) {
 // tile + 4-node halo of u and two stage buffers in LDS, one launch per step.
 bool fused_geometry(mol_handle* h, const mol_desc* d, const KArgs<double>& ka, const KCore<double>& kc, int ncore,
                     JitGeom& g, std::string& why) {
-  if (d->ndim > 2 || d->nclosures > 0) { why = "not a periodic 1D/2D system"; return false; }
+  if (d->ndim > 2) { why = "not a 1D/2D system"; return false; }
   for (int a = 0; a < d->ndim; ++a)
-    if (!ka.periodic[a] || ka.halo[a] > 1) { why = "not periodic with reach 1"; return false; }
+    if (ka.halo[a] > 1) { why = "reach above 1"; return false; }
+  // a ghost's closure reads unknowns inward from the box edge: they must lie in the tile region
+  int kmax = 0;
+  for (int c = 0; c < d->nclosures; ++c) kmax = std::max(kmax, d->closures_h[c].K + d->closures_h[c].nghost);
   static const int cand2[][3] = {{32, 16, 1}, {32, 8, 1}, {64, 8, 1}, {16, 16, 1}, {16, 8, 1}};
   static const int cand1[][3] = {{256, 1, 1}, {128, 1, 1}, {64, 1, 1}};
   const int (*cand)[3] = d->ndim == 2 ? cand2 : cand1;
@@ -663,6 +666,10 @@ bool fused_geometry(mol_handle* h, const mol_desc* d, const KArgs<double>& ka, c
     g.ntile[a] = (ext + g.tile[a] - 1) / g.tile[a];
     g.ext[a] = g.tile[a] + 2 * g.halo[a];
     g.vol *= g.ext[a];
+    if (a < d->ndim && !ka.periodic[a] && (g.tile[a] < kmax || ext < kmax)) {
+      why = "tile narrower than the boundary closures";
+      return false;
+    }
   }
   // two threads per tile node (every stage region is larger than the tile), at most 1024
   g.nthr = std::max(256, std::min(1024, (2 * g.tile[0] * g.tile[1] * g.tile[2] + 63) / 64 * 64));

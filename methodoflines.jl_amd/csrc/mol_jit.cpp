@@ -323,7 +323,7 @@ class Gen {
   void print_node(std::ostream& o, int id) const;
   void print_reverse(std::ostream& o, int r, const std::vector<std::pair<int, std::string>>& seeds) const;
   void partials(const Node& n, int id, std::vector<std::pair<int, std::string>>& out) const;
-  std::string stage_fn();
+  std::string stage_fn(bool lds = false);
   std::string scatter_fn();
   PrintGeo make_pg(const int* halo, const Usage& us) const;
   std::string geo_variant(const std::string& name, const int* halo, const Usage& us, bool core, int region,
@@ -1083,25 +1083,46 @@ void Gen::print_reverse(std::ostream& o, int r, const std::vector<std::pair<int,
 // Slow staging of one position: ring wrap, zero outside the grid and at corners, the unknown
 // inside the variable's box, else the ghost-node closure of the axis it leaves the box along
 // (field_at / closure_value of the interpreter; src/discretization/generate_bc_eqs.jl).
-std::string Gen::stage_fn() {
+std::string Gen::stage_fn(bool lds) {
   std::ostringstream o;
-  o << "  static __device__ T stage(" << (param_ ? "const BInst& hd, " : "")
-    << "const int f, int p0, int p1, int p2, const double t, const T* __restrict__ P,\n"
-       "                           const T* __restrict__ u, const JPtrs& jp) {\n"
-       "    (void)t; (void)P; (void)jp;\n";
+  // lds: the same values from the stage values in LDS (tile + halo with first position o0, o1,
+  // o2), positions unwrapped; only non-periodic axes have ghosts
+  auto rd = [&](int f, const std::string& q0, const std::string& q1, const std::string& q2) {
+    if (!lds) return "u[" + UIDX(f, q0, q1, q2) + "]";
+    return "L[" + istr(f) + " * VOL + ((" + q0 + ") - o0) + EX * (((" + q1 + ") - o1) + EY * ((" + q2 + ") - o2))]";
+  };
+  if (lds)
+    o << "  static __device__ T stage_lds(const int f, int p0, int p1, int p2, const double t, const T* __restrict__ P,\n"
+         "                               const T* __restrict__ L, const int o0, const int o1, const int o2,\n"
+         "                               const JPtrs& jp) {\n"
+         "    (void)t; (void)P; (void)jp;\n";
+  else
+    o << "  static __device__ T stage(" << (param_ ? "const BInst& hd, " : "")
+      << "const int f, int p0, int p1, int p2, const double t, const T* __restrict__ P,\n"
+         "                           const T* __restrict__ u, const JPtrs& jp) {\n"
+         "    (void)t; (void)P; (void)jp;\n";
   for (int a = 0; a < 3; ++a)
-    if (g_.periodic[a]) o << "    p" << a << " = mol_wrap(p" << a << ", " << NN(a) << ");\n";
-  o << "    if (p0 < 1 || p0 > " << NN(0) << " || p1 < 1 || p1 > " << NN(1) << " || p2 < 1 || p2 > " << NN(2)
-    << ") return T(0);\n";
+    if (g_.periodic[a] && !lds) o << "    p" << a << " = mol_wrap(p" << a << ", " << NN(a) << ");\n";
+  {
+    std::string cond;
+    for (int a = 0; a < 3; ++a) {
+      if (lds && g_.periodic[a]) continue;
+      const std::string c = "p" + istr(a) + " < 1 || p" + istr(a) + " > " + NN(a);
+      cond = cond.empty() ? c : cond + " || " + c;
+    }
+    if (!cond.empty()) o << "    if (" << cond << ") return T(0);\n";
+  }
   o << "    switch (f) {\n";
   for (int f = 0; f < d_->nfields; ++f) {
     o << "    case " << f << ": {\n";
-    o << "    const int nout = ";
+    o << "    const int nout = 0";
     for (int a = 0; a < 3; ++a)
-      o << (a ? " + " : "") << "(p" << a << " < " << LO(f, a) << " || p" << a << " > " << HI(f, a) << ")";
-    o << ";\n    if (nout == 0) return u[" << UIDX(f, "p0", "p1", "p2") << "];\n    if (nout > 1) return T(0);\n";
+      if (!(lds && g_.periodic[a]))
+        o << " + (p" << a << " < " << LO(f, a) << " || p" << a << " > " << HI(f, a) << ")";
+    o << ";\n    if (nout == 0) return " << rd(f, "p0", "p1", "p2") << ";\n    if (nout > 1) return T(0);\n";
     for (int a = 0; a < d_->ndim; ++a)
       for (int side = 0; side < 2; ++side) {
+        if (lds && g_.periodic[a]) continue;
         int cid = -1;
         for (int c = 0; c < d_->nclosures; ++c) {
           const mol_closure& cl = d_->closures_h[c];
@@ -1135,7 +1156,7 @@ std::string Gen::stage_fn() {
             if (c == 0.0 && !param_) continue;
             std::string q[3] = {"p0", "p1", "p2"};
             q[a] = closure_pos(f, a, side, m);
-            const std::string term = pw(c, cl.wC + gi * cl.K + m) + " * u[" + UIDX(f, q[0], q[1], q[2]) + "]";
+            const std::string term = pw(c, cl.wC + gi * cl.K + m) + " * " + rd(f, q[0], q[1], q[2]);
             e = e.empty() ? term : e + " + " + term;
           }
           if (cl.gprog >= 0) {
@@ -1564,17 +1585,18 @@ std::string Gen::source_kernel(bool f64, bool vjp) {
 // and only the tile's new state is written.  The update arithmetic is the stage-kernel
 // epilogue's, so the step equals four stage launches.
 std::string Gen::source_fused(bool f64) {
-  for (int a = 0; a < d_->ndim; ++a)
-    if (!g_.periodic[a]) { bad("fused RK4: axis " + istr(a) + " is not periodic"); return ""; }
   if (d_->ndim > 2) { bad("fused RK4: more than two dimensions"); return ""; }
-  if (d_->nclosures > 0) { bad("fused RK4: ghost closures"); return ""; }
+  bool allper = true;
+  for (int a = 0; a < d_->ndim; ++a) allper = allper && g_.periodic[a];
   for (int f = 1; f < d_->nfields; ++f)
     for (int a = 0; a < 3; ++a)
       if (g_.lo[f][a] != g_.lo[0][a] || g_.hi[f][a] != g_.hi[0][a]) { bad("fused RK4: unequal unknown boxes"); return ""; }
   std::vector<int> out_core;
   int r_core = -1;
   fused_ = true;
-  build_node(true, out_core, r_core);
+  // non-periodic axes: the node code with boundary rows (branches on the node index); the
+  // ghost positions of each stage's values are refilled from their closures (stage_lds)
+  build_node(allper, out_core, r_core);
   if (!ok_) return "";
   Usage us = usage();
   for (int k = d_->nfields; k < d_->nfields + d_->naux; ++k)
@@ -1582,6 +1604,7 @@ std::string Gen::source_fused(bool f64) {
   for (int f = 0; f < d_->nfields; ++f) us.staged[f] = true;
   int halo[3] = {0, 0, 0}, reach[3] = {0, 0, 0};
   for (int a = 0; a < d_->ndim; ++a) {
+    if (us.reach[a] > 1) { bad("fused RK4: reach above 1"); return ""; }
     reach[a] = us.reach[a];
     halo[a] = 4 * us.reach[a];
   }
@@ -1596,12 +1619,14 @@ std::string Gen::source_fused(bool f64) {
   o << "__shared__ double mol_fc[3][" << emax << "];\n__shared__ int mol_fo[3];\n\n";
   const std::string stage = stage_fn();
   o << "struct GeoBase {\n" << geo_common() << stage << "};\n\n";
-  std::string var = geo_variant("GeoF", halo, us, true, r_core, out_core, false);
+  std::string var = geo_variant("GeoF", halo, us, allper, r_core, out_core, false);
   // stage regions: the reach per axis
   const std::string close = "};\n\n";
   var = var.substr(0, var.size() - close.size());
   o << var;
   o << "  static constexpr int RX = " << reach[0] << ", RY = " << reach[1] << ", RZ = " << reach[2] << ";\n";
+  o << "  static constexpr bool FIX = " << (allper ? "false" : "true") << ";\n";
+  if (!allper) o << stage_fn(true);
   o << close;
   o << "extern \"C\" __global__ void __launch_bounds__(" << g_.nthr
     << ") mol_rk4_fused(const T* __restrict__ u, T* __restrict__ unew, Epi ep, JPtrs jp) {\n"

@@ -577,6 +577,40 @@ __device__ __forceinline__ void mol_rk4_stage(const T* Lin, T* Lout, const T* U,
   }
 }
 
+// After stage S (1..3) of a fused step on a non-periodic system: the positions of the stage
+// region (tile + (4 - S) reach) outside the unknown box along a non-periodic axis get the value
+// of their boundary equations from the stage values just written, at the next stage's time
+// (what staging does for each stage launch of the unfused path).  Returns whether it wrote.
+template <class G, int S>
+__device__ __forceinline__ bool mol_rk4_fix_ghosts(T* L, const int o0, const int o1, const int o2, const double tn,
+                                                   const T* __restrict__ P, const JPtrs& jp) {
+  constexpr int M = 4 - S;
+  constexpr int W0 = G::TX + 2 * M * G::RX, W1 = G::TY + 2 * M * G::RY, W2 = G::TZ + 2 * M * G::RZ;
+  constexpr int NV = W0 * W1 * W2;
+  // the whole work-group takes the same branch: the region reaches outside the box or not
+  const bool out0 = !G::PER0 && (o0 - M * G::RX < (int)G::ULO0 || o0 + G::TX - 1 + M * G::RX > (int)G::UHI0);
+  const bool out1 = !G::PER1 && G::RY > 0 && (o1 - M * G::RY < (int)G::ULO1 || o1 + G::TY - 1 + M * G::RY > (int)G::UHI1);
+  if (!out0 && !out1) return false;
+  for (int q = threadIdx.x; q < NV; q += G::NTHR) {
+    const int z = q / (W0 * W1);
+    const int r = q - z * (W0 * W1);
+    const int y = r / W0;
+    const int x = r - y * W0;
+    const int lx = x - M * G::RX, ly = y - M * G::RY, lz = z - M * G::RZ;
+    const int p0 = o0 + lx, p1 = o1 + ly, p2 = o2 + lz;
+    const bool ghost = (!G::PER0 && (p0 < (int)G::ULO0 || p0 > (int)G::UHI0)) ||
+                       (!G::PER1 && G::RY > 0 && (p1 < (int)G::ULO1 || p1 > (int)G::UHI1));
+    if (!ghost) continue;
+    const int b = (lx + G::HX) + G::EX * ((ly + G::HY) + G::EY * (lz + G::HZ));
+    T v[G::NF];
+#pragma unroll
+    for (int f = 0; f < G::NF; ++f) v[f] = G::stage_lds(f, p0, p1, p2, tn, P, L, o0 - G::HX, o1 - G::HY, o2 - G::HZ, jp);
+#pragma unroll
+    for (int f = 0; f < G::NF; ++f) L[f * G::VOL + b] = v[f];
+  }
+  return true;
+}
+
 template <class G>
 __device__ __forceinline__ void mol_rk4_fused_tile(T* U, T* Ya, T* Yb, T* ACC, double* fc, const int fcs, int* fo,
                                                    const int tx, const int ty, const int tz, const T* __restrict__ u,
@@ -603,10 +637,19 @@ __device__ __forceinline__ void mol_rk4_fused_tile(T* U, T* Ya, T* Yb, T* ACC, d
   __syncthreads();
   mol_rk4_stage<G, 1>(U, Ya, U, ACC, o0, o1, o2, u, unew, ep, jp, t, dt);
   __syncthreads();
+  if constexpr (G::FIX) {
+    if (mol_rk4_fix_ghosts<G, 1>(Ya, o0, o1, o2, t + dt / 2, ep.p, jp)) __syncthreads();
+  }
   mol_rk4_stage<G, 2>(Ya, Yb, U, ACC, o0, o1, o2, u, unew, ep, jp, t, dt);
   __syncthreads();
+  if constexpr (G::FIX) {
+    if (mol_rk4_fix_ghosts<G, 2>(Yb, o0, o1, o2, t + dt / 2, ep.p, jp)) __syncthreads();
+  }
   mol_rk4_stage<G, 3>(Yb, Ya, U, ACC, o0, o1, o2, u, unew, ep, jp, t, dt);
   __syncthreads();
+  if constexpr (G::FIX) {
+    if (mol_rk4_fix_ghosts<G, 3>(Ya, o0, o1, o2, t + dt, ep.p, jp)) __syncthreads();
+  }
   mol_rk4_stage<G, 4>(Ya, nullptr, U, ACC, o0, o1, o2, u, unew, ep, jp, t, dt);
 }
 

@@ -250,3 +250,58 @@ def test_generated_vjp_gather_on_ragged_tilings(name, tile):
     d2 = (val(eps / 2) - val(-eps / 2)) / eps
     fd = (4 * d2 - d1) / 3
     assert abs(out @ v - fd) <= 1e-7 * abs(fd) + 1e-9 * np.linalg.norm(out) * np.linalg.norm(v), (name, tile)
+
+
+def _rk4_numpy(F, u, t, dt):
+    k1 = F(u, t)
+    k2 = F(u + dt / 2 * k1, t + dt / 2)
+    k3 = F(u + dt / 2 * k2, t + dt / 2)
+    k4 = F(u + dt * k3, t + dt)
+    return u + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+
+
+@pytest.mark.parametrize("bcl,bch", [((1, 0, 0.0), (1, 0, 0.0)), ((0, 1, 0.5), (1, 0, 0.2)),
+                                     ((1, 2, 0.1), (2, -1, 0.4))])
+@pytest.mark.parametrize("tile", [(16, 1, 1), (64, 1, 1), None])
+def test_fused_rk4_step_with_boundary_closures_1d(bcl, bch, tile):
+    """Non-periodic fused step: ghost positions refilled from their closures after every stage
+    (Dirichlet, Neumann, Robin ends) equal the oracle's RK4 step."""
+    n = 101
+    low = lower(*systems.heat1d(n=n, D=1.0, bcl=bcl, bch=bch))
+    u = np.random.default_rng(1).uniform(-1, 1, low.n_unknowns)
+    got, _ = genhost.rk4_fused(low, u, 0.0, 1e-5, tile=tile)
+    ref = orc.diffusion1d_integrate(n, 1.0, bcl, bch, u, 0.0, 1e-5, 1)
+    assert np.max(np.abs(got - ref)) <= 1e-14 * (1 + np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("bct", [(0, 0, 0, 0), (1, 1, 1, 1), (0, 1, 1, 0), (1, 0, 0, 1)])
+@pytest.mark.parametrize("tile", [(8, 8, 1), (32, 16, 1)])
+def test_fused_rk4_step_with_boundary_closures_2d(bct, tile):
+    n = 41
+    low = lower(*systems.diffreact2d(n, 0.05, 2.5, bct))
+    u = np.random.default_rng(sum(bct)).uniform(0, 1, low.n_unknowns)
+    dt = 2e-4
+    got, _ = genhost.rk4_fused(low, u, 0.0, dt, tile=tile)
+    ref = _rk4_numpy(lambda w, t: orc.diffreact2d_rhs(n, 0.05, 2.5, bct, w, t), u, 0.0, dt)
+    assert np.max(np.abs(got - ref)) <= 1e-13 * (1 + np.max(np.abs(ref)))
+
+
+def test_fused_rk4_step_with_time_dependent_boundary_values():
+    """u(t, 0) = sin(5t), u_x(t, 1) = cos(3t): each stage's ghosts use the stage's time; checked
+    against RK4 over the generated right-hand side evaluated at the stage times."""
+    from molgpu.pdesys import Differential, Eq, Interval, PDESystem, parameters, variables
+    from molgpu.discretization import MOLFiniteDifference
+    import sympy as sp
+    t, x = parameters("t x")
+    u = variables("u")
+    Dx = Differential(x)
+    eq = Eq(Differential(t)(u(t, x)), 0.5 * (Dx ** 2)(u(t, x)) + sp.sin(x) * u(t, x))
+    bcs = [Eq(u(0, x), x * (1 - x)), Eq(u(t, 0), sp.sin(5 * t)), Eq(Dx(u(t, 1)), sp.cos(3 * t))]
+    low = lower(PDESystem([eq], bcs, [Interval(t, 0.0, 1.0), Interval(x, 0.0, 1.0)], [t, x], [u(t, x)]),
+                MOLFiniteDifference({x: 61}, t))
+    u0 = np.random.default_rng(4).uniform(-1, 1, low.n_unknowns)
+    t0, dt = 0.3, 1e-3
+    got, tn = genhost.rk4_fused(low, u0, t0, dt, tile=(16, 1, 1))
+    ref = _rk4_numpy(lambda w, tt: genhost.rhs(low, w, t=tt), u0, t0, dt)
+    assert tn == pytest.approx(t0 + dt, abs=1e-15)
+    assert np.max(np.abs(got - ref)) <= 1e-13 * (1 + np.max(np.abs(ref)))

@@ -149,6 +149,43 @@ def test_brusselator_fused_rk4_step_equals_stage_launches(graph):
     assert_parity(res[True], ref, rtol=1e-12, what="fused rk4 vs oracle")
 
 
+@pytest.mark.parametrize("case", ["heat_dirichlet", "heat_neumann_robin", "diffreact_mixed", "diffreact_neumann"])
+def test_fused_rk4_with_boundary_closures_equals_stage_launches(case):
+    """Non-periodic systems take the fused step too (ghosts refilled from their closures after
+    each stage): equal to the four stage launches, and to the oracle's RK4."""
+    torch = _torch()
+    if case.startswith("heat"):
+        n = 1001
+        bcl, bch = ((1, 0, 0.0), (1, 0, 0.0)) if case == "heat_dirichlet" else ((0, 1, 0.5), (2, -1, 0.4))
+        prob = molgpu.discretize(*systems.heat1d(n=n, D=1.0, bcl=bcl, bch=bch))
+        dt, N = 5e-8, 41
+        ref = lambda u0: orc.diffusion1d_integrate(n, 1.0, bcl, bch, u0, 0.0, dt, N)
+    else:
+        n = 257
+        bct = (0, 1, 1, 0) if case == "diffreact_mixed" else (1, 1, 1, 1)
+        prob = molgpu.discretize(*systems.diffreact2d(n, 0.05, 2.5, bct))
+        dt, N = 1e-4, 41
+
+        def ref(u0):
+            u = u0.copy()
+            F = lambda w: orc.diffreact2d_rhs(n, 0.05, 2.5, bct, w)
+            for _ in range(N):
+                k1 = F(u); k2 = F(u + dt / 2 * k1); k3 = F(u + dt / 2 * k2); k4 = F(u + dt * k3)
+                u = u + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+            return u
+    assert prob.plan()["fused_tiles"] > 0, prob.plan()
+    u0 = np.random.default_rng(5).uniform(0, 1, prob.lowered.n_unknowns)
+    res = {}
+    for fused in (True, False):
+        u = torch.tensor(u0, dtype=torch.float64, device="cuda")
+        print("  %s fused=%s" % (case, fused), flush=True)
+        prob.integrate(u, 0.0, dt, N, molgpu.RK4, graph=True, fused=fused)
+        torch.cuda.synchronize()
+        res[fused] = u.cpu().numpy()
+    assert_parity(res[True], res[False], rtol=1e-13, what="fused vs stage launches")
+    assert_parity(res[True], ref(u0), rtol=1e-12, what="fused vs oracle")
+
+
 # ---------------------------------------------------------------- Burgers WENO5 (config 3)
 @pytest.mark.parametrize("n", [33, 129])
 def test_burgers_weno_rhs(n, rhs_path):
