@@ -818,6 +818,10 @@ S mixed_at(const Field<S>& F, const Op& Dx, const Op& Dy, const int* I, int ax, 
 
 // ------------------------------------------------------------------ systems
 // A system maps the unknown vector to du (compact ordering, variables concatenated).
+// Length of the next state array passed in (set by orc_expect, checked and cleared by the
+// first System::rhs of the call; -1: not stated).
+static thread_local long long g_expect = -1;
+
 template <typename S>
 struct System {
   Space sp;
@@ -831,6 +835,12 @@ struct System {
     return s;
   }
   void rhs(const S* u, double t, S* du, int threads) {
+    if (g_expect >= 0) {
+      // the caller stated the length of u (orc_expect): it must be the number of unknowns
+      const long long e = g_expect;
+      g_expect = -1;
+      if (e != (long long)nunk()) throw std::runtime_error("state length differs from the number of unknowns");
+    }
     size_t off = 0;
     for (auto& f : F) { scatter(f, u + off); off += f.box(); }
     for (auto& f : F) fill_boundaries(f, t);
@@ -912,6 +922,11 @@ int orc_half_weights(int d, int p, double dx, double* st, double* low, double* h
     std::memcpy(low + r * o.BL, o.low[r].data(), sizeof(double) * o.BL);
     std::memcpy(high + r * o.BL, o.high[r].data(), sizeof(double) * o.BL);
   }
+  return 0;
+}
+
+int orc_expect(long long n) {
+  g_expect = n;
   return 0;
 }
 

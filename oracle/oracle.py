@@ -56,6 +56,7 @@ def lib():
             "orc_spherical1d_rhs": [C.c_int, C.c_int, C.c_double, _dp, C.c_double, _dp],
             "orc_nonlinlap1d_rhs": [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp, C.c_double, _dp],
             "orc_num_threads": [],
+            "orc_expect": [C.c_longlong],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -67,6 +68,14 @@ def lib():
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _state(u):
+    """A state vector for an oracle right-hand side: its length is checked against the system's
+    number of unknowns by the call that follows."""
+    u = _f64(u)
+    lib().orc_expect(len(u))
+    return u
 
 
 def _ok(rc):
@@ -121,7 +130,7 @@ def weno_nu(u5, x5, eps, target):
 def weno_nu1d_rhs(x, c, eps, periodic, gl, gh, u, t=0.0):
     """u_t = -c u_x with nonuniform WENO5 on nodes x (Dirichlet gl/gh or periodic)."""
     x = _f64(x)
-    u = _f64(u)
+    u = _state(u)
     du = np.empty_like(u)
     _ok(lib().orc_weno_nu1d_rhs(len(x), x, float(c), float(eps), 1 if periodic else 0, float(gl), float(gh),
                                 u, float(t), du))
@@ -132,7 +141,7 @@ def nonuniform1d_rhs(x, D, c, p, uo, bcl, bch, u, t=0.0):
     """u_t = D u_xx - c u_x on the nodes x: centred (uo = 0) or upwind (order uo) advection,
     a0 u + a1 u_x = g at each end (bcl, bch = (a0, a1, g))."""
     x = _f64(x)
-    u = _f64(u)
+    u = _state(u)
     du = np.empty_like(u)
     _ok(lib().orc_nonuniform1d_rhs(len(x), x, float(D), float(c), int(p), int(uo), _f64(bcl), _f64(bch), u,
                                    float(t), du))
@@ -141,7 +150,7 @@ def nonuniform1d_rhs(x, D, c, p, uo, bcl, bch, u, t=0.0):
 
 def nonlinlap1d_nu_rhs(x, p, gl, gh, u, t=0.0):
     """u_t = d/dx((1 + u^2) u_x) on the nodes x, Dirichlet gl / gh."""
-    x, u = _f64(x), _f64(u)
+    x, u = _f64(x), _state(u)
     du = np.empty_like(u)
     _ok(lib().orc_nonlinlap1d_nu_rhs(len(x), x, int(p), float(gl), float(gh), u, float(t), du))
     return du
@@ -149,7 +158,7 @@ def nonlinlap1d_nu_rhs(x, p, gl, gh, u, t=0.0):
 
 def spherical1d_nu_rhs(x, p, g1, u, t=0.0):
     """Spherical Laplacian on the nodes x (r from 0), u_r(0) = 0, u(1) = g1."""
-    x, u = _f64(x), _f64(u)
+    x, u = _f64(x), _state(u)
     du = np.empty_like(u)
     _ok(lib().orc_spherical1d_nu_rhs(len(x), x, int(p), float(g1), u, float(t), du))
     return du
@@ -162,7 +171,7 @@ def diffreact_nd_rhs(grids, p, D, k, bctype, u, t=0.0):
     keep = [None if np.isscalar(g) else _f64(g) for g in grids]
     n = np.array([int(g) if np.isscalar(g) else len(g) for g in grids] + [1] * (3 - nd), dtype=np.int32)
     xs = (C.c_void_p * 3)(*[(k_.ctypes.data if k_ is not None else None) for k_ in keep] + [None] * (3 - nd))
-    u = _f64(u)
+    u = _state(u)
     du = np.empty_like(u)
     _ok(lib().orc_diffreact_nd_rhs(nd, n, xs, int(p), float(D), float(k),
                                    np.ascontiguousarray(bctype, dtype=np.int32), u, float(t), du))
@@ -170,60 +179,60 @@ def diffreact_nd_rhs(grids, p, D, k, bctype, u, t=0.0):
 
 
 def diffusion1d_rhs(n, D, bcl, bch, u, t=0.0, p=2, xlo=0.0, xhi=1.0):
-    u = _f64(u)
+    u = _state(u)
     du = np.empty_like(u)
     _ok(lib().orc_diffusion1d_rhs(n, xlo, xhi, p, D, _f64(bcl), _f64(bch), u, float(t), du))
     return du
 
 
 def diffusion1d_integrate(n, D, bcl, bch, u, t0, dt, nsteps, scheme=0, p=2):
-    u = _f64(u).copy()
+    u = _state(u).copy()
     _ok(lib().orc_diffusion1d_integrate(n, p, D, _f64(bcl), _f64(bch), u, t0, dt, nsteps, scheme))
     return u
 
 
 def bruss2d_rhs(n, u, t, alpha=10.0, p=2, threads=1):
-    u = _f64(u)
+    u = _state(u)
     du = np.empty_like(u)
     _ok(lib().orc_bruss2d_rhs(n, p, alpha, u, float(t), du, threads))
     return du
 
 
 def bruss2d_integrate(n, u, t0, dt, nsteps, alpha=10.0, p=2, threads=1):
-    u = _f64(u).copy()
+    u = _state(u).copy()
     _ok(lib().orc_bruss2d_integrate(n, p, alpha, u, t0, dt, nsteps, threads))
     return u
 
 
 def burgers2d_rhs(n, u, eps=1e-6, t=0.0, threads=1):
-    u = _f64(u)
+    u = _state(u)
     du = np.empty_like(u)
     _ok(lib().orc_burgers2d_rhs(n, eps, u, float(t), du, threads))
     return du
 
 
 def burgers2d_integrate(n, u, t0, dt, nsteps, eps=1e-6, threads=1):
-    u = _f64(u).copy()
+    u = _state(u).copy()
     _ok(lib().orc_burgers2d_integrate(n, eps, u, t0, dt, nsteps, threads))
     return u
 
 
 def weno1d_dirichlet_rhs(n, L, c, eps, gl, gh, u, t=0.0):
-    u = _f64(u)
+    u = _state(u)
     du = np.empty_like(u)
     _ok(lib().orc_weno1d_dirichlet_rhs(n, L, c, eps, gl, gh, u, float(t), du))
     return du
 
 
 def diffreact2d_rhs(n, D, k, bctype, u, t=0.0):
-    u = _f64(u)
+    u = _state(u)
     du = np.empty_like(u)
     _ok(lib().orc_diffreact2d_rhs(n, D, k, np.ascontiguousarray(bctype, dtype=np.int32), u, float(t), du))
     return du
 
 
 def nonlindiff3d_rhs(n, d0, src, u, t=0.0, p=2, threads=1):
-    u = _f64(u)
+    u = _state(u)
     du = np.empty_like(u)
     _ok(lib().orc_nonlindiff3d_rhs(n, p, _f64(d0), _f64(src), u, float(t), du, threads))
     return du
@@ -237,28 +246,28 @@ def nonlindiff3d_jvp(n, d0, vdir, src, u0, target, dt, nsteps, p=2):
 
 
 def upwind1d_rhs(n, uo, c0, c1, gl, gh, u, t=0.0):
-    u = _f64(u)
+    u = _state(u)
     du = np.empty_like(u)
     _ok(lib().orc_upwind1d_rhs(n, uo, c0, c1, gl, gh, u, float(t), du))
     return du
 
 
 def mixed2d_rhs(n, c, u, p=2, t=0.0):
-    u = _f64(u)
+    u = _state(u)
     du = np.empty_like(u)
     _ok(lib().orc_mixed2d_rhs(n, p, c, u, float(t), du))
     return du
 
 
 def spherical1d_rhs(n, g1, u, p=2, t=0.0):
-    u = _f64(u)
+    u = _state(u)
     du = np.empty_like(u)
     _ok(lib().orc_spherical1d_rhs(n, p, g1, u, float(t), du))
     return du
 
 
 def nonlinlap1d_rhs(n, u, periodic, gl=0.0, gh=0.0, p=2, t=0.0):
-    u = _f64(u)
+    u = _state(u)
     du = np.empty_like(u)
     _ok(lib().orc_nonlinlap1d_rhs(n, p, int(periodic), gl, gh, u, float(t), du))
     return du

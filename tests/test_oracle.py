@@ -303,3 +303,18 @@ def test_nonlindiff3d_reduces_to_laplacian_for_constant_coefficient():
     for ax in range(3):
         lap += (np.roll(full, -1, ax) - 2 * full + np.roll(full, 1, ax))[1:-1, 1:-1, 1:-1] / h ** 2
     np.testing.assert_allclose(du.reshape((n - 2,) * 3, order="F"), lap, rtol=1e-6, atol=1e-9)
+
+
+def test_oracle_rejects_a_state_of_the_wrong_length():
+    """The oracle's right-hand sides read as many values as the system has unknowns; a state of
+    another length is refused instead of read past its end."""
+    import numpy as np
+    import oracle as orc
+    import pytest
+    n = 21
+    orc.spherical1d_rhs(n, 0.4, np.zeros(n - 2))
+    for bad in (n - 1, n - 3):
+        with pytest.raises(RuntimeError):
+            orc.spherical1d_rhs(n, 0.4, np.zeros(bad))
+    with pytest.raises(RuntimeError):
+        orc.diffusion1d_rhs(n, 1.0, (1, 0, 0), (1, 0, 0), np.zeros(n))
