@@ -1632,13 +1632,12 @@ std::string Gen::source_fused(bool f64) {
     << ") mol_rk4_fused(const T* __restrict__ u, T* __restrict__ unew, Epi ep, JPtrs jp) {\n"
        "  __shared__ T U[GeoF::NST * GeoF::VOL];\n  __shared__ T Ya[GeoF::NST * GeoF::VOL];\n"
        "  __shared__ T Yb[GeoF::NST * GeoF::VOL];\n"
-       "  __shared__ T ACC[GeoF::NF * GeoF::TX * GeoF::TY * GeoF::TZ];\n"
        "  int b = mol_xcd_tile<GeoBase::NBLK>((int)blockIdx.x);\n"
        "  const int tx = b % GeoBase::NT0;\n  b /= GeoBase::NT0;\n"
        "  const int ty = b % GeoBase::NT1;\n  const int tz = b / GeoBase::NT1;\n"
        "  const double t = ep.tnow ? ep.tnow[0] : 0.0;\n"
        "  mol_advance_clock(ep);\n"
-       "  mol_rk4_fused_tile<GeoF>(U, Ya, Yb, ACC, &mol_fc[0][0], (int)(sizeof(mol_fc[0]) / sizeof(double)), mol_fo,\n"
+       "  mol_rk4_fused_tile<GeoF>(U, Ya, Yb, &mol_fc[0][0], (int)(sizeof(mol_fc[0]) / sizeof(double)), mol_fo,\n"
        "                           tx, ty, tz, u, unew, ep, jp, t);\n}\n";
   return o.str();
 }

@@ -530,48 +530,89 @@ __device__ __forceinline__ void mol_tiles_persistent(T* L, const T* __restrict__
 }
 
 // ---------------------------------------------------------------------------------------
-// Fused classic RK4 step (periodic systems): stage S of tile (o0, o1, o2) on the tile grown by
-// (4 - S) reach per axis.  Lin holds the stage input (u, y1, y2, y3), Lout receives the stage
-// output y_S = u + c_S dt k_S, ACC the running k1 + 2 k2 + 2 k3 on the tile; stage 4 writes
-// u + dt/6 k4 + dt/6 ACC.  The arithmetic is that of the stage kernels' epilogue (mode 1 with
-// ca = 1, acc_f = 1 or 2, acc_keep = 0 or 1), so a fused step equals four stage launches.
+// Fused classic RK4 step: stage S of tile (o0, o1, o2) on the tile grown by (4 - S) reach per
+// axis.  The items of a stage are the tile's nodes (0 .. TVOL-1) and then the frame of the
+// grown region; item q belongs to thread q mod NTHR, so every tile node keeps its thread across
+// the four stages and its step-initial value Ur and running sum Ar = k1 + 2 k2 + 2 k3 stay in
+// that thread's registers.  Lin holds the stage input (u, y1, y2, y3), Lout receives
+// y_S = u + c_S dt k_S; stage 4 writes u + dt/6 k4 + dt/6 Ar.  The arithmetic is that of the
+// stage kernels' epilogue (mode 1 with ca = 1, acc_f = 1 or 2, acc_keep = 0 or 1), so a fused
+// step equals four stage launches.
+template <class G>
+struct MolFusedItems {
+  static constexpr int TVOL = G::TX * G::TY * G::TZ;
+  static constexpr int NJ = (TVOL + G::NTHR - 1) / G::NTHR;   // tile nodes per thread
+};
+
+// Position (tile-relative) of frame item h of the region grown by M (2D / 1D: RZ = 0).
+template <class G, int M>
+__device__ __forceinline__ void mol_frame_pos(const int h, int& lx, int& ly) {
+  constexpr int MR0 = M * G::RX, MR1 = M * G::RY, W0 = G::TX + 2 * MR0;
+  if (h < MR1 * W0) {
+    ly = -MR1 + h / W0;
+    lx = -MR0 + h % W0;
+  } else if (h < 2 * MR1 * W0) {
+    const int h2 = h - MR1 * W0;
+    ly = G::TY + h2 / W0;
+    lx = -MR0 + h2 % W0;
+  } else {
+    const int h3 = h - 2 * MR1 * W0;
+    const int c = h3 % (2 * MR0);
+    ly = h3 / (2 * MR0);
+    lx = c < MR0 ? -MR0 + c : G::TX + (c - MR0);
+  }
+}
+
 template <class G, int S>
-__device__ __forceinline__ void mol_rk4_stage(const T* Lin, T* Lout, const T* U, T* ACC, const int o0, const int o1,
+__device__ __forceinline__ void mol_rk4_stage(const T* Lin, T* Lout, const T* U, T (&Ur)[MolFusedItems<G>::NJ][G::NF],
+                                              T (&Ar)[MolFusedItems<G>::NJ][G::NF], const int o0, const int o1,
                                               const int o2, const T* __restrict__ u, T* __restrict__ unew,
                                               const Epi& ep, const JPtrs& jp, const double t, const double dt) {
   constexpr int M = 4 - S;
-  constexpr int W0 = G::TX + 2 * M * G::RX, W1 = G::TY + 2 * M * G::RY, W2 = G::TZ + 2 * M * G::RZ;
-  constexpr int NV = W0 * W1 * W2, TVOL = G::TX * G::TY * G::TZ;
+  constexpr int TVOL = MolFusedItems<G>::TVOL, NJ = MolFusedItems<G>::NJ;
+  constexpr int FR = (G::TX + 2 * M * G::RX) * (G::TY + 2 * M * G::RY) - TVOL;
+  static_assert(G::TZ == 1 && G::RZ == 0, "fused RK4 steps are 1D / 2D");
   const double ts = t + (S == 1 ? 0.0 : (S == 4 ? dt : dt / 2));
   const double cf = S == 1 || S == 2 ? dt / 2 : (S == 3 ? dt : dt / 6);
-  for (int q = threadIdx.x; q < NV; q += G::NTHR) {
-    const int z = q / (W0 * W1);
-    const int r = q - z * (W0 * W1);
-    const int y = r / W0;
-    const int x = r - y * W0;
-    const int lx = x - M * G::RX, ly = y - M * G::RY, lz = z - M * G::RZ;   // tile-relative
-    const int b = (lx + G::HX) + G::EX * ((ly + G::HY) + G::EY * (lz + G::HZ));
-    const int I0 = o0 + lx, I1 = o1 + ly, I2 = o2 + lz;
-    // unwrapped indices: coordinates come from the staged copy, grid fields wrap in gidx
+  // tile nodes: registers carry u and the running sum
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int q = (int)threadIdx.x + j * G::NTHR;
+    if (TVOL % G::NTHR != 0 && q >= TVOL) break;
+    const int ly = q / G::TX, lx = q - ly * G::TX;
+    const int b = (lx + G::HX) + G::EX * (ly + G::HY);
+    const int I0 = o0 + lx, I1 = o1 + ly, I2 = o2;
     T out[G::NF];
     G::node(Lin, b, I0, I1, I2, ts, ep.p, u, jp, out);
-    const int tl = lx + G::TX * (ly + G::TY * lz);
-    const bool in_tile = M == 0 || (lx >= 0 && lx < G::TX && ly >= 0 && ly < G::TY && lz >= 0 && lz < G::TZ);
 #pragma unroll
     for (int f = 0; f < G::NF; ++f) {
-      const T uv = U[f * G::VOL + b];
       T v = (T)cf * out[f];
-      v += uv;
+      v += Ur[j][f];
       if (S < 4) {
         Lout[f * G::VOL + b] = v;
-        if (in_tile) {
-          T a = (S == 1 ? T(1) : T(2)) * out[f];
-          if (S > 1) a += ACC[f * TVOL + tl];
-          ACC[f * TVOL + tl] = a;
-        }
+        T a = (S == 1 ? T(1) : T(2)) * out[f];
+        if (S > 1) a += Ar[j][f];
+        Ar[j][f] = a;
       } else {
-        v += (T)cf * ACC[f * TVOL + tl];
+        v += (T)cf * Ar[j][f];
         if (I0 <= G::UHI0 && I1 <= G::UHI1 && I2 <= G::UHI2) unew[G::uidx(f, I0, I1, I2)] = v;
+      }
+    }
+  }
+  // frame of the grown region (stages 1-3): u from LDS, no running sum
+  if (M > 0) {
+    for (int q = (int)threadIdx.x; q < TVOL + FR; q += G::NTHR) {
+      if (q < TVOL) continue;
+      int lx, ly;
+      mol_frame_pos<G, M>(q - TVOL, lx, ly);
+      const int b = (lx + G::HX) + G::EX * (ly + G::HY);
+      T out[G::NF];
+      G::node(Lin, b, o0 + lx, o1 + ly, o2, ts, ep.p, u, jp, out);
+#pragma unroll
+      for (int f = 0; f < G::NF; ++f) {
+        T v = (T)cf * out[f];
+        v += U[f * G::VOL + b];
+        Lout[f * G::VOL + b] = v;
       }
     }
   }
@@ -612,7 +653,7 @@ __device__ __forceinline__ bool mol_rk4_fix_ghosts(T* L, const int o0, const int
 }
 
 template <class G>
-__device__ __forceinline__ void mol_rk4_fused_tile(T* U, T* Ya, T* Yb, T* ACC, double* fc, const int fcs, int* fo,
+__device__ __forceinline__ void mol_rk4_fused_tile(T* U, T* Ya, T* Yb, double* fc, const int fcs, int* fo,
                                                    const int tx, const int ty, const int tz, const T* __restrict__ u,
                                                    T* __restrict__ unew, const Epi& ep, const JPtrs& jp,
                                                    const double t) {
@@ -635,22 +676,35 @@ __device__ __forceinline__ void mol_rk4_fused_tile(T* U, T* Ya, T* Yb, T* ACC, d
         }
   }
   __syncthreads();
-  mol_rk4_stage<G, 1>(U, Ya, U, ACC, o0, o1, o2, u, unew, ep, jp, t, dt);
+  constexpr int NJ = MolFusedItems<G>::NJ;
+  T Ur[NJ][G::NF], Ar[NJ][G::NF];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int q = (int)threadIdx.x + j * G::NTHR;
+    const int ly = q / G::TX, lx = q - ly * G::TX;
+    const int b = (lx + G::HX) + G::EX * (ly + G::HY);
+#pragma unroll
+    for (int f = 0; f < G::NF; ++f) {
+      Ur[j][f] = q < MolFusedItems<G>::TVOL ? U[f * G::VOL + b] : T(0);
+      Ar[j][f] = T(0);
+    }
+  }
+  mol_rk4_stage<G, 1>(U, Ya, U, Ur, Ar, o0, o1, o2, u, unew, ep, jp, t, dt);
   __syncthreads();
   if constexpr (G::FIX) {
     if (mol_rk4_fix_ghosts<G, 1>(Ya, o0, o1, o2, t + dt / 2, ep.p, jp)) __syncthreads();
   }
-  mol_rk4_stage<G, 2>(Ya, Yb, U, ACC, o0, o1, o2, u, unew, ep, jp, t, dt);
+  mol_rk4_stage<G, 2>(Ya, Yb, U, Ur, Ar, o0, o1, o2, u, unew, ep, jp, t, dt);
   __syncthreads();
   if constexpr (G::FIX) {
     if (mol_rk4_fix_ghosts<G, 2>(Yb, o0, o1, o2, t + dt / 2, ep.p, jp)) __syncthreads();
   }
-  mol_rk4_stage<G, 3>(Yb, Ya, U, ACC, o0, o1, o2, u, unew, ep, jp, t, dt);
+  mol_rk4_stage<G, 3>(Yb, Ya, U, Ur, Ar, o0, o1, o2, u, unew, ep, jp, t, dt);
   __syncthreads();
   if constexpr (G::FIX) {
     if (mol_rk4_fix_ghosts<G, 3>(Ya, o0, o1, o2, t + dt, ep.p, jp)) __syncthreads();
   }
-  mol_rk4_stage<G, 4>(Ya, nullptr, U, ACC, o0, o1, o2, u, unew, ep, jp, t, dt);
+  mol_rk4_stage<G, 4>(Ya, nullptr, U, Ur, Ar, o0, o1, o2, u, unew, ep, jp, t, dt);
 }
 
 // Arguments of the generated vector-Jacobian product (layout identical to mol::VArgs<T>).
