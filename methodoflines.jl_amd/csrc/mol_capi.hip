@@ -632,10 +632,12 @@ bool fused_geometry(mol_handle* h, const mol_desc* d, const KArgs<double>& ka, c
   // a ghost's closure reads unknowns inward from the box edge: they must lie in the tile region
   int kmax = 0;
   for (int c = 0; c < d->nclosures; ++c) kmax = std::max(kmax, d->closures_h[c].K + d->closures_h[c].nghost);
-  static const int cand2[][3] = {{32, 16, 1}, {32, 8, 1}, {64, 8, 1}, {16, 16, 1}, {16, 8, 1}};
+  // the largest tile that leaves at least one tile per CU (fewer halo positions per node);
+  // three stage buffers of the tile + halo, two work-groups per CU within 160 KB
+  static const int cand2[][3] = {{32, 32, 1}, {32, 16, 1}, {32, 8, 1}, {64, 8, 1}, {16, 16, 1}, {16, 8, 1}};
   static const int cand1[][3] = {{256, 1, 1}, {128, 1, 1}, {64, 1, 1}};
   const int (*cand)[3] = d->ndim == 2 ? cand2 : cand1;
-  const int nc = d->ndim == 2 ? 5 : 3;
+  const int nc = d->ndim == 2 ? 6 : 3;
   int best = -1;
   int64_t best_n = -1;
   for (int i = 0; i < nc; ++i) {
@@ -647,8 +649,9 @@ bool fused_geometry(mol_handle* h, const mol_desc* d, const KArgs<double>& ka, c
       tv *= tl;
       nt *= (ext + tl - 1) / tl;
     }
-    const size_t lds = (size_t)(3 * vol + tv) * d->nfields * h->esize();
-    if (lds > 65536) continue;
+    (void)tv;
+    const size_t lds = (size_t)3 * vol * d->nfields * h->esize();
+    if (lds > 81920) continue;
     if (nt >= 256) { best = i; break; }
     if (nt > best_n) { best_n = nt; best = i; }
   }

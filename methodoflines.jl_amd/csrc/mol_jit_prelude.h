@@ -541,7 +541,15 @@ __device__ __forceinline__ void mol_tiles_persistent(T* L, const T* __restrict__
 template <class G>
 struct MolFusedItems {
   static constexpr int TVOL = G::TX * G::TY * G::TZ;
-  static constexpr int NJ = (TVOL + G::NTHR - 1) / G::NTHR;   // tile nodes per thread
+  static constexpr int NJ = (TVOL + G::NTHR - 1) / G::NTHR;   // tile nodes per thread, NTHR apart
+  static __device__ __forceinline__ void pos(const int j, int& lx, int& ly) {
+    const int q = (int)threadIdx.x + j * G::NTHR;
+    ly = q / G::TX;
+    lx = q - ly * G::TX;
+  }
+  static __device__ __forceinline__ bool valid(const int j) {
+    return TVOL % G::NTHR == 0 || (int)threadIdx.x + j * G::NTHR < TVOL;
+  }
 };
 
 // Position (tile-relative) of frame item h of the region grown by M (2D / 1D: RZ = 0).
@@ -575,22 +583,28 @@ __device__ __forceinline__ void mol_rk4_stage(const T* Lin, T* Lout, const T* U,
   const double ts = t + (S == 1 ? 0.0 : (S == 4 ? dt : dt / 2));
   const double cf = S == 1 || S == 2 ? dt / 2 : (S == 3 ? dt : dt / 6);
   // tile nodes: registers carry u and the running sum
+  T out[NJ][G::NF];
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
-    const int q = (int)threadIdx.x + j * G::NTHR;
-    if (TVOL % G::NTHR != 0 && q >= TVOL) break;
-    const int ly = q / G::TX, lx = q - ly * G::TX;
+    if (!MolFusedItems<G>::valid(j)) break;
+    int lx, ly;
+    MolFusedItems<G>::pos(j, lx, ly);
+    G::node(Lin, (lx + G::HX) + G::EX * (ly + G::HY), o0 + lx, o1 + ly, o2, ts, ep.p, u, jp, out[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    if (!MolFusedItems<G>::valid(j)) break;
+    int lx, ly;
+    MolFusedItems<G>::pos(j, lx, ly);
     const int b = (lx + G::HX) + G::EX * (ly + G::HY);
     const int I0 = o0 + lx, I1 = o1 + ly, I2 = o2;
-    T out[G::NF];
-    G::node(Lin, b, I0, I1, I2, ts, ep.p, u, jp, out);
 #pragma unroll
     for (int f = 0; f < G::NF; ++f) {
-      T v = (T)cf * out[f];
+      T v = (T)cf * out[j][f];
       v += Ur[j][f];
       if (S < 4) {
         Lout[f * G::VOL + b] = v;
-        T a = (S == 1 ? T(1) : T(2)) * out[f];
+        T a = (S == 1 ? T(1) : T(2)) * out[j][f];
         if (S > 1) a += Ar[j][f];
         Ar[j][f] = a;
       } else {
@@ -680,12 +694,13 @@ __device__ __forceinline__ void mol_rk4_fused_tile(T* U, T* Ya, T* Yb, double* f
   T Ur[NJ][G::NF], Ar[NJ][G::NF];
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
-    const int q = (int)threadIdx.x + j * G::NTHR;
-    const int ly = q / G::TX, lx = q - ly * G::TX;
-    const int b = (lx + G::HX) + G::EX * (ly + G::HY);
+    int lx, ly;
+    MolFusedItems<G>::pos(j, lx, ly);
+    const bool ok = MolFusedItems<G>::valid(j);
+    const int b = ok ? (lx + G::HX) + G::EX * (ly + G::HY) : 0;
 #pragma unroll
     for (int f = 0; f < G::NF; ++f) {
-      Ur[j][f] = q < MolFusedItems<G>::TVOL ? U[f * G::VOL + b] : T(0);
+      Ur[j][f] = ok ? U[f * G::VOL + b] : T(0);
       Ar[j][f] = T(0);
     }
   }
