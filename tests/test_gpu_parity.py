@@ -149,6 +149,27 @@ def test_brusselator_fused_rk4_step_equals_stage_launches(graph):
     assert_parity(res[True], ref, rtol=1e-12, what="fused rk4 vs oracle")
 
 
+def test_fp32_fused_rk4_step():
+    """FP32 systems (without the compensated update) take the fused step too: equal to the
+    stage launches to FP32 rounding, and within FP32 accuracy of the FP64 oracle."""
+    torch = _torch()
+    n = 257
+    prob = molgpu.discretize(*systems.brusselator2d(n=n, dtype="f32"))
+    assert prob.plan()["fused_tiles"] > 0
+    dt, N = 2e-7, 40     # stable: dt * 8 * alpha / dx^2 = 1.05
+    res = {}
+    for fused in (True, False):
+        u = torch.tensor(prob.u0, dtype=torch.float32, device="cuda")
+        print("  bruss fp32 fused=%s" % fused, flush=True)
+        prob.integrate(u, 0.0, dt, N, molgpu.RK4, graph=True, fused=fused)
+        torch.cuda.synchronize()
+        res[fused] = u.cpu().numpy().astype(np.float64)
+    scale = np.max(np.abs(res[False]))
+    assert np.max(np.abs(res[True] - res[False])) <= 1e-6 * scale
+    ref = orc.bruss2d_integrate(n, np.asarray(prob.u0, dtype=np.float64), 0.0, dt, N, threads=8)
+    assert np.max(np.abs(res[True] - ref)) <= 2e-5 * scale
+
+
 @pytest.mark.parametrize("case", ["heat_dirichlet", "heat_neumann_robin", "diffreact_mixed", "diffreact_neumann"])
 def test_fused_rk4_with_boundary_closures_equals_stage_launches(case):
     """Non-periodic systems take the fused step too (ghosts refilled from their closures after
