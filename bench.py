@@ -43,9 +43,10 @@ HBM_PEAK_GBS = 8000.0
 # profiles/brusselator512_rk4_stage_kernel.md.  Below the 17.8 MB of algorithmic bytes: halo
 # re-reads and the stage vectors written by one launch and read by the next hit in L2.
 STAGE_TRAFFIC_BYTES = (3960.27 * 2.0 + 5869.53 * 1.0) * 1024
-# The same measurement for the fused RK4 step kernel (one launch per step): FETCH_SIZE 2372.6
-# KiB, WRITE_SIZE 4096.0 KiB per launch (8.39 MB algorithmic: the halo re-reads hit in L2).
-FUSED_TRAFFIC_BYTES = (2372.6 * 2.0 + 4096.0 * 1.0) * 1024
+# The same measurement for the fused RK4 step kernel (one launch per step, 32x32 tiles):
+# FETCH_SIZE 2375.6 KiB, WRITE_SIZE 4096.0 KiB per launch (8.39 MB algorithmic; the excess is
+# the tile halos).
+FUSED_TRAFFIC_BYTES = (2375.6 * 2.0 + 4096.0 * 1.0) * 1024
 
 
 def events():
