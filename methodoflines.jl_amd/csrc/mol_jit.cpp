@@ -250,6 +250,7 @@ class Gen {
     std::vector<bool> staged;
     int reach[3] = {0, 0, 0};
     int core_reach[3] = {0, 0, 0};
+    bool separable = true;   // every tap offset lies along one axis (no diagonal taps)
   };
   // Build the node IR (both variants) and report the array usage; empty on failure.
   Usage analyze() {
@@ -264,12 +265,15 @@ class Gen {
     u.staged.assign(d_->nfields + d_->naux, false);
     for (const Node& n : nodes_) {
       if (n.kind != N_LDS) continue;
+      int naxes = 0;
       for (int a = 0; a < 3; ++a) {
         const int r = n.dv[a] < 0 ? -n.dv[a] : n.dv[a];
         if (r) u.staged[n.arr] = true;
+        if (r) ++naxes;
         u.reach[a] = std::max(u.reach[a], r);
         if (n.core) u.core_reach[a] = std::max(u.core_reach[a], r);
       }
+      if (naxes > 1) u.separable = false;
     }
     return u;
   }
@@ -1326,6 +1330,11 @@ std::string Gen::geo_variant(const std::string& name, const int* halo, const Usa
   c("EX", pg_.ext[0]); c("EY", pg_.ext[1]); c("EZ", pg_.ext[2]);
   c("VOL", pg_.vol);
   c("NST", nst); c("NSTF", nstf); c("NSTA", nsta);
+  // no diagonal taps: halo positions outside the tile along two or more axes are never read
+  // and are not staged (3D, where they are 27-40 % of the box; not for the fused step, whose
+  // stage regions grow along every axis)
+  o << "  static constexpr bool CROSS = " << (us.separable && !fused_ && !param_ && d_->ndim >= 3 ? "true" : "false")
+    << ";\n";
   // tiles staged with plain loads: tile + halo inside every variable's box (after the wrap)
   for (int a = 0; a < 3; ++a) {
     int slo = 0, shi = g_.ntile[a] - 1;

@@ -366,6 +366,15 @@ __device__ __forceinline__ int mol_stage_mode(const int tx, const int ty, const 
   return 2;
 }
 
+// Position (x, y, z) of the tile + halo box lies outside the tile along two or more axes and
+// variant G reads no such position (G::CROSS): it is not staged.
+template <class G>
+__device__ __forceinline__ bool mol_unread(const int x, const int y, const int z) {
+  if (!G::CROSS) return false;
+  return (int)(x < G::HX || x >= G::HX + G::TX) + (int)(y < G::HY || y >= G::HY + G::TY) +
+             (int)(z < G::HZ || z >= G::HZ + G::TZ) >= 2;
+}
+
 // Tile + halo of the staged arrays into LDS.
 template <class G>
 __device__ __forceinline__ void mol_stage_tile(T* L, const int o0, const int o1, const int o2, const int mode,
@@ -373,7 +382,8 @@ __device__ __forceinline__ void mol_stage_tile(T* L, const int o0, const int o1,
                                                const JPtrs& jp) {
   MolWalk<G::EX, G::EY, G::NTHR> w((int)threadIdx.x);
   for (int q = threadIdx.x; q < G::VOL; q += G::NTHR, w.step())
-    G::stage_pos(L, q, o0 + w.x - G::HX, o1 + w.y - G::HY, o2 + w.z - G::HZ, mode, t, P, u, jp);
+    if (!mol_unread<G>(w.x, w.y, w.z))
+      G::stage_pos(L, q, o0 + w.x - G::HX, o1 + w.y - G::HY, o2 + w.z - G::HZ, mode, t, P, u, jp);
 }
 
 // Node phase of one staged tile in variant G: each thread runs G::RB consecutive nodes along
@@ -431,8 +441,14 @@ __device__ __forceinline__ void mol_fetch(T (&R)[NPF][NSR], const int tx, const 
 #pragma unroll
   for (int i = 0; i < NPF; ++i) {
     if (i * G::NTHR >= G::VOL) break;
-    if ((int)threadIdx.x + i * G::NTHR < G::VOL)
-      G::stage_vals(R[i], o0 + w.x - G::HX, o1 + w.y - G::HY, o2 + w.z - G::HZ, mode, t, P, u, jp);
+    if ((int)threadIdx.x + i * G::NTHR < G::VOL) {
+      if (mol_unread<G>(w.x, w.y, w.z)) {
+#pragma unroll
+        for (int s = 0; s < NSR; ++s) R[i][s] = T(0);
+      } else {
+        G::stage_vals(R[i], o0 + w.x - G::HX, o1 + w.y - G::HY, o2 + w.z - G::HZ, mode, t, P, u, jp);
+      }
+    }
     w.step();
   }
 }
@@ -891,7 +907,8 @@ __device__ __forceinline__ void mol_vjp_tile(T* L, T* Ab, double* Aa, const int 
     const int mode = mol_stage_mode<G>(tx, ty, tz);
     MolWalk<G::EX, G::EY, G::NTHR> w((int)threadIdx.x);
     for (int q = threadIdx.x; q < G::VOL; q += G::NTHR, w.step()) {
-      G::stage_pos(L, q, o0 + w.x - G::HX, o1 + w.y - G::HY, o2 + w.z - G::HZ, mode, t, va.p, va.u, jp);
+      if (!mol_unread<G>(w.x, w.y, w.z))
+        G::stage_pos(L, q, o0 + w.x - G::HX, o1 + w.y - G::HY, o2 + w.z - G::HZ, mode, t, va.p, va.u, jp);
       G::zero_pos(Ab, Aa, q);
     }
   }
