@@ -628,7 +628,7 @@ bool fused_geometry(mol_handle* h, const mol_desc* d, const KArgs<double>& ka, c
                     JitGeom& g, std::string& why) {
   if (d->ndim > 2) { why = "not a 1D/2D system"; return false; }
   for (int a = 0; a < d->ndim; ++a)
-    if (ka.halo[a] > 1) { why = "reach above 1"; return false; }
+    if (ka.halo[a] > 2 || (ka.halo[a] > 1 && !ka.periodic[a])) { why = "reach above 1 (2 on periodic axes)"; return false; }
   // a ghost's closure reads unknowns inward from the box edge: they must lie in the tile region
   int kmax = 0;
   for (int c = 0; c < d->nclosures; ++c) kmax = std::max(kmax, d->closures_h[c].K + d->closures_h[c].nghost);

@@ -305,3 +305,17 @@ def test_fused_rk4_step_with_time_dependent_boundary_values():
     ref = _rk4_numpy(lambda w, tt: genhost.rhs(low, w, t=tt), u0, t0, dt)
     assert tn == pytest.approx(t0 + dt, abs=1e-15)
     assert np.max(np.abs(got - ref)) <= 1e-13 * (1 + np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("name", ["brusselator_p4", "nonlinlap_periodic"])
+def test_fused_rk4_step_reach_two_periodic(name):
+    """Periodic systems of reach 2 (4th-order Laplacian, nonlinear Laplacian) take the fused step
+    with an 8-node halo; equal to RK4 over the generated right-hand side."""
+    if name == "brusselator_p4":
+        low = lower(*systems.brusselator2d(n=65, approx_order=4))
+    else:
+        low = lower(*systems.nonlinlap1d(201, True, 0, 0, 2))
+    u = np.asarray(low.u0, dtype=np.float64) + 0.01 * np.random.default_rng(1).standard_normal(low.n_unknowns)
+    got, _ = genhost.rk4_fused(low, u, 1.2, 1e-6)
+    ref = _rk4_numpy(lambda w, tt: genhost.rhs(low, w, t=tt), u, 1.2, 1e-6)
+    assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref))

@@ -149,6 +149,33 @@ def test_brusselator_fused_rk4_step_equals_stage_launches(graph):
     assert_parity(res[True], ref, rtol=1e-12, what="fused rk4 vs oracle")
 
 
+@pytest.mark.parametrize("case", ["brusselator_p4", "nonlinlap_periodic"])
+def test_fused_rk4_reach_two_equals_stage_launches(case):
+    torch = _torch()
+    if case == "brusselator_p4":
+        n = 257
+        prob = molgpu.discretize(*systems.brusselator2d(n=n, approx_order=4))
+        dt, N = 1e-7, 31
+        ref = lambda u0: orc.bruss2d_integrate(n, u0, 0.0, dt, N, p=4, threads=8)
+    else:
+        n = 2001
+        prob = molgpu.discretize(*systems.nonlinlap1d(n, True, 0.0, 0.0, 2))
+        dt, N = 2e-8, 31
+        ref = None
+    assert prob.plan()["fused_tiles"] > 0, prob.plan()
+    u0 = np.asarray(prob.u0, dtype=np.float64)
+    res = {}
+    for fused in (True, False):
+        u = torch.tensor(u0, dtype=torch.float64, device="cuda")
+        print("  %s fused=%s" % (case, fused), flush=True)
+        prob.integrate(u, 0.0, dt, N, molgpu.RK4, graph=True, fused=fused)
+        torch.cuda.synchronize()
+        res[fused] = u.cpu().numpy()
+    assert_parity(res[True], res[False], rtol=1e-13, what="fused vs stage launches")
+    if ref is not None:
+        assert_parity(res[True], ref(u0), rtol=1e-12, what="fused vs oracle")
+
+
 def test_fp32_fused_rk4_step():
     """FP32 systems (without the compensated update) take the fused step too: equal to the
     stage launches to FP32 rounding, and within FP32 accuracy of the FP64 oracle."""
