@@ -119,6 +119,22 @@ __device__ __forceinline__ U mol_weno5_t(U um2, U um1, U u0, U up1, U up2, T eps
   return (Np * Dm - Nm * Dp) / (Dp * Dm) * (inv_dx * T(1.0 / 6.0));
 }
 
+// Reciprocal and quotient for the FP32 WENO5 weights: the hardware reciprocal (v_rcp_f32,
+// within 1 ulp) refined by one Newton step, and the quotient corrected by its residual
+// (Markstein), which is the correctly rounded result but for rare ties, in 3 and 5
+// instructions against about 10 for the IEEE division sequence.  FP64: plain divisions.
+__device__ __forceinline__ float mol_rcp(const float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  return fmaf(r, fmaf(-x, r, 1.0f), r);
+}
+__device__ __forceinline__ double mol_rcp(const double x) { return 1.0 / x; }
+__device__ __forceinline__ float mol_div(const float n, const float d) {
+  const float r = mol_rcp(d);
+  const float q = n * r;
+  return fmaf(fmaf(-d, q, n), r, q);
+}
+__device__ __forceinline__ double mol_div(const double n, const double d) { return n / d; }
+
 // The same derivative in the reference's normalised-weight form (weno_f_uniform,
 // src/discretization/schemes/WENO/WENO.jl:6-57).  Used in FP32, where the products of four
 // squared smoothness indicators of the cleared form (~eps^8) underflow.
@@ -132,7 +148,7 @@ __device__ __forceinline__ U mol_weno5_ref_t(U um2, U um1, U u0, U up1, U up2, T
   const U b3 = T(13) * c5 * c5 / T(12) + c6 * c6 / T(4);
   const U q1 = (eps + b1) * (eps + b1), q2 = (eps + b2) * (eps + b2), q3 = (eps + b3) * (eps + b3);
   // the six weights share three reciprocals and each normalisation one division
-  const U r1 = T(1) / q1, r2 = T(1) / q2, r3 = T(1) / q3;
+  const U r1 = mol_rcp(q1), r2 = mol_rcp(q2), r3 = mol_rcp(q3);
   const U om1 = T(0.1) * r1, om2 = T(0.6) * r2, om3 = T(0.3) * r3;
   const U op1 = T(0.3) * r1, op2 = T(0.6) * r2, op3 = T(0.1) * r3;
   const U wmd = om1 + om2 + om3, wpd = op1 + op2 + op3;
@@ -143,8 +159,8 @@ __device__ __forceinline__ U mol_weno5_ref_t(U um2, U um1, U u0, U up1, U up2, T
   const U hp1 = T(2) * u0 + T(5) * up1 - up2;
   const U hp2 = T(5) * u0 + T(2) * up1 - um1;
   const U hp3 = T(11) * u0 - T(7) * um1 + T(2) * um2;
-  const U hp = (op1 * hp1 + op2 * hp2 + op3 * hp3) / wpd;
-  const U hm = (om1 * hm1 + om2 * hm2 + om3 * hm3) / wmd;
+  const U hp = mol_div(op1 * hp1 + op2 * hp2 + op3 * hp3, wpd);
+  const U hm = mol_div(om1 * hm1 + om2 * hm2 + om3 * hm3, wmd);
   return (hp - hm) * (inv_dx * T(1.0 / 6.0));
 }
 
@@ -175,6 +191,11 @@ __device__ __forceinline__ MolDual operator/(T a, MolDual b) {
   const T q = a / b.v;
   return {q, -q * b.d / b.v};
 }
+__device__ __forceinline__ MolDual mol_rcp(const MolDual x) {
+  const T r = mol_rcp(x.v);
+  return {r, -(x.d * r) * r};
+}
+__device__ __forceinline__ MolDual mol_div(const MolDual n, const MolDual d) { return n / d; }
 
 // Gradient of the WENO5 node derivative with respect to its five inputs.
 __device__ __noinline__ void mol_weno5_grad(T um2, T um1, T u0, T up1, T up2, T eps, T inv_dx, T* g, bool cleared) {

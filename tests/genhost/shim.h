@@ -19,6 +19,7 @@ struct MolDim3 {
 };
 static MolDim3 threadIdx, blockIdx, gridDim, blockDim;
 inline void __syncthreads() {}
+#define __builtin_amdgcn_rcpf(x) (1.0f / (x))
 template <class T>
 inline T atomicAdd(T* p, T v) {
   const T o = *p;
