@@ -380,6 +380,31 @@ def test_nonuniform_multidimensional_diffusion(which, dtype):
             assert np.max(np.abs(got - ref)) <= 2e-6 * scale, np.max(np.abs(got - ref))
 
 
+def test_nonuniform_fused_rk4_equals_stage_launches():
+    torch = _torch()
+    grids = [systems.stretched_grid(257, 1.0, seed=4), systems.stretched_grid(193, 1.0, seed=5)]
+    bct = (0, 1, 1, 0)
+    prob = molgpu.discretize(*systems.diffreact_grids(grids, 0.05, 2.5, bct, 2))
+    assert prob.plan()["fused_tiles"] > 0, prob.plan()
+    u0 = np.random.default_rng(5).uniform(0, 1, prob.lowered.n_unknowns)
+    h = min(np.min(np.diff(g)) for g in grids)
+    dt, N = 0.1 * h * h / 0.05 / 8, 21
+    res = {}
+    for fused in (True, False):
+        u = torch.tensor(u0, dtype=torch.float64, device="cuda")
+        print("  nonuniform diffreact fused=%s" % fused, flush=True)
+        prob.integrate(u, 0.0, dt, N, molgpu.RK4, graph=True, fused=fused)
+        torch.cuda.synchronize()
+        res[fused] = u.cpu().numpy()
+    assert_parity(res[True], res[False], rtol=1e-13, what="nonuniform fused vs stage launches")
+    ref = u0.copy()
+    F = lambda w: orc.diffreact_nd_rhs(grids, 2, 0.05, 2.5, bct, w)
+    for _ in range(N):
+        k1 = F(ref); k2 = F(ref + dt / 2 * k1); k3 = F(ref + dt / 2 * k2); k4 = F(ref + dt * k3)
+        ref = ref + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+    assert_parity(res[True], ref, rtol=1e-12, what="nonuniform fused vs oracle")
+
+
 def test_nonuniform_operators_refuse_the_interpreter(monkeypatch):
     """With code generation switched off the handle fails loudly instead of running a kernel
     that has no per-node rows."""

@@ -231,3 +231,24 @@ def test_generated_multidimensional_nonuniform_matches_oracle(which, p):
     for tile in tiles:
         got = genhost.rhs(low, u, tile=tile)
         assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+def _rk4(F, u, t, dt):
+    k1 = F(u, t)
+    k2 = F(u + dt / 2 * k1, t + dt / 2)
+    k3 = F(u + dt / 2 * k2, t + dt / 2)
+    k4 = F(u + dt * k3, t + dt)
+    return u + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+
+
+@pytest.mark.parametrize("c", [0.8, -0.8])
+def test_fused_rk4_step_on_a_nonuniform_grid(c):
+    """The fused step reads per-node rows from the pool in its core positions and refills the
+    Neumann / Robin ghosts each stage: equal to RK4 over the oracle."""
+    x = systems.stretched_grid(301, 1.5, seed=2)
+    bcl, bch = (0, 1, 0.5), (1, 2, 0.1)
+    low = lower(*systems.nonuniform1d(x, 0.7, c, 2, 1, bcl, bch))
+    u = np.random.default_rng(3).uniform(-1, 1, low.n_unknowns)
+    got, _ = genhost.rk4_fused(low, u, 0.1, 1e-6, tile=(64, 1, 1))
+    ref = _rk4(lambda w, t: orc.nonuniform1d_rhs(x, 0.7, c, 2, 1, bcl, bch, w, t), u, 0.1, 1e-6)
+    assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref))
