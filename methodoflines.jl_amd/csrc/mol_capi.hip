@@ -656,6 +656,11 @@ bool fused_geometry(mol_handle* h, const mol_desc* d, const KArgs<double>& ka, c
     if (nt > best_n) { best_n = nt; best = i; }
   }
   if (best < 0) { why = "tile does not fit LDS"; return false; }
+  // a 1D box of up to 1024 unknowns is one tile: the kernel then runs many steps per launch
+  // with the state resident in LDS
+  static const int whole1[3] = {1024, 1, 1};
+  const bool whole = d->ndim == 1 && ka.uhi[0] - ka.ulo[0] + 1 <= whole1[0];
+  if (whole) cand = &whole1, best = 0;
   g = geom_of(ka, kc, ncore);
   g.vol = 1;
   for (int a = 0; a < 3; ++a) {
@@ -698,6 +703,7 @@ void setup_fused(mol_handle* h, const mol_desc* d, const KArgs<double>& ka, cons
   }
   h->fused_tiles = g.ntile[0] * g.ntile[1] * g.ntile[2];
   h->fused_nthr = g.nthr;
+  h->fused_resident = src.find("// resident steps per launch: any") != std::string::npos;
 }
 
 int setup_paths(mol_handle* h, const mol_desc* d) {
@@ -823,10 +829,12 @@ int enqueue_steps(mol_handle* h, T* u, const T* p, double dt, int64_t nsteps, in
   T* acc = (T*)h->w_acc;
   T* comp = (T*)h->w_comp;
   if (scheme == MOL_RK4 && !kahan && h->fused_fn && !h->no_fuse) {
-    // one launch per step, the state alternating between u and the workspace
+    // one launch per step (or per chunk of resident steps), the state alternating between u
+    // and the workspace
     T* src = u;
     T* dst = y1;
-    for (int64_t step = 0; step < nsteps; ++step) {
+    for (int64_t step = 0; step < nsteps;) {
+      const int nsub = h->fused_resident ? (int)std::min<int64_t>(nsteps - step, 1024) : 1;
       Epi<T> e{};
       e.tnow = h->d_tnow + h->clk;
       e.tnext = h->d_tnow + (1 - h->clk);
@@ -834,12 +842,14 @@ int enqueue_steps(mol_handle* h, T* u, const T* p, double dt, int64_t nsteps, in
       e.p = p;
       JitPtrs<T> jp = L.jp;
       const T* sp = src;
-      void* args[] = {(void*)&sp, (void*)&dst, (void*)&e, (void*)&jp};
+      int ns = nsub;
+      void* args[] = {(void*)&sp, (void*)&dst, (void*)&e, (void*)&jp, (void*)&ns};
       if (hipModuleLaunchKernel(h->fused_fn, h->fused_tiles, 1, 1, h->fused_nthr, 1, 1, 0, s, args, nullptr) !=
           hipSuccess)
         return fail(h, MOL_ERR_HIP, "fused RK4 launch failed");
       h->clk ^= 1;
       std::swap(src, dst);
+      step += nsub;
     }
     if (src != u) HIPCHK(h, hipMemcpyAsync(u, src, (size_t)h->nunk * sizeof(T), hipMemcpyDeviceToDevice, s));
     return 0;

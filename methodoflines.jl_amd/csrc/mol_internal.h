@@ -199,6 +199,7 @@ struct mol_handle {
   hipFunction_t fused_fn = nullptr;       // generated fused RK4 step (periodic 1D/2D systems)
   int fused_tiles = 0;
   int fused_nthr = 256;
+  bool fused_resident = false;     // one 1D tile: many steps per fused launch
   std::string fused_reason;               // why there is no fused kernel
   bool no_fuse = false;                   // this integration runs the stage kernels (MOL_FLAG_UNFUSED)
   int vjp_nthr = 256;                     // threads per work-group of the generated VJP kernel

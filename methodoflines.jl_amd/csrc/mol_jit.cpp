@@ -1639,17 +1639,26 @@ std::string Gen::source_fused(bool f64) {
   o << "  static constexpr bool FIX = " << (allper ? "false" : "true") << ";\n";
   if (!allper) o << stage_fn(true);
   o << close;
+  // one 1D tile holding the whole system: nsub resident steps per launch; otherwise one step
+  const bool resident = g_.ntile[0] * g_.ntile[1] * g_.ntile[2] == 1 && d_->ndim == 1 && g_.tile[0] >= halo[0];
   o << "extern \"C\" __global__ void __launch_bounds__(" << g_.nthr
-    << ") mol_rk4_fused(const T* __restrict__ u, T* __restrict__ unew, Epi ep, JPtrs jp) {\n"
+    << ") mol_rk4_fused(const T* __restrict__ u, T* __restrict__ unew, Epi ep, JPtrs jp, int nsub) {\n"
        "  __shared__ T U[GeoF::NST * GeoF::VOL];\n  __shared__ T Ya[GeoF::NST * GeoF::VOL];\n"
        "  __shared__ T Yb[GeoF::NST * GeoF::VOL];\n"
-       "  int b = mol_xcd_tile<GeoBase::NBLK>((int)blockIdx.x);\n"
-       "  const int tx = b % GeoBase::NT0;\n  b /= GeoBase::NT0;\n"
-       "  const int ty = b % GeoBase::NT1;\n  const int tz = b / GeoBase::NT1;\n"
-       "  const double t = ep.tnow ? ep.tnow[0] : 0.0;\n"
-       "  mol_advance_clock(ep);\n"
-       "  mol_rk4_fused_tile<GeoF>(U, Ya, Yb, &mol_fc[0][0], (int)(sizeof(mol_fc[0]) / sizeof(double)), mol_fo,\n"
-       "                           tx, ty, tz, u, unew, ep, jp, t);\n}\n";
+       "  const double t = ep.tnow ? ep.tnow[0] : 0.0;\n";
+  if (resident) {
+    o << "  mol_rk4_fused_resident<GeoF>(U, Ya, Yb, &mol_fc[0][0], (int)(sizeof(mol_fc[0]) / sizeof(double)), mol_fo,\n"
+         "                               u, unew, ep, jp, t, nsub);\n}\n";
+  } else {
+    o << "  (void)nsub;\n"
+         "  int b = mol_xcd_tile<GeoBase::NBLK>((int)blockIdx.x);\n"
+         "  const int tx = b % GeoBase::NT0;\n  b /= GeoBase::NT0;\n"
+         "  const int ty = b % GeoBase::NT1;\n  const int tz = b / GeoBase::NT1;\n"
+         "  mol_advance_clock(ep);\n"
+         "  mol_rk4_fused_tile<GeoF>(U, Ya, Yb, &mol_fc[0][0], (int)(sizeof(mol_fc[0]) / sizeof(double)), mol_fo,\n"
+         "                           tx, ty, tz, u, unew, ep, jp, t);\n}\n";
+  }
+  o << "// resident steps per launch: " << (resident ? "any" : "1") << "\n";
   return o.str();
 }
 

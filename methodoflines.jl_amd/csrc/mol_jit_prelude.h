@@ -612,7 +612,10 @@ template <class G, int S>
 __device__ __forceinline__ void mol_rk4_stage(const T* Lin, T* Lout, const T* U, T (&Ur)[MolFusedItems<G>::NJ][G::NF],
                                               T (&Ar)[MolFusedItems<G>::NJ][G::NF], const int o0, const int o1,
                                               const int o2, const T* __restrict__ u, T* __restrict__ unew,
-                                              const Epi& ep, const JPtrs& jp, const double t, const double dt) {
+                                              const Epi& ep, const JPtrs& jp, const double t, const double dt,
+                                              T* Ukeep = nullptr, const bool store = true) {
+  // Ukeep (resident steps): stage 4 also leaves the new state in Ukeep and in Ur for the next
+  // step; store: stage 4 writes it to unew
   constexpr int M = 4 - S;
   constexpr int TVOL = MolFusedItems<G>::TVOL, NJ = MolFusedItems<G>::NJ;
   constexpr int FR = (G::TX + 2 * M * G::RX) * (G::TY + 2 * M * G::RY) - TVOL;
@@ -646,7 +649,12 @@ __device__ __forceinline__ void mol_rk4_stage(const T* Lin, T* Lout, const T* U,
         Ar[j][f] = a;
       } else {
         v += (T)cf * Ar[j][f];
-        if (I0 <= G::UHI0 && I1 <= G::UHI1 && I2 <= G::UHI2) unew[G::uidx(f, I0, I1, I2)] = v;
+        if (store && I0 <= G::UHI0 && I1 <= G::UHI1 && I2 <= G::UHI2) unew[G::uidx(f, I0, I1, I2)] = v;
+        if (Ukeep) {
+          Ukeep[f * G::VOL + b] = v;
+          Ur[j][f] = v;
+          Ar[j][f] = T(0);
+        }
       }
     }
   }
@@ -757,6 +765,84 @@ __device__ __forceinline__ void mol_rk4_fused_tile(T* U, T* Ya, T* Yb, double* f
     if (mol_rk4_fix_ghosts<G, 3>(Ya, o0, o1, o2, t + dt, ep.p, jp)) __syncthreads();
   }
   mol_rk4_stage<G, 4>(Ya, nullptr, U, Ur, Ar, o0, o1, o2, u, unew, ep, jp, t, dt);
+}
+
+// Resident steps of a 1D system held by a single tile: nsub RK4 steps in one launch with the
+// state in LDS (and, for the tile nodes, in registers).  Between steps the halo of U is
+// refreshed: periodic images copied from the tile (which covers the whole ring), ghosts of a
+// non-periodic axis from their closures at the new time.  The time advances by the same
+// sequence of additions as the device clock of one launch per step.
+template <class G>
+__device__ __forceinline__ void mol_rk4_fused_resident(T* U, T* Ya, T* Yb, double* fc, const int fcs, int* fo,
+                                                       const T* __restrict__ u, T* __restrict__ unew, const Epi& ep,
+                                                       const JPtrs& jp, const double t0, const int nsub) {
+  static_assert(G::NBLK == 1 && G::RY == 0 && G::TY == 1, "resident steps: one 1D tile");
+  const int o0 = G::ULO0, o1 = G::ULO1, o2 = G::ULO2;
+  const double dt = ep.tadv;
+  mol_stage_tile<G>(U, o0, o1, o2, mol_stage_mode<G>(0, 0, 0), t0, ep.p, u, jp);
+  {
+    const int org = o0 - G::HX;
+    if (threadIdx.x == 0) fo[0] = org;
+    if (jp.coord[0])
+      for (int e = threadIdx.x; e < G::EX; e += G::NTHR) {
+        int p = org + e;
+        if (G::PER0) p = mol_wrap(mol_wrap(p, G::N0), G::N0);
+        fc[e] = jp.coord[0][min(max(p, 1), (int)G::N0) - 1];
+      }
+  }
+  __syncthreads();
+  constexpr int NJ = MolFusedItems<G>::NJ;
+  T Ur[NJ][G::NF], Ar[NJ][G::NF];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    int lx, ly;
+    MolFusedItems<G>::pos(j, lx, ly);
+    const bool ok = MolFusedItems<G>::valid(j);
+    const int b = ok ? (lx + G::HX) + G::EX * (ly + G::HY) : 0;
+#pragma unroll
+    for (int f = 0; f < G::NF; ++f) {
+      Ur[j][f] = ok ? U[f * G::VOL + b] : T(0);
+      Ar[j][f] = T(0);
+    }
+  }
+  double t = t0;
+  for (int k = 0; k < nsub; ++k) {
+    const bool last = k == nsub - 1;
+    mol_rk4_stage<G, 1>(U, Ya, U, Ur, Ar, o0, o1, o2, u, unew, ep, jp, t, dt);
+    __syncthreads();
+    if constexpr (G::FIX) {
+      if (mol_rk4_fix_ghosts<G, 1>(Ya, o0, o1, o2, t + dt / 2, ep.p, jp)) __syncthreads();
+    }
+    mol_rk4_stage<G, 2>(Ya, Yb, U, Ur, Ar, o0, o1, o2, u, unew, ep, jp, t, dt);
+    __syncthreads();
+    if constexpr (G::FIX) {
+      if (mol_rk4_fix_ghosts<G, 2>(Yb, o0, o1, o2, t + dt / 2, ep.p, jp)) __syncthreads();
+    }
+    mol_rk4_stage<G, 3>(Yb, Ya, U, Ur, Ar, o0, o1, o2, u, unew, ep, jp, t, dt);
+    __syncthreads();
+    if constexpr (G::FIX) {
+      if (mol_rk4_fix_ghosts<G, 3>(Ya, o0, o1, o2, t + dt, ep.p, jp)) __syncthreads();
+    }
+    // the stage-4 frame is empty: U is only read at tile positions by stage 4 (through Ur)
+    mol_rk4_stage<G, 4>(Ya, nullptr, U, Ur, Ar, o0, o1, o2, u, unew, ep, jp, t, dt, U, last);
+    t = t + dt;
+    if (last) break;
+    __syncthreads();
+    // refresh the halo of U for the next step
+    if constexpr (G::PER0) {
+      constexpr int P = G::TX;   // the tile is the whole ring of unknowns
+      for (int q = threadIdx.x; q < 2 * G::HX; q += G::NTHR) {
+        const int lx = q < G::HX ? q - G::HX : G::TX + (q - G::HX);
+        const int src = lx < 0 ? lx + P : lx - P;
+#pragma unroll
+        for (int f = 0; f < G::NF; ++f) U[f * G::VOL + lx + G::HX] = U[f * G::VOL + src + G::HX];
+      }
+    } else {
+      mol_rk4_fix_ghosts<G, 0>(U, o0, o1, o2, t, ep.p, jp);
+    }
+    __syncthreads();
+  }
+  if (ep.tnext && threadIdx.x == 0) ep.tnext[0] = t;
 }
 
 // Arguments of the generated vector-Jacobian product (layout identical to mol::VArgs<T>).

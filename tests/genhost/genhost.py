@@ -163,15 +163,17 @@ extern "C" __attribute__((visibility("default"))) void run(VArgs va, JPtrs jp, i
     return out, g[: len(low.aux) * ng].reshape(len(low.aux), ng) if low.aux else None
 
 
-def rk4_fused(low, u, t0, dt, p=None, aux=None, tile=None):
-    """Generated fused RK4 step kernel on the CPU: one classic RK4 step from (u, t0)."""
+def rk4_fused(low, u, t0, dt, p=None, aux=None, tile=None, nsub=1):
+    """Generated fused RK4 step kernel on the CPU: one classic RK4 step from (u, t0), or nsub
+    steps when the kernel holds the system in one resident tile."""
     import molgpu
     src = molgpu.codegen(low, tile=tile, fused=True)
     entry = """
-extern "C" __attribute__((visibility("default"))) void run(const T* u, T* unew, Epi ep, JPtrs jp, int nblk) {
+extern "C" __attribute__((visibility("default"))) void run(const T* u, T* unew, Epi ep, JPtrs jp, int nblk,
+                                                            int nsub) {
   gridDim.x = nblk;
   threadIdx.x = 0;
-  for (int b = 0; b < nblk; ++b) { blockIdx.x = b; mol_rk4_fused(u, unew, ep, jp); }
+  for (int b = 0; b < nblk; ++b) { blockIdx.x = b; mol_rk4_fused(u, unew, ep, jp, nsub); }
 }
 """
     lib = _compile(_host_source(src, entry))
@@ -185,6 +187,6 @@ extern "C" __attribute__((visibility("default"))) void run(const T* u, T* unew, 
     ep.tadv = dt
     ep.p = pv.ctypes.data if pv.size else None
     jp, keep = _ptrs(low, aux or {})
-    lib.run.argtypes = [C.c_void_p, C.c_void_p, Epi, JPtrs, C.c_int]
-    lib.run(u.ctypes.data, unew.ctypes.data, ep, jp, _nblocks(src))
+    lib.run.argtypes = [C.c_void_p, C.c_void_p, Epi, JPtrs, C.c_int, C.c_int]
+    lib.run(u.ctypes.data, unew.ctypes.data, ep, jp, _nblocks(src), nsub)
     return unew, clock[1]

@@ -319,3 +319,33 @@ def test_fused_rk4_step_reach_two_periodic(name):
     got, _ = genhost.rk4_fused(low, u, 1.2, 1e-6)
     ref = _rk4_numpy(lambda w, tt: genhost.rhs(low, w, t=tt), u, 1.2, 1e-6)
     assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("bcl,bch", [((1, 0, 0.0), (1, 0, 0.0)), ((0, 1, 0.5), (2, -1, 0.4))])
+def test_fused_resident_steps_1d(bcl, bch):
+    """A 1D system held by one tile runs several RK4 steps per launch with the state resident
+    (ghosts refilled between steps): equal to the oracle's steps, and the clock equals the
+    sequence of single-step additions."""
+    n = 101
+    low = lower(*systems.heat1d(n=n, D=1.0, bcl=bcl, bch=bch))
+    import molgpu
+    assert "// resident steps per launch: any" in molgpu.codegen(low, fused=True)
+    u = np.random.default_rng(1).uniform(-1, 1, low.n_unknowns)
+    got, tn = genhost.rk4_fused(low, u, 0.3, 1e-5, nsub=7)
+    ref = orc.diffusion1d_integrate(n, 1.0, bcl, bch, u, 0.3, 1e-5, 7)
+    tref = 0.3
+    for _ in range(7):
+        tref = tref + 1e-5
+    assert tn == tref
+    assert np.max(np.abs(got - ref)) <= 1e-14 * (1 + np.max(np.abs(ref)))
+
+
+def test_fused_resident_steps_periodic():
+    low = lower(*systems.nonlinlap1d(101, True, 0, 0, 2))
+    u = np.asarray(low.u0, dtype=np.float64)
+    ref, t = u.copy(), 0.0
+    for _ in range(5):
+        ref = _rk4_numpy(lambda w, tt: genhost.rhs(low, w, t=tt), ref, t, 1e-5)
+        t += 1e-5
+    got, _ = genhost.rk4_fused(low, u, 0.0, 1e-5, nsub=5)
+    assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref))
