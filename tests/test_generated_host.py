@@ -349,3 +349,25 @@ def test_fused_resident_steps_periodic():
         t += 1e-5
     got, _ = genhost.rk4_fused(low, u, 0.0, 1e-5, nsub=5)
     assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref))
+
+
+def test_fused_resident_steps_with_time_dependent_boundary_values():
+    from molgpu.pdesys import Differential, Eq, Interval, PDESystem, parameters, variables
+    from molgpu.discretization import MOLFiniteDifference
+    import sympy as sp
+    t, x = parameters("t x")
+    u = variables("u")
+    Dx = Differential(x)
+    eq = Eq(Differential(t)(u(t, x)), 0.5 * (Dx ** 2)(u(t, x)) + sp.sin(x) * u(t, x))
+    bcs = [Eq(u(0, x), x * (1 - x)), Eq(u(t, 0), sp.sin(5 * t)), Eq(Dx(u(t, 1)), sp.cos(3 * t))]
+    low = lower(PDESystem([eq], bcs, [Interval(t, 0.0, 1.0), Interval(x, 0.0, 1.0)], [t, x], [u(t, x)]),
+                MOLFiniteDifference({x: 61}, t))
+    u0 = np.random.default_rng(4).uniform(-1, 1, low.n_unknowns)
+    t0, dt = 0.3, 1e-3
+    ref, tt = u0.copy(), t0
+    for _ in range(3):
+        ref = _rk4_numpy(lambda w, s: genhost.rhs(low, w, t=s), ref, tt, dt)
+        tt = tt + dt
+    got, tn = genhost.rk4_fused(low, u0, t0, dt, nsub=3)
+    assert tn == tt
+    assert np.max(np.abs(got - ref)) <= 1e-13 * (1 + np.max(np.abs(ref)))
