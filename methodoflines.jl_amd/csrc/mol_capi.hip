@@ -290,7 +290,7 @@ void fill_kargs(mol_handle* h, const mol_desc* d, KArgs<T>& ka, size_t lds_pos =
     const int (*cand)[3] = d->ndim == 2 ? cand2 : cand3;
     const int nc = d->ndim == 2 ? 6 : 7;
     const bool rev = jit_tiles == 2;
-    const size_t budget = rev ? 163000 : (d->ndim == 2 ? 40960 : 57344);
+    const size_t budget = rev ? 163000 : (d->ndim == 2 ? 40960 : 81920);
     const int64_t enough = rev ? 256 : 1024;
     int best = -1;
     int64_t best_tiles = -1;
@@ -316,6 +316,7 @@ void fill_kargs(mol_handle* h, const mol_desc* d, KArgs<T>& ka, size_t lds_pos =
       for (int a = 0; a < 3; ++a)
         tile[a] = a >= d->ndim ? 1 : std::max(1, std::min(cand[best][a], ka.uhi[a] - ka.ulo[a] + 1));
   }
+  if (jit_tiles == 1 && d->ndim == 3) lds_cap = std::max<size_t>(lds_cap, 81920);
   while (lds_pos * vol_of(tile) > lds_cap) {
     int big = 0;
     for (int a = 1; a < 3; ++a)
@@ -617,6 +618,8 @@ int kernel_threads(const JitGeom& g, bool vjp) {
   const char* e = std::getenv(vjp ? "MOL_VJP_NTHR" : "MOL_NTHR");
   int n = 256;
   if (vjp) n = g.tile[0] * g.tile[1] * g.tile[2];
+  // 3D forward tiles: two nodes per thread (16x8x8: 512 threads, two work-groups per CU)
+  else if (g.tile[2] > 1) n = std::max(256, g.tile[0] * g.tile[1] * g.tile[2] / 2);
   if (e) n = std::atoi(e);
   n = (std::max(64, std::min(1024, n)) + 63) / 64 * 64;
   return vjp ? std::max(256, n) : n;
