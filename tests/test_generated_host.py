@@ -203,8 +203,13 @@ def test_generated_forward_kernel_persistent_launch(name, grid):
     turn) with the next tile's staging prefetched, against the oracle."""
     low, u, t, aux, ref, scale = CASES[name]()
     tile = {1: (16, 1, 1), 2: (8, 8, 1), 3: (4, 4, 4)}[low.ndim]
+    if low.ndim == 3:
+        # 3D tiles run one work-group per tile (the library launches the persistent grid only
+        # for a source with the persistent loop)
+        assert "mol_tiles_persistent<" not in __import__("molgpu").codegen(low, tile=tile)
+        grid = None
     got = genhost.rhs(low, u, t=t, aux=aux, tile=tile, grid=grid)
-    assert_parity(got, ref, scale, what="%s grid %d" % (name, grid))
+    assert_parity(got, ref, scale, what="%s grid %s" % (name, grid))
 
 
 @pytest.mark.parametrize("n,tile", [(33, None), (33, (16, 8, 1)), (37, (16, 16, 1)), (29, (8, 8, 1))])
