@@ -950,6 +950,10 @@ struct MolAxis {
 template <class GC, class GB, bool HASC, int VOLX, int NS, typename B>
 __device__ __forceinline__ double mol_tile_buf(const B* __restrict__ base, const int s, const int t0, const int t1,
                                                const int t2, const int l0, const int l1, const int l2) {
+  // cross-shaped staging: positions outside the tile along two or more axes hold no adjoint
+  // (no diagonal taps) and the tile kernel leaves their buffer entries unwritten
+  if (GB::CROSS && (int)(l0 < 0 || l0 >= GB::TX) + (int)(l1 < 0 || l1 >= GB::TY) + (int)(l2 < 0 || l2 >= GB::TZ) >= 2)
+    return 0.0;
   const int tl = t0 + GB::NT0 * (t1 + GB::NT1 * t2);
   int q;
   if (HASC && GB::is_core(t0, t1, t2)) {
@@ -1014,9 +1018,10 @@ __device__ __forceinline__ void mol_vjp_tile(T* L, T* Ab, double* Aa, const int 
     const int mode = mol_stage_mode<G>(tx, ty, tz);
     MolWalk<G::EX, G::EY, G::NTHR> w((int)threadIdx.x);
     for (int q = threadIdx.x; q < G::VOL; q += G::NTHR, w.step()) {
-      if (!mol_unread<G>(w.x, w.y, w.z))
+      if (!mol_unread<G>(w.x, w.y, w.z)) {
         G::stage_pos(L, q, o0 + w.x - G::HX, o1 + w.y - G::HY, o2 + w.z - G::HZ, mode, t, va.p, va.u, jp);
-      G::zero_pos(Ab, Aa, q);
+        G::zero_pos(Ab, Aa, q);
+      }
     }
   }
   __syncthreads();
@@ -1054,7 +1059,8 @@ __device__ __forceinline__ void mol_vjp_tile(T* L, T* Ab, double* Aa, const int 
   const bool acc = (va.mode & 4) != 0;
   MolWalk<G::EX, G::EY, G::NTHR> w((int)threadIdx.x);
   for (int q = threadIdx.x; q < G::VOL; q += G::NTHR, w.step())
-    G::flush_pos(Ab, Aa, q, o0 + w.x - G::HX, o1 + w.y - G::HY, o2 + w.z - G::HZ, va.out, bft, bat, VOLX, acc);
+    if (!mol_unread<G>(w.x, w.y, w.z))
+      G::flush_pos(Ab, Aa, q, o0 + w.x - G::HX, o1 + w.y - G::HY, o2 + w.z - G::HZ, va.out, bft, bat, VOLX, acc);
 }
 
 // ---------------------------------------------------------------------------------------
