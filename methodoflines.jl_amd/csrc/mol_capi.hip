@@ -998,6 +998,7 @@ int ensure_vjp(mol_handle* h) {
   std::string err;
   const std::string src = jit_vjp_source(d, g, err);
   if (src.empty()) return fail(h, MOL_ERR_UNSUPPORTED, "vector-Jacobian product: " + err);
+  h->vjp_store = src.find("// adjoint gather stores every unknown") != std::string::npos;
   if (jit_kernel(src, "mol_vjp_tiles", h->device, &h->vjp_fn, err) ||
       jit_kernel(src, "mol_vjp_gather", h->device, &h->vjp_gather_fn, err)) {
     h->vjp_fn = nullptr;
@@ -1189,9 +1190,10 @@ int adjoint_impl(mol_handle* h, const T* p, T* lam, double* g_aux, hipStream_t s
   T* S = (T*)h->seg_u;
   auto st = [&](int64_t j, int k) { return S + (size_t)(4 * j + k) * n; };   // state k of step j
   T* Y[4] = {(T*)h->adj_y[0], (T*)h->adj_y[1], (T*)h->adj_y[2], (T*)h->adj_y[3]};
-  // the stage adjoints accumulate: cleared here, then by every step's sum kernel; the tile
-  // buffers of the grid-field adjoints accumulate over the whole sweep and are gathered once
-  for (int k = 0; k < 4; ++k)
+  // the stage adjoints accumulate unless the gather stores them: cleared here, then by every
+  // step's sum kernel; the tile buffers of the grid-field adjoints accumulate over the whole
+  // sweep and are gathered once
+  for (int k = 0; k < 4 && !h->vjp_store; ++k)
     if ((rc = launch_fill_zero<T>(Y[k], n, s))) return fail(h, rc, "zero fill launch failed");
   const bool acc_aux = g_aux && h->vjp_ba && h->vjp_ba_count > 0;
   if (acc_aux && (rc = launch_fill_zero<double>((double*)h->vjp_ba, (int64_t)h->vjp_ba_count, s)))
@@ -1232,7 +1234,8 @@ int adjoint_impl(mol_handle* h, const T* p, T* lam, double* g_aux, hipStream_t s
         va.mode = acc_aux ? (VJP_GATHER_F | VJP_ACC_A) : (VJP_GATHER_F | VJP_GATHER_A);
         if ((rc = launch_vjp<T>(h, va, s))) return rc;
       }
-      if ((rc = launch_adjoint_sum<T>(lam, Y[0], Y[1], Y[2], Y[3], n, s))) return fail(h, rc, "adjoint update launch failed");
+      if ((rc = launch_adjoint_sum<T>(lam, Y[0], Y[1], Y[2], Y[3], n, !h->vjp_store, s)))
+        return fail(h, rc, "adjoint update launch failed");
     }
   }
   if (acc_aux) {
@@ -1679,13 +1682,13 @@ int mol_vjp(mol_handle* h, const void* u, const void* p, double t, const void* l
     VArgs<double> va{(const double*)u, (const double*)lam, nullptr, 1.0, 0.0, (double*)lam_dot, g_aux,
                      (const double*)p, nullptr, t};
     va.mode = VJP_GATHER_F | VJP_GATHER_A;
-    if (launch_fill_zero<double>((double*)lam_dot, h->nunk, s)) return fail(h, MOL_ERR_HIP, "zero fill launch failed");
+    if (!h->vjp_store && launch_fill_zero<double>((double*)lam_dot, h->nunk, s)) return fail(h, MOL_ERR_HIP, "zero fill launch failed");
     return launch_vjp<double>(h, va, s);
   }
   VArgs<float> va{(const float*)u, (const float*)lam, nullptr, 1.0, 0.0, (float*)lam_dot, g_aux,
                   (const float*)p, nullptr, t};
   va.mode = VJP_GATHER_F | VJP_GATHER_A;
-  if (launch_fill_zero<float>((float*)lam_dot, h->nunk, s)) return fail(h, MOL_ERR_HIP, "zero fill launch failed");
+  if (!h->vjp_store && launch_fill_zero<float>((float*)lam_dot, h->nunk, s)) return fail(h, MOL_ERR_HIP, "zero fill launch failed");
   return launch_vjp<float>(h, va, s);
 }
 

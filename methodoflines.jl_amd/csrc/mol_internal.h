@@ -121,9 +121,9 @@ int launch_weno2d(const KCore<T>& kc, const WenoPlan& plan, int nblocks, const T
 constexpr int WENO_TILE_X = 64, WENO_TILE_Y = 32;
 template <typename T>
 int launch_fill_zero(T* p, int64_t n, hipStream_t stream);
-// lam += y0 + y1 + y2 + y3 (the adjoint of one RK4 step), then y0..y3 = 0
+// lam += y0 + y1 + y2 + y3 (the adjoint of one RK4 step), then y0..y3 = 0 when `zero`
 template <typename T>
-int launch_adjoint_sum(T* lam, T* y0, T* y1, T* y2, T* y3, int64_t n, hipStream_t stream);
+int launch_adjoint_sum(T* lam, T* y0, T* y1, T* y2, T* y3, int64_t n, bool zero, hipStream_t stream);
 
 // Set the message of the calling thread's last failure (used when no handle exists yet).
 void set_global_error(const std::string& msg);
@@ -204,6 +204,7 @@ struct mol_handle {
   bool no_fuse = false;                   // this integration runs the stage kernels (MOL_FLAG_UNFUSED)
   int vjp_nthr = 256;                     // threads per work-group of the generated VJP kernel
   hipFunction_t vjp_fn = nullptr;         // generated vector-Jacobian product kernel
+  bool vjp_store = false;                 // its gather stores every unknown (no zeroing of the output)
   hipFunction_t vjp_gather_fn = nullptr;  // its gather of the per-tile adjoint buffers
   void* vjp_bf = nullptr;                 // per-tile adjoint buffers: staged variables (T)
   void* vjp_ba = nullptr;                 //   and staged grid fields (FP64)

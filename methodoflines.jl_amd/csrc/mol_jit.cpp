@@ -244,6 +244,14 @@ class Gen {
     std::vector<int> aslot;    // per grid field: adjoint slot (reverse mode)
   };
   PrintGeo pg_;
+  // reverse mode: every variable is staged, so the tile kernel writes only the tile buffers and
+  // the gather kernel stores each unknown's whole adjoint (no zeroing, no atomics)
+  bool gather_stores() const {
+    if (param_ || pg_.fslot.size() < (size_t)d_->nfields) return false;
+    for (int f = 0; f < d_->nfields; ++f)
+      if (pg_.fslot[f] < 0) return false;
+    return true;
+  }
  public:
   // Arrays read away from the node (these are staged in LDS) and their reach per axis, over
   // all variants and over the interior-rows variant only.
@@ -1470,6 +1478,11 @@ std::string Gen::geo_variant(const std::string& name, const int* halo, const Usa
     for (int f = 0; f < d_->nfields; ++f)
       if (pg_.fslot[f] >= 0) {
         const long long k = pg_.fslot[f];
+        if (gather_stores()) {
+          // the gather applies the transposed closures to the ghost positions' buffer entries
+          o << "    bft[" << k << " * vs + q] = Ab[" << k * pg_.vol << " + q];\n";
+          continue;
+        }
         o << "    { const T v = Ab[" << k * pg_.vol << " + q];\n"
           << "      if (inbox(" << f << ", w0, w1, w2)) bft[" << k << " * vs + q] = v;\n"
           << "      else { bft[" << k << " * vs + q] = T(0); if (v != T(0)) scatter(" << f << ", p0, p1, p2, v, outp); } }\n";
@@ -1571,6 +1584,7 @@ std::string Gen::source_kernel(bool f64, bool vjp) {
       for (int a = 0; a < 3; ++a) sz *= g_.hi[f][a] - g_.lo[f][a] + 1;
       nunk = std::max(nunk, (long long)g_.fofs[f] + sz);
     }
+    if (gather_stores()) o << "// adjoint gather stores every unknown\n";
     o << "extern \"C\" __global__ void __launch_bounds__(256) mol_vjp_gather(VArgs va) {\n"
          "  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;\n"
          "  (void)va;\n";
@@ -1581,12 +1595,36 @@ std::string Gen::source_kernel(bool f64, bool vjp) {
       long long sz = 1;
       for (int a = 0; a < 3; ++a) sz *= g_.hi[f][a] - g_.lo[f][a] + 1;
       const int bx = g_.hi[f][0] - g_.lo[f][0] + 1, by = g_.hi[f][1] - g_.lo[f][1] + 1;
+      const std::string gp = "mol_gather_pos<" + gc + ", GeoB, " + hasc + ", VOLX, GeoB::NSTF>(va.bf, " +
+                             istr(pg_.fslot[f]) + ", ";
       o << "    if (i >= " << g_.fofs[f] << "LL && i < " << (long long)g_.fofs[f] + sz << "LL) {\n"
         << "      const int r = (int)(i - " << g_.fofs[f] << "LL), r1 = r / " << bx << ";\n"
         << "      const int p0 = " << g_.lo[f][0] << " + (r - r1 * " << bx << "), p1 = " << g_.lo[f][1] << " + r1 % " << by
         << ", p2 = " << g_.lo[f][2] << " + r1 / " << by << ";\n"
-        << "      va.out[i] += (T)mol_gather_pos<" << gc << ", GeoB, " << hasc << ", VOLX, GeoB::NSTF>(va.bf, "
-        << pg_.fslot[f] << ", p0, p1, p2);\n    }\n";
+        << "      double s = " << gp << "p0, p1, p2);\n";
+      if (gather_stores()) {
+        // transposed ghost closures: the unknown at source m of a closure receives
+        // sum_g w[g][m] * (adjoint of ghost g at the same position along the other axes)
+        for (int c = 0; c < d_->nclosures; ++c) {
+          const mol_closure& cl = d_->closures_h[c];
+          if (cl.field != f) continue;
+          const int a = cl.axis;
+          for (int m = 0; m < cl.K; ++m) {
+            std::ostringstream terms;
+            for (int gi = 0; gi < cl.nghost; ++gi) {
+              const double w = d_->pool_h[cl.wC + gi * cl.K + m];
+              const int gpos = cl.side == 0 ? g_.lo[f][a] - 1 - gi : g_.hi[f][a] + 1 + gi;
+              if (w == 0.0 || gpos < 1 || gpos > g_.n[a]) continue;
+              std::string q[3] = {"p0", "p1", "p2"};
+              q[a] = istr(gpos);
+              terms << "        s += (double)" << lit(w) << " * " << gp << q[0] << ", " << q[1] << ", " << q[2] << ");\n";
+            }
+            if (terms.str().empty()) continue;
+            o << "      if (p" << a << " == " << closure_pos(f, a, cl.side, m) << ") {\n" << terms.str() << "      }\n";
+          }
+        }
+      }
+      o << "      va.out[i] " << (gather_stores() ? "=" : "+=") << " (T)s;\n    }\n";
     }
     o << "    return;\n  }\n";
     bool any_aux = false;

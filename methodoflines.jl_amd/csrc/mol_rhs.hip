@@ -259,27 +259,29 @@ int launch_fill_zero(T* p, int64_t n, hipStream_t stream) {
 // buffers are cleared for the next step (they accumulate with atomics)
 template <typename T>
 __global__ void adjoint_sum_kernel(T* __restrict__ lam, T* __restrict__ y0, T* __restrict__ y1, T* __restrict__ y2,
-                                   T* __restrict__ y3, int64_t n) {
+                                   T* __restrict__ y3, int64_t n, bool zero) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     lam[i] = (((lam[i] + y0[i]) + y1[i]) + y2[i]) + y3[i];
-    y0[i] = T(0);
-    y1[i] = T(0);
-    y2[i] = T(0);
-    y3[i] = T(0);
+    if (zero) {
+      y0[i] = T(0);
+      y1[i] = T(0);
+      y2[i] = T(0);
+      y3[i] = T(0);
+    }
   }
 }
 
 template <typename T>
-int launch_adjoint_sum(T* lam, T* y0, T* y1, T* y2, T* y3, int64_t n, hipStream_t stream) {
+int launch_adjoint_sum(T* lam, T* y0, T* y1, T* y2, T* y3, int64_t n, bool zero, hipStream_t stream) {
   if (n <= 0) return 0;
   const int blocks = (int)std::min<int64_t>((n + 255) / 256, 8192);
-  hipLaunchKernelGGL(adjoint_sum_kernel<T>, dim3(blocks), dim3(256), 0, stream, lam, y0, y1, y2, y3, n);
+  hipLaunchKernelGGL(adjoint_sum_kernel<T>, dim3(blocks), dim3(256), 0, stream, lam, y0, y1, y2, y3, n, zero);
   return hipGetLastError() == hipSuccess ? 0 : MOL_ERR_HIP;
 }
 
 template int launch_fill_zero<double>(double*, int64_t, hipStream_t);
 template int launch_fill_zero<float>(float*, int64_t, hipStream_t);
-template int launch_adjoint_sum<double>(double*, double*, double*, double*, double*, int64_t, hipStream_t);
-template int launch_adjoint_sum<float>(float*, float*, float*, float*, float*, int64_t, hipStream_t);
+template int launch_adjoint_sum<double>(double*, double*, double*, double*, double*, int64_t, bool, hipStream_t);
+template int launch_adjoint_sum<float>(float*, float*, float*, float*, float*, int64_t, bool, hipStream_t);
 
 }  // namespace mol
