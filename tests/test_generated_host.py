@@ -376,3 +376,27 @@ def test_fused_resident_steps_with_time_dependent_boundary_values():
     got, tn = genhost.rk4_fused(low, u0, t0, dt, nsub=3)
     assert tn == tt
     assert np.max(np.abs(got - ref)) <= 1e-13 * (1 + np.max(np.abs(ref)))
+
+
+def test_operations_without_consumers_do_not_widen_the_halo():
+    """The outer boundary row of the 3D nonlinear Laplacian has a zero third weight
+    (half_offset_centred_difference.jl boundary rows); the interpolation it would weight is not
+    generated, so the boundary tiles stage the interior reach (2), not 3."""
+    import re
+    import molgpu
+    low = lower(*systems.nonlindiff3d(n=24))
+    assert low.halo == [3, 3, 3]                 # the lowering's conservative span of the rows
+    for vjp in (False, True):
+        src = molgpu.codegen(low, tile=(8, 4, 4), vjp=vjp)
+        assert set(re.findall(r"static constexpr int H[XYZ] = (\d+);", src)) == {"2"}, vjp
+
+
+def test_reverse_gather_stores_with_every_variable_staged():
+    """With every variable staged the gather stores each unknown (ghost adjoints through the
+    transposed closures in the gather, no atomics on the output)."""
+    import molgpu
+    src = molgpu.codegen(lower(*systems.nonlindiff3d(n=12)), tile=(8, 4, 4), vjp=True)
+    assert "// adjoint gather stores every unknown" in src
+    assert "va.out[i] = (T)s;" in src and "va.out[i] +=" not in src
+    body = src[src.index("mol_vjp_gather(VArgs va)"):]
+    assert "T(0x1.5555555555556p+0) * mol_gather_pos" in body   # Neumann ghost u1 = 4/3 u2 - 1/3 u3
