@@ -1093,6 +1093,7 @@ void Gen::print_reverse(std::ostream& o, int r, const std::vector<std::pair<int,
     }
     o << "\n";
   }
+  std::ostringstream aux_adds;
   for (const auto& kv : acc) {
     const int arr = kv.first[0];
     const bool field = arr < d_->nfields;
@@ -1112,8 +1113,11 @@ void Gen::print_reverse(std::ostream& o, int r, const std::vector<std::pair<int,
     if (field)
       o << "    atomicAdd(&Ab[" << (long long)sl * pg_.vol << " + " << pos << "], " << kv.second << ");\n";
     else
-      o << "    atomicAdd(&Aa[" << (long long)sl * pg_.vol << " + " << pos << "], (double)(" << kv.second << "));\n";
+      aux_adds << "      atomicAdd(&Aa[" << (long long)sl * pg_.vol << " + " << pos << "], (double)(" << kv.second
+               << "));\n";
   }
+  // grid-field adjoints only when the caller asks for them (a uniform branch)
+  if (!aux_adds.str().empty()) o << "    if (gaux) {\n" << aux_adds.str() << "    }\n";
 }
 
 // Slow staging of one position: ring wrap, zero outside the grid and at corners, the unknown
