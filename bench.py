@@ -212,7 +212,7 @@ def bench_adjoint(n, nsteps, every, fd_cells):
     tgd = torch.tensor(tgt, device="cuda")
     g = torch.zeros(2 * ng, dtype=torch.float64, device="cuda")
 
-    def run(ck):
+    def run(ck, grid_grads=True):
         u = u0d.clone()
         e0, e1 = events()
         e2 = torch.cuda.Event(enable_timing=True)
@@ -220,8 +220,9 @@ def bench_adjoint(n, nsteps, every, fd_cells):
         p.forward_checkpointed(u, 0.0, dt, nsteps, ck)
         e1.record()
         lam = (u - tgd).contiguous()
-        g.zero_()
-        p.adjoint(lam, g)
+        if grid_grads:
+            g.zero_()
+        p.adjoint(lam, g if grid_grads else None)
         e2.record()
         torch.cuda.synchronize()
         return e0.elapsed_time(e1), e1.elapsed_time(e2), u
@@ -244,7 +245,11 @@ def bench_adjoint(n, nsteps, every, fd_cells):
         fc_ms, a_ms, uT = run(ck)
         modes["ckpt_every=%d" % ck] = {"forward_ckpt_ms": fc_ms, "adjoint_ms": a_ms, "ratio": a_ms / fwd_ms,
                                        "forward_plus_adjoint_ratio": (fc_ms + a_ms) / fwd_ms}
-    # g holds the gradient of the last mode run (the automatic choice)
+    # the initial-state gradient alone (no grid-field gradients), every stage state kept
+    run(0, False)
+    _, a0_ms, _ = run(0, False)
+    modes["ckpt_every=0, initial state only"] = {"adjoint_ms": a0_ms, "ratio": a0_ms / fwd_ms}
+    # g holds the gradient of the last mode run with grid-field gradients (the automatic choice)
     ia = p.lowered.aux.index("d0")
     gd0 = g[ia * ng:(ia + 1) * ng].cpu().numpy()
     # finite differences of the GPU solve on seeded cells (Richardson-extrapolated central)
