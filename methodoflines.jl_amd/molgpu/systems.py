@@ -124,6 +124,19 @@ def diffreact_grids(grids, D: float, k: float, bctype, approx_order: int = 2, dt
     return sys_, MOLFiniteDifference({x: g for x, g in zip(xs, grids)}, t, approx_order=approx_order, dtype=dtype)
 
 
+def diffusion_ode1d(n: int = 41, D: float = 0.3):
+    """u_t = D u_xx - u v, v_t = u^2 - v/2 on [0, 1]: v has no spatial derivative, so every
+    node of v is an unknown and v is read only at its node (u(t,0) = 0, u_x(t,1) = 0)."""
+    t, x = parameters("t x")
+    u, v = variables("u v")
+    Dx = Differential(x)
+    U, V = u(t, x), v(t, x)
+    eqs = [Eq(Differential(t)(U), D * (Dx ** 2)(U) - U * V), Eq(Differential(t)(V), U * U - 0.5 * V)]
+    bcs = [Eq(u(0, x), x * (1 - x)), Eq(v(0, x), 1 + x), Eq(u(t, 0), 0), Eq(Dx(u(t, 1)), 0)]
+    return (PDESystem(eqs, bcs, [Interval(t, 0.0, 1.0), Interval(x, 0.0, 1.0)], [t, x], [U, V]),
+            MOLFiniteDifference({x: n}, t))
+
+
 def nonlindiff3d(n: int = 128, approx_order: int = 2, dtype: str = "f64"):
     """u_t = div(d0 (1 + u^2) grad u) + s on [0,1]^3 with Neumann BCs; d0 and s are grid fields."""
     t, x, y, z = parameters("t x y z")

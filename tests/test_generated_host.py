@@ -400,3 +400,25 @@ def test_reverse_gather_stores_with_every_variable_staged():
     assert "va.out[i] = (T)s;" in src and "va.out[i] +=" not in src
     body = src[src.index("mol_vjp_gather(VArgs va)"):]
     assert "T(0x1.5555555555556p+0) * mol_gather_pos" in body   # Neumann ghost u1 = 4/3 u2 - 1/3 u3
+
+
+def test_reverse_mode_with_a_variable_read_only_at_its_node():
+    """v of the diffusion + ODE system is not staged: its adjoint goes straight to the output,
+    so the gather accumulates (output cleared first) instead of storing."""
+    import molgpu
+    low = lower(*systems.diffusion_ode1d(41))
+    assert "// adjoint gather stores every unknown" not in molgpu.codegen(low, vjp=True)
+    rng = np.random.default_rng(31)
+    u = rng.uniform(0.2, 1.2, low.n_unknowns)
+    lam = rng.standard_normal(low.n_unknowns)
+    v = rng.standard_normal(low.n_unknowns)
+    for tile in [(16, 1, 1), None]:
+        out, _ = genhost.vjp(low, u, lam, tile=tile)
+
+        def val(e):
+            return lam @ genhost.rhs(low, u + e * v, tile=tile)
+
+        d1 = (val(1e-4) - val(-1e-4)) / 2e-4
+        d2 = (val(5e-5) - val(-5e-5)) / 1e-4
+        fd = (4 * d2 - d1) / 3
+        assert abs(out @ v - fd) <= 1e-7 * abs(fd) + 1e-9 * np.linalg.norm(out) * np.linalg.norm(v), tile

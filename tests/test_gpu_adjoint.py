@@ -126,7 +126,7 @@ def _gpu_dot_fd(prob, u, lam, v, t=0.0, eps=1e-5):
 @pytest.mark.parametrize("name", ["brusselator", "burgers_weno", "mixed_p4", "spherical", "upwind_o2",
                                   "nonlinlap_periodic_p4", "weno_dirichlet", "diffreact_mixed_bcs",
                                   "weno_nonuniform", "weno_nonuniform_periodic", "nonuniform_centered_upwind",
-                                  "nonuniform_nonlinlap_p4", "nonuniform_spherical"])
+                                  "nonuniform_nonlinlap_p4", "nonuniform_spherical", "node_only_variable"])
 def test_vjp_dot_product_every_slot_kind(name):
     """Reverse mode of every operator-slot kind against the forward kernel's directional
     derivative (small tiles: boundary-row variants and ghost closures included)."""
@@ -150,6 +150,8 @@ def test_vjp_dot_product_every_slot_kind(name):
                                                                 4), (-1, 1), 0.0),
         "nonuniform_spherical": (lambda: systems.spherical1d(systems.stretched_grid(61, 1.0, 5), 0.25, 2), (-1, 1),
                                  0.0),
+        # v is read only at its node: the output is cleared and the gather accumulates
+        "node_only_variable": (lambda: systems.diffusion_ode1d(61), (0.2, 1.2), 0.0),
     }
     build, (lo, hi), t = mk[name]
     sys_, disc = build()
