@@ -318,3 +318,179 @@ def test_oracle_rejects_a_state_of_the_wrong_length():
             orc.spherical1d_rhs(n, 0.4, np.zeros(bad))
     with pytest.raises(RuntimeError):
         orc.diffusion1d_rhs(n, 1.0, (1, 0, 0), (1, 0, 0), np.zeros(n))
+
+
+# ---------------------------------------------------------------- WENO5 weights and smoothness indicators
+def test_weno_linear_limit_is_the_fourth_order_central_difference():
+    """With eps far above every smoothness indicator the nonlinear weights equal the ideal weights,
+    and (u-_{i+1/2} - u+_{i-1/2})/dx of the two fifth-order reconstructions collapses to the
+    five-point central difference (u_{-2} - 8u_{-1} + 8u_1 - u_2)/(12 dx) for ANY data.  Swapping the
+    two ideal-weight sets of WENO.jl:14-43 (or any other choice) breaks this identity."""
+    rng = np.random.default_rng(21)
+    for _ in range(200):
+        u = rng.uniform(-3, 3, 5)
+        dx = rng.uniform(1e-3, 1.0)
+        exact = (u[0] - 8 * u[1] + 8 * u[3] - u[4]) / (12 * dx)
+        got = orc.weno_f(u, 1e30, dx)
+        assert abs(got - exact) <= 1e-12 * np.abs(u).max() / dx, (u, dx, got, exact)
+
+
+def _jiang_shu_from_definitions(u, eps, dx):
+    """Jiang & Shu (J. Comput. Phys. 126, 1996), built from its definitions rather than from
+    its closed-form constants: the five values are cell averages over cells of width dx centred
+    at j*dx (j = -2..2); each candidate quadratic p_k matches the averages of three adjacent cells;
+    the smoothness indicator is beta_k = sum_{l=1,2} dx^(2l-1) * integral over cell 0 of
+    (d^l p_k / dx^l)^2; the ideal weights are the ones whose combination of the candidates equals
+    the quartic that matches all five averages, at x = +dx/2 and at x = -dx/2 separately.
+    Returns (u^-_{+1/2} - u^+_{-1/2}) / dx with weights gamma_k / (eps + beta_k)^2, normalised."""
+    from numpy.polynomial import polynomial as P
+
+    def avg_row(j, deg):
+        # cell average of x^m over [(j - 1/2)dx, (j + 1/2)dx], m = 0..deg
+        a, b = (j - 0.5) * dx, (j + 0.5) * dx
+        return np.array([(b ** (m + 1) - a ** (m + 1)) / ((m + 1) * dx) for m in range(deg + 1)])
+
+    def fit(cells, vals):
+        A = np.array([avg_row(j, len(cells) - 1) for j in cells])
+        return np.linalg.solve(A, vals)            # monomial coefficients
+
+    stencils = [(-2, -1, 0), (-1, 0, 1), (0, 1, 2)]
+    polys = [fit(s, u[[j + 2 for j in s]]) for s in stencils]
+    beta = []
+    for c in polys:
+        tot = 0.0
+        for l in (1, 2):
+            d = P.polyder(c, l)
+            sq = P.polyint(P.polymul(d, d))
+            tot += dx ** (2 * l - 1) * (P.polyval(dx / 2, sq) - P.polyval(-dx / 2, sq))
+        beta.append(tot)
+    beta = np.array(beta)
+
+    def ideal(xe):
+        # coefficients of u_{-2..2} in each candidate's value at xe, and in the quartic's value
+        def row(cells):
+            A = np.array([avg_row(j, len(cells) - 1) for j in cells])
+            mon = np.array([xe ** m for m in range(len(cells))])
+            coef = np.zeros(5)
+            coef[[j + 2 for j in cells]] = np.linalg.solve(A.T, mon)
+            return coef
+        M = np.array([row(s) for s in stencils]).T   # 5 x 3
+        g, *_ = np.linalg.lstsq(M, row((-2, -1, 0, 1, 2)), rcond=None)
+        return g
+
+    def recon(xe):
+        g = ideal(xe)
+        w = g / (eps + beta) ** 2
+        w = w / w.sum()
+        return sum(w[k] * P.polyval(xe, polys[k]) for k in range(3))
+
+    return (recon(dx / 2) - recon(-dx / 2)) / dx
+
+
+def test_weno_ideal_weights_from_the_definition():
+    """The derived ideal weights are the published (1/10, 6/10, 3/10) at the right face and their
+    mirror at the left face (a check of the independent build itself)."""
+    from numpy.polynomial import polynomial as P  # noqa: F401
+    rng = np.random.default_rng(0)
+    u = rng.uniform(-1, 1, 5)
+    # eps -> infinity: the independent build must reduce to the five-point central difference too
+    got = _jiang_shu_from_definitions(u, 1e30, 0.25)
+    exact = (u[0] - 8 * u[1] + 8 * u[3] - u[4]) / (12 * 0.25)
+    assert abs(got - exact) < 1e-12 * np.abs(u).max() / 0.25
+
+
+@pytest.mark.parametrize("eps", [1e-6, 1e-2, 1.0])
+def test_weno_matches_an_independent_jiang_shu_build(eps):
+    """The oracle's WENO5 (WENO.jl:6-57) against the independent build above on non-smooth random
+    data, where the nonlinear weights depend on every coefficient of the smoothness indicators
+    (the 13/12 and 1/4 of WENO.jl:19-21) and on which ideal-weight set goes with which face."""
+    rng = np.random.default_rng(31)
+    for trial in range(300):
+        dx = float(rng.choice([1.0, 0.1, 1 / 64]))
+        kind = trial % 3
+        if kind == 0:
+            u = rng.uniform(-1, 1, 5)                           # rough
+        elif kind == 1:
+            u = np.where(np.arange(5) >= rng.integers(1, 5), 1.0, 0.0) + 0.01 * rng.standard_normal(5)  # a jump
+        else:
+            x = (np.arange(5) - 2) * dx
+            u = np.sin(3 * x + rng.uniform(0, 6)) + rng.uniform(-1, 1) * x ** 3   # smooth
+        ref = _jiang_shu_from_definitions(u, eps, dx)
+        got = orc.weno_f(u, eps, dx)
+        assert abs(got - ref) <= 1e-11 * np.abs(u).max() / dx, (trial, u, dx, got, ref)
+
+
+def test_weno_coefficient_mistakes_are_caught():
+    """The independent build separates the published indicator from plausible misprints: with
+    13/12 -> 1 or 1/4 -> 1/2 the nonlinear result moves far beyond the tolerance above."""
+    u = np.array([0.0, 0.1, 1.0, 1.05, 0.2])
+    dx, eps = 0.1, 1e-6
+    good = _jiang_shu_from_definitions(u, eps, dx)
+
+    def variant(c1, c2):
+        b1 = c1 * (u[2] - 2 * u[3] + u[4]) ** 2 + c2 * (3 * u[2] - 4 * u[3] + u[4]) ** 2
+        b2 = c1 * (u[1] - 2 * u[2] + u[3]) ** 2 + c2 * (u[1] - u[3]) ** 2
+        b3 = c1 * (u[0] - 2 * u[1] + u[2]) ** 2 + c2 * (u[0] - 4 * u[1] + 3 * u[2]) ** 2
+        b = np.array([b3, b2, b1])                       # stencils left, centre, right
+        hp = np.array([(2 * u[0] - 7 * u[1] + 11 * u[2]) / 6, (-u[1] + 5 * u[2] + 2 * u[3]) / 6,
+                       (2 * u[2] + 5 * u[3] - u[4]) / 6])
+        hm = np.array([(-u[0] + 5 * u[1] + 2 * u[2]) / 6, (2 * u[1] + 5 * u[2] - u[3]) / 6,
+                       (11 * u[2] - 7 * u[3] + 2 * u[4]) / 6])
+        wp = np.array([0.1, 0.6, 0.3]) / (eps + b) ** 2
+        wm = np.array([0.3, 0.6, 0.1]) / (eps + b) ** 2
+        return ((wp / wp.sum()) @ hp - (wm / wm.sum()) @ hm) / dx
+
+    assert abs(variant(13 / 12, 1 / 4) - good) < 1e-11 / dx
+    assert abs(variant(1.0, 1 / 4) - good) > 1e-6
+    assert abs(variant(13 / 12, 1 / 2) - good) > 1e-6
+
+
+# ---------------------------------------------------------------- SSPRK33: closed form on eigenvectors
+@pytest.mark.parametrize("mode,z", [(19, -2.3), (10, -1.2), (5, -0.35)])
+def test_ssprk33_closed_form(mode, z):
+    """Shu-Osher SSPRK(3,3) on a linear system advances an eigenvector by the stability polynomial
+    R(z) = 1 + z + z^2/2 + z^3/6 per step.  Dirichlet-0 heat on n nodes has eigenvectors
+    sin(k pi x_j) with eigenvalue -(4D/dx^2) sin^2(k pi dx/2); dt is chosen so that dt * lambda = z,
+    where a dropped or misplaced stage coefficient changes R by O(z^2) per step.  Every other mode
+    has |dt * lambda| below 2.5, inside both schemes' stability intervals, so rounding in the
+    other modes stays bounded."""
+    n, D = 21, 1.0
+    dx = 1.0 / (n - 1)
+    x = np.linspace(0, 1, n)
+    lam = -(4 * D / dx ** 2) * np.sin(mode * np.pi * dx / 2) ** 2
+    dt = z / lam
+    nsteps = 40
+    u0 = np.sin(mode * np.pi * x[1:-1])
+    got = orc.diffusion1d_integrate(n, D, [1, 0, 0], [1, 0, 0], u0, 0.0, dt, nsteps, scheme=1)
+    R = 1 + z + z ** 2 / 2 + z ** 3 / 6
+    np.testing.assert_allclose(got, R ** nsteps * u0, rtol=0, atol=1e-12 * max(abs(R) ** nsteps, 1e-3))
+    # and RK4 on the same data is the fourth-order polynomial: the two schemes differ by z^4/24
+    got4 = orc.diffusion1d_integrate(n, D, [1, 0, 0], [1, 0, 0], u0, 0.0, dt, nsteps, scheme=0)
+    R4 = R + z ** 4 / 24
+    np.testing.assert_allclose(got4, R4 ** nsteps * u0, rtol=0, atol=1e-12 * max(abs(R4) ** nsteps, 1e-3))
+
+
+# ---------------------------------------------------------------- nonuniform first-order upwind rows
+@pytest.mark.parametrize("c", [1.3, -0.8])
+def test_nonuniform_first_order_upwind_rows_brute_force(c):
+    """CompleteUpwindDifference on nodes (upwind_diff_weights.jl:116-127) stores the backward
+    operator's rows with offside 0, and _upwind_difference (upwind_difference.jl:154-158) reads
+    row II for node II.  At first order that row was computed at x[II+1] on x[II..II+1] and is
+    applied to u[II-1], u[II]: du = -c (u[II] - u[II-1]) / (x[II+1] - x[II]) for c > 0; the
+    forward operator (c <= 0) gives -c (u[II+1] - u[II]) / (x[II+1] - x[II]).  Written out node by
+    node on a random 7-node grid with Dirichlet ends."""
+    rng = np.random.default_rng(41)
+    x = np.sort(np.concatenate([[0.0, 1.0], rng.uniform(0.05, 0.95, 5)]))
+    n = len(x)
+    gl, gh = 0.35, -0.6
+    u = rng.uniform(-1, 1, n - 2)
+    full = np.concatenate([[gl], u, [gh]])
+    expected = np.empty(n - 2)
+    for i in range(1, n - 1):          # 0-based node i is the reference's II = i + 1
+        h = x[i + 1] - x[i]
+        if c > 0:
+            expected[i - 1] = -c * (full[i] - full[i - 1]) / h
+        else:
+            expected[i - 1] = -c * (full[i + 1] - full[i]) / h
+    got = orc.nonuniform1d_rhs(x, 0.0, c, 2, 1, (1, 0, gl), (1, 0, gh), u)
+    np.testing.assert_allclose(got, expected, rtol=1e-13, atol=1e-13)
