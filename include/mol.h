@@ -194,8 +194,12 @@ int mol_create(const mol_desc* d, mol_handle** out);
  * aux: device pointers of the naux full-grid arrays (host array of pointers, may be NULL). */
 int mol_rhs(mol_handle* h, void* du, const void* u, const void* p, double t, void* stream);
 
-/* Host-pointer convenience for the literal ODEFunction drop-in: copies in, runs, copies
- * out, synchronizes. */
+/* Host-pointer convenience for the literal ODEFunction drop-in (the integrator owns u and du as
+ * host arrays and calls f serially, src/MOL_discretization.jl:171-201): copies u (and p) in,
+ * runs, copies du out.  Uses device copies owned by the handle (allocated on the first call,
+ * released by mol_destroy) and a stream of the handle that is ordered after the legacy default
+ * stream; returns when du_h holds the result (only that stream is synchronized).  Calls on one
+ * handle must not overlap. */
 int mol_rhs_h(mol_handle* h, void* du_h, const void* u_h, const void* p_h, double t);
 
 /* Replaces: the reshaping of a solution into full-grid arrays per variable in
@@ -247,8 +251,15 @@ typedef enum { MOL_FLAG_KAHAN = 1, MOL_FLAG_GRAPH = 2, MOL_FLAG_UNFUSED = 4 } mo
 
 /* Replaces: solve(prob, RK4() | SSPRK33(); dt, adaptive = false) of OrdinaryDiffEq as used in
  * benchmark/weno/suite.jl:50-55 — the fixed-step stage loop, device-resident.  u is advanced in
- * place from t0 by nsteps steps of size dt.  MOL_FLAG_KAHAN: compensated (Kahan) accumulation of
- * the step update (FP32 state).  MOL_FLAG_GRAPH: stages captured once in a hipGraph and replayed.
+ * place from t0 by nsteps steps of size dt.  Asynchronous: everything is enqueued on `stream` and
+ * the call returns without waiting for the device (t0 travels as a kernel argument).
+ * MOL_FLAG_KAHAN: compensated (Kahan) accumulation of the step update (FP32 state).
+ * MOL_FLAG_GRAPH: chunks of 64, 16 and 4 steps are captured as hipGraphs on the first call for a
+ * given (u, p, dt, scheme, flags) and kept on the handle; later calls with the same key replay
+ * them (fewer than 4 leftover steps are launched directly).  The chunks are dropped by
+ * mol_set_aux, mol_set_tile and mol_destroy, and all of them when more than eight keys are in
+ * use (that drop waits for the device).  A 1D system held by one resident tile runs up to 1024
+ * steps per launch and is launched directly.
  * Periodic 1D/2D systems of reach 1 run RK4 (without Kahan) as one fused launch per step (the
  * four stages on a tile with a 4-node halo, same arithmetic as the stage launches);
  * MOL_FLAG_UNFUSED runs the four stage launches instead. */

@@ -52,18 +52,31 @@ int upload(mol_handle* h, void** dst, const T* src, size_t count) {
   return 0;
 }
 
+// Release the captured step chunks (after mol_set_aux / mol_set_tile they are stale).  A
+// replay may still be queued, so the device is drained first; this runs only when the
+// handle is reconfigured, destroyed, or more than eight step keys have been captured.
+void drop_graphs(mol_handle* h) {
+  if (h->graphs.empty()) return;
+  (void)hipDeviceSynchronize();
+  for (auto& c : h->graphs) {
+    if (c.exec) (void)hipGraphExecDestroy(c.exec);
+    if (c.graph) (void)hipGraphDestroy(c.graph);
+  }
+  h->graphs.clear();
+}
+
 void free_handle(mol_handle* h) {
   if (!h) return;
+  drop_graphs(h);
+  if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
+  if (h->rhs_stream) (void)hipStreamDestroy(h->rhs_stream);
   void* ptrs[] = {h->d_tabs, h->d_slots, h->d_clos, h->d_pool, h->d_code, h->d_cpool,
                   h->d_prog, h->d_coord[0], h->d_coord[1], h->d_coord[2], h->w_y1,
                   h->w_y2, h->w_acc, h->w_comp, h->d_tnow, h->d_ka64, h->d_ka32, h->d_tiles,
                   h->ck_u, h->ck_t, h->seg_u, h->seg_t, h->adj_y[0], h->adj_y[1], h->adj_y[2], h->adj_y[3],
-                  h->d_insts, h->d_wl, h->d_bdata, h->vjp_bf, h->vjp_ba};
+                  h->d_insts, h->d_wl, h->d_bdata, h->vjp_bf, h->vjp_ba, h->r_u, h->r_du, h->r_p};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
-  if (h->h_u) (void)hipHostFree(h->h_u);
-  if (h->h_du) (void)hipHostFree(h->h_du);
-  if (h->h_p) (void)hipHostFree(h->h_p);
   delete h;
 }
 
@@ -823,14 +836,26 @@ int sync_kargs(mol_handle* h) {
   return 0;
 }
 
+// Workspace of one integration: stage vectors (acc: RK4 stage launches; comp: Kahan) and the
+// double-buffered device clock.
+struct Work {
+  void* y1;
+  void* y2;
+  void* acc;
+  void* comp;
+  double* tnow;
+};
+
+Work handle_work(const mol_handle* h) { return Work{h->w_y1, h->w_y2, h->w_acc, h->w_comp, h->d_tnow}; }
+
 template <typename T>
-int enqueue_steps(mol_handle* h, T* u, const T* p, double dt, int64_t nsteps, int scheme,
+int enqueue_steps(mol_handle* h, const Work& w, T* u, const T* p, double dt, int64_t nsteps, int scheme,
                   bool kahan, hipStream_t s) {
   const Launch<T> L = launch_of<T>(h);
-  T* y1 = (T*)h->w_y1;
-  T* y2 = (T*)h->w_y2;
-  T* acc = (T*)h->w_acc;
-  T* comp = (T*)h->w_comp;
+  T* y1 = (T*)w.y1;
+  T* y2 = (T*)w.y2;
+  T* acc = (T*)w.acc;
+  T* comp = (T*)w.comp;
   if (scheme == MOL_RK4 && !kahan && h->fused_fn && !h->no_fuse) {
     // one launch per step (or per chunk of resident steps), the state alternating between u
     // and the workspace
@@ -839,8 +864,8 @@ int enqueue_steps(mol_handle* h, T* u, const T* p, double dt, int64_t nsteps, in
     for (int64_t step = 0; step < nsteps;) {
       const int nsub = h->fused_resident ? (int)std::min<int64_t>(nsteps - step, 1024) : 1;
       Epi<T> e{};
-      e.tnow = h->d_tnow + h->clk;
-      e.tnext = h->d_tnow + (1 - h->clk);
+      e.tnow = w.tnow + h->clk;
+      e.tnext = w.tnow + (1 - h->clk);
       e.tadv = dt;
       e.p = p;
       JitPtrs<T> jp = L.jp;
@@ -858,8 +883,8 @@ int enqueue_steps(mol_handle* h, T* u, const T* p, double dt, int64_t nsteps, in
     return 0;
   }
   for (int64_t step = 0; step < nsteps; ++step) {
-    double* tcur = h->d_tnow + h->clk;
-    double* tnxt = h->d_tnow + (1 - h->clk);
+    double* tcur = w.tnow + h->clk;
+    double* tnxt = w.tnow + (1 - h->clk);
     Epi<T> e{};
     e.tnow = tcur;
     e.p = p;
@@ -896,6 +921,34 @@ int enqueue_steps(mol_handle* h, T* u, const T* p, double dt, int64_t nsteps, in
   return 0;
 }
 
+// Step counts of the captured chunks, largest first.  Each is even: a chunk flips the device
+// clock and the fused step's state buffers an even number of times, so it starts and ends at
+// clock slot 0 with the state in u, and any sequence of chunks can be replayed.
+constexpr int64_t kGraphChunks[3] = {64, 16, 4};
+
+template <typename T>
+int capture_chunk(mol_handle* h, T* u, const T* p, double dt, int64_t steps, int scheme, bool kahan,
+                  mol_handle::GraphChunk& c) {
+  if (!h->cap_stream) HIPCHK(h, hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
+  const int clk0 = h->clk;
+  hipGraph_t g = nullptr;
+  HIPCHK(h, hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_steps<T>(h, handle_work(h), u, p, dt, steps, scheme, kahan, h->cap_stream);
+  const hipError_t ee = hipStreamEndCapture(h->cap_stream, &g);
+  h->clk = clk0;
+  if (rc || ee != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc ? rc : fail(h, MOL_ERR_HIP, "graph capture failed");
+  }
+  hipGraphExec_t ge = nullptr;
+  if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return fail(h, MOL_ERR_HIP, "graph instantiation failed");
+  }
+  c = mol_handle::GraphChunk{u, p, dt, scheme, kahan ? 1 : 0, h->no_fuse ? 1 : 0, steps, g, ge};
+  return 0;
+}
+
 template <typename T>
 int integrate_impl(mol_handle* h, T* u, const T* p, double t0, double dt, int64_t nsteps,
                    int scheme, int flags, hipStream_t s) {
@@ -903,35 +956,34 @@ int integrate_impl(mol_handle* h, T* u, const T* p, double t0, double dt, int64_
   if (rc) return rc;
   const bool kahan = (flags & MOL_FLAG_KAHAN) != 0;
   h->no_fuse = (flags & MOL_FLAG_UNFUSED) != 0;
-  HIPCHK(h, hipMemcpyAsync(h->d_tnow + h->clk, &t0, sizeof(double), hipMemcpyHostToDevice, s));
-  if (kahan) HIPCHK(h, hipMemsetAsync(h->w_comp, 0, (size_t)h->nunk * h->esize(), s));
-  if (!(flags & MOL_FLAG_GRAPH) || nsteps < 4) {
-    return enqueue_steps<T>(h, u, p, dt, nsteps, scheme, kahan, s);
+  h->clk = 0;
+  if (launch_set_clock(h->d_tnow, t0, s)) return fail(h, MOL_ERR_HIP, "clock launch failed");
+  if (kahan && launch_fill_zero<T>((T*)h->w_comp, h->nunk, s)) return fail(h, MOL_ERR_HIP, "zero fill launch failed");
+  // a resident fused kernel already runs up to 1024 steps per launch and flips the clock once
+  // per launch, whatever the step count: it is launched directly
+  const bool resident = scheme == MOL_RK4 && !kahan && h->fused_fn && !h->no_fuse && h->fused_resident;
+  if (!(flags & MOL_FLAG_GRAPH) || resident || nsteps < kGraphChunks[2])
+    return enqueue_steps<T>(h, handle_work(h), u, p, dt, nsteps, scheme, kahan, s);
+  hipGraphExec_t ex[3] = {nullptr, nullptr, nullptr};
+  for (const auto& c : h->graphs)
+    if (c.u == u && c.p == p && c.dt == dt && c.scheme == scheme && c.kahan == (kahan ? 1 : 0) &&
+        c.nofuse == (h->no_fuse ? 1 : 0))
+      for (int k = 0; k < 3; ++k)
+        if (c.steps == kGraphChunks[k]) ex[k] = c.exec;
+  if (!ex[0] || !ex[1] || !ex[2]) {
+    if (h->graphs.size() >= 24) drop_graphs(h);
+    for (int k = 0; k < 3; ++k) {
+      if (ex[k]) continue;
+      mol_handle::GraphChunk c{};
+      if ((rc = capture_chunk<T>(h, u, p, dt, kGraphChunks[k], scheme, kahan, c))) return rc;
+      h->graphs.push_back(c);
+      ex[k] = c.exec;
+    }
   }
-  // capture a block of steps once, replay it; remainder launched directly
-  const int64_t chunk = std::min<int64_t>(nsteps, 64);
-  hipStream_t cs;
-  HIPCHK(h, hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-  hipGraph_t g = nullptr;
-  hipGraphExec_t ge = nullptr;
-  HIPCHK(h, hipStreamSynchronize(s));
-  HIPCHK(h, hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-  rc = enqueue_steps<T>(h, u, p, dt, chunk, scheme, kahan, cs);
-  hipError_t ee = hipStreamEndCapture(cs, &g);
-  if (rc || ee != hipSuccess) {
-    if (g) (void)hipGraphDestroy(g);
-    (void)hipStreamDestroy(cs);
-    return rc ? rc : fail(h, MOL_ERR_HIP, "graph capture failed");
-  }
-  HIPCHK(h, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-  const int64_t reps = nsteps / chunk;
-  for (int64_t r = 0; r < reps; ++r) HIPCHK(h, hipGraphLaunch(ge, s));
-  rc = enqueue_steps<T>(h, u, p, dt, nsteps - reps * chunk, scheme, kahan, s);
-  HIPCHK(h, hipStreamSynchronize(s));
-  (void)hipGraphExecDestroy(ge);
-  (void)hipGraphDestroy(g);
-  (void)hipStreamDestroy(cs);
-  return rc;
+  int64_t left = nsteps;
+  for (int k = 0; k < 3; ++k)
+    for (; left >= kGraphChunks[k]; left -= kGraphChunks[k]) HIPCHK(h, hipGraphLaunch(ex[k], s));
+  return enqueue_steps<T>(h, handle_work(h), u, p, dt, left, scheme, kahan, s);
 }
 
 // Does program `prog` read opcode `op` (e.g. an auxiliary array)?
@@ -1118,7 +1170,7 @@ int forward_ckpt_impl(mol_handle* h, T* u, const T* p, double t0, double dt, int
     const int64_t n = h->nunk;
     T* S = (T*)h->seg_u;
     auto st = [&](int64_t j, int k) { return S + (size_t)(4 * j + k) * n; };
-    HIPCHK(h, hipMemcpyAsync(h->d_tnow + h->clk, &t0, sizeof(double), hipMemcpyHostToDevice, s));
+    if (launch_set_clock(h->d_tnow + h->clk, t0, s)) return fail(h, MOL_ERR_HIP, "clock launch failed");
     HIPCHK(h, hipMemcpyAsync(st(0, 0), u, vbytes, hipMemcpyDeviceToDevice, s));
     for (int64_t j = 0; j < nsteps; ++j) {
       HIPCHK(h, hipMemcpyAsync(h->seg_t + j, h->d_tnow + h->clk, sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -1144,12 +1196,12 @@ int forward_ckpt_impl(mol_handle* h, T* u, const T* p, double t0, double dt, int
     h->ck_cap = (size_t)nseg;
   }
   h->ck_valid = false;
-  HIPCHK(h, hipMemcpyAsync(h->d_tnow + h->clk, &t0, sizeof(double), hipMemcpyHostToDevice, s));
+  if (launch_set_clock(h->d_tnow + h->clk, t0, s)) return fail(h, MOL_ERR_HIP, "clock launch failed");
   for (int64_t g = 0; g < nseg; ++g) {
     const int64_t len = std::min<int64_t>(every, nsteps - g * every);
     HIPCHK(h, hipMemcpyAsync((char*)h->ck_u + g * vbytes, u, vbytes, hipMemcpyDeviceToDevice, s));
     HIPCHK(h, hipMemcpyAsync(h->ck_t + g, h->d_tnow + h->clk, sizeof(double), hipMemcpyDeviceToDevice, s));
-    rc = enqueue_steps<T>(h, u, p, dt, len, MOL_RK4, false, s);
+    rc = enqueue_steps<T>(h, handle_work(h), u, p, dt, len, MOL_RK4, false, s);
     if (rc) return rc;
   }
   h->ck_nsteps = nsteps;
@@ -1536,6 +1588,7 @@ int mol_set_tile(mol_handle* h, int32_t tx, int32_t ty, int32_t tz) {
   if (h->batch) return fail(h, MOL_ERR_UNSUPPORTED, "batched handles take their tile at creation");
   if (tx < 0 || ty < 0 || tz < 0 || (int64_t)std::max(tx, 1) * std::max(ty, 1) * std::max(tz, 1) > 4096)
     return fail(h, MOL_ERR_INVALID, "tile shape out of range (at most 4096 nodes)");
+  drop_graphs(h);
   h->tile_req[0] = tx;
   h->tile_req[1] = ty;
   h->tile_req[2] = tz;
@@ -1576,6 +1629,7 @@ int mol_get_plan(const mol_handle* h, int32_t* info, int32_t ninfo) {
 int mol_set_aux(mol_handle* h, void* const* aux) {
   if (!h) return fail(nullptr, MOL_ERR_INVALID, "handle is NULL");
   if (h->naux > 0 && !aux) return fail(h, MOL_ERR_INVALID, "aux pointers missing");
+  drop_graphs(h);
   for (int a = 0; a < h->naux; ++a) {
     if (!aux[a]) return fail(h, MOL_ERR_INVALID, "aux pointer is NULL");
     h->ka64.aux[a] = (const double*)aux[a];
@@ -1633,23 +1687,22 @@ int mol_expand(mol_handle* h, void* full, const void* u, const void* p, double t
 int mol_rhs_h(mol_handle* h, void* du_h, const void* u_h, const void* p_h, double t) {
   if (!h) return fail(nullptr, MOL_ERR_INVALID, "handle is NULL");
   if (!du_h || !u_h) return fail(h, MOL_ERR_INVALID, "du/u is NULL");
+  if (h->nparams > 0 && !p_h) return fail(h, MOL_ERR_INVALID, "parameters missing");
   const size_t bytes = (size_t)h->nunk * h->esize();
-  void *du = nullptr, *u = nullptr, *p = nullptr;
-  HIPCHK(h, hipMalloc(&du, bytes));
-  HIPCHK(h, hipMalloc(&u, bytes));
-  if (h->nparams > 0) {
-    HIPCHK(h, hipMalloc(&p, h->nparams * h->esize()));
-    HIPCHK(h, hipMemcpy(p, p_h, h->nparams * h->esize(), hipMemcpyHostToDevice));
-  }
-  HIPCHK(h, hipMemcpy(u, u_h, bytes, hipMemcpyHostToDevice));
-  int rc = mol_rhs(h, du, u, p, t, nullptr);
-  if (rc == 0) {
-    HIPCHK(h, hipDeviceSynchronize());
-    HIPCHK(h, hipMemcpy(du_h, du, bytes, hipMemcpyDeviceToHost));
-  }
-  (void)hipFree(du);
-  (void)hipFree(u);
-  if (p) (void)hipFree(p);
+  const size_t pbytes = (size_t)h->nparams * h->esize();
+  // device copies owned by the handle (kept for later calls, released by mol_destroy) and a
+  // stream of its own, ordered after the legacy default stream like the caller's work
+  if (!h->rhs_stream) HIPCHK(h, hipStreamCreate(&h->rhs_stream));
+  if (!h->r_u) HIPCHK(h, hipMalloc(&h->r_u, bytes));
+  if (!h->r_du) HIPCHK(h, hipMalloc(&h->r_du, bytes));
+  if (pbytes && !h->r_p) HIPCHK(h, hipMalloc(&h->r_p, pbytes));
+  hipStream_t s = h->rhs_stream;
+  HIPCHK(h, hipMemcpyAsync(h->r_u, u_h, bytes, hipMemcpyHostToDevice, s));
+  if (pbytes) HIPCHK(h, hipMemcpyAsync(h->r_p, p_h, pbytes, hipMemcpyHostToDevice, s));
+  const int rc = mol_rhs(h, h->r_du, h->r_u, pbytes ? h->r_p : nullptr, t, s);
+  if (rc == 0) HIPCHK(h, hipMemcpyAsync(du_h, h->r_du, bytes, hipMemcpyDeviceToHost, s));
+  // the host arrays are the caller's: every copy has finished when the call returns
+  HIPCHK(h, hipStreamSynchronize(s));
   return rc;
 }
 

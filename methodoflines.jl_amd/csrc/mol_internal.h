@@ -124,6 +124,8 @@ int launch_fill_zero(T* p, int64_t n, hipStream_t stream);
 // lam += y0 + y1 + y2 + y3 (the adjoint of one RK4 step), then y0..y3 = 0 when `zero`
 template <typename T>
 int launch_adjoint_sum(T* lam, T* y0, T* y1, T* y2, T* y3, int64_t n, bool zero, hipStream_t stream);
+// *slot = t on `stream` (t passed by value in the launch)
+int launch_set_clock(double* slot, double t, hipStream_t stream);
 
 // Set the message of the calling thread's last failure (used when no handle exists yet).
 void set_global_error(const std::string& msg);
@@ -244,10 +246,26 @@ struct mol_handle {
   void* w_comp = nullptr;
   double* d_tnow = nullptr;                // double-buffered device clock: d_tnow[clk] is current
   int clk = 0;
-  // rhs_h staging
-  void* h_u = nullptr;
-  void* h_du = nullptr;
-  void* h_p = nullptr;
+  // Instantiated step chunks of mol_integrate (MOL_FLAG_GRAPH), captured once per key and
+  // replayed: the kernels of a chunk take their pointers and dt by value, so a chunk is
+  // valid only for the state, parameters and step it was captured with.  Every chunk is an
+  // even number of clock flips and starts at clock slot 0.
+  struct GraphChunk {
+    const void* u;
+    const void* p;
+    double dt;
+    int scheme, kahan, nofuse;
+    int64_t steps;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphChunk> graphs;
+  hipStream_t cap_stream = nullptr;       // capture-only stream (nothing executes on it)
+  // mol_rhs_h: device copies of u, du and p, allocated on the first call
+  void* r_u = nullptr;
+  void* r_du = nullptr;
+  void* r_p = nullptr;
+  hipStream_t rhs_stream = nullptr;
   std::string err;
   size_t esize() const { return dtype == MOL_F64 ? sizeof(double) : sizeof(float); }
 };

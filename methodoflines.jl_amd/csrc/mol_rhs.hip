@@ -279,6 +279,15 @@ int launch_adjoint_sum(T* lam, T* y0, T* y1, T* y2, T* y3, int64_t n, bool zero,
   return hipGetLastError() == hipSuccess ? 0 : MOL_ERR_HIP;
 }
 
+// The device clock slot takes the start time from the kernel argument: the value travels in
+// the launch packet, so no host memory has to outlive the call.
+__global__ void set_clock_kernel(double* __restrict__ slot, double t) { slot[0] = t; }
+
+int launch_set_clock(double* slot, double t, hipStream_t stream) {
+  hipLaunchKernelGGL(set_clock_kernel, dim3(1), dim3(64), 0, stream, slot, t);
+  return hipGetLastError() == hipSuccess ? 0 : MOL_ERR_HIP;
+}
+
 template int launch_fill_zero<double>(double*, int64_t, hipStream_t);
 template int launch_fill_zero<float>(float*, int64_t, hipStream_t);
 template int launch_adjoint_sum<double>(double*, double*, double*, double*, double*, int64_t, bool, hipStream_t);
