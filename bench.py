@@ -49,6 +49,21 @@ STAGE_TRAFFIC_BYTES = (3960.27 * 2.0 + 5869.53 * 1.0) * 1024
 FUSED_TRAFFIC_BYTES = (2375.6 * 2.0 + 4096.0 * 1.0) * 1024
 
 
+def host_cpu():
+    """Model name and logical CPU count of the host (from /proc/cpuinfo)."""
+    model, n = None, 0
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    n += 1
+                    model = model or line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": n or os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
 def events():
     import torch
     return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -305,6 +320,72 @@ def bench_adjoint(n, nsteps, every, fd_cells):
             "fd_cells": int(fd_cells)}
 
 
+def bench_adjoint_fp32(n, nsteps, every):
+    """Config 5 in FP32: state, stencil arithmetic and stage adjoints in FP32, the grid-field
+    gradient reduced in FP64.  Plain forward, and per checkpoint mode the recording forward and
+    the adjoint sweep; the final state and the d0 gradient against the FP64 run of the same
+    (FP32-representable) inputs."""
+    import numpy as np
+    import torch
+    import molgpu
+    from molgpu import systems
+    r32 = lambda a: np.asarray(a, dtype=np.float32).astype(np.float64)
+    d0 = r32(0.5 + 0.5 * np.abs(systems.gaussian_field3d(n, 1)))
+    src = r32(systems.gaussian_field3d(n, 2))
+    probs = {}
+    for name in ("f64", "f32"):
+        s, d = systems.nonlindiff3d(n, dtype=name)
+        probs[name] = molgpu.discretize(s, d)
+        probs[name].set_grid_field("d0", d0)
+        probs[name].set_grid_field("s", src)
+    p64, p32 = probs["f64"], probs["f32"]
+    (l0, l1, l2), (h0, h1, h2) = p64.lowered.lo[0], p64.lowered.hi[0]
+    box = lambda f: f[l0 - 1:h0, l1 - 1:h1, l2 - 1:h2].reshape(-1, order="F")
+    u0 = r32(0.3 * box(systems.gaussian_field3d(n, 5)))
+    tgt = r32(0.2 * box(systems.gaussian_field3d(n, 6)))
+    dx = 1.0 / (n - 1)
+    dt = 0.5 * 2.78 * dx * dx / (12.0 * 2.0 * d0.max())
+    ng = n ** 3
+    ia = p64.lowered.aux.index("d0")
+
+    def adjoint(p, dtype, ck):
+        u = torch.tensor(u0, dtype=dtype, device="cuda")
+        g = torch.zeros(2 * ng, dtype=torch.float64, device="cuda")
+        e0, e1 = events()
+        e2 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        p.forward_checkpointed(u, 0.0, dt, nsteps, ck)
+        e1.record()
+        lam = (u - torch.tensor(tgt, dtype=dtype, device="cuda")).contiguous()
+        p.adjoint(lam, g)
+        e2.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1), e1.elapsed_time(e2), u.double(), g[ia * ng:(ia + 1) * ng]
+
+    _, _, u64, g64 = adjoint(p64, torch.float64, every)
+    u = torch.tensor(u0, dtype=torch.float32, device="cuda")
+    p32.integrate(u, 0.0, dt, nsteps, molgpu.RK4, graph=True)
+    u = torch.tensor(u0, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    e0, e1 = events()
+    e0.record()
+    p32.integrate(u, 0.0, dt, nsteps, molgpu.RK4, graph=True)
+    e1.record()
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1)
+    modes = {}
+    g32 = None
+    for ck in sorted({every, p32.checkpoint_interval(nsteps)}, reverse=True):
+        adjoint(p32, torch.float32, ck)
+        fc_ms, a_ms, u32, g32 = adjoint(p32, torch.float32, ck)
+        modes["ckpt_every=%d" % ck] = {"forward_ckpt_ms": fc_ms, "adjoint_ms": a_ms, "ratio": a_ms / fwd_ms,
+                                       "forward_plus_adjoint_ratio": (fc_ms + a_ms) / fwd_ms}
+    return {"grid": "%d^3" % n, "steps": nsteps, "dt": dt, "forward_ms": fwd_ms, "modes": modes,
+            "state_max_abs_err_vs_fp64": float((u32 - u64).abs().max()), "state_max_abs": float(u64.abs().max()),
+            "grad_d0_rel_err_vs_fp64": float((g32 - g64).norm() / g64.norm()),
+            "grad_d0_max_err_vs_max": float((g32 - g64).abs().max() / g64.abs().max())}
+
+
 def bench_heat(n=101, D=1.0, dt=1e-5, t_end=0.1):
     """Config 0: 1D heat u_t = D u_xx, Dirichlet 0, u0 = sin(pi x), RK4 at dt to t_end: time per
     step, error against the analytic solution exp(-pi^2 D t) sin(pi x) and against the oracle's RK4
@@ -411,7 +492,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only (no weno/adjoint/batch sections)")
     ap.add_argument("--section", default="all",
-                    choices=["all", "weno", "wenosolve", "adjoint", "batch", "rhs3d", "copy", "heat"],
+                    choices=["all", "weno", "wenosolve", "adjoint", "adjoint32", "batch", "rhs3d", "copy", "heat"],
                     help="run one section alone (for profiling); prints its JSON")
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--reps", type=int, default=200, help="back-to-back RHS launches of the weno/rhs3d sections")
@@ -432,6 +513,8 @@ def main():
             res = bench_weno_solve(1024, 500, 3)
         elif args.section == "adjoint":
             res = bench_adjoint(128, args.adj_steps, 20, 2)
+        elif args.section == "adjoint32":
+            res = bench_adjoint_fp32(128, args.adj_steps, 20)
         elif args.section == "batch":
             res = bench_batch(512, 50, quick=True)
         elif args.section == "heat":
@@ -530,6 +613,7 @@ def main():
         out["weno5"] = {"1024": bench_weno(1024, 200), "4096": bench_weno(4096, 20),
                         "solve_1024": bench_weno_solve(1024, 500, 3)}
         out["adjoint"] = bench_adjoint(128, 200, 20, 32)
+        out["adjoint"]["fp32"] = bench_adjoint_fp32(128, 200, 20)
         out["heat"] = bench_heat()
         out["batch"] = bench_batch(512, 200, quick=False)
 
@@ -537,17 +621,25 @@ def main():
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as orc
-        threads = 1
         ucpu = prob.u0.copy()
-        reps = 0
-        c0 = time.perf_counter()
-        while time.perf_counter() - c0 < 10.0:      # bounded sample: ~10 s of CPU work
-            orc.bruss2d_rhs(n, ucpu, 1.2, threads=threads)
-            reps += 1
-        csec = time.perf_counter() - c0
-        cpu = {"value": points * reps / csec / 1e9, "unit": "Gpoint-RHS/s", "cores": threads, "kind": "port",
+
+        def sample(threads, secs):
+            reps = 0
+            c0 = time.perf_counter()
+            while time.perf_counter() - c0 < secs:      # bounded sample of CPU work
+                orc.bruss2d_rhs(n, ucpu, 1.2, threads=threads)
+                reps += 1
+            return reps, points * reps / (time.perf_counter() - c0) / 1e9
+
+        # the host share of one GPU: at most 16 threads, whatever the machine reports
+        allc = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
+        reps1, v1 = sample(1, 8.0)
+        repsn, vn = sample(allc, 8.0)
+        cpu = {"value": vn, "unit": "Gpoint-RHS/s", "cores": allc, "kind": "port",
                "sample": "%d right-hand sides of the same 512^2 Brusselator in the C++ oracle (CPU restatement, "
-                         "not MethodOfLines.jl), FP64, 1 thread" % reps}
+                         "not MethodOfLines.jl), FP64, OpenMP over %d threads; %d more on 1 thread"
+                         % (repsn, allc, reps1),
+               "one_thread": v1, "host": host_cpu()}
     out["cpu_baseline"] = cpu
     print(json.dumps(out))
 

@@ -302,3 +302,89 @@ def test_adjoint_with_a_variable_read_only_at_its_node():
     d2 = (loss(5e-5) - loss(-5e-5)) / 1e-4
     fd = (4 * d2 - d1) / 3
     assert abs(grad @ w - fd) <= 1e-7 * abs(fd) + 1e-10 * np.linalg.norm(grad) * np.linalg.norm(w), (grad @ w, fd)
+
+
+# ---------------------------------------------------------------- config 5 in FP32
+def _nonlin32(n, seed=1):
+    """The 3D nonlinear diffusion in FP32 with FP32-representable grid fields (so that the FP64
+    oracle sees the same inputs and the difference is FP32 arithmetic only)."""
+    sys_, disc = systems.nonlindiff3d(n=n, dtype="f32")
+    prob = molgpu.discretize(sys_, disc)
+    d0 = (0.5 + 0.5 * np.abs(systems.gaussian_field3d(n, seed))).astype(np.float32).astype(np.float64)
+    src = systems.gaussian_field3d(n, seed + 1).astype(np.float32).astype(np.float64)
+    prob.set_grid_field("d0", d0)
+    prob.set_grid_field("s", src)
+    return prob, d0.reshape(-1, order="F"), src.reshape(-1, order="F")
+
+
+def test_fp32_nonlinear_diffusion_rhs_and_rk4_vs_fp64_oracle():
+    """FP32 state and arithmetic at 34^3: one RHS and 20 RK4 steps against the FP64 oracle, with
+    the FP32 error stated against the magnitude of the summed terms."""
+    torch = _torch()
+    n = 34
+    prob, d0, src = _nonlin32(n)
+    N = prob.lowered.n_unknowns
+    u = np.random.default_rng(2).uniform(-1, 1, N).astype(np.float32).astype(np.float64)
+    ug = _dev(u, torch.float32)
+    du = torch.empty_like(ug)
+    print("  fp32 nonlin rhs n=%d" % n, flush=True)
+    prob.f(du, ug, None, 0.0)
+    torch.cuda.synchronize()
+    got = du.cpu().numpy().astype(np.float64)
+    ref = orc.nonlindiff3d_rhs(n, d0, src, u, threads=8)
+    dx = 1.0 / (n - 1)
+    # each of the 3 axes sums 2 half-node fluxes of size a * |du| / dx^2, a <= d0 (1 + u^2)
+    scale = 3 * 2 * 2 * np.max(d0) * 2.0 * np.max(np.abs(u)) / dx ** 2 + np.max(np.abs(src))
+    err = np.max(np.abs(got - ref))
+    print("  fp32 rhs max error %.3e (scale %.3e)" % (err, scale))
+    assert err <= 1e-6 * scale
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-5
+    # 20 RK4 steps against RK4 over the FP64 oracle
+    dt = 0.1 * dx * dx / (2.0 * np.max(d0))
+    steps = 20
+    print("  fp32 nonlin rk4 %d steps" % steps, flush=True)
+    prob.integrate(ug, 0.0, dt, steps, molgpu.RK4, graph=True)
+    w = u.copy()
+    F = lambda x, t: orc.nonlindiff3d_rhs(n, d0, src, x, t=t, threads=8)
+    for k in range(steps):
+        t = k * dt
+        k1 = F(w, t); k2 = F(w + dt / 2 * k1, t + dt / 2); k3 = F(w + dt / 2 * k2, t + dt / 2)
+        k4 = F(w + dt * k3, t + dt)
+        w = w + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+    torch.cuda.synchronize()
+    gotT = ug.cpu().numpy().astype(np.float64)
+    errT = np.max(np.abs(gotT - w))
+    print("  fp32 rk4 max error %.3e (max |u| %.3e)" % (errT, np.max(np.abs(w))))
+    assert errT <= 2e-6 * (1 + np.max(np.abs(w)))
+
+
+def test_fp32_adjoint_d0_gradient_vs_fp64():
+    """FP32 forward and reverse sweep with FP64 gradient reduction: the d0 gradient of
+    L = 1/2 |u(T) - target|^2 against the FP64 adjoint and the oracle's forward mode."""
+    torch = _torch()
+    n, nsteps, every = 16, 24, 5
+    prob32, d0, src = _nonlin32(n, seed=3)
+    prob64 = molgpu.discretize(*systems.nonlindiff3d(n=n))
+    prob64.set_grid_field("d0", d0.reshape((n, n, n), order="F"))
+    prob64.set_grid_field("s", src.reshape((n, n, n), order="F"))
+    u0 = (0.3 * _on_box(prob64, systems.gaussian_field3d(n, 5))).astype(np.float32).astype(np.float64)
+    target = (0.2 * _on_box(prob64, systems.gaussian_field3d(n, 6))).astype(np.float32).astype(np.float64)
+    dx = 1.0 / (n - 1)
+    dt = 0.1 * dx * dx / (2.0 * np.max(d0))
+    grads = {}
+    for name, prob, dtype in (("f64", prob64, torch.float64), ("f32", prob32, torch.float32)):
+        print("  %s adjoint n=%d" % (name, n), flush=True)
+        u = _dev(u0, dtype)
+        prob.forward_checkpointed(u, 0.0, dt, nsteps, every)
+        lam = (u - _dev(target, dtype)).contiguous()
+        g = torch.zeros(2 * n ** 3, dtype=torch.float64, device="cuda")
+        prob.adjoint(lam, g)
+        grads[name] = _grad_of(prob, g, "d0")
+    rel = np.linalg.norm(grads["f32"] - grads["f64"]) / np.linalg.norm(grads["f64"])
+    vdir = np.random.default_rng(11).standard_normal(n ** 3)
+    _, dL = orc.nonlindiff3d_jvp(n, d0, vdir, src, u0, target, dt, nsteps)
+    dir_err = abs(grads["f32"] @ vdir - dL) / abs(dL)
+    print("  fp32 d0 gradient: norm-wise error vs fp64 %.3e, directional vs oracle %.3e" % (rel, dir_err))
+    assert abs(grads["f64"] @ vdir - dL) <= 1e-8 * abs(dL)
+    assert rel <= 1e-4
+    assert dir_err <= 1e-4
