@@ -20,6 +20,7 @@ struct MolDim3 {
 static MolDim3 threadIdx, blockIdx, gridDim, blockDim;
 inline void __syncthreads() {}
 #define __builtin_amdgcn_rcpf(x) (1.0f / (x))
+#define __builtin_amdgcn_rcp(x) (1.0 / (x))
 template <class T>
 inline T atomicAdd(T* p, T v) {
   const T o = *p;
