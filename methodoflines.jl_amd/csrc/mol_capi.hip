@@ -82,23 +82,6 @@ void free_handle(mol_handle* h) {
 
 }  // namespace
 
-namespace mol {
-// Maximum stack depth of a program (checked to follow the stack discipline at creation).
-int program_depth(const mol_desc* d, int prog) {
-  const int st = d->prog_h[2 * prog], ln = d->prog_h[2 * prog + 1];
-  int depth = 0, mx = 0;
-  for (int pc = st; pc < st + ln; ++pc) {
-    const uint32_t ins = d->code_h[pc];
-    const int op = ins & 0xff;
-    const bool bc = (ins >> 20) & 1u;
-    if (op < 16) depth += 1;
-    else if (op == MOL_OP_SELECT) depth -= bc ? 1 : 2;
-    else if (op < 48 && !bc) depth -= 1;
-    mx = std::max(mx, depth);
-  }
-  return mx;
-}
-}  // namespace mol
 
 namespace {
 
@@ -361,20 +344,8 @@ template <>
 KCore<float>& kcore<float>(mol_handle* h) { return h->kc32; }
 
 template <typename T>
-KCore<T>& kcorew(mol_handle* h);
-template <>
-KCore<double>& kcorew<double>(mol_handle* h) { return h->kcw64; }
-template <>
-KCore<float>& kcorew<float>(mol_handle* h) { return h->kcw32; }
-
-template <typename T>
 Launch<T> launch_of(mol_handle* h) {
   Launch<T> L;
-  if (h->use_weno) {
-    L.kcw = &kcorew<T>(h);
-    L.plan = &h->wplan;
-    L.nweno = h->nweno;
-  }
   L.ka = &kargs<T>(h);
   L.dka = dkargs<T>(h);
   L.kc = &kcore<T>(h);
@@ -727,7 +698,6 @@ int setup_paths(mol_handle* h, const mol_desc* d) {
   int n64 = 0, n32 = 0;
   build_core<double>(h, d, h->ka64, h->kc64, t64, n64);
   build_core<float>(h, d, h->ka32, h->kc32, t32, n32);
-  h->use_weno = false;
   h->jit_fn = nullptr;
   h->jit_reason.clear();
   bool retiled = false;
@@ -780,19 +750,6 @@ int setup_paths(mol_handle* h, const mol_desc* d) {
     t64.clear();
     for (int i = 0; i < ka.ntile[0] * ka.ntile[1] * ka.ntile[2]; ++i) t64.push_back(i);
     n64 = 0;
-  }
-  const char* wf = std::getenv("MOL_WENO2D");
-  const bool want_weno = h->jit_fn == nullptr || (wf && wf[0] == '1');
-  if (want_weno && n64 > 0 && t64.empty() && h->core_interp_ok && weno_plan(d, h->wplan)) {
-    // every tile is a core tile: retile 64 x 32 for the WENO5 kernel
-    h->kcw64 = h->kc64;
-    h->kcw32 = h->kc32;
-    const int ntx = (h->kc64.hi[0] - h->kc64.lo[0] + WENO_TILE_X) / WENO_TILE_X;
-    const int nty = (h->kc64.hi[1] - h->kc64.lo[1] + WENO_TILE_Y) / WENO_TILE_Y;
-    for (KCore<double>* k : {&h->kcw64}) { k->ct_lo[0] = k->ct_lo[1] = k->ct_lo[2] = 0; k->ct_n[0] = ntx; k->ct_n[1] = nty; k->ct_n[2] = 1; }
-    for (KCore<float>* k : {&h->kcw32}) { k->ct_lo[0] = k->ct_lo[1] = k->ct_lo[2] = 0; k->ct_n[0] = ntx; k->ct_n[1] = nty; k->ct_n[2] = 1; }
-    h->nweno = ntx * nty;
-    h->use_weno = std::getenv("MOL_DISABLE_WENO_FAST") == nullptr;
   }
   h->ncore = n64;
   h->nbound = (int)t64.size();
@@ -1605,8 +1562,6 @@ int mol_set_tile(mol_handle* h, int32_t tx, int32_t ty, int32_t tz) {
   for (int a = 0; a < MOL_MAX_AUX; ++a) {
     h->kc64.aux[a] = (const double*)aux64[a];
     h->kc32.aux[a] = (const float*)aux64[a];
-    h->kcw64.aux[a] = (const double*)aux64[a];
-    h->kcw32.aux[a] = (const float*)aux64[a];
   }
   return 0;
 }
@@ -1620,7 +1575,7 @@ int mol_get_plan(const mol_handle* h, int32_t* info, int32_t ninfo) {
     return 0;
   }
   int32_t v[9] = {h->ka64.tile[0], h->ka64.tile[1], h->ka64.tile[2], h->ncore, h->nbound,
-                  h->use_weno ? h->nweno : 0, h->jit_fn != nullptr && !h->use_weno ? 1 : 0,
+                  0, h->jit_fn != nullptr ? 1 : 0,
                   h->core_interp_ok ? 1 : 0, h->fused_fn ? h->fused_tiles : 0};
   for (int i = 0; i < ninfo && i < 9; ++i) info[i] = v[i];
   return 0;
@@ -1636,8 +1591,6 @@ int mol_set_aux(mol_handle* h, void* const* aux) {
     h->ka32.aux[a] = (const float*)aux[a];
     h->kc64.aux[a] = (const double*)aux[a];
     h->kc32.aux[a] = (const float*)aux[a];
-    h->kcw64.aux[a] = (const double*)aux[a];
-    h->kcw32.aux[a] = (const float*)aux[a];
   }
   return sync_kargs(h);
 }

@@ -9,13 +9,6 @@
 
 namespace mol {
 
-// WENO5 fast-path plan: the WENO slots along x and y and their variables.
-struct WenoPlan {
-  int nx, ny;
-  int sx[2], sy[2];
-  int fx[2], fy[2];
-};
-
 // Geometry of the core-tile region handed to the code generator.
 struct JitGeom {
   int n[3], periodic[3];
@@ -89,9 +82,6 @@ struct Launch {
   const KCore<T>* kc = nullptr;      // core kernel arguments (host)
   const int32_t* tiles = nullptr;    // device list of generic tiles
   int ncore = 0, nbound = 0;
-  const KCore<T>* kcw = nullptr;     // WENO fast-path arguments (nullptr: not used)
-  const WenoPlan* plan = nullptr;
-  int nweno = 0;
   hipFunction_t jit = nullptr;       // generated kernel over all tiles (nullptr: interpreter)
   int ntiles = 0;
   int jgrid = 0;                  // work-groups of the persistent generated kernel
@@ -110,15 +100,6 @@ int launch_rhs(const Launch<T>& L, const T* u, const Epi<T>& ep, hipStream_t str
 template <typename T>
 int launch_expand(const Launch<T>& L, const T* u, const T* p, double t, T* full, hipStream_t stream);
 
-// WENO5 fast path (2D, periodic, every slot a WENO5 derivative)
-bool weno_plan(const mol_desc* d, WenoPlan& plan);
-int program_depth(const mol_desc* d, int prog);
-template <typename T>
-size_t weno_lds_bytes(int nfields, int nx);
-template <typename T>
-int launch_weno2d(const KCore<T>& kc, const WenoPlan& plan, int nblocks, const T* u, const Epi<T>& ep,
-                  hipStream_t stream);
-constexpr int WENO_TILE_X = 64, WENO_TILE_Y = 32;
 template <typename T>
 int launch_fill_zero(T* p, int64_t n, hipStream_t stream);
 // lam += y0 + y1 + y2 + y3 (the adjoint of one RK4 step), then y0..y3 = 0 when `zero`
@@ -225,11 +206,6 @@ struct mol_handle {
   double* seg_t = nullptr;                // device clock at each step of the segment
   size_t seg_cap = 0;
   void* adj_y[4] = {nullptr, nullptr, nullptr, nullptr};
-  bool use_weno = false;                  // WENO5 fast path
-  mol::WenoPlan wplan{};
-  mol::KCore<double> kcw64{};
-  mol::KCore<float> kcw32{};
-  int nweno = 0;
   // device copies of the descriptor tables
   void* d_tabs = nullptr;
   void* d_slots = nullptr;

@@ -169,7 +169,6 @@ __global__ void __launch_bounds__(256) rhs_core_kernel(const KCore<T> kc_arg, co
 
 template <typename T>
 int launch_rhs(const Launch<T>& L, const T* u, const Epi<T>& ep, hipStream_t stream) {
-  if (L.kcw) return launch_weno2d<T>(*L.kcw, *L.plan, L.nweno, u, ep, stream);
   const size_t lds = (size_t)(L.ka->nfields + L.ka->naux) * L.ka->vol * sizeof(T);
   if (L.batch) {
     const T* uu = u;

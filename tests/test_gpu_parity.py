@@ -522,20 +522,6 @@ def test_nonlinear_laplacian_1d(periodic, p, rhs_path):
     assert_parity(got, ref, lap_scale(u, 8.0, [dx]) * 4, what="nonlinlap periodic=%s p=%d" % (periodic, p))
 
 
-def test_weno2d_specialised_kernel_matches_oracle(monkeypatch):
-    """The hand-written 2D WENO5 kernel (64 x 32 tiles, shared differences) on request."""
-    monkeypatch.setenv("MOL_WENO2D", "1")
-    n = 129
-    sys_, disc = systems.burgers2d(n=n)
-    prob = molgpu.discretize(sys_, disc)
-    assert prob.plan()["weno_tiles"] > 0
-    rng = np.random.default_rng(5)
-    u = rng.uniform(0.2, 1.2, prob.lowered.n_unknowns)
-    got = gpu_rhs(prob, u)
-    ref = orc.burgers2d_rhs(n, u, threads=8)
-    assert_parity(got, ref, np.max(np.abs(u)) ** 2 * 4 * (n - 1) + 1, what="weno2d n=%d" % n)
-
-
 def test_host_pointer_entry_point():
     n = 17
     sys_, disc = systems.brusselator2d(n=n)
