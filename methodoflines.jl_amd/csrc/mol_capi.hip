@@ -181,11 +181,8 @@ int validate(const mol_desc* d, std::string& msg) {
       if (!tab_ok(s.tab[q])) { msg = "slot table out of range"; return MOL_ERR_INVALID; }
       const mol_optab& tb = d->tabs_h[s.tab[q]];
       if (tb.pn_hi > 0) {
-        if (s.kind == MOL_SLOT_MIXED) {
-          msg = "per-node rows do not serve mixed-derivative slots";
-          return MOL_ERR_UNSUPPORTED;
-        }
-        if (d->periodic[s.axis] || tb.pn_hi > d->n[s.axis]) { msg = "per-node rows on this axis"; return MOL_ERR_INVALID; }
+        const int tax = s.kind == MOL_SLOT_MIXED && q == 1 ? s.axis2 : s.axis;   // the table's axis
+        if (d->periodic[tax] || tb.pn_hi > d->n[tax]) { msg = "per-node rows on this axis"; return MOL_ERR_INVALID; }
       }
     }
   }

@@ -731,23 +731,37 @@ int Gen::slot_body(int k, const Var* v) {
       return op(X_SELECT, {op(X_SPHCOND, {rr}), at0, gen});
     }
     case MOL_SLOT_MIXED: {
+      // sum_i sum_j (wx_i * wy_j) u[i, j] (2nd_order_mixed_deriv.jl:5-22); on a nonuniform
+      // axis the interior row of the node is read from the pool at run time
       const RowR rx = row(d_->tabs_h[s.tab[0]], ax, v[ax], 0);
       const RowR ry = row(d_->tabs_h[s.tab[1]], s.axis2, v[s.axis2], 0);
       if (!rx.valid || !ry.valid) return lit_node(0.0);
+      const bool runtime = !rx.wx.empty() || !ry.wx.empty();
       std::vector<double> w;
       std::vector<long long> wi, wj;
+      std::vector<std::string> wx;
       std::vector<int> args;
       for (size_t i = 0; i < rx.w.size(); ++i)
         for (size_t j = 0; j < ry.w.size(); ++j) {
-          w.push_back(rx.w[i] * ry.w[j]);
+          w.push_back(runtime ? 1.0 : rx.w[i] * ry.w[j]);
           wi.push_back(rx.widx[i]);
           wj.push_back(ry.widx[j]);
+          if (runtime)
+            wx.push_back("(" + (rx.wx.empty() ? lit(rx.w[i]) : rx.wx[i]) + " * " +
+                         (ry.wx.empty() ? lit(ry.w[j]) : ry.wx[j]) + ")");
           int dd[3] = {0, 0, 0};
           dd[ax] += rx.rel[i];
           dd[s.axis2] += ry.rel[j];
           args.push_back(lds(s.field, dd[0], dd[1], dd[2]));
         }
-      return lincomb(w, args, wi, wj);
+      if (!runtime) return lincomb(w, args, wi, wj);
+      Node n;
+      n.kind = N_OP;
+      n.op = X_LINCOMB;
+      n.w = w;
+      n.a = args;
+      n.wx = wx;
+      return add(n);
     }
     default:
       bad("slot kind " + istr(s.kind) + " not supported by the generator");

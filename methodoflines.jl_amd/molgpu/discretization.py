@@ -578,9 +578,10 @@ class _Lowering:
             if any(v == self.t for v, _ in vc):
                 raise NotImplementedError("time derivative inside the right-hand side: %s" % d)
             if len(vc) == 2 and vc[0][1] == 1 and vc[1][1] == 1:
+                # on a nonuniform axis the first-derivative table has a row per node, as
+                # mixed_central_difference takes it (2nd_order_mixed_deriv.jl:5-22 through
+                # central_difference_weights_and_stencil on nodes, centered_difference.jl:32-57)
                 ax, ay = self.axes.index(vc[0][0]), self.axes.index(vc[1][0])
-                if self.nonuniform[ax] or self.nonuniform[ay]:
-                    raise NotImplementedError("mixed derivatives on a nonuniform axis")
                 k = self.slot(dict(kind=SLOT_MIXED, field=fvar, axis=ax, axis2=ay,
                                    tab=[self.table(("centered", ax, 1)), self.table(("centered", ay, 1))],
                                    prog=-1, eps=0.0, inv_dx=0.0))

@@ -247,18 +247,23 @@ def nonuniform1d(x_nodes, D: float, c: float, approx_order: int = 2, upwind_orde
                                      advection_scheme=UpwindScheme(upwind_order))
 
 
-def mixed2d(n: int, c: float, approx_order: int = 2):
+def mixed2d(n, c: float, approx_order: int = 2, q=(0.0, 1.0, 2.0, 0.0, 0.0, 0.0)):
+    """u_t = u_xx + u_yy + c u_xy on [0,1]^2, Dirichlet u = q0 + q1 x + q2 y + q3 x^2 + q4 x y +
+    q5 y^2 on all sides.  n: nodes per axis, or a pair (per axis: a node count or the nodes, both
+    ends 0 and 1 included)."""
     t, x, y = parameters("t x y")
     u = variables("u")
     Dt, Dx, Dy = Differential(t), Differential(x), Differential(y)
     U = u(t, x, y)
     eq = Eq(Dt(U), (Dx ** 2)(U) + (Dy ** 2)(U) + c * Dx(Dy(U)))
-    g = lambda xx, yy: xx + 2 * yy
+    q = [sp.nsimplify(v) if float(v) == int(v) else float(v) for v in q]
+    g = lambda xx, yy: q[0] + q[1] * xx + q[2] * yy + q[3] * xx ** 2 + q[4] * xx * yy + q[5] * yy ** 2
     bcs = [Eq(u(0, x, y), 0), Eq(u(t, 0, y), g(0, y)), Eq(u(t, 1, y), g(1, y)),
            Eq(u(t, x, 0), g(x, 0)), Eq(u(t, x, 1), g(x, 1))]
     sys_ = PDESystem([eq], bcs, [Interval(t, 0.0, 1.0), Interval(x, 0.0, 1.0), Interval(y, 0.0, 1.0)],
                      [t, x, y], [U])
-    return sys_, MOLFiniteDifference({x: n, y: n}, t, approx_order=approx_order)
+    nx, ny = (n, n) if np.isscalar(n) else n
+    return sys_, MOLFiniteDifference({x: nx, y: ny}, t, approx_order=approx_order)
 
 
 def spherical1d(n, g1: float, approx_order: int = 2):

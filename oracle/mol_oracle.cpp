@@ -28,6 +28,7 @@
 //   interior (unknown set) .......... src/system_parsing/interior_map.jl:1-10,53-139
 //   RK4 / SSPRK33 ................... textbook tableaus (OrdinaryDiffEq is not vendored)
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -793,7 +794,7 @@ S mixed_at(const Field<S>& F, const Op& Dx, const Op& Dy, const int* I, int ax, 
     const std::vector<double>* ww; int first;
     if (!per && I[a] <= op.bpc) { ww = &op.low[I[a] - 1]; first = 1; }
     else if (!per && I[a] > n - op.bpc) { ww = &op.high[n - I[a]]; first = n - op.BL + 1; }
-    else { ww = &op.st; first = I[a] - op.L / 2; }
+    else { ww = &op.row(I[a]); first = I[a] - op.L / 2; }   // per-node row on a nonuniform axis
     w = *ww;
     taps.clear();
     for (size_t k = 0; k < w.size(); ++k) taps.push_back(sp.wrap(a, first + (int)k));
@@ -1298,6 +1299,56 @@ int orc_mixed2d_rhs(int n, int p, double c, const double* u, double t, double* d
     Op d1x = centered(1, p, s.sp.ax[0].dx), d1y = centered(1, p, s.sp.ax[1].dx);
     s.node_rhs = [c, d2x, d2y, d1x, d1y](System<double>& S, const int* I, double, double* out) {
       out[0] = central(S.F[0], d2x, I, 0) + central(S.F[0], d2y, I, 1) + c * mixed_at(S.F[0], d1x, d1y, I, 0, 1);
+    };
+    s.rhs(u, t, du, 1);
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
+// ---- the same system on any mix of uniform and nonuniform axes (x[a] NULL: n[a] uniform nodes on
+//      [0,1], else the nodes), Dirichlet u = q(x, y) = q0 + q1 x + q2 y + q3 x^2 + q4 x y + q5 y^2
+//      on all sides.  Interior rows of a nonuniform axis per node (centered_nu), as
+//      mixed_central_difference takes them (2nd_order_mixed_deriv.jl:5-22 through
+//      central_difference_weights_and_stencil on nodes, centered_difference.jl:32-57).
+int orc_mixed2d_grid_rhs(const int* n, const double* const* x, int p, double c, const double* q, const double* u,
+                         double t, double* du) {
+  try {
+    System<double> s;
+    s.p = p;
+    s.sp.nd = 2;
+    Op d2[2], d1[2];
+    for (int a = 0; a < 2; ++a) {
+      make_axis(s.sp.ax[a], n[a], 0.0, 1.0, false);
+      if (x && x[a]) {
+        for (int i = 0; i < n[a]; ++i) s.sp.ax[a].x[i] = x[a][i];
+        s.sp.ax[a].nonuniform = true;
+        s.sp.ax[a].dx = std::nan("");
+        d2[a] = centered_nu(2, p, s.sp.ax[a].x);
+        d1[a] = centered_nu(1, p, s.sp.ax[a].x);
+      } else {
+        d2[a] = centered(2, p, s.sp.ax[a].dx);
+        d1[a] = centered(1, p, s.sp.ax[a].dx);
+      }
+    }
+    s.F.resize(1);
+    Field<double>& f = s.F[0];
+    Space* spp = &s.sp;
+    const std::array<double, 6> qc = {q[0], q[1], q[2], q[3], q[4], q[5]};
+    for (int a = 0; a < 2; ++a)
+      for (int sd = 0; sd < 2; ++sd) {
+        f.side[a][sd].has = true;
+        f.side[a][sd].a = {1.0};
+        f.side[a][sd].g = [spp, qc](double, const int* I) {
+          const double X = spp->ax[0].x[I[0] - 1], Y = spp->ax[1].x[I[1] - 1];
+          return qc[0] + qc[1] * X + qc[2] * Y + qc[3] * X * X + qc[4] * X * Y + qc[5] * Y * Y;
+        };
+      }
+    box_default(s, f);
+    f.p = p;
+    s.node_rhs = [c, d2, d1](System<double>& S, const int* I, double, double* out) {
+      out[0] = central(S.F[0], d2[0], I, 0) + central(S.F[0], d2[1], I, 1) + c * mixed_at(S.F[0], d1[0], d1[1], I, 0, 1);
     };
     s.rhs(u, t, du, 1);
     return 0;

@@ -53,6 +53,7 @@ def lib():
             "orc_nonlindiff3d_jvp": [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_longlong, C.POINTER(C.c_double), C.POINTER(C.c_double)],
             "orc_upwind1d_rhs": [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp, C.c_double, _dp],
             "orc_mixed2d_rhs": [C.c_int, C.c_int, C.c_double, _dp, C.c_double, _dp],
+            "orc_mixed2d_grid_rhs": [_ip, C.POINTER(C.c_void_p), C.c_int, C.c_double, _dp, _dp, C.c_double, _dp],
             "orc_spherical1d_rhs": [C.c_int, C.c_int, C.c_double, _dp, C.c_double, _dp],
             "orc_nonlinlap1d_rhs": [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp, C.c_double, _dp],
             "orc_num_threads": [],
@@ -256,6 +257,18 @@ def mixed2d_rhs(n, c, u, p=2, t=0.0):
     u = _state(u)
     du = np.empty_like(u)
     _ok(lib().orc_mixed2d_rhs(n, p, c, u, float(t), du))
+    return du
+
+
+def mixed2d_grid_rhs(grids, c, u, p=2, q=(0.0, 1.0, 2.0, 0.0, 0.0, 0.0), t=0.0):
+    """u_t = u_xx + u_yy + c u_xy on [0,1]^2 (grids[a]: a node count or the nodes), Dirichlet
+    u = q0 + q1 x + q2 y + q3 x^2 + q4 x y + q5 y^2 on all sides."""
+    keep = [None if np.isscalar(g) else _f64(g) for g in grids]
+    n = np.array([int(g) if np.isscalar(g) else len(g) for g in grids], dtype=np.int32)
+    xs = (C.c_void_p * 2)(*[(k_.ctypes.data if k_ is not None else None) for k_ in keep])
+    u = _state(u)
+    du = np.empty_like(u)
+    _ok(lib().orc_mixed2d_grid_rhs(n, xs, int(p), float(c), _f64(q), u, float(t), du))
     return du
 
 

@@ -483,6 +483,30 @@ def test_upwind_sign_selection(order, rhs_path):
 
 
 @pytest.mark.parametrize("p", [2, 4])
+@pytest.mark.parametrize("which", ["x", "xy"])
+def test_mixed_derivative_on_nonuniform_axes(p, which):
+    """Per-node first-derivative rows of nonuniform axes in the mixed slot (generated kernel
+    only), small tiles for core and boundary tiles, against the oracle."""
+    torch = _torch()
+    gx = systems.stretched_grid(41, 1.0, seed=8)
+    gy = systems.stretched_grid(33, 1.0, seed=9)
+    grids = (gx, 33) if which == "x" else (gx, gy)
+    q = (0.3, -0.7, 1.1, 0.9, -1.3, 0.4)
+    prob = molgpu.discretize(*systems.mixed2d(grids, 0.6, approx_order=p, q=q))
+    u = np.random.default_rng(p).uniform(-1, 1, prob.lowered.n_unknowns)
+    ref = orc.mixed2d_grid_rhs(list(grids), 0.6, u, p=p, q=q)
+    for tile in [(16, 8, 1), None]:
+        if tile:
+            prob.set_tile(*tile)
+        ug = torch.tensor(u, device="cuda")
+        du = torch.empty_like(ug)
+        print("  nonuniform mixed p=%d %s tile=%s" % (p, which, tile), flush=True)
+        prob.f(du, ug, None, 0.0)
+        torch.cuda.synchronize()
+        assert_parity(du.cpu().numpy(), ref, rtol=1e-12, what="nonuniform mixed p=%d %s" % (p, which))
+
+
+@pytest.mark.parametrize("p", [2, 4])
 def test_mixed_derivative(p, rhs_path):
     n = 25
     sys_, disc = systems.mixed2d(n, 0.7, p)

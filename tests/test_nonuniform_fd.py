@@ -252,3 +252,77 @@ def test_fused_rk4_step_on_a_nonuniform_grid(c):
     got, _ = genhost.rk4_fused(low, u, 0.1, 1e-6, tile=(64, 1, 1))
     ref = _rk4(lambda w, t: orc.nonuniform1d_rhs(x, 0.7, c, 2, 1, bcl, bch, w, t), u, 0.1, 1e-6)
     assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref))
+
+
+# ---------------------------------------------------------------- mixed derivatives on nodes
+@pytest.mark.parametrize("p", [2, 4])
+def test_mixed_derivative_oracle_on_uniform_nodes_equals_the_uniform_oracle(p):
+    """Node arrays of a uniform grid reproduce the uniform mixed-derivative system."""
+    n = 17
+    x = np.linspace(0.0, 1.0, n)
+    u = np.random.default_rng(p).uniform(-1, 1, (n - 2) ** 2)
+    a = orc.mixed2d_grid_rhs([x, x], 0.7, u, p=p)
+    b = orc.mixed2d_rhs(n, 0.7, u, p=p)
+    assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
+
+
+@pytest.mark.parametrize("p", [2, 4])
+@pytest.mark.parametrize("which", ["x", "xy"])
+def test_mixed_derivative_exact_for_quadratics_on_random_nodes(p, which):
+    """With u = q(x, y) a quadratic on the nodes and on the Dirichlet sides, the right-hand side
+    is q_xx + q_yy + c q_xy = 2 q3 + 2 q5 + c q4 at every node whose mixed stencil reads no
+    corner (corner nodes are 0, generate_bc_eqs.jl:396-416): the per-node centered rows (first
+    and second derivative, p + 1 or more nodes) are exact for degree 2.  A row taken at the
+    wrong node, or the uniform row on a nonuniform axis, breaks it."""
+    rng = np.random.default_rng(11 + p)
+    gx = np.concatenate([[0.0], np.sort(rng.uniform(0.02, 0.98, 13)), [1.0]])
+    gy = np.concatenate([[0.0], np.sort(rng.uniform(0.02, 0.98, 11)), [1.0]]) if which == "xy" else 13
+    q = (0.3, -0.7, 1.1, 0.9, -1.3, 0.4)
+    c = 0.6
+    xs = gx
+    ys = gy if which == "xy" else np.linspace(0.0, 1.0, gy)
+    X, Y = np.meshgrid(xs[1:-1], ys[1:-1], indexing="ij")
+    u = (q[0] + q[1] * X + q[2] * Y + q[3] * X ** 2 + q[4] * X * Y + q[5] * Y ** 2).reshape(-1, order="F")
+    du = orc.mixed2d_grid_rhs([gx, gy], c, u, p=p, q=q).reshape(X.shape, order="F")
+    exact = 2 * q[3] + 2 * q[5] + c * q[4]
+    # node i (1-based) reads a boundary node along an axis of n nodes when i <= p/2 + 1 or
+    # i >= n - p/2; the mixed term reads a corner where that holds along both axes
+    touch = lambda m: np.array([i <= p // 2 + 1 or i >= m - p // 2 for i in range(2, m)])
+    keep = ~np.logical_and.outer(touch(len(xs)), touch(len(ys)))
+    assert keep.sum() > 0.5 * keep.size
+    err = np.abs(du - exact)[keep]
+    assert np.max(err) <= 1e-9 * (1 + abs(exact)), np.max(err)
+
+
+@pytest.mark.parametrize("p", [2, 4])
+@pytest.mark.parametrize("which", ["x", "y", "xy"])
+def test_generated_mixed_derivative_on_nonuniform_axes_matches_oracle(p, which):
+    """The generated kernel's mixed slot reads per-node rows of nonuniform axes (core tiles from
+    the pool, boundary tiles as literals): equal to the oracle on random data."""
+    gx = systems.stretched_grid(27, 1.0, seed=8)
+    gy = systems.stretched_grid(21, 1.0, seed=9)
+    grids = {"x": (gx, 21), "y": (27, gy), "xy": (gx, gy)}[which]
+    q = (0.3, -0.7, 1.1, 0.9, -1.3, 0.4)
+    low = lower(*systems.mixed2d(grids, 0.6, approx_order=p, q=q))
+    u = np.random.default_rng(p).uniform(-1, 1, low.n_unknowns)
+    ref = orc.mixed2d_grid_rhs(list(grids), 0.6, u, p=p, q=q)
+    for tile in [(8, 4, 1), (16, 8, 1), None]:
+        got = genhost.rhs(low, u, tile=tile)
+        assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref)), tile
+
+
+def test_generated_mixed_derivative_reverse_mode_matches_finite_differences():
+    """The reverse sweep of the mixed slot with per-node rows on both axes: the map is affine in
+    u, so <J^T lam, v> equals the central difference of <lam, f> up to rounding."""
+    gx = systems.stretched_grid(23, 1.0, seed=8)
+    gy = systems.stretched_grid(19, 1.0, seed=9)
+    q = (0.3, -0.7, 1.1, 0.9, -1.3, 0.4)
+    low = lower(*systems.mixed2d((gx, gy), 0.6, approx_order=2, q=q))
+    rng = np.random.default_rng(6)
+    N = low.n_unknowns
+    u, lam, v = rng.uniform(-1, 1, N), rng.uniform(-1, 1, N), rng.uniform(-1, 1, N)
+    for tile in [(8, 4, 1), None]:
+        g, _ = genhost.vjp(low, u, lam, tile=tile)
+        F = lambda w: lam @ orc.mixed2d_grid_rhs([gx, gy], 0.6, w, p=2, q=q)
+        fd = (F(u + v) - F(u - v)) / 2
+        assert abs(g @ v - fd) <= 1e-10 * (abs(fd) + np.linalg.norm(g) * np.linalg.norm(v)), tile
