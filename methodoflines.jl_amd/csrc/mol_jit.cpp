@@ -18,6 +18,8 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <set>
 #include <sstream>
 #include <string>
@@ -1816,6 +1818,83 @@ struct CacheEntry {
 std::mutex g_jit_mu;
 std::map<std::string, CacheEntry> g_jit_cache;   // key: generated source
 
+// On-disk cache of compiled code objects, shared by processes: one file per generated source,
+// named by its hash, holding a header (magic, hash of the embedded prelude and of the compiler
+// version and options, source length), the source itself and the code object.  A file is used
+// only when the whole source and the header match, so a hash collision or a library with
+// another prelude compiles afresh.  Directory: $MOL_JIT_CACHE ("0" disables the cache), else
+// $XDG_CACHE_HOME/molgpu/hiprtc, else $HOME/.cache/molgpu/hiprtc; failures to read or write
+// it are ignored.
+constexpr uint64_t kDiskMagic = 0x31657661636c6f6dull;   // "molcache1" tag
+
+std::string disk_cache_dir() {
+  const char* env = std::getenv("MOL_JIT_CACHE");
+  if (env && env[0] == '0' && env[1] == '\0') return std::string();
+  if (env && env[0]) return env;
+  if (const char* x = std::getenv("XDG_CACHE_HOME"); x && x[0]) return std::string(x) + "/molgpu/hiprtc";
+  if (const char* h = std::getenv("HOME"); h && h[0]) return std::string(h) + "/.cache/molgpu/hiprtc";
+  return std::string();
+}
+
+uint64_t compiler_tag() {
+  static const uint64_t tag = [] {
+    int major = 0, minor = 0;
+    (void)hiprtcVersion(&major, &minor);
+    return fnv1a(std::string(mol_jit_prelude_text)) ^ (((uint64_t)major << 40) | ((uint64_t)minor << 24)) ^
+           fnv1a("--offload-arch=gfx950 -O3 -std=c++17 -packed-fp32-ops");
+  }();
+  return tag;
+}
+
+std::string disk_cache_file(const std::string& dir, const std::string& src) {
+  char name[48];
+  std::snprintf(name, sizeof(name), "/%016llx.co", (unsigned long long)fnv1a(src));
+  return dir + name;
+}
+
+bool disk_cache_load(const std::string& src, std::vector<char>& code) {
+  const std::string dir = disk_cache_dir();
+  if (dir.empty()) return false;
+  FILE* f = std::fopen(disk_cache_file(dir, src).c_str(), "rb");
+  if (!f) return false;
+  uint64_t hdr[3] = {0, 0, 0};
+  bool ok = std::fread(hdr, sizeof(hdr), 1, f) == 1 && hdr[0] == kDiskMagic && hdr[1] == compiler_tag() &&
+            hdr[2] == src.size();
+  if (ok) {
+    std::string s(src.size(), '\0');
+    ok = std::fread(&s[0], 1, s.size(), f) == s.size() && s == src;
+  }
+  if (ok) {
+    std::vector<char> buf;
+    char chunk[1 << 16];
+    size_t n;
+    while ((n = std::fread(chunk, 1, sizeof(chunk), f)) > 0) buf.insert(buf.end(), chunk, chunk + n);
+    ok = !buf.empty();
+    if (ok) code.swap(buf);
+  }
+  std::fclose(f);
+  return ok;
+}
+
+void disk_cache_store(const std::string& src, const std::vector<char>& code) {
+  const std::string dir = disk_cache_dir();
+  if (dir.empty()) return;
+  // create the directory chain (mkdir -p); errors surface as a failed open below
+  for (size_t i = 1; i <= dir.size(); ++i)
+    if (i == dir.size() || dir[i] == '/') (void)mkdir(dir.substr(0, i).c_str(), 0755);
+  const std::string path = disk_cache_file(dir, src);
+  char tmp_suffix[40];
+  std::snprintf(tmp_suffix, sizeof(tmp_suffix), ".tmp%d", (int)getpid());
+  const std::string tmp = path + tmp_suffix;
+  FILE* f = std::fopen(tmp.c_str(), "wb");
+  if (!f) return;
+  const uint64_t hdr[3] = {kDiskMagic, compiler_tag(), (uint64_t)src.size()};
+  bool ok = std::fwrite(hdr, sizeof(hdr), 1, f) == 1 && std::fwrite(src.data(), 1, src.size(), f) == src.size() &&
+            std::fwrite(code.data(), 1, code.size(), f) == code.size();
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok || std::rename(tmp.c_str(), path.c_str()) != 0) (void)std::remove(tmp.c_str());
+}
+
 }  // namespace
 
 std::string jit_fused_source(const mol_desc* d, const JitGeom& g, std::string& err) {
@@ -1889,13 +1968,14 @@ int jit_compile(const std::string& src, std::vector<char>& code, std::string& lo
 int jit_kernel(const std::string& src, const char* name, int device, hipFunction_t* fn, std::string& err) {
   std::lock_guard<std::mutex> lk(g_jit_mu);
   CacheEntry& e = g_jit_cache[src];
-  if (e.code.empty()) {
+  if (e.code.empty() && !disk_cache_load(src, e.code)) {
     std::string log;
     if (jit_compile(src, e.code, log)) {
       g_jit_cache.erase(src);
       err = "hiprtc: " + log;
       return MOL_ERR_HIP;
     }
+    disk_cache_store(src, e.code);
     if (const char* dir = std::getenv("MOL_JIT_DUMP")) {
       char fname[64];
       std::snprintf(fname, sizeof(fname), "/mol_gen_%016llx.hip", (unsigned long long)fnv1a(src));
