@@ -266,6 +266,21 @@ typedef enum { MOL_FLAG_KAHAN = 1, MOL_FLAG_GRAPH = 2, MOL_FLAG_UNFUSED = 4 } mo
 int mol_integrate(mol_handle* h, void* u, const void* p, double t0, double dt, int64_t nsteps,
                   int32_t scheme, int32_t flags, void* stream);
 
+/* No counterpart in the reference: mol_integrate for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, arithmetic and results as mol_integrate; the stage vectors and the device clock
+ * live in a library-owned device array of MOL_NOALLOC_SCRATCH_BYTES per device (reserved when
+ * the library is loaded on the device, held until the process ends; its address is looked up by
+ * mol_create on the handle's device).  It calls no hipMalloc/hipFree, creates no graph
+ * (MOL_FLAG_GRAPH is MOL_ERR_INVALID) and records no memset.  Needs 256 bytes plus, per stage
+ * vector rounded up to 256 bytes, one vector for fused RK4 steps, three for RK4 stage launches,
+ * two for SSPRK33, one more with MOL_FLAG_KAHAN; larger integrations fail with MOL_ERR_SHAPE
+ * before anything is enqueued.  Every call on one device shares that device's array: calls that
+ * overlap on one device (any handle, any stream) race. */
+#define MOL_NOALLOC_SCRATCH_BYTES (64u << 20)
+int mol_integrate_noalloc(mol_handle* h, void* u, const void* p, double t0, double dt, int64_t nsteps,
+                          int32_t scheme, int32_t flags, void* stream);
+
 /* ---- adjoint ------------------------------------------------------------------------
  * Replaces: reverse-mode AD of the generated f (what SciMLSensitivity's adjoint methods
  * obtain by differentiating the RuntimeGeneratedFunction of src/MOL_discretization.jl:171-201

@@ -2,6 +2,7 @@
 #include <array>
 #include <cstdio>
 #include <map>
+#include <mutex>
 #include <sstream>
 #include <cstdlib>
 #include <cstring>
@@ -19,7 +20,45 @@ const char* global_error() { return g_err.c_str(); }
 
 using namespace mol;
 
+// Workspace of mol_integrate_noalloc: a device array of this library's code object, reserved on
+// each device when the code object is loaded there and held until the process ends.  Its name
+// is a host-side stand-in; the device address comes from hipGetSymbolAddress per device.
+static __device__ __align__(256) unsigned char scratch[MOL_NOALLOC_SCRATCH_BYTES];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};          // device address of `scratch` on each device
+hipError_t g_scratch_err[kMaxDevices] = {};     // result of the lookup on each device
+bool g_scratch_done[kMaxDevices] = {};
+
+// Look up the current device's address of `scratch` once per device (called by mol_create and
+// mol_create_batched with the handle's device current).
+void scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_done[device]) return;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  g_scratch_err[device] = e;
+  g_scratch_ptr[device] = e == hipSuccess ? ptr : nullptr;
+  g_scratch_done[device] = true;
+}
+
+void* scratch_of(int device, std::string& err) {
+  if (device < 0 || device >= kMaxDevices) {
+    err = "device index out of range for the scratch table";
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_done[device]) {
+    err = "scratch of this device was not looked up (no handle created on it)";
+    return nullptr;
+  }
+  if (!g_scratch_ptr[device]) err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(g_scratch_err[device]);
+  return g_scratch_ptr[device];
+}
 
 #define HIPCHK(h, call)                                                           \
   do {                                                                            \
@@ -940,6 +979,43 @@ int integrate_impl(mol_handle* h, T* u, const T* p, double t0, double dt, int64_
   return enqueue_steps<T>(h, handle_work(h), u, p, dt, left, scheme, kahan, s);
 }
 
+// mol_integrate without any allocation: the stage vectors and the device clock live in this
+// device's `scratch`; no graphs (instantiation allocates), no memset nodes (a caller may
+// capture the launches).  The arrays a path reads before writing (the Kahan compensation) are
+// cleared by a fill kernel, since the scratch keeps the last call's contents.
+template <typename T>
+int integrate_noalloc_impl(mol_handle* h, T* u, const T* p, double t0, double dt, int64_t nsteps, int scheme,
+                           int flags, hipStream_t s) {
+  std::string err;
+  char* base = (char*)scratch_of(h->device, err);
+  if (!base) return fail(h, MOL_ERR_HIP, err);
+  const bool kahan = (flags & MOL_FLAG_KAHAN) != 0;
+  h->no_fuse = (flags & MOL_FLAG_UNFUSED) != 0;
+  const bool fused = scheme == MOL_RK4 && !kahan && h->fused_fn && !h->no_fuse;
+  // stage vectors each path touches: fused RK4 y1; RK4 stage launches y1, y2, acc; SSPRK33 y1,
+  // y2; Kahan adds the compensation
+  const size_t vb = ((size_t)h->nunk * sizeof(T) + 255) / 256 * 256;
+  const int narr = fused ? 1 : (scheme == MOL_RK4 ? 3 : 2) + (kahan ? 1 : 0);
+  const size_t need = 256 + (size_t)narr * vb;
+  if (need > (size_t)MOL_NOALLOC_SCRATCH_BYTES)
+    return fail(h, MOL_ERR_SHAPE, "mol_integrate_noalloc: this integration needs " + std::to_string(need) +
+                                      " bytes of scratch, more than its " +
+                                      std::to_string((size_t)MOL_NOALLOC_SCRATCH_BYTES) + " bytes");
+  Work w{};
+  w.tnow = (double*)base;
+  char* q = base + 256;
+  w.y1 = q;
+  if (!fused) {
+    w.y2 = q + vb;
+    if (scheme == MOL_RK4) w.acc = q + 2 * vb;
+    if (kahan) w.comp = q + (size_t)(narr - 1) * vb;
+  }
+  h->clk = 0;
+  if (launch_set_clock(w.tnow, t0, s)) return fail(h, MOL_ERR_HIP, "clock launch failed");
+  if (kahan && launch_fill_zero<T>((T*)w.comp, h->nunk, s)) return fail(h, MOL_ERR_HIP, "zero fill launch failed");
+  return enqueue_steps<T>(h, w, u, p, dt, nsteps, scheme, kahan, s);
+}
+
 // Does program `prog` read opcode `op` (e.g. an auxiliary array)?
 bool prog_uses(const mol_desc* d, int prog, uint32_t op) {
   if (prog < 0) return false;
@@ -1419,6 +1495,7 @@ int create_batched(const mol_desc* const* ds, const double* const* params, int32
   h->batch = true;
   h->desc.assign(ds[0]);
   (void)hipGetDevice(&h->device);
+  scratch_setup(h->device);
   h->dtype = ds[0]->dtype;
   h->ndim = ds[0]->ndim;
   h->nfields = ds[0]->nfields;
@@ -1470,6 +1547,7 @@ int mol_create(const mol_desc* d, mol_handle** out) {
   h->desc.assign(d);
   d = &h->desc.d;
   (void)hipGetDevice(&h->device);
+  scratch_setup(h->device);
   h->dtype = d->dtype;
   h->ndim = d->ndim;
   h->nfields = d->nfields;
@@ -1668,6 +1746,22 @@ int mol_integrate(mol_handle* h, void* u, const void* p, double t0, double dt, i
   if (h->dtype == MOL_F64)
     return integrate_impl<double>(h, (double*)u, (const double*)p, t0, dt, nsteps, scheme, flags, s);
   return integrate_impl<float>(h, (float*)u, (const float*)p, t0, dt, nsteps, scheme, flags, s);
+}
+
+int mol_integrate_noalloc(mol_handle* h, void* u, const void* p, double t0, double dt, int64_t nsteps,
+                          int32_t scheme, int32_t flags, void* stream) {
+  if (!h) return fail(nullptr, MOL_ERR_INVALID, "handle is NULL");
+  if (!u) return fail(h, MOL_ERR_INVALID, "u is NULL");
+  if (nsteps < 0) return fail(h, MOL_ERR_INVALID, "nsteps < 0");
+  if (scheme != MOL_RK4 && scheme != MOL_SSPRK33) return fail(h, MOL_ERR_INVALID, "unknown scheme");
+  if (flags & MOL_FLAG_GRAPH)
+    return fail(h, MOL_ERR_INVALID, "mol_integrate_noalloc takes no MOL_FLAG_GRAPH (instantiating a graph allocates)");
+  if (h->nparams > 0 && !p) return fail(h, MOL_ERR_INVALID, "parameters missing");
+  if (int rc = check_aux(h)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (h->dtype == MOL_F64)
+    return integrate_noalloc_impl<double>(h, (double*)u, (const double*)p, t0, dt, nsteps, scheme, flags, s);
+  return integrate_noalloc_impl<float>(h, (float*)u, (const float*)p, t0, dt, nsteps, scheme, flags, s);
 }
 
 int mol_vjp(mol_handle* h, const void* u, const void* p, double t, const void* lam, void* lam_dot, double* g_aux,
