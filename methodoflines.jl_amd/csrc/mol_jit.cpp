@@ -132,6 +132,9 @@ class Gen {
   bool ok_ = true;
   bool param_ = false;                // parametric (batched) printing
   bool fused_ = false;                // fused RK4 step printing (coordinates from LDS)
+  // printing overrides: nodes not printed, and nodes referenced by another expression
+  std::vector<char> skip_;
+  std::map<int, std::string> subst_;
 
   // instance-dependent numbers: literal in a specialised kernel, header loads in a class body
   std::string pw(double v, long long idx) const { return param_ ? "(T)hd.W[" + istr(idx) + "]" : lit(v); }
@@ -342,6 +345,11 @@ class Gen {
   std::string stage_fn(bool lds = false);
   std::string scatter_fn();
   PrintGeo make_pg(const int* halo, const Usage& us) const;
+  // Node-invariant part of region r (fused step): values that depend only on constants,
+  // parameters and coordinates.  inv[id] marks them; the returned frontier lists those that a
+  // time- or state-dependent node (or an output) uses.
+  std::vector<int> invariant_frontier(int r, const std::vector<int>& outs, std::vector<char>& inv) const;
+  std::string node_hoisted(int region, const std::vector<int>& outs);
   std::string geo_variant(const std::string& name, const int* halo, const Usage& us, bool core, int region,
                           const std::vector<int>& outs, bool vjp);
   std::string source_kernel(bool f64, bool vjp);
@@ -863,6 +871,10 @@ void Gen::build_node(bool core, std::vector<int>& outs, int& region) {
 
 std::string Gen::ref(int id) const {
   const Node& n = nodes_[id];
+  if (!subst_.empty()) {
+    const auto it = subst_.find(id);
+    if (it != subst_.end()) return it->second;
+  }
   switch (n.kind) {
     case N_LIT: return n.src == 2 ? pk(n.val, n.sidx) : lit(n.val);
     case N_LDS: {
@@ -971,6 +983,7 @@ std::string Gen::op_expr(const Node& n, int id) const {
 
 void Gen::print_node(std::ostream& o, int id) const {
   const Node& n = nodes_[id];
+  if (id < (int)skip_.size() && skip_[id]) return;
   if (n.kind == N_OP) {
     o << "    const T v" << id << " = " << op_expr(n, id) << ";\n";
     return;
@@ -1368,6 +1381,87 @@ Gen::PrintGeo Gen::make_pg(const int* halo, const Usage& us) const {
 
 // One tile variant: geometry constants, staging of one position, node function (forward or
 // reverse mode) and, for reverse mode, the flush of one position's adjoints.
+std::vector<int> Gen::invariant_frontier(int r, const std::vector<int>& outs, std::vector<char>& inv) const {
+  inv.assign(nodes_.size(), 0);
+  for (size_t id = 0; id < nodes_.size(); ++id) {
+    const Node& n = nodes_[id];
+    if (n.kind == N_LIT) inv[id] = 1;
+    else if (n.kind == N_EXPR) inv[id] = n.text != "(T)t";
+  }
+  // operations of the top-level region in evaluation order (operands come first); nodes of
+  // branch regions stay in place (boundary rows)
+  for (int id : regs_[r].nodes) {
+    const Node& n = nodes_[id];
+    if (n.kind != N_OP) continue;
+    bool all = true;
+    for (int a : n.a) all = all && inv[a];
+    inv[id] = all;
+  }
+  std::vector<char> used(nodes_.size(), 0);
+  std::vector<int> stack{r};
+  while (!stack.empty()) {
+    const int q = stack.back();
+    stack.pop_back();
+    for (int id : regs_[q].nodes) {
+      const Node& n = nodes_[id];
+      if (n.kind == N_BRANCH) {
+        for (int sr : n.reg) {
+          stack.push_back(sr);
+          if (regs_[sr].result >= 0) used[regs_[sr].result] = 1;
+        }
+        continue;
+      }
+      if (inv[id] && std::find(regs_[r].nodes.begin(), regs_[r].nodes.end(), id) != regs_[r].nodes.end()) continue;
+      for (int a : n.a) used[a] = 1;
+    }
+  }
+  for (int f : outs) used[f] = 1;
+  // hoist the top-level invariant operations and the coordinate reads a dependent node uses
+  // (literals cost nothing and parameters are uniform)
+  std::vector<int> front;
+  for (size_t id = 0; id < nodes_.size(); ++id) {
+    if (!used[id] || !inv[id]) continue;
+    const Node& n = nodes_[id];
+    if (n.kind == N_LIT) continue;
+    if (n.kind == N_EXPR && n.text.compare(0, 2, "P[") == 0) continue;
+    if (n.kind == N_OP && std::find(regs_[r].nodes.begin(), regs_[r].nodes.end(), (int)id) == regs_[r].nodes.end())
+      continue;
+    front.push_back((int)id);
+  }
+  return front;
+}
+
+// node_inv (the frontier values of a node, computed once) and node_h (the node function reading
+// them) of the fused step's variant; NINV = 0 when nothing is worth hoisting.
+std::string Gen::node_hoisted(int region, const std::vector<int>& outs) {
+  std::vector<char> inv;
+  const std::vector<int> front = invariant_frontier(region, outs, inv);
+  std::ostringstream o;
+  o << "  static constexpr int NINV = " << front.size() << ";\n";
+  if (front.empty()) return o.str();
+  o << "  static __device__ __forceinline__ void node_inv(const int I0, const int I1, const int I2,\n"
+       "      const T* __restrict__ P, const JPtrs& jp, T* inv) {\n"
+       "    (void)I0; (void)I1; (void)I2; (void)P; (void)jp;\n";
+  for (int id : regs_[region].nodes)
+    if (nodes_[id].kind == N_OP && inv[id]) print_node(o, id);
+  for (size_t k = 0; k < front.size(); ++k) o << "    inv[" << k << "] = " << ref(front[k]) << ";\n";
+  o << "  }\n";
+  o << "  static __device__ __forceinline__ void node_h(const T* __restrict__ L, const int b, const int I0, const int I1,\n"
+       "      const int I2, const double t, const T* __restrict__ P, const T* __restrict__ u, const JPtrs& jp,\n"
+       "      const T* inv, T* out) {\n"
+       "    (void)I0; (void)I1; (void)I2; (void)t; (void)P; (void)jp; (void)L; (void)b; (void)u; (void)inv;\n";
+  skip_.assign(nodes_.size(), 0);
+  for (int id : regs_[region].nodes)
+    if (nodes_[id].kind == N_OP && inv[id]) skip_[id] = 1;
+  for (size_t k = 0; k < front.size(); ++k) subst_[front[k]] = "inv[" + istr((long long)k) + "]";
+  print_region(o, region);
+  for (int f = 0; f < d_->nfields; ++f) o << "    out[" << f << "] = " << ref(outs[f]) << ";\n";
+  skip_.clear();
+  subst_.clear();
+  o << "  }\n";
+  return o.str();
+}
+
 std::string Gen::geo_variant(const std::string& name, const int* halo, const Usage& us, bool core, int region,
                              const std::vector<int>& outs, bool vjp) {
   pg_ = make_pg(halo, us);
@@ -1469,6 +1563,7 @@ std::string Gen::geo_variant(const std::string& name, const int* halo, const Usa
     print_region(o, region);
     for (int f = 0; f < d_->nfields; ++f) o << "    out[" << f << "] = " << ref(outs[f]) << ";\n";
     o << "  }\n";
+    if (fused_) o << node_hoisted(region, outs);
   } else {
     o << "  static __device__ __forceinline__ void zero_pos(T* Ab, double* Aa, const int q) {\n"
          "    (void)Ab; (void)Aa;\n";

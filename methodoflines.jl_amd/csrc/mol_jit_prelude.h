@@ -645,9 +645,29 @@ __device__ __forceinline__ void mol_frame_pos(const int h, int& lx, int& ly) {
   }
 }
 
+// Node-invariant values of the tile nodes (G::NINV per node: what depends only on constants,
+// parameters and coordinates), computed once per launch and read by G::node_h in every stage.
+template <class G>
+struct MolFusedInv {
+  static constexpr int NI = G::NINV > 0 ? G::NINV : 1;
+  T v[MolFusedItems<G>::NJ][NI];
+  __device__ __forceinline__ void init(const int o0, const int o1, const int o2, const T* __restrict__ P,
+                                       const JPtrs& jp) {
+    if constexpr (G::NINV > 0) {
+#pragma unroll
+      for (int j = 0; j < MolFusedItems<G>::NJ; ++j) {
+        int lx, ly;
+        MolFusedItems<G>::pos(j, lx, ly);
+        if (MolFusedItems<G>::valid(j)) G::node_inv(o0 + lx, o1 + ly, o2, P, jp, v[j]);
+      }
+    }
+  }
+};
+
 template <class G, int S>
 __device__ __forceinline__ void mol_rk4_stage(const T* Lin, T* Lout, const T* U, T (&Ur)[MolFusedItems<G>::NJ][G::NF],
-                                              T (&Ar)[MolFusedItems<G>::NJ][G::NF], const int o0, const int o1,
+                                              T (&Ar)[MolFusedItems<G>::NJ][G::NF], const MolFusedInv<G>& Iv,
+                                              const int o0, const int o1,
                                               const int o2, const T* __restrict__ u, T* __restrict__ unew,
                                               const Epi& ep, const JPtrs& jp, const double t, const double dt,
                                               T* Ukeep = nullptr, const bool store = true) {
@@ -666,7 +686,10 @@ __device__ __forceinline__ void mol_rk4_stage(const T* Lin, T* Lout, const T* U,
     if (!MolFusedItems<G>::valid(j)) break;
     int lx, ly;
     MolFusedItems<G>::pos(j, lx, ly);
-    G::node(Lin, (lx + G::HX) + G::EX * (ly + G::HY), o0 + lx, o1 + ly, o2, ts, ep.p, u, jp, out[j]);
+    if constexpr (G::NINV > 0)
+      G::node_h(Lin, (lx + G::HX) + G::EX * (ly + G::HY), o0 + lx, o1 + ly, o2, ts, ep.p, u, jp, Iv.v[j], out[j]);
+    else
+      G::node(Lin, (lx + G::HX) + G::EX * (ly + G::HY), o0 + lx, o1 + ly, o2, ts, ep.p, u, jp, out[j]);
   }
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
@@ -755,22 +778,34 @@ __device__ __forceinline__ void mol_rk4_fused_tile(T* U, T* Ya, T* Yb, double* f
                                                    const double t) {
   const int o0 = G::ULO0 + tx * G::TX, o1 = G::ULO1 + ty * G::TY, o2 = G::ULO2 + tz * G::TZ;
   const double dt = ep.tadv;
-  mol_stage_tile<G>(U, o0, o1, o2, mol_stage_mode<G>(tx, ty, tz), t, ep.p, u, jp);
-  // coordinates of the tile + halo positions (periodic images wrapped), per axis
-  {
-    const int org[3] = {o0 - G::HX, o1 - G::HY, o2 - G::HZ};
-    const int ext[3] = {G::EX, G::EY, G::EZ};
-    const int nn[3] = {G::N0, G::N1, G::N2};
-    const bool per[3] = {G::PER0 != 0, G::PER1 != 0, G::PER2 != 0};
-    for (int a = (int)threadIdx.x; a < 3; a += G::NTHR) fo[a] = org[a];
-    for (int a = 0; a < 3; ++a)
-      if (jp.coord[a])
-        for (int e = threadIdx.x; e < ext[a]; e += G::NTHR) {
-          int p = org[a] + e;
-          if (per[a]) p = mol_wrap(mol_wrap(p, nn[a]), nn[a]);
-          fc[a * fcs + e] = jp.coord[a][min(max(p, 1), nn[a]) - 1];
-        }
+  // coordinates of the tile + halo positions (periodic images wrapped), per axis: position c of
+  // the concatenated extents EX, EY, EZ; loaded before the staging loads are awaited, so both
+  // are in flight together
+  constexpr int NCP = G::EX + G::EY + G::EZ, NCI = (NCP + G::NTHR - 1) / G::NTHR;
+  double cv[NCI];
+  int cdst[NCI];
+#pragma unroll
+  for (int k = 0; k < NCI; ++k) {
+    const int c = (int)threadIdx.x + k * G::NTHR;
+    const int a = c < G::EX ? 0 : (c < G::EX + G::EY ? 1 : 2);
+    const int e = c - (a == 0 ? 0 : (a == 1 ? G::EX : G::EX + G::EY));
+    const int org = a == 0 ? o0 - G::HX : (a == 1 ? o1 - G::HY : o2 - G::HZ);
+    const int nn = a == 0 ? G::N0 : (a == 1 ? G::N1 : G::N2);
+    const bool per = a == 0 ? G::PER0 != 0 : (a == 1 ? G::PER1 != 0 : G::PER2 != 0);
+    cdst[k] = -1;
+    cv[k] = 0.0;
+    if (c < NCP && jp.coord[a]) {
+      int p = org + e;
+      if (per) p = mol_wrap(mol_wrap(p, nn), nn);
+      cv[k] = jp.coord[a][min(max(p, 1), nn) - 1];
+      cdst[k] = a * fcs + e;
+    }
   }
+  for (int a = (int)threadIdx.x; a < 3; a += G::NTHR) fo[a] = a == 0 ? o0 - G::HX : (a == 1 ? o1 - G::HY : o2 - G::HZ);
+  mol_stage_tile<G>(U, o0, o1, o2, mol_stage_mode<G>(tx, ty, tz), t, ep.p, u, jp);
+#pragma unroll
+  for (int k = 0; k < NCI; ++k)
+    if (cdst[k] >= 0) fc[cdst[k]] = cv[k];
   __syncthreads();
   constexpr int NJ = MolFusedItems<G>::NJ;
   T Ur[NJ][G::NF], Ar[NJ][G::NF];
@@ -786,22 +821,24 @@ __device__ __forceinline__ void mol_rk4_fused_tile(T* U, T* Ya, T* Yb, double* f
       Ar[j][f] = T(0);
     }
   }
-  mol_rk4_stage<G, 1>(U, Ya, U, Ur, Ar, o0, o1, o2, u, unew, ep, jp, t, dt);
+  MolFusedInv<G> Iv;
+  Iv.init(o0, o1, o2, ep.p, jp);
+  mol_rk4_stage<G, 1>(U, Ya, U, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt);
   __syncthreads();
   if constexpr (G::FIX) {
     if (mol_rk4_fix_ghosts<G, 1>(Ya, o0, o1, o2, t + dt / 2, ep.p, jp)) __syncthreads();
   }
-  mol_rk4_stage<G, 2>(Ya, Yb, U, Ur, Ar, o0, o1, o2, u, unew, ep, jp, t, dt);
+  mol_rk4_stage<G, 2>(Ya, Yb, U, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt);
   __syncthreads();
   if constexpr (G::FIX) {
     if (mol_rk4_fix_ghosts<G, 2>(Yb, o0, o1, o2, t + dt / 2, ep.p, jp)) __syncthreads();
   }
-  mol_rk4_stage<G, 3>(Yb, Ya, U, Ur, Ar, o0, o1, o2, u, unew, ep, jp, t, dt);
+  mol_rk4_stage<G, 3>(Yb, Ya, U, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt);
   __syncthreads();
   if constexpr (G::FIX) {
     if (mol_rk4_fix_ghosts<G, 3>(Ya, o0, o1, o2, t + dt, ep.p, jp)) __syncthreads();
   }
-  mol_rk4_stage<G, 4>(Ya, nullptr, U, Ur, Ar, o0, o1, o2, u, unew, ep, jp, t, dt);
+  mol_rk4_stage<G, 4>(Ya, nullptr, U, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt);
 }
 
 // Resident steps of a 1D system held by a single tile: nsub RK4 steps in one launch with the
@@ -842,26 +879,28 @@ __device__ __forceinline__ void mol_rk4_fused_resident(T* U, T* Ya, T* Yb, doubl
       Ar[j][f] = T(0);
     }
   }
+  MolFusedInv<G> Iv;
+  Iv.init(o0, o1, o2, ep.p, jp);
   double t = t0;
   for (int k = 0; k < nsub; ++k) {
     const bool last = k == nsub - 1;
-    mol_rk4_stage<G, 1>(U, Ya, U, Ur, Ar, o0, o1, o2, u, unew, ep, jp, t, dt);
+    mol_rk4_stage<G, 1>(U, Ya, U, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt);
     __syncthreads();
     if constexpr (G::FIX) {
       if (mol_rk4_fix_ghosts<G, 1>(Ya, o0, o1, o2, t + dt / 2, ep.p, jp)) __syncthreads();
     }
-    mol_rk4_stage<G, 2>(Ya, Yb, U, Ur, Ar, o0, o1, o2, u, unew, ep, jp, t, dt);
+    mol_rk4_stage<G, 2>(Ya, Yb, U, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt);
     __syncthreads();
     if constexpr (G::FIX) {
       if (mol_rk4_fix_ghosts<G, 2>(Yb, o0, o1, o2, t + dt / 2, ep.p, jp)) __syncthreads();
     }
-    mol_rk4_stage<G, 3>(Yb, Ya, U, Ur, Ar, o0, o1, o2, u, unew, ep, jp, t, dt);
+    mol_rk4_stage<G, 3>(Yb, Ya, U, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt);
     __syncthreads();
     if constexpr (G::FIX) {
       if (mol_rk4_fix_ghosts<G, 3>(Ya, o0, o1, o2, t + dt, ep.p, jp)) __syncthreads();
     }
     // the stage-4 frame is empty: U is only read at tile positions by stage 4 (through Ur)
-    mol_rk4_stage<G, 4>(Ya, nullptr, U, Ur, Ar, o0, o1, o2, u, unew, ep, jp, t, dt, U, last);
+    mol_rk4_stage<G, 4>(Ya, nullptr, U, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt, U, last);
     t = t + dt;
     if (last) break;
     __syncthreads();
