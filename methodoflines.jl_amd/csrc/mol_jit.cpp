@@ -1868,19 +1868,19 @@ std::string Gen::source_class(const std::string& name) {
   c("VOL", pg_.vol);
   c("NTHR", 256);
   o << stage;
-  o << "  static __device__ __forceinline__ void stage_pos(const BInst& hd, T* L, const int q, int p0, int p1, int p2,\n"
+  // values of the staged variables at one tile + halo position, v[slot]: plain loads (after the
+  // periodic wrap) in simple tiles, else through the ghost closures
+  o << "  static __device__ __forceinline__ void stage_vals(const BInst& hd, T* v, int p0, int p1, int p2,\n"
        "      const bool simple, const double t, const T* __restrict__ P, const T* __restrict__ u, const JPtrs& jp) {\n"
-       "    (void)t; (void)P; (void)jp; (void)u; (void)simple; (void)hd;\n"
+       "    (void)t; (void)P; (void)jp; (void)u; (void)simple; (void)hd; (void)v;\n"
        "    if (simple) {\n";
   for (int a = 0; a < 3; ++a)
     if (g_.periodic[a]) o << "      p" << a << " = mol_wrap(mol_wrap(p" << a << ", hd.n[" << a << "]), hd.n[" << a << "]);\n";
   for (int f = 0; f < d_->nfields; ++f)
-    if (pg_.slot[f] >= 0)
-      o << "      L[" << (long long)pg_.slot[f] * pg_.vol << " + q] = u[mol_b_uidx(hd, " << f << ", p0, p1, p2)];\n";
+    if (pg_.slot[f] >= 0) o << "      v[" << pg_.slot[f] << "] = u[mol_b_uidx(hd, " << f << ", p0, p1, p2)];\n";
   o << "    } else {\n";
   for (int f = 0; f < d_->nfields; ++f)
-    if (pg_.slot[f] >= 0)
-      o << "      L[" << (long long)pg_.slot[f] * pg_.vol << " + q] = stage(hd, " << f << ", p0, p1, p2, t, P, u, jp);\n";
+    if (pg_.slot[f] >= 0) o << "      v[" << pg_.slot[f] << "] = stage(hd, " << f << ", p0, p1, p2, t, P, u, jp);\n";
   o << "    }\n  }\n";
   o << "  template <bool CORE>\n"
        "  static __device__ __forceinline__ void node(const BInst& hd, const T* __restrict__ L, const int b, const int I0,\n"

@@ -1189,13 +1189,41 @@ __device__ __forceinline__ void mol_batch_tile(T* L, const BInst& hd, const int 
   jp.coord[2] = hd.coord[2];
   const T* P = nullptr;
   constexpr int EXY = G::EX * G::EY;
-  if (G::NST > 0) {
-    for (int q = threadIdx.x; q < G::VOL; q += G::NTHR) {
-      const int lz = q / EXY;
-      const int r = q - lz * EXY;
-      const int ly = r / G::EX;
-      const int lx = r - ly * G::EX;
-      G::stage_pos(hd, L, q, o0 + lx - G::HX, o1 + ly - G::HY, o2 + lz - G::HZ, simple, t, P, u, jp);
+  if constexpr (G::NST > 0) {
+    // every load of the thread is issued before its first LDS store (one exposed latency per
+    // tile), unless the positions per thread are many
+    constexpr int IT = (G::VOL + G::NTHR - 1) / G::NTHR;
+    constexpr int NS = G::NST > 0 ? G::NST : 1;
+    if constexpr (IT * G::NST <= 16) {
+      T v[IT][NS];
+#pragma unroll
+      for (int k = 0; k < IT; ++k) {
+        const int q = (int)threadIdx.x + k * G::NTHR;
+        if (G::VOL % G::NTHR != 0 && q >= G::VOL) break;
+        const int lz = q / EXY;
+        const int r = q - lz * EXY;
+        const int ly = r / G::EX;
+        const int lx = r - ly * G::EX;
+        G::stage_vals(hd, v[k], o0 + lx - G::HX, o1 + ly - G::HY, o2 + lz - G::HZ, simple, t, P, u, jp);
+      }
+#pragma unroll
+      for (int k = 0; k < IT; ++k) {
+        const int q = (int)threadIdx.x + k * G::NTHR;
+        if (G::VOL % G::NTHR != 0 && q >= G::VOL) break;
+#pragma unroll
+        for (int s = 0; s < G::NST; ++s) L[s * G::VOL + q] = v[k][s];
+      }
+    } else {
+      for (int q = threadIdx.x; q < G::VOL; q += G::NTHR) {
+        const int lz = q / EXY;
+        const int r = q - lz * EXY;
+        const int ly = r / G::EX;
+        const int lx = r - ly * G::EX;
+        T v[NS];
+        G::stage_vals(hd, v, o0 + lx - G::HX, o1 + ly - G::HY, o2 + lz - G::HZ, simple, t, P, u, jp);
+#pragma unroll
+        for (int s = 0; s < G::NST; ++s) L[s * G::VOL + q] = v[s];
+      }
     }
     __syncthreads();
   }
