@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define MOL_ABI_VERSION 2
+#define MOL_ABI_VERSION 3
 #define MOL_MAX_DIMS 3
 #define MOL_MAX_FIELDS 8
 #define MOL_MAX_AUX 8
@@ -116,13 +116,18 @@ typedef struct {
  * ghost g = 0 is the node adjacent to the box, g = nghost-1 the domain edge node.
  * G is program `gprog` (or 0 when gprog = -1).  This is what mtkcompile's alias
  * elimination leaves of the BC / extrapolation equations of
- * src/discretization/generate_bc_eqs.jl:24-58,238-311,336-392. */
+ * src/discretization/generate_bc_eqs.jl:24-58,238-311,336-392.
+ * An interface between two variables on adjacent ranges of one axis (src >= 0, K = 0,
+ * gprog = -1): the nghost positions past this side of the box hold variable `src` at the same
+ * position (_wrapinterface, src/discretization/interface_boundary.jl:79-111; the eliminated
+ * lower node of the upper variable is the alias of generate_bc_eqs.jl:35-58). */
 typedef struct {
   int32_t field, axis, side;  /* side 0 = lower, 1 = upper */
   int32_t nghost, K;
   int32_t wC;                 /* pool offset of C (nghost*K) */
   int32_t wc;                 /* pool offset of c (nghost) */
   int32_t gprog;              /* program id of G or -1 */
+  int32_t src;                /* interface: the variable read past this side, else -1 */
 } mol_closure;
 
 /* ---- pointwise programs ---------------------------------------------------------

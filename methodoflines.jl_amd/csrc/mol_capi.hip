@@ -243,6 +243,8 @@ int validate(const mol_desc* d, std::string& msg) {
     const mol_closure& cl = d->closures_h[c];
     if (cl.field < 0 || cl.field >= d->nfields || cl.axis < 0 || cl.axis >= d->ndim || cl.side < 0 || cl.side > 1) { msg = "closure target"; return MOL_ERR_INVALID; }
     if (cl.K < 0 || cl.K > MOL_MAX_CLOSURE_K || cl.nghost < 0 || cl.nghost > MOL_MAX_GHOST) { msg = "closure size"; return MOL_ERR_INVALID; }
+    if (cl.src < -1 || cl.src >= d->nfields || cl.src == cl.field) { msg = "interface variable out of range"; return MOL_ERR_INVALID; }
+    if (cl.src >= 0 && (cl.K != 0 || cl.gprog >= 0 || cl.nghost < 1)) { msg = "interface closure"; return MOL_ERR_INVALID; }
     int64_t boxlen = d->hi[cl.field][cl.axis] - d->lo[cl.field][cl.axis] + 1;
     if (cl.K > boxlen) { msg = "closure reaches past the unknown box"; return MOL_ERR_SHAPE; }
     if (cl.wC + cl.nghost * cl.K > d->npool || cl.wc + cl.nghost > d->npool) { msg = "closure weights out of the pool"; return MOL_ERR_INVALID; }
@@ -421,7 +423,15 @@ void interior_range(const mol_optab& tb, int n, int& lo, int& hi) {
 
 // The descriptor uses operators only the generated kernels implement: nonuniform WENO5 or
 // per-node weight rows (nonuniform centered / upwind).
+// Any closure joining two variables (interface boundaries run in generated kernels only).
+bool has_interface(const mol_desc* d) {
+  for (int c = 0; c < d->nclosures; ++c)
+    if (d->closures_h[c].src >= 0) return true;
+  return false;
+}
+
 bool generated_only(const mol_desc* d) {
+  if (has_interface(d)) return true;
   for (int k = 0; k < d->nslots; ++k)
     if (d->slots_h[k].kind == MOL_SLOT_WENO_NU) return true;
   for (int t = 0; t < d->ntabs; ++t)
@@ -650,6 +660,7 @@ int kernel_threads(const JitGeom& g, bool vjp) {
 bool fused_geometry(mol_handle* h, const mol_desc* d, const KArgs<double>& ka, const KCore<double>& kc, int ncore,
                     JitGeom& g, std::string& why) {
   if (d->ndim > 2) { why = "not a 1D/2D system"; return false; }
+  if (has_interface(d)) { why = "interface boundaries between variables"; return false; }
   for (int a = 0; a < d->ndim; ++a)
     if (ka.halo[a] > 2 || (ka.halo[a] > 1 && !ka.periodic[a])) { why = "reach above 1 (2 on periodic axes)"; return false; }
   // a ghost's closure reads unknowns inward from the box edge: they must lie in the tile region
@@ -1376,7 +1387,7 @@ int plan_batched(const mol_desc* const* ds, const double* const* params, int32_t
       return fail(nullptr, MOL_ERR_INVALID, "instances differ in dtype, dimension or number of variables");
     if (ds[i]->naux != 0) return fail(nullptr, MOL_ERR_UNSUPPORTED, "batched instances take no grid fields");
     if (generated_only(ds[i]))
-      return fail(nullptr, MOL_ERR_UNSUPPORTED, "batched instances take no nonuniform-grid operators");
+      return fail(nullptr, MOL_ERR_UNSUPPORTED, "batched instances take no nonuniform-grid operators or interfaces");
     if (ds[i]->nparams > 0 && (!params || !params[i]))
       return fail(nullptr, MOL_ERR_INVALID, "parameter values of instance " + std::to_string(i) + " missing");
   }
@@ -1699,6 +1710,8 @@ int mol_rhs(mol_handle* h, void* du, const void* u, const void* p, double t, voi
 int mol_expand(mol_handle* h, void* full, const void* u, const void* p, double t, void* stream) {
   if (!h) return fail(nullptr, MOL_ERR_INVALID, "handle is NULL");
   if (h->batch) return fail(h, MOL_ERR_UNSUPPORTED, "batched handles have no full-grid expansion");
+  if (has_interface(&h->desc.d))
+    return fail(h, MOL_ERR_UNSUPPORTED, "full-grid expansion of variables joined by an interface");
   if (!full || !u) return fail(h, MOL_ERR_INVALID, "full/u is NULL");
   if (h->nparams > 0 && !p) return fail(h, MOL_ERR_INVALID, "parameters missing");
   if (int rc = check_aux(h)) return rc;

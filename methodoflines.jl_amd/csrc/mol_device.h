@@ -25,7 +25,7 @@ struct Slot {
 static_assert(sizeof(Slot) == sizeof(mol_slot), "slot layout");
 
 struct Clos {
-  int32_t field, axis, side, nghost, K, wC, wc, gprog;
+  int32_t field, axis, side, nghost, K, wC, wc, gprog, src;
 };
 static_assert(sizeof(Clos) == sizeof(mol_closure), "closure layout");
 

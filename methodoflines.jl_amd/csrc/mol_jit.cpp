@@ -132,6 +132,7 @@ class Gen {
   bool ok_ = true;
   bool param_ = false;                // parametric (batched) printing
   bool fused_ = false;                // fused RK4 step printing (coordinates from LDS)
+  bool stage_noif_ = false;           // printing stage_ni (interface sides read nothing)
   // printing overrides: nodes not printed, and nodes referenced by another expression
   std::vector<char> skip_;
   std::map<int, std::string> subst_;
@@ -1154,6 +1155,16 @@ void Gen::print_reverse(std::ostream& o, int r, const std::vector<std::pair<int,
 // (field_at / closure_value of the interpreter; src/discretization/generate_bc_eqs.jl).
 std::string Gen::stage_fn(bool lds) {
   std::ostringstream o;
+  // interfaces between variables: the ghost of one variable is the other variable at the same
+  // position, staged by stage_ni (the same function with interface sides returning 0: the
+  // other variable's far side is never an interface back)
+  bool iface = false;
+  for (int c = 0; c < d_->nclosures; ++c) iface = iface || d_->closures_h[c].src >= 0;
+  if (iface && !lds && !stage_noif_) {
+    stage_noif_ = true;
+    o << stage_fn(false);
+    stage_noif_ = false;
+  }
   // lds: the same values from the stage values in LDS (tile + halo with first position o0, o1,
   // o2), positions unwrapped; only non-periodic axes have ghosts
   auto rd = [&](int f, const std::string& q0, const std::string& q1, const std::string& q2) {
@@ -1166,7 +1177,7 @@ std::string Gen::stage_fn(bool lds) {
          "                               const JPtrs& jp) {\n"
          "    (void)t; (void)P; (void)jp;\n";
   else
-    o << "  static __device__ T stage(" << (param_ ? "const BInst& hd, " : "")
+    o << "  static __device__ T " << (stage_noif_ ? "stage_ni(" : "stage(") << (param_ ? "const BInst& hd, " : "")
       << "const int f, int p0, int p1, int p2, const double t, const T* __restrict__ P,\n"
          "                           const T* __restrict__ u, const JPtrs& jp) {\n"
          "    (void)t; (void)P; (void)jp;\n";
@@ -1204,6 +1215,11 @@ std::string Gen::stage_fn(bool lds) {
           continue;
         }
         const mol_closure& cl = d_->closures_h[cid];
+        if (cl.src >= 0) {
+          if (stage_noif_ || lds) o << "    return T(0);\n    }\n";
+          else o << "    return stage_ni(" << cl.src << ", p0, p1, p2, t, P, u, jp);\n    }\n";
+          continue;
+        }
         std::string G;
         if (cl.gprog >= 0) {
           const int r = new_region();
@@ -1841,7 +1857,14 @@ std::string Gen::source_fused(bool f64) {
 }
 
 std::string Gen::source(bool f64) { return source_kernel(f64, false); }
-std::string Gen::source_vjp(bool f64) { return source_kernel(f64, true); }
+std::string Gen::source_vjp(bool f64) {
+  for (int c = 0; c < d_->nclosures; ++c)
+    if (d_->closures_h[c].src >= 0) {
+      bad("reverse mode across interface boundaries between variables");
+      return "";
+    }
+  return source_kernel(f64, true);
+}
 
 // Class body of batched instances: one geometry (the full reach as halo), numbers from `hd`.
 std::string Gen::source_class(const std::string& name) {

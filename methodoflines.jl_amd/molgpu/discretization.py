@@ -129,6 +129,7 @@ class Closure:
     C: np.ndarray          # nghost x K
     c: np.ndarray          # nghost
     gprog: int             # program id or -1
+    src: int = -1          # interface: the variable read past this side (C, c empty)
 
 
 @dataclass
@@ -159,6 +160,9 @@ class Lowered:
     time: Optional[sp.Symbol]
     tspan: Tuple[float, float]
     rhs_exprs: List = field(default_factory=list)   # the lowered expressions (for inspection)
+    # variables on adjacent ranges of one axis joined by interface conditions: axis, and per
+    # variable its range of combined-grid nodes (1-based, inclusive) and its own spatial symbol
+    interface: Optional[dict] = None
 
     @property
     def nfields(self) -> int:
@@ -220,30 +224,34 @@ class _Lowering:
         if self.t is None:
             raise NotImplementedError("time-independent systems (NonlinearProblem) are not supported")
         self.heads = []
-        self.field_args = None
+        fargs = []
         for dv in sys.dvs:
             if not isinstance(dv, sp.Function) or not isinstance(dv, sp.core.function.AppliedUndef):
                 raise ValueError("dependent variables must be applied functions such as u(t, x)")
             self.heads.append(dv.func)
-            sargs = [a for a in dv.args if a != self.t]
             if self.t not in dv.args:
                 raise ValueError("dependent variable %s does not depend on time" % dv)
-            if self.field_args is None:
-                self.field_args = sargs
-            elif sargs != self.field_args:
-                raise NotImplementedError("all dependent variables must be defined over the same spatial variables")
+            fargs.append([a for a in dv.args if a != self.t])
+        dom = {iv.var: iv for iv in sys.domain}
+        self.domain = dom
+        self.iface = None
+        self.sym_map = {}
+        if all(a == fargs[0] for a in fargs):
+            self.field_args = fargs[0]
+        else:
+            self._setup_interface(fargs)
         self.axes = list(self.field_args)
         self.ndim = len(self.axes)
         if self.ndim < 1 or self.ndim > 3:
             raise NotImplementedError("1 to 3 spatial dimensions are supported")
-        dom = {iv.var: iv for iv in sys.domain}
-        self.domain = dom
         self.coords, self.n = [], []
-        for x in self.axes:
+        for ax, x in enumerate(self.axes):
             if x not in disc.dxs:
                 raise ValueError("Variable %s has no step size" % x)
             dx = disc.dxs[x]
-            if isinstance(dx, (list, tuple, np.ndarray)):
+            if self.iface is not None and ax == self.iface["axis"]:
+                g = self.iface["grid"]
+            elif isinstance(dx, (list, tuple, np.ndarray)):
                 # nonuniform grid: the nodes themselves (generate_grid, discretize_vars.jl:219-233)
                 g = np.asarray(dx, dtype=np.float64).copy()
                 if g.ndim != 1 or len(g) < 5 or not np.all(np.diff(g) > 0):
@@ -285,6 +293,103 @@ class _Lowering:
         self.aux_heads = [self.aux_heads[i] for i in order]
         self.aux_values = [self.aux_values[i] for i in order]
 
+    def _setup_interface(self, fargs):
+        """Variables over different spatial variables: supported when they differ along one axis
+        only, each variable owns one interval along it, the intervals meet end to end and carry
+        grids of one spacing.  The intervals become one combined axis (the first interval's
+        symbol over their union); a variable's range is its own nodes, the shared end node of
+        two neighbours counted in both (interface_boundary.jl:79-111, generate_bc_eqs.jl:35-58)."""
+        nd = len(fargs[0])
+        if any(len(a) != nd for a in fargs):
+            raise NotImplementedError("all dependent variables must have the same number of spatial variables")
+        diff = [k for k in range(nd) if len({a[k] for a in fargs}) > 1]
+        if len(diff) != 1 or any(a[k] != fargs[0][k] for a in fargs for k in range(nd) if k != diff[0]):
+            raise NotImplementedError("variables over different spatial variables must differ along one axis "
+                                      "(an interface between adjacent intervals)")
+        ia = diff[0]
+        dom = self.domain
+        syms = sorted({a[ia] for a in fargs}, key=lambda x: float(dom[x].lo))
+        owners = {x: [f for f, a in enumerate(fargs) if a[ia] == x] for x in syms}
+        if any(len(v) != 1 for v in owners.values()):
+            raise NotImplementedError("one variable per interval along an interface axis")
+        for x0, x1 in zip(syms, syms[1:]):
+            if abs(float(dom[x0].hi) - float(dom[x1].lo)) > 1e-12 * max(1.0, abs(float(dom[x0].hi))):
+                raise NotImplementedError("intervals of %s and %s do not meet" % (x0, x1))
+        grids, dxs = [], []
+        for x in syms:
+            dx = self.disc.dxs.get(x)
+            if dx is None:
+                raise ValueError("Variable %s has no step size" % x)
+            if isinstance(dx, (list, tuple, np.ndarray)):
+                raise NotImplementedError("interfaces between nonuniform grids are not supported (%s)" % x)
+            g = uniform_grid(dom[x].lo, dom[x].hi, dx)
+            grids.append(g)
+            dxs.append((float(dom[x].hi) - float(dom[x].lo)) / (len(g) - 1))
+        if max(dxs) - min(dxs) > 1e-12 * max(dxs):
+            # the reference's interior stencils wrap by index across an interface
+            # (interface_boundary.jl:79-111): only one spacing keeps them consistent
+            raise NotImplementedError("interfaces between grids of different spacing are not supported")
+        combined = np.concatenate([grids[0]] + [g[1:] for g in grids[1:]])
+        start, end, k = [], [], 1
+        for g in grids:
+            start.append(k)
+            end.append(k + len(g) - 1)
+            k += len(g) - 1
+        order = [owners[x][0] for x in syms]
+        X = syms[0]
+        self.sym_map = {x: X for x in syms[1:]}
+        self.field_args = [X if k == ia else fargs[order[0]][k] for k in range(nd)]
+
+        class _Iv:
+            pass
+        iv = _Iv()
+        iv.var, iv.lo, iv.hi = X, dom[syms[0]].lo, dom[syms[-1]].hi
+        self.domain = dict(dom)
+        self.domain[X] = iv
+        nf = len(fargs)
+        rng = [None] * nf
+        lohi = [None] * nf
+        for j, f in enumerate(order):
+            rng[f] = (start[j], end[j])
+            lohi[f] = (float(dom[syms[j]].lo), float(dom[syms[j]].hi))
+        self.iface = dict(axis=ia, grid=combined, range=rng, coord_range=lohi,
+                          symbols=[fargs[f][ia] for f in range(nf)], dx=dxs[0])
+
+    def _interface_bc(self, bc, lhs, rhs, spos, axis):
+        """u1(t, .., m, ..) ~ u2(t, .., m, ..) at the point m where the interval of one variable
+        ends and the other's begins.  The lower variable keeps m as an unknown, the upper one
+        takes it as the alias u2[first] = u1[last] (clip_interior!!, interior_map.jl:1-10;
+        generate_bc_eqs.jl:35-58)."""
+        if self.iface is None or axis != self.iface["axis"]:
+            raise NotImplementedError("interface condition away from an interface axis: %s" % bc)
+        fa, fb = self.heads.index(lhs.func), self.heads.index(rhs.func)
+        if any(lhs.args[i] != rhs.args[i] for i in spos if i != spos[axis]):
+            raise NotImplementedError("unsupported interface condition %s" % bc)
+        m = float(lhs.args[spos[axis]])
+        if abs(float(rhs.args[spos[axis]]) - m) > 1e-12 * max(1.0, abs(m)):
+            raise NotImplementedError("interface condition at two different points: %s" % bc)
+        cr = self.iface["coord_range"]
+        tol = 1e-12 * max(1.0, abs(m))
+        if abs(cr[fa][1] - m) <= tol and abs(cr[fb][0] - m) <= tol:
+            lower, upper = fa, fb
+        elif abs(cr[fb][1] - m) <= tol and abs(cr[fa][0] - m) <= tol:
+            lower, upper = fb, fa
+        else:
+            raise NotImplementedError("interface condition away from the meeting point of %s and %s: %s"
+                                      % (self.heads[fa], self.heads[fb], bc))
+        self.bc[lower][axis][1] = ("iface", upper)
+        self.bc[upper][axis][0] = ("iface", lower)
+
+    def _is_iface(self, f, a, side):
+        b = self.bc[f][a][side]
+        return isinstance(b, tuple) and len(b) == 2 and b[0] == "iface"
+
+    def _range(self, f, a):
+        """Nodes (1-based, inclusive) of variable f along axis a."""
+        if self.iface is not None and a == self.iface["axis"]:
+            return self.iface["range"][f]
+        return 1, self.n[a]
+
     # ---------------------------------------------------------------- boundary conditions
     def parse_bcs(self):
         """Classify each condition: initial value, Dirichlet/Neumann/Robin (affine), periodic."""
@@ -296,6 +401,8 @@ class _Lowering:
         t0 = self.tspan[0]
         for bc in self.sys.bcs:
             lhs, rhs = bc.lhs, bc.rhs
+            if self.sym_map:
+                lhs, rhs = sp.sympify(lhs).xreplace(self.sym_map), sp.sympify(rhs).xreplace(self.sym_map)
             apps = _applied_heads(sp.Add(lhs, rhs, evaluate=False), self.heads)
             if not apps:
                 raise ValueError("condition %s references no dependent variable" % bc)
@@ -304,7 +411,8 @@ class _Lowering:
             tpos = list(dv.args).index(self.t)
             spos = [i for i in range(len(dv.args)) if i != tpos]
             # initial condition: u(t0, x...) ~ expr(x...)
-            if lhs in apps and len(apps) == 1 and lhs.args[tpos] != self.t:
+            if lhs in apps and len(apps) == 1 and lhs.args[tpos] != self.t and not (
+                    isinstance(rhs, sp.core.function.AppliedUndef) and rhs.func in self.heads):
                 tv = lhs.args[tpos]
                 if not tv.is_number or abs(float(tv) - t0) > 1e-12:
                     raise ValueError("initial condition not at the domain start: %s" % bc)
@@ -321,11 +429,14 @@ class _Lowering:
             val = float(val)
             x = self.axes[axis]
             lo, hi = self.domain[x].lo, self.domain[x].hi
+            if self.iface is not None and axis == self.iface["axis"]:
+                lo, hi = self.iface["coord_range"][f]
             # periodic: u(.., lo, ..) ~ u(.., hi, ..)  (an interface of a variable with itself)
             if (isinstance(lhs, sp.core.function.AppliedUndef) and isinstance(rhs, sp.core.function.AppliedUndef)
                     and lhs.func in self.heads and rhs.func in self.heads):
                 if lhs.func != rhs.func:
-                    raise NotImplementedError("interface boundary conditions between variables are not supported: %s" % bc)
+                    self._interface_bc(bc, lhs, rhs, spos, axis)
+                    continue
                 av = sorted([float(lhs.args[spos[axis]]), float(rhs.args[spos[axis]])])
                 if av != sorted([float(lo), float(hi)]):
                     raise NotImplementedError("unsupported interface condition %s" % bc)
@@ -364,6 +475,12 @@ class _Lowering:
                 coefs[k] = float(c)
             g = sp.sympify(-R2.xreplace({s: 0 for s in Ukeys.values()}))
             self.bc[f][axis][side] = (coefs, g)
+        if self.iface is not None:
+            ax = self.iface["axis"]
+            for f in range(nf):
+                s0, s1 = self.iface["range"][f]
+                if (s0 > 1 and self.bc[f][ax][0] is None) or (s1 < self.n[ax] and self.bc[f][ax][1] is None):
+                    raise NotImplementedError("%s meets another variable without an interface condition" % self.heads[f])
         for f in range(nf):
             if self.ic[f] is None:
                 raise ValueError("no initial condition for %s" % self.heads[f])
@@ -698,8 +815,11 @@ class _Lowering:
         for f in range(nf):
             pde = self.eq_of[f]
             for a in range(self.ndim):
+                # an interface clips the lower node of the upper variable (its alias) and keeps the
+                # upper node of the lower one (clip_interior!!, interior_map.jl:1-10)
+                up_if = self._is_iface(f, a, 1)
                 cl = 1 if (self.periodic[a] or self.bc[f][a][0] is not None) else 0
-                cu = 1 if (not self.periodic[a] and self.bc[f][a][1] is not None) else 0
+                cu = 1 if (not self.periodic[a] and self.bc[f][a][1] is not None and not up_if) else 0
                 self.vlower[f][a], self.vupper[f][a] = cl, cu
                 ext = 0
                 x = self.axes[a]
@@ -711,10 +831,13 @@ class _Lowering:
                                 if cnt == 1 and isinstance(self.scheme, WENOScheme) and not self.nonuniform[a]:
                                     ext = max(ext, 2)
                 el, eu = (ext if not self.periodic[a] else 0), (ext if not self.periodic[a] else 0)
+                if ext and (self._is_iface(f, a, 0) or up_if):
+                    raise NotImplementedError("WENO extrapolated nodes next to an interface are not supported")
                 self.extent[f][a] = (el, eu)
-                self.lo[f][a] = 1 + max(cl, el)
-                self.hi[f][a] = self.n[a] - max(cu, eu)
-                if self.lo[f][a] + max(cu, eu) > self.n[a]:
+                r0, r1 = self._range(f, a)
+                self.lo[f][a] = r0 + max(cl, el)
+                self.hi[f][a] = r1 - max(cu, eu)
+                if self.lo[f][a] + max(cu, eu) > r1:
                     raise ValueError("The domain is too small to support the requested discretization")
 
     # ---------------------------------------------------------------- ghost closures
@@ -736,14 +859,21 @@ class _Lowering:
                 if self.periodic[a]:
                     continue
                 n = self.n[a]
+                r0, r1 = self._range(f, a)
                 for side in (0, 1):
                     lo, hi = self.lo[f][a], self.hi[f][a]
-                    ghosts = list(range(1, lo)) if side == 0 else list(range(hi + 1, n + 1))
+                    if self._is_iface(f, a, side):
+                        # positions past the box on this side hold the other variable
+                        reach = max(1, self.reach[a])
+                        out.append(Closure(f, a, side, np.zeros((reach, 0)), np.zeros(reach), -1,
+                                           src=self.bc[f][a][side][1]))
+                        continue
+                    ghosts = list(range(r0, lo)) if side == 0 else list(range(hi + 1, r1 + 1))
                     if not ghosts:
                         continue
                     rows = {g: [] for g in ghosts}   # g -> list of (coef dict node->w, gcoef)
                     bc = self.bc[f][a][side]
-                    bnode = 1 if side == 0 else n
+                    bnode = r0 if side == 0 else r1
                     if bc is not None:
                         coefs, gexpr = bc
                         row = {}
@@ -908,6 +1038,8 @@ def lower(pdesys: PDESystem, disc: MOLFiniteDifference) -> Lowered:
     L.eq_of = [None] * nf
     # map equations to variables by their time derivative (the tearing of interior_map.jl:53-103)
     for eq in pdesys.eqs:
+        if L.sym_map:
+            eq = sp.Eq(sp.sympify(eq.lhs).xreplace(L.sym_map), sp.sympify(eq.rhs).xreplace(L.sym_map), evaluate=False)
         fvar, r = L.lower_equation(eq)
         if rhs[fvar] is not None:
             raise ValueError("two equations solve for %s" % L.heads[fvar])
@@ -923,8 +1055,9 @@ def lower(pdesys: PDESystem, disc: MOLFiniteDifference) -> Lowered:
             extra[sp.Symbol("__S%d" % k)] = ("SLOT", k)
         L.rhs_exprs.append(expr)
         rhs_prog.append(L.add_program(expr, extra))
-    closures = L.closures()
     halo = L.halo()
+    L.reach = halo
+    closures = L.closures()
     # initial values (generate_ic_defaults.jl:1-27), evaluated in FP64 at every unknown
     u0 = []
     for f in range(nf):
@@ -944,4 +1077,6 @@ def lower(pdesys: PDESystem, disc: MOLFiniteDifference) -> Lowered:
         tables=L.tables, slots=L.slots, closures=closures, programs=L.programs,
         cpool=L.cpool.values, rhs_prog=rhs_prog, params=L.params, param_values=L.param_values,
         aux=[h.__name__ for h in L.aux_heads], aux_heads=L.aux_heads, dtype=disc.dtype,
-        u0=u0, time=L.t, tspan=L.tspan, rhs_exprs=L.rhs_exprs)
+        u0=u0, time=L.t, tspan=L.tspan, rhs_exprs=L.rhs_exprs,
+        interface=None if L.iface is None else dict(axis=L.iface["axis"], range=L.iface["range"],
+                                                   symbols=L.iface["symbols"]))

@@ -11,7 +11,7 @@ from typing import List
 
 import numpy as np
 
-MOL_ABI_VERSION = 2
+MOL_ABI_VERSION = 3
 MOL_MAX_DIMS = 3
 MOL_MAX_FIELDS = 8
 MOL_MAX_AUX = 8
@@ -44,7 +44,7 @@ class mol_slot(C.Structure):
 
 
 class mol_closure(C.Structure):
-    _fields_ = [(n, C.c_int32) for n in ("field", "axis", "side", "nghost", "K", "wC", "wc", "gprog")]
+    _fields_ = [(n, C.c_int32) for n in ("field", "axis", "side", "nghost", "K", "wC", "wc", "gprog", "src")]
 
 
 class mol_desc(C.Structure):
@@ -199,6 +199,7 @@ def pack_descriptor(low) -> tuple:
         c.wc = len(pool)
         pool.extend(float(v) for v in cl.c)
         c.gprog = cl.gprog
+        c.src = getattr(cl, "src", -1)
     code: List[int] = []
     progs: List[int] = []
     for pr in low.programs:

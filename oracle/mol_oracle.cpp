@@ -999,6 +999,61 @@ int orc_diffusion1d_rhs(int n, double xlo, double xhi, int p, double D, const do
   }
 }
 
+// ---- two variables on adjacent intervals joined by an interface condition c1(t, m) ~ c2(t, m):
+//      Dt c_k = D_k Dxx c_k, c1 on n1 nodes of [a, m], c2 on n2 nodes of [m, b] (same spacing
+//      dx), Dirichlet c1 = gl at a and c2 = gr at b.  The interface keeps the upper node of c1 as
+//      an unknown (clip_interior!!, interior_map.jl:1-10: an upper interface clips nothing) and
+//      eliminates the lower node of c2 by c2[1] = c1[n1] (generate_bc_eqs.jl:35-58).  Stencils
+//      reaching past the interface read the other variable (_wrapinterface,
+//      interface_boundary.jl:79-111): c1 at node i > n1 is c2 at i - n1 + 1, c2 at node k <= 1 is
+//      c1 at k + n1 - 1; next to an interface the interior row is used (no boundary rows,
+//      centered_difference.jl:5-30 with haslower / hasupper).  State: c1 at 2..n1, then c2 at
+//      2..n2-1.
+extern "C" int orc_interface1d_rhs(int n1, int n2, double dx, int p, double D1, double D2, double gl, double gr,
+                                   const double* u, double t, double* du) {
+  (void)t;
+  try {
+    if (n1 < 3 || n2 < 3) return -1;
+    const Op op = centered(2, p, dx);
+    const double* u1 = u;             // c1 nodes 2..n1
+    const double* u2 = u + (n1 - 1);  // c2 nodes 2..n2-1
+    std::function<double(int)> get1, get2;
+    get1 = [&](int i) -> double {
+      if (i == 1) return gl;
+      if (i >= 2 && i <= n1) return u1[i - 2];
+      if (i > n1) return get2(i - n1 + 1);
+      throw std::runtime_error("c1 read outside its grid");
+    };
+    get2 = [&](int k) -> double {
+      if (k <= 1) return get1(k + n1 - 1);
+      if (k <= n2 - 1) return u2[k - 2];
+      if (k == n2) return gr;
+      throw std::runtime_error("c2 read outside its grid");
+    };
+    const int bpc = op.bpc, half = op.L / 2;
+    for (int i = 2; i <= n1; ++i) {
+      // lower end Dirichlet: boundary rows; upper end an interface: interior rows with wrapping
+      const std::vector<double>& w = i <= bpc ? op.low[i - 1] : op.st;
+      const int first = i <= bpc ? 1 : i - half;
+      double s = 0.0;
+      for (size_t k = 0; k < w.size(); ++k) s += w[k] * get1(first + (int)k);
+      du[i - 2] = D1 * s;
+    }
+    for (int k = 2; k <= n2 - 1; ++k) {
+      // lower end an interface: interior rows; upper end Dirichlet: boundary rows
+      const bool high = k > n2 - bpc;
+      const std::vector<double>& w = high ? op.high[n2 - k] : op.st;
+      const int first = high ? n2 - op.BL + 1 : k - half;
+      double s = 0.0;
+      for (size_t m = 0; m < w.size(); ++m) s += w[m] * get2(first + (int)m);
+      du[(n1 - 1) + (k - 2)] = D2 * s;
+    }
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
 // ---- u_t = D u_xx - c u_x on the nodes x[0..n-1] (nonuniform), approximation order p; the
 //      advection term upwinded with order uo (cardinalized coefficient c: backward rows where
 //      c > 0, upwind_difference.jl:184-199) or, with uo = 0, centred; a0 u + a1 u_x = g at each end.

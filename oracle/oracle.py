@@ -53,6 +53,8 @@ def lib():
             "orc_nonlindiff3d_jvp": [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_longlong, C.POINTER(C.c_double), C.POINTER(C.c_double)],
             "orc_upwind1d_rhs": [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp, C.c_double, _dp],
             "orc_mixed2d_rhs": [C.c_int, C.c_int, C.c_double, _dp, C.c_double, _dp],
+            "orc_interface1d_rhs": [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, _dp, C.c_double, _dp],
             "orc_mixed2d_grid_rhs": [_ip, C.POINTER(C.c_void_p), C.c_int, C.c_double, _dp, _dp, C.c_double, _dp],
             "orc_spherical1d_rhs": [C.c_int, C.c_int, C.c_double, _dp, C.c_double, _dp],
             "orc_nonlinlap1d_rhs": [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp, C.c_double, _dp],
@@ -269,6 +271,17 @@ def mixed2d_grid_rhs(grids, c, u, p=2, q=(0.0, 1.0, 2.0, 0.0, 0.0, 0.0), t=0.0):
     u = _state(u)
     du = np.empty_like(u)
     _ok(lib().orc_mixed2d_grid_rhs(n, xs, int(p), float(c), _f64(q), u, float(t), du))
+    return du
+
+
+def interface1d_rhs(n1, n2, dx, D1, D2, u, p=2, gl=0.0, gr=0.0, t=0.0):
+    """c1 on n1 nodes and c2 on n2 nodes of one spacing dx, joined at c1's last / c2's first node
+    by an interface condition; Dt c_k = D_k Dxx c_k, Dirichlet gl and gr at the outer ends.
+    State: c1 at nodes 2..n1, then c2 at nodes 2..n2-1."""
+    u = _state(u)
+    du = np.empty_like(u)
+    _ok(lib().orc_interface1d_rhs(int(n1), int(n2), float(dx), int(p), float(D1), float(D2), float(gl), float(gr),
+                                  u, float(t), du))
     return du
 
 
