@@ -492,7 +492,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only (no weno/adjoint/batch sections)")
     ap.add_argument("--section", default="all",
-                    choices=["all", "weno", "wenosolve", "adjoint", "adjoint32", "batch", "rhs3d", "copy", "heat"],
+                    choices=["all", "weno", "weno4096", "wenosolve", "adjoint", "adjoint32", "batch", "rhs3d", "copy",
+                             "heat"],
                     help="run one section alone (for profiling); prints its JSON")
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--reps", type=int, default=200, help="back-to-back RHS launches of the weno/rhs3d sections")
@@ -509,6 +510,8 @@ def main():
     if args.section != "all":
         if args.section == "weno":
             res = {"1024": bench_weno(1024, args.reps)}
+        elif args.section == "weno4096":
+            res = {"4096": bench_weno(4096, args.reps)}
         elif args.section == "wenosolve":
             res = bench_weno_solve(1024, 500, 3)
         elif args.section == "adjoint":
@@ -542,16 +545,33 @@ def main():
     u_start = u.clone()
     fused = prob.plan()["fused_tiles"] > 0 and not args.unfused
 
+    def stepper(use_fused, nsteps):
+        """mol_integrate through the C ABI with its arguments prepared (as a Julia ccall binding
+        would hold them), so the timed region holds the library's own work only."""
+        import ctypes as C
+        from molgpu import native
+        h = prob.handle
+        pp = prob.device_params()
+        flags = (native.MOL_FLAG_GRAPH if args.graph else 0) | (0 if use_fused else native.MOL_FLAG_UNFUSED)
+        a = (h.h, C.c_void_p(u.data_ptr()), C.c_void_p(pp.data_ptr()) if pp is not None else None, C.c_double(0.0),
+             C.c_double(dt), C.c_int64(nsteps), C.c_int32(molgpu.RK4.scheme), C.c_int32(flags),
+             C.c_void_p(stream.cuda_stream))
+
+        def run():
+            native.check(h.lib, h.lib.mol_integrate(*a), h.h)
+        return run
+
     def timed(use_fused):
+        warm, step = stepper(use_fused, max(args.warmup, 1)), stepper(use_fused, args.steps)
         u.copy_(u_start)
-        prob.integrate(u, 0.0, dt, max(args.warmup, 1), molgpu.RK4, graph=bool(args.graph), fused=use_fused)
+        warm()
         torch.cuda.synchronize()
         u.copy_(u_start)
         torch.cuda.synchronize()
         ev0, ev1 = events()
         t0 = time.perf_counter()
         ev0.record(stream)
-        prob.integrate(u, 0.0, dt, args.steps, molgpu.RK4, graph=bool(args.graph), fused=use_fused)
+        step()
         ev1.record(stream)
         torch.cuda.synchronize()
         return ev0.elapsed_time(ev1), time.perf_counter() - t0

@@ -315,6 +315,14 @@ int mol_forward_checkpointed(mol_handle* h, void* u, const void* p, double t0, d
  * VJPs per step; after a ckpt_every = 0 run the sweep uses the stored stage states. */
 int mol_adjoint(mol_handle* h, const void* p, void* lam, double* g_aux, void* stream);
 
+/* FP64 error norms of a state against a reference, the counterpart of the final-time error checks
+ * the reference's tests make on the CPU (isapprox against the analytic solution,
+ * test/Diffusion/MOL_1D_Linear_Diffusion.jl:82,128): out (device,
+ * 3 doubles) = max |u - ref|, ||u - ref||_2, ||ref||_2 over the unknowns, accumulated in FP64 with
+ * 64-lane wavefront reductions in a fixed order.  u: device state of the handle's dtype; ref: device
+ * FP64 array of the same length, or NULL (the norms of u).  Asynchronous on `stream`. */
+int mol_error_norms(mol_handle* h, const void* u, const double* ref, double* out, void* stream);
+
 /* Number of unknowns (sum over variables of the unknown box sizes). */
 int64_t mol_num_unknowns(const mol_handle* h);
 

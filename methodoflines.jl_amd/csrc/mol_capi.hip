@@ -113,7 +113,8 @@ void free_handle(mol_handle* h) {
                   h->d_prog, h->d_coord[0], h->d_coord[1], h->d_coord[2], h->w_y1,
                   h->w_y2, h->w_acc, h->w_comp, h->d_tnow, h->d_ka64, h->d_ka32, h->d_tiles,
                   h->ck_u, h->ck_t, h->seg_u, h->seg_t, h->adj_y[0], h->adj_y[1], h->adj_y[2], h->adj_y[3],
-                  h->d_insts, h->d_wl, h->d_bdata, h->vjp_bf, h->vjp_ba, h->r_u, h->r_du, h->r_p};
+                  h->d_insts, h->d_wl, h->d_bdata, h->vjp_bf, h->vjp_ba, h->r_u, h->r_du, h->r_p,
+                  h->d_norm_part};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete h;
@@ -1039,11 +1040,11 @@ bool prog_uses(const mol_desc* d, int prog, uint32_t op) {
 // Generate and load the vector-Jacobian product kernel (first use).  Its tiles are sized for
 // the state, the auxiliary arrays and their adjoint buffers in LDS.
 // Tiling of the reverse-mode kernel: LDS holds the state, the grid fields and their adjoint
-// buffers (FP64 for the grid fields).
+// buffers (FP64 for every array).
 JitGeom vjp_geometry(mol_handle* h, const mol_desc* d, int* ntiles) {
   KArgs<double> kv;
   // staged arrays of the reverse-mode kernel, estimated first with every array staged
-  fill_kargs<double>(h, d, kv, (2 * (size_t)d->nfields + d->naux) * h->esize() + (size_t)d->naux * sizeof(double));
+  fill_kargs<double>(h, d, kv, ((size_t)d->nfields + d->naux) * (h->esize() + sizeof(double)));
   {
     KCore<double> kc0;
     std::vector<int32_t> gen0;
@@ -1826,6 +1827,19 @@ int mol_adjoint(mol_handle* h, const void* p, void* lam, double* g_aux, void* st
   hipStream_t s = (hipStream_t)stream;
   if (h->dtype == MOL_F64) return adjoint_impl<double>(h, (const double*)p, (double*)lam, g_aux, s);
   return adjoint_impl<float>(h, (const float*)p, (float*)lam, g_aux, s);
+}
+
+int mol_error_norms(mol_handle* h, const void* u, const double* ref, double* out, void* stream) {
+  if (!h) return fail(nullptr, MOL_ERR_INVALID, "handle is NULL");
+  if (!u || !out) return fail(h, MOL_ERR_INVALID, "u/out is NULL");
+  if (!h->d_norm_part) HIPCHK(h, hipMalloc((void**)&h->d_norm_part, 3 * kNormBlocks * sizeof(double)));
+  hipStream_t s = (hipStream_t)stream;
+  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kNormBlocks, (h->nunk + 255) / 256));
+  const int rc = h->dtype == MOL_F64
+                     ? launch_norms<double>((const double*)u, ref, h->nunk, h->d_norm_part, nb, out, s)
+                     : launch_norms<float>((const float*)u, ref, h->nunk, h->d_norm_part, nb, out, s);
+  if (rc) return fail(h, rc, std::string("norm launch: ") + hipGetErrorString(hipGetLastError()));
+  return 0;
 }
 
 int64_t mol_num_unknowns(const mol_handle* h) { return h ? h->nunk : -1; }

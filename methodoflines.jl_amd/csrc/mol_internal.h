@@ -107,6 +107,11 @@ template <typename T>
 int launch_adjoint_sum(T* lam, T* y0, T* y1, T* y2, T* y3, int64_t n, bool zero, hipStream_t stream);
 // *slot = t on `stream` (t passed by value in the launch)
 int launch_set_clock(double* slot, double t, hipStream_t stream);
+// FP64 norms of u - ref over n values (ref may be null): out[0..2] = max |u - ref|,
+// ||u - ref||_2, ||ref||_2; part holds 3 * nb doubles of per-block partials
+template <typename T>
+int launch_norms(const T* u, const double* ref, int64_t n, double* part, int nb, double* out, hipStream_t stream);
+constexpr int kNormBlocks = 1024;
 
 // Set the message of the calling thread's last failure (used when no handle exists yet).
 void set_global_error(const std::string& msg);
@@ -186,6 +191,7 @@ struct mol_handle {
   std::string fused_reason;               // why there is no fused kernel
   bool no_fuse = false;                   // this integration runs the stage kernels (MOL_FLAG_UNFUSED)
   int vjp_nthr = 256;                     // threads per work-group of the generated VJP kernel
+  double* d_norm_part = nullptr;          // per-block partials of mol_error_norms
   hipFunction_t vjp_fn = nullptr;         // generated vector-Jacobian product kernel
   bool vjp_store = false;                 // its gather stores every unknown (no zeroing of the output)
   hipFunction_t vjp_gather_fn = nullptr;  // its gather of the per-tile adjoint buffers

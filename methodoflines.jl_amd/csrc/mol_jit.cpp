@@ -1141,7 +1141,7 @@ void Gen::print_reverse(std::ostream& o, int r, const std::vector<std::pair<int,
     const long long off = kv.first[1] + (long long)pg_.ext[0] * (kv.first[2] + (long long)pg_.ext[1] * kv.first[3]);
     const std::string pos = off == 0 ? "b" : (off > 0 ? "b + " + istr(off) : "b - " + istr(-off));
     if (field)
-      o << "    atomicAdd(&Ab[" << (long long)sl * pg_.vol << " + " << pos << "], " << kv.second << ");\n";
+      o << "    atomicAdd(&Ab[" << (long long)sl * pg_.vol << " + " << pos << "], (double)(" << kv.second << "));\n";
     else
       aux_adds << "      atomicAdd(&Aa[" << (long long)sl * pg_.vol << " + " << pos << "], (double)(" << kv.second
                << "));\n";
@@ -1581,12 +1581,12 @@ std::string Gen::geo_variant(const std::string& name, const int* halo, const Usa
     o << "  }\n";
     if (fused_) o << node_hoisted(region, outs);
   } else {
-    o << "  static __device__ __forceinline__ void zero_pos(T* Ab, double* Aa, const int q) {\n"
+    o << "  static __device__ __forceinline__ void zero_pos(double* Ab, double* Aa, const int q) {\n"
          "    (void)Ab; (void)Aa;\n";
-    for (int k = 0; k < nstf; ++k) o << "    Ab[" << (long long)k * pg_.vol << " + q] = T(0);\n";
+    for (int k = 0; k < nstf; ++k) o << "    Ab[" << (long long)k * pg_.vol << " + q] = 0.0;\n";
     for (int k = 0; k < nsta; ++k) o << "    Aa[" << (long long)k * pg_.vol << " + q] = 0.0;\n";
     o << "  }\n";
-    o << "  static __device__ __forceinline__ void vjp(const T* __restrict__ L, T* Ab, double* Aa, const int b,\n"
+    o << "  static __device__ __forceinline__ void vjp(const T* __restrict__ L, double* Ab, double* Aa, const int b,\n"
          "      const int I0, const int I1, const int I2, const double t, const T* __restrict__ P,\n"
          "      const T* __restrict__ u, const JPtrs& jp, const T* lam, T* __restrict__ outp, double* __restrict__ gaux) {\n"
          "    (void)I0; (void)I1; (void)I2; (void)t; (void)P; (void)jp; (void)L; (void)b; (void)Ab; (void)Aa; (void)u;\n"
@@ -1599,7 +1599,7 @@ std::string Gen::geo_variant(const std::string& name, const int* halo, const Usa
     // adjoints of one tile + halo position: positions of a variable's unknown box (after the
     // periodic wrap) and of the grid go to the tile's buffers (slot stride vs; zero elsewhere),
     // ghost positions through the transposed closures to the unknowns (atomics)
-    o << "  static __device__ __forceinline__ void flush_pos(const T* Ab, const double* Aa, const int q, const int p0,\n"
+    o << "  static __device__ __forceinline__ void flush_pos(const double* Ab, const double* Aa, const int q, const int p0,\n"
          "      const int p1, const int p2, T* __restrict__ outp, T* __restrict__ bft, double* __restrict__ bat,\n"
          "      const int vs, const bool acc) {\n"
          "    (void)Ab; (void)Aa; (void)bat; (void)vs; (void)outp; (void)acc;\n";
@@ -1614,10 +1614,10 @@ std::string Gen::geo_variant(const std::string& name, const int* halo, const Usa
         const long long k = pg_.fslot[f];
         if (gather_stores()) {
           // the gather applies the transposed closures to the ghost positions' buffer entries
-          o << "    bft[" << k << " * vs + q] = Ab[" << k * pg_.vol << " + q];\n";
+          o << "    bft[" << k << " * vs + q] = (T)Ab[" << k * pg_.vol << " + q];\n";
           continue;
         }
-        o << "    { const T v = Ab[" << k * pg_.vol << " + q];\n"
+        o << "    { const T v = (T)Ab[" << k * pg_.vol << " + q];\n"
           << "      if (inbox(" << f << ", w0, w1, w2)) bft[" << k << " * vs + q] = v;\n"
           << "      else { bft[" << k << " * vs + q] = T(0); if (v != T(0)) scatter(" << f << ", p0, p1, p2, v, outp); } }\n";
       }
@@ -1702,7 +1702,7 @@ std::string Gen::source_kernel(bool f64, bool vjp) {
     // tile kernel: reverse sweep of every node, tile + halo adjoints to per-tile buffers
     o << "constexpr int VOLX = " << npf << ";\n\n";
     o << "extern \"C\" __global__ void __launch_bounds__(" << g_.nthr << ") mol_vjp_tiles(VArgs va, JPtrs jp) {\n"
-         "  __shared__ T L[LDSN];\n  __shared__ T Ab[LDSF];\n  __shared__ double Aa[LDSA];\n" << decode
+         "  __shared__ T L[LDSN];\n  __shared__ double Ab[LDSF];\n  __shared__ double Aa[LDSA];\n" << decode
       << "  const double t = (va.tnow ? va.tnow[0] : 0.0) + va.tofs;\n";
     if (g_.ncore > 0)
       o << "  if (GeoBase::is_core(tx, ty, tz)) mol_vjp_tile<GeoC, VOLX>(L, Ab, Aa, tx, ty, tz, va, jp, t);\n  else ";
@@ -2036,7 +2036,7 @@ size_t jit_lds_per_pos(const mol_desc* d, const JitGeom& g, size_t esize, bool v
   size_t nf = 0, na = 0;
   for (int k = 0; k < d->nfields + d->naux; ++k)
     if (us.staged[k]) (k < d->nfields ? nf : na)++;
-  const size_t bytes = vjp ? (2 * nf + na) * esize + na * sizeof(double) : (nf + na) * esize;
+  const size_t bytes = vjp ? (nf + na) * (esize + sizeof(double)) : (nf + na) * esize;
   return bytes > 0 ? bytes : esize;
 }
 

@@ -1082,11 +1082,12 @@ __device__ __forceinline__ double mol_gather_pos(const B* __restrict__ base, con
 }
 
 // One tile of the reverse-mode sweep in tile variant G: stage as the forward kernel, run each
-// node's adjoint (tap adjoints summed in LDS with LDS atomics; arrays read only at the node go
+// node's adjoint (tap adjoints summed in FP64 in LDS with LDS atomics, whatever the state's
+// type; arrays read only at the node go
 // straight to global memory), then add the LDS adjoints of tile + halo to the unknowns (through
 // the transposed ghost closures) and to the grid-field gradients.
 template <class G, int VOLX>
-__device__ __forceinline__ void mol_vjp_tile(T* L, T* Ab, double* Aa, const int tx, const int ty, const int tz,
+__device__ __forceinline__ void mol_vjp_tile(T* L, double* Ab, double* Aa, const int tx, const int ty, const int tz,
                                              const VArgs& va, const JPtrs& jp, const double t) {
   const int o0 = G::ULO0 + tx * G::TX, o1 = G::ULO1 + ty * G::TY, o2 = G::ULO2 + tz * G::TZ;
   constexpr int EXY = G::EX * G::EY;

@@ -287,6 +287,103 @@ int launch_set_clock(double* slot, double t, hipStream_t stream) {
   return hipGetLastError() == hipSuccess ? 0 : MOL_ERR_HIP;
 }
 
+// ---- FP64 error norms ---------------------------------------------------------------------
+// One 64-lane wavefront reduction step: every lane adds (or takes the larger of) its value and
+// the value DPP control CTRL moves into it; lanes whose source is masked off by ROW_MASK see 0,
+// the identity of both reductions here (the values are sums of squares or absolute values).
+// The two 32-bit halves of the double move with the same control.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(const double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum (MAX = false) or maximum (MAX = true) over the 64 lanes of a wavefront, in lane 63:
+// quad_perm [1,0,3,2] and [2,3,0,1] combine lanes within quads, row_ror 4 and 8 within rows of
+// 16, row_bcast 15 adds row r's lane 15 into row r + 1 (rows 1 and 3), row_bcast 31 lane 31
+// into rows 2 and 3.  Returned to every lane by readlane 63.
+template <bool MAX>
+__device__ __forceinline__ double wave_reduce(double v) {
+  auto op = [](double a, double b) { return MAX ? fmax(a, b) : a + b; };
+  v = op(v, dpp_move<0xb1, 0xf>(v));    // quad_perm [1,0,3,2]
+  v = op(v, dpp_move<0x4e, 0xf>(v));    // quad_perm [2,3,0,1]
+  v = op(v, dpp_move<0x124, 0xf>(v));   // row_ror 4
+  v = op(v, dpp_move<0x128, 0xf>(v));   // row_ror 8
+  v = op(v, dpp_move<0x142, 0xa>(v));   // row_bcast 15 into rows 1, 3
+  v = op(v, dpp_move<0x143, 0xc>(v));   // row_bcast 31 into rows 2, 3
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
+}
+
+// Partial norms of blocks of 256 threads: part[3 b + k] for block b, k = 0 max |u - r|,
+// 1 sum (u - r)^2, 2 sum r^2 (r = 0 when ref is null), accumulated in FP64 in a fixed order.
+template <typename T>
+__global__ void __launch_bounds__(256) norms_partial_kernel(const T* __restrict__ u, const double* __restrict__ ref,
+                                                            int64_t n, double* __restrict__ part) {
+  double mx = 0.0, sd = 0.0, sr = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const double r = ref ? ref[i] : 0.0;
+    const double d = (double)u[i] - r;
+    mx = fmax(mx, fabs(d));
+    sd = fma(d, d, sd);
+    sr = fma(r, r, sr);
+  }
+  mx = wave_reduce<true>(mx);
+  sd = wave_reduce<false>(sd);
+  sr = wave_reduce<false>(sr);
+  __shared__ double w[3][4];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) {
+    w[0][wave] = mx;
+    w[1][wave] = sd;
+    w[2][wave] = sr;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x + 0] = fmax(fmax(w[0][0], w[0][1]), fmax(w[0][2], w[0][3]));
+    part[3 * blockIdx.x + 1] = ((w[1][0] + w[1][1]) + w[1][2]) + w[1][3];
+    part[3 * blockIdx.x + 2] = ((w[2][0] + w[2][1]) + w[2][2]) + w[2][3];
+  }
+}
+
+// The partials of nb blocks (one work-group of 256 threads): out = max, sqrt of both sums.
+__global__ void __launch_bounds__(256) norms_final_kernel(const double* __restrict__ part, int nb,
+                                                          double* __restrict__ out) {
+  double mx = 0.0, sd = 0.0, sr = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) {
+    mx = fmax(mx, part[3 * b]);
+    sd += part[3 * b + 1];
+    sr += part[3 * b + 2];
+  }
+  mx = wave_reduce<true>(mx);
+  sd = wave_reduce<false>(sd);
+  sr = wave_reduce<false>(sr);
+  __shared__ double w[3][4];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) {
+    w[0][wave] = mx;
+    w[1][wave] = sd;
+    w[2][wave] = sr;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = fmax(fmax(w[0][0], w[0][1]), fmax(w[0][2], w[0][3]));
+    out[1] = sqrt(((w[1][0] + w[1][1]) + w[1][2]) + w[1][3]);
+    out[2] = sqrt(((w[2][0] + w[2][1]) + w[2][2]) + w[2][3]);
+  }
+}
+
+template <typename T>
+int launch_norms(const T* u, const double* ref, int64_t n, double* part, int nb, double* out, hipStream_t stream) {
+  hipLaunchKernelGGL(norms_partial_kernel<T>, dim3(nb), dim3(256), 0, stream, u, ref, n, part);
+  hipLaunchKernelGGL(norms_final_kernel, dim3(1), dim3(256), 0, stream, part, nb, out);
+  return hipGetLastError() == hipSuccess ? 0 : MOL_ERR_HIP;
+}
+template int launch_norms<double>(const double*, const double*, int64_t, double*, int, double*, hipStream_t);
+template int launch_norms<float>(const float*, const double*, int64_t, double*, int, double*, hipStream_t);
+
 template int launch_fill_zero<double>(double*, int64_t, hipStream_t);
 template int launch_fill_zero<float>(float*, int64_t, hipStream_t);
 template int launch_adjoint_sum<double>(double*, double*, double*, double*, double*, int64_t, bool, hipStream_t);

@@ -80,6 +80,7 @@ _SIGNATURES = {
     "mol_integrate_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int64,
                                         C.c_int32, C.c_int32, C.c_void_p]),
     "mol_num_unknowns": (C.c_int64, [C.c_void_p]),
+    "mol_error_norms": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mol_set_tile": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "mol_get_plan": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32]),
     "mol_vjp": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,

@@ -310,6 +310,24 @@ class ODEProblem:
                                C.c_void_p(stream))
         native.check(h.lib, rc, h.h)
 
+    # ---------------------------------------------------------------- error checks
+    def error_norms(self, u, ref=None):
+        """FP64 norms of the device state against a float64 reference of the same length:
+        a float64 CUDA tensor [max |u - ref|, ||u - ref||_2, ||ref||_2] (the norms of u itself
+        when ``ref`` is None), computed on the device without a host copy of the state."""
+        torch = _torch()
+        h = self.handle
+        self._check_tensor(u)
+        if ref is not None and (ref.dtype != torch.float64 or not ref.is_cuda or not ref.is_contiguous()
+                                or ref.numel() != self.lowered.n_unknowns):
+            raise ValueError("ref must be a contiguous float64 CUDA tensor of %d values" % self.lowered.n_unknowns)
+        out = torch.empty(3, dtype=torch.float64, device=u.device)
+        stream = torch.cuda.current_stream().cuda_stream
+        rc = h.lib.mol_error_norms(h.h, u.data_ptr(), ref.data_ptr() if ref is not None else None,
+                                   out.data_ptr(), C.c_void_p(stream))
+        native.check(h.lib, rc, h.h)
+        return out
+
     # ---------------------------------------------------------------- layout helpers
     def unflatten(self, u) -> List[np.ndarray]:
         """Split a state vector into per-variable grids over their unknown boxes (first axis first)."""
