@@ -47,6 +47,11 @@ STAGE_TRAFFIC_BYTES = (3960.27 * 2.0 + 5869.53 * 1.0) * 1024
 # FETCH_SIZE 2375.6 KiB, WRITE_SIZE 4096.0 KiB per launch (8.39 MB algorithmic; the excess is
 # the tile halos).
 FUSED_TRAFFIC_BYTES = (2375.6 * 2.0 + 4096.0 * 1.0) * 1024
+# Issue floor of the fused step kernel from its SQ counters (profiles/brusselator512_rk4_fused_step.md):
+# 345.7 VALU instructions per wave over 4096 waves; a wave64 VALU instruction occupies a SIMD for
+# 4 cycles, and the chip has 1024 SIMDs at 2.4 GHz.
+FUSED_VALU_PER_WAVE, FUSED_WAVES = 345.7, 4096
+VALU_FLOOR_US = FUSED_VALU_PER_WAVE * FUSED_WAVES * 4 / (1024 * 2.4e9) * 1e6
 
 
 def host_cpu():
@@ -624,6 +629,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kname, "avg_us": launch_us, "bytes_per_launch": bytes_launch,
+                     # floors of the launch: HBM (algorithmic bytes at peak) and VALU issue (the
+                     # binding one of the two; the rest is the serial load / compute / store phases)
+                     "hbm_floor_us": bytes_launch / (HBM_PEAK_GBS * 1e3),
+                     "valu_floor_us": VALU_FLOOR_US if fused else None,
                      "stage_kernels": stage,
                      "rhs_only": {"avg_us": rhs_us, "bytes_per_launch": points * 32, "achieved": rhs_ach,
                                   "frac": rhs_ach / HBM_PEAK_GBS}},

@@ -81,3 +81,32 @@ def test_batched_fp32_kahan_error_vs_fp64():
     err = float((u32.double() - u64).abs().max())
     print("  fp32+kahan max error vs fp64: %.3e" % err)
     assert err < 1e-5
+
+
+def test_autotuner_times_every_candidate_and_keeps_the_fastest(tmp_path, monkeypatch):
+    """tune_batch times each candidate tile on the device, returns the fastest, caches it under
+    the batch's key, and a batch built with the chosen tile matches the oracle."""
+    from molgpu import autotune
+    torch = _torch()
+    monkeypatch.setenv("MOL_TUNE_CACHE", str(tmp_path / "tune.json"))
+    inst = _instances(8, 5)
+    lows = [p.lowered for _, p in inst]
+    cands = [(16, 4, 1), (8, 8, 1), (32, 2, 1)]
+    print("  autotune %d instances over %s" % (len(lows), cands), flush=True)
+    tile, table = autotune.tune_batch(lows, candidates=cands)
+    assert sorted(table) == sorted("%d,%d,%d" % c for c in cands)
+    assert all(v > 0 for v in table.values())
+    assert tile in cands and table["%d,%d,%d" % tile] == min(table.values())
+    # the cache answers the second call without timing
+    again, table2 = autotune.tune_batch(lows, candidates=[(4, 4, 1)])
+    assert again == tile and table2 == table
+    batch = BatchProblem([p for _, p in inst], tile=tile)
+    rng = np.random.default_rng(6)
+    u = rng.uniform(0, 1, batch.n_unknowns)
+    ug = torch.tensor(u, device="cuda")
+    du = torch.empty_like(ug)
+    batch.f(du, ug, 0.0)
+    for ((n, D, k, bct), _), ui, gi in zip(inst, batch.split(u), batch.split(du.cpu().numpy())):
+        ref = orc.diffreact2d_rhs(n, D, k, bct, ui)
+        dx = 1.0 / (n - 1)
+        assert_parity(gi, ref, np.max(np.abs(ui)) * D * 8 / dx ** 2 + k + 1, what="tuned batch n=%d" % n)
