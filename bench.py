@@ -110,6 +110,22 @@ def bench_weno(n, reps):
             "bytes_per_launch": pts * 16, "achieved_gbs": ach, "hbm_frac": ach / HBM_PEAK_GBS}
 
 
+def bench_weno_nonuniform(n, reps):
+    """The WENO5 Burgers RHS on seeded stretched node arrays along both axes (n^2 unknowns, FP64):
+    the nonuniform operator of weno_f_nonuniform, its coordinate-only coefficients read from the
+    per-axis tables the library builds at handle creation."""
+    import torch
+    import molgpu
+    from molgpu import systems
+    prob = molgpu.discretize(*systems.burgers2d_nonuniform(n=n + 1))
+    u = torch.tensor(prob.u0, dtype=torch.float64, device="cuda")
+    us = rhs_time_us(prob, u, reps)
+    pts = n * n
+    ach = pts * 16 / (us * 1e-6) / 1e9
+    return {"grid": "%dx%d stretched" % (n, n), "rhs_us": us, "gpoint_s": pts / (us * 1e-6) / 1e9,
+            "bytes_per_launch": pts * 16, "achieved_gbs": ach, "hbm_frac": ach / HBM_PEAK_GBS}
+
+
 def bench_weno_solve(n, nsteps, oracle_steps):
     """The WENO5 Burgers workload as an integration: SSPRK3 at CFL 0.4 for `nsteps` steps in FP64,
     in FP32 with the compensated update and in plain FP32; errors of the FP32 runs against the
@@ -497,8 +513,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only (no weno/adjoint/batch sections)")
     ap.add_argument("--section", default="all",
-                    choices=["all", "weno", "weno4096", "wenosolve", "adjoint", "adjoint32", "batch", "rhs3d", "copy",
-                             "heat"],
+                    choices=["all", "weno", "weno4096", "wenonu", "wenosolve", "adjoint", "adjoint32", "batch", "rhs3d",
+                             "copy", "heat"],
                     help="run one section alone (for profiling); prints its JSON")
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--reps", type=int, default=200, help="back-to-back RHS launches of the weno/rhs3d sections")
@@ -517,6 +533,8 @@ def main():
             res = {"1024": bench_weno(1024, args.reps)}
         elif args.section == "weno4096":
             res = {"4096": bench_weno(4096, args.reps)}
+        elif args.section == "wenonu":
+            res = {"1024": bench_weno_nonuniform(1024, args.reps)}
         elif args.section == "wenosolve":
             res = bench_weno_solve(1024, 500, 3)
         elif args.section == "adjoint":
@@ -640,6 +658,7 @@ def main():
     }
     if not args.quick:
         out["weno5"] = {"1024": bench_weno(1024, 200), "4096": bench_weno(4096, 20),
+                        "nonuniform_1024": bench_weno_nonuniform(1024, 200),
                         "solve_1024": bench_weno_solve(1024, 500, 3)}
         out["adjoint"] = bench_adjoint(128, 200, 20, 32)
         out["adjoint"]["fp32"] = bench_adjoint_fp32(128, 200, 20)

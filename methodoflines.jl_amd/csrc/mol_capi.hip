@@ -114,7 +114,7 @@ void free_handle(mol_handle* h) {
                   h->w_y2, h->w_acc, h->w_comp, h->d_tnow, h->d_ka64, h->d_ka32, h->d_tiles,
                   h->ck_u, h->ck_t, h->seg_u, h->seg_t, h->adj_y[0], h->adj_y[1], h->adj_y[2], h->adj_y[3],
                   h->d_insts, h->d_wl, h->d_bdata, h->vjp_bf, h->vjp_ba, h->r_u, h->r_du, h->r_p,
-                  h->d_norm_part};
+                  h->d_norm_part, h->d_wtab[0], h->d_wtab[1], h->d_wtab[2]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete h;
@@ -406,6 +406,7 @@ Launch<T> launch_of(mol_handle* h) {
   for (int a = 0; a < 3; ++a) L.jp.coord[a] = ka.coord[a];
   for (int a = 0; a < MOL_MAX_AUX; ++a) L.jp.aux[a] = ka.aux[a];
   L.jp.pool = ka.pool;
+  for (int a = 0; a < 3; ++a) L.jp.wtab[a] = h->d_wtab[a];
   return L;
 }
 
@@ -720,6 +721,35 @@ bool fused_geometry(mol_handle* h, const mol_desc* d, const KArgs<double>& ka, c
   return true;
 }
 
+// Coefficient rows of the nonuniform WENO5 derivative (coordinates only) at every node of each
+// axis that has such a slot, computed once on the device by the generated mol_wenonu_table.
+int build_wenonu_tables(mol_handle* h, const mol_desc* d, const std::string& src, std::string& err) {
+  hipFunction_t fn = nullptr;
+  if (jit_kernel(src, "mol_wenonu_table", h->device, &fn, err)) return MOL_ERR_HIP;
+  constexpr int kNC = 56;   // coefficients per node (MOL_NU_NC of the generated code)
+  for (int a = 0; a < d->ndim; ++a) {
+    bool used = false;
+    for (int k = 0; k < d->nslots; ++k) used = used || (d->slots_h[k].kind == MOL_SLOT_WENO_NU && d->slots_h[k].axis == a);
+    if (!used) continue;
+    const int n = d->n[a];
+    if (!h->d_wtab[a] && hipMalloc((void**)&h->d_wtab[a], (size_t)n * kNC * sizeof(double)) != hipSuccess) {
+      (void)hipGetLastError();
+      err = "not enough device memory for the nonuniform WENO5 coefficient rows";
+      return MOL_ERR_HIP;
+    }
+    const double* x = (const double*)h->d_coord[a];
+    int nn = n, per = h->ka64.periodic[a] ? 1 : 0;
+    double* out = h->d_wtab[a];
+    void* args[] = {(void*)&x, (void*)&nn, (void*)&per, (void*)&out};
+    if (hipModuleLaunchKernel(fn, (unsigned)((n + 255) / 256), 1, 1, 256, 1, 1, 0, nullptr, args, nullptr) != hipSuccess ||
+        hipStreamSynchronize(nullptr) != hipSuccess) {
+      err = std::string("nonuniform WENO5 table launch: ") + hipGetErrorString(hipGetLastError());
+      return MOL_ERR_HIP;
+    }
+  }
+  return 0;
+}
+
 void setup_fused(mol_handle* h, const mol_desc* d, const KArgs<double>& ka, const KCore<double>& kc, int ncore) {
   h->fused_fn = nullptr;
   h->fused_tiles = 0;
@@ -776,6 +806,10 @@ int setup_paths(mol_handle* h, const mol_desc* d) {
     const std::string src = jit_source(d, jg, err);
     if (src.empty()) h->jit_reason = err;
     else if (jit_kernel(src, "mol_rhs_tiles", h->device, &h->jit_fn, err)) {
+      h->jit_fn = nullptr;
+      h->jit_reason = err;
+    }
+    if (h->jit_fn && src.find("mol_wenonu_table(") != std::string::npos && build_wenonu_tables(h, d, src, err)) {
       h->jit_fn = nullptr;
       h->jit_reason = err;
     }
@@ -853,9 +887,11 @@ struct Work {
 
 Work handle_work(const mol_handle* h) { return Work{h->w_y1, h->w_y2, h->w_acc, h->w_comp, h->d_tnow}; }
 
+// t0v: the start time, passed by value to the first fused launch instead of read from the
+// device clock (which that launch then sets for the next); null: the clock holds it.
 template <typename T>
 int enqueue_steps(mol_handle* h, const Work& w, T* u, const T* p, double dt, int64_t nsteps, int scheme,
-                  bool kahan, hipStream_t s) {
+                  bool kahan, hipStream_t s, const double* t0v = nullptr) {
   const Launch<T> L = launch_of<T>(h);
   T* y1 = (T*)w.y1;
   T* y2 = (T*)w.y2;
@@ -873,6 +909,10 @@ int enqueue_steps(mol_handle* h, const Work& w, T* u, const T* p, double dt, int
       e.tnext = w.tnow + (1 - h->clk);
       e.tadv = dt;
       e.p = p;
+      if (t0v && step == 0 && !h->fused_resident) {
+        e.tnow = nullptr;
+        e.tofs = *t0v;
+      }
       JitPtrs<T> jp = L.jp;
       const T* sp = src;
       int ns = nsub;
@@ -962,13 +1002,17 @@ int integrate_impl(mol_handle* h, T* u, const T* p, double t0, double dt, int64_
   const bool kahan = (flags & MOL_FLAG_KAHAN) != 0;
   h->no_fuse = (flags & MOL_FLAG_UNFUSED) != 0;
   h->clk = 0;
-  if (launch_set_clock(h->d_tnow, t0, s)) return fail(h, MOL_ERR_HIP, "clock launch failed");
-  if (kahan && launch_fill_zero<T>((T*)h->w_comp, h->nunk, s)) return fail(h, MOL_ERR_HIP, "zero fill launch failed");
   // a resident fused kernel already runs up to 1024 steps per launch and flips the clock once
   // per launch, whatever the step count: it is launched directly
-  const bool resident = scheme == MOL_RK4 && !kahan && h->fused_fn && !h->no_fuse && h->fused_resident;
-  if (!(flags & MOL_FLAG_GRAPH) || resident || nsteps < kGraphChunks[2])
-    return enqueue_steps<T>(h, handle_work(h), u, p, dt, nsteps, scheme, kahan, s);
+  const bool fused = scheme == MOL_RK4 && !kahan && h->fused_fn && !h->no_fuse;
+  const bool resident = fused && h->fused_resident;
+  const bool direct = !(flags & MOL_FLAG_GRAPH) || resident || nsteps < kGraphChunks[2];
+  // direct fused steps take the start time by value in their first launch: no clock launch
+  if (direct && fused && !resident && nsteps > 0)
+    return enqueue_steps<T>(h, handle_work(h), u, p, dt, nsteps, scheme, kahan, s, &t0);
+  if (launch_set_clock(h->d_tnow, t0, s)) return fail(h, MOL_ERR_HIP, "clock launch failed");
+  if (kahan && launch_fill_zero<T>((T*)h->w_comp, h->nunk, s)) return fail(h, MOL_ERR_HIP, "zero fill launch failed");
+  if (direct) return enqueue_steps<T>(h, handle_work(h), u, p, dt, nsteps, scheme, kahan, s);
   hipGraphExec_t ex[3] = {nullptr, nullptr, nullptr};
   for (const auto& c : h->graphs)
     if (c.u == u && c.p == p && c.dt == dt && c.scheme == scheme && c.kahan == (kahan ? 1 : 0) &&
@@ -1139,6 +1183,7 @@ int launch_vjp(mol_handle* h, const VArgs<T>& va, hipStream_t s, bool tiles = tr
   for (int a = 0; a < 3; ++a) jp.coord[a] = ka.coord[a];
   for (int a = 0; a < MOL_MAX_AUX; ++a) jp.aux[a] = ka.aux[a];
   jp.pool = ka.pool;
+  for (int a = 0; a < 3; ++a) jp.wtab[a] = h->d_wtab[a];
   VArgs<T> v = va;
   v.bf = (T*)h->vjp_bf;
   v.ba = (double*)h->vjp_ba;

@@ -26,6 +26,7 @@ struct JitPtrs {
   const double* coord[3];
   const T* aux[MOL_MAX_AUX];
   const double* pool;   // descriptor weight pool (per-node rows of nonuniform axes)
+  const double* wtab[3];   // per axis: nonuniform WENO5 coefficient rows (generated kernels)
 };
 
 // Code generation and in-process compilation of the tile kernel (mol_jit.cpp).
@@ -192,6 +193,7 @@ struct mol_handle {
   bool no_fuse = false;                   // this integration runs the stage kernels (MOL_FLAG_UNFUSED)
   int vjp_nthr = 256;                     // threads per work-group of the generated VJP kernel
   double* d_norm_part = nullptr;          // per-block partials of mol_error_norms
+  double* d_wtab[3] = {nullptr, nullptr, nullptr};   // nonuniform WENO5 coefficient rows per axis
   hipFunction_t vjp_fn = nullptr;         // generated vector-Jacobian product kernel
   bool vjp_store = false;                 // its gather stores every unknown (no zeroing of the output)
   hipFunction_t vjp_gather_fn = nullptr;  // its gather of the per-tile adjoint buffers

@@ -67,7 +67,8 @@ struct Node {
   bool core = false;         //        built for the interior-rows variant
   std::string text;          // N_EXPR
   int imm = 0;               // MOL_OP_POWI exponent; X_WENO / X_WENONU: slot
-  int target = 3;            // X_WENONU: reconstruction node 1..5 of the stencil (operands 0..4: u, 5..9: x)
+  int target = 3;            // X_WENONU: reconstruction node 1..5 of the stencil (operands 0..4: u);
+                             //   ax: its axis, text: the node's index along it (row of the table)
   // N_BRANCH: region reg[0] when lo <= I_ax <= hi, region reg[k] when I_ax == cval[k], else 0
   int ax = 0, lo = 0, hi = 0;
   std::vector<int> cval, reg;
@@ -338,6 +339,10 @@ class Gen {
 
   // ---- printing
   std::string ref(int id) const;
+  // row of the nonuniform-WENO coefficient table of the axis of X_WENONU node n at its node
+  std::string wrow(const Node& n) const {
+    return "(jp.wtab[" + istr(n.ax) + "] + MOL_NU_NC * (" + n.text + " - 1))";
+  }
   std::string op_expr(const Node& n, int id) const;
   void print_region(std::ostream& o, int r) const;
   void print_node(std::ostream& o, int id) const;
@@ -711,19 +716,15 @@ int Gen::slot_body(int k, const Var* v) {
           first = n - 4 - c;
         }
       }
+      // the coordinate-only coefficients of the node come from its row of the axis table
+      // (mol_weno_nu_row, built once per handle by mol_wenonu_table)
       Node nd;
       nd.kind = N_OP;
       nd.op = X_WENONU;
       nd.target = target;
+      nd.ax = ax;
+      nd.text = (!g_.periodic[ax] && !v[ax].interior) ? IX(ax, v[ax].i) : "I" + istr(ax);
       for (int q = 0; q < 5; ++q) nd.a.push_back(lds_ax(s.field, ax, first + q));
-      const std::string I = "I" + istr(ax), cx = "jp.coord[" + istr(ax) + "]";
-      for (int q = 0; q < 5; ++q) {
-        const int off = first + q;
-        std::string pos = off == 0 ? I : "(" + I + (off > 0 ? " + " : " - ") + istr(off > 0 ? off : -off) + ")";
-        if (!g_.periodic[ax] && !v[ax].interior) pos = IX(ax, v[ax].i + off);
-        nd.a.push_back(expr(g_.periodic[ax] ? "mol_pcoord(" + cx + ", " + pos + ", " + NN(ax) + ")"
-                                            : cx + "[" + pos + " - 1]"));
-      }
       nd.w = {s.eps};
       nd.imm = k;
       return add(nd);
@@ -967,9 +968,8 @@ std::string Gen::op_expr(const Node& n, int id) const {
       return e;
     }
     case X_WENONU:
-      return "mol_weno5_nu<" + istr(n.target) + ">(" + A(0) + ", " + A(1) + ", " + A(2) + ", " + A(3) + ", " + A(4) +
-             ", " + A(5) + ", " + A(6) + ", " + A(7) + ", " + A(8) + ", " + A(9) + ", " +
-             pk(n.w[0], d_->ncpool + 2LL * n.imm) + ")";
+      return "mol_weno5_nu_tab(" + A(0) + ", " + A(1) + ", " + A(2) + ", " + A(3) + ", " + A(4) + ", " + wrow(n) +
+             ", " + pk(n.w[0], d_->ncpool + 2LL * n.imm) + ")";
     case X_WENO:
       // FP64: one-division cleared form; FP32: normalised weights (the cleared form underflows)
       return std::string(d_->dtype == MOL_F64 ? "mol_weno5(" : "mol_weno5_ref(") + A(0) + ", " + A(1) + ", " +
@@ -1100,9 +1100,9 @@ void Gen::print_reverse(std::ostream& o, int r, const std::vector<std::pair<int,
           << ", " << (d_->dtype == MOL_F64 ? "true" : "false") << ");\n";
       }
       if (n.op == X_WENONU) {
-        o << "    T wd" << id << "[5];\n    mol_weno5_nu_grad<" << n.target << ">(";
-        for (int q = 0; q < 10; ++q) o << ref(n.a[q]) << ", ";
-        o << pk(n.w[0], d_->ncpool + 2LL * n.imm) << ", wd" << id << ");\n";
+        o << "    T wd" << id << "[5];\n    mol_weno5_nu_tab_grad(";
+        for (int q = 0; q < 5; ++q) o << ref(n.a[q]) << ", ";
+        o << wrow(n) << ", " << pk(n.w[0], d_->ncpool + 2LL * n.imm) << ", wd" << id << ");\n";
       }
       std::vector<std::pair<int, std::string>> ps;
       partials(n, id, ps);
@@ -1780,6 +1780,16 @@ std::string Gen::source_kernel(bool f64, bool vjp) {
     }
     o << "}\n";
   }
+  bool nu = false;
+  for (int k = 0; k < d_->nslots; ++k) nu = nu || d_->slots_h[k].kind == MOL_SLOT_WENO_NU;
+  if (!vjp && nu)
+    // coefficient rows of the nonuniform WENO5 derivative at every node of an axis: built once
+    // per handle, read by the node functions (jp.wtab)
+    o << "\nextern \"C\" __global__ void __launch_bounds__(256) mol_wenonu_table(const double* __restrict__ x, int n,\n"
+         "                                                                 int periodic, double* __restrict__ out) {\n"
+         "  const int i = (int)(blockIdx.x * 256 + threadIdx.x) + 1;\n"
+         "  if (i <= n) mol_weno_nu_row(x, n, periodic != 0, i, out + (i64)MOL_NU_NC * (i - 1));\n"
+         "}\n";
   return o.str();
 }
 
@@ -1839,7 +1849,8 @@ std::string Gen::source_fused(bool f64) {
     << ") mol_rk4_fused(const T* __restrict__ u, T* __restrict__ unew, Epi ep, JPtrs jp, int nsub) {\n"
        "  __shared__ T U[GeoF::NST * GeoF::VOL];\n  __shared__ T Ya[GeoF::NST * GeoF::VOL];\n"
        "  __shared__ T Yb[GeoF::NST * GeoF::VOL];\n"
-       "  const double t = ep.tnow ? ep.tnow[0] : 0.0;\n";
+       "  // the step's time from the device clock, or (first launch of a call) by value in tofs\n"
+       "  const double t = ep.tnow ? ep.tnow[0] : ep.tofs;\n";
   if (resident) {
     o << "  mol_rk4_fused_resident<GeoF>(U, Ya, Yb, &mol_fc[0][0], (int)(sizeof(mol_fc[0]) / sizeof(double)), mol_fo,\n"
          "                               u, unew, ep, jp, t, nsub);\n}\n";
@@ -1848,7 +1859,7 @@ std::string Gen::source_fused(bool f64) {
          "  int b = mol_xcd_tile<GeoBase::NBLK>((int)blockIdx.x);\n"
          "  const int tx = b % GeoBase::NT0;\n  b /= GeoBase::NT0;\n"
          "  const int ty = b % GeoBase::NT1;\n  const int tz = b / GeoBase::NT1;\n"
-         "  mol_advance_clock(ep);\n"
+         "  if (ep.tnext && blockIdx.x == 0 && threadIdx.x == 0) ep.tnext[0] = t + ep.tadv;\n"
          "  mol_rk4_fused_tile<GeoF>(U, Ya, Yb, &mol_fc[0][0], (int)(sizeof(mol_fc[0]) / sizeof(double)), mol_fo,\n"
          "                           tx, ty, tz, u, unew, ep, jp, t);\n}\n";
   }

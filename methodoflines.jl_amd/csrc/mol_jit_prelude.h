@@ -37,7 +37,8 @@ __device__ __forceinline__ void mol_advance_clock(const Epi& ep) {
 struct JPtrs {
   const double* coord[3];
   const T* aux[8];
-  const double* pool;   // weight pool: per-node rows of nonuniform axes
+  const double* pool;      // weight pool: per-node rows of nonuniform axes
+  const double* wtab[3];   // per axis: nonuniform WENO5 coefficient rows (MOL_NU_NC per node)
 };
 
 // Fused epilogue of one Runge-Kutta stage:
@@ -267,27 +268,26 @@ __device__ __forceinline__ U mol_dot3(const double* w, U a, U b, U c) {
   return (T)w[0] * a + (T)w[1] * b + (T)w[2] * c;
 }
 
-template <class U>
-__device__ __forceinline__ void mol_nu_beta_r(const double a0, const double a1, const double a2, U ua, U ub, U uc,
-                                              const double xi, const double xL, const double xM, const double xph,
-                                              const double dx, U& beta, U& r) {
-  double m0[3], m1i[3], m1L[3], m1M[3], m2M[3], m1R[3], t2[3];
-  mol_fornberg3(a0, a1, a2, xi, m0, m1i, t2);
-  mol_fornberg3(a0, a1, a2, xL, m0, m1L, t2);
-  mol_fornberg3(a0, a1, a2, xM, m0, m1M, m2M);
-  mol_fornberg3(a0, a1, a2, xph, m0, m1R, t2);
-  r = mol_dot3(m1i, ua, ub, uc);
-  const U pL = mol_dot3(m1L, ua, ub, uc), pM = mol_dot3(m1M, ua, ub, uc), pR = mol_dot3(m1R, ua, ub, uc);
-  const U pp = mol_dot3(m2M, ua, ub, uc);
-  const U I1 = (T)(dx / 6) * (pL * pL + T(4) * (pM * pM) + pR * pR);
-  const U I2 = (T)dx * (pp * pp);
-  const U v = (T)dx * I1 + (T)(dx * dx * dx) * I2;
-  beta = mol_val(v) > T(0) ? v : U{T(0)};
+// The coordinate-only part of the nonuniform WENO5 derivative at one node (target TG of the
+// stencil x1..x5), MOL_NU_NC values: for each sub-stencil k = 0, 1, 2 the weights of r_k
+// (derivative at the target) and of the interval values pL, pM, pR (first derivative at the left
+// end, midpoint and right end) and pp (second derivative), 15 per sub-stencil; then dx/6, dx,
+// dx^3 of the Simpson smoothness indicator, the split ideal weights dp_k/sgp and dm_k/sgm, and
+// sgp, sgm.  mol_weno5_nu_apply turns them and the five values into the derivative.
+constexpr int MOL_NU_NC = 56;
+
+__device__ __forceinline__ void mol_nu_sub(const double a0, const double a1, const double a2, const double xi,
+                                           const double xL, const double xM, const double xph, double* c) {
+  double m0[3], t2[3];
+  mol_fornberg3(a0, a1, a2, xi, m0, c + 0, t2);
+  mol_fornberg3(a0, a1, a2, xL, m0, c + 3, t2);
+  mol_fornberg3(a0, a1, a2, xM, m0, c + 6, c + 12);
+  mol_fornberg3(a0, a1, a2, xph, m0, c + 9, t2);
 }
 
-template <int TG, class U>
-__device__ __forceinline__ U mol_weno5_nu_t(U u1, U u2, U u3, U u4, U u5, const double x1, const double x2,
-                                            const double x3, const double x4, const double x5, const T eps) {
+template <int TG>
+__device__ __forceinline__ void mol_weno5_nu_coefs(const double x1, const double x2, const double x3, const double x4,
+                                                   const double x5, double* c) {
   double xi, xL, xph, d0, d2;
   if (TG == 1) { xi = x1; xL = x1; xph = (x1 + x2) / 2; }
   else if (TG == 2) { xi = x2; xL = (x1 + x2) / 2; xph = (x2 + x3) / 2; }
@@ -313,29 +313,93 @@ __device__ __forceinline__ U mol_weno5_nu_t(U u1, U u2, U u3, U u4, U u5, const 
     d2 = ((x3 - x1) * (x3 - x2)) / ((x5 - x1) * (x5 - x2));
   }
   const double dx = xph - xL, xM = (xL + xph) / 2;
-  U b0, b1, b2, r0, r1, r2;
-  mol_nu_beta_r(x1, x2, x3, u1, u2, u3, xi, xL, xM, xph, dx, b0, r0);
-  mol_nu_beta_r(x2, x3, x4, u2, u3, u4, xi, xL, xM, xph, dx, b1, r1);
-  mol_nu_beta_r(x3, x4, x5, u3, u4, u5, xi, xL, xM, xph, dx, b2, r2);
+  mol_nu_sub(x1, x2, x3, xi, xL, xM, xph, c + 0);
+  mol_nu_sub(x2, x3, x4, xi, xL, xM, xph, c + 15);
+  mol_nu_sub(x3, x4, x5, xi, xL, xM, xph, c + 30);
+  c[45] = dx / 6;
+  c[46] = dx;
+  c[47] = dx * dx * dx;
   const double d1 = 1.0 - d0 - d2, th = 3.0, half = 0.5;
   const double dp0 = half * (d0 + th * fabs(d0)), dp1 = half * (d1 + th * fabs(d1)), dp2 = half * (d2 + th * fabs(d2));
   const double dm0 = dp0 - d0, dm1 = dp1 - d1, dm2 = dp2 - d2;
   const double sgp = dp0 + dp1 + dp2, sgm = dm0 + dm1 + dm2;
+  c[48] = dp0 / sgp;
+  c[49] = dp1 / sgp;
+  c[50] = dp2 / sgp;
+  c[51] = dm0 / sgm;
+  c[52] = dm1 / sgm;
+  c[53] = dm2 / sgm;
+  c[54] = sgp;
+  c[55] = sgm;
+}
+
+template <class U>
+__device__ __forceinline__ void mol_nu_beta_r(const double* c, const double* k, U ua, U ub, U uc, U& beta, U& r) {
+  r = mol_dot3(c + 0, ua, ub, uc);
+  const U pL = mol_dot3(c + 3, ua, ub, uc), pM = mol_dot3(c + 6, ua, ub, uc), pR = mol_dot3(c + 9, ua, ub, uc);
+  const U pp = mol_dot3(c + 12, ua, ub, uc);
+  const U I1 = (T)k[0] * (pL * pL + T(4) * (pM * pM) + pR * pR);
+  const U I2 = (T)k[1] * (pp * pp);
+  const U v = (T)k[1] * I1 + (T)k[2] * I2;
+  beta = mol_val(v) > T(0) ? v : U{T(0)};
+}
+
+template <class U>
+__device__ __forceinline__ U mol_weno5_nu_apply(U u1, U u2, U u3, U u4, U u5, const double* c, const T eps) {
+  U b0, b1, b2, r0, r1, r2;
+  mol_nu_beta_r(c + 0, c + 45, u1, u2, u3, b0, r0);
+  mol_nu_beta_r(c + 15, c + 45, u2, u3, u4, b1, r1);
+  mol_nu_beta_r(c + 30, c + 45, u3, u4, u5, b2, r2);
   const U e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
   const U q0 = e0 * e0, q1 = e1 * e1, q2 = e2 * e2;
-  const U ap0 = (T)(dp0 / sgp) / q0, ap1 = (T)(dp1 / sgp) / q1, ap2 = (T)(dp2 / sgp) / q2;
+  const U ap0 = (T)c[48] / q0, ap1 = (T)c[49] / q1, ap2 = (T)c[50] / q2;
   const U sp = ap0 + ap1 + ap2;
-  const U am0 = (T)(dm0 / sgm) / q0, am1 = (T)(dm1 / sgm) / q1, am2 = (T)(dm2 / sgm) / q2;
+  const U am0 = (T)c[51] / q0, am1 = (T)c[52] / q1, am2 = (T)c[53] / q2;
   const U sm = am0 + am1 + am2;
   const U Rp = (ap0 / sp) * r0 + (ap1 / sp) * r1 + (ap2 / sp) * r2;
   const U Rm = (am0 / sm) * r0 + (am1 / sm) * r1 + (am2 / sm) * r2;
-  return (T)sgp * Rp - (T)sgm * Rm;
+  return (T)c[54] * Rp - (T)c[55] * Rm;
+}
+
+template <int TG, class U>
+__device__ __forceinline__ U mol_weno5_nu_t(U u1, U u2, U u3, U u4, U u5, const double x1, const double x2,
+                                            const double x3, const double x4, const double x5, const T eps) {
+  double c[MOL_NU_NC];
+  mol_weno5_nu_coefs<TG>(x1, x2, x3, x4, x5, c);
+  return mol_weno5_nu_apply<U>(u1, u2, u3, u4, u5, c, eps);
 }
 
 template <int TG>
 __device__ __forceinline__ T mol_weno5_nu(T u1, T u2, T u3, T u4, T u5, double x1, double x2, double x3, double x4,
                                           double x5, T eps) {
   return mol_weno5_nu_t<TG, T>(u1, u2, u3, u4, u5, x1, x2, x3, x4, x5, eps);
+}
+
+// The FP64 derivative from the coefficients with the normalised nonlinear weights cleared, as
+// in the uniform case: alpha_k / sum_j alpha_j = g_k P_k / sum_j g_j P_j with P_k the product of
+// the other two q_j, so sgp Rp - sgm Rm = (sgp Np Dm - sgm Nm Dp) / (Dp Dm) costs one quotient
+// (mol_quot) instead of twelve divisions.  The q_j are at least eps^2, so the products of four
+// stay far above the FP64 underflow threshold (FP32 keeps the normalised form).
+__device__ __forceinline__ double mol_weno5_nu_cleared(double u1, double u2, double u3, double u4, double u5,
+                                                       const double* c, double eps) {
+  double b0, b1, b2, r0, r1, r2;
+  mol_nu_beta_r<double>(c + 0, c + 45, u1, u2, u3, b0, r0);
+  mol_nu_beta_r<double>(c + 15, c + 45, u2, u3, u4, b1, r1);
+  mol_nu_beta_r<double>(c + 30, c + 45, u3, u4, u5, b2, r2);
+  const double e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
+  const double q0 = e0 * e0, q1 = e1 * e1, q2 = e2 * e2;
+  const double P0 = q1 * q2, P1 = q0 * q2, P2 = q0 * q1;
+  const double A0 = c[48] * P0, A1 = c[49] * P1, A2 = c[50] * P2;
+  const double B0 = c[51] * P0, B1 = c[52] * P1, B2 = c[53] * P2;
+  const double Dp = A0 + A1 + A2, Dm = B0 + B1 + B2;
+  const double Np = A0 * r0 + A1 * r1 + A2 * r2, Nm = B0 * r0 + B1 * r1 + B2 * r2;
+  return mol_quot(c[54] * Np * Dm - c[55] * Nm * Dp, Dp * Dm);
+}
+
+// The same derivative from a node's row of precomputed coefficients (mol_weno_nu_table).
+__device__ __forceinline__ T mol_weno5_nu_tab(T u1, T u2, T u3, T u4, T u5, const double* c, T eps) {
+  if constexpr (sizeof(T) == 8) return (T)mol_weno5_nu_cleared(u1, u2, u3, u4, u5, c, eps);
+  else return mol_weno5_nu_apply<T>(u1, u2, u3, u4, u5, c, eps);
 }
 
 // Coordinate of (unwrapped) node k of a periodic axis: the image across an end is continued by
@@ -355,6 +419,45 @@ __device__ __noinline__ void mol_weno5_nu_grad(T u1, T u2, T u3, T u4, T u5, dou
     MolDual d[5];
     for (int j = 0; j < 5; ++j) d[j] = MolDual{x[j], j == k ? T(1) : T(0)};
     g[k] = mol_weno5_nu_t<TG, MolDual>(d[0], d[1], d[2], d[3], d[4], x1, x2, x3, x4, x5, eps).d;
+  }
+}
+
+// The gradient from a node's row of precomputed coefficients.
+__device__ __noinline__ void mol_weno5_nu_tab_grad(T u1, T u2, T u3, T u4, T u5, const double* c, T eps, T* g) {
+  const T x[5] = {u1, u2, u3, u4, u5};
+  for (int k = 0; k < 5; ++k) {
+    MolDual d[5];
+    for (int j = 0; j < 5; ++j) d[j] = MolDual{x[j], j == k ? T(1) : T(0)};
+    g[k] = mol_weno5_nu_apply<MolDual>(d[0], d[1], d[2], d[3], d[4], c, eps).d;
+  }
+}
+
+// Row of coefficients (MOL_NU_NC values) of node i = 1..n of an axis with coordinates x: the
+// stencil and target of the generated code (one-sided at nodes 1, 2, n-1, n of a non-periodic
+// axis, centred elsewhere; periodic images through mol_pcoord).
+__device__ __forceinline__ void mol_weno_nu_row(const double* x, const int n, const bool periodic, const int i,
+                                                double* c) {
+  int first = -2, target = 3;
+  if (!periodic) {
+    if (i <= 2) {
+      target = i;
+      first = 1 - i;
+    } else if (i >= n - 1) {
+      target = 5 - (n - i);
+      first = n - 4 - i;
+    }
+  }
+  double xs[5];
+  for (int q = 0; q < 5; ++q) {
+    const int k = i + first + q;
+    xs[q] = periodic ? mol_pcoord(x, k, n) : x[k - 1];
+  }
+  switch (target) {
+    case 1: mol_weno5_nu_coefs<1>(xs[0], xs[1], xs[2], xs[3], xs[4], c); break;
+    case 2: mol_weno5_nu_coefs<2>(xs[0], xs[1], xs[2], xs[3], xs[4], c); break;
+    case 4: mol_weno5_nu_coefs<4>(xs[0], xs[1], xs[2], xs[3], xs[4], c); break;
+    case 5: mol_weno5_nu_coefs<5>(xs[0], xs[1], xs[2], xs[3], xs[4], c); break;
+    default: mol_weno5_nu_coefs<3>(xs[0], xs[1], xs[2], xs[3], xs[4], c); break;
   }
 }
 

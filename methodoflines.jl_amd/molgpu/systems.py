@@ -75,6 +75,16 @@ def burgers2d(n: int = 1025, eps: float = 1e-6, seed: int = 0, dtype: str = "f64
     return sys_, MOLFiniteDifference({x: n, y: n}, t, advection_scheme=WENOScheme(eps), dtype=dtype)
 
 
+def burgers2d_nonuniform(n: int = 1025, seed: int = 0, spread: float = 0.3, eps: float = 1e-6,
+                         dtype: str = "f64"):
+    """The WENO5 Burgers system of `burgers2d` on seeded stretched node arrays along x and y
+    (n nodes each, periodic)."""
+    sys_, disc = burgers2d(n=n, eps=eps, seed=seed, dtype=dtype)
+    x, y = sys_.ivs[1], sys_.ivs[2]
+    nodes = {x: stretched_grid(n, 1.0, seed + 1, spread), y: stretched_grid(n, 1.0, seed + 2, spread)}
+    return sys_, MOLFiniteDifference(nodes, disc.time, advection_scheme=WENOScheme(eps), dtype=dtype)
+
+
 def diffreact2d(n: int, D: float, k: float, bctype=(0, 0, 0, 0), dtype: str = "f64"):
     """u_t = D Lap u + k u (1 - u) on [0,1]^2; bctype per side (x-lo, x-hi, y-lo, y-hi):
     0 Dirichlet u = 0, 1 Neumann du/dn = 0."""

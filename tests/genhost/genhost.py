@@ -25,7 +25,14 @@ def _host_source(src: str, entry: str) -> str:
     shim = open(os.path.join(HERE, "shim.h")).read()
     src = src.replace('#include "mol_jit_prelude.h"', shim + "\n" + prelude)
     src = re.sub(r"static constexpr int NTHR = \d+;", "static constexpr int NTHR = 1;", src)
-    return src + entry
+    # rows of the nonuniform WENO5 coefficient tables, as the library builds them on the device
+    wtab = """
+extern "C" __attribute__((visibility("default"))) void mol_host_wtab_row(const double* x, int n, int per, int i,
+                                                                          double* c) {
+  mol_weno_nu_row(x, n, per != 0, i, c);
+}
+"""
+    return src + entry + wtab
 
 
 class Epi(C.Structure):
@@ -36,7 +43,9 @@ class Epi(C.Structure):
 
 
 class JPtrs(C.Structure):
-    _fields_ = [("coord", C.c_void_p * 3), ("aux", C.c_void_p * 8), ("pool", C.c_void_p)]
+    _fields_ = [("coord", C.c_void_p * 3), ("aux", C.c_void_p * 8), ("pool", C.c_void_p), ("wtab", C.c_void_p * 3)]
+
+NU_NC = 56   # coefficients per node of a nonuniform WENO5 table row
 
 
 class VArgs(C.Structure):
@@ -73,8 +82,9 @@ def _const(src: str, key: str, start: str = "") -> int:
     return int(src[j:src.index(";", j)].split(",")[0])
 
 
-def _ptrs(low, aux):
+def _ptrs(low, aux, lib=None):
     from molgpu import native
+    from molgpu.discretization import SLOT_WENO_NU
     jp = JPtrs()
     keep = []
     pool = np.ascontiguousarray(native.weight_pool(low), dtype=np.float64)
@@ -88,6 +98,18 @@ def _ptrs(low, aux):
         v = np.ascontiguousarray(np.asarray(aux[name], dtype=np.float64).reshape(-1, order="F"))
         keep.append(v)
         jp.aux[i] = v.ctypes.data
+    for a in range(low.ndim):
+        if lib is None or not any(s["kind"] == SLOT_WENO_NU and s["axis"] == a for s in low.slots):
+            continue
+        x = np.ascontiguousarray(low.coords[a], dtype=np.float64)
+        n = len(x)
+        tab = np.zeros(n * NU_NC)
+        f = lib.mol_host_wtab_row
+        f.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
+        for i in range(1, n + 1):
+            f(x.ctypes.data, n, int(bool(low.periodic[a])), i, tab.ctypes.data + 8 * NU_NC * (i - 1))
+        keep += [x, tab]
+        jp.wtab[a] = tab.ctypes.data
     return jp, keep
 
 
@@ -111,7 +133,7 @@ extern "C" __attribute__((visibility("default"))) void run(const T* u, Epi ep, J
     ep.out = du.ctypes.data
     ep.tofs = t
     ep.p = pv.ctypes.data if pv.size else None
-    jp, keep = _ptrs(low, aux or {})
+    jp, keep = _ptrs(low, aux or {}, lib)
     lib.run.argtypes = [C.c_void_p, Epi, JPtrs, C.c_int]
     lib.run(u.ctypes.data, ep, jp, grid if grid is not None else _nblocks(src))
     return du
@@ -149,7 +171,7 @@ extern "C" __attribute__((visibility("default"))) void run(VArgs va, JPtrs jp, i
     va.g_aux = g.ctypes.data
     va.p = pv.ctypes.data if pv.size else None
     va.tofs = t
-    jp, keep = _ptrs(low, aux or {})
+    jp, keep = _ptrs(low, aux or {}, lib)
     nblk = _nblocks(src)
     volx = _const(src, "constexpr int VOLX = ")
     bf = np.zeros(max(1, nblk * volx * _const(src, "static constexpr int NSTF = ", "struct GeoB")))
@@ -186,7 +208,7 @@ extern "C" __attribute__((visibility("default"))) void run(const T* u, T* unew, 
     ep.tnext = clock.ctypes.data + 8
     ep.tadv = dt
     ep.p = pv.ctypes.data if pv.size else None
-    jp, keep = _ptrs(low, aux or {})
+    jp, keep = _ptrs(low, aux or {}, lib)
     lib.run.argtypes = [C.c_void_p, C.c_void_p, Epi, JPtrs, C.c_int, C.c_int]
     lib.run(u.ctypes.data, unew.ctypes.data, ep, jp, _nblocks(src), nsub)
     return unew, clock[1]
