@@ -6,7 +6,7 @@ The Python surface mirrors MethodOfLines.jl (``MOLFiniteDifference``, ``discreti
 from .pdesys import (PDESystem, Differential, Interval, Eq, ifelse, parameters, variables,
                      grid_field, GridField)
 from .discretization import (MOLFiniteDifference, UpwindScheme, WENOScheme, Lowered, lower,
-                             uniform_grid)
+                             uniform_grid, center_align, edge_align)
 from .fornberg import calculate_weights
 from .problem import (ODEProblem, ODESolution, RK4, SSPRK33, codegen, discretize,
                       symbolic_discretize, solve)
@@ -15,6 +15,6 @@ from . import native
 __all__ = [
     "PDESystem", "Differential", "Interval", "Eq", "ifelse", "parameters", "variables",
     "grid_field", "GridField", "MOLFiniteDifference", "UpwindScheme", "WENOScheme", "Lowered",
-    "lower", "uniform_grid", "calculate_weights", "ODEProblem", "ODESolution", "RK4", "SSPRK33",
+    "lower", "uniform_grid", "center_align", "edge_align", "calculate_weights", "ODEProblem", "ODESolution", "RK4", "SSPRK33",
     "codegen", "discretize", "symbolic_discretize", "solve", "native",
 ]

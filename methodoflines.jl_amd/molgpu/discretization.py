@@ -46,22 +46,30 @@ class WENOScheme:
         self.epsilon = float(epsilon)
 
 
+# grid alignments (src/interface/grid_types.jl:1-25): nodes on the domain ends, or cell centres
+# with one ghost node past each end (the boundary lies half-way between the ghost and the first
+# cell centre)
+center_align = "center"
+edge_align = "edge"
+
+
 class MOLFiniteDifference:
     """Discretization settings (src/interface/MOLFiniteDifference.jl:31-76).
 
-    ``dxs`` maps each spatial variable to a step (float) or a node count (int).
+    ``dxs`` maps each spatial variable to a step (float) or a node count (int; with
+    ``grid_align=edge_align`` a cell count, prepare_dx, discretize_vars.jl:283-285).
     ``dtype`` ("f64" / "f32") is the state and arithmetic type of the GPU path.
     """
 
     def __init__(self, dxs, time=None, approx_order: int = 2, advection_scheme=None,
-                 grid_align: str = "center", dtype: str = "f64", **kwargs):
+                 grid_align: str = center_align, dtype: str = "f64", **kwargs):
         if approx_order % 2 != 0:
             warnings.warn("Discretization approx_order must be even, rounding up to %d" % (approx_order + 1))
             approx_order += 1
         if approx_order < 1:
             raise ValueError("approx_order must be at least 1")
-        if grid_align != "center":
-            raise NotImplementedError("only center-aligned grids are supported")
+        if grid_align not in (center_align, edge_align):
+            raise NotImplementedError("grid_align must be center_align or edge_align (staggered grids are not supported)")
         if dtype not in ("f64", "f32"):
             raise ValueError("dtype must be 'f64' or 'f32'")
         self.dxs = dict(dxs)
@@ -110,6 +118,42 @@ def uniform_grid(lo: float, hi: float, dx) -> np.ndarray:
         warnings.warn("d%s does not divide domain exactly, adding grid point at %s" % ("x", hi))
         nodes.append(hi)
     return np.array(nodes, dtype=np.float64)
+
+
+def edge_step(lo: float, hi: float, dx) -> float:
+    """Step of an edge-aligned axis: a float dx as given, an integer one a cell count
+    (prepare_dx(dx::Integer, xdomain, ::EdgeAlignedGrid), discretize_vars.jl:284)."""
+    if isinstance(dx, (int, np.integer)) and not isinstance(dx, bool):
+        if dx < 1:
+            raise ValueError("need at least one cell")
+        return (float(hi) - float(lo)) / int(dx)
+    return float(dx)
+
+
+def edge_grid(lo: float, hi: float, h: float) -> np.ndarray:
+    """Nodes (lo - h/2):h:(hi + h/2) of an edge-aligned axis as Julia's float range builds
+    them (discretize_vars.jl:380-382): the ends computed in FP64, as many steps as fit (with a
+    relative tolerance of 1e-10 on the quotient), node k the correctly rounded start + k*h.
+    Unlike the centre-aligned grid no node is appended at the end."""
+    start, stop = float(lo) - h / 2, float(hi) + h / 2
+    q = (stop - start) / h
+    m = int(np.floor(q + 1e-10 * max(1.0, q)))
+    if m < 2:
+        raise ValueError("an edge-aligned axis needs at least one cell")
+    fs, fh = Fraction(start), Fraction(h)
+    return np.array([float(fs + k * fh) for k in range(m + 1)], dtype=np.float64)
+
+
+def half_row(op: "fornberg.DerivativeOperator", i: int, n: int):
+    """(weights, first tap) of a half-point operator at index i (the half point between nodes i
+    and i+1) on a non-periodic axis of n nodes (get_half_offset_weights_and_stencil,
+    half_offset_centred_difference.jl:9-36)."""
+    bpc, BL, L = op.boundary_point_count, op.boundary_stencil_length, op.stencil_length
+    if i <= bpc:
+        return op.low_boundary_coefs[i - 1], 1
+    if i > n - bpc:
+        return op.high_boundary_coefs[n - i - 1], n - BL + 1
+    return op.stencil_coefs, i + 1 - L // 2
 
 
 # --------------------------------------------------------------------------------------
@@ -234,6 +278,7 @@ class _Lowering:
             fargs.append([a for a in dv.args if a != self.t])
         dom = {iv.var: iv for iv in sys.domain}
         self.domain = dom
+        self.edge = disc.grid_align == edge_align
         self.iface = None
         self.sym_map = {}
         if all(a == fargs[0] for a in fargs):
@@ -253,9 +298,15 @@ class _Lowering:
                 g = self.iface["grid"]
             elif isinstance(dx, (list, tuple, np.ndarray)):
                 # nonuniform grid: the nodes themselves (generate_grid, discretize_vars.jl:219-233)
+                if disc.grid_align == edge_align:
+                    raise NotImplementedError("edge-aligned grids are supported on uniform axes only")
                 g = np.asarray(dx, dtype=np.float64).copy()
                 if g.ndim != 1 or len(g) < 5 or not np.all(np.diff(g) > 0):
                     raise ValueError("a nonuniform grid must be at least 5 strictly increasing nodes")
+            elif disc.grid_align == edge_align:
+                # cell centres plus a ghost past each end: (lo - h/2):h:(hi + h/2)
+                # (generate_grid, discretize_vars.jl:373-390; prepare_dx, :283-285)
+                g = edge_grid(dom[x].lo, dom[x].hi, edge_step(dom[x].lo, dom[x].hi, dx))
             else:
                 g = uniform_grid(dom[x].lo, dom[x].hi, dx)
             self.coords.append(g)
@@ -268,6 +319,8 @@ class _Lowering:
             self.nonuniform.append(nu)
             if nu:
                 dxv = float("nan")
+            elif self.edge:
+                dxv = edge_step(dom[x].lo, dom[x].hi, dxv)
             elif isinstance(dxv, (int, np.integer)) and not isinstance(dxv, bool):
                 dxv = (dom[x].hi - dom[x].lo) / (dxv - 1)
             self.dx.append(float(dxv))
@@ -833,6 +886,9 @@ class _Lowering:
                 el, eu = (ext if not self.periodic[a] else 0), (ext if not self.periodic[a] else 0)
                 if ext and (self._is_iface(f, a, 0) or up_if):
                     raise NotImplementedError("WENO extrapolated nodes next to an interface are not supported")
+                if self.edge and (ext or self.periodic[a] or self._is_iface(f, a, 0) or up_if):
+                    raise NotImplementedError("edge-aligned grids are supported with boundary conditions at both ends "
+                                              "and no extrapolated nodes (no periodic axes, interfaces or WENO)")
                 self.extent[f][a] = (el, eu)
                 r0, r1 = self._range(f, a)
                 self.lo[f][a] = r0 + max(cl, el)
@@ -874,7 +930,23 @@ class _Lowering:
                     rows = {g: [] for g in ghosts}   # g -> list of (coef dict node->w, gcoef)
                     bc = self.bc[f][a][side]
                     bnode = r0 if side == 0 else r1
-                    if bc is not None:
+                    if bc is not None and self.edge:
+                        # the condition holds at the cell edge between the ghost and the first cell
+                        # centre: values there by half-point interpolation (order max(4, p)), derivatives
+                        # by half-point differences (boundary_value_maps, generate_bc_eqs.jl:79-161;
+                        # the upper edge is half point n - 1, newindex(shift = true), MOL_utils.jl:87)
+                        coefs, gexpr = bc
+                        row = {}
+                        hp = 1 if side == 0 else n - 1
+                        for d, cval in coefs.items():
+                            if cval == 0.0:
+                                continue
+                            op = fornberg.half_centered_difference(d, max(4, self.p) if d == 0 else self.p, self.dx[a])
+                            w, first = half_row(op, hp, n)
+                            for k, wk in enumerate(w):
+                                row[first + k] = row.get(first + k, 0.0) + cval * wk
+                        rows[bnode].append((row, 1.0))
+                    elif bc is not None:
                         coefs, gexpr = bc
                         row = {}
                         for d, cval in coefs.items():

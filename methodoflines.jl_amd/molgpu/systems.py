@@ -13,8 +13,9 @@ from .pdesys import Differential, Eq, Interval, PDESystem, grid_field, ifelse, p
 
 
 def heat1d(n: int = 101, D: float = 1.0, bcl=(1.0, 0.0, 0.0), bch=(1.0, 0.0, 0.0), approx_order: int = 2,
-           dtype: str = "f64"):
-    """u_t = D u_xx on [0, 1]; boundary a0 u + a1 u_x = g at each end; u(0, x) = sin(pi x)."""
+           dtype: str = "f64", grid_align: str = "center"):
+    """u_t = D u_xx on [0, 1]; boundary a0 u + a1 u_x = g at each end; u(0, x) = sin(pi x).
+    n: nodes of a centre-aligned grid, cells of an edge-aligned one (grid_align="edge")."""
     t, x = parameters("t x")
     u = variables("u")
     Dt, Dx = Differential(t), Differential(x)
@@ -27,7 +28,7 @@ def heat1d(n: int = 101, D: float = 1.0, bcl=(1.0, 0.0, 0.0), bch=(1.0, 0.0, 0.0
 
     bcs = [Eq(u(0, x), sp.sin(sp.pi * x)), bc(0, bcl), bc(1, bch)]
     sys_ = PDESystem([eq], bcs, [Interval(t, 0.0, 0.1), Interval(x, 0.0, 1.0)], [t, x], [u(t, x)])
-    return sys_, MOLFiniteDifference({x: n}, t, approx_order=approx_order, dtype=dtype)
+    return sys_, MOLFiniteDifference({x: n}, t, approx_order=approx_order, dtype=dtype, grid_align=grid_align)
 
 
 def brusselator2d(n: int = 513, alpha: float = 10.0, approx_order: int = 2, dtype: str = "f64"):

@@ -999,6 +999,81 @@ int orc_diffusion1d_rhs(int n, double xlo, double xhi, int p, double D, const do
   }
 }
 
+// ---- 1D diffusion u_t = D u_xx on an edge-aligned grid of [xlo, xhi]: ncell cells of width
+//      h = (xhi - xlo) / ncell, nodes x_i = xlo - h/2 + (i - 1) h for i = 1..n = ncell + 2
+//      (generate_grid, discretize_vars.jl:373-390), unknowns at the cell centres 2..n-1.  The
+//      condition a0 u + a1 u_x = g of each end holds at the cell edge: u there by the half-point
+//      interpolation of order max(4, p), u_x by the half-point difference of order p
+//      (boundary_value_maps, generate_bc_eqs.jl:79-161), each row chosen by
+//      get_half_offset_weights_and_stencil (half_offset_centred_difference.jl:9-36) at half point
+//      1 (lower edge) or n - 1 (upper edge, newindex(shift = true), MOL_utils.jl:87); the row is
+//      solved for the ghost node.  Interior nodes take the node-centred rows of D_xx
+//      (centered_difference.jl:5-30).  Constant g.
+extern "C" int orc_diffusion1d_edge_rhs(int ncell, double xlo, double xhi, int p, double D, const double* bcl,
+                                        const double* bch, const double* u, double t, double* du) {
+  (void)t;
+  try {
+    if (ncell < 4) return -1;
+    const int n = ncell + 2;
+    const double h = (xhi - xlo) / ncell;
+    std::vector<double> v(n + 1, 0.0);
+    for (int i = 2; i <= n - 1; ++i) v[i] = u[i - 2];
+    auto hrow = [n](const Op& o, int i, int& first) -> const std::vector<double>& {
+      if (i <= o.bpc) {
+        first = 1;
+        return o.low[i - 1];
+      }
+      if (i > n - o.bpc) {
+        first = n - o.BL + 1;
+        return o.high[n - i - 1];
+      }
+      first = i + 1 - o.L / 2;
+      return o.st;
+    };
+    const Op oi = half_op(0, std::max(4, p), h), od = half_op(1, p, h);
+    for (int side = 0; side < 2; ++side) {
+      const double* b = side ? bch : bcl;
+      const int ghost = side ? n : 1, hp = side ? n - 1 : 1;
+      std::vector<double> c(n + 1, 0.0);
+      int f = 0;
+      if (b[0] != 0.0) {
+        const std::vector<double>& w = hrow(oi, hp, f);
+        for (size_t k = 0; k < w.size(); ++k) c[f + k] += b[0] * w[k];
+      }
+      if (b[1] != 0.0) {
+        const std::vector<double>& w = hrow(od, hp, f);
+        for (size_t k = 0; k < w.size(); ++k) c[f + k] += b[1] * w[k];
+      }
+      if (c[side ? 1 : n] != 0.0) return -1;   // a row reaching the other ghost: grid too small
+      double rest = b[2];
+      for (int k = 1; k <= n; ++k)
+        if (k != ghost) rest -= c[k] * v[k];
+      v[ghost] = rest / c[ghost];
+    }
+    const Op oc = centered(2, p, h);
+    for (int i = 2; i <= n - 1; ++i) {
+      const std::vector<double>* w;
+      int first;
+      if (i <= oc.bpc) {
+        w = &oc.low[i - 1];
+        first = 1;
+      } else if (i > n - oc.bpc) {
+        w = &oc.high[n - i];
+        first = n - oc.BL + 1;
+      } else {
+        w = &oc.st;
+        first = i - oc.L / 2;
+      }
+      double acc = 0.0;
+      for (size_t k = 0; k < w->size(); ++k) acc += (*w)[k] * v[first + k];
+      du[i - 2] = D * acc;
+    }
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
 // ---- two variables on adjacent intervals joined by an interface condition c1(t, m) ~ c2(t, m):
 //      Dt c_k = D_k Dxx c_k, c1 on n1 nodes of [a, m], c2 on n2 nodes of [m, b] (same spacing
 //      dx), Dirichlet c1 = gl at a and c2 = gr at b.  The interface keeps the upper node of c1 as

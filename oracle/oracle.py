@@ -42,6 +42,8 @@ def lib():
             "orc_diffreact_nd_rhs": [C.c_int, _ip, C.POINTER(C.c_void_p), C.c_int, C.c_double, C.c_double, _ip,
                                      _dp, C.c_double, _dp],
             "orc_diffusion1d_rhs": [C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double, _dp],
+            "orc_diffusion1d_edge_rhs": [C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double,
+                                         _dp],
             "orc_diffusion1d_integrate": [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double, C.c_double, C.c_longlong, C.c_int],
             "orc_bruss2d_rhs": [C.c_int, C.c_int, C.c_double, _dp, C.c_double, _dp, C.c_int],
             "orc_bruss2d_integrate": [C.c_int, C.c_int, C.c_double, _dp, C.c_double, C.c_double, C.c_longlong, C.c_int],
@@ -185,6 +187,14 @@ def diffusion1d_rhs(n, D, bcl, bch, u, t=0.0, p=2, xlo=0.0, xhi=1.0):
     u = _state(u)
     du = np.empty_like(u)
     _ok(lib().orc_diffusion1d_rhs(n, xlo, xhi, p, D, _f64(bcl), _f64(bch), u, float(t), du))
+    return du
+
+
+def diffusion1d_edge_rhs(ncell, D, bcl, bch, u, t=0.0, p=2, xlo=0.0, xhi=1.0):
+    """u_t = D u_xx on the edge-aligned grid of ncell cells (state: the ncell cell centres)."""
+    u = _state(u)
+    du = np.empty_like(u)
+    _ok(lib().orc_diffusion1d_edge_rhs(ncell, xlo, xhi, p, D, _f64(bcl), _f64(bch), u, float(t), du))
     return du
 
 
