@@ -110,3 +110,33 @@ def test_edge_grid_refusals():
         lower(sys_, MOLFiniteDifference({sys_.ivs[1]: np.linspace(0, 1, 11)}, t, grid_align=edge_align))
     with pytest.raises(NotImplementedError):
         MOLFiniteDifference({x: 0.1}, t, grid_align="stagger")
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_generated_2d_kernel_exact_on_polynomials(p):
+    """u_t = u_xx + u_yy on [0,1]^2 with edge-aligned axes and Dirichlet conditions taken from
+    u = polynomial (degree 2 at p = 2, 3 at p = 4, in each variable): the generated kernel's
+    right-hand side is the exact Laplacian at every cell centre, so the 2D closures (tangential
+    coordinates of the edge ghosts at cell centres) are right."""
+    t, x, y = parameters("t x y")
+    u = variables("u")
+    Dx, Dy = Differential(x), Differential(y)
+    deg = 2 if p == 2 else 3
+    P = (0.3 + 0.7 * x - 0.4 * x ** 2 + (0.2 * x ** 3 if deg == 3 else 0)) * (1.1 - 0.5 * y + 0.6 * y ** 2 +
+                                                                              (-0.3 * y ** 3 if deg == 3 else 0))
+    eq = Eq(Differential(t)(u(t, x, y)), (Dx ** 2)(u(t, x, y)) + (Dy ** 2)(u(t, x, y)))
+    bcs = [Eq(u(0, x, y), P), Eq(u(t, 0, y), P.subs(x, 0)), Eq(u(t, 1, y), P.subs(x, 1)),
+           Eq(u(t, x, 0), P.subs(y, 0)), Eq(u(t, x, 1), P.subs(y, 1))]
+    sys_ = PDESystem([eq], bcs, [Interval(t, 0.0, 1.0), Interval(x, 0.0, 1.0), Interval(y, 0.0, 1.0)],
+                     [t, x, y], [u(t, x, y)])
+    low = lower(sys_, MOLFiniteDifference({x: 12, y: 10}, t, approx_order=p, grid_align=edge_align))
+    assert low.n == [14, 12]
+    xc, yc = low.coords[0][1:-1], low.coords[1][1:-1]
+    X, Y = np.meshgrid(xc, yc, indexing="ij")
+    f = sp.lambdify((x, y), P)
+    lap = sp.lambdify((x, y), sp.diff(P, x, 2) + sp.diff(P, y, 2))
+    u0 = f(X, Y).reshape(-1, order="F")
+    assert np.allclose(low.u0, u0, rtol=0, atol=1e-14)
+    got = genhost.rhs(low, u0, tile=(8, 4, 1))
+    want = lap(X, Y).reshape(-1, order="F")
+    assert np.max(np.abs(got - want)) < 1e-9 * max(1.0, np.max(np.abs(want)))
