@@ -19,7 +19,9 @@
  *   - pointers named *_h are host pointers; all other array pointers are device
  *     pointers.  `stream` is a hipStream_t passed as void* (NULL = default stream).
  *   - calls are asynchronous on `stream` unless named *_h (those synchronize).
- *   - one handle must not be used from two threads at once.
+ *   - one handle must not be used from two threads at once, and work it enqueues on two
+ *     streams must not overlap: its workspace (stage vectors, reverse-mode buffers, norm
+ *     partials) is shared by all of its calls.
  *   - state vectors use the reference's unknown ordering: variable-major, and within
  *     a variable the first spatial axis fastest (docs/src/generated/bruss_code.md:48-79,
  *     src/interface/solution/timedep.jl:30-47).
