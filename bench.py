@@ -622,12 +622,15 @@ def main():
     stage = {"kernel": "mol_rhs_tiles (generated; RK4 stage epilogue)", "avg_us": st_us, "bytes_per_launch": st_bytes,
              "achieved": st_ach, "frac": st_ach / HBM_PEAK_GBS, "traffic": STAGE_TRAFFIC_BYTES,
              "value": points * 4 * args.steps / (ms_st / 1e3) / 1e9}
+    spl = max(1, prob.plan()["fused_steps_per_launch"]) if fused else 1
     if fused:
-        # Fused path: one launch per step reads u (with a 4-node halo, from L2) and writes u:
-        # 32 B per point algorithmic (2 variables, FP64).
-        launch_us = ms * 1e3 / args.steps
-        bytes_launch = points * 32
-        kname = "mol_rk4_fused (generated; whole RK4 step per launch)"
+        # Fused path: a launch takes spl whole RK4 steps (spl = 2: an 8-node halo).  Algorithmic
+        # bytes of one step: read u, write u, 32 B per point (2 variables, FP64); a launch of
+        # spl steps is priced at spl such steps (it moves the bytes of one, which is the point of
+        # fusing them).  Launch time = the time of spl steps.
+        launch_us = ms * 1e3 * spl / args.steps
+        bytes_launch = points * 32 * spl
+        kname = "mol_rk4_fused (generated; %d whole RK4 step%s per launch)" % (spl, "s" if spl > 1 else "")
         traffic = FUSED_TRAFFIC_BYTES
     else:
         launch_us, bytes_launch, kname, traffic = st_us, st_bytes, stage["kernel"], STAGE_TRAFFIC_BYTES
@@ -650,7 +653,7 @@ def main():
         "data": "synthetic: the configured initial condition u=22(y(1-y))^1.5, v=27(x(1-x))^1.5",
         "config": {"workload": "brusselator2d periodic 512x512 unknowns/variable, 2 variables, approx_order 2, "
                                "RK4 fixed dt=1e-5", "points": points, "rhs_per_step": 4,
-                   "graph": bool(args.graph), "fused_step": fused},
+                   "graph": bool(args.graph), "fused_step": fused, "steps_per_launch": spl},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kname, "avg_us": launch_us, "bytes_per_launch": bytes_launch,

@@ -247,7 +247,8 @@ int mol_set_tile(mol_handle* h, int32_t tx, int32_t ty, int32_t tz);
 
 /* Launch plan: info[0..2] tile shape, info[3] core tiles, info[4] generic tiles, info[5] WENO tiles,
  * info[6] 1 when the core tiles run the generated kernel, info[7] 1 when the IR fits the
- * interpreter's core kernel, info[8] tiles of the fused RK4 step kernel (0: none). */
+ * interpreter's core kernel, info[8] tiles of the fused RK4 step kernel (0: none), info[9] RK4 steps
+ * per fused launch (1, or 2 with an 8-reach halo; 0: none, or a resident 1D tile). */
 int mol_get_plan(const mol_handle* h, int32_t* info, int32_t ninfo);
 
 /* Bind auxiliary full-grid arrays (device pointers, naux of them, each prod(n) values). */

@@ -718,6 +718,28 @@ bool fused_geometry(mol_handle* h, const mol_desc* d, const KArgs<double>& ka, c
   // two threads per tile node (every stage region is larger than the tile), at most 1024
   g.nthr = std::max(256, std::min(1024, (2 * g.tile[0] * g.tile[1] * g.tile[2] + 63) / 64 * 64));
   if (const char* ne = std::getenv("MOL_FUSED_NTHR")) g.nthr = (std::max(64, std::min(1024, std::atoi(ne))) + 63) / 64 * 64;
+  // Two steps per launch (an 8-reach halo, a fourth LDS array for the first step's running
+  // sums) when every tile is resident at once: the launch overhead, the staging latency and the
+  // store drain then come once per two steps, against stage regions about a quarter larger
+  // (Brusselator 512^2, 32x32 tiles: 6168 against 4920 node evaluations per tile and step).
+  // With more tiles than compute units, work-groups of the one-step kernel overlap each
+  // other's phases instead.  MOL_FUSED_STEPS=1 or 2 overrides.
+  g.fsteps = 1;
+  if (d->ndim > 1 || g.ntile[0] > 1) {   // not one resident 1D tile
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess) ncu = 0;
+    int64_t vol2 = 1, nt = 1;
+    bool reach1 = true;
+    for (int a = 0; a < 3; ++a) {
+      vol2 *= g.tile[a] + 16 * (a < d->ndim ? ka.halo[a] : 0);
+      nt *= g.ntile[a];
+      if (a < d->ndim) reach1 = reach1 && ka.halo[a] <= 1;
+    }
+    const size_t lds2 = (size_t)4 * vol2 * d->nfields * h->esize();
+    bool two = reach1 && lds2 <= 150 * 1024 && ncu > 0 && nt <= ncu;
+    if (const char* se = std::getenv("MOL_FUSED_STEPS")) two = std::atoi(se) == 2 && reach1 && lds2 <= 150 * 1024;
+    if (two) g.fsteps = 2;
+  }
   return true;
 }
 
@@ -769,6 +791,7 @@ void setup_fused(mol_handle* h, const mol_desc* d, const KArgs<double>& ka, cons
   h->fused_tiles = g.ntile[0] * g.ntile[1] * g.ntile[2];
   h->fused_nthr = g.nthr;
   h->fused_resident = src.find("// resident steps per launch: any") != std::string::npos;
+  h->fused_steps = src.find("// resident steps per launch: 2") != std::string::npos ? 2 : 1;
 }
 
 int setup_paths(mol_handle* h, const mol_desc* d) {
@@ -903,7 +926,8 @@ int enqueue_steps(mol_handle* h, const Work& w, T* u, const T* p, double dt, int
     T* src = u;
     T* dst = y1;
     for (int64_t step = 0; step < nsteps;) {
-      const int nsub = h->fused_resident ? (int)std::min<int64_t>(nsteps - step, 1024) : 1;
+      const int nsub = h->fused_resident ? (int)std::min<int64_t>(nsteps - step, 1024)
+                                         : (h->fused_steps == 2 && nsteps - step >= 2 ? 2 : 1);
       Epi<T> e{};
       e.tnow = w.tnow + h->clk;
       e.tnext = w.tnow + (1 - h->clk);
@@ -1706,10 +1730,11 @@ int mol_get_plan(const mol_handle* h, int32_t* info, int32_t ninfo) {
     for (int i = 0; i < ninfo && i < 8; ++i) info[i] = b[i];
     return 0;
   }
-  int32_t v[9] = {h->ka64.tile[0], h->ka64.tile[1], h->ka64.tile[2], h->ncore, h->nbound,
-                  0, h->jit_fn != nullptr ? 1 : 0,
-                  h->core_interp_ok ? 1 : 0, h->fused_fn ? h->fused_tiles : 0};
-  for (int i = 0; i < ninfo && i < 9; ++i) info[i] = v[i];
+  int32_t v[10] = {h->ka64.tile[0], h->ka64.tile[1], h->ka64.tile[2], h->ncore, h->nbound,
+                   0, h->jit_fn != nullptr ? 1 : 0,
+                   h->core_interp_ok ? 1 : 0, h->fused_fn ? h->fused_tiles : 0,
+                   h->fused_fn ? (h->fused_resident ? 0 : h->fused_steps) : 0};
+  for (int i = 0; i < ninfo && i < 10; ++i) info[i] = v[i];
   return 0;
 }
 

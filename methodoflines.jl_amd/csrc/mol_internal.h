@@ -18,6 +18,7 @@ struct JitGeom {
   int tile[3], halo[3], ext[3], vol, ntile[3];
   int ct_lo[3], ct_n[3], ncore;                       // core tiles (interior rows only)
   int nthr;                                           // threads per work-group (multiple of 64)
+  int fsteps = 1;                                     // fused RK4: steps per launch (1 or 2)
 };
 
 // Pointer arguments of the generated kernel (layout identical to JPtrs of mol_jit_prelude.h).
@@ -189,6 +190,7 @@ struct mol_handle {
   int fused_tiles = 0;
   int fused_nthr = 256;
   bool fused_resident = false;     // one 1D tile: many steps per fused launch
+  int fused_steps = 1;             // steps per fused launch otherwise (1, or 2 with an 8-reach halo)
   std::string fused_reason;               // why there is no fused kernel
   bool no_fuse = false;                   // this integration runs the stage kernels (MOL_FLAG_UNFUSED)
   int vjp_nthr = 256;                     // threads per work-group of the generated VJP kernel

@@ -336,6 +336,7 @@ class Gen {
   int slot_branch(int k, const std::vector<int>& axes, size_t a, Var* v);
   int nonlinlap(const mol_slot& s, int qo, int qi, int qp, const Var& v);
   void build_node(bool core, std::vector<int>& outs, int& region);
+  void fold_linear(int first_region, const std::vector<int>& outs);
 
   // ---- printing
   std::string ref(int id) const;
@@ -847,6 +848,7 @@ void Gen::build_node(bool core, std::vector<int>& outs, int& region) {
   node.kind = CTX_NODE;
   outs.clear();
   for (int f = 0; f < d_->nfields; ++f) outs.push_back(prog(d_->rhs_prog[f], node));
+  if (ok_) fold_linear(region, outs);
   // drop the operations no output depends on (e.g. the terms of a boundary row whose weight is
   // zero, built before the row drops them): their taps would widen the staged halo
   live_.resize(nodes_.size(), 0);
@@ -865,6 +867,140 @@ void Gen::build_node(bool core, std::vector<int>& outs, int& region) {
   for (size_t r = (size_t)region; r < regs_.size(); ++r) {
     std::vector<int>& v = regs_[r].nodes;
     v.erase(std::remove_if(v.begin(), v.end(), [&](int id) { return !live_[id]; }), v.end());
+  }
+}
+
+// Linear forms collected as mtkcompile's expand-and-collect does before build_function: the
+// reference's generated f carries one coefficient per distinct tap (the Brusselator row reads
+// 1.0 + (-644.4) u[i,j] + 160.0 u[i+1,j] + ... + u[i,j]^2 v[i,j],
+// docs/src/generated/bruss_code.md:46-118), where the operator tree of the equation has a
+// stencil sum per derivative, a coefficient product per term and a sum over terms.  Chains of
+// additions, subtractions, negations, products with (or divisions by) a literal and stencil
+// combinations with numeric weights, whose intermediate values have one use, become one
+// X_LINCOMB: a constant plus one weight per distinct operand (taps, and values the chain does
+// not absorb).  Values that read no LDS tap are never absorbed, so what depends only on
+// constants, coordinates and time keeps its own node (hoisted by the fused step).  The weights
+// are products and sums of the literals in FP64; the result differs from the unfolded tree by
+// rounding only.  Batched class bodies read their weights at run time and are not folded.
+void Gen::fold_linear(int first_region, const std::vector<int>& outs) {
+  if (param_) return;
+  const int N = (int)nodes_.size();
+  std::vector<int> reg_of(N, -1), uses(N, 0), consumer(N, -1);
+  for (size_t r = (size_t)first_region; r < regs_.size(); ++r)
+    for (int id : regs_[r].nodes) reg_of[id] = (int)r;
+  auto use = [&](int a, int by) {
+    uses[a]++;
+    consumer[a] = by;
+  };
+  for (size_t r = (size_t)first_region; r < regs_.size(); ++r)
+    for (int id : regs_[r].nodes) {
+      const Node& n = nodes_[id];
+      for (int a : n.a) use(a, id);
+      if (n.kind == N_BRANCH)
+        for (int sr : n.reg)
+          if (regs_[sr].result >= 0) use(regs_[sr].result, -1);
+    }
+  for (int f : outs) use(f, -1);
+  auto is_lit = [&](int id) { return nodes_[id].kind == N_LIT; };
+  auto foldable = [&](int id) {
+    const Node& n = nodes_[id];
+    if (n.kind != N_OP || reg_of[id] < 0) return false;
+    switch (n.op) {
+      case MOL_OP_ADD: case MOL_OP_SUB: case MOL_OP_NEG: return true;
+      case MOL_OP_MUL: return is_lit(n.a[0]) || is_lit(n.a[1]);
+      case MOL_OP_DIV: return is_lit(n.a[1]) && nodes_[n.a[1]].val != 0.0;
+      case X_LINCOMB: return n.wx.empty();
+      default: return false;
+    }
+  };
+  // ids grow from operands to users, so one pass in id order is a topological order
+  std::vector<char> dyn(N, 0), fold(N, 0), absorbed(N, 0);
+  for (int id = 0; id < N; ++id) {
+    const Node& n = nodes_[id];
+    if (n.kind == N_LDS) dyn[id] = 1;
+    for (int a : n.a) dyn[id] |= dyn[a];
+    if (n.kind == N_BRANCH)
+      for (int sr : n.reg)
+        if (regs_[sr].result >= 0) dyn[id] |= dyn[regs_[sr].result];
+    fold[id] = foldable(id);
+  }
+  for (int id = 0; id < N; ++id)
+    absorbed[id] = fold[id] && dyn[id] && uses[id] == 1 && consumer[id] >= 0 && fold[consumer[id]] &&
+                   reg_of[consumer[id]] == reg_of[id];
+  struct Form {
+    double c = 0.0;
+    std::vector<std::pair<int, double>> t;   // operand, weight (first-appearance order)
+  };
+  std::map<int, Form> F;
+  auto axpy = [](Form& y, double s, const Form& x) {
+    y.c += s * x.c;
+    for (const auto& tw : x.t) {
+      auto it = std::find_if(y.t.begin(), y.t.end(), [&](const std::pair<int, double>& p) { return p.first == tw.first; });
+      if (it == y.t.end()) y.t.push_back({tw.first, s * tw.second});
+      else it->second += s * tw.second;
+    }
+  };
+  // reads of one array at one displacement are one operand, whichever node made them
+  std::map<std::array<int, 5>, int> tap;
+  std::vector<int> canon(N);
+  for (int id = 0; id < N; ++id) {
+    const Node& n = nodes_[id];
+    canon[id] = id;
+    if (n.kind == N_LDS)
+      canon[id] = tap.emplace(std::array<int, 5>{n.arr, n.dv[0], n.dv[1], n.dv[2], n.core ? 1 : 0}, id).first->second;
+  }
+  auto G = [&](int a) {
+    Form g;
+    if (is_lit(a)) g.c = nodes_[a].val;
+    else if (absorbed[a]) g = F[a];
+    else g.t.push_back({canon[a], 1.0});
+    return g;
+  };
+  for (int id = 0; id < N; ++id) {
+    if (!fold[id]) continue;
+    const Node& n = nodes_[id];
+    Form f;
+    switch (n.op) {
+      case MOL_OP_ADD: axpy(f, 1.0, G(n.a[0])); axpy(f, 1.0, G(n.a[1])); break;
+      case MOL_OP_SUB: axpy(f, 1.0, G(n.a[0])); axpy(f, -1.0, G(n.a[1])); break;
+      case MOL_OP_NEG: axpy(f, -1.0, G(n.a[0])); break;
+      case MOL_OP_MUL:
+        if (is_lit(n.a[0])) axpy(f, nodes_[n.a[0]].val, G(n.a[1]));
+        else axpy(f, nodes_[n.a[1]].val, G(n.a[0]));
+        break;
+      case MOL_OP_DIV: axpy(f, 1.0 / nodes_[n.a[1]].val, G(n.a[0])); break;
+      default:   // numeric stencil combination
+        for (size_t k = 0; k < n.a.size(); ++k) axpy(f, n.w[k], G(n.a[k]));
+        break;
+    }
+    F[id] = std::move(f);
+  }
+  // each chain's last value becomes one combination; absorbed values lose their last user
+  for (int id = 0; id < N; ++id) {
+    if (!fold[id] || absorbed[id] || !dyn[id]) continue;
+    const Form& f = F[id];
+    std::vector<int> args;
+    std::vector<double> w;
+    if (f.c != 0.0) {
+      args.push_back(lit_node(f.c));
+      w.push_back(1.0);
+    }
+    for (const auto& tw : f.t)
+      if (tw.second != 0.0) {
+        args.push_back(tw.first);
+        w.push_back(tw.second);
+      }
+    if (args.empty()) {
+      args.push_back(lit_node(0.0));
+      w.push_back(1.0);
+    }
+    Node& n = nodes_[id];
+    n.op = X_LINCOMB;
+    n.a = std::move(args);
+    n.w = std::move(w);
+    n.wi.clear();
+    n.wj.clear();
+    n.wx.clear();
   }
 }
 
@@ -961,7 +1097,14 @@ std::string Gen::op_expr(const Node& n, int id) const {
     case MOL_OP_POWI: return powi_expr(A(0), n.imm);
     case X_LINCOMB: {
       std::string e;
+      const bool numeric = n.wx.empty() && (!param_ || n.wi.empty());
       for (size_t k = 0; k < n.a.size(); ++k) {
+        // unit weights (folded linear forms) multiply nothing
+        if (numeric && (n.w[k] == 1.0 || n.w[k] == -1.0)) {
+          const bool neg = n.w[k] < 0.0;
+          e = k == 0 ? (neg ? "(-" + A((int)k) + ")" : A((int)k)) : e + (neg ? " - " : " + ") + A((int)k);
+          continue;
+        }
         const std::string term = weight(n, k) + " * " + A((int)k);
         e = k == 0 ? term : e + " + " + term;
       }
@@ -1821,7 +1964,7 @@ std::string Gen::source_fused(bool f64) {
     // boundary rows' width, which the reach-1 geometry keeps inside the tile
     if (us.reach[a] > 2 || (us.reach[a] > 1 && !g_.periodic[a])) { bad("fused RK4: reach too large"); return ""; }
     reach[a] = us.reach[a];
-    halo[a] = 4 * us.reach[a];
+    halo[a] = 4 * g_.fsteps * us.reach[a];
   }
   std::ostringstream o;
   o << "// generated fused RK4 step of one PDE system\n";
@@ -1848,22 +1991,25 @@ std::string Gen::source_fused(bool f64) {
   o << "extern \"C\" __global__ void __launch_bounds__(" << g_.nthr
     << ") mol_rk4_fused(const T* __restrict__ u, T* __restrict__ unew, Epi ep, JPtrs jp, int nsub) {\n"
        "  __shared__ T U[GeoF::NST * GeoF::VOL];\n  __shared__ T Ya[GeoF::NST * GeoF::VOL];\n"
-       "  __shared__ T Yb[GeoF::NST * GeoF::VOL];\n"
-       "  // the step's time from the device clock, or (first launch of a call) by value in tofs\n"
+       "  __shared__ T Yb[GeoF::NST * GeoF::VOL];\n";
+  // two steps per launch: the running sums of the first step's frame positions
+  if (g_.fsteps == 2 && !resident) o << "  __shared__ T Ac[GeoF::NST * GeoF::VOL];\n";
+  else o << "  T* const Ac = nullptr;\n";
+  o << "  // the step's time from the device clock, or (first launch of a call) by value in tofs\n"
        "  const double t = ep.tnow ? ep.tnow[0] : ep.tofs;\n";
   if (resident) {
     o << "  mol_rk4_fused_resident<GeoF>(U, Ya, Yb, &mol_fc[0][0], (int)(sizeof(mol_fc[0]) / sizeof(double)), mol_fo,\n"
          "                               u, unew, ep, jp, t, nsub);\n}\n";
   } else {
-    o << "  (void)nsub;\n"
-         "  int b = mol_xcd_tile<GeoBase::NBLK>((int)blockIdx.x);\n"
+    // nsub steps (1, or 2 with the 8-reach halo); the clock advances as by nsub launches
+    o << "  int b = mol_xcd_tile<GeoBase::NBLK>((int)blockIdx.x);\n"
          "  const int tx = b % GeoBase::NT0;\n  b /= GeoBase::NT0;\n"
          "  const int ty = b % GeoBase::NT1;\n  const int tz = b / GeoBase::NT1;\n"
-         "  if (ep.tnext && blockIdx.x == 0 && threadIdx.x == 0) ep.tnext[0] = t + ep.tadv;\n"
-         "  mol_rk4_fused_tile<GeoF>(U, Ya, Yb, &mol_fc[0][0], (int)(sizeof(mol_fc[0]) / sizeof(double)), mol_fo,\n"
-         "                           tx, ty, tz, u, unew, ep, jp, t);\n}\n";
+         "  if (ep.tnext && blockIdx.x == 0 && threadIdx.x == 0) ep.tnext[0] = nsub == 2 ? (t + ep.tadv) + ep.tadv : t + ep.tadv;\n"
+         "  mol_rk4_fused_tile<GeoF>(U, Ya, Yb, Ac, &mol_fc[0][0], (int)(sizeof(mol_fc[0]) / sizeof(double)), mol_fo,\n"
+         "                           tx, ty, tz, u, unew, ep, jp, t, nsub);\n}\n";
   }
-  o << "// resident steps per launch: " << (resident ? "any" : "1") << "\n";
+  o << "// resident steps per launch: " << (resident ? "any" : (g_.fsteps == 2 ? "2" : "1")) << "\n";
   return o.str();
 }
 

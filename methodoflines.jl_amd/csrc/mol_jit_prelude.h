@@ -767,21 +767,27 @@ struct MolFusedInv {
   }
 };
 
-template <class G, int S>
-__device__ __forceinline__ void mol_rk4_stage(const T* Lin, T* Lout, const T* U, T (&Ur)[MolFusedItems<G>::NJ][G::NF],
+template <class G, int S, int M = 4 - S, int ME = 0>
+__device__ __forceinline__ void mol_rk4_stage(const T* Lin, T* Lout, T* U, T (&Ur)[MolFusedItems<G>::NJ][G::NF],
                                               T (&Ar)[MolFusedItems<G>::NJ][G::NF], const MolFusedInv<G>& Iv,
                                               const int o0, const int o1,
                                               const int o2, const T* __restrict__ u, T* __restrict__ unew,
                                               const Epi& ep, const JPtrs& jp, const double t, const double dt,
-                                              T* Ukeep = nullptr, const bool store = true) {
+                                              T* Ukeep = nullptr, const bool store = true, T* Ac = nullptr) {
+  // M: the stage region is the tile grown by M reach per axis.  ME: the step's result is
+  // wanted on the tile grown by ME (ME > 0: the first of two steps in one launch, whose result
+  // is the next step's tile + halo); the frame positions inside it keep their running sum in
+  // Ac, and stage 4 (M == ME) leaves the new state in U, in place, at every position of its
+  // region and in Ur for the tile nodes.
   // Ukeep (resident steps): stage 4 also leaves the new state in Ukeep and in Ur for the next
   // step; store: stage 4 writes it to unew
-  constexpr int M = 4 - S;
+  static_assert(S < 4 ? M == ME + 4 - S : M == ME, "stage regions shrink by one reach per stage");
   constexpr int TVOL = MolFusedItems<G>::TVOL, NJ = MolFusedItems<G>::NJ;
   constexpr int FR = (G::TX + 2 * M * G::RX) * (G::TY + 2 * M * G::RY) - TVOL;
   static_assert(G::TZ == 1 && G::RZ == 0, "fused RK4 steps are 1D / 2D");
   const double ts = t + (S == 1 ? 0.0 : (S == 4 ? dt : dt / 2));
   const double cf = S == 1 || S == 2 ? dt / 2 : (S == 3 ? dt : dt / 6);
+  if (ME > 0 && S == 4) Ukeep = U;
   // tile nodes: registers carry u and the running sum
   T out[NJ][G::NF];
 #pragma unroll
@@ -812,7 +818,7 @@ __device__ __forceinline__ void mol_rk4_stage(const T* Lin, T* Lout, const T* U,
         Ar[j][f] = a;
       } else {
         v += (T)cf * Ar[j][f];
-        if (store && I0 <= G::UHI0 && I1 <= G::UHI1 && I2 <= G::UHI2) unew[G::uidx(f, I0, I1, I2)] = v;
+        if (ME == 0 && store && I0 <= G::UHI0 && I1 <= G::UHI1 && I2 <= G::UHI2) unew[G::uidx(f, I0, I1, I2)] = v;
         if (Ukeep) {
           Ukeep[f * G::VOL + b] = v;
           Ur[j][f] = v;
@@ -821,7 +827,8 @@ __device__ __forceinline__ void mol_rk4_stage(const T* Lin, T* Lout, const T* U,
       }
     }
   }
-  // frame of the grown region (stages 1-3): u from LDS, no running sum
+  // frame of the grown region: u from LDS, and inside the step's result region the running
+  // sum from Ac
   if (M > 0) {
     for (int q = (int)threadIdx.x; q < TVOL + FR; q += G::NTHR) {
       if (q < TVOL) continue;
@@ -830,24 +837,35 @@ __device__ __forceinline__ void mol_rk4_stage(const T* Lin, T* Lout, const T* U,
       const int b = (lx + G::HX) + G::EX * (ly + G::HY);
       T out[G::NF];
       G::node(Lin, b, o0 + lx, o1 + ly, o2, ts, ep.p, u, jp, out);
+      const bool keep = ME > 0 && lx >= -ME * G::RX && lx < G::TX + ME * G::RX && ly >= -ME * G::RY &&
+                        ly < G::TY + ME * G::RY;
 #pragma unroll
       for (int f = 0; f < G::NF; ++f) {
         T v = (T)cf * out[f];
         v += U[f * G::VOL + b];
-        Lout[f * G::VOL + b] = v;
+        if (S < 4) {
+          Lout[f * G::VOL + b] = v;
+          if (keep) {
+            T a = (S == 1 ? T(1) : T(2)) * out[f];
+            if (S > 1) a += Ac[f * G::VOL + b];
+            Ac[f * G::VOL + b] = a;
+          }
+        } else {
+          v += (T)cf * Ac[f * G::VOL + b];
+          U[f * G::VOL + b] = v;
+        }
       }
     }
   }
 }
 
-// After stage S (1..3) of a fused step on a non-periodic system: the positions of the stage
-// region (tile + (4 - S) reach) outside the unknown box along a non-periodic axis get the value
-// of their boundary equations from the stage values just written, at the next stage's time
-// (what staging does for each stage launch of the unfused path).  Returns whether it wrote.
-template <class G, int S>
+// After a stage of a fused step on a non-periodic system: the positions of the stage region
+// (tile + M reach) outside the unknown box along a non-periodic axis get the value of their
+// boundary equations from the stage values just written, at the next stage's time (what
+// staging does for each stage launch of the unfused path).  Returns whether it wrote.
+template <class G, int M>
 __device__ __forceinline__ bool mol_rk4_fix_ghosts(T* L, const int o0, const int o1, const int o2, const double tn,
                                                    const T* __restrict__ P, const JPtrs& jp) {
-  constexpr int M = 4 - S;
   constexpr int W0 = G::TX + 2 * M * G::RX, W1 = G::TY + 2 * M * G::RY, W2 = G::TZ + 2 * M * G::RZ;
   constexpr int NV = W0 * W1 * W2;
   // the whole work-group takes the same branch: the region reaches outside the box or not
@@ -874,11 +892,46 @@ __device__ __forceinline__ bool mol_rk4_fix_ghosts(T* L, const int o0, const int
   return true;
 }
 
+// One RK4 step of a fused launch from U (tile + halo) at time t: stage regions of margin
+// ME + 3 .. ME; ME = 4: the first of two steps, whose result (tile + 4 reach) stays in U with its
+// ghosts refilled at t + dt; ME = 0: the last step, stored to unew.
+template <class G, int ME>
+__device__ __forceinline__ void mol_rk4_step(T* U, T* Ya, T* Yb, T* Ac, T (&Ur)[MolFusedItems<G>::NJ][G::NF],
+                                             T (&Ar)[MolFusedItems<G>::NJ][G::NF], const MolFusedInv<G>& Iv,
+                                             const int o0, const int o1, const int o2, const T* __restrict__ u,
+                                             T* __restrict__ unew, const Epi& ep, const JPtrs& jp, const double t,
+                                             const double dt) {
+  mol_rk4_stage<G, 1, ME + 3, ME>(U, Ya, U, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt, nullptr, true, Ac);
+  __syncthreads();
+  if constexpr (G::FIX) {
+    if (mol_rk4_fix_ghosts<G, ME + 3>(Ya, o0, o1, o2, t + dt / 2, ep.p, jp)) __syncthreads();
+  }
+  mol_rk4_stage<G, 2, ME + 2, ME>(Ya, Yb, U, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt, nullptr, true, Ac);
+  __syncthreads();
+  if constexpr (G::FIX) {
+    if (mol_rk4_fix_ghosts<G, ME + 2>(Yb, o0, o1, o2, t + dt / 2, ep.p, jp)) __syncthreads();
+  }
+  mol_rk4_stage<G, 3, ME + 1, ME>(Yb, Ya, U, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt, nullptr, true, Ac);
+  __syncthreads();
+  if constexpr (G::FIX) {
+    if (mol_rk4_fix_ghosts<G, ME + 1>(Ya, o0, o1, o2, t + dt, ep.p, jp)) __syncthreads();
+  }
+  mol_rk4_stage<G, 4, ME, ME>(Ya, nullptr, U, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt, nullptr, true, Ac);
+  if constexpr (ME > 0) {
+    __syncthreads();
+    if constexpr (G::FIX) {
+      if (mol_rk4_fix_ghosts<G, ME>(U, o0, o1, o2, t + dt, ep.p, jp)) __syncthreads();
+    }
+  }
+}
+
+// A fused launch of nsteps (1, or 2 when the halo is 8 reach wide) RK4 steps on one tile.  The
+// second step starts at t + dt, the device clock's next value.
 template <class G>
-__device__ __forceinline__ void mol_rk4_fused_tile(T* U, T* Ya, T* Yb, double* fc, const int fcs, int* fo,
+__device__ __forceinline__ void mol_rk4_fused_tile(T* U, T* Ya, T* Yb, T* Ac, double* fc, const int fcs, int* fo,
                                                    const int tx, const int ty, const int tz, const T* __restrict__ u,
                                                    T* __restrict__ unew, const Epi& ep, const JPtrs& jp,
-                                                   const double t) {
+                                                   const double t, const int nsteps) {
   const int o0 = G::ULO0 + tx * G::TX, o1 = G::ULO1 + ty * G::TY, o2 = G::ULO2 + tz * G::TZ;
   const double dt = ep.tadv;
   // coordinates of the tile + halo positions (periodic images wrapped), per axis: position c of
@@ -926,22 +979,14 @@ __device__ __forceinline__ void mol_rk4_fused_tile(T* U, T* Ya, T* Yb, double* f
   }
   MolFusedInv<G> Iv;
   Iv.init(o0, o1, o2, ep.p, jp);
-  mol_rk4_stage<G, 1>(U, Ya, U, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt);
-  __syncthreads();
-  if constexpr (G::FIX) {
-    if (mol_rk4_fix_ghosts<G, 1>(Ya, o0, o1, o2, t + dt / 2, ep.p, jp)) __syncthreads();
+  if constexpr (G::HX >= 8 * G::RX && G::HY >= 8 * G::RY) {
+    if (nsteps == 2) {
+      mol_rk4_step<G, 4>(U, Ya, Yb, Ac, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt);
+      mol_rk4_step<G, 0>(U, Ya, Yb, Ac, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t + dt, dt);
+      return;
+    }
   }
-  mol_rk4_stage<G, 2>(Ya, Yb, U, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt);
-  __syncthreads();
-  if constexpr (G::FIX) {
-    if (mol_rk4_fix_ghosts<G, 2>(Yb, o0, o1, o2, t + dt / 2, ep.p, jp)) __syncthreads();
-  }
-  mol_rk4_stage<G, 3>(Yb, Ya, U, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt);
-  __syncthreads();
-  if constexpr (G::FIX) {
-    if (mol_rk4_fix_ghosts<G, 3>(Ya, o0, o1, o2, t + dt, ep.p, jp)) __syncthreads();
-  }
-  mol_rk4_stage<G, 4>(Ya, nullptr, U, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt);
+  mol_rk4_step<G, 0>(U, Ya, Yb, Ac, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt);
 }
 
 // Resident steps of a 1D system held by a single tile: nsub RK4 steps in one launch with the
@@ -990,7 +1035,7 @@ __device__ __forceinline__ void mol_rk4_fused_resident(T* U, T* Ya, T* Yb, doubl
     mol_rk4_stage<G, 1>(U, Ya, U, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt);
     __syncthreads();
     if constexpr (G::FIX) {
-      if (mol_rk4_fix_ghosts<G, 1>(Ya, o0, o1, o2, t + dt / 2, ep.p, jp)) __syncthreads();
+      if (mol_rk4_fix_ghosts<G, 3>(Ya, o0, o1, o2, t + dt / 2, ep.p, jp)) __syncthreads();
     }
     mol_rk4_stage<G, 2>(Ya, Yb, U, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt);
     __syncthreads();
@@ -1000,7 +1045,7 @@ __device__ __forceinline__ void mol_rk4_fused_resident(T* U, T* Ya, T* Yb, doubl
     mol_rk4_stage<G, 3>(Yb, Ya, U, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt);
     __syncthreads();
     if constexpr (G::FIX) {
-      if (mol_rk4_fix_ghosts<G, 3>(Ya, o0, o1, o2, t + dt, ep.p, jp)) __syncthreads();
+      if (mol_rk4_fix_ghosts<G, 1>(Ya, o0, o1, o2, t + dt, ep.p, jp)) __syncthreads();
     }
     // the stage-4 frame is empty: U is only read at tile positions by stage 4 (through Ur)
     mol_rk4_stage<G, 4>(Ya, nullptr, U, Ur, Ar, Iv, o0, o1, o2, u, unew, ep, jp, t, dt, U, last);
@@ -1017,7 +1062,7 @@ __device__ __forceinline__ void mol_rk4_fused_resident(T* U, T* Ya, T* Yb, doubl
         for (int f = 0; f < G::NF; ++f) U[f * G::VOL + lx + G::HX] = U[f * G::VOL + src + G::HX];
       }
     } else {
-      mol_rk4_fix_ghosts<G, 0>(U, o0, o1, o2, t, ep.p, jp);
+      mol_rk4_fix_ghosts<G, 4>(U, o0, o1, o2, t, ep.p, jp);
     }
     __syncthreads();
   }

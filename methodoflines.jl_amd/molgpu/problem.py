@@ -137,10 +137,11 @@ class ODEProblem:
 
     def plan(self) -> dict:
         h = self.handle
-        info = (C.c_int32 * 9)()
-        h.lib.mol_get_plan(h.h, info, 9)
+        info = (C.c_int32 * 10)()
+        h.lib.mol_get_plan(h.h, info, 10)
         return {"tile": (info[0], info[1], info[2]), "core_tiles": info[3], "generic_tiles": info[4],
-                "weno_tiles": info[5], "generated": info[6], "interp_core_ok": info[7], "fused_tiles": info[8]}
+                "weno_tiles": info[5], "generated": info[6], "interp_core_ok": info[7], "fused_tiles": info[8],
+                "fused_steps_per_launch": info[9]}
 
     def aux_tensor(self, name: str):
         self.handle

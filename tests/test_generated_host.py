@@ -312,6 +312,59 @@ def test_fused_rk4_step_with_time_dependent_boundary_values():
     assert np.max(np.abs(got - ref)) <= 1e-13 * (1 + np.max(np.abs(ref)))
 
 
+def _td_heat1d(n):
+    from molgpu.pdesys import Differential, Eq, Interval, PDESystem, parameters, variables
+    from molgpu.discretization import MOLFiniteDifference
+    import sympy as sp
+    t, x = parameters("t x")
+    u = variables("u")
+    Dx = Differential(x)
+    eq = Eq(Differential(t)(u(t, x)), 0.5 * (Dx ** 2)(u(t, x)) + sp.sin(x) * u(t, x))
+    bcs = [Eq(u(0, x), x * (1 - x)), Eq(u(t, 0), sp.sin(5 * t)), Eq(Dx(u(t, 1)), sp.cos(3 * t))]
+    return lower(PDESystem([eq], bcs, [Interval(t, 0.0, 1.0), Interval(x, 0.0, 1.0)], [t, x], [u(t, x)]),
+                 MOLFiniteDifference({x: n}, t))
+
+
+@pytest.mark.parametrize("case", ["bruss33", "bruss37_16x16", "bruss29_8x8", "diffreact_0000", "diffreact_1111",
+                                  "diffreact_0110", "heat_td"])
+def test_fused_two_steps_per_launch(case, monkeypatch):
+    """Two RK4 steps per fused launch (8-reach halo; the first step's result, with its running
+    sums for the frame positions in LDS and its ghosts refilled at t + dt, is the second step's
+    tile + halo): equal to two steps of RK4 over the right-hand side, clock advanced twice."""
+    monkeypatch.setenv("MOL_FUSED_STEPS", "2")
+    tile = None
+    if case.startswith("bruss"):
+        n = int(case[5:7])
+        tile = {"bruss33": None, "bruss37_16x16": (16, 16, 1), "bruss29_8x8": (8, 8, 1)}[case]
+        low = lower(*systems.brusselator2d(n=n))
+        t0, dt = 1.1 - 1.5e-5, 1e-5   # the forcing switches on inside the second step
+        F = lambda w, tt: orc.bruss2d_rhs(n, w, tt)
+        u = np.random.default_rng(n).uniform(0.5, 1.5, low.n_unknowns)
+    elif case.startswith("diffreact"):
+        bct = tuple(int(c) for c in case[-4:])
+        n = 41
+        low = lower(*systems.diffreact2d(n, 0.05, 2.5, bct))
+        t0, dt, tile = 0.0, 2e-4, (16, 8, 1)
+        F = lambda w, tt: orc.diffreact2d_rhs(n, 0.05, 2.5, bct, w, tt)
+        u = np.random.default_rng(sum(bct)).uniform(0, 1, low.n_unknowns)
+    else:
+        low = _td_heat1d(61)
+        t0, dt, tile = 0.3, 1e-3, (16, 1, 1)
+        F = lambda w, tt: genhost.rhs(low, w, t=tt)
+        u = np.random.default_rng(4).uniform(-1, 1, low.n_unknowns)
+    src = __import__("molgpu").codegen(low, tile=tile, fused=True)
+    assert "// resident steps per launch: 2" in src
+    got, tn = genhost.rk4_fused(low, u, t0, dt, tile=tile, nsub=2)
+    ref = _rk4_numpy(F, _rk4_numpy(F, u, t0, dt), t0 + dt, dt)
+    assert tn == (t0 + dt) + dt
+    assert np.max(np.abs(got - ref)) <= 1e-13 * (1 + np.max(np.abs(ref))), case
+    # the same kernel takes a single step when asked for one
+    one, tn1 = genhost.rk4_fused(low, u, t0, dt, tile=tile, nsub=1)
+    assert tn1 == t0 + dt
+    ref1 = _rk4_numpy(F, u, t0, dt)
+    assert np.max(np.abs(one - ref1)) <= 1e-13 * (1 + np.max(np.abs(ref1))), case
+
+
 @pytest.mark.parametrize("name", ["brusselator_p4", "nonlinlap_periodic"])
 def test_fused_rk4_step_reach_two_periodic(name):
     """Periodic systems of reach 2 (4th-order Laplacian, nonlinear Laplacian) take the fused step

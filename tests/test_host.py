@@ -302,12 +302,29 @@ def test_generated_vjp_kernel_compiles_for_gfx950(name):
 
 
 def test_generated_kernel_folds_exact_weights():
-    """Interior weights appear as exact hexadecimal literals: 1/dx^2 = 64^2 for dx = 1/64."""
+    """Interior weights appear as exact hexadecimal literals, collected per tap as the reference's
+    generated code has them (docs/src/generated/bruss_code.md:46-118: 160.0 per neighbour and
+    -644.4 at the node for alpha = 10, dx = 1/4): alpha/dx^2 = 10 * 64^2 per neighbour and
+    -4 alpha/dx^2 - 4.4 at the node for dx = 1/64."""
     import molgpu
     sys_, disc = systems_bruss = __import__("molgpu").systems.brusselator2d(65)
     src = molgpu.codegen(lower(sys_, disc))
-    assert "T(0x1p+12) * L[b - 1] + T(-0x1p+13) * L[b] + T(0x1p+12) * L[b + 1]" in src
+    nb = "T(%s)" % _c_hex(10.0 * 64 ** 2)
+    for tap in ("L[b - 1]", "L[b + 1]"):
+        assert "%s * %s" % (nb, tap) in src, tap
+    assert "T(%s) * L[b]" % _c_hex(-4 * 10.0 * 64 ** 2 - 4.4) in src
     assert "static constexpr int PER0 = 1;" in src and "static constexpr int N0 = 65;" in src
+
+
+def test_folded_weights_match_reference_fixture():
+    """At the reference's own fixture size (n = 5, dx = 1/4) the collected weights are the
+    numbers of its generated code: 160.0 per neighbour, -644.4 at the node for u and -640.0 for
+    v, 3.4 for u in the v row (docs/src/generated/bruss_code.md:46-118)."""
+    import molgpu
+    sys_, disc = molgpu.systems.brusselator2d(5)
+    src = molgpu.codegen(lower(sys_, disc))
+    for w in (160.0, -644.4, -640.0, 3.4):
+        assert "T(%s) * L[" % _c_hex(w) in src, w
 
 
 def _c_hex(x: float) -> str:
