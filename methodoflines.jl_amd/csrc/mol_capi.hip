@@ -559,6 +559,10 @@ void build_core(mol_handle* h, const mol_desc* d, const KArgs<T>& ka, KCore<T>& 
       }
     }
   }
+  for (int a = 0; a < 3; ++a) {
+    kc.nd_lo[a] = clo[a];
+    kc.nd_hi[a] = chi[a];
+  }
   int ct_lo[3], ct_hi[3];
   for (int a = 0; a < 3; ++a) {
     const int T_ = kc.tile[a];
@@ -1504,6 +1508,8 @@ int plan_batched(const mol_desc* const* ds, const double* const* params, int32_t
       simple_range(d, ka, a, b.slo[a], b.shi[a]);
       b.clo[a] = ncore > 0 ? th.kc64.ct_lo[a] : 1;
       b.chi[a] = ncore > 0 ? th.kc64.ct_lo[a] + th.kc64.ct_n[a] - 1 : 0;
+      b.ilo[a] = th.kc64.nd_lo[a];
+      b.ihi[a] = th.kc64.nd_hi[a];
     }
     for (int f = 0; f < MOL_MAX_FIELDS; ++f) {
       for (int a = 0; a < 3; ++a) {

@@ -29,6 +29,7 @@ struct KCore {
   int32_t lo[3], hi[3];            // shared unknown box of all variables
   int32_t tile[3], halo[3], ext[3], vol;
   int32_t ct_lo[3], ct_n[3];       // core tiles: first tile index and count per axis
+  int32_t nd_lo[3], nd_hi[3];     // nodes whose rows are all interior and whose halo lies in the box
   uint32_t magic0, magic1;         // floor(q / ext0) = umulhi(q, magic0) (0: divisor 1), same for ext0*ext1
   int64_t box;                     // unknowns per variable
   int32_t rhs_prog[MOL_MAX_FIELDS];

@@ -48,6 +48,7 @@ struct BInst {
   int64_t fofs[MOL_MAX_FIELDS];
   int32_t ulo[3], uhi[3], nt[3];
   int32_t slo[3], shi[3], clo[3], chi[3];
+  int32_t ilo[3], ihi[3];   // nodes where the interior-rows code applies
   int32_t cls, first;
   const double* coord[3];
   const double* W;

@@ -2047,6 +2047,15 @@ std::string Gen::source_class(const std::string& name) {
   c("EX", pg_.ext[0]); c("EY", pg_.ext[1]); c("EZ", pg_.ext[2]);
   c("VOL", pg_.vol);
   c("NTHR", 256);
+  // the variable staged in each LDS slot
+  {
+    std::vector<int> sf(std::max(nst, 1), 0);
+    for (int f = 0; f < d_->nfields; ++f)
+      if (pg_.slot[f] >= 0) sf[pg_.slot[f]] = f;
+    o << "  static constexpr int SF[" << sf.size() << "] = {";
+    for (size_t k = 0; k < sf.size(); ++k) o << (k ? ", " : "") << sf[k];
+    o << "};\n";
+  }
   o << stage;
   // values of the staged variables at one tile + halo position, v[slot]: plain loads (after the
   // periodic wrap) in simple tiles, else through the ghost closures

@@ -1302,16 +1302,19 @@ struct BInst {
   i64 fofs[8];
   int ulo[3], uhi[3], nt[3];
   int slo[3], shi[3], clo[3], chi[3];
+  int ilo[3], ihi[3];   // nodes where the interior-rows code applies
   int cls, first;
   const double* coord[3];
   const double* W;
   const double* K;
 };
 
+// Index of node (p0, p1, p2) of variable f in the batch state: the instance's offset plus a
+// 32-bit offset inside the variable's box (each box holds fewer than 2^31 unknowns).
 __device__ __forceinline__ i64 mol_b_uidx(const BInst& hd, const int f, const int p0, const int p1, const int p2) {
-  return hd.fofs[f] + (i64)(p0 - hd.lo[f][0]) +
-         (i64)(hd.hi[f][0] - hd.lo[f][0] + 1) *
-             ((i64)(p1 - hd.lo[f][1]) + (i64)(hd.hi[f][1] - hd.lo[f][1] + 1) * (i64)(p2 - hd.lo[f][2]));
+  return hd.fofs[f] + (i64)((p0 - hd.lo[f][0]) +
+                            (hd.hi[f][0] - hd.lo[f][0] + 1) *
+                                ((p1 - hd.lo[f][1]) + (hd.hi[f][1] - hd.lo[f][1] + 1) * (p2 - hd.lo[f][2])));
 }
 
 __device__ __forceinline__ bool mol_b_inbox(const BInst& hd, const int f, const int I0, const int I1, const int I2) {
@@ -1319,78 +1322,254 @@ __device__ __forceinline__ bool mol_b_inbox(const BInst& hd, const int f, const 
          I2 <= hd.hi[f][2];
 }
 
-// One tile of one instance (L: the kernel's LDS tile, sized for the largest class).
+// Position walker over a box of run-time extents (x fastest), advanced by the work-group's
+// thread count: one division per axis when a tile starts, none per position.
+// Positions of a box of run-time extents (x fastest), visited by the threads of a work-group
+// with stride NTHR: one division per axis when a walk starts, none per position.  Z: the box
+// has more than one plane.
+template <bool Z>
+struct MolRtWalk {
+  int x, y, z, dx, dy, dz, ex, ey;
+  __device__ __forceinline__ MolRtWalk(const int q, const int ex_, const int ey_, const int stride) : ex(ex_), ey(ey_) {
+    if constexpr (Z) {
+      const int exy = ex * ey;
+      z = q / exy;
+      const int r = q - z * exy;
+      y = r / ex;
+      x = r - y * ex;
+      dz = stride / exy;
+      const int rs = stride - dz * exy;
+      dy = rs / ex;
+      dx = rs - dy * ex;
+    } else {
+      z = dz = 0;
+      y = q / ex;
+      x = q - y * ex;
+      dy = stride / ex;
+      dx = stride - dy * ex;
+    }
+  }
+  __device__ __forceinline__ void step() {
+    x += dx;
+    y += dy;
+    if (x >= ex) { x -= ex; ++y; }
+    if constexpr (Z) {
+      z += dz;
+      if (y >= ey) { y -= ey; ++z; }
+    }
+  }
+};
+
+// Inclusive integer box.
+struct MolBox {
+  int lo[3], hi[3];
+  __device__ __forceinline__ bool empty() const { return hi[0] < lo[0] || hi[1] < lo[1] || hi[2] < lo[2]; }
+  __device__ __forceinline__ int ext(const int a) const { return hi[a] - lo[a] + 1; }
+  __device__ __forceinline__ int vol() const { return ext(0) * ext(1) * ext(2); }
+  __device__ __forceinline__ MolBox meet(const MolBox& o) const {
+    MolBox m;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      m.lo[a] = max(lo[a], o.lo[a]);
+      m.hi[a] = min(hi[a], o.hi[a]);
+    }
+    return m;
+  }
+};
+
+// The positions of box B outside its sub-box A (A non-empty), as up to six slabs: below and
+// above A along z (whole planes of B), then along y (within A's planes), then along x (within
+// A's rows).  count(): their number; at(k): the k-th position.
+struct MolFrame {
+  MolBox B, A;
+  int n[6];
+  __device__ __forceinline__ MolFrame(const MolBox& b, const MolBox& a) : B(b), A(a) {
+    const int ex = B.ext(0), ey = B.ext(1), ay = A.ext(1), az = A.ext(2);
+    n[0] = (A.lo[2] - B.lo[2]) * ex * ey;
+    n[1] = (B.hi[2] - A.hi[2]) * ex * ey;
+    n[2] = (A.lo[1] - B.lo[1]) * ex * az;
+    n[3] = (B.hi[1] - A.hi[1]) * ex * az;
+    n[4] = (A.lo[0] - B.lo[0]) * ay * az;
+    n[5] = (B.hi[0] - A.hi[0]) * ay * az;
+  }
+  __device__ __forceinline__ int count() const { return n[0] + n[1] + n[2] + n[3] + n[4] + n[5]; }
+  __device__ __forceinline__ void at(int k, int& p0, int& p1, int& p2) const {
+    const int ex = B.ext(0), ey = B.ext(1);
+    int s = 0;
+    while (s < 5 && k >= n[s]) k -= n[s++];
+    if (s < 2) {   // planes of B
+      const int exy = ex * ey, pz = k / exy, r = k - pz * exy, py = r / ex;
+      p2 = (s == 0 ? B.lo[2] : A.hi[2] + 1) + pz;
+      p1 = B.lo[1] + py;
+      p0 = B.lo[0] + (r - py * ex);
+    } else if (s < 4) {   // rows of B within A's planes
+      const int w = (s == 2 ? A.lo[1] - B.lo[1] : B.hi[1] - A.hi[1]);
+      const int pz = k / (ex * w), r = k - pz * ex * w, py = r / ex;
+      p2 = A.lo[2] + pz;
+      p1 = (s == 2 ? B.lo[1] : A.hi[1] + 1) + py;
+      p0 = B.lo[0] + (r - py * ex);
+    } else {   // columns within A's rows
+      const int w = (s == 4 ? A.lo[0] - B.lo[0] : B.hi[0] - A.hi[0]), ay = A.ext(1);
+      const int pz = k / (w * ay), r = k - pz * w * ay, py = r / w;
+      p2 = A.lo[2] + pz;
+      p1 = A.lo[1] + py;
+      p0 = (s == 4 ? B.lo[0] : A.hi[0] + 1) + (r - py * w);
+    }
+  }
+};
+
+// Position inside every variable's unknown box.
+template <int NF>
+__device__ __forceinline__ bool mol_b_inall(const BInst& hd, const int p0, const int p1, const int p2) {
+  bool in = true;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) in = in && mol_b_inbox(hd, f, p0, p1, p2);
+  return in;
+}
+
+// One node of a batched tile whose LDS box starts at (s0, s1, s2), and its stage epilogue.
+// INTERIOR: the interior-rows code, at a node inside every variable's box.
+template <class G, bool INTERIOR>
+__device__ __forceinline__ void mol_batch_node(const T* L, const BInst& hd, const int I0, const int I1, const int I2,
+                                               const int s0, const int s1, const int s2, const T* __restrict__ u,
+                                               const Epi& ep, const JPtrs& jp, const double t) {
+  const int base = (I0 - s0) + G::EX * ((I1 - s1) + G::EY * (I2 - s2));
+  T out[G::NF];
+  G::template node<INTERIOR>(hd, L, base, I0, I1, I2, t, nullptr, u, jp, out);
+#pragma unroll
+  for (int f = 0; f < G::NF; ++f)
+    if (INTERIOR || mol_b_inbox(hd, f, I0, I1, I2)) mol_epilogue(ep, mol_b_uidx(hd, f, I0, I1, I2), out[f]);
+}
+
+// One tile of one instance (L: the kernel's LDS tile, sized for the largest class).  Only the
+// part of the tile inside the instance's union box is staged and evaluated, so tiles at least
+// as wide as an instance cover whole rows of it (contiguous in memory: staging loads and the
+// epilogue touch no cache line of another row) and narrow instances leave no idle lanes in a
+// wide tile.  Staging: plain loads where the tile + halo lies inside every variable's box,
+// the ghost closures (and periodic images) on the frame around it.  Nodes: the interior-rows
+// code inside the instance's interior node range, the boundary-rows code on the frame around it.
 template <class G>
 __device__ __forceinline__ void mol_batch_tile(T* L, const BInst& hd, const int tile, const T* __restrict__ u,
                                                const Epi& ep, const double t) {
+  constexpr bool Z = G::TZ > 1 || G::HZ > 0;
   const int tx = tile % hd.nt[0];
   const int r1 = tile / hd.nt[0];
   const int ty = r1 % hd.nt[1];
   const int tz = r1 / hd.nt[1];
-  const int o0 = hd.ulo[0] + tx * G::TX, o1 = hd.ulo[1] + ty * G::TY, o2 = hd.ulo[2] + tz * G::TZ;
-  const bool simple = tx >= hd.slo[0] && tx <= hd.shi[0] && ty >= hd.slo[1] && ty <= hd.shi[1] &&
-                      tz >= hd.slo[2] && tz <= hd.shi[2];
-  const bool core = tx >= hd.clo[0] && tx <= hd.chi[0] && ty >= hd.clo[1] && ty <= hd.chi[1] &&
-                    tz >= hd.clo[2] && tz <= hd.chi[2];
+  MolBox N;   // the tile's nodes inside the union box
+  N.lo[0] = hd.ulo[0] + tx * G::TX;
+  N.lo[1] = hd.ulo[1] + ty * G::TY;
+  N.lo[2] = hd.ulo[2] + tz * G::TZ;
+  N.hi[0] = min(N.lo[0] + G::TX - 1, hd.uhi[0]);
+  N.hi[1] = min(N.lo[1] + G::TY - 1, hd.uhi[1]);
+  N.hi[2] = min(N.lo[2] + G::TZ - 1, hd.uhi[2]);
+  MolBox S = N;   // with the halo: the LDS box
+  S.lo[0] -= G::HX; S.hi[0] += G::HX;
+  S.lo[1] -= G::HY; S.hi[1] += G::HY;
+  S.lo[2] -= G::HZ; S.hi[2] += G::HZ;
   JPtrs jp;
   jp.coord[0] = hd.coord[0];
   jp.coord[1] = hd.coord[1];
   jp.coord[2] = hd.coord[2];
-  const T* P = nullptr;
-  constexpr int EXY = G::EX * G::EY;
   if constexpr (G::NST > 0) {
-    // every load of the thread is issued before its first LDS store (one exposed latency per
-    // tile), unless the positions per thread are many
+    MolBox V = S;   // inside every variable's box: plain loads
+#pragma unroll
+    for (int f = 0; f < G::NF; ++f) {
+      MolBox bf;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        bf.lo[a] = hd.lo[f][a];
+        bf.hi[a] = hd.hi[f][a];
+      }
+      V = V.meet(bf);
+    }
     constexpr int IT = (G::VOL + G::NTHR - 1) / G::NTHR;
     constexpr int NS = G::NST > 0 ? G::NST : 1;
-    if constexpr (IT * G::NST <= 16) {
-      T v[IT][NS];
+    if (!V.empty()) {
+      // variable boxes and the first plain position, per staged slot
+      i64 vb[NS];
+      int sx[NS], sxy[NS];
 #pragma unroll
-      for (int k = 0; k < IT; ++k) {
-        const int q = (int)threadIdx.x + k * G::NTHR;
-        if (G::VOL % G::NTHR != 0 && q >= G::VOL) break;
-        const int lz = q / EXY;
-        const int r = q - lz * EXY;
-        const int ly = r / G::EX;
-        const int lx = r - ly * G::EX;
-        G::stage_vals(hd, v[k], o0 + lx - G::HX, o1 + ly - G::HY, o2 + lz - G::HZ, simple, t, P, u, jp);
+      for (int s = 0; s < G::NST; ++s) {
+        const int f = G::SF[s];
+        sx[s] = hd.hi[f][0] - hd.lo[f][0] + 1;
+        sxy[s] = sx[s] * (hd.hi[f][1] - hd.lo[f][1] + 1);
+        vb[s] = mol_b_uidx(hd, f, V.lo[0], V.lo[1], V.lo[2]);
       }
+      const int nv = V.vol();
+      const int lofs = (V.lo[0] - S.lo[0]) + G::EX * ((V.lo[1] - S.lo[1]) + G::EY * (V.lo[2] - S.lo[2]));
+      MolRtWalk<Z> w((int)threadIdx.x, V.ext(0), V.ext(1), G::NTHR);
+      if constexpr (IT * G::NST <= 16) {
+        // every load of the thread is issued before its first LDS store
+        T v[IT][NS];
+        int dst[IT];
 #pragma unroll
-      for (int k = 0; k < IT; ++k) {
-        const int q = (int)threadIdx.x + k * G::NTHR;
-        if (G::VOL % G::NTHR != 0 && q >= G::VOL) break;
+        for (int k = 0; k < IT; ++k, w.step()) {
+          dst[k] = (int)threadIdx.x + k * G::NTHR < nv ? lofs + w.x + G::EX * (w.y + G::EY * w.z) : -1;
+          if (dst[k] >= 0)
 #pragma unroll
-        for (int s = 0; s < G::NST; ++s) L[s * G::VOL + q] = v[k][s];
+            for (int s = 0; s < G::NST; ++s) v[k][s] = u[vb[s] + (w.x + sx[s] * w.y + sxy[s] * w.z)];
+        }
+#pragma unroll
+        for (int k = 0; k < IT; ++k)
+          if (dst[k] >= 0)
+#pragma unroll
+            for (int s = 0; s < G::NST; ++s) L[s * G::VOL + dst[k]] = v[k][s];
+      } else {
+        for (int q = threadIdx.x; q < nv; q += G::NTHR, w.step()) {
+          const int d = lofs + w.x + G::EX * (w.y + G::EY * w.z);
+#pragma unroll
+          for (int s = 0; s < G::NST; ++s) L[s * G::VOL + d] = u[vb[s] + (w.x + sx[s] * w.y + sxy[s] * w.z)];
+        }
       }
-    } else {
-      for (int q = threadIdx.x; q < G::VOL; q += G::NTHR) {
-        const int lz = q / EXY;
-        const int r = q - lz * EXY;
-        const int ly = r / G::EX;
-        const int lx = r - ly * G::EX;
-        T v[NS];
-        G::stage_vals(hd, v, o0 + lx - G::HX, o1 + ly - G::HY, o2 + lz - G::HZ, simple, t, P, u, jp);
-#pragma unroll
-        for (int s = 0; s < G::NST; ++s) L[s * G::VOL + q] = v[s];
+    }
+    // the frame of ghost positions and periodic images (everything when nothing is plain)
+    const MolFrame F(S, V.empty() ? S : V);
+    const int nf = V.empty() ? S.vol() : F.count();
+    for (int q = threadIdx.x; q < nf; q += G::NTHR) {
+      int p0, p1, p2;
+      if (V.empty()) {
+        const int e0 = S.ext(0), e1 = S.ext(1), pz = q / (e0 * e1), r = q - pz * e0 * e1, py = r / e0;
+        p0 = S.lo[0] + (r - py * e0);
+        p1 = S.lo[1] + py;
+        p2 = S.lo[2] + pz;
+      } else {
+        F.at(q, p0, p1, p2);
       }
+      T v[NS];
+      G::stage_vals(hd, v, p0, p1, p2, false, t, nullptr, u, jp);
+      const int d = (p0 - S.lo[0]) + G::EX * ((p1 - S.lo[1]) + G::EY * (p2 - S.lo[2]));
+#pragma unroll
+      for (int s = 0; s < G::NST; ++s) L[s * G::VOL + d] = v[s];
     }
     __syncthreads();
   }
-  constexpr int TXY = G::TX * G::TY, TVOL = TXY * G::TZ;
-  for (int q = threadIdx.x; q < TVOL; q += G::NTHR) {
-    const int lz = q / TXY;
-    const int rr = q - lz * TXY;
-    const int ly = rr / G::TX;
-    const int lx = rr - ly * G::TX;
-    const int I0 = o0 + lx, I1 = o1 + ly, I2 = o2 + lz;
-    if (I0 > hd.uhi[0] || I1 > hd.uhi[1] || I2 > hd.uhi[2]) continue;
-    const int base = (lx + G::HX) + G::EX * (ly + G::HY) + EXY * (lz + G::HZ);
-    T out[G::NF];
-    if (core) G::template node<true>(hd, L, base, I0, I1, I2, t, P, u, jp, out);
-    else G::template node<false>(hd, L, base, I0, I1, I2, t, P, u, jp, out);
+  MolBox I;   // nodes where the interior-rows code applies
 #pragma unroll
-    for (int f = 0; f < G::NF; ++f)
-      if (mol_b_inbox(hd, f, I0, I1, I2)) mol_epilogue(ep, mol_b_uidx(hd, f, I0, I1, I2), out[f]);
+  for (int a = 0; a < 3; ++a) {
+    I.lo[a] = hd.ilo[a];
+    I.hi[a] = hd.ihi[a];
+  }
+  I = I.meet(N);
+  if (!I.empty()) {
+    const int ni = I.vol();
+    MolRtWalk<Z> w((int)threadIdx.x, I.ext(0), I.ext(1), G::NTHR);
+    for (int q = threadIdx.x; q < ni; q += G::NTHR, w.step())
+      mol_batch_node<G, true>(L, hd, I.lo[0] + w.x, I.lo[1] + w.y, I.lo[2] + w.z, S.lo[0], S.lo[1], S.lo[2], u, ep,
+                              jp, t);
+    const MolFrame F(N, I);
+    const int nf = F.count();
+    for (int q = threadIdx.x; q < nf; q += G::NTHR) {
+      int p0, p1, p2;
+      F.at(q, p0, p1, p2);
+      mol_batch_node<G, false>(L, hd, p0, p1, p2, S.lo[0], S.lo[1], S.lo[2], u, ep, jp, t);
+    }
+  } else {
+    const int nn = N.vol();
+    MolRtWalk<Z> w((int)threadIdx.x, N.ext(0), N.ext(1), G::NTHR);
+    for (int q = threadIdx.x; q < nn; q += G::NTHR, w.step())
+      mol_batch_node<G, false>(L, hd, N.lo[0] + w.x, N.lo[1] + w.y, N.lo[2] + w.z, S.lo[0], S.lo[1], S.lo[2], u,
+                               ep, jp, t);
   }
   __syncthreads();
 }

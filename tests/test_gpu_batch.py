@@ -30,10 +30,13 @@ def _instances(count, seed, dtype="f64", nmin=10, nmax=41):
     return out
 
 
-def test_batched_rhs_matches_oracle_per_instance():
+@pytest.mark.parametrize("tile", [(16, 8, 1), (7, 5, 1), (64, 4, 1), (256, 8, 1)])
+def test_batched_rhs_matches_oracle_per_instance(tile):
+    """Tiles narrower than the instances (ragged last tiles), and tiles wider than every instance
+    (each tile is a band of whole rows, trimmed to the instance's box)."""
     torch = _torch()
     inst = _instances(14, 1)
-    batch = BatchProblem([p for _, p in inst], tile=(16, 8, 1))
+    batch = BatchProblem([p for _, p in inst], tile=tile)
     plan = batch.plan()
     assert plan["classes"] == len({meta[3] for meta, _ in inst})
     rng = np.random.default_rng(2)
