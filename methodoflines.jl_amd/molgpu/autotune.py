@@ -19,7 +19,7 @@ CANDIDATES = {
     1: [(256, 1, 1), (128, 1, 1), (512, 1, 1)],
     # 2D: tiles of whole rows of the instances (256 wide; a tile is trimmed to its instance)
     # first, then square-ish tiles
-    2: [(256, 8, 1), (256, 16, 1), (128, 16, 1), (64, 16, 1), (64, 8, 1), (32, 16, 1)],
+    2: [(256, 16, 1), (256, 12, 1), (256, 8, 1), (128, 16, 1), (64, 16, 1), (32, 16, 1)],
     3: [(32, 4, 2), (16, 8, 2), (16, 4, 4), (32, 8, 1)],
 }
 
