@@ -5,12 +5,14 @@
 Headline (`value`): stencil RHS Gpoint/s of the 2D Brusselator (512^2 unknowns per variable,
 FP64, periodic, 2nd-order Laplacian) driven by the device-resident RK4 loop at dt = 1e-5.
 One step = one classic RK4 step = 4 right-hand sides over all 512^2 points (a point is one
-unknown grid node with both variables); value = points * 4 * K / elapsed / 1e9.  The step
-runs as one fused launch (the four stages on a tile with a 4-node halo in LDS) where the
-system allows it; `roofline.stage_kernels` reports the four-launches-per-step path too.
+unknown grid node with both variables); value = points * 4 * K / elapsed / 1e9.  The steps run
+as fused launches of two steps each (the eight stages on a tile with an 8-node halo in LDS)
+where the system allows it; `roofline.stage_kernels` reports the four-launches-per-step path
+too.
 
 The same JSON line carries the other parts of the metric:
-  roofline     the dominant kernel of the timed region (the fused step kernel) against HBM peak;
+  roofline     the dominant kernel of the timed region (the fused step kernel) against HBM peak,
+               priced at the algorithmic bytes of the steps a launch takes;
   weno5        one FP64 WENO5 Burgers RHS at 1024^2 (and 4096^2, beyond the 256 MB MALL), and the
                1024^2 SSPRK3 solve (CFL 0.4, 500 steps) in FP64, FP32 + Kahan and plain FP32 with
                the FP32 errors against FP64 and the FP64 error against the oracle;
@@ -43,14 +45,14 @@ HBM_PEAK_GBS = 8000.0
 # profiles/brusselator512_rk4_stage_kernel.md.  Below the 17.8 MB of algorithmic bytes: halo
 # re-reads and the stage vectors written by one launch and read by the next hit in L2.
 STAGE_TRAFFIC_BYTES = (3960.27 * 2.0 + 5869.53 * 1.0) * 1024
-# The same measurement for the fused RK4 step kernel (one launch per step, 32x32 tiles):
-# FETCH_SIZE 2375.6 KiB, WRITE_SIZE 4096.0 KiB per launch (8.39 MB algorithmic; the excess is
-# the tile halos).
-FUSED_TRAFFIC_BYTES = (2375.6 * 2.0 + 4096.0 * 1.0) * 1024
-# Issue floor of the fused step kernel from its SQ counters (profiles/brusselator512_rk4_fused_step.md):
-# 345.7 VALU instructions per wave over 4096 waves; a wave64 VALU instruction occupies a SIMD for
+# The same measurement for the fused RK4 kernel (two steps per launch, 32x32 tiles, 8-node
+# halo): FETCH_SIZE 2689.9 KiB, WRITE_SIZE 4096.0 KiB per launch (16.8 MB algorithmic for the
+# two steps; a launch reads u once with its halo and writes it once).
+FUSED_TRAFFIC_BYTES = (2689.9 * 2.0 + 4096.0 * 1.0) * 1024
+# Issue floor of one fused launch from its SQ counters (profiles/brusselator512_rk4_fused_step.md):
+# 696.4 VALU instructions per wave over 4096 waves; a wave64 VALU instruction occupies a SIMD for
 # 4 cycles, and the chip has 1024 SIMDs at 2.4 GHz.
-FUSED_VALU_PER_WAVE, FUSED_WAVES = 345.7, 4096
+FUSED_VALU_PER_WAVE, FUSED_WAVES = 696.4, 4096
 VALU_FLOOR_US = FUSED_VALU_PER_WAVE * FUSED_WAVES * 4 / (1024 * 2.4e9) * 1e6
 
 
@@ -631,7 +633,7 @@ def main():
         launch_us = ms * 1e3 * spl / args.steps
         bytes_launch = points * 32 * spl
         kname = "mol_rk4_fused (generated; %d whole RK4 step%s per launch)" % (spl, "s" if spl > 1 else "")
-        traffic = FUSED_TRAFFIC_BYTES
+        traffic = FUSED_TRAFFIC_BYTES if spl == 2 else None
     else:
         launch_us, bytes_launch, kname, traffic = st_us, st_bytes, stage["kernel"], STAGE_TRAFFIC_BYTES
     achieved = bytes_launch / (launch_us * 1e-6) / 1e9
@@ -660,7 +662,7 @@ def main():
                      # floors of the launch: HBM (algorithmic bytes at peak) and VALU issue (the
                      # binding one of the two; the rest is the serial load / compute / store phases)
                      "hbm_floor_us": bytes_launch / (HBM_PEAK_GBS * 1e3),
-                     "valu_floor_us": VALU_FLOOR_US if fused else None,
+                     "valu_floor_us": VALU_FLOOR_US if fused and spl == 2 else None,
                      "stage_kernels": stage,
                      "rhs_only": {"avg_us": rhs_us, "bytes_per_launch": points * 32, "achieved": rhs_ach,
                                   "frac": rhs_ach / HBM_PEAK_GBS}},
