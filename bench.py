@@ -503,7 +503,13 @@ def bench_batch(count, nsteps, quick):
     scale = float(u64.abs().max())
     err_k = float((u.double() - u64).abs().max())
     err_p = float((u32.double() - u64).abs().max())
+    # algorithmic bytes of an FP32 + Kahan RK4 step per unknown: stage 1 reads u and writes y1 and
+    # acc (12 B); stages 2 and 3 read y, u, acc and write y', acc (20 B each); stage 4 reads y3, u,
+    # acc and the compensation, writes the compensation and u (24 B): 76 B
+    step_bytes = pts * 76
+    ach = step_bytes / (ms / nsteps * 1e-3) / 1e9
     return {"instances": count, "unknowns": pts, "classes": b.plan()["classes"], "tile": list(tile),
+            "bytes_per_step": step_bytes, "achieved_gbs": ach, "hbm_frac": ach / HBM_PEAK_GBS,
             "tuning": table, "steps": nsteps, "dt": dt, "ms": ms, "ms_per_step": ms / nsteps,
             "gpoint_rhs_s": pts * 4 * nsteps / (ms / 1e3) / 1e9, "lowering_s": lower_s,
             "finite": bool(torch.isfinite(u).all()),

@@ -225,8 +225,8 @@ int mol_expand(mol_handle* h, void* full, const void* u, const void* p, double t
  * discretize/solve over instances (e.g. an EnsembleProblem of MOL problems).  The handle's state
  * vector is the concatenation of the instances' state vectors (instance-major, each in its own
  * unknown order, first offsets from mol_batch_offsets); mol_rhs, mol_rhs_h and mol_integrate
- * then act on all instances with one persistent launch per stage.  Instances must share dtype,
- * dimension and number of variables and take no grid fields; params[i] holds the nparams values
+ * then act on all instances with one launch per stage (a work-group per tile of an instance,
+ * largest tiles first).  Instances must share dtype, dimension and number of variables and take no grid fields; params[i] holds the nparams values
  * of instance i (host, FP64; fixed for the handle's life, the p argument of mol_rhs/mol_integrate
  * is ignored).  Instances of one structure (same operators and boundary kinds) share generated
  * code; numbers (grid sizes, weights, coefficients) are read per instance.  tile = tx,ty,tz for
@@ -234,7 +234,7 @@ int mol_expand(mol_handle* h, void* full, const void* u, const void* p, double t
 int mol_create_batched(const mol_desc* const* ds, const double* const* params, int32_t ninst, const int32_t* tile,
                        mol_handle** out);
 
-/* Generated source of the persistent kernel of a batch (no GPU needed); flags as mol_codegen. */
+/* Generated source of the kernel of a batch (no GPU needed); flags as mol_codegen. */
 int64_t mol_codegen_batched(const mol_desc* const* ds, const double* const* params, int32_t ninst, const int32_t* tile,
                             char* buf, int64_t buflen, int32_t flags);
 

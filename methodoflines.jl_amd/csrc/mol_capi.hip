@@ -1519,10 +1519,22 @@ int plan_batched(const mol_desc* const* ds, const double* const* params, int32_t
       b.fofs[f] = P.nunk + ka.fofs[f];
     }
     b.cls = cls;
-    b.first = (int32_t)P.wl.size();
+    b.first = (int32_t)(P.wl.size() / 2);
     const int64_t nt = (int64_t)ka.ntile[0] * ka.ntile[1] * ka.ntile[2];
-    if ((int64_t)P.wl.size() + nt > (int64_t)INT32_MAX / 2) return fail(nullptr, MOL_ERR_SHAPE, "batch too large");
-    for (int64_t k = 0; k < nt; ++k) P.wl.push_back(i);
+    if ((int64_t)P.wl.size() / 2 + nt > (int64_t)INT32_MAX / 4) return fail(nullptr, MOL_ERR_SHAPE, "batch too large");
+    {
+      int64_t box = 1;
+      for (int f = 0; f < d->nfields; ++f) {
+        int64_t v = 1;
+        for (int a = 0; a < 3; ++a) v *= ka.hi[f][a] - ka.lo[f][a] + 1;
+        box = std::max(box, v);
+      }
+      if (box >= (int64_t)INT32_MAX) return fail(nullptr, MOL_ERR_SHAPE, "a batched instance holds 2^31 or more unknowns of one variable");
+    }
+    for (int64_t k = 0; k < nt; ++k) {
+      P.wl.push_back(i);
+      P.wl.push_back((int32_t)k);
+    }
     P.inst_off.push_back(P.nunk);
     P.nunk += th.nunk;
     std::vector<double>& data = P.data;
@@ -1542,9 +1554,34 @@ int plan_batched(const mol_desc* const* ds, const double* const* params, int32_t
     }
   }
   P.nclasses = (int)cls_src.size();
-  // one kernel: the class bodies and the persistent loop over the work list
+  // Work items largest first (nodes of the tile inside its instance's box): the kernel runs one
+  // work-group per item, so the dispatcher hands the small items of the tail to the compute
+  // units that finish first.
+  {
+    const size_t nw = P.wl.size() / 2;
+    std::vector<std::pair<int64_t, size_t>> order(nw);
+    for (size_t k = 0; k < nw; ++k) {
+      const BInst& b = P.hdr[P.wl[2 * k]];
+      int64_t t = P.wl[2 * k + 1], nodes = 1;
+      for (int a = 0; a < 3; ++a) {
+        const int ta = (int)(t % b.nt[a]);
+        t /= b.nt[a];
+        const int lo = b.ulo[a] + ta * P.tile[a];
+        nodes *= std::min(P.tile[a], b.uhi[a] - lo + 1);
+      }
+      order[k] = {-nodes, k};
+    }
+    std::stable_sort(order.begin(), order.end());
+    std::vector<int32_t> wl(P.wl.size());
+    for (size_t k = 0; k < nw; ++k) {
+      wl[2 * k] = P.wl[2 * order[k].second];
+      wl[2 * k + 1] = P.wl[2 * order[k].second + 1];
+    }
+    P.wl.swap(wl);
+  }
+  // one kernel: the class bodies and the loop over the work list
   std::ostringstream src;
-  src << "// generated persistent kernel of a batch of " << ninst << " instances, " << cls_src.size() << " classes\n";
+  src << "// generated kernel of a batch of " << ninst << " instances, " << cls_src.size() << " classes\n";
   src << "#define MOL_T " << (ds[0]->dtype == MOL_F64 ? "double" : "float") << "\n#include \"mol_jit_prelude.h\"\n\n";
   src << "static_assert(sizeof(BInst) == " << sizeof(BInst) << " && sizeof(Epi) == "
       << (ds[0]->dtype == MOL_F64 ? sizeof(Epi<double>) : sizeof(Epi<float>))
@@ -1562,8 +1599,8 @@ int plan_batched(const mol_desc* const* ds, const double* const* params, int32_t
          "  const double t = (ep.tnow ? ep.tnow[0] : 0.0) + ep.tofs;\n"
          "  mol_advance_clock(ep);\n"
          "  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {\n"
-         "    const BInst& hd = insts[wl[w]];\n"
-         "    const int tile = w - hd.first;\n"
+         "    const BInst& hd = insts[wl[2 * w]];\n"
+         "    const int tile = wl[2 * w + 1];\n"
          "    switch (hd.cls) {\n";
   for (size_t c = 0; c < cls_src.size(); ++c)
     src << "    case " << c << ": mol_batch_tile<Geo" << c << ">(L, hd, tile, u, ep, t); break;\n";
@@ -1590,7 +1627,7 @@ int create_batched(const mol_desc* const* ds, const double* const* params, int32
   for (int a = 0; a < 3; ++a) h->ka64.tile[a] = P.tile[a];
   h->nunk = P.nunk;
   h->inst_off = P.inst_off;
-  h->nwork = (int)P.wl.size();
+  h->nwork = (int)(P.wl.size() / 2);
   h->nclasses = P.nclasses;
   int r = upload(h, &h->d_bdata, P.data.data(), P.data.size());
   const double* base = (const double*)h->d_bdata;
@@ -1612,7 +1649,7 @@ int create_batched(const mol_desc* const* ds, const double* const* params, int32
     free_handle(h);
     return MOL_ERR_HIP;
   }
-  h->batch_grid = std::min(h->nwork, 2048);
+  h->batch_grid = std::min(h->nwork, 1 << 24);
   *out = h;
   return 0;
 }

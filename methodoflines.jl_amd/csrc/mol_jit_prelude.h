@@ -72,6 +72,60 @@ __device__ __forceinline__ void mol_epilogue(const Epi& ep, const I idx, T f) {
   ep.out[idx] = s;
 }
 
+// The epilogue of CH nodes at once (ok[k]: node k exists; out[k][f] at index idx[k] + dv[f]):
+// every operand load of the CH nodes is issued before the first store, so one memory latency
+// is exposed per chunk instead of one per node.
+template <int CH, int NF>
+__device__ __forceinline__ void mol_epilogue_chunk(const Epi& ep, const i64 (&idx)[CH], const bool (&ok)[CH],
+                                                   const i64 (&dv)[NF], const T (&out)[CH][NF]) {
+  if (ep.mode == 0) {
+#pragma unroll
+    for (int k = 0; k < CH; ++k)
+      if (ok[k])
+#pragma unroll
+        for (int f = 0; f < NF; ++f) ep.out[idx[k] + dv[f]] = out[k][f];
+    return;
+  }
+  T a[CH][NF], b[CH][NF], c[CH][NF];
+  const bool keep = ep.mode == 1 && ep.acc && ep.acc_keep != 0.0;
+#pragma unroll
+  for (int k = 0; k < CH; ++k)
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const i64 i = idx[k] + dv[f];
+      a[k][f] = ok[k] && ep.A ? ep.A[i] : T(0);
+      b[k][f] = ok[k] && ep.B ? ep.B[i] : T(0);
+      c[k][f] = ok[k] && (ep.mode == 2 || keep) ? (ep.mode == 2 ? ep.comp[i] : ep.acc[i]) : T(0);
+    }
+#pragma unroll
+  for (int k = 0; k < CH; ++k) {
+    if (!ok[k]) continue;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const i64 i = idx[k] + dv[f];
+      const T fv = out[k][f];
+      if (ep.mode == 1) {
+        T v = (T)ep.cf * fv;
+        if (ep.A) v += (T)ep.ca * a[k][f];
+        if (ep.B) v += (T)ep.cb * b[k][f];
+        if (ep.acc) {
+          T ac = (T)ep.acc_f * fv;
+          if (ep.acc_keep != 0.0) ac += (T)ep.acc_keep * c[k][f];
+          ep.acc[i] = ac;
+        }
+        ep.out[i] = v;
+      } else {
+        T incr = (T)ep.cf * fv;
+        if (ep.B) incr += (T)ep.cb * (ep.subA ? (b[k][f] - a[k][f]) : b[k][f]);
+        const T y = incr - c[k][f];
+        const T sm = a[k][f] + y;
+        ep.comp[i] = (sm - a[k][f]) - y;
+        ep.out[i] = sm;
+      }
+    }
+  }
+}
+
 // Single periodic wrap of a node index onto the ring 2..n (node 1 is the image of node n).
 __device__ __forceinline__ int mol_wrap(int k, int n) {
   return k <= 1 ? k + (n - 1) : (k > n ? k - (n - 1) : k);
@@ -1499,28 +1553,24 @@ __device__ __forceinline__ void mol_batch_tile(T* L, const BInst& hd, const int 
       const int nv = V.vol();
       const int lofs = (V.lo[0] - S.lo[0]) + G::EX * ((V.lo[1] - S.lo[1]) + G::EY * (V.lo[2] - S.lo[2]));
       MolRtWalk<Z> w((int)threadIdx.x, V.ext(0), V.ext(1), G::NTHR);
-      if constexpr (IT * G::NST <= 16) {
-        // every load of the thread is issued before its first LDS store
-        T v[IT][NS];
-        int dst[IT];
+      // chunks of CH positions per thread: all loads of a chunk are issued before its LDS
+      // stores (one exposed memory latency per chunk, not per position)
+      constexpr int CH = IT * G::NST <= 16 ? IT : (16 / G::NST > 0 ? 16 / G::NST : 1);
+      for (int q0 = (int)threadIdx.x; q0 < nv; q0 += CH * G::NTHR) {
+        T v[CH][NS];
+        int dst[CH];
 #pragma unroll
-        for (int k = 0; k < IT; ++k, w.step()) {
-          dst[k] = (int)threadIdx.x + k * G::NTHR < nv ? lofs + w.x + G::EX * (w.y + G::EY * w.z) : -1;
+        for (int k = 0; k < CH; ++k, w.step()) {
+          dst[k] = q0 + k * G::NTHR < nv ? lofs + w.x + G::EX * (w.y + G::EY * w.z) : -1;
           if (dst[k] >= 0)
 #pragma unroll
             for (int s = 0; s < G::NST; ++s) v[k][s] = u[vb[s] + (w.x + sx[s] * w.y + sxy[s] * w.z)];
         }
 #pragma unroll
-        for (int k = 0; k < IT; ++k)
+        for (int k = 0; k < CH; ++k)
           if (dst[k] >= 0)
 #pragma unroll
             for (int s = 0; s < G::NST; ++s) L[s * G::VOL + dst[k]] = v[k][s];
-      } else {
-        for (int q = threadIdx.x; q < nv; q += G::NTHR, w.step()) {
-          const int d = lofs + w.x + G::EX * (w.y + G::EY * w.z);
-#pragma unroll
-          for (int s = 0; s < G::NST; ++s) L[s * G::VOL + d] = u[vb[s] + (w.x + sx[s] * w.y + sxy[s] * w.z)];
-        }
       }
     }
     // the frame of ghost positions and periodic images (everything when nothing is plain)
@@ -1552,11 +1602,31 @@ __device__ __forceinline__ void mol_batch_tile(T* L, const BInst& hd, const int 
   }
   I = I.meet(N);
   if (!I.empty()) {
+    // interior nodes lie in every variable's box (the interior range is empty otherwise), so
+    // the variables' indices differ by their offsets only; chunks of CN nodes per thread share
+    // one exposed latency of the epilogue's loads
+    constexpr int CN = 4;
+    i64 dv[G::NF];
+#pragma unroll
+    for (int f = 0; f < G::NF; ++f) dv[f] = hd.fofs[f] - hd.fofs[0];
     const int ni = I.vol();
     MolRtWalk<Z> w((int)threadIdx.x, I.ext(0), I.ext(1), G::NTHR);
-    for (int q = threadIdx.x; q < ni; q += G::NTHR, w.step())
-      mol_batch_node<G, true>(L, hd, I.lo[0] + w.x, I.lo[1] + w.y, I.lo[2] + w.z, S.lo[0], S.lo[1], S.lo[2], u, ep,
-                              jp, t);
+    for (int q0 = (int)threadIdx.x; q0 < ni; q0 += CN * G::NTHR) {
+      T out[CN][G::NF];
+      i64 idx[CN];
+      bool ok[CN];
+#pragma unroll
+      for (int k = 0; k < CN; ++k, w.step()) {
+        ok[k] = q0 + k * G::NTHR < ni;
+        idx[k] = 0;
+        if (!ok[k]) continue;
+        const int I0 = I.lo[0] + w.x, I1 = I.lo[1] + w.y, I2 = I.lo[2] + w.z;
+        const int base = (I0 - S.lo[0]) + G::EX * ((I1 - S.lo[1]) + G::EY * (I2 - S.lo[2]));
+        G::template node<true>(hd, L, base, I0, I1, I2, t, nullptr, u, jp, out[k]);
+        idx[k] = mol_b_uidx(hd, 0, I0, I1, I2);
+      }
+      mol_epilogue_chunk<CN, G::NF>(ep, idx, ok, dv, out);
+    }
     const MolFrame F(N, I);
     const int nf = F.count();
     for (int q = threadIdx.x; q < nf; q += G::NTHR) {

@@ -2,9 +2,10 @@
 
 The reference discretizes and solves one problem at a time (an ensemble of MOL problems is a
 loop of ``discretize``/``solve``).  A ``BatchProblem`` takes many lowered instances — uneven
-grids, coefficients and boundary kinds — and runs their right-hand sides in one persistent
-launch per Runge–Kutta stage: instances of one structure share generated code, per-instance
-numbers are read from instance headers (``mol_create_batched`` in include/mol.h).
+grids, coefficients and boundary kinds — and runs their right-hand sides in one launch per
+Runge–Kutta stage (a work-group per tile of an instance, largest tiles first): instances of one
+structure share generated code, per-instance numbers are read from instance headers
+(``mol_create_batched`` in include/mol.h).
 """
 from __future__ import annotations
 
@@ -35,7 +36,7 @@ def _descs(lows: Sequence[Lowered]):
 
 
 def codegen_batched(lows: Sequence[Lowered], compile: bool = False, tile=None) -> str:
-    """Source of the persistent kernel of a batch (no GPU needed)."""
+    """Source of the kernel of a batch (no GPU needed)."""
     lib = native.load_library()
     ds, ps, keep = _descs(lows)
     tl = (C.c_int32 * 3)(*tile) if tile is not None else None
