@@ -1399,9 +1399,17 @@ int adjoint_impl(mol_handle* h, const T* p, T* lam, double* g_aux, hipStream_t s
         va.tnow = h->seg_t + j;
         va.tofs = tofs[q];
         va.mode = acc_aux ? (VJP_GATHER_F | VJP_ACC_A) : (VJP_GATHER_F | VJP_GATHER_A);
+        if (q == 3 && h->vjp_store) {
+          // the last gather of the step stores lam + Y0 + Y1 + Y2 + its own sum into lam
+          va.mode |= VJP_SUM;
+          va.lam = lam;
+          va.y0 = Y[0];
+          va.y1 = Y[1];
+          va.y2 = Y[2];
+        }
         if ((rc = launch_vjp<T>(h, va, s))) return rc;
       }
-      if ((rc = launch_adjoint_sum<T>(lam, Y[0], Y[1], Y[2], Y[3], n, !h->vjp_store, s)))
+      if (!h->vjp_store && (rc = launch_adjoint_sum<T>(lam, Y[0], Y[1], Y[2], Y[3], n, true, s)))
         return fail(h, rc, "adjoint update launch failed");
     }
   }

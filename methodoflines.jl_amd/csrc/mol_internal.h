@@ -72,9 +72,13 @@ struct VArgs {
   double tofs;
   T* bf;            // per-tile adjoint buffers of the staged variables
   double* ba;       // per-tile adjoint buffers of the staged grid fields
-  int mode;         // VJP_GATHER_F | VJP_GATHER_A | VJP_ACC_A
+  int mode;         // VJP_GATHER_F | VJP_GATHER_A | VJP_ACC_A | VJP_SUM
+  T* lam;           // VJP_SUM: the gather (which stores) finishes the step, lam = lam + y0 + y1 + y2 + gathered
+  const T* y0;
+  const T* y1;
+  const T* y2;
 };
-enum { VJP_GATHER_F = 1, VJP_GATHER_A = 2, VJP_ACC_A = 4 };
+enum { VJP_GATHER_F = 1, VJP_GATHER_A = 2, VJP_ACC_A = 4, VJP_SUM = 8 };
 
 // What one right-hand-side launch needs: the core-tile kernel arguments and the list of tiles
 // left to the generic kernel.

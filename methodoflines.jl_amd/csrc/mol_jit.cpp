@@ -1901,7 +1901,17 @@ std::string Gen::source_kernel(bool f64, bool vjp) {
           }
         }
       }
-      o << "      va.out[i] " << (gather_stores() ? "=" : "+=") << " (T)s;\n    }\n";
+      if (gather_stores())
+        // the last stage of an adjoint RK4 step: the step's update of lam (the sum kernel's
+        // association, with this stage's adjoint rounded to T first) instead of a store
+        o << "      if (va.mode & 8) {\n"
+             "        const T y3 = (T)s;\n"
+             "        va.lam[i] = (((va.lam[i] + va.y0[i]) + va.y1[i]) + va.y2[i]) + y3;\n"
+             "      } else {\n"
+             "        va.out[i] = (T)s;\n"
+             "      }\n    }\n";
+      else
+        o << "      va.out[i] += (T)s;\n    }\n";
     }
     o << "    return;\n  }\n";
     bool any_aux = false;

@@ -1141,7 +1141,12 @@ struct VArgs {
   T* bf;
   double* ba;
   int mode;   // 1: gather the variables, 2: gather the grid fields, 4: add the tile kernel's
-              // grid-field adjoints to the buffers instead of storing them
+              // grid-field adjoints to the buffers instead of storing them, 8: the gather ends
+              // an RK4 step of the adjoint sweep: lam = lam + y0 + y1 + y2 + (this gather)
+  T* lam;
+  const T* y0;
+  const T* y1;
+  const T* y2;
 };
 
 // Floor of a / b for b > 0.

@@ -51,7 +51,8 @@ NU_NC = 56   # coefficients per node of a nonuniform WENO5 table row
 class VArgs(C.Structure):
     _fields_ = [("u", C.c_void_p), ("s1", C.c_void_p), ("s2", C.c_void_p), ("c1", C.c_double), ("c2", C.c_double),
                 ("out", C.c_void_p), ("g_aux", C.c_void_p), ("p", C.c_void_p), ("tnow", C.c_void_p),
-                ("tofs", C.c_double), ("bf", C.c_void_p), ("ba", C.c_void_p), ("mode", C.c_int32)]
+                ("tofs", C.c_double), ("bf", C.c_void_p), ("ba", C.c_void_p), ("mode", C.c_int32),
+                ("lam", C.c_void_p), ("y0", C.c_void_p), ("y1", C.c_void_p), ("y2", C.c_void_p)]
 
 
 def _compile(src: str) -> C.CDLL:
