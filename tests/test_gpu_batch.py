@@ -113,3 +113,69 @@ def test_autotuner_times_every_candidate_and_keeps_the_fastest(tmp_path, monkeyp
         ref = orc.diffreact2d_rhs(n, D, k, bct, ui)
         dx = 1.0 / (n - 1)
         assert_parity(gi, ref, np.max(np.abs(ui)) * D * 8 / dx ** 2 + k + 1, what="tuned batch n=%d" % n)
+
+
+def _heat3d(n, D, bc):
+    """u_t = D (u_xx + u_yy + u_zz) + x y, Dirichlet (bc 0) or Neumann (bc 1) on every face."""
+    from molgpu.pdesys import Differential, Eq, Interval, PDESystem, parameters, variables
+    from molgpu.discretization import MOLFiniteDifference
+    t, x, y, z = parameters("t x y z")
+    u = variables("u")
+    Dx, Dy, Dz = Differential(x), Differential(y), Differential(z)
+    eq = Eq(Differential(t)(u(t, x, y, z)),
+            D * ((Dx ** 2)(u(t, x, y, z)) + (Dy ** 2)(u(t, x, y, z)) + (Dz ** 2)(u(t, x, y, z))) + x * y)
+    bcs = [Eq(u(0, x, y, z), x * y * z)]
+    for var, D_ in ((x, Dx), (y, Dy), (z, Dz)):
+        for end in (0, 1):
+            args = {x: x, y: y, z: z}
+            args[var] = end
+            w = u(t, args[x], args[y], args[z])
+            bcs.append(Eq(w, 0) if bc == 0 else Eq(D_(w), 0))
+    doms = [Interval(t, 0.0, 1.0)] + [Interval(v, 0.0, 1.0) for v in (x, y, z)]
+    return PDESystem([eq], bcs, doms, [t, x, y, z], [u(t, x, y, z)]), MOLFiniteDifference({x: n, y: n, z: n}, t)
+
+
+def _family(name, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    if name == "heat1d":
+        for _ in range(9):
+            n = int(rng.integers(12, 300))
+            bcl = [(1.0, 0.0, 0.3), (0.0, 1.0, -0.2), (1.0, 0.5, 0.1)][int(rng.integers(0, 3))]
+            bch = [(1.0, 0.0, -0.4), (0.0, 1.0, 0.25), (2.0, -1.0, 0.0)][int(rng.integers(0, 3))]
+            out.append(molgpu.discretize(*systems.heat1d(n=n, D=float(rng.uniform(0.1, 1.0)), bcl=bcl, bch=bch)))
+    elif name == "bruss_periodic":
+        for _ in range(5):
+            out.append(molgpu.discretize(*systems.brusselator2d(n=int(rng.integers(9, 70)))))
+    else:
+        for _ in range(4):
+            n = int(rng.integers(7, 24))
+            out.append(molgpu.discretize(*_heat3d(n, float(rng.uniform(0.05, 0.5)), int(rng.integers(0, 2)))))
+    return out
+
+
+@pytest.mark.parametrize("name,tile", [("heat1d", (64, 1, 1)), ("heat1d", (512, 1, 1)), ("bruss_periodic", (256, 16, 1)),
+                                       ("bruss_periodic", (16, 8, 1)), ("heat3d", (8, 4, 4)), ("heat3d", (32, 8, 2))])
+def test_batched_rhs_equals_each_instance_alone(name, tile):
+    """1D instances with Dirichlet / Neumann / Robin ends, periodic 2D instances (the frame of a
+    whole-row tile wraps around the ring) and 3D instances (planes of the frame, interior boxes of
+    three axes): the batched right-hand side of every instance equals that instance's own
+    generated kernel (itself checked against the oracle elsewhere)."""
+    torch = _torch()
+    probs = _family(name, 11)
+    batch = BatchProblem(probs, tile=tile)
+    rng = np.random.default_rng(12)
+    u = rng.uniform(0.5, 1.5, batch.n_unknowns)
+    ug = torch.tensor(u, device="cuda")
+    du = torch.empty_like(ug)
+    print("  batch %s tile %s: %d instances, %d unknowns" % (name, tile, len(probs), batch.n_unknowns), flush=True)
+    batch.f(du, ug, 0.7)
+    got = batch.split(du.cpu().numpy())
+    for p, ui, gi in zip(probs, batch.split(u), got):
+        ut = torch.tensor(ui, device="cuda")
+        ref = torch.empty_like(ut)
+        p.f(ref, ut, p.p, 0.7)
+        r = ref.cpu().numpy()
+        # terms of the operator rows: up to 4 alpha / dx^2 |u| (alpha <= 10) plus the reaction
+        scale = 40 * np.max(np.abs(ui)) * max(p.lowered.n) ** 2 + 10
+        assert_parity(gi, r, scale, what="batch %s vs alone" % name)
