@@ -62,3 +62,50 @@ def test_every_header_field_is_mirrored():
                 if tok:
                     fields.add(tok[-1])
         assert fields == {f for f, _ in cls._fields_}, (name, fields ^ {f for f, _ in cls._fields_})
+
+
+# ---- the Julia mirror documented in INTEGRATION.md ---------------------------------------
+
+_JL_SCALARS = {"Int32": C.c_int32, "Int64": C.c_int64, "UInt32": C.c_uint32, "Float64": C.c_double}
+
+
+def _jl_structs():
+    """The struct definitions of the Julia binding in INTEGRATION.md, as ctypes structures built
+    with Julia's isbits layout rules (each field at its natural alignment, as in C)."""
+    import re
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    consts = {m.group(1): int(m.group(2)) for m in re.finditer(r"const (\w+) = (\d+)", text)}
+    out = {}
+
+    def ctype(t):
+        t = t.strip()
+        m = re.fullmatch(r"NTuple\{(\w+),(.+)\}", t)
+        if m:
+            k = m.group(1)
+            return ctype(m.group(2)) * (consts[k] if k in consts else int(k))
+        if t.startswith("Ptr{"):
+            return C.c_void_p
+        return _JL_SCALARS[t]
+
+    for m in re.finditer(r"struct (Mol\w+)\n(.*?)\nend", text, re.S):
+        fields = []
+        for decl in re.split(r"[;\n]", m.group(2)):
+            decl = decl.strip()
+            if decl:
+                name, t = decl.split("::", 1)
+                fields.append((name.strip(), ctype(t)))
+        out[m.group(1)] = type(m.group(1), (C.Structure,), {"_fields_": fields})
+    return out
+
+
+@pytest.mark.parametrize("jl,cname", [("MolOptab", "mol_optab"), ("MolSlot", "mol_slot"),
+                                      ("MolClosure", "mol_closure"), ("MolDesc", "mol_desc")])
+def test_julia_mirror_matches_the_c_layout(jl, cname, c_layout):
+    structs = _jl_structs()
+    assert jl in structs, jl
+    cls = structs[jl]
+    # same field names in the same order as the C struct, same offsets and size
+    assert [f for f, _ in cls._fields_] == [f for f, _ in STRUCTS[cname]._fields_], jl
+    assert C.sizeof(cls) == c_layout[(cname, "size")], jl
+    for fname, _ in cls._fields_:
+        assert getattr(cls, fname).offset == c_layout[(cname, fname)], (jl, fname)
