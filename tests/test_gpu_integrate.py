@@ -139,3 +139,21 @@ def test_host_pointer_entry_point_repeated_calls():
     per_copy = (time.perf_counter() - t0) / calls
     print("  rhs_h %.1f us per call, copies %.1f us" % (per_call * 1e6, per_copy * 1e6))
     assert per_call <= 2 * per_copy, (per_call, per_copy)
+
+
+@pytest.mark.parametrize("bcl,bch", [((1.0, 0.0, 0.3), (0.0, 1.0, -0.2)), ((1.0, 2.0, 0.1), (2.0, -1.0, 0.4))])
+def test_fused_two_step_launches_1d_against_oracle(bcl, bch):
+    """A 1D system too large for one resident tile runs fused launches of two RK4 steps (ghosts
+    refilled from the Dirichlet / Neumann / Robin closures after every stage, and after the first
+    step of a launch) plus one single step for the odd remainder: equal to the oracle's RK4."""
+    torch = _torch()
+    n = 4001
+    prob = molgpu.discretize(*systems.heat1d(n=n, D=1.0, bcl=bcl, bch=bch))
+    plan = prob.plan()
+    assert plan["fused_tiles"] > 1 and plan["fused_steps_per_launch"] == 2, plan
+    u0 = np.random.default_rng(5).uniform(-1, 1, prob.lowered.n_unknowns)
+    dt, N = 2e-8, 37
+    print("  1d fused two-step n=%d N=%d %s %s" % (n, N, bcl, bch), flush=True)
+    got = _run(prob, u0, 0.0, dt, N)
+    ref = orc.diffusion1d_integrate(n, 1.0, bcl, bch, u0, 0.0, dt, N)
+    assert np.max(np.abs(got - ref)) <= 1e-13 * (1 + np.max(np.abs(ref)))
