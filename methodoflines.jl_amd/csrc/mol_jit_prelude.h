@@ -1272,9 +1272,14 @@ __device__ __forceinline__ double mol_gather_pos(const B* __restrict__ base, con
 #pragma unroll
       for (int b = 0; b < 3; ++b)
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
+        for (int a = 0; a < 3; ++a) {
+          // slot 1 is the tile owning the position along that axis; with cross-shaped staging a
+          // neighbour along two or more axes holds the position outside its tile along those
+          // axes, where its buffer is never written: those combinations drop out at compile time
+          if (GB::CROSS && (int)(a != 1) + (int)(b != 1) + (int)(c != 1) >= 2) continue;
           if (c0.v[a] && c1.v[b] && c2.v[c])
             sum += mol_tile_buf<GC, GB, HASC, VOLX, NS>(base, s, c0.t[a], c1.t[b], c2.t[c], c0.l[a], c1.l[b], c2.l[c]);
+        }
     return sum;
   }
   int t0[10], l0[10], t1[10], l1[10], t2[10], l2[10];
