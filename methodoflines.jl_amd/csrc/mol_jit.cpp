@@ -395,7 +395,10 @@ class Gen {
     // 3D tiles: no blocking (128^3 nonlinear diffusion, 16x8x8 tile, 512 threads: 31.3 us with
     // two nodes per thread in turn, 37.7 us with two z-neighbours blocked in registers)
     if (!e && d_->ndim == 3) return;
-    const int want = e ? std::max(1, std::atoi(e)) : 4;
+    // 2D work-groups of 512 threads (64x32 tiles under reach-two stencils): two nodes per
+    // thread in registers, two node groups in turn (four in registers: 10.3 against 9.1 us for
+    // the Burgers WENO5 right-hand side at 1024^2)
+    const int want = e ? std::max(1, std::atoi(e)) : (g_.nthr >= 512 ? 2 : 4);
     for (int r = want; r > 1; r /= 2)
       if (g_.tile[ba] % r == 0 && (e || vol / r >= g_.nthr)) {
         rb = r;
