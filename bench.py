@@ -50,9 +50,9 @@ STAGE_TRAFFIC_BYTES = (3960.27 * 2.0 + 5869.53 * 1.0) * 1024
 # two steps; a launch reads u once with its halo and writes it once).
 FUSED_TRAFFIC_BYTES = (2689.9 * 2.0 + 4096.0 * 1.0) * 1024
 # Issue floor of one fused launch from its SQ counters (profiles/brusselator512_rk4_fused_step.md):
-# 696.4 VALU instructions per wave over 4096 waves; a wave64 VALU instruction occupies a SIMD for
+# 676.0 VALU instructions per wave over 4096 waves; a wave64 VALU instruction occupies a SIMD for
 # 4 cycles, and the chip has 1024 SIMDs at 2.4 GHz.
-FUSED_VALU_PER_WAVE, FUSED_WAVES = 696.4, 4096
+FUSED_VALU_PER_WAVE, FUSED_WAVES = 676.0, 4096
 VALU_FLOOR_US = FUSED_VALU_PER_WAVE * FUSED_WAVES * 4 / (1024 * 2.4e9) * 1e6
 
 

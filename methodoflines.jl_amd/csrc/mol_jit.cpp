@@ -1997,6 +1997,10 @@ std::string Gen::source_fused(bool f64) {
   o << var;
   o << "  static constexpr int RX = " << reach[0] << ", RY = " << reach[1] << ", RZ = " << reach[2] << ";\n";
   o << "  static constexpr bool FIX = " << (allper ? "false" : "true") << ";\n";
+  // stage regions by L1 distance from the tile: every tap lies along one axis, reach one,
+  // periodic axes (no ghost refill, whose closures read along the square region's rows)
+  const bool diam = us.separable && allper && d_->ndim == 2 && reach[0] == 1 && reach[1] == 1;
+  o << "  static constexpr bool DIAM = " << (diam ? "true" : "false") << ";\n";
   if (!allper) o << stage_fn(true);
   o << close;
   // one 1D tile holding the whole system: nsub resident steps per launch; otherwise one step

@@ -783,9 +783,46 @@ struct MolFusedItems {
   }
 };
 
+// Frame items of the region grown by M: the square (tile + M reach per axis) less the tile,
+// or, for 2D cross-shaped stencils of reach one on periodic axes (G::DIAM), the positions
+// within M steps of the tile along the axes (L1 distance), which are all that the remaining
+// stages read: 2 M (TX + TY) + 2 M (M - 1) items instead of 2 M (TX + TY) + 4 M^2.
+template <class G, int M>
+struct MolFrame2 {
+  static constexpr int N = G::DIAM ? 2 * M * (G::TX + G::TY) + 2 * M * (M - 1)
+                                   : (G::TX + 2 * M * G::RX) * (G::TY + 2 * M * G::RY) - G::TX * G::TY;
+};
+
 // Position (tile-relative) of frame item h of the region grown by M (2D / 1D: RZ = 0).
 template <class G, int M>
 __device__ __forceinline__ void mol_frame_pos(const int h, int& lx, int& ly) {
+  if constexpr (G::DIAM) {
+    // rows above the tile from the outermost (ly = -M, TX wide) inwards (ly = -d spans
+    // TX + 2 (M - d) positions centred on the tile), the same rows below, then M positions on
+    // either side of each tile row
+    constexpr int TOP = M * G::TX + M * (M - 1);
+    if (h < 2 * TOP) {
+      const bool below = h >= TOP;
+      int r = below ? h - TOP : h;
+      int d = M;
+#pragma unroll
+      for (int k = M; k > 1; --k) {
+        const int w = G::TX + 2 * (M - k);
+        if (d == k && r >= w) {
+          r -= w;
+          d = k - 1;
+        }
+      }
+      ly = below ? G::TY - 1 + d : -d;
+      lx = -(M - d) + r;
+    } else {
+      const int h3 = h - 2 * TOP;
+      const int c = h3 % (2 * M);
+      ly = h3 / (2 * M);
+      lx = c < M ? -M + c : G::TX + (c - M);
+    }
+    return;
+  }
   constexpr int MR0 = M * G::RX, MR1 = M * G::RY, W0 = G::TX + 2 * MR0;
   if (h < MR1 * W0) {
     ly = -MR1 + h / W0;
@@ -837,8 +874,9 @@ __device__ __forceinline__ void mol_rk4_stage(const T* Lin, T* Lout, T* U, T (&U
   // step; store: stage 4 writes it to unew
   static_assert(S < 4 ? M == ME + 4 - S : M == ME, "stage regions shrink by one reach per stage");
   constexpr int TVOL = MolFusedItems<G>::TVOL, NJ = MolFusedItems<G>::NJ;
-  constexpr int FR = (G::TX + 2 * M * G::RX) * (G::TY + 2 * M * G::RY) - TVOL;
+  constexpr int FR = MolFrame2<G, M>::N;
   static_assert(G::TZ == 1 && G::RZ == 0, "fused RK4 steps are 1D / 2D");
+  static_assert(!G::DIAM || (G::RX == 1 && G::RY == 1 && !G::FIX), "L1 stage regions: reach one, periodic");
   const double ts = t + (S == 1 ? 0.0 : (S == 4 ? dt : dt / 2));
   const double cf = S == 1 || S == 2 ? dt / 2 : (S == 3 ? dt : dt / 6);
   if (ME > 0 && S == 4) Ukeep = U;
